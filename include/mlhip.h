@@ -1,0 +1,200 @@
+/*
+ * mlhip.h — C ABI of libmlhip.so, the MI355X (gfx950) implementation of the
+ * single-level implicit solve of SOFTBALL (jrgparkinson/mushy-layer):
+ *   - the nonlinear FAS multigrid for enthalpy H and bulk concentration
+ *     Theta of AMRNonLinearMultiCompOp, with the phase-diagram closure fused
+ *     into red/black Gauss-Seidel smoothers;
+ *   - the variable-coefficient Helmholtz / Poisson operators behind
+ *     DarcyBrinkmanOp and AMRProjectionOp.
+ *
+ * Each operator entry point replaces one method of the reference's Chombo
+ * AMRLevelOp<LevelData<FArrayBox>> implementation; the citation names the
+ * reference method (file:line).  All field arguments are DEVICE pointers in
+ * Chombo FAB layout (one ghost cell, component outermost):
+ *     idx(i,j,k,c) = (i+1) + (n0+2)*((j+1) + (n1+2)*(k+gz)) + c*ncell
+ * gz = 1 in 3D and 0 in 2D; ncell = ml_cell_count().  Face coefficients of
+ * direction d live on surroundingNodes(valid box, d) without ghosts
+ * (ml_face_count()), component outermost.  Only single-box levels whose box
+ * starts at the origin are accepted (lo == 0): red/black colouring uses the
+ * absolute parity of i+j+k, as the reference's Fortran does.
+ *
+ * Return codes: 0 success; negative values are errors (ML_E*), with a
+ * message from ml_last_error().  Whole solves report the multigrid exit
+ * status separately (ml_solve_stats.exit_status, bit set 1/2/4/8, as the
+ * reference inspects at srcSubcycle/AMRLevelMushyLayer.cpp:715-749).
+ *
+ * Threading: a context owns one HIP stream; calls on a context are
+ * asynchronous on that stream and must not be issued concurrently from
+ * several host threads.  ml_buf_download and ml_ctx_synchronize block.
+ */
+#ifndef MLHIP_H
+#define MLHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MLHIP_ABI_VERSION 1
+
+enum {
+  ML_OK = 0,
+  ML_EINVAL = -1,   /* invalid argument */
+  ML_EHIP = -2,     /* HIP runtime error */
+  ML_ENOTSUP = -3,  /* unsupported configuration */
+  ML_ENOMEM = -4    /* device allocation failed */
+};
+
+enum { ML_BC_DIRICHLET = 0, ML_BC_NEUMANN = 1 };
+enum { ML_PRECISION_FP64 = 0, ML_PRECISION_FP32_REFINED = 1 };
+
+typedef struct ml_ctx_s* ml_ctx;
+typedef struct ml_hc_op_s* ml_hc_op;
+
+/* Physical parameters (MushyLayerParams, src/MushyLayerParams.h;
+ * nondimensional defaults src/MushyLayerParams.cpp:266-270,:340-365,:476-484,
+ * :736-753). */
+typedef struct {
+  double stefan;
+  double compositionRatio;
+  double waterDistributionCoeff;
+  double specificHeatRatio;
+  double heatConductivityRatio;
+  double thetaEutectic;
+  double ThetaEutectic;
+  double heatDiffusionCoeff;   /* D_H */
+  double saltDiffusionCoeff;   /* D_Theta = 1/Le */
+} ml_params;
+
+/* Boundary condition per [dir][side lo=0/hi=1][component]: type
+ * ML_BC_DIRICHLET (order 1, BCutil/BCFuncF.ChF:14-56) or ML_BC_NEUMANN
+ * (BCutil/BCFuncF.ChF:63-96); ignored in periodic directions.  For the HC
+ * operator it is the (theta, Theta_l) derived-variable BC
+ * (PhysBCUtil::temperatureLiquidSalinityBC, BCutil/PhysBCUtil.cpp:3741-3787)
+ * and `apply` is m_apply_bcs_to_diagnostic_var
+ * (src/AMRNonLinearMultiCompOp.cpp:92-96). */
+typedef struct {
+  int type[3][2][2];
+  double val[3][2][2];
+  int apply;
+} ml_bc;
+
+/* Multigrid options (defaults srcSubcycle/MushyLayerSubcycleUtils.cpp:664-676:
+ * nu1 = 1, nu2 = 4, bottom = 40, max_iter = 10, eps = hang = norm_thresh =
+ * 1e-10, precision FP64). */
+typedef struct {
+  int nu1, nu2, bottom, max_iter;
+  double eps, hang, norm_thresh;
+  int precision;
+} ml_solve_opts;
+
+typedef struct {
+  int exit_status;   /* 1 |r|<=eps|r0|, 2 iteration max, 4 hang, 8 |r|<=norm_thresh */
+  int iterations;    /* V-cycles */
+  double initial_residual;
+  double final_residual;
+} ml_solve_stats;
+
+/* ---- context, buffers, layout ---------------------------------------- */
+int ml_ctx_create(int device, unsigned flags, ml_ctx* out);
+int ml_ctx_destroy(ml_ctx ctx);
+const char* ml_last_error(ml_ctx ctx);
+int ml_ctx_synchronize(ml_ctx ctx);
+/* the HIP stream (hipStream_t) the context launches on */
+void* ml_ctx_stream(ml_ctx ctx);
+int ml_buf_alloc(ml_ctx ctx, size_t bytes, void** dptr);
+int ml_buf_free(ml_ctx ctx, void* dptr);
+int ml_buf_upload(ml_ctx ctx, void* dst_dev, const void* src_host, size_t bytes);
+int ml_buf_download(ml_ctx ctx, void* dst_host, const void* src_dev, size_t bytes);
+int ml_buf_copy(ml_ctx ctx, void* dst_dev, const void* src_dev, size_t bytes);
+size_t ml_cell_count(int dim, const int n[]);
+size_t ml_face_count(int dim, const int n[], int dir);
+/* number of MG levels: coarsen while every length divides by 2^(l+1)
+ * (src/AMRNonLinearMultiCompOp.cpp:1316-1325 with s_maxCoarse = 2) */
+int ml_mg_depth(int dim, const int n[]);
+void ml_default_opts(ml_solve_opts* o);
+
+/* ---- nonlinear (H, Theta) operator: AMRNonLinearMultiCompOp ------------ */
+
+/* AMRNonLinearMultiCompOpFactory::define + MGnewOp for every depth
+ * (src/AMRNonLinearMultiCompOp.cpp:1201-1274, :1285-1501). */
+int ml_hc_op_create(ml_ctx ctx, int dim, const int lo[], const int hi[], double dx,
+                    const int periodic[], const ml_params* params, const ml_bc* bc,
+                    ml_hc_op* out);
+int ml_hc_op_destroy(ml_hc_op op);
+/* n and dx of MG level `depth` (0 = finest) */
+int ml_hc_level_info(ml_hc_op op, int depth, int n_out[3], double* dx_out);
+int ml_hc_num_levels(ml_hc_op op);
+
+/* setCoefs (src/AMRNonLinearMultiCompOp.cpp:525-538) on the finest level and
+ * the MG coarsening of aCoef/bCoef to every depth with arithmetic averaging
+ * (MGnewOp, src/AMRNonLinearMultiCompOp.cpp:1423-1494).  aCoef may be NULL
+ * (aCoef == 1).  Coefficients are copied. */
+int ml_hc_set_coefs(ml_hc_op op, const double* aCoef, const double* const bCoef[],
+                    double alpha, double beta);
+/* setAlphaAndBeta (src/AMRNonLinearMultiCompOp.cpp:515-523) */
+int ml_hc_set_alpha_beta(ml_hc_op op, double alpha, double beta);
+/* diagonalScale (src/AMRNonLinearMultiCompOp.H:75-83): rhs *= aCoef of level
+ * `depth` (identity when aCoef was NULL) */
+int ml_hc_diagonal_scale(ml_hc_op op, int depth, double* rhs);
+
+/* computeDiffusedVar (src/AMRNonLinearMultiCompOp.cpp:36-98): (theta, Theta_l)
+ * on valid cells and wall ghosts of level `depth`. */
+int ml_hc_compute_diffused_var(ml_hc_op op, int depth, const double* phi, double* out,
+                               int homogeneous);
+/* relax -> levelGSRB x iters (src/AMRNonLinearMultiCompOp.cpp:815-949) */
+int ml_hc_relax(ml_hc_op op, int depth, double* phi, const double* rhs, int iters);
+/* residualI (src/AMRNonLinearMultiCompOp.cpp:100-166): res = rhs - L(phi) */
+int ml_hc_residual(ml_hc_op op, int depth, double* res, double* phi, const double* rhs,
+                   int homogeneous);
+/* applyOpI (src/AMRNonLinearMultiCompOp.cpp:228-300): lhs = L(phi) */
+int ml_hc_apply_op(ml_hc_op op, int depth, double* lhs, double* phi, int homogeneous);
+/* restrictResidual (src/AMRNonLinearMultiCompOp.cpp:373-451):
+ * resCoarse (level depth+1) = R(rhs - L(phi)) */
+int ml_hc_restrict_residual(ml_hc_op op, int depth, double* resCoarse, double* phiFine,
+                            const double* rhsFine);
+/* restrictR (src/AMRNonLinearMultiCompOp.cpp:492-514): phiCoarse = R phiFine */
+int ml_hc_restrict_r(ml_hc_op op, int depth, double* phiCoarse, const double* phiFine);
+/* prolongIncrement (src/AMRNonLinearMultiCompOp.cpp:454-482):
+ * phiFine += P corrCoarse (linear, ratio 2) */
+int ml_hc_prolong_increment(ml_hc_op op, int depth, double* phiFine, const double* corrCoarse);
+
+/* AMRFASMultiGrid::solve with this operator (Chombo driver; build-owned
+ * specification).  phi is the initial guess and the result. */
+int ml_hc_solve(ml_hc_op op, double* phi, const double* rhs, const ml_solve_opts* opts,
+                ml_solve_stats* stats);
+/* LevelBackwardEuler::updateSoln as driven by multiCompAdvectDiffuse
+ * (srcSubcycle/AMRLevelMushyLayer.cpp:1645-1831): alpha = 1, beta = -dt,
+ * rhs = aCoef*phi_old + dt*src (src may be NULL), solve from phi_new. */
+int ml_hc_solve_be(ml_hc_op op, double* phi_new, const double* phi_old, const double* src,
+                   double dt, const ml_solve_opts* opts, ml_solve_stats* stats);
+
+/* ---- state update and coefficient assembly ----------------------------- */
+/* porosity of every valid cell (CALCULATEPOROSITY_MULTICOMP,
+ * src/EnthalpyVariablesF.ChF:56-94, as in updateEnthalpyVariables
+ * src/phaseDiagram.cpp:92-177) */
+int ml_hc_porosity(ml_hc_op op, const double* phi, double* chi);
+/* HC bCoef from cell porosity (srcSubcycle/AMRLevelMushyLayerInit.cpp:914-953,
+ * geometric faces src/CellToEdge2F.ChF:20-46) */
+int ml_hc_assemble_bcoef(ml_hc_op op, const double* chi, double* const bCoef[]);
+
+/* ---- norms and LevelData arithmetic ------------------------------------ */
+/* max |x| over valid cells and ncomp components (AMRPoissonOp::norm, type 0) */
+int ml_norm_max(ml_ctx ctx, int dim, const int n[], int ncomp, const double* x, double* out);
+/* Elementwise over n doubles (the LinearOp vector functions the reference's
+ * operators inherit from AMRPoissonOp):
+ *   axby: lhs = a*x + b*y  (y may be NULL: lhs = a*x)
+ *   incr: lhs = lhs + scale*x
+ *   mult: lhs = lhs*x      (diagonalScale with aCoef, src/AMRNonLinearMultiCompOp.H:75-83)
+ *   set:  lhs = v */
+int ml_vec_axby(ml_ctx ctx, size_t n, double* lhs, const double* x, const double* y, double a,
+                double b);
+int ml_vec_incr(ml_ctx ctx, size_t n, double* lhs, const double* x, double scale);
+int ml_vec_mult(ml_ctx ctx, size_t n, double* lhs, const double* x);
+int ml_vec_set(ml_ctx ctx, size_t n, double* lhs, double v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

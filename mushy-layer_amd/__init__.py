@@ -1,0 +1,328 @@
+"""Python binding of libmlhip.so — the MI355X implementation of the
+single-level implicit mushy-layer solve (SOFTBALL's AMRNonLinearMultiCompOp
+FAS multigrid and the variable-coefficient Helmholtz/Poisson operators).
+
+The package directory name contains a hyphen; import it with
+``importlib.import_module("mushy-layer_amd")``.
+
+The compute path is the HIP library only: if ``libmlhip.so`` is missing or
+the GPU is absent, the calls raise — there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmlhip.so")
+
+_LIB = None
+
+
+class MLError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [(k, C.c_double) for k in (
+        "stefan", "compositionRatio", "waterDistributionCoeff", "specificHeatRatio",
+        "heatConductivityRatio", "thetaEutectic", "ThetaEutectic", "heatDiffusionCoeff",
+        "saltDiffusionCoeff")]
+
+
+class BC(C.Structure):
+    _fields_ = [("type", C.c_int * 2 * 2 * 3), ("val", C.c_double * 2 * 2 * 3), ("apply", C.c_int)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("nu1", C.c_int), ("nu2", C.c_int), ("bottom", C.c_int), ("max_iter", C.c_int),
+                ("eps", C.c_double), ("hang", C.c_double), ("norm_thresh", C.c_double),
+                ("precision", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("exit_status", C.c_int), ("iterations", C.c_int),
+                ("initial_residual", C.c_double), ("final_residual", C.c_double)]
+
+
+_vp = C.c_void_p
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); mirrors include/mlhip.h
+SIGNATURES = {
+    "ml_ctx_create": (C.c_int, [C.c_int, C.c_uint, C.POINTER(_vp)]),
+    "ml_ctx_destroy": (C.c_int, [_vp]),
+    "ml_last_error": (C.c_char_p, [_vp]),
+    "ml_ctx_synchronize": (C.c_int, [_vp]),
+    "ml_ctx_stream": (_vp, [_vp]),
+    "ml_buf_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "ml_buf_free": (C.c_int, [_vp, _vp]),
+    "ml_buf_upload": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "ml_buf_download": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "ml_buf_copy": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "ml_cell_count": (C.c_size_t, [C.c_int, _ip]),
+    "ml_face_count": (C.c_size_t, [C.c_int, _ip, C.c_int]),
+    "ml_mg_depth": (C.c_int, [C.c_int, _ip]),
+    "ml_default_opts": (None, [C.POINTER(SolveOpts)]),
+    "ml_hc_op_create": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_double, _ip, C.POINTER(Params),
+                                  C.POINTER(BC), C.POINTER(_vp)]),
+    "ml_hc_op_destroy": (C.c_int, [_vp]),
+    "ml_hc_level_info": (C.c_int, [_vp, C.c_int, _ip, _dp]),
+    "ml_hc_num_levels": (C.c_int, [_vp]),
+    "ml_hc_set_coefs": (C.c_int, [_vp, _vp, C.POINTER(_vp), C.c_double, C.c_double]),
+    "ml_hc_set_alpha_beta": (C.c_int, [_vp, C.c_double, C.c_double]),
+    "ml_hc_diagonal_scale": (C.c_int, [_vp, C.c_int, _vp]),
+    "ml_hc_compute_diffused_var": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
+    "ml_hc_relax": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
+    "ml_hc_residual": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int]),
+    "ml_hc_apply_op": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
+    "ml_hc_restrict_residual": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
+    "ml_hc_restrict_r": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "ml_hc_prolong_increment": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "ml_hc_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "ml_hc_solve_be": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.POINTER(SolveOpts),
+                                 C.POINTER(SolveStats)]),
+    "ml_hc_porosity": (C.c_int, [_vp, _vp, _vp]),
+    "ml_hc_assemble_bcoef": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+    "ml_norm_max": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _vp, _dp]),
+    "ml_vec_axby": (C.c_int, [_vp, C.c_size_t, _vp, _vp, _vp, C.c_double, C.c_double]),
+    "ml_vec_incr": (C.c_int, [_vp, C.c_size_t, _vp, _vp, C.c_double]),
+    "ml_vec_mult": (C.c_int, [_vp, C.c_size_t, _vp, _vp]),
+    "ml_vec_set": (C.c_int, [_vp, C.c_size_t, _vp, C.c_double]),
+}
+
+
+def lib():
+    """Load libmlhip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise MLError("libmlhip.so not built: run python mushy-layer_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _iarr(v, n=3, fill=0):
+    a = (C.c_int * n)()
+    for i in range(n):
+        a[i] = int(v[i]) if i < len(v) else fill
+    return a
+
+
+def default_opts(**kw):
+    o = SolveOpts()
+    lib().ml_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def make_params(**kw):
+    p = Params()
+    p.stefan = kw.get("stefan", 5.0)
+    p.compositionRatio = kw.get("compositionRatio", 2.0)
+    p.waterDistributionCoeff = kw.get("waterDistributionCoeff", 1e-5)
+    p.specificHeatRatio = kw.get("specificHeatRatio", 1.0)
+    p.heatConductivityRatio = kw.get("heatConductivityRatio", 1.0)
+    p.thetaEutectic = kw.get("thetaEutectic", 0.0)
+    p.ThetaEutectic = kw.get("ThetaEutectic", 0.0)
+    p.heatDiffusionCoeff = kw.get("heatDiffusionCoeff", 1.0)
+    p.saltDiffusionCoeff = kw.get("saltDiffusionCoeff", 1.0 / kw.get("lewis", 200.0))
+    return p
+
+
+def make_bc(spec=None, apply=1):
+    """spec: {(dir, side, comp): (type, value)}; unspecified entries Neumann 0."""
+    b = BC()
+    for d in range(3):
+        for s in range(2):
+            for c in range(2):
+                b.type[d][s][c] = 1
+                b.val[d][s][c] = 0.0
+    for (d, s, c), (t, v) in (spec or {}).items():
+        b.type[d][s][c] = t
+        b.val[d][s][c] = v
+    b.apply = apply
+    return b
+
+
+class Context:
+    """One device context (a HIP stream) of libmlhip.so."""
+
+    def __init__(self, device=0):
+        self.lib = lib()
+        h = _vp()
+        rc = self.lib.ml_ctx_create(device, 0, C.byref(h))
+        if rc != 0:
+            raise MLError("ml_ctx_create failed (%d): is a GPU visible?" % rc)
+        self.h = h
+        self._bufs = []
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.ml_last_error(self.h)
+            raise MLError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def alloc(self, nbytes):
+        p = _vp()
+        self.check(self.lib.ml_buf_alloc(self.h, nbytes, C.byref(p)), "ml_buf_alloc")
+        return p.value
+
+    def free(self, p):
+        self.check(self.lib.ml_buf_free(self.h, p), "ml_buf_free")
+
+    def upload(self, dptr, arr):
+        arr = np.ascontiguousarray(arr)
+        self.check(self.lib.ml_buf_upload(self.h, dptr, arr.ctypes.data, arr.nbytes), "ml_buf_upload")
+
+    def download(self, dptr, arr):
+        assert arr.flags.c_contiguous
+        self.check(self.lib.ml_buf_download(self.h, arr.ctypes.data, dptr, arr.nbytes), "ml_buf_download")
+        return arr
+
+    def sync(self):
+        self.check(self.lib.ml_ctx_synchronize(self.h), "ml_ctx_synchronize")
+
+    def close(self):
+        if self.h:
+            self.lib.ml_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device buffer holding a numpy-shaped float64 array."""
+
+    def __init__(self, ctx, shape, dtype=np.float64, host=None):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = ctx.alloc(self.nbytes)
+        if host is not None:
+            self.set(host)
+        else:
+            ctx.check(ctx.lib.ml_vec_set(ctx.h, self.nbytes // 8, self.ptr, 0.0), "ml_vec_set")
+
+    def set(self, host):
+        host = np.ascontiguousarray(host, dtype=self.dtype).reshape(self.shape)
+        self.ctx.upload(self.ptr, host)
+
+    def get(self):
+        out = np.empty(self.shape, self.dtype)
+        return self.ctx.download(self.ptr, out)
+
+    def free(self):
+        if self.ptr:
+            self.ctx.free(self.ptr)
+            self.ptr = None
+
+
+def cell_shape(n, ncomp):
+    if len(n) == 3:
+        return (ncomp, n[2] + 2, n[1] + 2, n[0] + 2)
+    return (ncomp, 1, n[1] + 2, n[0] + 2)
+
+
+def face_shape(n, d, ncomp):
+    m = list(n) + [1] * (3 - len(n))
+    m[d] += 1
+    return (ncomp, m[2], m[1], m[0])
+
+
+class HCOp:
+    """AMRNonLinearMultiCompOp (+ its factory's MG hierarchy) on the GPU."""
+
+    def __init__(self, ctx, n, dx, periodic, params, bc):
+        self.ctx = ctx
+        self.n = list(n)
+        self.dim = len(n)
+        self.dx = dx
+        h = _vp()
+        lo = _iarr([0] * 3)
+        hi = _iarr([x - 1 for x in n])
+        ctx.check(ctx.lib.ml_hc_op_create(ctx.h, self.dim, lo, hi, dx, _iarr(periodic), C.byref(params),
+                                          C.byref(bc), C.byref(h)), "ml_hc_op_create")
+        self.h = h
+        self.nlev = ctx.lib.ml_hc_num_levels(h)
+
+    def level_n(self, depth):
+        n = (C.c_int * 3)()
+        dx = C.c_double()
+        self.ctx.check(self.ctx.lib.ml_hc_level_info(self.h, depth, n, C.byref(dx)), "ml_hc_level_info")
+        return [n[d] for d in range(self.dim)], dx.value
+
+    def set_coefs(self, a, b, alpha, beta):
+        arr = (_vp * 3)()
+        for d in range(self.dim):
+            arr[d] = b[d].ptr
+        self.ctx.check(self.ctx.lib.ml_hc_set_coefs(self.h, a.ptr if a is not None else None, arr, alpha, beta),
+                       "ml_hc_set_coefs")
+
+    def set_alpha_beta(self, alpha, beta):
+        self.ctx.check(self.ctx.lib.ml_hc_set_alpha_beta(self.h, alpha, beta), "ml_hc_set_alpha_beta")
+
+    def relax(self, depth, phi, rhs, iters):
+        self.ctx.check(self.ctx.lib.ml_hc_relax(self.h, depth, phi.ptr, rhs.ptr, iters), "ml_hc_relax")
+
+    def residual(self, depth, res, phi, rhs, homogeneous=0):
+        self.ctx.check(self.ctx.lib.ml_hc_residual(self.h, depth, res.ptr, phi.ptr, rhs.ptr, homogeneous),
+                       "ml_hc_residual")
+
+    def apply_op(self, depth, lhs, phi, homogeneous=0):
+        self.ctx.check(self.ctx.lib.ml_hc_apply_op(self.h, depth, lhs.ptr, phi.ptr, homogeneous),
+                       "ml_hc_apply_op")
+
+    def diffused_var(self, depth, phi, out, homogeneous=0):
+        self.ctx.check(self.ctx.lib.ml_hc_compute_diffused_var(self.h, depth, phi.ptr, out.ptr, homogeneous),
+                       "ml_hc_compute_diffused_var")
+
+    def restrict_residual(self, depth, resc, phi, rhs):
+        self.ctx.check(self.ctx.lib.ml_hc_restrict_residual(self.h, depth, resc.ptr, phi.ptr, rhs.ptr),
+                       "ml_hc_restrict_residual")
+
+    def restrict_r(self, depth, phic, phif):
+        self.ctx.check(self.ctx.lib.ml_hc_restrict_r(self.h, depth, phic.ptr, phif.ptr), "ml_hc_restrict_r")
+
+    def prolong_increment(self, depth, phif, corrc):
+        self.ctx.check(self.ctx.lib.ml_hc_prolong_increment(self.h, depth, phif.ptr, corrc.ptr),
+                       "ml_hc_prolong_increment")
+
+    def porosity(self, phi, chi):
+        self.ctx.check(self.ctx.lib.ml_hc_porosity(self.h, phi.ptr, chi.ptr), "ml_hc_porosity")
+
+    def assemble_bcoef(self, chi, b):
+        arr = (_vp * 3)()
+        for d in range(self.dim):
+            arr[d] = b[d].ptr
+        self.ctx.check(self.ctx.lib.ml_hc_assemble_bcoef(self.h, chi.ptr, arr), "ml_hc_assemble_bcoef")
+
+    def solve(self, phi, rhs, opts=None):
+        st = SolveStats()
+        o = opts or default_opts()
+        self.ctx.check(self.ctx.lib.ml_hc_solve(self.h, phi.ptr, rhs.ptr, C.byref(o), C.byref(st)), "ml_hc_solve")
+        return st
+
+    def solve_be(self, phi_new, phi_old, dt, src=None, opts=None):
+        st = SolveStats()
+        o = opts or default_opts()
+        self.ctx.check(self.ctx.lib.ml_hc_solve_be(self.h, phi_new.ptr, phi_old.ptr,
+                                                   src.ptr if src is not None else None, dt, C.byref(o),
+                                                   C.byref(st)), "ml_hc_solve_be")
+        return st
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.ml_hc_op_destroy(self.h)
+            self.h = None

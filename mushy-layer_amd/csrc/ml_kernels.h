@@ -1,0 +1,96 @@
+// ml_kernels.h — launch interfaces of the HIP kernels (internal to
+// libmlhip.so; the public boundary is include/mlhip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ml_common.h"
+
+namespace ml {
+
+// LDS budget per workgroup used by the tiled kernels.
+constexpr size_t kMaxLds = 64 * 1024;
+
+// Arguments of the nonlinear (H, Theta) operator on one MG level.
+template <typename T>
+struct HCArgs {
+  Geom g;
+  ClosureP<T> cp;
+  BCDesc bc;
+  T alpha, beta;
+  const T* a;      // aCoef (2 comps) or nullptr for aCoef == 1
+  const T* b[3];   // face coefficients (2 comps each)
+};
+
+// Max-reduction of a non-negative value over the workgroup, then one
+// atomicMax on its IEEE bit pattern (monotone for non-negative doubles; a
+// NaN anywhere wins).
+template <typename T>
+__device__ inline void block_max_atomic(T v, int nan, unsigned long long* out) {
+  double d = nan ? __longlong_as_double(0x7ff8000000000000LL) : double(v);
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(d, off, 64);
+    d = (o != o || d != d) ? __longlong_as_double(0x7ff8000000000000LL) : (o > d ? o : d);
+  }
+  __shared__ double wmax[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) wmax[w] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmax[0];
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 1; i < nw; ++i) {
+      const double o = wmax[i];
+      m = (o != o || m != m) ? __longlong_as_double(0x7ff8000000000000LL) : (o > m ? o : m);
+    }
+    atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// ---- nonlinear (H, Theta) operator (ml_hc_kernels.hip)
+template <typename T>
+hipError_t hc_relax(const HCArgs<T>& A, T* phi, const T* rhs, int npass, int first, hipStream_t s);
+template <typename T>
+hipError_t hc_residual(const HCArgs<T>& A, T* phi, const T* rhs, T* res, unsigned long long* norm,
+                       int homog, hipStream_t s);
+template <typename T>
+hipError_t hc_apply_op(const HCArgs<T>& A, T* phi, T* lhs, const T* add, int homog, hipStream_t s);
+template <typename T>
+hipError_t hc_restrict_residual(const HCArgs<T>& A, const Geom& gc, T* phi, const T* rhs, T* resc,
+                                int homog, hipStream_t s);
+template <typename T>
+hipError_t hc_diffused_var(const HCArgs<T>& A, const T* phi, T* out, int homog, hipStream_t s);
+template <typename T>
+int max_tile_passes(int dim);
+template <typename T>
+bool small_level_fits(const Geom& g);
+
+// ---- transfers, norms, coefficients (ml_transfer_kernels.hip)
+template <typename T>
+hipError_t restrict_r(const Geom& gf, int ncomp, const T* phif, T* phic, T* phic0, hipStream_t s);
+template <typename T>
+hipError_t prolong_linear(const Geom& gf, int ncomp, T* phif, const T* phic, const T* phic0, hipStream_t s);
+template <typename T>
+hipError_t prolong_constant(const Geom& gf, int ncomp, T* phif, const T* corr, hipStream_t s);
+template <typename T>
+hipError_t norm_max(const Geom& g, int ncomp, const T* x, unsigned long long* out, hipStream_t s);
+hipError_t zero_u64(unsigned long long* p, int n, hipStream_t s);
+template <typename T>
+hipError_t average_faces(const Geom& gf, int ncomp, const T* const* bf, int ratio, T* const* bc, hipStream_t s);
+template <typename T>
+hipError_t average_cells(const Geom& gf, int ncomp, const T* af, int ratio, T* ac, hipStream_t s);
+hipError_t porosity_field(const Geom& g, const ClosureP<double>& cp, const double* phi, double* chi,
+                          hipStream_t s);
+hipError_t hc_bcoef(const Geom& g, double k, double DH, double DC, const double* chi, double* const* b,
+                    hipStream_t s);
+template <typename T>
+hipError_t be_rhs(const Geom& g, int ncomp, T* y, const T* x, const T* acoef, T dt, const T* src, hipStream_t s);
+
+}  // namespace ml
+
+namespace ml {
+hipError_t vec_axby(long long n, double* lhs, const double* x, const double* y, double a, double b, hipStream_t s);
+hipError_t vec_incr(long long n, double* lhs, const double* x, double sc, hipStream_t s);
+hipError_t vec_mult(long long n, double* lhs, const double* x, hipStream_t s);
+hipError_t vec_set(long long n, double* lhs, double v, hipStream_t s);
+}  // namespace ml

@@ -1,0 +1,384 @@
+// ml_transfer_kernels.hip — MG transfers (restriction of the state,
+// prolongation of corrections), max-norms, coefficient averaging for the MG
+// hierarchy and the HC coefficient assembly, for gfx950.
+#include "ml_common.h"
+#include "ml_kernels.h"
+
+namespace ml {
+
+namespace {
+
+__device__ inline int wrap_clamp(int v, int n, int periodic) {
+  if (v >= 0 && v < n) return v;
+  if (periodic) return ((v % n) + n) % n;
+  return v < 0 ? 0 : n - 1;
+}
+
+__device__ inline void unflatten(long long c, int nx, int ny, int& i, int& j, int& k) {
+  i = int(c % nx);
+  const long long r = c / nx;
+  j = int(r % ny);
+  k = int(r / ny);
+}
+
+inline int grid_for(long long n, int threads = 256) {
+  long long b = (n + threads - 1) / threads;
+  if (b > 65535 * 4) b = 65535 * 4;
+  if (b < 1) b = 1;
+  return int(b);
+}
+
+// RESTRICT (src/AMRNonLinearMultiCompOpF.ChF:555-585): phi_c = 0 + sum of
+// children/2^d in Fortran order; optionally also stored in phic0 (the FAS
+// initial coarse state).
+template <typename T, int DIM>
+__global__ void restrict_r_kernel(Geom gf, Geom gc, int ncomp, const T* __restrict__ phif,
+                                  T* __restrict__ phic, T* __restrict__ phic0) {
+  const long long N = gc.nvalid();
+  const T denom = DIM == 3 ? T(8) : T(4);
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N;
+       c += (long long)gridDim.x * blockDim.x) {
+    int I, J, K;
+    unflatten(c, gc.n[0], gc.n[1], I, J, K);
+    const long long qc = gc.ci(I, J, K);
+    for (int n = 0; n < ncomp; ++n) {
+      T s = T(0);
+      for (int kk = 0; kk < (DIM == 3 ? 2 : 1); ++kk)
+        for (int jj = 0; jj < 2; ++jj)
+          for (int ii = 0; ii < 2; ++ii)
+            s = s + phif[gf.ci(2 * I + ii, 2 * J + jj, DIM == 3 ? 2 * K + kk : 0) + n * gf.ncell] / denom;
+      phic[qc + n * gc.ncell] = s;
+      if (phic0) phic0[qc + n * gc.ncell] = s;
+    }
+  }
+}
+
+// Linear prolongation of e_c = phic - phic0 (PROLONG_JP weights,
+// src/AMRProjectionOpF.ChF:15-77); coarse ghosts by periodic image or the
+// no-flux rule (nearest interior index).
+template <typename T, int DIM>
+__global__ void prolong_linear_kernel(Geom gf, Geom gc, int ncomp, T* __restrict__ phif,
+                                      const T* __restrict__ phic, const T* __restrict__ phic0) {
+  const long long N = gf.nvalid();
+  const T den = T(1) / (DIM == 3 ? T(64) : T(16));
+  const T fx1 = T(3) * den, fx2 = T(9) * den, fx3 = T(27) * den, f0 = T(1) * den;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N;
+       c += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    unflatten(c, gf.n[0], gf.n[1], i, j, k);
+    const int ic = i / 2, jc = j / 2, kc = DIM == 3 ? k / 2 : 0;
+    const int ox = 2 * (i % 2) - 1, oy = 2 * (j % 2) - 1, oz = DIM == 3 ? 2 * (k % 2) - 1 : 0;
+    const int X0 = ic, X1 = wrap_clamp(ic + ox, gc.n[0], gc.periodic[0]);
+    const int Y0 = jc, Y1 = wrap_clamp(jc + oy, gc.n[1], gc.periodic[1]);
+    const int Z0 = kc, Z1 = DIM == 3 ? wrap_clamp(kc + oz, gc.n[2], gc.periodic[2]) : 0;
+    const long long qf = gf.ci(i, j, k);
+    for (int n = 0; n < ncomp; ++n) {
+      const T* pc = phic + n * gc.ncell;
+      const T* p0 = phic0 ? phic0 + n * gc.ncell : nullptr;
+      auto e = [&](int x, int y, int z) -> T {
+        const long long q = gc.ci(x, y, z);
+        return phic0 ? pc[q] - p0[q] : pc[q];
+      };
+      T p = phif[qf + n * gf.ncell];
+      if (DIM == 2) {
+        p = p + fx2 * e(X0, Y0, 0) + f0 * e(X1, Y1, 0);
+        p = p + fx1 * (e(X1, Y0, 0) + e(X0, Y1, 0));
+      } else {
+        p = p + fx3 * e(X0, Y0, Z0) + f0 * e(X1, Y1, Z1);
+        p = p + fx2 * (e(X1, Y0, Z0) + e(X0, Y1, Z0) + e(X0, Y0, Z1));
+        p = p + fx1 * (e(X1, Y1, Z0) + e(X0, Y1, Z1) + e(X1, Y0, Z1));
+      }
+      phif[qf + n * gf.ncell] = p;
+    }
+  }
+}
+
+// phi_f += e_c(i/2) (piecewise constant, Chombo FORT_PROLONG semantics).
+template <typename T, int DIM>
+__global__ void prolong_constant_kernel(Geom gf, Geom gc, int ncomp, T* __restrict__ phif,
+                                        const T* __restrict__ corr) {
+  const long long N = gf.nvalid();
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N;
+       c += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    unflatten(c, gf.n[0], gf.n[1], i, j, k);
+    const long long qf = gf.ci(i, j, k), qc = gc.ci(i / 2, j / 2, DIM == 3 ? k / 2 : 0);
+    for (int n = 0; n < ncomp; ++n) phif[qf + n * gf.ncell] = phif[qf + n * gf.ncell] + corr[qc + n * gc.ncell];
+  }
+}
+
+template <typename T>
+__global__ void norm_max_kernel(Geom g, int ncomp, const T* __restrict__ x, unsigned long long* out) {
+  const long long N = g.nvalid();
+  T m = T(0);
+  int nan = 0;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N * ncomp;
+       c += (long long)gridDim.x * blockDim.x) {
+    const int n = int(c / N);
+    int i, j, k;
+    unflatten(c % N, g.n[0], g.n[1], i, j, k);
+    const T v = fabs(x[g.ci(i, j, k) + n * g.ncell]);
+    if (v != v) nan = 1;
+    else if (v > m) m = v;
+  }
+  block_max_atomic<T>(m, nan, out);
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// Arithmetic CoarseAverageFace by `ratio` from the finest level
+// (src/AMRNonLinearMultiCompOp.cpp:1450-1469): sub-faces summed lowest
+// transverse index fastest, times 1/ratio^(d-1).
+template <typename T, int DIM>
+__global__ void average_faces_kernel(Geom gf, Geom gc, int ncomp, int d, int ratio, const T* __restrict__ bf,
+                                     T* __restrict__ bc) {
+  int r[3] = {ratio, ratio, DIM == 3 ? ratio : 1};
+  r[d] = 1;
+  const T scale = T(1) / T(r[0] * r[1] * r[2]);
+  const long long N = gc.nface[d];
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N * ncomp;
+       c += (long long)gridDim.x * blockDim.x) {
+    const int n = int(c / N);
+    int I, J, K;
+    unflatten(c % N, int(gc.fn[d][0]), int(gc.fn[d][1]), I, J, K);
+    const int base[3] = {I * (d == 0 ? ratio : r[0]), J * (d == 1 ? ratio : r[1]), K * (d == 2 ? ratio : r[2])};
+    T s = T(0);
+    for (int kk = 0; kk < r[2]; ++kk)
+      for (int jj = 0; jj < r[1]; ++jj)
+        for (int ii = 0; ii < r[0]; ++ii)
+          s = s + bf[gf.fi(d, base[0] + ii, base[1] + jj, base[2] + kk) + n * gf.nface[d]];
+    bc[gc.fi(d, I, J, K) + n * gc.nface[d]] = s * scale;
+  }
+}
+
+template <typename T, int DIM>
+__global__ void average_cells_kernel(Geom gf, Geom gc, int ncomp, int ratio, const T* __restrict__ af,
+                                     T* __restrict__ ac) {
+  const int rz = DIM == 3 ? ratio : 1;
+  const T scale = T(1) / T(ratio * ratio * rz);
+  const long long N = gc.nvalid();
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N * ncomp;
+       c += (long long)gridDim.x * blockDim.x) {
+    const int n = int(c / N);
+    int I, J, K;
+    unflatten(c % N, gc.n[0], gc.n[1], I, J, K);
+    T s = T(0);
+    for (int kk = 0; kk < rz; ++kk)
+      for (int jj = 0; jj < ratio; ++jj)
+        for (int ii = 0; ii < ratio; ++ii)
+          s = s + af[gf.ci(I * ratio + ii, J * ratio + jj, K * rz + kk) + n * gf.ncell];
+    ac[gc.ci(I, J, K) + n * gc.ncell] = s * scale;
+  }
+}
+
+// CALCULATEPOROSITY_MULTICOMP over valid cells (src/EnthalpyVariablesF.ChF:56-94).
+__global__ void porosity_kernel(Geom g, ClosureP<double> cp, const double* __restrict__ phi,
+                                double* __restrict__ chi) {
+  const long long N = g.nvalid();
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N;
+       c += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    unflatten(c, g.n[0], g.n[1], i, j, k);
+    const long long q = g.ci(i, j, k);
+    chi[q] = porosity<double>(phi[q], phi[q + g.ncell], cp);
+  }
+}
+
+// HC bCoef assembly (srcSubcycle/AMRLevelMushyLayerInit.cpp:914-953) with
+// geometric face porosity (src/CellToEdge2F.ChF:20-46).
+template <int DIM>
+__global__ void hc_bcoef_kernel(Geom g, int d, double kcond, double DH, double DC,
+                                const double* __restrict__ chi, double* __restrict__ b) {
+  const long long N = g.nface[d];
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N;
+       c += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    unflatten(c, int(g.fn[d][0]), int(g.fn[d][1]), i, j, k);
+    const int li = wrap_clamp(i - (d == 0), g.n[0], g.periodic[0]);
+    const int lj = wrap_clamp(j - (d == 1), g.n[1], g.periodic[1]);
+    const int lk = DIM == 3 ? wrap_clamp(k - (d == 2), g.n[2], g.periodic[2]) : 0;
+    const int hi = wrap_clamp(i, g.n[0], g.periodic[0]);
+    const int hj = wrap_clamp(j, g.n[1], g.periodic[1]);
+    const int hk = DIM == 3 ? wrap_clamp(k, g.n[2], g.periodic[2]) : 0;
+    const double lo = chi[g.ci(li, lj, lk)], up = chi[g.ci(hi, hj, hk)];
+    const double cf = sqrt(fabs(lo * up));
+    double bC = 1.0 * cf;
+    double bH = 1.0 - cf;
+    bH = bH * kcond;
+    bH = bH + cf;
+    bH = bH * (-DH);
+    bC = bC * (-DC);
+    b[c] = bH;
+    b[c + N] = bC;
+  }
+}
+
+// y = x*a + b*z over valid cells (z may be null): backward-Euler rhs
+// a*phi_old + dt*src with a == 1 when the aCoef array is absent.
+template <typename T>
+__global__ void be_rhs_kernel(Geom g, int ncomp, T* __restrict__ y, const T* __restrict__ x,
+                              const T* __restrict__ acoef, T dt, const T* __restrict__ src) {
+  const long long N = g.ncell * ncomp;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N;
+       c += (long long)gridDim.x * blockDim.x) {
+    T r = x[c] * (acoef ? acoef[c] : T(1));
+    if (src) r = r + dt * src[c];
+    y[c] = r;
+  }
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t restrict_r(const Geom& gf, int ncomp, const T* phif, T* phic, T* phic0, hipStream_t s) {
+  Geom gc = coarsen_geom(gf, 2);
+  const int b = grid_for(gc.nvalid());
+  if (gf.dim == 2)
+    hipLaunchKernelGGL((restrict_r_kernel<T, 2>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, phif, phic, phic0);
+  else
+    hipLaunchKernelGGL((restrict_r_kernel<T, 3>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, phif, phic, phic0);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t prolong_linear(const Geom& gf, int ncomp, T* phif, const T* phic, const T* phic0, hipStream_t s) {
+  Geom gc = coarsen_geom(gf, 2);
+  const int b = grid_for(gf.nvalid());
+  if (gf.dim == 2)
+    hipLaunchKernelGGL((prolong_linear_kernel<T, 2>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, phif, phic, phic0);
+  else
+    hipLaunchKernelGGL((prolong_linear_kernel<T, 3>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, phif, phic, phic0);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t prolong_constant(const Geom& gf, int ncomp, T* phif, const T* corr, hipStream_t s) {
+  Geom gc = coarsen_geom(gf, 2);
+  const int b = grid_for(gf.nvalid());
+  if (gf.dim == 2)
+    hipLaunchKernelGGL((prolong_constant_kernel<T, 2>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, phif, corr);
+  else
+    hipLaunchKernelGGL((prolong_constant_kernel<T, 3>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, phif, corr);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t norm_max(const Geom& g, int ncomp, const T* x, unsigned long long* out, hipStream_t s) {
+  int b = grid_for(g.nvalid() * ncomp);
+  if (b > 1024) b = 1024;
+  hipLaunchKernelGGL((norm_max_kernel<T>), dim3(b), dim3(256), 0, s, g, ncomp, x, out);
+  return hipGetLastError();
+}
+
+hipError_t zero_u64(unsigned long long* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t average_faces(const Geom& gf, int ncomp, const T* const* bf, int ratio, T* const* bc, hipStream_t s) {
+  Geom gc = coarsen_geom(gf, ratio);
+  for (int d = 0; d < gf.dim; ++d) {
+    const int b = grid_for(gc.nface[d] * ncomp);
+    if (gf.dim == 2)
+      hipLaunchKernelGGL((average_faces_kernel<T, 2>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, d, ratio, bf[d], bc[d]);
+    else
+      hipLaunchKernelGGL((average_faces_kernel<T, 3>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, d, ratio, bf[d], bc[d]);
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t average_cells(const Geom& gf, int ncomp, const T* af, int ratio, T* ac, hipStream_t s) {
+  Geom gc = coarsen_geom(gf, ratio);
+  const int b = grid_for(gc.nvalid() * ncomp);
+  if (gf.dim == 2)
+    hipLaunchKernelGGL((average_cells_kernel<T, 2>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, ratio, af, ac);
+  else
+    hipLaunchKernelGGL((average_cells_kernel<T, 3>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, ratio, af, ac);
+  return hipGetLastError();
+}
+
+hipError_t porosity_field(const Geom& g, const ClosureP<double>& cp, const double* phi, double* chi,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(porosity_kernel, dim3(grid_for(g.nvalid())), dim3(256), 0, s, g, cp, phi, chi);
+  return hipGetLastError();
+}
+
+hipError_t hc_bcoef(const Geom& g, double k, double DH, double DC, const double* chi, double* const* b,
+                    hipStream_t s) {
+  for (int d = 0; d < g.dim; ++d) {
+    const int blocks = grid_for(g.nface[d]);
+    if (g.dim == 2)
+      hipLaunchKernelGGL((hc_bcoef_kernel<2>), dim3(blocks), dim3(256), 0, s, g, d, k, DH, DC, chi, b[d]);
+    else
+      hipLaunchKernelGGL((hc_bcoef_kernel<3>), dim3(blocks), dim3(256), 0, s, g, d, k, DH, DC, chi, b[d]);
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t be_rhs(const Geom& g, int ncomp, T* y, const T* x, const T* acoef, T dt, const T* src, hipStream_t s) {
+  hipLaunchKernelGGL((be_rhs_kernel<T>), dim3(grid_for(g.ncell * ncomp)), dim3(256), 0, s, g, ncomp, y, x, acoef,
+                     dt, src);
+  return hipGetLastError();
+}
+
+template hipError_t restrict_r<double>(const Geom&, int, const double*, double*, double*, hipStream_t);
+template hipError_t prolong_linear<double>(const Geom&, int, double*, const double*, const double*, hipStream_t);
+template hipError_t prolong_constant<double>(const Geom&, int, double*, const double*, hipStream_t);
+template hipError_t norm_max<double>(const Geom&, int, const double*, unsigned long long*, hipStream_t);
+template hipError_t average_faces<double>(const Geom&, int, const double* const*, int, double* const*, hipStream_t);
+template hipError_t average_cells<double>(const Geom&, int, const double*, int, double*, hipStream_t);
+template hipError_t be_rhs<double>(const Geom&, int, double*, const double*, const double*, double, const double*,
+                                   hipStream_t);
+
+}  // namespace ml
+
+// ---- elementwise LevelData arithmetic ------------------------------------
+namespace ml {
+namespace {
+__global__ void vec_axby_kernel(long long n, double* __restrict__ lhs, const double* __restrict__ x,
+                                const double* __restrict__ y, double a, double b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    lhs[i] = y ? a * x[i] + b * y[i] : a * x[i];
+}
+__global__ void vec_incr_kernel(long long n, double* __restrict__ lhs, const double* __restrict__ x, double s) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    lhs[i] = lhs[i] + s * x[i];
+}
+__global__ void vec_mult_kernel(long long n, double* __restrict__ lhs, const double* __restrict__ x) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    lhs[i] = lhs[i] * x[i];
+}
+__global__ void vec_set_kernel(long long n, double* __restrict__ lhs, double v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    lhs[i] = v;
+}
+inline int vgrid(long long n) {
+  long long b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  return b < 1 ? 1 : int(b);
+}
+}  // namespace
+
+hipError_t vec_axby(long long n, double* lhs, const double* x, const double* y, double a, double b, hipStream_t s) {
+  hipLaunchKernelGGL(vec_axby_kernel, dim3(vgrid(n)), dim3(256), 0, s, n, lhs, x, y, a, b);
+  return hipGetLastError();
+}
+hipError_t vec_incr(long long n, double* lhs, const double* x, double sc, hipStream_t s) {
+  hipLaunchKernelGGL(vec_incr_kernel, dim3(vgrid(n)), dim3(256), 0, s, n, lhs, x, sc);
+  return hipGetLastError();
+}
+hipError_t vec_mult(long long n, double* lhs, const double* x, hipStream_t s) {
+  hipLaunchKernelGGL(vec_mult_kernel, dim3(vgrid(n)), dim3(256), 0, s, n, lhs, x);
+  return hipGetLastError();
+}
+hipError_t vec_set(long long n, double* lhs, double v, hipStream_t s) {
+  hipLaunchKernelGGL(vec_set_kernel, dim3(vgrid(n)), dim3(256), 0, s, n, lhs, v);
+  return hipGetLastError();
+}
+}  // namespace ml

@@ -1,0 +1,127 @@
+"""Seeded synthetic inputs of the benchmark and parity workloads.
+
+Every generator returns host numpy arrays in the FAB layout of the C ABI
+(``cell_shape`` / ``face_shape``) plus the parameter / BC descriptions, so
+the CPU oracle and the GPU library receive bit-identical inputs.
+
+Workloads (named by what they are):
+  * ``diffusive_64``  — 64x64 diffusive solidification, unit square,
+    periodic in x, cold bottom theta = -1, warm top theta = 1.2, St = 5,
+    CR = 2, Le = 200, H = St + 1.2, Theta = -1 + 1e-3 U(-1, 1), dt = 1e-3,
+    10 backward-Euler steps.
+  * ``mushy_256`` — 256x256 mushy layer with porosity from a smooth seeded
+    Gaussian random field (correlation length 0.1) in [0.05, 1], St = 5,
+    CR = 2, Kozeny-Carman permeability, Da = 1e-3, Ra_C = 200.
+"""
+import numpy as np
+
+DIRICHLET, NEUMANN = 0, 1
+
+
+def cell_shape(n, ncomp):
+    if len(n) == 3:
+        return (ncomp, n[2] + 2, n[1] + 2, n[0] + 2)
+    return (ncomp, 1, n[1] + 2, n[0] + 2)
+
+
+def face_shape(n, d, ncomp):
+    m = list(n) + [1] * (3 - len(n))
+    m[d] += 1
+    return (ncomp, m[2], m[1], m[0])
+
+
+def valid(n, x):
+    if len(n) == 3:
+        return x[:, 1:-1, 1:-1, 1:-1]
+    return x[:, :, 1:-1, 1:-1]
+
+
+def closure_point(H, C, St, CR, pc=1e-5, c=1.0, te=0.0, Te=0.0):
+    """Scalar phase-diagram closure used to derive boundary values
+    (computeEnthalpyVars, src/phaseDiagram.cpp:180-223)."""
+    hs = 0.0 if C < -CR else c * ((-C - CR) / pc)
+    hs = max(hs, 0.0) + te * c
+    he = (CR + C) / (CR + Te) * (St + te * (1 - c)) + c * te
+    hl = St - C + te + Te
+    if H <= hs:
+        return H / c, 0.0
+    if H <= he:
+        return te, Te
+    if H < hl:
+        vc = max(C, -CR)
+        a = CR * (c - 1) + St * (pc - 1)
+        b = CR * (1 - 2 * c) + H * (1 - pc) - vc * (c - 1) - pc * St
+        q = (CR + vc) * c + pc * H
+        chi = (-b - np.sqrt(b * b - 4 * a * q)) / (2 * a)
+        cl = (C + CR * (1 - chi)) / (chi + pc * (1 - chi))
+        return -cl, cl
+    return H - St, C
+
+
+def diffusive(n=(64, 64), seed=1, St=5.0, CR=2.0, lewis=200.0, theta_lo=-1.0, theta_top=1.2,
+              dt=1e-3, steps=10, noise=1e-3):
+    """Diffusive solidification (no flow): HC state, derived-variable BCs."""
+    n = list(n)
+    dim = len(n)
+    H_top = St + theta_top
+    C0 = -1.0
+    # boundary (theta, Theta_l) from the boundary (H, Theta) by the closure
+    # (MushyLayerParams::computeDerivedBCs, src/MushyLayerParams.cpp:130-170)
+    t_lo, cl_lo = closure_point(theta_lo, C0, St, CR)     # H_lo = theta_lo (solid)
+    t_hi, cl_hi = closure_point(H_top, C0, St, CR)
+    vd = dim - 1
+    bc = {(vd, 0, 0): (DIRICHLET, t_lo), (vd, 1, 0): (DIRICHLET, t_hi),
+          (vd, 0, 1): (DIRICHLET, cl_lo), (vd, 1, 1): (DIRICHLET, cl_hi)}
+    periodic = [1] * (dim - 1) + [0]
+    rng = np.random.default_rng(seed)
+    phi = np.zeros(cell_shape(n, 2))
+    v = valid(n, phi)
+    v[0] = H_top
+    v[1] = C0 + noise * rng.uniform(-1.0, 1.0, size=v[1].shape)
+    return dict(n=n, dim=dim, dx=1.0 / n[0], periodic=periodic, bc=bc, phi=phi, dt=dt, steps=steps,
+                params=dict(stefan=St, compositionRatio=CR, lewis=lewis))
+
+
+def gaussian_field(n, corr, seed, lo=0.05, hi=1.0):
+    """Periodic smooth Gaussian random field affinely mapped to [lo, hi]:
+    white noise convolved with a separable periodic Gaussian of standard
+    deviation corr/dx cells, truncated at 4 sigma."""
+    rng = np.random.default_rng(seed)
+    shape = tuple(reversed(n))
+    f = rng.standard_normal(shape)
+    sig = corr * n[0]
+    r = int(np.ceil(4 * sig))
+    k = np.exp(-0.5 * (np.arange(-r, r + 1) / sig) ** 2)
+    k /= k.sum()
+    for ax in range(len(shape)):
+        acc = np.zeros_like(f)
+        for o, w in zip(range(-r, r + 1), k):
+            acc += w * np.roll(f, o, axis=ax)
+        f = acc
+    f = (f - f.min()) / (f.max() - f.min())
+    return lo + (hi - lo) * f
+
+
+def mushy(n=(256, 256), seed=2, St=5.0, CR=2.0, lewis=200.0, corr=0.1, theta_l=-1.0, dt=1e-3, steps=20):
+    """Mushy-layer state whose porosity is a seeded smooth random field:
+    invert chi -> (H, Theta) at a uniform liquid concentration Theta_l,
+    Theta = Theta_l (chi + pc(1-chi)) - CR(1-chi), H = chi St + theta with
+    theta = -Theta_l (c = 1, pc = 1e-5)."""
+    n = list(n)
+    dim = len(n)
+    pc = 1e-5
+    chi = gaussian_field(n, corr, seed)
+    Theta = theta_l * (chi + pc * (1 - chi)) - CR * (1 - chi)
+    H = chi * St + (-theta_l)
+    phi = np.zeros(cell_shape(n, 2))
+    v = valid(n, phi)
+    v[0] = H.reshape(v[0].shape)
+    v[1] = Theta.reshape(v[1].shape)
+    vd = dim - 1
+    # cold bottom at the eutectic, top at the initial liquid state
+    t_hi, cl_hi = closure_point(St + 0.2, -1.0, St, CR)
+    bc = {(vd, 0, 0): (DIRICHLET, 0.0), (vd, 1, 0): (DIRICHLET, 0.2),
+          (vd, 0, 1): (DIRICHLET, 0.0), (vd, 1, 1): (DIRICHLET, cl_hi)}
+    periodic = [1] * (dim - 1) + [0]
+    return dict(n=n, dim=dim, dx=1.0 / n[0], periodic=periodic, bc=bc, phi=phi, dt=dt, steps=steps,
+                params=dict(stefan=St, compositionRatio=CR, lewis=lewis), chi=chi)

@@ -1,0 +1,321 @@
+"""ctypes wrapper of the CPU oracle ``libmlref.so``.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by the CPU-baseline leg of ``bench.py``.  The product package never
+imports this module.
+
+Arrays follow the Chombo FAB layout used throughout (one ghost layer,
+component outermost); see ``oracle/mlref.h``.  Cell arrays are numpy float64
+arrays of shape ``(ncomp, nz+2|1, ny+2, nx+2)``; face arrays of direction d
+have shape ``(ncomp, fz, fy, fx)`` with ``f = n + e_d``.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class Grid(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n", C.c_int * 3), ("periodic", C.c_int * 3), ("dx", C.c_double)]
+
+
+class Params(C.Structure):
+    _fields_ = [(k, C.c_double) for k in (
+        "stefan", "compositionRatio", "waterDistributionCoeff", "specificHeatRatio",
+        "heatConductivityRatio", "thetaEutectic", "ThetaEutectic", "heatDiffusionCoeff",
+        "saltDiffusionCoeff")]
+
+
+class BC(C.Structure):
+    _fields_ = [("type", C.c_int * 2 * 2 * 3), ("val", C.c_double * 2 * 2 * 3), ("apply", C.c_int)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("nu1", C.c_int), ("nu2", C.c_int), ("bottom", C.c_int), ("max_iter", C.c_int),
+                ("eps", C.c_double), ("hang", C.c_double), ("norm_thresh", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("exit_status", C.c_int), ("iterations", C.c_int),
+                ("initial_residual", C.c_double), ("final_residual", C.c_double)]
+
+
+def build():
+    """Compile libmlref.so with the committed Makefile (gcc, -O2 -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libmlref.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+        _declare(_LIB)
+    return _LIB
+
+
+_dp = C.POINTER(C.c_double)
+
+
+def _declare(L):
+    G = C.POINTER(Grid)
+    P = C.POINTER(Params)
+    B = C.POINTER(BC)
+    pp = C.POINTER(_dp)
+    sig = {
+        "mlr_ncell": (C.c_long, [G]),
+        "mlr_nface": (C.c_long, [G, C.c_int]),
+        "mlr_mg_depth": (C.c_int, [G]),
+        "mlr_bounding_energy": (None, [C.c_double, C.c_double, P, _dp, _dp, _dp]),
+        "mlr_porosity_mush": (C.c_double, [C.c_double, C.c_double, P]),
+        "mlr_closure_point": (None, [C.c_double, C.c_double, P, _dp, _dp]),
+        "mlr_porosity_point": (C.c_double, [C.c_double, C.c_double, P]),
+        "mlr_exchange": (None, [G, C.c_int, _dp]),
+        "mlr_noflux_bc": (None, [G, C.c_int, _dp]),
+        "mlr_scalar_bc": (None, [G, C.c_int, B, C.c_int, _dp]),
+        "mlr_hc_diffused_var": (None, [G, P, B, _dp, _dp, C.c_int]),
+        "mlr_lambda": (None, [G, C.c_int, C.c_double, _dp, C.c_double, pp, _dp, _dp]),
+        "mlr_hc_gsrb": (None, [G, P, B, C.c_double, _dp, C.c_double, pp, _dp, _dp, _dp, _dp]),
+        "mlr_hc_residual": (None, [G, P, B, C.c_double, _dp, C.c_double, pp, _dp, _dp, _dp, _dp]),
+        "mlr_hc_apply_op": (None, [G, P, B, C.c_double, _dp, C.c_double, pp, _dp, _dp, _dp]),
+        "mlr_hc_restrict_residual": (None, [G, P, B, C.c_double, _dp, C.c_double, pp, _dp, _dp, _dp, _dp]),
+        "mlr_restrict_r": (None, [G, C.c_int, _dp, _dp]),
+        "mlr_prolong_linear": (None, [G, C.c_int, _dp, _dp]),
+        "mlr_prolong_constant": (None, [G, C.c_int, _dp, _dp]),
+        "mlr_average_cells": (None, [G, C.c_int, _dp, C.c_int, _dp]),
+        "mlr_average_faces": (None, [G, C.c_int, pp, C.c_int, pp]),
+        "mlr_norm_max": (C.c_double, [G, C.c_int, _dp]),
+        "mlr_porosity_field": (None, [G, P, _dp, _dp]),
+        "mlr_geometric_faces": (None, [G, _dp, pp]),
+        "mlr_hc_bcoef": (None, [G, P, _dp, pp]),
+        "mlr_default_opts": (None, [C.POINTER(Opts)]),
+        "mlr_hc_fas_solve": (C.c_int, [G, P, B, C.c_double, _dp, C.c_double, pp, _dp, _dp,
+                                       C.POINTER(Opts), C.POINTER(Stats)]),
+        "mlr_hc_be_step": (C.c_int, [G, P, B, _dp, pp, C.c_double, _dp, _dp, _dp,
+                                     C.POINTER(Opts), C.POINTER(Stats)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+# ---------------------------------------------------------------- helpers
+
+def make_grid(n, dx=None, periodic=(0, 0, 0)):
+    n = list(n)
+    dim = len(n)
+    g = Grid()
+    g.dim = dim
+    for d in range(3):
+        g.n[d] = n[d] if d < dim else 1
+        g.periodic[d] = int(periodic[d]) if d < len(periodic) else 0
+    g.dx = float(dx if dx is not None else 1.0 / n[0])
+    return g
+
+
+def cell_shape(g, ncomp):
+    if g.dim == 3:
+        return (ncomp, g.n[2] + 2, g.n[1] + 2, g.n[0] + 2)
+    return (ncomp, 1, g.n[1] + 2, g.n[0] + 2)
+
+
+def face_shape(g, d, ncomp):
+    n = [g.n[0], g.n[1], g.n[2] if g.dim == 3 else 1]
+    n[d] += 1
+    return (ncomp, n[2], n[1], n[0])
+
+
+def valid(g, x):
+    """View of the valid cells of a cell array: (ncomp, nz, ny, nx)."""
+    if g.dim == 3:
+        return x[:, 1:-1, 1:-1, 1:-1]
+    return x[:, :, 1:-1, 1:-1]
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def ptrs(arrs):
+    arr = (_dp * 3)()
+    for d, a in enumerate(arrs):
+        arr[d] = ptr(a)
+    return arr
+
+
+def default_params(**kw):
+    """Reference defaults (src/MushyLayerParams.cpp): pc = 1e-5, c = k = 1,
+    theta_e = Theta_e = 0, D_H = 1, D_Theta = 1/Le (Le given as lewis)."""
+    p = Params()
+    p.stefan = kw.get("stefan", 5.0)
+    p.compositionRatio = kw.get("compositionRatio", 2.0)
+    p.waterDistributionCoeff = kw.get("waterDistributionCoeff", 1e-5)
+    p.specificHeatRatio = kw.get("specificHeatRatio", 1.0)
+    p.heatConductivityRatio = kw.get("heatConductivityRatio", 1.0)
+    p.thetaEutectic = kw.get("thetaEutectic", 0.0)
+    p.ThetaEutectic = kw.get("ThetaEutectic", 0.0)
+    p.heatDiffusionCoeff = kw.get("heatDiffusionCoeff", 1.0)
+    p.saltDiffusionCoeff = kw.get("saltDiffusionCoeff", 1.0 / kw.get("lewis", 200.0))
+    return p
+
+
+def make_bc(spec=None, apply=1):
+    """spec: dict {(dir, side, comp): (type, value)}; default Neumann 0."""
+    b = BC()
+    for d in range(3):
+        for s in range(2):
+            for c in range(2):
+                b.type[d][s][c] = 1
+                b.val[d][s][c] = 0.0
+    for (d, s, c), (t, v) in (spec or {}).items():
+        b.type[d][s][c] = t
+        b.val[d][s][c] = v
+    b.apply = apply
+    return b
+
+
+def default_opts(**kw):
+    o = Opts()
+    lib().mlr_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+# ------------------------------------------------------------ operations
+
+def closure_point(H, C_, p):
+    t = C.c_double()
+    cl = C.c_double()
+    lib().mlr_closure_point(H, C_, C.byref(p), C.byref(t), C.byref(cl))
+    return t.value, cl.value
+
+
+def porosity_point(H, C_, p):
+    return lib().mlr_porosity_point(H, C_, C.byref(p))
+
+
+def bounding_energy(H, C_, p):
+    hs, he, hl = C.c_double(), C.c_double(), C.c_double()
+    lib().mlr_bounding_energy(H, C_, C.byref(p), C.byref(hs), C.byref(he), C.byref(hl))
+    return hs.value, he.value, hl.value
+
+
+def hc_bcoef(g, p, chi):
+    chi = np.ascontiguousarray(chi, dtype=np.float64).copy()
+    b = [np.zeros(face_shape(g, d, 2)) for d in range(g.dim)]
+    lib().mlr_hc_bcoef(C.byref(g), C.byref(p), ptr(chi), ptrs(b))
+    return b
+
+
+def porosity_field(g, p, phi):
+    chi = np.zeros(cell_shape(g, 1))
+    lib().mlr_porosity_field(C.byref(g), C.byref(p), ptr(phi), ptr(chi))
+    return chi
+
+
+def lam(g, ncomp, alpha, a, beta, b, c=None):
+    out = np.zeros(cell_shape(g, ncomp))
+    lib().mlr_lambda(C.byref(g), ncomp, alpha, ptr(a), beta, ptrs(b), ptr(c), ptr(out))
+    return out
+
+
+def hc_gsrb(g, p, bc, alpha, a, beta, b, phi, rhs, iters=1):
+    lm = lam(g, 2, alpha, a, beta, b)
+    scratch = np.zeros(cell_shape(g, 2))
+    for _ in range(iters):
+        lib().mlr_hc_gsrb(C.byref(g), C.byref(p), C.byref(bc), alpha, ptr(a), beta, ptrs(b),
+                          ptr(lm), ptr(phi), ptr(rhs), ptr(scratch))
+    return phi
+
+
+def hc_residual(g, p, bc, alpha, a, beta, b, phi, rhs):
+    res = np.zeros(cell_shape(g, 2))
+    scratch = np.zeros(cell_shape(g, 2))
+    lib().mlr_hc_residual(C.byref(g), C.byref(p), C.byref(bc), alpha, ptr(a), beta, ptrs(b),
+                          ptr(phi), ptr(rhs), ptr(res), ptr(scratch))
+    return res
+
+
+def hc_apply_op(g, p, bc, alpha, a, beta, b, phi):
+    lhs = np.zeros(cell_shape(g, 2))
+    scratch = np.zeros(cell_shape(g, 2))
+    lib().mlr_hc_apply_op(C.byref(g), C.byref(p), C.byref(bc), alpha, ptr(a), beta, ptrs(b),
+                          ptr(phi), ptr(lhs), ptr(scratch))
+    return lhs
+
+
+def hc_diffused_var(g, p, bc, phi, homogeneous=0):
+    out = np.zeros(cell_shape(g, 2))
+    lib().mlr_hc_diffused_var(C.byref(g), C.byref(p), C.byref(bc), ptr(phi), ptr(out), homogeneous)
+    return out
+
+
+def hc_restrict_residual(g, p, bc, alpha, a, beta, b, phi, rhs):
+    gc = coarsen(g, 2)
+    resc = np.zeros(cell_shape(gc, 2))
+    scratch = np.zeros(cell_shape(g, 2))
+    lib().mlr_hc_restrict_residual(C.byref(g), C.byref(p), C.byref(bc), alpha, ptr(a), beta,
+                                   ptrs(b), ptr(phi), ptr(rhs), ptr(resc), ptr(scratch))
+    return resc
+
+
+def coarsen(g, r):
+    n = [g.n[d] // r for d in range(g.dim)]
+    gc = make_grid(n, g.dx * r, [g.periodic[d] for d in range(3)])
+    return gc
+
+
+def restrict_r(g, ncomp, phif):
+    gc = coarsen(g, 2)
+    out = np.zeros(cell_shape(gc, ncomp))
+    lib().mlr_restrict_r(C.byref(g), ncomp, ptr(phif), ptr(out))
+    return out
+
+
+def prolong_linear(g, ncomp, phif, corrc):
+    corrc = corrc.copy()
+    lib().mlr_prolong_linear(C.byref(g), ncomp, ptr(phif), ptr(corrc))
+    return phif
+
+
+def prolong_constant(g, ncomp, phif, corrc):
+    lib().mlr_prolong_constant(C.byref(g), ncomp, ptr(phif), ptr(corrc))
+    return phif
+
+
+def average_faces(g, ncomp, bf, ratio):
+    gc = coarsen(g, ratio)
+    out = [np.zeros(face_shape(gc, d, ncomp)) for d in range(g.dim)]
+    lib().mlr_average_faces(C.byref(g), ncomp, ptrs(bf), ratio, ptrs(out))
+    return out
+
+
+def norm_max(g, ncomp, x):
+    return lib().mlr_norm_max(C.byref(g), ncomp, ptr(x))
+
+
+def hc_fas_solve(g, p, bc, alpha, a, beta, b, phi, rhs, opts=None):
+    st = Stats()
+    o = opts or default_opts()
+    lib().mlr_hc_fas_solve(C.byref(g), C.byref(p), C.byref(bc), alpha, ptr(a), beta, ptrs(b),
+                           ptr(phi), ptr(rhs), C.byref(o), C.byref(st))
+    return st
+
+
+def hc_be_step(g, p, bc, a, b, dt, phi_new, phi_old, src=None, opts=None):
+    st = Stats()
+    o = opts or default_opts()
+    lib().mlr_hc_be_step(C.byref(g), C.byref(p), C.byref(bc), ptr(a), ptrs(b), dt, ptr(phi_new),
+                         ptr(phi_old), ptr(src), C.byref(o), C.byref(st))
+    return st
