@@ -103,6 +103,11 @@ const char* ml_last_error(ml_ctx ctx);
 int ml_ctx_synchronize(ml_ctx ctx);
 /* the HIP stream (hipStream_t) the context launches on */
 void* ml_ctx_stream(ml_ctx ctx);
+/* HIP events on the context stream (slots 0..15) for timing a sequence of
+ * calls: record a slot, then read the milliseconds between two recorded
+ * slots (blocks until slot_b has completed). */
+int ml_ctx_event_record(ml_ctx ctx, int slot);
+int ml_ctx_event_elapsed(ml_ctx ctx, int slot_a, int slot_b, float* ms);
 int ml_buf_alloc(ml_ctx ctx, size_t bytes, void** dptr);
 int ml_buf_free(ml_ctx ctx, void* dptr);
 int ml_buf_upload(ml_ctx ctx, void* dst_dev, const void* src_host, size_t bytes);

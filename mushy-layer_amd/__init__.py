@@ -56,6 +56,8 @@ SIGNATURES = {
     "ml_last_error": (C.c_char_p, [_vp]),
     "ml_ctx_synchronize": (C.c_int, [_vp]),
     "ml_ctx_stream": (_vp, [_vp]),
+    "ml_ctx_event_record": (C.c_int, [_vp, C.c_int]),
+    "ml_ctx_event_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "ml_buf_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
     "ml_buf_free": (C.c_int, [_vp, _vp]),
     "ml_buf_upload": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
@@ -188,6 +190,15 @@ class Context:
 
     def sync(self):
         self.check(self.lib.ml_ctx_synchronize(self.h), "ml_ctx_synchronize")
+
+    def record(self, slot):
+        """Record HIP event `slot` on the library stream."""
+        self.check(self.lib.ml_ctx_event_record(self.h, slot), "ml_ctx_event_record")
+
+    def elapsed_ms(self, a, b):
+        ms = C.c_float()
+        self.check(self.lib.ml_ctx_event_elapsed(self.h, a, b, C.byref(ms)), "ml_ctx_event_elapsed")
+        return ms.value
 
     def close(self):
         if self.h:

@@ -23,6 +23,7 @@ struct ml_ctx_s {
   std::string err;
   unsigned long long* d_norm = nullptr;   // device scratch for max-norms
   unsigned long long* h_norm = nullptr;   // pinned host mirror
+  hipEvent_t events[16] = {};
 };
 
 namespace {
@@ -116,6 +117,8 @@ int ml_ctx_destroy(ml_ctx ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   if (ctx->d_norm) hipFree(ctx->d_norm);
   if (ctx->h_norm) hipHostFree(ctx->h_norm);
+  for (auto& e : ctx->events)
+    if (e) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return ML_OK;
@@ -130,6 +133,21 @@ int ml_ctx_synchronize(ml_ctx ctx) {
 }
 
 void* ml_ctx_stream(ml_ctx ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int ml_ctx_event_record(ml_ctx ctx, int slot) {
+  if (!ctx || slot < 0 || slot >= 16) return ML_EINVAL;
+  if (!ctx->events[slot]) ML_HIP(ctx, hipEventCreate(&ctx->events[slot]));
+  ML_HIP(ctx, hipEventRecord(ctx->events[slot], ctx->stream));
+  return ML_OK;
+}
+
+int ml_ctx_event_elapsed(ml_ctx ctx, int a, int b, float* ms) {
+  if (!ctx || !ms || a < 0 || a >= 16 || b < 0 || b >= 16 || !ctx->events[a] || !ctx->events[b])
+    return ML_EINVAL;
+  ML_HIP(ctx, hipEventSynchronize(ctx->events[b]));
+  ML_HIP(ctx, hipEventElapsedTime(ms, ctx->events[a], ctx->events[b]));
+  return ML_OK;
+}
 
 int ml_buf_alloc(ml_ctx ctx, size_t bytes, void** dptr) {
   if (!ctx || !dptr) return ML_EINVAL;

@@ -102,26 +102,34 @@ def gaussian_field(n, corr, seed, lo=0.05, hi=1.0):
     return lo + (hi - lo) * f
 
 
-def mushy(n=(256, 256), seed=2, St=5.0, CR=2.0, lewis=200.0, corr=0.1, theta_l=-1.0, dt=1e-3, steps=20):
-    """Mushy-layer state whose porosity is a seeded smooth random field:
-    invert chi -> (H, Theta) at a uniform liquid concentration Theta_l,
-    Theta = Theta_l (chi + pc(1-chi)) - CR(1-chi), H = chi St + theta with
-    theta = -Theta_l (c = 1, pc = 1e-5)."""
+def mushy(n=(256, 256), seed=2, St=5.0, CR=2.0, lewis=200.0, corr=0.1, theta_bottom=0.2, theta_top=1.0,
+          theta_wall=0.0, dt=1e-3, steps=20):
+    """Mushy layer whose porosity is a seeded smooth random field.
+
+    The temperature starts linear in z from theta_bottom to theta_top with
+    the liquid on the liquidus (Theta_l = -theta); each cell's (H, Theta)
+    follows from its porosity by inverting the mush relations
+    Theta = Theta_l (chi + pc(1-chi)) - CR(1-chi), H = chi St + theta (c = 1).
+    The bottom wall is held at theta_wall (the eutectic by default, colder
+    than the initial profile, so a solidification front starts there); the
+    top wall keeps theta_top.  Periodic sideways."""
     n = list(n)
     dim = len(n)
     pc = 1e-5
     chi = gaussian_field(n, corr, seed)
+    z = (np.arange(n[-1]) + 0.5) / n[-1]
+    shape = [n[-1]] + [1] * (dim - 1)
+    theta = (theta_bottom + (theta_top - theta_bottom) * z).reshape(shape)
+    theta_l = -theta
     Theta = theta_l * (chi + pc * (1 - chi)) - CR * (1 - chi)
-    H = chi * St + (-theta_l)
+    H = chi * St + theta
     phi = np.zeros(cell_shape(n, 2))
     v = valid(n, phi)
     v[0] = H.reshape(v[0].shape)
     v[1] = Theta.reshape(v[1].shape)
     vd = dim - 1
-    # cold bottom at the eutectic, top at the initial liquid state
-    t_hi, cl_hi = closure_point(St + 0.2, -1.0, St, CR)
-    bc = {(vd, 0, 0): (DIRICHLET, 0.0), (vd, 1, 0): (DIRICHLET, 0.2),
-          (vd, 0, 1): (DIRICHLET, 0.0), (vd, 1, 1): (DIRICHLET, cl_hi)}
+    bc = {(vd, 0, 0): (DIRICHLET, theta_wall), (vd, 1, 0): (DIRICHLET, theta_top),
+          (vd, 0, 1): (DIRICHLET, -theta_wall), (vd, 1, 1): (DIRICHLET, -theta_top)}
     periodic = [1] * (dim - 1) + [0]
     return dict(n=n, dim=dim, dx=1.0 / n[0], periodic=periodic, bc=bc, phi=phi, dt=dt, steps=steps,
                 params=dict(stefan=St, compositionRatio=CR, lewis=lewis), chi=chi)
