@@ -82,12 +82,17 @@ typedef struct {
 
 /* Multigrid options (defaults srcSubcycle/MushyLayerSubcycleUtils.cpp:664-676:
  * nu1 = 1, nu2 = 4, bottom = 40, max_iter = 10, eps = hang = norm_thresh =
- * 1e-10, precision FP64). */
+ * 1e-10, precision FP64).  flags: ML_SOLVE_HOST_DRIVER runs the solve through
+ * the per-operator entry points with the host C++ driver (host/FASMultiGrid.H)
+ * instead of the fused engine; both give identical FP64 results. */
 typedef struct {
   int nu1, nu2, bottom, max_iter;
   double eps, hang, norm_thresh;
   int precision;
+  int flags;
 } ml_solve_opts;
+
+enum { ML_SOLVE_HOST_DRIVER = 1 };
 
 typedef struct {
   int exit_status;   /* 1 |r|<=eps|r0|, 2 iteration max, 4 hang, 8 |r|<=norm_thresh */

@@ -37,7 +37,11 @@ class BC(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("nu1", C.c_int), ("nu2", C.c_int), ("bottom", C.c_int), ("max_iter", C.c_int),
                 ("eps", C.c_double), ("hang", C.c_double), ("norm_thresh", C.c_double),
-                ("precision", C.c_int)]
+                ("precision", C.c_int), ("flags", C.c_int)]
+
+
+SOLVE_HOST_DRIVER = 1
+PRECISION_FP64, PRECISION_FP32_REFINED = 0, 1
 
 
 class SolveStats(C.Structure):

@@ -1,7 +1,9 @@
 // ml_capi.cpp — implementation of the C ABI declared in include/mlhip.h:
 // contexts, device buffers, the nonlinear (H, Theta) operator with its MG
-// hierarchy, and the whole-solve entry points, which run the host C++ FAS
-// driver (host/FASMultiGrid.H) over the per-operator ABI calls.
+// hierarchy, and the whole-solve entry points.  Whole solves run the fused
+// FAS engine (ml_hc_engine.cpp); with ML_SOLVE_HOST_DRIVER they instead run
+// the reference-API driver (host/FASMultiGrid.H) over the per-operator ABI
+// calls, as a host C++ caller of this library would.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -13,38 +15,27 @@
 
 #include "../host/FASMultiGrid.H"
 #include "../host/HipNonLinearMultiCompOp.H"
-#include "ml_common.h"
-#include "ml_kernels.h"
-#include "mlhip.h"
+#include "ml_internal.h"
 
-struct ml_ctx_s {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  unsigned long long* d_norm = nullptr;   // device scratch for max-norms
-  unsigned long long* h_norm = nullptr;   // pinned host mirror
-  hipEvent_t events[16] = {};
-};
-
-namespace {
-
-struct HCLevel {
-  ml::Geom g;
-  double* b[3] = {nullptr, nullptr, nullptr};
-  double* a = nullptr;
-};
+namespace ml {
 
 int fail(ml_ctx ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
   return code;
 }
 
-#define ML_HIP(ctx, expr)                                                                       \
-  do {                                                                                          \
-    hipError_t _e = (expr);                                                                     \
-    if (_e != hipSuccess)                                                                       \
-      return fail((ctx), ML_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));           \
-  } while (0)
+struct HostDriver {
+  std::vector<std::unique_ptr<mlhost::HipNonLinearMultiCompOp>> owned;
+  std::vector<mlhost::HipNonLinearMultiCompOp*> ops;
+  mlhost::FASMultiGrid<mlhost::HipNonLinearMultiCompOp> fas;
+  mlhost::LevelData rhs;   // backward-Euler right-hand side
+};
+
+}  // namespace ml
+
+namespace {
+
+using ml::fail;
 
 bool valid_grid(int dim, const int n[], const int periodic[], double dx) {
   if (dim != 2 && dim != 3) return false;
@@ -56,38 +47,10 @@ bool valid_grid(int dim, const int n[], const int periodic[], double dx) {
   return true;
 }
 
-struct HCSolver {
-  std::vector<std::unique_ptr<mlhost::HipNonLinearMultiCompOp>> owned;
-  std::vector<mlhost::HipNonLinearMultiCompOp*> ops;
-  mlhost::FASMultiGrid<mlhost::HipNonLinearMultiCompOp> fas;
-  mlhost::LevelData rhs;   // backward-Euler right-hand side
-};
+void delete_host_driver(ml::HostDriver* p) { delete p; }
+void delete_engine(ml::HCEngine* p) { delete p; }
 
 }  // namespace
-
-struct ml_hc_op_s {
-  ml_ctx ctx = nullptr;
-  int dim = 2;
-  ml::ClosureP<double> cp{};
-  ml::BCDesc bc{};
-  ml_params params{};
-  double alpha = 0.0, beta = -1.0;   // factory defaults (src/AMRNonLinearMultiCompOp.cpp:1622-1633)
-  bool has_a = false;
-  std::vector<HCLevel> lev;
-  std::unique_ptr<HCSolver> solver;   // FAS driver state, built on first solve
-
-  ml::HCArgs<double> args(int depth) const {
-    ml::HCArgs<double> A;
-    A.g = lev[depth].g;
-    A.cp = cp;
-    A.bc = bc;
-    A.alpha = alpha;
-    A.beta = beta;
-    A.a = lev[depth].a;
-    for (int d = 0; d < 3; ++d) A.b[d] = lev[depth].b[d];
-    return A;
-  }
-};
 
 extern "C" {
 
@@ -224,6 +187,7 @@ void ml_default_opts(ml_solve_opts* o) {
   o->hang = 1e-10;
   o->norm_thresh = 1e-10;
   o->precision = ML_PRECISION_FP64;
+  o->flags = 0;
 }
 
 int ml_norm_max(ml_ctx ctx, int dim, const int n[], int ncomp, const double* x, double* out) {
@@ -294,7 +258,7 @@ int ml_hc_op_create(ml_ctx ctx, int dim, const int lo[], const int hi[], double 
   ml::Geom g0 = ml::make_geom(dim, n, periodic, dx);
   op->lev.resize(depth);
   for (int l = 0; l < depth; ++l) {
-    HCLevel& L = op->lev[l];
+    ml::HCLevel& L = op->lev[l];
     L.g = l == 0 ? g0 : ml::coarsen_geom(g0, 1 << l);
     for (int d = 0; d < dim; ++d) {
       void* p = nullptr;
@@ -344,9 +308,8 @@ int ml_hc_set_coefs(ml_hc_op op, const double* aCoef, const double* const bCoef[
   for (int d = 0; d < op->dim; ++d)
     ML_HIP(ctx, hipMemcpyAsync(op->lev[0].b[d], bCoef[d], sizeof(double) * 2 * g0.nface[d], hipMemcpyDeviceToDevice,
                                ctx->stream));
-  op->has_a = aCoef != nullptr;
   for (size_t l = 0; l < op->lev.size(); ++l) {
-    HCLevel& L = op->lev[l];
+    ml::HCLevel& L = op->lev[l];
     if (aCoef && !L.a) {
       void* p = nullptr;
       if (hipMalloc(&p, sizeof(double) * 2 * L.g.ncell) != hipSuccess) return fail(ctx, ML_ENOMEM, "hipMalloc aCoef");
@@ -461,22 +424,19 @@ int ml_hc_assemble_bcoef(ml_hc_op op, const double* chi, double* const bCoef[]) 
 // ------------------------------------------------------------- whole solves
 namespace {
 
-void set_params(mlhost::FASParams& p, const ml_solve_opts* o) {
-  ml_solve_opts d;
-  ml_default_opts(&d);
-  if (!o) o = &d;
-  p.nu1 = o->nu1;
-  p.nu2 = o->nu2;
-  p.bottom = o->bottom;
-  p.max_iter = o->max_iter;
-  p.eps = o->eps;
-  p.hang = o->hang;
-  p.norm_thresh = o->norm_thresh;
+void set_params(mlhost::FASParams& p, const ml_solve_opts& o) {
+  p.nu1 = o.nu1;
+  p.nu2 = o.nu2;
+  p.bottom = o.bottom;
+  p.max_iter = o.max_iter;
+  p.eps = o.eps;
+  p.hang = o.hang;
+  p.norm_thresh = o.norm_thresh;
 }
 
-HCSolver& solver_of(ml_hc_op op) {
-  if (!op->solver) {
-    std::unique_ptr<HCSolver> S(new HCSolver());
+ml::HostDriver& host_driver_of(ml_hc_op op) {
+  if (!op->host_driver) {
+    std::unique_ptr<ml::HostDriver> S(new ml::HostDriver());
     for (int l = 0; l < int(op->lev.size()); ++l) {
       S->owned.emplace_back(new mlhost::HipNonLinearMultiCompOp(op, op->ctx, l));
       S->ops.push_back(S->owned.back().get());
@@ -484,9 +444,15 @@ HCSolver& solver_of(ml_hc_op op) {
     const ml::Geom& g = op->lev[0].g;
     S->rhs.define(op->ctx, g.dim, g.n, 2);
     S->fas.define(S->ops, S->rhs);
-    op->solver = std::move(S);
+    op->host_driver = std::unique_ptr<ml::HostDriver, void (*)(ml::HostDriver*)>(S.release(), delete_host_driver);
   }
-  return *op->solver;
+  return *op->host_driver;
+}
+
+ml::HCEngine& engine_of(ml_hc_op op) {
+  if (!op->engine)
+    op->engine = std::unique_ptr<ml::HCEngine, void (*)(ml::HCEngine*)>(new ml::HCEngine(op), delete_engine);
+  return *op->engine;
 }
 
 void fill_stats(ml_solve_stats* st, const mlhost::SolveStats& s) {
@@ -497,19 +463,29 @@ void fill_stats(ml_solve_stats* st, const mlhost::SolveStats& s) {
   st->final_residual = s.final_residual;
 }
 
+int check_opts(ml_hc_op op, const ml_solve_opts* opts, ml_solve_opts& o) {
+  ml_default_opts(&o);
+  if (opts) o = *opts;
+  if (o.precision != ML_PRECISION_FP64) return fail(op->ctx, ML_ENOTSUP, "precision form not available");
+  if (o.nu1 < 0 || o.nu2 < 0 || o.bottom < 0 || o.max_iter < 0)
+    return fail(op->ctx, ML_EINVAL, "negative smoothing or iteration count");
+  return ML_OK;
+}
+
 }  // namespace
 
 int ml_hc_solve(ml_hc_op op, double* phi, const double* rhs, const ml_solve_opts* opts, ml_solve_stats* stats) {
   if (!op || !phi || !rhs) return ML_EINVAL;
-  if (opts && opts->precision != ML_PRECISION_FP64)
-    return fail(op->ctx, ML_ENOTSUP, "ml_hc_solve: precision form not available");
+  ml_solve_opts o;
+  if (int rc = check_opts(op, opts, o)) return rc;
   try {
+    if (!(o.flags & ML_SOLVE_HOST_DRIVER)) return engine_of(op).solve(phi, rhs, o, stats);
     const ml::Geom& g = op->lev[0].g;
-    HCSolver& S = solver_of(op);
+    ml::HostDriver& S = host_driver_of(op);
     mlhost::LevelData vphi, vrhs;
     vphi.alias(op->ctx, g.dim, g.n, 2, phi);
     vrhs.alias(op->ctx, g.dim, g.n, 2, const_cast<double*>(rhs));
-    set_params(S.fas.params(), opts);
+    set_params(S.fas.params(), o);
     mlhost::SolveStats st;
     S.fas.solve(vphi, vrhs, &st);
     fill_stats(stats, st);
@@ -522,16 +498,17 @@ int ml_hc_solve(ml_hc_op op, double* phi, const double* rhs, const ml_solve_opts
 int ml_hc_solve_be(ml_hc_op op, double* phi_new, const double* phi_old, const double* src, double dt,
                    const ml_solve_opts* opts, ml_solve_stats* stats) {
   if (!op || !phi_new || !phi_old) return ML_EINVAL;
-  if (opts && opts->precision != ML_PRECISION_FP64)
-    return fail(op->ctx, ML_ENOTSUP, "ml_hc_solve_be: precision form not available");
+  ml_solve_opts o;
+  if (int rc = check_opts(op, opts, o)) return rc;
   try {
+    if (!(o.flags & ML_SOLVE_HOST_DRIVER)) return engine_of(op).solve_be(phi_new, phi_old, src, dt, o, stats);
     const ml::Geom& g = op->lev[0].g;
-    HCSolver& S = solver_of(op);
+    ml::HostDriver& S = host_driver_of(op);
     mlhost::LevelData vnew, vold, vsrc;
     vnew.alias(op->ctx, g.dim, g.n, 2, phi_new);
     vold.alias(op->ctx, g.dim, g.n, 2, const_cast<double*>(phi_old));
     if (src) vsrc.alias(op->ctx, g.dim, g.n, 2, const_cast<double*>(src));
-    set_params(S.fas.params(), opts);
+    set_params(S.fas.params(), o);
     mlhost::SolveStats st;
     mlhost::backwardEulerStep(S.ops, S.fas, vnew, vold, src ? &vsrc : nullptr, dt, S.rhs, &st);
     fill_stats(stats, st);

@@ -1,0 +1,106 @@
+// ml_internal.h — library-internal state behind the opaque C-ABI handles:
+// the device context, the nonlinear (H, Theta) operator with its MG
+// hierarchy, and the fused FAS engine used by the whole-solve entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ml_common.h"
+#include "ml_kernels.h"
+#include "mlhip.h"
+
+struct ml_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  unsigned long long* d_norm = nullptr;   // device scratch for max-norms
+  unsigned long long* h_norm = nullptr;   // pinned host mirror
+  hipEvent_t events[16] = {};
+};
+
+namespace ml {
+
+int fail(ml_ctx ctx, int code, const std::string& msg);
+
+#define ML_HIP(ctx, expr)                                                                     \
+  do {                                                                                        \
+    hipError_t _e = (expr);                                                                   \
+    if (_e != hipSuccess)                                                                     \
+      return ::ml::fail((ctx), ML_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));   \
+  } while (0)
+
+struct HCLevel {
+  Geom g;
+  double* b[3] = {nullptr, nullptr, nullptr};
+  double* a = nullptr;
+};
+
+struct HostDriver;   // reference-API FAS driver state (ml_capi.cpp)
+class HCEngine;      // fused FAS engine (ml_hc_engine.cpp)
+
+}  // namespace ml
+
+struct ml_hc_op_s {
+  ml_ctx ctx = nullptr;
+  int dim = 2;
+  ml::ClosureP<double> cp{};
+  ml::BCDesc bc{};
+  ml_params params{};
+  double alpha = 0.0, beta = -1.0;   // factory defaults (src/AMRNonLinearMultiCompOp.cpp:1622-1633)
+  std::vector<ml::HCLevel> lev;
+  std::unique_ptr<ml::HostDriver, void (*)(ml::HostDriver*)> host_driver{nullptr, nullptr};
+  std::unique_ptr<ml::HCEngine, void (*)(ml::HCEngine*)> engine{nullptr, nullptr};
+
+  ml::HCArgs<double> args(int depth) const {
+    ml::HCArgs<double> A;
+    A.g = lev[depth].g;
+    A.cp = cp;
+    A.bc = bc;
+    A.alpha = alpha;
+    A.beta = beta;
+    A.a = lev[depth].a;
+    for (int d = 0; d < 3; ++d) A.b[d] = lev[depth].b[d];
+    return A;
+  }
+};
+
+namespace ml {
+
+// FAS V-cycles of the (H, Theta) operator with fused kernels: tiled levels
+// above `lc`, one single-workgroup kernel for the levels from `lc` down.
+// Same operation sequence (and FP64 results) as the reference-API driver.
+class HCEngine {
+ public:
+  explicit HCEngine(ml_hc_op op);
+  ~HCEngine();
+  HCEngine(const HCEngine&) = delete;
+  HCEngine& operator=(const HCEngine&) = delete;
+
+  int solve(double* phi, const double* rhs, const ml_solve_opts& o, ml_solve_stats* st);
+  int solve_be(double* phi_new, const double* phi_old, const double* src, double dt, const ml_solve_opts& o,
+               ml_solve_stats* st);
+
+ private:
+  int init();
+  int vcycle(double* phi, const double* rhs, const ml_solve_opts& o);
+  int residual_norm(double* phi, const double* rhs, int slot);
+  int read_norm(int slot, double* out);
+  int upload_desc(double* phi0, const double* rhs0, const ml_solve_opts& o);
+
+  ml_hc_op m_op;
+  int m_L = 0, m_lc = 0;
+  std::vector<double*> m_phi, m_rhs, m_phi0;   // per level (level 0: caller's arrays)
+  double* m_rhs_be = nullptr;                   // backward-Euler rhs of level 0
+  unsigned long long* m_slots = nullptr;       // max-norm accumulators
+  unsigned long long* m_hslots = nullptr;      // pinned mirror
+  int m_nslots = 0;
+  CoarseDesc<double>* m_desc = nullptr;        // device
+  CoarseDesc<double>* m_hdesc = nullptr;       // pinned host staging
+  bool m_ok = false;
+};
+
+}  // namespace ml

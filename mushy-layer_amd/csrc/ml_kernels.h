@@ -47,9 +47,92 @@ __device__ inline void block_max_atomic(T v, int nan, unsigned long long* out) {
   }
 }
 
+// Inputs/outputs of the tiled kernels.
+template <typename T>
+struct TileIO {
+  T* phi;
+  const T* rhs;
+  T* out;
+  const T* add;
+  unsigned long long* norm;
+  Geom gc;        // coarse level (restriction outputs / prolongation inputs)
+  T* phic;        // restricted state (optional)
+  T* phic0;       // copy of the restricted state (FAS initial coarse state)
+  const T* pc;    // prolongation-on-load: coarse state ...
+  const T* pc0;   // ... minus its initial value (nullable)
+  int npass, first, homog, ntx, nty;
+};
+
+// One MG level as seen by the single-workgroup kernels (device memory).
+template <typename T>
+struct LevelDesc {
+  Geom g;
+  T* phi;
+  T* rhs;
+  T* phi0;
+  T* tmp;
+  const T* a;
+  const T* b[3];
+};
+
+template <typename T>
+struct CoarseDesc {
+  int l0, nlev;
+  int nu1, nu2, bottom;
+  ClosureP<T> cp;
+  BCDesc bc;
+  T alpha, beta;
+  LevelDesc<T> lev[kMaxLevels];
+};
+
+__host__ __device__ inline int wrap_clamp(int v, int n, int periodic) {
+  if (v >= 0 && v < n) return v;
+  if (periodic) return ((v % n) + n) % n;
+  return v < 0 ? 0 : n - 1;
+}
+
+// Linear prolongation (ratio 2) of the coarse correction pc - pc0 (pc alone
+// when pc0 is null) to fine cell (i,j,k), added to p: PROLONG_JP weights
+// (src/AMRProjectionOpF.ChF:15-77), coarse ghosts by periodic image or the
+// no-flux rule (nearest interior index).
+template <typename T, int DIM>
+__device__ inline T prolong_value(const Geom& gc, int i, int j, int k, const T* pc, const T* pc0, T p) {
+  const T den = T(1) / (DIM == 3 ? T(64) : T(16));
+  const T fx1 = T(3) * den, fx2 = T(9) * den, fx3 = T(27) * den, f0 = T(1) * den;
+  const int ic = i / 2, jc = j / 2, kc = DIM == 3 ? k / 2 : 0;
+  const int ox = 2 * (i % 2) - 1, oy = 2 * (j % 2) - 1, oz = DIM == 3 ? 2 * (k % 2) - 1 : 0;
+  const int X0 = ic, X1 = wrap_clamp(ic + ox, gc.n[0], gc.periodic[0]);
+  const int Y0 = jc, Y1 = wrap_clamp(jc + oy, gc.n[1], gc.periodic[1]);
+  const int Z0 = kc, Z1 = DIM == 3 ? wrap_clamp(kc + oz, gc.n[2], gc.periodic[2]) : 0;
+  auto e = [&](int x, int y, int z) -> T {
+    const long long q = gc.ci(x, y, z);
+    return pc0 ? pc[q] - pc0[q] : pc[q];
+  };
+  if (DIM == 2) {
+    p = p + fx2 * e(X0, Y0, 0) + f0 * e(X1, Y1, 0);
+    p = p + fx1 * (e(X1, Y0, 0) + e(X0, Y1, 0));
+  } else {
+    p = p + fx3 * e(X0, Y0, Z0) + f0 * e(X1, Y1, Z1);
+    p = p + fx2 * (e(X1, Y0, Z0) + e(X0, Y1, Z0) + e(X0, Y0, Z1));
+    p = p + fx1 * (e(X1, Y1, Z0) + e(X0, Y1, Z1) + e(X1, Y0, Z1));
+  }
+  return p;
+}
+
 // ---- nonlinear (H, Theta) operator (ml_hc_kernels.hip)
 template <typename T>
 hipError_t hc_relax(const HCArgs<T>& A, T* phi, const T* rhs, int npass, int first, hipStream_t s);
+// relax tiles with the linear prolongation of pc - pc0 (level gc) fused on load
+template <typename T>
+hipError_t hc_relax_tiles(const HCArgs<T>& A, T* phi, const T* rhs, int npass, int first, const Geom* gc,
+                          const T* pc, const T* pc0, hipStream_t s);
+template <typename T>
+hipError_t hc_relax_small(const CoarseDesc<T>* dev_desc, int dim, int level, int npass, hipStream_t s);
+template <typename T>
+hipError_t hc_coarse_vcycle(const CoarseDesc<T>* dev_desc, int dim, int apply_first, hipStream_t s);
+template <typename T>
+hipError_t hc_restrict_all(const HCArgs<T>& A, const Geom& gc, T* phi, const T* rhs, T* resc, T* phic, T* phic0,
+                           int homog, hipStream_t s);
 template <typename T>
 hipError_t hc_residual(const HCArgs<T>& A, T* phi, const T* rhs, T* res, unsigned long long* norm,
                        int homog, hipStream_t s);

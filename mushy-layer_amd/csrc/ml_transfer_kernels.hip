@@ -8,12 +8,6 @@ namespace ml {
 
 namespace {
 
-__device__ inline int wrap_clamp(int v, int n, int periodic) {
-  if (v >= 0 && v < n) return v;
-  if (periodic) return ((v % n) + n) % n;
-  return v < 0 ? 0 : n - 1;
-}
-
 __device__ inline void unflatten(long long c, int nx, int ny, int& i, int& j, int& k) {
   i = int(c % nx);
   const long long r = c / nx;
@@ -60,35 +54,15 @@ template <typename T, int DIM>
 __global__ void prolong_linear_kernel(Geom gf, Geom gc, int ncomp, T* __restrict__ phif,
                                       const T* __restrict__ phic, const T* __restrict__ phic0) {
   const long long N = gf.nvalid();
-  const T den = T(1) / (DIM == 3 ? T(64) : T(16));
-  const T fx1 = T(3) * den, fx2 = T(9) * den, fx3 = T(27) * den, f0 = T(1) * den;
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N;
        c += (long long)gridDim.x * blockDim.x) {
     int i, j, k;
     unflatten(c, gf.n[0], gf.n[1], i, j, k);
-    const int ic = i / 2, jc = j / 2, kc = DIM == 3 ? k / 2 : 0;
-    const int ox = 2 * (i % 2) - 1, oy = 2 * (j % 2) - 1, oz = DIM == 3 ? 2 * (k % 2) - 1 : 0;
-    const int X0 = ic, X1 = wrap_clamp(ic + ox, gc.n[0], gc.periodic[0]);
-    const int Y0 = jc, Y1 = wrap_clamp(jc + oy, gc.n[1], gc.periodic[1]);
-    const int Z0 = kc, Z1 = DIM == 3 ? wrap_clamp(kc + oz, gc.n[2], gc.periodic[2]) : 0;
     const long long qf = gf.ci(i, j, k);
     for (int n = 0; n < ncomp; ++n) {
       const T* pc = phic + n * gc.ncell;
       const T* p0 = phic0 ? phic0 + n * gc.ncell : nullptr;
-      auto e = [&](int x, int y, int z) -> T {
-        const long long q = gc.ci(x, y, z);
-        return phic0 ? pc[q] - p0[q] : pc[q];
-      };
-      T p = phif[qf + n * gf.ncell];
-      if (DIM == 2) {
-        p = p + fx2 * e(X0, Y0, 0) + f0 * e(X1, Y1, 0);
-        p = p + fx1 * (e(X1, Y0, 0) + e(X0, Y1, 0));
-      } else {
-        p = p + fx3 * e(X0, Y0, Z0) + f0 * e(X1, Y1, Z1);
-        p = p + fx2 * (e(X1, Y0, Z0) + e(X0, Y1, Z0) + e(X0, Y0, Z1));
-        p = p + fx1 * (e(X1, Y1, Z0) + e(X0, Y1, Z1) + e(X1, Y0, Z1));
-      }
-      phif[qf + n * gf.ncell] = p;
+      phif[qf + n * gf.ncell] = prolong_value<T, DIM>(gc, i, j, k, pc, p0, phif[qf + n * gf.ncell]);
     }
   }
 }

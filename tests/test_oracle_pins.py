@@ -1,0 +1,410 @@
+"""Pins of the CPU oracle against what the mathematics fixes, independent of
+its own formulas: closed forms of the phase diagram, an independent root
+solve of the mush relations, dense linear algebra for the liquid (linear)
+case, textbook red/black Gauss-Seidel, discrete conservation, the analytic
+steady conduction profile, dense Newton on a small nonlinear case, and exact
+reproduction of (bi)linear fields by the transfers.
+
+Each test is chosen so that a plausible slip in the oracle (a dropped term,
+a wrong sign or index, a transposed face) fails it.
+"""
+import numpy as np
+import pytest
+from scipy.optimize import brentq
+
+from oracle import mlref as O
+
+P0 = dict(stefan=5.0, compositionRatio=2.0, lewis=200.0)
+
+
+def params(**kw):
+    d = dict(P0)
+    d.update(kw)
+    return O.default_params(**d)
+
+
+# ------------------------------------------------------------- phase diagram
+
+@pytest.mark.parametrize("St,CR,pc", [(5.0, 2.0, 1e-5), (1.0, 20.0, 1e-5), (10.0, 1.0, 0.1)])
+def test_bounding_energies_closed_form(St, CR, pc):
+    p = params(stefan=St, compositionRatio=CR, waterDistributionCoeff=pc)
+    for C in [-CR - 0.5, -CR, -1.5, -0.3, 0.0]:
+        Hs, He, Hl = O.bounding_energy(0.0, C, p)
+        # solidus: H_s = max(0, (-C - CR)/pc) for c = 1, theta_e = 0; zero when C < -CR
+        assert Hs == (0.0 if C < -CR else max(0.0, (-C - CR) / pc))
+        # eutectic: porosity at the eutectic (CR + C)/CR times St
+        assert He == pytest.approx((CR + C) / CR * St, rel=1e-15, abs=1e-15)
+        # liquidus: H_l = St - C
+        assert Hl == pytest.approx(St - C, rel=1e-15)
+
+
+def test_closure_regimes_closed_forms():
+    p = params()
+    St = 5.0
+    # liquid: theta = H - St, Theta_l = Theta
+    for H, C in [(6.2, -1.0), (7.0, -0.5), (10.0, -1.9)]:
+        t, cl = O.closure_point(H, C, p)
+        assert t == H - St and cl == C
+        assert O.porosity_point(H, C, p) == 1.0
+    # solid (H below the solidus): theta = H / c, Theta_l = Theta_e = 0
+    for H, C in [(-1.0, -1.0), (-3.0, -2.5), (0.0, -1.0)]:
+        t, cl = O.closure_point(H, C, p)
+        assert t == H and cl == 0.0
+        assert O.porosity_point(H, C, p) == 0.0
+    # eutectic: theta = theta_e = 0, porosity = H / St
+    for H, C in [(1.0, -1.0), (2.4, -1.0)]:
+        t, cl = O.closure_point(H, C, p)
+        assert t == 0.0 and cl == 0.0
+        assert O.porosity_point(H, C, p) == pytest.approx(H / St)
+
+
+def _mush_state(chi, theta, St, CR, pc):
+    """Forward mush relations (c = 1, theta_e = Theta_e = 0):
+    Theta_l = -theta, Theta = Theta_l(chi + pc(1-chi)) - CR(1-chi),
+    H = chi*St + theta."""
+    Tl = -theta
+    Theta = Tl * (chi + pc * (1 - chi)) - CR * (1 - chi)
+    H = chi * St + theta
+    return H, Theta
+
+
+@pytest.mark.parametrize("St,CR,pc", [(5.0, 2.0, 1e-5), (3.0, 10.0, 1e-3), (8.0, 1.5, 1e-4)])
+def test_mush_inversion_round_trip(St, CR, pc):
+    """The closure inverts the forward mush relations: (chi, theta) ->
+    (H, Theta) -> closure gives theta back and the porosity chi back."""
+    p = params(stefan=St, compositionRatio=CR, waterDistributionCoeff=pc)
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        chi = rng.uniform(0.02, 0.98)
+        theta = rng.uniform(0.01, 1.5)
+        H, Theta = _mush_state(chi, theta, St, CR, pc)
+        t, cl = O.closure_point(H, Theta, p)
+        assert t == pytest.approx(theta, rel=1e-9, abs=1e-11)
+        assert cl == pytest.approx(-theta, rel=1e-9, abs=1e-11)
+        assert O.porosity_point(H, Theta, p) == pytest.approx(chi, rel=1e-9, abs=1e-11)
+
+
+def test_mush_porosity_matches_independent_root_solve():
+    """Porosity from the quadratic formula agrees with a bracketing root
+    solve of H = chi*St + theta(chi), theta(chi) = -(Theta + CR(1-chi)) /
+    (chi + pc(1-chi)) -- the physical relations, not the quadratic."""
+    St, CR, pc = 5.0, 2.0, 1e-5
+    p = params()
+    rng = np.random.default_rng(1)
+    n = 0
+    while n < 200:
+        H = rng.uniform(-1, 8)
+        C = rng.uniform(-1.9, -0.1)
+        Hs, He, Hl = O.bounding_energy(H, C, p)
+        if not (He < H < Hl):
+            continue
+        f = lambda chi: H - chi * St + (C + CR * (1 - chi)) / (chi + pc * (1 - chi))
+        chi_root = brentq(f, 1e-9, 1.0, xtol=1e-15, rtol=1e-15)
+        assert O.porosity_point(H, C, p) == pytest.approx(chi_root, rel=1e-10, abs=1e-12)
+        n += 1
+
+
+def test_closure_continuous_across_regime_boundaries():
+    p = params()
+    for C in [-1.5, -1.0, -0.3]:
+        Hs, He, Hl = O.bounding_energy(0.0, C, p)
+        for Hb in [He, Hl]:
+            lo = O.closure_point(Hb * (1 - 1e-12), C, p)
+            hi = O.closure_point(Hb * (1 + 1e-12), C, p)
+            assert lo[0] == pytest.approx(hi[0], abs=1e-9)
+            assert lo[1] == pytest.approx(hi[1], abs=1e-9)
+
+
+def test_porosity_in_unit_interval():
+    """Physical states (-CR < Theta < Theta_e = 0) have porosity in [0, 1]."""
+    p = params()
+    rng = np.random.default_rng(2)
+    for _ in range(2000):
+        H = rng.uniform(-3, 9)
+        C = rng.uniform(-1.99, -1e-3)
+        chi = O.porosity_point(H, C, p)
+        assert -1e-12 <= chi <= 1 + 1e-12
+
+
+# ------------------------------------------------------ linear (liquid) case
+
+def _grid(n, periodic=(1, 0, 0)):
+    return O.make_grid(list(n), 1.0 / n[0], list(periodic))
+
+
+def _dense_liquid_matrix(g, n, alpha, beta, b, bc, St, periodic_y=False):
+    """Independent assembly of the HC operator in the all-liquid regime,
+    where theta = H - St and Theta_l = Theta: two decoupled linear problems
+    L(x) = A x + f.  Unknowns: valid cells (i fastest), comp 0 then comp 1.
+    Each face contributes c*(u_nb - u) with c = -beta*b/dx^2 and
+    u = x - shift; wall ghosts follow the Dirichlet (u_g = 2v - u) and
+    Neumann (u_g = u + isign*dx*v) rules; x (and optionally y) periodic."""
+    nx, ny = n
+    N = nx * ny
+    A = np.zeros((2 * N, 2 * N))
+    f = np.zeros(2 * N)
+    dx = g.dx
+    for comp in range(2):
+        s = St if comp == 0 else 0.0
+        b0 = b[0][comp, 0]     # (ny, nx+1)
+        b1 = b[1][comp, 0]     # (ny+1, nx)
+        for j in range(ny):
+            for i in range(nx):
+                r = comp * N + i + nx * j
+                A[r, r] += alpha
+                for d, side, bf in ((0, 0, b0[j, i]), (0, 1, b0[j, i + 1]), (1, 0, b1[j, i]), (1, 1, b1[j + 1, i])):
+                    c = -beta * bf / dx ** 2
+                    ii, jj = i, j
+                    if d == 0:
+                        ii = (i + (1 if side else -1)) % nx
+                    else:
+                        jj = j + (1 if side else -1)
+                        if periodic_y:
+                            jj %= ny
+                    if d == 0 or 0 <= jj < ny:
+                        A[r, comp * N + ii + nx * jj] += c
+                        A[r, r] -= c
+                    else:
+                        t, v = bc[(1, side, comp)]
+                        if t == 0:
+                            A[r, r] -= 2 * c
+                            f[r] += c * (2 * v + 2 * s)
+                        else:
+                            f[r] += c * (1 if side else -1) * dx * v
+    return A, f
+
+
+def _liquid_case(n=(8, 8), periodic_y=False):
+    g = _grid(n, periodic=(1, 1 if periodic_y else 0, 0))
+    p = params()
+    bc_spec = {(1, 0, 0): (0, 1.0), (1, 1, 0): (0, 1.5), (1, 0, 1): (1, 0.3), (1, 1, 1): (0, -0.8)}
+    bc = O.make_bc(bc_spec)
+    rng = np.random.default_rng(3)
+    phi = np.zeros(O.cell_shape(g, 2))
+    v = O.valid(g, phi)
+    v[0] = 6.5 + 0.5 * rng.random(v[0].shape)
+    v[1] = -0.8 + 0.05 * rng.random(v[1].shape)
+    chi = np.ones(O.cell_shape(g, 1)) * 0.999
+    chi[:, :, 1:-1, 1:-1] = 0.5 + 0.5 * rng.random((1, 1, n[1], n[0]))
+    b = O.hc_bcoef(g, p, chi)
+    return g, p, bc, bc_spec, phi, b
+
+
+def test_liquid_operator_matches_dense_assembly():
+    g, p, bc, spec, phi, b = _liquid_case()
+    n = (g.n[0], g.n[1])
+    alpha, beta = 1.0, -1e-3
+    A, f = _dense_liquid_matrix(g, n, alpha, beta, b, spec, 5.0)
+    x = np.concatenate([O.valid(g, phi)[c].ravel() for c in range(2)])
+    Lx = O.hc_apply_op(g, p, bc, alpha, None, beta, b, phi.copy())
+    Lv = np.concatenate([O.valid(g, Lx)[c].ravel() for c in range(2)])
+    np.testing.assert_allclose(Lv, A @ x + f, rtol=1e-12, atol=1e-10)
+
+
+def test_liquid_fas_solve_matches_dense_solve():
+    g, p, bc, spec, phi, b = _liquid_case()
+    n = (g.n[0], g.n[1])
+    alpha, beta = 1.0, -1e-3
+    A, f = _dense_liquid_matrix(g, n, alpha, beta, b, spec, 5.0)
+    rhs = phi.copy()
+    r = np.concatenate([O.valid(g, rhs)[c].ravel() for c in range(2)])
+    x_dense = np.linalg.solve(A, r - f)
+    sol = phi.copy()
+    st = O.hc_fas_solve(g, p, bc, alpha, None, beta, b, sol, rhs, O.default_opts(max_iter=40, eps=1e-14,
+                                                                                    norm_thresh=1e-14))
+    xs = np.concatenate([O.valid(g, sol)[c].ravel() for c in range(2)])
+    np.testing.assert_allclose(xs, x_dense, rtol=1e-11, atol=1e-11)
+    assert st.exit_status & (1 | 8)
+
+
+def test_gsrb_is_textbook_red_black_gauss_seidel_in_liquid():
+    """In the liquid regime the closure slope is 1, so on a doubly periodic
+    grid lambda is the inverse diagonal and one levelGSRB iteration is
+    red/black Gauss-Seidel on the assembled matrix."""
+    g, p, bc, spec, phi, b = _liquid_case(periodic_y=True)
+    n = (g.n[0], g.n[1])
+    alpha, beta = 1.0, -1e-3
+    A, f = _dense_liquid_matrix(g, n, alpha, beta, b, spec, 5.0, periodic_y=True)
+    rng = np.random.default_rng(5)
+    rhs = phi + 1e-3 * rng.standard_normal(phi.shape)
+    r = np.concatenate([O.valid(g, rhs)[c].ravel() for c in range(2)])
+    x = np.concatenate([O.valid(g, phi)[c].ravel() for c in range(2)])
+    N = n[0] * n[1]
+    for colour in (0, 1):
+        xo = x.copy()
+        for comp in range(2):
+            for j in range(n[1]):
+                for i in range(n[0]):
+                    if (i + j + colour) % 2:
+                        continue
+                    q = comp * N + i + n[0] * j
+                    x[q] = xo[q] + (r[q] - f[q] - A[q] @ xo) / A[q, q]
+    out = O.hc_gsrb(g, p, bc, alpha, None, beta, b, phi.copy(), rhs, iters=1)
+    xs = np.concatenate([O.valid(g, out)[c].ravel() for c in range(2)])
+    np.testing.assert_allclose(xs, x, rtol=1e-13, atol=1e-13)
+
+
+def test_steady_conduction_is_the_linear_profile():
+    """Liquid column with Dirichlet theta at the walls: the backward-Euler
+    steady state is the linear profile between the wall values, exactly
+    representable with the order-1 Dirichlet ghost (cf. the reference's
+    test/diffusiveGrowth/testDiffusionAndBCs.py:58-65)."""
+    n = (4, 32)
+    g = O.make_grid(list(n), 1.0 / n[1], [1, 0, 0])    # unit-height column
+    p = params()
+    tlo, thi = 0.4, 1.3
+    spec = {(1, 0, 0): (0, tlo), (1, 1, 0): (0, thi), (1, 0, 1): (1, 0.0), (1, 1, 1): (1, 0.0)}
+    bc = O.make_bc(spec)
+    phi = np.zeros(O.cell_shape(g, 2))
+    O.valid(g, phi)[0] = 6.0
+    O.valid(g, phi)[1] = -0.2          # liquidus at theta = 0.2: the column stays liquid
+    chi = np.ones(O.cell_shape(g, 1))
+    b = O.hc_bcoef(g, p, chi)
+    for _ in range(6):
+        old = phi.copy()
+        O.hc_be_step(g, p, bc, None, b, 10.0, phi, old, opts=O.default_opts(max_iter=100, eps=1e-15,
+                                                                           norm_thresh=1e-13))
+    z = (np.arange(n[1]) + 0.5) / n[1]
+    theta = O.valid(g, phi)[0, 0, :, 0] - 5.0
+    np.testing.assert_allclose(theta, tlo + (thi - tlo) * z, atol=1e-9)
+
+
+def test_no_flux_conservation():
+    """With no-flux walls and a periodic side the flux form telescopes: a
+    converged backward-Euler step conserves sum(H) and sum(Theta)."""
+    g = _grid((16, 16))
+    p = params()
+    spec = {(1, s, c): (1, 0.0) for s in (0, 1) for c in (0, 1)}
+    bc = O.make_bc(spec)
+    rng = np.random.default_rng(6)
+    phi = np.zeros(O.cell_shape(g, 2))
+    v = O.valid(g, phi)
+    v[0] = rng.uniform(1.0, 7.0, v[0].shape)
+    v[1] = rng.uniform(-1.5, -0.5, v[1].shape)
+    chi = O.porosity_field(g, p, phi)
+    b = O.hc_bcoef(g, p, chi)
+    old = phi.copy()
+    st = O.hc_be_step(g, p, bc, None, b, 1e-3, phi, old, opts=O.default_opts(max_iter=200, hang=-1.0,
+                                                                            eps=1e-13))
+    assert st.exit_status & (1 | 8)
+    for c in range(2):
+        s0, s1 = O.valid(g, old)[c].sum(), O.valid(g, phi)[c].sum()
+        assert abs(s1 - s0) <= 1e-9 * max(1.0, abs(s0))
+    assert np.abs(O.valid(g, phi) - O.valid(g, old)).max() > 1e-4   # something happened
+
+
+def test_nonlinear_fas_matches_dense_newton():
+    """Small phase-change problem: the FAS solution satisfies the discrete
+    nonlinear system as found by an independent dense Newton iteration with a
+    finite-difference Jacobian on the residual."""
+    n = (8, 8)
+    g = _grid(n)
+    p = params()
+    spec = {(1, 0, 0): (0, -0.5), (1, 1, 0): (0, 1.0), (1, 0, 1): (0, 0.0), (1, 1, 1): (0, -1.0)}
+    bc = O.make_bc(spec)
+    phi = np.zeros(O.cell_shape(g, 2))
+    v = O.valid(g, phi)
+    z = ((np.arange(n[1]) + 0.5) / n[1]).reshape(-1, 1)
+    v[0] = 1.0 + 5.0 * z
+    v[1] = -1.0
+    chi = O.porosity_field(g, p, phi)
+    b = O.hc_bcoef(g, p, chi)
+    alpha, beta = 1.0, -2e-3
+    rhs = phi.copy()
+    sol = phi.copy()
+    st = O.hc_fas_solve(g, p, bc, alpha, None, beta, b, sol, rhs,
+                        O.default_opts(max_iter=200, hang=-1.0, eps=1e-14, norm_thresh=1e-13))
+    assert st.exit_status & (1 | 8)
+
+    def R(xv):
+        ph = phi.copy()
+        w = O.valid(g, ph)
+        w[0] = xv[:64].reshape(8, 8)
+        w[1] = xv[64:].reshape(8, 8)
+        res = O.hc_residual(g, p, bc, alpha, None, beta, b, ph, rhs)
+        return np.concatenate([O.valid(g, res)[c].ravel() for c in range(2)])
+
+    x = np.concatenate([O.valid(g, phi)[c].ravel() for c in range(2)])
+    for _ in range(30):
+        r = R(x)
+        if np.abs(r).max() < 1e-13:
+            break
+        J = np.empty((128, 128))
+        h = 1e-7
+        for k in range(128):
+            e = np.zeros(128)
+            e[k] = h
+            J[:, k] = (R(x - e) - R(x + e)) / (2 * h)   # R = rhs - L  =>  dL/dx = -dR/dx
+        x = x + np.linalg.solve(J, r)
+    xs = np.concatenate([O.valid(g, sol)[c].ravel() for c in range(2)])
+    np.testing.assert_allclose(xs, x, rtol=1e-9, atol=1e-9)
+
+
+# ------------------------------------------------------------- transfers
+
+def test_prolongation_reproduces_bilinear_fields():
+    """Interior fine cells receive exactly the bilinear interpolant of a
+    bilinear coarse correction; a constant is reproduced everywhere (the
+    weights sum to one)."""
+    g = _grid((16, 16), periodic=(0, 0, 0))
+    gc = O.coarsen(g, 2)
+    corr = np.zeros(O.cell_shape(gc, 1))
+    Xc = (np.arange(8) + 0.5) / 8
+    cv = O.valid(gc, corr)
+    cv[0, 0] = 0.3 + 1.7 * Xc.reshape(1, -1) - 0.9 * Xc.reshape(-1, 1) + 0.5 * np.outer(Xc, Xc)
+    phi = np.zeros(O.cell_shape(g, 1))
+    O.prolong_linear(g, 1, phi, corr)
+    Xf = (np.arange(16) + 0.5) / 16
+    exact = 0.3 + 1.7 * Xf.reshape(1, -1) - 0.9 * Xf.reshape(-1, 1) + 0.5 * np.outer(Xf, Xf)
+    got = O.valid(g, phi)[0, 0]
+    np.testing.assert_allclose(got[1:-1, 1:-1], exact[1:-1, 1:-1], rtol=0, atol=1e-14)
+    one = np.ones(O.cell_shape(gc, 1))
+    phi = np.zeros(O.cell_shape(g, 1))
+    O.prolong_linear(g, 1, phi, one)
+    np.testing.assert_array_equal(O.valid(g, phi), 1.0)
+
+
+def test_restriction_averages_children():
+    g = _grid((8, 8))
+    rng = np.random.default_rng(7)
+    phi = np.zeros(O.cell_shape(g, 2))
+    O.valid(g, phi)[:] = rng.standard_normal(O.valid(g, phi).shape)
+    out = O.restrict_r(g, 2, phi)
+    v = O.valid(g, phi)[:, 0]
+    expect = 0.25 * (v[:, 0::2, 0::2] + v[:, 0::2, 1::2] + v[:, 1::2, 0::2] + v[:, 1::2, 1::2])
+    np.testing.assert_allclose(O.valid(O.coarsen(g, 2), out)[:, 0], expect, rtol=1e-15, atol=1e-15)
+
+
+def test_face_averaging_preserves_uniform_and_averages_subfaces():
+    g = _grid((16, 8), periodic=(0, 0, 0))
+    rng = np.random.default_rng(8)
+    bf = [rng.standard_normal(O.face_shape(g, d, 2)) for d in range(2)]
+    bc = O.average_faces(g, 2, bf, 4)
+    # direction 0: coarse face I at fine face 4I, averaging 4 fine rows
+    expect0 = bf[0][:, :, :, ::4].reshape(2, 1, 2, 4, 5).mean(axis=3)
+    np.testing.assert_allclose(bc[0], expect0, rtol=1e-14)
+    expect1 = bf[1][:, :, ::4, :].reshape(2, 1, 3, 4, 4).mean(axis=4)
+    np.testing.assert_allclose(bc[1], expect1, rtol=1e-14)
+
+
+def test_hc_coefficients_from_porosity():
+    """b_Theta = -chi_f/Le with chi_f the geometric face mean; with k = 1 the
+    heat coefficient is -D_H whatever the porosity; wall faces take the
+    adjacent cell's porosity."""
+    g = _grid((6, 4))
+    p = params()
+    rng = np.random.default_rng(9)
+    chi = np.zeros(O.cell_shape(g, 1))
+    cv = rng.uniform(0.05, 1.0, (4, 6))
+    chi[0, 0, 1:-1, 1:-1] = cv
+    b = O.hc_bcoef(g, p, chi)
+    np.testing.assert_allclose(b[0][0], -1.0, rtol=1e-15)
+    np.testing.assert_allclose(b[1][0], -1.0, rtol=1e-15)
+    # x faces (periodic): face i between cells i-1 (mod n) and i
+    left = np.roll(cv, 1, axis=1)
+    np.testing.assert_allclose(b[0][1, 0, :, :6], -np.sqrt(left * cv) / 200.0, rtol=1e-14)
+    np.testing.assert_allclose(b[0][1, 0, :, 6], b[0][1, 0, :, 0], rtol=0)
+    # y faces: interior between rows, walls copy the adjacent row
+    np.testing.assert_allclose(b[1][1, 0, 1:4, :], -np.sqrt(cv[:-1] * cv[1:]) / 200.0, rtol=1e-14)
+    np.testing.assert_allclose(b[1][1, 0, 0, :], -cv[0] / 200.0, rtol=1e-14)
+    np.testing.assert_allclose(b[1][1, 0, 4, :], -cv[-1] / 200.0, rtol=1e-14)
