@@ -466,7 +466,10 @@ void fill_stats(ml_solve_stats* st, const mlhost::SolveStats& s) {
 int check_opts(ml_hc_op op, const ml_solve_opts* opts, ml_solve_opts& o) {
   ml_default_opts(&o);
   if (opts) o = *opts;
-  if (o.precision != ML_PRECISION_FP64) return fail(op->ctx, ML_ENOTSUP, "precision form not available");
+  if (o.precision != ML_PRECISION_FP64 && o.precision != ML_PRECISION_FP32_REFINED)
+    return fail(op->ctx, ML_EINVAL, "unknown precision form");
+  if (o.precision != ML_PRECISION_FP64 && (o.flags & ML_SOLVE_HOST_DRIVER))
+    return fail(op->ctx, ML_ENOTSUP, "the host driver runs the FP64 form only");
   if (o.nu1 < 0 || o.nu2 < 0 || o.bottom < 0 || o.max_iter < 0)
     return fail(op->ctx, ML_EINVAL, "negative smoothing or iteration count");
   return ML_OK;

@@ -22,7 +22,138 @@ constexpr int kSmallCells = 1024;
 
 HCEngine::HCEngine(ml_hc_op op) : m_op(op) { m_ok = init() == ML_OK; }
 
+// ------------------------------------------------------------------------
+// FP32-refined form.  Each outer iteration (FP64): phi += e; r = rhs - L(phi)
+// with the nonlinear operator; M = closure Jacobian.  Then one FP32 linear
+// V-cycle on J e = r (tiled levels above m_lc, one workgroup below, FP64
+// bottom solve).  Stopping rule and status bits as for the FP64 FAS solve,
+// with iterations counting outer iterations.
+int HCEngine::init_refined() {
+  ml_ctx ctx = m_op->ctx;
+  if (m_lc >= m_L) return fail(ctx, ML_ENOTSUP, "FP32-refined form needs an MG level of at most 1024 cells");
+  if (m_op->lev[m_L - 1].g.nvalid() > 512)
+    return fail(ctx, ML_ENOTSUP, "FP32-refined form needs a coarsest level of at most 512 cells");
+  m_lin.assign(m_L, LinLevel<float>{});
+  auto alloc = [&](size_t n, float** p) -> int {
+    void* v = nullptr;
+    ML_HIP(ctx, hipMalloc(&v, n * sizeof(float)));
+    *p = static_cast<float*>(v);
+    m_lin_owned.push_back(*p);
+    ML_HIP(ctx, hipMemsetAsync(v, 0, n * sizeof(float), ctx->stream));
+    return ML_OK;
+  };
+  for (int l = 0; l < m_L; ++l) {
+    LinLevel<float>& L = m_lin[l];
+    const Geom& g = m_op->lev[l].g;
+    L.g = g;
+    float* p = nullptr;
+    int rc;
+    if ((rc = alloc(2 * g.ncell, &p))) return rc;
+    L.e = p;
+    if ((rc = alloc(2 * g.ncell, &p))) return rc;
+    L.r = p;
+    if ((rc = alloc(4 * g.ncell, &p))) return rc;
+    L.M = p;
+    for (int d = 0; d < 3; ++d) {
+      L.b[d] = nullptr;
+      if (d < g.dim) {
+        if ((rc = alloc(2 * g.nface[d], &p))) return rc;
+        L.b[d] = p;
+      }
+    }
+    L.a = nullptr;
+  }
+  void* v = nullptr;
+  ML_HIP(ctx, hipMalloc(&v, sizeof(LinDesc)));
+  m_lindesc = static_cast<LinDesc*>(v);
+  ML_HIP(ctx, hipHostMalloc(&v, sizeof(LinDesc), hipHostMallocDefault));
+  m_hlindesc = static_cast<LinDesc*>(v);
+  m_refined_ok = true;
+  return ML_OK;
+}
+
+int HCEngine::lin_vcycle(const ml_solve_opts& o) {
+  ml_ctx ctx = m_op->ctx;
+  hipStream_t s = ctx->stream;
+  const float alpha = float(m_op->alpha), beta = float(m_op->beta);
+  for (int l = 0; l < m_lc; ++l) {
+    ML_HIP(ctx, lin_relax(m_lin[l], m_op->bc, alpha, beta, 2 * o.nu1, true, nullptr, nullptr, s));
+    ML_HIP(ctx, lin_restrict(m_lin[l], m_lin[l + 1], m_op->bc, alpha, beta, s));
+  }
+  ML_HIP(ctx, lin_coarse_vcycle(m_lindesc, m_op->dim, s));
+  for (int l = m_lc - 1; l >= 0; --l) {
+    const Geom gc = m_lin[l + 1].g;
+    ML_HIP(ctx, lin_relax(m_lin[l], m_op->bc, alpha, beta, 2 * o.nu2, false, &gc, m_lin[l + 1].e, s));
+  }
+  return ML_OK;
+}
+
+int HCEngine::solve_refined(double* phi, const double* rhs, const ml_solve_opts& o, ml_solve_stats* st) {
+  ml_ctx ctx = m_op->ctx;
+  hipStream_t s = ctx->stream;
+  if (!m_refined_ok) {
+    int rc = init_refined();
+    if (rc) return rc;
+  }
+  if (o.max_iter + 1 > m_nslots) return fail(ctx, ML_EINVAL, "max_iter too large");
+  if (o.nu1 < 1 || o.nu2 < 1 || o.bottom < 1)
+    return fail(ctx, ML_EINVAL, "FP32-refined form needs nu1, nu2, bottom >= 1");
+  // FP32 copies of the face coefficients (and aCoef) of every level
+  for (int l = 0; l < m_L; ++l) {
+    const Geom& g = m_op->lev[l].g;
+    for (int d = 0; d < g.dim; ++d)
+      ML_HIP(ctx, cast_f64_f32(2 * g.nface[d], m_op->lev[l].b[d], const_cast<float*>(m_lin[l].b[d]), s));
+    if (m_op->lev[l].a) return fail(ctx, ML_ENOTSUP, "FP32-refined form supports aCoef == 1 only");
+  }
+  ML_HIP(ctx, hipStreamSynchronize(s));   // the pinned descriptor may still be in flight
+  LinDesc& D = *m_hlindesc;
+  std::memset(&D, 0, sizeof(D));
+  D.l0 = m_lc;
+  D.nlev = m_L;
+  D.nu1 = o.nu1;
+  D.nu2 = o.nu2;
+  D.bottom = o.bottom;
+  D.bc = m_op->bc;
+  D.alpha = m_op->alpha;
+  D.beta = m_op->beta;
+  for (int l = 0; l < m_L; ++l) D.lev[l] = m_lin[l];
+  for (int d = 0; d < 3; ++d) D.b64[d] = m_op->lev[m_L - 1].b[d];
+  D.a64 = nullptr;
+  ML_HIP(ctx, hipMemcpyAsync(m_lindesc, m_hlindesc, sizeof(D), hipMemcpyHostToDevice, s));
+  ML_HIP(ctx, zero_u64(m_slots, o.max_iter + 1, s));
+
+  const HCArgs<double> A = m_op->args(0);
+  ML_HIP(ctx, hc_residual_jacobian(A, phi, rhs, nullptr, m_lin[0].r, m_lin[0].M, m_slots, s));
+  double r0 = 0;
+  int rc;
+  if ((rc = read_norm(0, &r0))) return rc;
+  double norm = r0, prev = r0;
+  int iter = 0, status = 0;
+  if (r0 <= o.norm_thresh) status = 8;
+  while (status == 0 && iter < o.max_iter) {
+    if ((rc = lin_vcycle(o))) return rc;
+    ++iter;
+    ML_HIP(ctx, hc_residual_jacobian(A, phi, rhs, m_lin[0].e, m_lin[0].r, m_lin[0].M, m_slots + iter, s));
+    if ((rc = read_norm(iter, &norm))) return rc;
+    if (norm <= o.eps * r0) status |= 1;
+    if (norm <= o.norm_thresh) status |= 8;
+    if (status == 0 && norm > (1.0 - o.hang) * prev) status |= 4;
+    prev = norm;
+  }
+  if (status == 0) status = 2;
+  if (st) {
+    st->exit_status = status;
+    st->iterations = iter;
+    st->initial_residual = r0;
+    st->final_residual = norm;
+  }
+  return ML_OK;
+}
+
 HCEngine::~HCEngine() {
+  for (float* p : m_lin_owned) hipFree(p);
+  if (m_lindesc) hipFree(m_lindesc);
+  if (m_hlindesc) hipHostFree(m_hlindesc);
   for (size_t l = 1; l < m_phi.size(); ++l) {
     if (m_phi[l]) hipFree(m_phi[l]);
     if (m_rhs[l]) hipFree(m_rhs[l]);
@@ -154,6 +285,7 @@ int HCEngine::read_norm(int slot, double* out) {
 int HCEngine::solve(double* phi, const double* rhs, const ml_solve_opts& o, ml_solve_stats* st) {
   ml_ctx ctx = m_op->ctx;
   if (!m_ok) return fail(ctx, ML_ENOMEM, "FAS engine initialisation failed");
+  if (o.precision == ML_PRECISION_FP32_REFINED) return solve_refined(phi, rhs, o, st);
   if (o.max_iter + 1 > m_nslots) return fail(ctx, ML_EINVAL, "max_iter too large");
   if (o.nu1 < 0 || o.nu2 < 0 || o.bottom < 0) return fail(ctx, ML_EINVAL, "negative smoothing count");
   int rc = upload_desc(phi, rhs, o);

@@ -10,145 +10,18 @@
 // bit for bit.
 #include "ml_common.h"
 #include "ml_kernels.h"
+#include "ml_tile.h"
 
 namespace ml {
 
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kWGThreads = 1024;   // single-workgroup kernels
+constexpr int kWGThreads = 256;    // single-workgroup kernels: threads ...
+constexpr int kWGCells = 1024;     // ... and cells of the largest level they hold
+constexpr int kCPT = kWGCells / kWGThreads;
 
 enum : int { kResid = 1, kApply = 2, kRestrict = 3 };
-
-// Position classification of an (unwrapped) index vector of a tile's
-// extended region: in-domain (after periodic wrap), one-cell ghost of a
-// non-periodic wall (exactly one direction outside), or invalid.
-struct Pos {
-  int i, j, k;   // wrapped coordinates; for a ghost: the interior neighbour
-  int kind;      // 0 invalid, 1 in-domain, 2 wall ghost
-  int gd, gs;    // ghost direction and side (0 lo, 1 hi)
-};
-
-template <int DIM>
-__device__ inline Pos classify(const Geom& g, int ci, int cj, int ck) {
-  int c[3] = {ci, cj, ck};
-  Pos P;
-  P.gd = 0;
-  P.gs = 0;
-  int nout = 0, far = 0;
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    const int n = g.n[d];
-    if (c[d] < 0 || c[d] >= n) {
-      if (g.periodic[d]) {
-        c[d] = ((c[d] % n) + n) % n;
-      } else if (c[d] == -1) {
-        P.gd = d; P.gs = 0; c[d] = 0; ++nout;
-      } else if (c[d] == n) {
-        P.gd = d; P.gs = 1; c[d] = n - 1; ++nout;
-      } else {
-        far = 1;
-      }
-    }
-  }
-  if (DIM == 2) c[2] = 0;
-  P.i = c[0];
-  P.j = c[1];
-  P.k = c[2];
-  P.kind = (far || nout > 1) ? 0 : (nout == 1 ? 2 : 1);
-  return P;
-}
-
-// Face coefficients around a cell, both components: bf[n][2d+s] is the
-// lo (s = 0) / hi (s = 1) face of direction d.
-template <typename T, int DIM>
-struct CellFaces {
-  T v[2][2 * DIM];
-};
-
-template <typename T, int DIM>
-__device__ inline void load_faces(const HCArgs<T>& A, int i, int j, int k, CellFaces<T, DIM>& F) {
-  const Geom& g = A.g;
-#pragma unroll
-  for (int n = 0; n < 2; ++n)
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      const T* bd = A.b[d] + (long long)n * g.nface[d];
-      F.v[n][2 * d] = bd[g.fi(d, i, j, k)];
-      F.v[n][2 * d + 1] = bd[g.fi(d, i + (d == 0), j + (d == 1), k + (d == 2))];
-    }
-}
-
-// lambda = 1/(alpha*a + sum_d scale*beta*(b_hi + b_lo))
-// (src/AMRNonLinearMultiCompOp.cpp:540-573, SUMFACESNLVC).
-template <typename T, int DIM>
-__device__ inline T lambda_of(const CellFaces<T, DIM>& F, int n, T aq, T alpha, T beta, T scale) {
-  T lam = aq * alpha;
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) lam = lam + scale * beta * (F.v[n][2 * d + 1] + F.v[n][2 * d]);
-  return T(1) / lam;
-}
-
-// The L(phi) bracket (src/AMRNonLinearMultiCompOpF.ChF:125-160) from the
-// seven stencil values p[0] = centre, p[2d+1] = +e_d, p[2d+2] = -e_d.
-template <typename T, int DIM>
-__device__ inline T lof_of(const CellFaces<T, DIM>& F, int n, const T* p, T aq, T phic, T alpha, T beta,
-                           T dxinv) {
-  const T p0 = p[0];
-  T s = F.v[n][1] * (p[1] - p0) - F.v[n][0] * (p0 - p[2]) + F.v[n][3] * (p[3] - p0) - F.v[n][2] * (p0 - p[4]);
-  if (DIM == 3) s = s + F.v[n][5] * (p[5] - p0) - F.v[n][4] * (p0 - p[6]);
-  return alpha * aq * phic - beta * s * dxinv;
-}
-
-// Extended-region bookkeeping of one tile.
-template <int DIM, int TX, int TY, int TZ>
-struct Tile {
-  int H, EX, EY, EZ, EXY, E;
-  int x0, y0, z0;
-  __device__ inline void init(int halo, int ntx, int nty) {
-    H = halo;
-    EX = TX + 2 * H;
-    EY = TY + 2 * H;
-    EZ = DIM == 3 ? TZ + 2 * H : 1;
-    EXY = EX * EY;
-    E = EXY * EZ;
-    const int b = blockIdx.x;
-    const int tx = b % ntx, r = b / ntx, ty = r % nty, tz = r / nty;
-    x0 = tx * TX;
-    y0 = ty * TY;
-    z0 = DIM == 3 ? tz * TZ : 0;
-  }
-  __device__ inline void coords(int e, int& ex, int& ey, int& ez) const {
-    ex = e % EX;
-    const int r = e / EX;
-    ey = r % EY;
-    ez = r / EY;
-  }
-  // within the core grown by `ext` cells
-  __device__ inline bool within(int ex, int ey, int ez, int ext) const {
-    const int ox = ex - H, oy = ey - H, oz = ez - (DIM == 3 ? H : 0);
-    bool ok = ox >= -ext && ox < TX + ext && oy >= -ext && oy < TY + ext;
-    if (DIM == 3) ok = ok && oz >= -ext && oz < TZ + ext;
-    return ok;
-  }
-  __device__ inline Pos pos(const Geom& g, int ex, int ey, int ez) const {
-    return classify<DIM>(g, x0 - H + ex, y0 - H + ey, DIM == 3 ? z0 - H + ez : 0);
-  }
-};
-
-// Gather the 2*DIM+1 phi_l stencil values of LDS position e.
-template <typename T, int DIM>
-__device__ inline void gather_lds(const T* pl, int e, int EX, int EXY, T* p) {
-  p[0] = pl[e];
-  p[1] = pl[e + 1];
-  p[2] = pl[e - 1];
-  p[3] = pl[e + EX];
-  p[4] = pl[e - EX];
-  if (DIM == 3) {
-    p[5] = pl[e + EXY];
-    p[6] = pl[e - EXY];
-  }
-}
 
 // Closure of every in-domain position within ext, then the derived-variable
 // BC at wall ghosts within ext (computeDiffusedVar,
@@ -197,17 +70,19 @@ __device__ inline void tile_diffused_var(const HCArgs<T>& A, const Tile<DIM, TX,
 // Red/black relaxation of one tile: `npass` colour passes with halo npass
 // staged in LDS (levelGSRB, src/AMRNonLinearMultiCompOp.cpp:815-949, and
 // NONLINEARSMOOTHING, src/AMRNonLinearMultiCompOpF.ChF:125-195: phi_l
-// recomputed before each pass, frozen within it).  The right-hand side and
-// face coefficients of the cells a thread updates are loaded into registers
-// before the closure of the pass, so their latency hides behind it.  When
-// io.pc is set, the staged state first receives the linear prolongation of
-// the coarse correction pc - pc0 (prolongIncrement fused with the
-// post-smoother).
-template <typename T, int DIM, int TX, int TY, int TZ, int HMAX>
-__global__ __launch_bounds__(kThreads) void hc_relax_tile_kernel(HCArgs<T> A, TileIO<T> io) {
+// recomputed before each pass, frozen within it).  Each thread owns fixed
+// LDS positions whose classification is computed once; the right-hand side
+// and face coefficients of the cells it updates are loaded into registers
+// before the closure of the pass, so their latency hides behind it; a wall
+// ghost's derived value is computed by its own thread from the interior
+// neighbour's state, so a pass needs two barriers.  When io.pc is set, the
+// staged state first receives the linear prolongation of the coarse
+// correction pc - pc0 (prolongIncrement fused with the post-smoother).
+template <typename T, int DIM, int TX, int TY, int TZ, int HMAX, int NT>
+__global__ __launch_bounds__(NT) void hc_relax_tile_kernel(HCArgs<T> A, TileIO<T> io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2 * HMAX) * (TY + 2 * HMAX) * (DIM == 3 ? TZ + 2 * HMAX : 1);
-  constexpr int KMAX = (EMAX + kThreads - 1) / kThreads;
+  constexpr int KMAX = (EMAX + NT - 1) / NT;
   const Geom& g = A.g;
   Tile<DIM, TX, TY, TZ> t;
   t.init(io.npass, io.ntx, io.nty);
@@ -218,53 +93,66 @@ __global__ __launch_bounds__(kThreads) void hc_relax_tile_kernel(HCArgs<T> A, Ti
   const T scale = dxinv;
   const Geom gc = io.gc;
 
-  for (int e = threadIdx.x; e < E; e += blockDim.x) {
-    int ex, ey, ez;
-    t.coords(e, ex, ey, ez);
-    const Pos P = t.pos(g, ex, ey, ez);
-    if (P.kind == 1) {
-      const long long q = g.ci(P.i, P.j, P.k);
-      T h = io.phi[q], c = io.phi[q + g.ncell];
-      if (io.pc) {
-        h = prolong_value<T, DIM>(gc, P.i, P.j, P.k, io.pc, io.pc0, h);
-        c = prolong_value<T, DIM>(gc, P.i, P.j, P.k, io.pc + gc.ncell, io.pc0 + gc.ncell, c);
-      }
-      sphi[e] = h;
-      sphi[E + e] = c;
+  PosRec R[KMAX];
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    pos_record<DIM, TX, TY, TZ>(g, t, threadIdx.x + kk * NT, R[kk]);
+  }
+
+  // stage (H, Theta), with the prolonged coarse correction when requested
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    if (r.kind != 1) continue;
+    T h = io.phi[r.q], c = io.phi[r.q + g.ncell];
+    if (io.pc) {
+      h = prolong_value<T, DIM>(gc, r.i, r.j, r.k, io.pc, io.pc0, h);
+      c = prolong_value<T, DIM>(gc, r.i, r.j, r.k, io.pc + gc.ncell, io.pc0 ? io.pc0 + gc.ncell : nullptr, c);
     }
+    sphi[r.e] = h;
+    sphi[E + r.e] = c;
   }
   __syncthreads();
 
   for (int p = 0; p < io.npass; ++p) {
     const int gext = t.H - p;
     const int colour = (io.first + p) & 1;
-    // prefetch the data of the cells this thread updates in this pass
     CellFaces<T, DIM> F[KMAX];
     T rr[KMAX][2], aa[KMAX][2];
     bool upd[KMAX];
 #pragma unroll
     for (int kk = 0; kk < KMAX; ++kk) {
-      const int e = threadIdx.x + kk * kThreads;
-      upd[kk] = false;
-      if (e >= E) continue;
-      int ex, ey, ez;
-      t.coords(e, ex, ey, ez);
-      if (!t.within(ex, ey, ez, gext - 1)) continue;
-      const Pos P = t.pos(g, ex, ey, ez);
-      if (P.kind != 1 || ((P.i + P.j + P.k + colour) & 1)) continue;
-      upd[kk] = true;
-      const long long q = g.ci(P.i, P.j, P.k);
-      load_faces<T, DIM>(A, P.i, P.j, P.k, F[kk]);
-      rr[kk][0] = io.rhs[q];
-      rr[kk][1] = io.rhs[q + g.ncell];
-      aa[kk][0] = A.a ? A.a[q] : T(1);
-      aa[kk][1] = A.a ? A.a[q + g.ncell] : T(1);
+      const PosRec& r = R[kk];
+      upd[kk] = r.kind == 1 && r.dist <= gext - 1 && !((r.i + r.j + r.k + colour) & 1);
+      if (!upd[kk]) continue;
+      load_faces<T, DIM>(A, r.i, r.j, r.k, F[kk]);
+      rr[kk][0] = io.rhs[r.q];
+      rr[kk][1] = io.rhs[r.q + g.ncell];
+      aa[kk][0] = A.a ? A.a[r.q] : T(1);
+      aa[kk][1] = A.a ? A.a[r.q + g.ncell] : T(1);
     }
-    tile_diffused_var<T, DIM, TX, TY, TZ>(A, t, sphi, sphl, gext, 0);
+    // closure of the staged cells; wall ghosts from their interior neighbour
+    // (computeDiffusedVar, src/AMRNonLinearMultiCompOp.cpp:57-98)
+#pragma unroll
+    for (int kk = 0; kk < KMAX; ++kk) {
+      const PosRec& r = R[kk];
+      if (r.kind == 0 || r.dist > gext) continue;
+      const int es = r.kind == 1 ? r.e : r.eint;
+      T th, cl;
+      closure<T>(sphi[es], sphi[E + es], A.cp, th, cl);
+      if (r.kind == 2 && A.bc.apply) {
+        const int isign = r.gs == 0 ? -1 : 1;
+        th = bc_ghost<T>(A.bc.type[r.gd][r.gs][0], T(A.bc.val[r.gd][r.gs][0]), th, isign, T(g.dx));
+        cl = bc_ghost<T>(A.bc.type[r.gd][r.gs][1], T(A.bc.val[r.gd][r.gs][1]), cl, isign, T(g.dx));
+      }
+      sphl[r.e] = th;
+      sphl[E + r.e] = cl;
+    }
+    __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < KMAX; ++kk) {
       if (!upd[kk]) continue;
-      const int e = threadIdx.x + kk * kThreads;
+      const int e = R[kk].e;
 #pragma unroll
       for (int n = 0; n < 2; ++n) {
         T st[7];
@@ -277,15 +165,12 @@ __global__ __launch_bounds__(kThreads) void hc_relax_tile_kernel(HCArgs<T> A, Ti
     }
     __syncthreads();
   }
-  for (int e = threadIdx.x; e < E; e += blockDim.x) {
-    int ex, ey, ez;
-    t.coords(e, ex, ey, ez);
-    if (!t.within(ex, ey, ez, 0)) continue;
-    const int gi = t.x0 - t.H + ex, gj = t.y0 - t.H + ey, gk = DIM == 3 ? t.z0 - t.H + ez : 0;
-    if (gi >= g.n[0] || gj >= g.n[1] || (DIM == 3 && gk >= g.n[2])) continue;
-    const long long q = g.ci(gi, gj, gk);
-    io.phi[q] = sphi[e];
-    io.phi[q + g.ncell] = sphi[E + e];
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    if (!r.owner) continue;
+    io.phi[r.q] = sphi[r.e];
+    io.phi[r.q + g.ncell] = sphi[E + r.e];
   }
 }
 
@@ -406,24 +291,32 @@ __global__ __launch_bounds__(kThreads) void hc_tile_kernel(HCArgs<T> A, TileIO<T
 }
 
 // ---------------------------------------------------------------------------
-// Single-workgroup building blocks over a whole (small) level: cell c of the
-// level is owned by thread c (N <= kWGThreads), phi and phi_l live in LDS,
-// neighbours are found by periodic wrap or the derived-variable wall rule.
+// Single-workgroup building blocks over a whole (small) level of at most
+// kWGCells cells: thread t owns cells t + m*kWGThreads, phi and phi_l live in
+// LDS, neighbours are found by periodic wrap or the derived-variable wall rule.
 template <typename T, int DIM>
 struct WGLevel {
   const LevelDesc<T>* L;
-  int N, i, j, k;
-  bool own;
+  int N, nx, nxy;
+  int c[kCPT], i[kCPT], j[kCPT], k[kCPT];
+  long long q[kCPT];
+  bool own[kCPT];
   __device__ inline void init(const LevelDesc<T>* lv) {
     L = lv;
     const Geom& g = L->g;
-    N = g.n[0] * g.n[1] * (DIM == 3 ? g.n[2] : 1);
-    const int c = threadIdx.x;
-    own = c < N;
-    i = c % g.n[0];
-    const int r = c / g.n[0];
-    j = r % g.n[1];
-    k = DIM == 3 ? r / g.n[1] : 0;
+    nx = g.n[0];
+    nxy = g.n[0] * g.n[1];
+    N = nxy * (DIM == 3 ? g.n[2] : 1);
+#pragma unroll
+    for (int m = 0; m < kCPT; ++m) {
+      c[m] = threadIdx.x + m * kWGThreads;
+      own[m] = c[m] < N;
+      i[m] = c[m] % nx;
+      const int r = c[m] / nx;
+      j[m] = r % g.n[1];
+      k[m] = DIM == 3 ? r / g.n[1] : 0;
+      q[m] = own[m] ? g.ci(i[m], j[m], k[m]) : 0;
+    }
   }
 };
 
@@ -455,79 +348,111 @@ __device__ inline void wg_stencil(const Geom& g, const BCDesc& bc, const T* pl, 
 }
 
 template <typename T, int DIM>
-__device__ inline void wg_closure(const ClosureP<T>& cp, const T* sphi, T* sphl, int N) {
-  const int c = threadIdx.x;
-  if (c < N) {
+__device__ inline void wg_closure(const ClosureP<T>& cp, const WGLevel<T, DIM>& W, const T* sphi, T* sphl) {
+#pragma unroll
+  for (int m = 0; m < kCPT; ++m) {
+    if (!W.own[m]) continue;
     T th, cl;
-    closure<T>(sphi[c], sphi[N + c], cp, th, cl);
-    sphl[c] = th;
-    sphl[N + c] = cl;
+    closure<T>(sphi[W.c[m]], sphi[W.N + W.c[m]], cp, th, cl);
+    sphl[W.c[m]] = th;
+    sphl[W.N + W.c[m]] = cl;
   }
 }
 
-// L(phi) of the owned cell (both components) from phi/phi_l in LDS.
 template <typename T, int DIM>
-__device__ inline void wg_lof(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, const CellFaces<T, DIM>& F,
+__device__ inline void wg_faces(const LevelDesc<T>* L, int i, int j, int k, CellFaces<T, DIM>& F, T* aq) {
+  HCArgs<T> A;
+  A.g = L->g;
+  for (int d = 0; d < 3; ++d) A.b[d] = L->b[d];
+  load_faces<T, DIM>(A, i, j, k, F);
+  const long long q = L->g.ci(i, j, k);
+  aq[0] = L->a ? L->a[q] : T(1);
+  aq[1] = L->a ? L->a[q + L->g.ncell] : T(1);
+}
+
+// L(phi) of owned cell m (both components) from phi/phi_l in LDS.
+template <typename T, int DIM>
+__device__ inline void wg_lof(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, int m, const CellFaces<T, DIM>& F,
                               const T* aq, const T* sphi, const T* sphl, T* lof) {
   const Geom& g = W.L->g;
   const T dxinv = T(1) / (T(g.dx) * T(g.dx));
-  const int c = threadIdx.x;
 #pragma unroll
   for (int n = 0; n < 2; ++n) {
     T st[7];
-    wg_stencil<T, DIM>(g, D.bc, sphl + n * W.N, c, W.i, W.j, W.k, n, g.n[0], g.n[0] * g.n[1], st);
-    lof[n] = lof_of<T, DIM>(F, n, st, aq[n], sphi[n * W.N + c], D.alpha, D.beta, dxinv);
+    wg_stencil<T, DIM>(g, D.bc, sphl + n * W.N, W.c[m], W.i[m], W.j[m], W.k[m], n, W.nx, W.nxy, st);
+    lof[n] = lof_of<T, DIM>(F, n, st, aq[n], sphi[n * W.N + W.c[m]], D.alpha, D.beta, dxinv);
   }
 }
 
-template <typename T, int DIM>
-__device__ inline void wg_load_cell(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, CellFaces<T, DIM>& F,
-                                    T* aq) {
-  HCArgs<T> A;
-  A.g = W.L->g;
-  for (int d = 0; d < 3; ++d) A.b[d] = W.L->b[d];
-  load_faces<T, DIM>(A, W.i, W.j, W.k, F);
-  const long long q = W.L->g.ci(W.i, W.j, W.k);
-  aq[0] = W.L->a ? W.L->a[q] : T(1);
-  aq[1] = W.L->a ? W.L->a[q + W.L->g.ncell] : T(1);
-}
-
-// relax: npass colour passes of the whole level, phi kept in LDS on exit.
+// relax: npass colour passes of the whole level (first colour red).
 template <typename T, int DIM>
 __device__ void wg_relax(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, T* sphi, T* sphl, int npass,
                          bool load) {
   const Geom& g = W.L->g;
-  const int c = threadIdx.x;
-  const long long q = W.own ? g.ci(W.i, W.j, W.k) : 0;
-  CellFaces<T, DIM> F;
-  T aq[2] = {T(1), T(1)}, rr[2] = {T(0), T(0)}, lam[2] = {T(0), T(0)};
-  if (W.own) {
-    wg_load_cell<T, DIM>(D, W, F, aq);
-    rr[0] = W.L->rhs[q];
-    rr[1] = W.L->rhs[q + g.ncell];
-    const T scale = T(1) / (T(g.dx) * T(g.dx));
-    lam[0] = lambda_of<T, DIM>(F, 0, aq[0], D.alpha, D.beta, scale);
-    lam[1] = lambda_of<T, DIM>(F, 1, aq[1], D.alpha, D.beta, scale);
+  CellFaces<T, DIM> F[kCPT];
+  T aq[kCPT][2], rr[kCPT][2], lam[kCPT][2];
+  const T scale = T(1) / (T(g.dx) * T(g.dx));
+#pragma unroll
+  for (int m = 0; m < kCPT; ++m) {
+    if (!W.own[m]) continue;
+    wg_faces<T, DIM>(W.L, W.i[m], W.j[m], W.k[m], F[m], aq[m]);
+    rr[m][0] = W.L->rhs[W.q[m]];
+    rr[m][1] = W.L->rhs[W.q[m] + g.ncell];
+    lam[m][0] = lambda_of<T, DIM>(F[m], 0, aq[m][0], D.alpha, D.beta, scale);
+    lam[m][1] = lambda_of<T, DIM>(F[m], 1, aq[m][1], D.alpha, D.beta, scale);
     if (load) {
-      sphi[c] = W.L->phi[q];
-      sphi[W.N + c] = W.L->phi[q + g.ncell];
+      sphi[W.c[m]] = W.L->phi[W.q[m]];
+      sphi[W.N + W.c[m]] = W.L->phi[W.q[m] + g.ncell];
     }
   }
   __syncthreads();
   for (int p = 0; p < npass; ++p) {
-    wg_closure<T, DIM>(D.cp, sphi, sphl, W.N);
+    wg_closure<T, DIM>(D.cp, W, sphi, sphl);
     __syncthreads();
-    if (W.own && !((W.i + W.j + W.k + p) & 1)) {
+#pragma unroll
+    for (int m = 0; m < kCPT; ++m) {
+      if (!W.own[m] || ((W.i[m] + W.j[m] + W.k[m] + p) & 1)) continue;
       T lof[2];
-      wg_lof<T, DIM>(D, W, F, aq, sphi, sphl, lof);
-      sphi[c] = sphi[c] - lam[0] * (lof[0] - rr[0]);
-      sphi[W.N + c] = sphi[W.N + c] - lam[1] * (lof[1] - rr[1]);
+      wg_lof<T, DIM>(D, W, m, F[m], aq[m], sphi, sphl, lof);
+      const int c = W.c[m];
+      sphi[c] = sphi[c] - lam[m][0] * (lof[0] - rr[m][0]);
+      sphi[W.N + c] = sphi[W.N + c] - lam[m][1] * (lof[1] - rr[m][1]);
     }
     __syncthreads();
   }
-  if (W.own) {
-    W.L->phi[q] = sphi[c];
-    W.L->phi[q + g.ncell] = sphi[W.N + c];
+#pragma unroll
+  for (int m = 0; m < kCPT; ++m) {
+    if (!W.own[m]) continue;
+    W.L->phi[W.q[m]] = sphi[W.c[m]];
+    W.L->phi[W.q[m] + g.ncell] = sphi[W.N + W.c[m]];
+  }
+  __syncthreads();
+}
+
+// rhs_l += L_l(phi_l): the FAS coarse right-hand side of level l.
+template <typename T, int DIM>
+__device__ void wg_apply_add(const CoarseDesc<T>& D, const LevelDesc<T>* lv, T* sphi, T* sphl) {
+  WGLevel<T, DIM> W;
+  W.init(lv);
+  const Geom& g = W.L->g;
+#pragma unroll
+  for (int m = 0; m < kCPT; ++m) {
+    if (!W.own[m]) continue;
+    sphi[W.c[m]] = W.L->phi[W.q[m]];
+    sphi[W.N + W.c[m]] = W.L->phi[W.q[m] + g.ncell];
+  }
+  __syncthreads();
+  wg_closure<T, DIM>(D.cp, W, sphi, sphl);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < kCPT; ++m) {
+    if (!W.own[m]) continue;
+    CellFaces<T, DIM> F;
+    T aq[2], lof[2];
+    wg_faces<T, DIM>(W.L, W.i[m], W.j[m], W.k[m], F, aq);
+    wg_lof<T, DIM>(D, W, m, F, aq, sphi, sphl, lof);
+    W.L->rhs[W.q[m]] = W.L->rhs[W.q[m]] + lof[0];
+    W.L->rhs[W.q[m] + g.ncell] = W.L->rhs[W.q[m] + g.ncell] + lof[1];
   }
   __syncthreads();
 }
@@ -543,49 +468,25 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const CoarseDesc<T>& D = *Dp;
   T* sphi = reinterpret_cast<T*>(smem_raw);
-  T* sphl = sphi + 2 * kWGThreads;
+  T* sphl = sphi + 2 * kWGCells;
   const int l0 = D.l0, L = D.nlev;
-  const int c = threadIdx.x;
   const T denom = DIM == 3 ? T(8) : T(4);
 
-  // rhs_l += L_l(phi_l): the FAS coarse right-hand side of level l
-  auto apply_add = [&](int l) {
-    WGLevel<T, DIM> W;
-    W.init(&D.lev[l]);
-    const Geom& g = W.L->g;
-    const long long q = W.own ? g.ci(W.i, W.j, W.k) : 0;
-    if (W.own) {
-      sphi[c] = W.L->phi[q];
-      sphi[W.N + c] = W.L->phi[q + g.ncell];
-    }
-    __syncthreads();
-    wg_closure<T, DIM>(D.cp, sphi, sphl, W.N);
-    __syncthreads();
-    if (W.own) {
-      CellFaces<T, DIM> F;
-      T aq[2], lof[2];
-      wg_load_cell<T, DIM>(D, W, F, aq);
-      wg_lof<T, DIM>(D, W, F, aq, sphi, sphl, lof);
-      W.L->rhs[q] = W.L->rhs[q] + lof[0];
-      W.L->rhs[q + g.ncell] = W.L->rhs[q + g.ncell] + lof[1];
-    }
-    __syncthreads();
-  };
-
-  if (apply_first) apply_add(l0);
+  if (apply_first) wg_apply_add<T, DIM>(D, &D.lev[l0], sphi, sphl);
 
   for (int l = l0; l < L - 1; ++l) {
     WGLevel<T, DIM> W;
     W.init(&D.lev[l]);
     wg_relax<T, DIM>(D, W, sphi, sphl, 2 * D.nu1, true);
     // restrict phi and rhs - L(phi) (phi is still in LDS)
-    wg_closure<T, DIM>(D.cp, sphi, sphl, W.N);
+    wg_closure<T, DIM>(D.cp, W, sphi, sphl);
     __syncthreads();
     {
       const Geom& g = W.L->g;
       const LevelDesc<T>& C = D.lev[l + 1];
       const Geom& gcc = C.g;
       const int NC = gcc.n[0] * gcc.n[1] * (DIM == 3 ? gcc.n[2] : 1);
+      const int c = threadIdx.x;
       if (c < NC) {
         const int I = c % gcc.n[0], r = c / gcc.n[0], J = r % gcc.n[1], K = DIM == 3 ? r / gcc.n[1] : 0;
         T s[2] = {T(0), T(0)}, sp[2] = {T(0), T(0)};
@@ -594,19 +495,16 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
           for (int jj = 0; jj < 2; ++jj)
             for (int ii = 0; ii < 2; ++ii) {
               const int fi = 2 * I + ii, fj = 2 * J + jj, fk = DIM == 3 ? 2 * K + kk : 0;
-              const int fc = fi + g.n[0] * (fj + g.n[1] * fk);
-              HCArgs<T> A;
-              A.g = g;
-              for (int d = 0; d < 3; ++d) A.b[d] = W.L->b[d];
+              const int fc = fi + W.nx * fj + W.nxy * fk;
               CellFaces<T, DIM> F;
-              load_faces<T, DIM>(A, fi, fj, fk, F);
+              T aq[2];
+              wg_faces<T, DIM>(W.L, fi, fj, fk, F, aq);
               const long long q = g.ci(fi, fj, fk);
 #pragma unroll
               for (int n = 0; n < 2; ++n) {
                 T st[7];
-                wg_stencil<T, DIM>(g, D.bc, sphl + n * W.N, fc, fi, fj, fk, n, g.n[0], g.n[0] * g.n[1], st);
-                const T aq = W.L->a ? W.L->a[q + n * g.ncell] : T(1);
-                const T lof = lof_of<T, DIM>(F, n, st, aq, sphi[n * W.N + fc], D.alpha, D.beta, dxinv);
+                wg_stencil<T, DIM>(g, D.bc, sphl + n * W.N, fc, fi, fj, fk, n, W.nx, W.nxy, st);
+                const T lof = lof_of<T, DIM>(F, n, st, aq[n], sphi[n * W.N + fc], D.alpha, D.beta, dxinv);
                 s[n] = s[n] + (W.L->rhs[q + n * g.ncell] - lof) / denom;
                 sp[n] = sp[n] + sphi[n * W.N + fc] / denom;
               }
@@ -621,7 +519,7 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
       }
     }
     __syncthreads();
-    apply_add(l + 1);
+    wg_apply_add<T, DIM>(D, &D.lev[l + 1], sphi, sphl);
   }
   {
     WGLevel<T, DIM> W;
@@ -632,13 +530,13 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
     WGLevel<T, DIM> W;
     W.init(&D.lev[l]);
     const LevelDesc<T>& C = D.lev[l + 1];
-    if (W.own) {
-      const long long q = W.L->g.ci(W.i, W.j, W.k);
-      const T h = prolong_value<T, DIM>(C.g, W.i, W.j, W.k, C.phi, C.phi0, W.L->phi[q]);
-      const T cc = prolong_value<T, DIM>(C.g, W.i, W.j, W.k, C.phi + C.g.ncell, C.phi0 + C.g.ncell,
-                                         W.L->phi[q + W.L->g.ncell]);
-      sphi[c] = h;
-      sphi[W.N + c] = cc;
+#pragma unroll
+    for (int m = 0; m < kCPT; ++m) {
+      if (!W.own[m]) continue;
+      const long long q = W.q[m];
+      sphi[W.c[m]] = prolong_value<T, DIM>(C.g, W.i[m], W.j[m], W.k[m], C.phi, C.phi0, W.L->phi[q]);
+      sphi[W.N + W.c[m]] = prolong_value<T, DIM>(C.g, W.i[m], W.j[m], W.k[m], C.phi + C.g.ncell,
+                                                 C.phi0 + C.g.ncell, W.L->phi[q + W.L->g.ncell]);
     }
     __syncthreads();
     wg_relax<T, DIM>(D, W, sphi, sphl, 2 * D.nu2, false);
@@ -652,7 +550,7 @@ __global__ __launch_bounds__(kWGThreads) void hc_relax_small_kernel(const Coarse
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const CoarseDesc<T>& D = *Dp;
   T* sphi = reinterpret_cast<T*>(smem_raw);
-  T* sphl = sphi + 2 * kWGThreads;
+  T* sphl = sphi + 2 * kWGCells;
   WGLevel<T, DIM> W;
   W.init(&D.lev[l]);
   wg_relax<T, DIM>(D, W, sphi, sphl, npass, true);
@@ -731,7 +629,7 @@ int max_tile_passes(int dim) {
 
 template <typename T>
 bool small_level_fits(const Geom& g) {
-  return g.nvalid() <= kWGThreads;
+  return g.nvalid() <= kWGCells;
 }
 
 template <typename T, int DIM, int MODE>
@@ -744,27 +642,50 @@ static hipError_t launch_tile(const HCArgs<T>& A, TileIO<T> io, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Passes per relax launch: small grids are latency bound, so up to four
+// red/black sweeps share one launch (halo 8); large grids use one sweep per
+// launch (halo 2) to keep the redundant halo work small.
+constexpr int kTilesForDeepHalo = 1024;
+
+template <typename T, int DIM>
+static int relax_passes_per_launch(const Geom& g) {
+  int ntx, nty;
+  const int nt = tiles_of<T, DIM>(g, ntx, nty);
+  return (DIM == 2 && nt <= kTilesForDeepHalo) ? 8 : 2;
+}
+
 template <typename T, int DIM>
 static hipError_t launch_relax_tile(const HCArgs<T>& A, TileIO<T> io, hipStream_t s) {
   using D = TileDims<T, DIM>;
   const int nt = tiles_of<T, DIM>(A.g, io.ntx, io.nty);
-  if (io.npass < 1 || io.npass > 2) return hipErrorInvalidValue;
+  const int pmax = relax_passes_per_launch<T, DIM>(A.g);
+  if (io.npass < 1 || io.npass > pmax) return hipErrorInvalidValue;
   const size_t lds = tile_lds<T, DIM>(io.npass, false);
   if (lds > kMaxLds) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((hc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, 2>), dim3(nt), dim3(kThreads), lds, s, A,
-                     io);
+  if constexpr (DIM == 2) {
+    if (pmax == 8)
+      hipLaunchKernelGGL((hc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, 8, 512>), dim3(nt), dim3(512), lds, s,
+                         A, io);
+    else
+      hipLaunchKernelGGL((hc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, 2, 256>), dim3(nt), dim3(256), lds, s,
+                         A, io);
+  } else {
+    hipLaunchKernelGGL((hc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, 2, 512>), dim3(nt), dim3(512), lds, s,
+                       A, io);
+  }
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t hc_relax_tiles(const HCArgs<T>& A, T* phi, const T* rhs, int npass, int first, const Geom* gc,
                           const T* pc, const T* pc0, hipStream_t s) {
+  const int pmax = A.g.dim == 2 ? relax_passes_per_launch<T, 2>(A.g) : relax_passes_per_launch<T, 3>(A.g);
   int done = 0;
   while (done < npass) {
     TileIO<T> io{};
     io.phi = phi;
     io.rhs = rhs;
-    io.npass = (npass - done) < 2 ? (npass - done) : 2;
+    io.npass = (npass - done) < pmax ? (npass - done) : pmax;
     io.first = (first + done) & 1;
     if (done == 0 && pc) {
       io.gc = *gc;
@@ -774,6 +695,10 @@ hipError_t hc_relax_tiles(const HCArgs<T>& A, T* phi, const T* rhs, int npass, i
     hipError_t e = A.g.dim == 2 ? launch_relax_tile<T, 2>(A, io, s) : launch_relax_tile<T, 3>(A, io, s);
     if (e != hipSuccess) return e;
     done += io.npass;
+  }
+  if (npass == 0 && pc) {
+    // a prolongation with no smoothing after it
+    return prolong_linear<T>(A.g, 2, phi, pc, pc0, s);
   }
   return hipSuccess;
 }
@@ -786,7 +711,7 @@ hipError_t hc_relax(const HCArgs<T>& A, T* phi, const T* rhs, int npass, int fir
 
 template <typename T>
 hipError_t hc_relax_small(const CoarseDesc<T>* dev_desc, int dim, int level, int npass, hipStream_t s) {
-  const size_t lds = size_t(4) * kWGThreads * sizeof(T);
+  const size_t lds = size_t(4) * kWGCells * sizeof(T);
   if (dim == 2)
     hipLaunchKernelGGL((hc_relax_small_kernel<T, 2>), dim3(1), dim3(kWGThreads), lds, s, dev_desc, level, npass);
   else
@@ -796,7 +721,7 @@ hipError_t hc_relax_small(const CoarseDesc<T>* dev_desc, int dim, int level, int
 
 template <typename T>
 hipError_t hc_coarse_vcycle(const CoarseDesc<T>* dev_desc, int dim, int apply_first, hipStream_t s) {
-  const size_t lds = size_t(4) * kWGThreads * sizeof(T);
+  const size_t lds = size_t(4) * kWGCells * sizeof(T);
   if (dim == 2)
     hipLaunchKernelGGL((hc_coarse_vcycle_kernel<T, 2>), dim3(1), dim3(kWGThreads), lds, s, dev_desc, apply_first);
   else

@@ -90,6 +90,17 @@ class HCEngine {
   int residual_norm(double* phi, const double* rhs, int slot);
   int read_norm(int slot, double* out);
   int upload_desc(double* phi0, const double* rhs0, const ml_solve_opts& o);
+  // FP32-refined form: FP64 defect correction around FP32 linear V-cycles
+  int solve_refined(double* phi, const double* rhs, const ml_solve_opts& o, ml_solve_stats* st);
+  int init_refined();
+  int lin_vcycle(const ml_solve_opts& o);
+
+  // FP32-refined state per level
+  std::vector<LinLevel<float>> m_lin;
+  std::vector<float*> m_lin_owned;
+  LinDesc* m_lindesc = nullptr;      // device
+  LinDesc* m_hlindesc = nullptr;     // pinned host staging
+  bool m_refined_ok = false;
 
   ml_hc_op m_op;
   int m_L = 0, m_lc = 0;

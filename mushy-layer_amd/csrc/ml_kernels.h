@@ -119,6 +119,45 @@ __device__ inline T prolong_value(const Geom& gc, int i, int j, int k, const T* 
   return p;
 }
 
+// ---- FP32-refined form (ml_fp32r.hip)
+// Linearised (H, Theta) operator J e = alpha*a*e - beta*div(b grad(M e)) on
+// one MG level: M is the per-cell 2x2 Jacobian of the closure
+// (theta, Theta_l)(H, Theta), components (M00, M01, M10, M11).
+template <typename T>
+struct LinLevel {
+  Geom g;
+  const T* b[3];   // face coefficients (2 comps)
+  const T* a;      // aCoef (2 comps) or nullptr
+  T* M;            // Jacobian (4 comps)
+  T* e;            // correction (2 comps)
+  T* r;            // right-hand side (2 comps)
+};
+
+struct LinDesc {
+  int l0, nlev, nu1, nu2, bottom;
+  BCDesc bc;
+  double alpha, beta;
+  LinLevel<float> lev[kMaxLevels];
+  const double* b64[3];   // coarsest level, FP64 faces for the bottom solve
+  const double* a64;
+};
+
+// fine level: phi += e (when e != nullptr, written back), r32 = rhs - L(phi),
+// M32 = closure Jacobian, max-norm of r into *norm
+hipError_t hc_residual_jacobian(const HCArgs<double>& A, double* phi, const double* rhs, const float* e,
+                                float* r32, float* M32, unsigned long long* norm, hipStream_t s);
+// red/black block relaxation of J e = r on a tiled level; zero_init starts
+// from e = 0; pc (coarse correction, level gc) is prolonged into e first
+hipError_t lin_relax(const LinLevel<float>& L, const BCDesc& bc, float alpha, float beta, int npass,
+                     bool zero_init, const Geom* gc, const float* pc, hipStream_t s);
+// r_c = R(r - J e), M_c = R(M) on the next coarser level
+hipError_t lin_restrict(const LinLevel<float>& L, const LinLevel<float>& C, const BCDesc& bc, float alpha,
+                        float beta, hipStream_t s);
+// single-workgroup linear V-cycle over levels l0.. of the device descriptor
+hipError_t lin_coarse_vcycle(const LinDesc* dev_desc, int dim, hipStream_t s);
+// cast of FP64 face coefficients to FP32
+hipError_t cast_f64_f32(long long n, const double* x, float* y, hipStream_t s);
+
 // ---- nonlinear (H, Theta) operator (ml_hc_kernels.hip)
 template <typename T>
 hipError_t hc_relax(const HCArgs<T>& A, T* phi, const T* rhs, int npass, int first, hipStream_t s);

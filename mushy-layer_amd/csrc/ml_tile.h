@@ -1,0 +1,188 @@
+// ml_tile.h — device helpers shared by the tiled stencil kernels: position
+// classification against periodic / wall boundaries, per-thread position
+// records of a tile's extended region, face-coefficient gathers and the
+// L(phi) bracket of the reference's Fortran stencils.
+#pragma once
+
+#include "ml_common.h"
+#include "ml_kernels.h"
+
+namespace ml {
+
+// Position classification of an (unwrapped) index vector of a tile's
+// extended region: in-domain (after periodic wrap), one-cell ghost of a
+// non-periodic wall (exactly one direction outside), or invalid.
+struct Pos {
+  int i, j, k;   // wrapped coordinates; for a ghost: the interior neighbour
+  int kind;      // 0 invalid, 1 in-domain, 2 wall ghost
+  int gd, gs;    // ghost direction and side (0 lo, 1 hi)
+};
+
+template <int DIM>
+__device__ inline Pos classify(const Geom& g, int ci, int cj, int ck) {
+  int c[3] = {ci, cj, ck};
+  Pos P;
+  P.gd = 0;
+  P.gs = 0;
+  int nout = 0, far = 0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const int n = g.n[d];
+    if (c[d] < 0 || c[d] >= n) {
+      if (g.periodic[d]) {
+        c[d] = ((c[d] % n) + n) % n;
+      } else if (c[d] == -1) {
+        P.gd = d; P.gs = 0; c[d] = 0; ++nout;
+      } else if (c[d] == n) {
+        P.gd = d; P.gs = 1; c[d] = n - 1; ++nout;
+      } else {
+        far = 1;
+      }
+    }
+  }
+  if (DIM == 2) c[2] = 0;
+  P.i = c[0];
+  P.j = c[1];
+  P.k = c[2];
+  P.kind = (far || nout > 1) ? 0 : (nout == 1 ? 2 : 1);
+  return P;
+}
+
+// Face coefficients around a cell, both components: bf[n][2d+s] is the
+// lo (s = 0) / hi (s = 1) face of direction d.
+template <typename T, int DIM>
+struct CellFaces {
+  T v[2][2 * DIM];
+};
+
+template <typename T, int DIM>
+__device__ inline void load_faces(const Geom& g, const T* const* b, int i, int j, int k, CellFaces<T, DIM>& F) {
+#pragma unroll
+  for (int n = 0; n < 2; ++n)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const T* bd = b[d] + (long long)n * g.nface[d];
+      F.v[n][2 * d] = bd[g.fi(d, i, j, k)];
+      F.v[n][2 * d + 1] = bd[g.fi(d, i + (d == 0), j + (d == 1), k + (d == 2))];
+    }
+}
+
+template <typename T, int DIM>
+__device__ inline void load_faces(const HCArgs<T>& A, int i, int j, int k, CellFaces<T, DIM>& F) {
+  load_faces<T, DIM>(A.g, A.b, i, j, k, F);
+}
+
+// lambda = 1/(alpha*a + sum_d scale*beta*(b_hi + b_lo))
+// (src/AMRNonLinearMultiCompOp.cpp:540-573, SUMFACESNLVC).
+template <typename T, int DIM>
+__device__ inline T lambda_of(const CellFaces<T, DIM>& F, int n, T aq, T alpha, T beta, T scale) {
+  T lam = aq * alpha;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) lam = lam + scale * beta * (F.v[n][2 * d + 1] + F.v[n][2 * d]);
+  return T(1) / lam;
+}
+
+// The L(phi) bracket (src/AMRNonLinearMultiCompOpF.ChF:125-160) from the
+// seven stencil values p[0] = centre, p[2d+1] = +e_d, p[2d+2] = -e_d.
+template <typename T, int DIM>
+__device__ inline T lof_of(const CellFaces<T, DIM>& F, int n, const T* p, T aq, T phic, T alpha, T beta,
+                           T dxinv) {
+  const T p0 = p[0];
+  T s = F.v[n][1] * (p[1] - p0) - F.v[n][0] * (p0 - p[2]) + F.v[n][3] * (p[3] - p0) - F.v[n][2] * (p0 - p[4]);
+  if (DIM == 3) s = s + F.v[n][5] * (p[5] - p0) - F.v[n][4] * (p0 - p[6]);
+  return alpha * aq * phic - beta * s * dxinv;
+}
+
+// Extended-region bookkeeping of one tile.
+template <int DIM, int TX, int TY, int TZ>
+struct Tile {
+  int H, EX, EY, EZ, EXY, E;
+  int x0, y0, z0;
+  __device__ inline void init(int halo, int ntx, int nty) {
+    H = halo;
+    EX = TX + 2 * H;
+    EY = TY + 2 * H;
+    EZ = DIM == 3 ? TZ + 2 * H : 1;
+    EXY = EX * EY;
+    E = EXY * EZ;
+    const int b = blockIdx.x;
+    const int tx = b % ntx, r = b / ntx, ty = r % nty, tz = r / nty;
+    x0 = tx * TX;
+    y0 = ty * TY;
+    z0 = DIM == 3 ? tz * TZ : 0;
+  }
+  __device__ inline void coords(int e, int& ex, int& ey, int& ez) const {
+    ex = e % EX;
+    const int r = e / EX;
+    ey = r % EY;
+    ez = r / EY;
+  }
+  // within the core grown by `ext` cells
+  __device__ inline bool within(int ex, int ey, int ez, int ext) const {
+    const int ox = ex - H, oy = ey - H, oz = ez - (DIM == 3 ? H : 0);
+    bool ok = ox >= -ext && ox < TX + ext && oy >= -ext && oy < TY + ext;
+    if (DIM == 3) ok = ok && oz >= -ext && oz < TZ + ext;
+    return ok;
+  }
+  __device__ inline Pos pos(const Geom& g, int ex, int ey, int ez) const {
+    return classify<DIM>(g, x0 - H + ex, y0 - H + ey, DIM == 3 ? z0 - H + ez : 0);
+  }
+};
+
+// Gather the 2*DIM+1 phi_l stencil values of LDS position e.
+template <typename T, int DIM>
+__device__ inline void gather_lds(const T* pl, int e, int EX, int EXY, T* p) {
+  p[0] = pl[e];
+  p[1] = pl[e + 1];
+  p[2] = pl[e - 1];
+  p[3] = pl[e + EX];
+  p[4] = pl[e - EX];
+  if (DIM == 3) {
+    p[5] = pl[e + EXY];
+    p[6] = pl[e - EXY];
+  }
+}
+
+// Per-thread record of one LDS position of a tile's extended region.
+struct PosRec {
+  long long q;   // global cell index (in-domain) or of the interior neighbour (ghost)
+  int e;         // LDS position
+  int kind;      // 0 unused/invalid, 1 in-domain, 2 wall ghost
+  int dist;      // distance outside the core (0 inside)
+  int eint;      // ghost: LDS position of the interior neighbour
+  int gd, gs;    // ghost direction / side
+  int i, j, k;   // wrapped coordinates
+  bool owner;    // core cell inside the domain: written back by this tile
+};
+
+
+// Fill the record of LDS position e of tile t.
+template <int DIM, int TX, int TY, int TZ>
+__device__ inline void pos_record(const Geom& g, const Tile<DIM, TX, TY, TZ>& t, int e, PosRec& r) {
+  r.e = e;
+  r.kind = 0;
+  r.owner = false;
+  r.dist = 1 << 20;
+  if (e >= t.E) return;
+  int ex, ey, ez;
+  t.coords(e, ex, ey, ez);
+  const int ox = ex - t.H, oy = ey - t.H, oz = DIM == 3 ? ez - t.H : 0;
+  const int dxo = ox < 0 ? -ox : (ox >= TX ? ox - TX + 1 : 0);
+  const int dyo = oy < 0 ? -oy : (oy >= TY ? oy - TY + 1 : 0);
+  const int dzo = DIM == 3 ? (oz < 0 ? -oz : (oz >= TZ ? oz - TZ + 1 : 0)) : 0;
+  r.dist = dxo > dyo ? (dxo > dzo ? dxo : dzo) : (dyo > dzo ? dyo : dzo);
+  const int gi = t.x0 + ox, gj = t.y0 + oy, gk = DIM == 3 ? t.z0 + oz : 0;
+  const Pos P = classify<DIM>(g, gi, gj, gk);
+  r.kind = P.kind;
+  r.i = P.i;
+  r.j = P.j;
+  r.k = P.k;
+  r.gd = P.gd;
+  r.gs = P.gs;
+  r.q = g.ci(P.i, P.j, P.k);
+  const int stride = P.gd == 0 ? 1 : (P.gd == 1 ? t.EX : t.EXY);
+  r.eint = P.gs == 0 ? e + stride : e - stride;
+  r.owner = r.dist == 0 && P.kind == 1 && gi < g.n[0] && gj < g.n[1] && (DIM == 2 || gk < g.n[2]);
+}
+
+}  // namespace ml
