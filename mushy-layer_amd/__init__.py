@@ -256,6 +256,13 @@ def face_shape(n, d, ncomp):
     return (ncomp, m[2], m[1], m[0])
 
 
+def valid(n, x):
+    """Valid-cell view (ghost ring removed) of a host array in cell layout."""
+    if len(n) == 3:
+        return x[:, 1:-1, 1:-1, 1:-1]
+    return x[:, :, 1:-1, 1:-1]
+
+
 class HCOp:
     """AMRNonLinearMultiCompOp (+ its factory's MG hierarchy) on the GPU."""
 

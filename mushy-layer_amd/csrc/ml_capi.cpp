@@ -281,6 +281,7 @@ int ml_hc_op_destroy(ml_hc_op op) {
     for (int d = 0; d < 3; ++d)
       if (L.b[d]) hipFree(L.b[d]);
     if (L.a) hipFree(L.a);
+    if (L.tmp) hipFree(L.tmp);
   }
   delete op;
   return ML_OK;
@@ -363,7 +364,22 @@ int ml_hc_compute_diffused_var(ml_hc_op op, int depth, const double* phi, double
 int ml_hc_relax(ml_hc_op op, int depth, double* phi, const double* rhs, int iters) {
   HC_DEPTH_CHECK(op, depth);
   if (!phi || !rhs || iters < 0) return ML_EINVAL;
-  ML_HIP(op->ctx, ml::hc_relax<double>(op->args(depth), phi, rhs, 2 * iters, 0, op->ctx->stream));
+  if (iters == 0) return ML_OK;
+  ml::HCLevel& L = op->lev[depth];
+  const size_t bytes = sizeof(double) * 2 * L.g.ncell;
+  if (!L.tmp) {
+    void* p = nullptr;
+    ML_HIP(op->ctx, hipMalloc(&p, bytes));
+    L.tmp = static_cast<double*>(p);
+  }
+  // the tiled smoother ping-pongs between phi and the level's partner
+  // array; the partner starts as a full copy so a state ending there can be
+  // copied back whole
+  ML_HIP(op->ctx, hipMemcpyAsync(L.tmp, phi, bytes, hipMemcpyDeviceToDevice, op->ctx->stream));
+  double* const buf[2] = {phi, L.tmp};
+  int cur = 0;
+  ML_HIP(op->ctx, ml::hc_relax<double>(op->args(depth), buf, &cur, rhs, 2 * iters, 0, op->ctx->stream));
+  if (cur) ML_HIP(op->ctx, hipMemcpyAsync(phi, L.tmp, bytes, hipMemcpyDeviceToDevice, op->ctx->stream));
   return ML_OK;
 }
 

@@ -73,7 +73,9 @@ __device__ inline void closure_jacobian(double H, double C, const ClosureP<doubl
 // FP64 residual of the nonlinear operator, fused with phi += e and the
 // Jacobian, over a tile with a one-cell halo.
 template <int DIM, int TX, int TY, int TZ>
-__global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_kernel(HCArgs<double> A, double* __restrict__ phi,
+__global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_kernel(HCArgs<double> A,
+                                                                           const double* __restrict__ phi,
+                                                                           double* __restrict__ phi_out,
                                                                            const double* __restrict__ rhs,
                                                                            const float* __restrict__ ecorr,
                                                                            float* __restrict__ r32,
@@ -128,8 +130,8 @@ __global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_kernel(HCArg
     const PosRec& r = R[kk];
     if (!r.owner) continue;
     if (ecorr) {
-      phi[r.q] = sphi[r.e];
-      phi[r.q + g.ncell] = sphi[E + r.e];
+      phi_out[r.q] = sphi[r.e];
+      phi_out[r.q + g.ncell] = sphi[E + r.e];
     }
     CellFaces<double, DIM> F;
     load_faces<double, DIM>(A, r.i, r.j, r.k, F);
@@ -194,8 +196,9 @@ __device__ inline void block_update(const CellFaces<T, DIM>& F, const T* aq, con
 
 // Red/black block relaxation of J e = r over a tile with halo npass.
 template <int DIM, int TX, int TY, int TZ, int HMAX, int NT>
-__global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, BCDesc bc, float alpha, float beta,
-                                                            int npass, int zero_init, Geom gc,
+__global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, const float* __restrict__ ein,
+                                                            float* __restrict__ eout, BCDesc bc, float alpha,
+                                                            float beta, int npass, int zero_init, Geom gc,
                                                             const float* __restrict__ pc, int ntx, int nty) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2 * HMAX) * (TY + 2 * HMAX) * (DIM == 3 ? TZ + 2 * HMAX : 1);
@@ -215,7 +218,7 @@ __global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, B
   for (int kk = 0; kk < KMAX; ++kk) {
     const PosRec& r = R[kk];
     if (r.kind != 1) continue;
-    float e0 = zero_init ? 0.0f : L.e[r.q], e1 = zero_init ? 0.0f : L.e[r.q + g.ncell];
+    float e0 = zero_init ? 0.0f : ein[r.q], e1 = zero_init ? 0.0f : ein[r.q + g.ncell];
     if (pc) {
       e0 = prolong_value<float, DIM>(gc, r.i, r.j, r.k, pc, nullptr, e0);
       e1 = prolong_value<float, DIM>(gc, r.i, r.j, r.k, pc + gc.ncell, nullptr, e1);
@@ -275,8 +278,8 @@ __global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, B
   for (int kk = 0; kk < KMAX; ++kk) {
     const PosRec& r = R[kk];
     if (!r.owner) continue;
-    L.e[r.q] = se[r.e];
-    L.e[r.q + g.ncell] = se[E + r.e];
+    eout[r.q] = se[r.e];
+    eout[r.q + g.ncell] = se[E + r.e];
   }
 }
 
@@ -642,27 +645,28 @@ size_t lin_extent(int halo) {
 
 }  // namespace
 
-hipError_t hc_residual_jacobian(const HCArgs<double>& A, double* phi, const double* rhs, const float* e, float* r32,
-                                float* M32, unsigned long long* norm, hipStream_t s) {
+hipError_t hc_residual_jacobian(const HCArgs<double>& A, const double* phi, double* phi_out, const double* rhs,
+                                const float* e, float* r32, float* M32, unsigned long long* norm, hipStream_t s) {
+  if (e && (!phi_out || phi_out == phi)) return hipErrorInvalidValue;
   int ntx, nty;
   if (A.g.dim == 2) {
     using D = LinTile<2>;
     const int nt = lin_tiles<2>(A.g, ntx, nty);
     const size_t lds = 4 * lin_extent<2>(1) * sizeof(double);
     hipLaunchKernelGGL((hc_residual_jacobian_kernel<2, D::TX, D::TY, D::TZ>), dim3(nt), dim3(kLinThreads), lds, s, A,
-                       phi, rhs, e, r32, M32, norm, ntx, nty);
+                       phi, phi_out, rhs, e, r32, M32, norm, ntx, nty);
   } else {
     using D = LinTile<3>;
     const int nt = lin_tiles<3>(A.g, ntx, nty);
     const size_t lds = 4 * lin_extent<3>(1) * sizeof(double);
     hipLaunchKernelGGL((hc_residual_jacobian_kernel<3, D::TX, D::TY, D::TZ>), dim3(nt), dim3(kLinThreads), lds, s, A,
-                       phi, rhs, e, r32, M32, norm, ntx, nty);
+                       phi, phi_out, rhs, e, r32, M32, norm, ntx, nty);
   }
   return hipGetLastError();
 }
 
-hipError_t lin_relax(const LinLevel<float>& L, const BCDesc& bc, float alpha, float beta, int npass, bool zero_init,
-                     const Geom* gc, const float* pc, hipStream_t s) {
+hipError_t lin_relax(const LinLevel<float>& L, float* const ebuf[2], int* cur, const BCDesc& bc, float alpha,
+                     float beta, int npass, bool zero_init, const Geom* gc, const float* pc, hipStream_t s) {
   int ntx, nty;
   const Geom gcc = gc ? *gc : L.g;
   int done = 0;
@@ -671,6 +675,10 @@ hipError_t lin_relax(const LinLevel<float>& L, const BCDesc& bc, float alpha, fl
   bool first = true;
   while (done < npass || (first && npass == 0 && (zero_init || pc))) {
     first = false;
+    const float* ein = ebuf[*cur];
+    float* eout = ebuf[1 - *cur];
+    const int zi = (done == 0 && zero_init) ? 1 : 0;
+    const float* p0 = done == 0 ? pc : nullptr;
     if (L.g.dim == 2) {
       using D = LinTile<2>;
       const int nt = lin_tiles<2>(L.g, ntx, nty);
@@ -679,24 +687,23 @@ hipError_t lin_relax(const LinLevel<float>& L, const BCDesc& bc, float alpha, fl
       const size_t lds = 8 * lin_extent<2>(p) * sizeof(float);
       if (pmax == 8)
         hipLaunchKernelGGL((lin_relax_tile_kernel<2, D::TX, D::TY, D::TZ, 8, 512>), dim3(nt), dim3(512), lds, s, L,
-                           bc, alpha, beta, p, (done == 0 && zero_init) ? 1 : 0, gcc, done == 0 ? pc : nullptr,
-                           ntx, nty);
+                           ein, eout, bc, alpha, beta, p, zi, gcc, p0, ntx, nty);
       else
         hipLaunchKernelGGL((lin_relax_tile_kernel<2, D::TX, D::TY, D::TZ, 2, 256>), dim3(nt), dim3(256), lds, s, L,
-                           bc, alpha, beta, p, (done == 0 && zero_init) ? 1 : 0, gcc, done == 0 ? pc : nullptr,
-                           ntx, nty);
+                           ein, eout, bc, alpha, beta, p, zi, gcc, p0, ntx, nty);
       done += p;
     } else {
       using D = LinTile<3>;
       const int nt = lin_tiles<3>(L.g, ntx, nty);
       const int p = (npass - done) < 2 ? (npass - done) : 2;
       const size_t lds = 8 * lin_extent<3>(p) * sizeof(float);
-      hipLaunchKernelGGL((lin_relax_tile_kernel<3, D::TX, D::TY, D::TZ, 2, 512>), dim3(nt), dim3(512), lds, s, L, bc,
-                         alpha, beta, p, (done == 0 && zero_init) ? 1 : 0, gcc, done == 0 ? pc : nullptr, ntx, nty);
+      hipLaunchKernelGGL((lin_relax_tile_kernel<3, D::TX, D::TY, D::TZ, 2, 512>), dim3(nt), dim3(512), lds, s, L, ein,
+                         eout, bc, alpha, beta, p, zi, gcc, p0, ntx, nty);
       done += p;
     }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
+    *cur = 1 - *cur;
   }
   return hipSuccess;
 }

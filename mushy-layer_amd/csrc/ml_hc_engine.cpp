@@ -10,6 +10,7 @@
 //   coarse levels:  one single-workgroup kernel for the whole sub-cycle,
 //                   including the 40-iteration relax bottom solve.
 // Norms accumulate into per-iteration slots zeroed once per solve.
+#include <algorithm>
 #include <cstring>
 
 #include "ml_internal.h"
@@ -63,6 +64,14 @@ int HCEngine::init_refined() {
     }
     L.a = nullptr;
   }
+  m_ealt.assign(m_L, nullptr);
+  m_ecur.assign(m_L, 0);
+  for (int l = 0; l < m_lc; ++l) {
+    float* p = nullptr;
+    int rc;
+    if ((rc = alloc(2 * m_op->lev[l].g.ncell, &p))) return rc;
+    m_ealt[l] = p;
+  }
   void* v = nullptr;
   ML_HIP(ctx, hipMalloc(&v, sizeof(LinDesc)));
   m_lindesc = static_cast<LinDesc*>(v);
@@ -76,14 +85,19 @@ int HCEngine::lin_vcycle(const ml_solve_opts& o) {
   ml_ctx ctx = m_op->ctx;
   hipStream_t s = ctx->stream;
   const float alpha = float(m_op->alpha), beta = float(m_op->beta);
+  auto ecur = [&](int l) { return m_ecur[l] ? m_ealt[l] : m_lin[l].e; };
   for (int l = 0; l < m_lc; ++l) {
-    ML_HIP(ctx, lin_relax(m_lin[l], m_op->bc, alpha, beta, 2 * o.nu1, true, nullptr, nullptr, s));
-    ML_HIP(ctx, lin_restrict(m_lin[l], m_lin[l + 1], m_op->bc, alpha, beta, s));
+    float* const eb[2] = {m_lin[l].e, m_ealt[l]};
+    ML_HIP(ctx, lin_relax(m_lin[l], eb, &m_ecur[l], m_op->bc, alpha, beta, 2 * o.nu1, true, nullptr, nullptr, s));
+    LinLevel<float> Lf = m_lin[l];
+    Lf.e = ecur(l);
+    ML_HIP(ctx, lin_restrict(Lf, m_lin[l + 1], m_op->bc, alpha, beta, s));
   }
   ML_HIP(ctx, lin_coarse_vcycle(m_lindesc, m_op->dim, s));
   for (int l = m_lc - 1; l >= 0; --l) {
     const Geom gc = m_lin[l + 1].g;
-    ML_HIP(ctx, lin_relax(m_lin[l], m_op->bc, alpha, beta, 2 * o.nu2, false, &gc, m_lin[l + 1].e, s));
+    float* const eb[2] = {m_lin[l].e, m_ealt[l]};
+    ML_HIP(ctx, lin_relax(m_lin[l], eb, &m_ecur[l], m_op->bc, alpha, beta, 2 * o.nu2, false, &gc, ecur(l + 1), s));
   }
   return ML_OK;
 }
@@ -123,7 +137,11 @@ int HCEngine::solve_refined(double* phi, const double* rhs, const ml_solve_opts&
   ML_HIP(ctx, zero_u64(m_slots, o.max_iter + 1, s));
 
   const HCArgs<double> A = m_op->args(0);
-  ML_HIP(ctx, hc_residual_jacobian(A, phi, rhs, nullptr, m_lin[0].r, m_lin[0].M, m_slots, s));
+  const size_t bytes0 = sizeof(double) * 2 * m_op->lev[0].g.ncell;
+  ML_HIP(ctx, hipMemcpyAsync(m_alt[0], phi, bytes0, hipMemcpyDeviceToDevice, s));
+  double* const pb[2] = {phi, m_alt[0]};
+  int pc = 0;
+  ML_HIP(ctx, hc_residual_jacobian(A, phi, nullptr, rhs, nullptr, m_lin[0].r, m_lin[0].M, m_slots, s));
   double r0 = 0;
   int rc;
   if ((rc = read_norm(0, &r0))) return rc;
@@ -133,7 +151,9 @@ int HCEngine::solve_refined(double* phi, const double* rhs, const ml_solve_opts&
   while (status == 0 && iter < o.max_iter) {
     if ((rc = lin_vcycle(o))) return rc;
     ++iter;
-    ML_HIP(ctx, hc_residual_jacobian(A, phi, rhs, m_lin[0].e, m_lin[0].r, m_lin[0].M, m_slots + iter, s));
+    const float* e0 = m_ecur[0] ? m_ealt[0] : m_lin[0].e;
+    ML_HIP(ctx, hc_residual_jacobian(A, pb[pc], pb[1 - pc], rhs, e0, m_lin[0].r, m_lin[0].M, m_slots + iter, s));
+    pc = 1 - pc;
     if ((rc = read_norm(iter, &norm))) return rc;
     if (norm <= o.eps * r0) status |= 1;
     if (norm <= o.norm_thresh) status |= 8;
@@ -141,6 +161,7 @@ int HCEngine::solve_refined(double* phi, const double* rhs, const ml_solve_opts&
     prev = norm;
   }
   if (status == 0) status = 2;
+  if (pc) ML_HIP(ctx, hipMemcpyAsync(phi, m_alt[0], bytes0, hipMemcpyDeviceToDevice, s));
   if (st) {
     st->exit_status = status;
     st->iterations = iter;
@@ -159,6 +180,8 @@ HCEngine::~HCEngine() {
     if (m_rhs[l]) hipFree(m_rhs[l]);
     if (m_phi0[l]) hipFree(m_phi0[l]);
   }
+  for (double* q : m_alt)
+    if (q) hipFree(q);
   if (m_rhs_be) hipFree(m_rhs_be);
   if (m_slots) hipFree(m_slots);
   if (m_hslots) hipHostFree(m_hslots);
@@ -190,6 +213,17 @@ int HCEngine::init() {
     ML_HIP(ctx, vec_set(2 * m_op->lev[l].g.ncell, m_phi[l], 0.0, ctx->stream));
     ML_HIP(ctx, vec_set(2 * m_op->lev[l].g.ncell, m_rhs[l], 0.0, ctx->stream));
     ML_HIP(ctx, vec_set(2 * m_op->lev[l].g.ncell, m_phi0[l], 0.0, ctx->stream));
+  }
+  // relax ping-pong partners: every level relaxed by the tiled kernels
+  // (above m_lc, or all levels when none fits one workgroup) and level 0
+  m_alt.assign(m_L, nullptr);
+  m_cur.assign(m_L, 0);
+  for (int l = 0; l < m_L; ++l) {
+    if (l > 0 && l >= m_lc) break;
+    void* q = nullptr;
+    ML_HIP(ctx, hipMalloc(&q, sizeof(double) * 2 * m_op->lev[l].g.ncell));
+    m_alt[l] = static_cast<double*>(q);
+    ML_HIP(ctx, vec_set(2 * m_op->lev[l].g.ncell, m_alt[l], 0.0, ctx->stream));
   }
   void* p = nullptr;
   ML_HIP(ctx, hipMalloc(&p, sizeof(double) * 2 * m_op->lev[0].g.ncell));
@@ -237,17 +271,20 @@ int HCEngine::upload_desc(double* phi0, const double* rhs0, const ml_solve_opts&
 int HCEngine::vcycle(double* phi, const double* rhs, const ml_solve_opts& o) {
   ml_ctx ctx = m_op->ctx;
   hipStream_t s = ctx->stream;
-  auto P = [&](int l) { return l == 0 ? phi : m_phi[l]; };
+  auto P = [&](int l) { return cur(l, phi); };
   auto R = [&](int l) { return l == 0 ? const_cast<double*>(rhs) : m_rhs[l]; };
+  double* B[2];
   const int ltiles = m_lc < m_L ? m_lc : m_L - 1;   // tiled levels above the coarse kernel
   if (m_lc >= m_L && m_L == 1) {
     // a single level too large for one workgroup: relax bottom with tiles
-    ML_HIP(ctx, hc_relax<double>(m_op->args(0), phi, rhs, 2 * o.bottom, 0, s));
+    bufs(0, phi, B);
+    ML_HIP(ctx, hc_relax<double>(m_op->args(0), B, &m_cur[0], rhs, 2 * o.bottom, 0, s));
     return ML_OK;
   }
   for (int l = 0; l < ltiles; ++l) {
     const HCArgs<double> A = m_op->args(l);
-    ML_HIP(ctx, hc_relax<double>(A, P(l), R(l), 2 * o.nu1, 0, s));
+    bufs(l, phi, B);
+    ML_HIP(ctx, hc_relax<double>(A, B, &m_cur[l], R(l), 2 * o.nu1, 0, s));
     ML_HIP(ctx, hc_restrict_all<double>(A, m_op->lev[l + 1].g, P(l), R(l), R(l + 1), P(l + 1), m_phi0[l + 1], 0,
                                         s));
     if (l + 1 < m_lc) ML_HIP(ctx, hc_apply_op<double>(m_op->args(l + 1), P(l + 1), R(l + 1), R(l + 1), 0, s));
@@ -257,12 +294,14 @@ int HCEngine::vcycle(double* phi, const double* rhs, const ml_solve_opts& o) {
   } else {
     // no level small enough for one workgroup: the coarsest tiled level is
     // the bottom
-    ML_HIP(ctx, hc_relax<double>(m_op->args(m_L - 1), P(m_L - 1), R(m_L - 1), 2 * o.bottom, 0, s));
+    bufs(m_L - 1, phi, B);
+    ML_HIP(ctx, hc_relax<double>(m_op->args(m_L - 1), B, &m_cur[m_L - 1], R(m_L - 1), 2 * o.bottom, 0, s));
   }
   for (int l = ltiles - 1; l >= 0; --l) {
     const HCArgs<double> A = m_op->args(l);
     const Geom gc = m_op->lev[l + 1].g;
-    ML_HIP(ctx, hc_relax_tiles<double>(A, P(l), R(l), 2 * o.nu2, 0, &gc, P(l + 1), m_phi0[l + 1], s));
+    bufs(l, phi, B);
+    ML_HIP(ctx, hc_relax_tiles<double>(A, B, &m_cur[l], R(l), 2 * o.nu2, 0, &gc, P(l + 1), m_phi0[l + 1], s));
   }
   return ML_OK;
 }
@@ -290,6 +329,11 @@ int HCEngine::solve(double* phi, const double* rhs, const ml_solve_opts& o, ml_s
   if (o.nu1 < 0 || o.nu2 < 0 || o.bottom < 0) return fail(ctx, ML_EINVAL, "negative smoothing count");
   int rc = upload_desc(phi, rhs, o);
   if (rc) return rc;
+  // the level-0 partner starts as a full copy (ghost cells included) so
+  // that a state ending in it can be copied back whole
+  const size_t bytes0 = sizeof(double) * 2 * m_op->lev[0].g.ncell;
+  ML_HIP(ctx, hipMemcpyAsync(m_alt[0], phi, bytes0, hipMemcpyDeviceToDevice, ctx->stream));
+  std::fill(m_cur.begin(), m_cur.end(), 0);
   ML_HIP(ctx, zero_u64(m_slots, o.max_iter + 1, ctx->stream));
   if ((rc = residual_norm(phi, rhs, 0))) return rc;
   double r0 = 0;
@@ -300,7 +344,7 @@ int HCEngine::solve(double* phi, const double* rhs, const ml_solve_opts& o, ml_s
   while (status == 0 && iter < o.max_iter) {
     if ((rc = vcycle(phi, rhs, o))) return rc;
     ++iter;
-    if ((rc = residual_norm(phi, rhs, iter))) return rc;
+    if ((rc = residual_norm(cur(0, phi), rhs, iter))) return rc;
     if ((rc = read_norm(iter, &norm))) return rc;
     if (norm <= o.eps * r0) status |= 1;
     if (norm <= o.norm_thresh) status |= 8;
@@ -308,6 +352,10 @@ int HCEngine::solve(double* phi, const double* rhs, const ml_solve_opts& o, ml_s
     prev = norm;
   }
   if (status == 0) status = 2;
+  if (m_cur[0]) {
+    ML_HIP(ctx, hipMemcpyAsync(phi, m_alt[0], bytes0, hipMemcpyDeviceToDevice, ctx->stream));
+    m_cur[0] = 0;
+  }
   if (st) {
     st->exit_status = status;
     st->iterations = iter;

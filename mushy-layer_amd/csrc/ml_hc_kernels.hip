@@ -169,8 +169,8 @@ __global__ __launch_bounds__(NT) void hc_relax_tile_kernel(HCArgs<T> A, TileIO<T
   for (int kk = 0; kk < KMAX; ++kk) {
     const PosRec& r = R[kk];
     if (!r.owner) continue;
-    io.phi[r.q] = sphi[r.e];
-    io.phi[r.q + g.ncell] = sphi[E + r.e];
+    io.phi_out[r.q] = sphi[r.e];
+    io.phi_out[r.q + g.ncell] = sphi[E + r.e];
   }
 }
 
@@ -677,13 +677,14 @@ static hipError_t launch_relax_tile(const HCArgs<T>& A, TileIO<T> io, hipStream_
 }
 
 template <typename T>
-hipError_t hc_relax_tiles(const HCArgs<T>& A, T* phi, const T* rhs, int npass, int first, const Geom* gc,
-                          const T* pc, const T* pc0, hipStream_t s) {
+hipError_t hc_relax_tiles(const HCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, int first,
+                          const Geom* gc, const T* pc, const T* pc0, hipStream_t s) {
   const int pmax = A.g.dim == 2 ? relax_passes_per_launch<T, 2>(A.g) : relax_passes_per_launch<T, 3>(A.g);
   int done = 0;
   while (done < npass) {
     TileIO<T> io{};
-    io.phi = phi;
+    io.phi = buf[*cur];
+    io.phi_out = buf[1 - *cur];
     io.rhs = rhs;
     io.npass = (npass - done) < pmax ? (npass - done) : pmax;
     io.first = (first + done) & 1;
@@ -694,19 +695,21 @@ hipError_t hc_relax_tiles(const HCArgs<T>& A, T* phi, const T* rhs, int npass, i
     }
     hipError_t e = A.g.dim == 2 ? launch_relax_tile<T, 2>(A, io, s) : launch_relax_tile<T, 3>(A, io, s);
     if (e != hipSuccess) return e;
+    *cur = 1 - *cur;
     done += io.npass;
   }
   if (npass == 0 && pc) {
-    // a prolongation with no smoothing after it
-    return prolong_linear<T>(A.g, 2, phi, pc, pc0, s);
+    // a prolongation with no smoothing after it (per cell, in place)
+    return prolong_linear<T>(A.g, 2, buf[*cur], pc, pc0, s);
   }
   return hipSuccess;
 }
 
 template <typename T>
-hipError_t hc_relax(const HCArgs<T>& A, T* phi, const T* rhs, int npass, int first, hipStream_t s) {
+hipError_t hc_relax(const HCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, int first,
+                    hipStream_t s) {
   if (npass <= 0) return hipSuccess;
-  return hc_relax_tiles<T>(A, phi, rhs, npass, first, nullptr, nullptr, nullptr, s);
+  return hc_relax_tiles<T>(A, buf, cur, rhs, npass, first, nullptr, nullptr, nullptr, s);
 }
 
 template <typename T>
@@ -782,9 +785,10 @@ hipError_t hc_diffused_var(const HCArgs<T>& A, const T* phi, T* out, int homog, 
   return hipGetLastError();
 }
 
-template hipError_t hc_relax<double>(const HCArgs<double>&, double*, const double*, int, int, hipStream_t);
-template hipError_t hc_relax_tiles<double>(const HCArgs<double>&, double*, const double*, int, int, const Geom*,
-                                           const double*, const double*, hipStream_t);
+template hipError_t hc_relax<double>(const HCArgs<double>&, double* const[2], int*, const double*, int, int,
+                                     hipStream_t);
+template hipError_t hc_relax_tiles<double>(const HCArgs<double>&, double* const[2], int*, const double*, int, int,
+                                           const Geom*, const double*, const double*, hipStream_t);
 template hipError_t hc_relax_small<double>(const CoarseDesc<double>*, int, int, int, hipStream_t);
 template hipError_t hc_coarse_vcycle<double>(const CoarseDesc<double>*, int, int, hipStream_t);
 template hipError_t hc_residual<double>(const HCArgs<double>&, double*, const double*, double*,

@@ -37,6 +37,7 @@ struct HCLevel {
   Geom g;
   double* b[3] = {nullptr, nullptr, nullptr};
   double* a = nullptr;
+  double* tmp = nullptr;   // ping-pong partner for the per-level relax entry point (lazy)
 };
 
 struct HostDriver;   // reference-API FAS driver state (ml_capi.cpp)
@@ -94,9 +95,18 @@ class HCEngine {
   int solve_refined(double* phi, const double* rhs, const ml_solve_opts& o, ml_solve_stats* st);
   int init_refined();
   int lin_vcycle(const ml_solve_opts& o);
+  // state buffer pair of level l: {level-0 caller array or m_phi[l], m_alt[l]}
+  void bufs(int l, double* phi, double* b[2]) const {
+    b[0] = l == 0 ? phi : m_phi[l];
+    b[1] = m_alt[l];
+  }
+  double* cur(int l, double* phi) const { return m_cur[l] ? m_alt[l] : (l == 0 ? phi : m_phi[l]); }
 
-  // FP32-refined state per level
+  // FP32-refined state per level; the corrections of tiled levels
+  // ping-pong between m_lin[l].e and m_ealt[l] (m_ecur[l] says which)
   std::vector<LinLevel<float>> m_lin;
+  std::vector<float*> m_ealt;
+  std::vector<int> m_ecur;
   std::vector<float*> m_lin_owned;
   LinDesc* m_lindesc = nullptr;      // device
   LinDesc* m_hlindesc = nullptr;     // pinned host staging
@@ -105,6 +115,8 @@ class HCEngine {
   ml_hc_op m_op;
   int m_L = 0, m_lc = 0;
   std::vector<double*> m_phi, m_rhs, m_phi0;   // per level (level 0: caller's arrays)
+  std::vector<double*> m_alt;                  // relax ping-pong partners (tiled levels, and level 0)
+  std::vector<int> m_cur;                      // which of the pair holds the state
   double* m_rhs_be = nullptr;                   // backward-Euler rhs of level 0
   unsigned long long* m_slots = nullptr;       // max-norm accumulators
   unsigned long long* m_hslots = nullptr;      // pinned mirror

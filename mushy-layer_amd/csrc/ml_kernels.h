@@ -51,6 +51,7 @@ __device__ inline void block_max_atomic(T v, int nan, unsigned long long* out) {
 template <typename T>
 struct TileIO {
   T* phi;
+  T* phi_out;     // relax: the relaxed state is written here (never == phi)
   const T* rhs;
   T* out;
   const T* add;
@@ -142,14 +143,16 @@ struct LinDesc {
   const double* a64;
 };
 
-// fine level: phi += e (when e != nullptr, written back), r32 = rhs - L(phi),
-// M32 = closure Jacobian, max-norm of r into *norm
-hipError_t hc_residual_jacobian(const HCArgs<double>& A, double* phi, const double* rhs, const float* e,
-                                float* r32, float* M32, unsigned long long* norm, hipStream_t s);
-// red/black block relaxation of J e = r on a tiled level; zero_init starts
-// from e = 0; pc (coarse correction, level gc) is prolonged into e first
-hipError_t lin_relax(const LinLevel<float>& L, const BCDesc& bc, float alpha, float beta, int npass,
-                     bool zero_init, const Geom* gc, const float* pc, hipStream_t s);
+// fine level: phi_out = phi + e (when e != nullptr; phi_out != phi), then
+// r32 = rhs - L(phi_out), M32 = closure Jacobian, max-norm of r into *norm
+hipError_t hc_residual_jacobian(const HCArgs<double>& A, const double* phi, double* phi_out, const double* rhs,
+                                const float* e, float* r32, float* M32, unsigned long long* norm, hipStream_t s);
+// red/black block relaxation of J e = r on a tiled level, ping-ponging
+// between ebuf[0] and ebuf[1] (*cur indexes the current correction; L.e is
+// ignored); zero_init starts from e = 0; pc (coarse correction, level gc) is
+// prolonged into e first
+hipError_t lin_relax(const LinLevel<float>& L, float* const ebuf[2], int* cur, const BCDesc& bc, float alpha,
+                     float beta, int npass, bool zero_init, const Geom* gc, const float* pc, hipStream_t s);
 // r_c = R(r - J e), M_c = R(M) on the next coarser level
 hipError_t lin_restrict(const LinLevel<float>& L, const LinLevel<float>& C, const BCDesc& bc, float alpha,
                         float beta, hipStream_t s);
@@ -159,12 +162,17 @@ hipError_t lin_coarse_vcycle(const LinDesc* dev_desc, int dim, hipStream_t s);
 hipError_t cast_f64_f32(long long n, const double* x, float* y, hipStream_t s);
 
 // ---- nonlinear (H, Theta) operator (ml_hc_kernels.hip)
+// Tiled red/black relaxation.  Every launch reads buf[*cur] and writes
+// buf[1 - *cur] (tiles read their halo from the input while other tiles
+// store), then flips *cur.
 template <typename T>
-hipError_t hc_relax(const HCArgs<T>& A, T* phi, const T* rhs, int npass, int first, hipStream_t s);
-// relax tiles with the linear prolongation of pc - pc0 (level gc) fused on load
+hipError_t hc_relax(const HCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, int first,
+                    hipStream_t s);
+// relax tiles with the linear prolongation of pc - pc0 (level gc) fused on
+// load; npass == 0 prolongs in place into buf[*cur]
 template <typename T>
-hipError_t hc_relax_tiles(const HCArgs<T>& A, T* phi, const T* rhs, int npass, int first, const Geom* gc,
-                          const T* pc, const T* pc0, hipStream_t s);
+hipError_t hc_relax_tiles(const HCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, int first,
+                          const Geom* gc, const T* pc, const T* pc0, hipStream_t s);
 template <typename T>
 hipError_t hc_relax_small(const CoarseDesc<T>* dev_desc, int dim, int level, int npass, hipStream_t s);
 template <typename T>

@@ -182,3 +182,35 @@ def test_be_step_parity(ml, gpu_ctx, wl, name, max_iter, driver):
     assert e <= TOL, e
     assert st.final_residual == st_ref.final_residual
     assert HC.bitwise_equal(HC.valid(n, out), HC.valid(n, new))
+
+
+LARGE = {
+    "2d_512": dict(n=(512, 512)),
+    "2d_1024x256": dict(n=(1024, 256), periodic=[0, 0]),
+    "3d_64": dict(n=(64, 64, 64)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(LARGE))
+@pytest.mark.parametrize("iters", [1, 4])
+def test_relax_parity_many_tiles(ml, gpu_ctx, wl, name, iters):
+    """Grids with more tiles than the chip holds at once: a tile that starts
+    after its neighbours have stored must still see the previous sweep's
+    state (the smoother writes a partner array, never its input)."""
+    case = _case(wl, **LARGE[name])
+    case["phi"] = _mixed_state(case, 11)
+    g, p, bc = HC.oracle_setup(case)
+    b = HC.bcoef_from_state(case)
+    op = HC.gpu_setup(ml, gpu_ctx, case)
+    db = [ml.DeviceArray(gpu_ctx, x.shape, host=x) for x in b]
+    n = case["n"]
+    dt = case["dt"]
+    phi = case["phi"].copy()
+    rhs = case["phi"].copy()
+    op.set_coefs(None, db, 1.0, -dt)
+    ref = O.hc_gsrb(g, p, bc, 1.0, None, -dt, b, phi.copy(), rhs, iters=iters)
+    dphi = ml.DeviceArray(gpu_ctx, phi.shape, host=phi)
+    drhs = ml.DeviceArray(gpu_ctx, rhs.shape, host=rhs)
+    op.relax(0, dphi, drhs, iters)
+    out = dphi.get()
+    assert HC.bitwise_equal(HC.valid(n, out), HC.valid(n, ref))
