@@ -154,6 +154,32 @@ int mlr_hc_be_step(const mlr_grid* g, const mlr_params* p, const mlr_bc* bc,
                    double* phi_new, const double* phi_old, const double* src,
                    const mlr_opts* o, mlr_stats* st);
 
+/* ---- linear variable-coefficient Helmholtz / Poisson (DarcyBrinkmanOp,
+ * AMRProjectionOp): L u = alpha*a*u - beta*c*u - beta*div(b grad u);
+ * a, c nullable (a == NULL: aCoef == 1; c == NULL: no c term). ----------- */
+void mlr_lin_gsrb(const mlr_grid* g, int ncomp, const mlr_bc* bc, double alpha,
+                  const double* a, double beta, const double* c,
+                  const double* const* b, const double* lambda, double* phi,
+                  const double* rhs);
+void mlr_lin_residual(const mlr_grid* g, int ncomp, const mlr_bc* bc,
+                      double alpha, const double* a, double beta,
+                      const double* c, const double* const* b, double* phi,
+                      const double* rhs, double* res, int homogeneous);
+void mlr_lin_apply_op(const mlr_grid* g, int ncomp, const mlr_bc* bc,
+                      double alpha, const double* a, double beta,
+                      const double* c, const double* const* b, double* phi,
+                      double* lhs, int homogeneous);
+void mlr_lin_restrict_residual(const mlr_grid* gf, int ncomp,
+                               const mlr_bc* bc, double alpha, const double* a,
+                               double beta, const double* c,
+                               const double* const* b, double* phi,
+                               const double* rhs, double* resc);
+int mlr_lin_mg_solve(const mlr_grid* g, int ncomp, const mlr_bc* bc,
+                     double alpha, const double* a, double beta,
+                     const double* c, const double* const* b, double* phi,
+                     const double* rhs, int prolong_linear, const mlr_opts* o,
+                     mlr_stats* st);
+
 #ifdef __cplusplus
 }
 #endif

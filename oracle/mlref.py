@@ -99,6 +99,14 @@ def _declare(L):
                                        C.POINTER(Opts), C.POINTER(Stats)]),
         "mlr_hc_be_step": (C.c_int, [G, P, B, _dp, pp, C.c_double, _dp, _dp, _dp,
                                      C.POINTER(Opts), C.POINTER(Stats)]),
+        "mlr_lin_gsrb": (None, [G, C.c_int, B, C.c_double, _dp, C.c_double, _dp, pp, _dp, _dp, _dp]),
+        "mlr_lin_residual": (None, [G, C.c_int, B, C.c_double, _dp, C.c_double, _dp, pp, _dp, _dp, _dp,
+                                    C.c_int]),
+        "mlr_lin_apply_op": (None, [G, C.c_int, B, C.c_double, _dp, C.c_double, _dp, pp, _dp, _dp, C.c_int]),
+        "mlr_lin_restrict_residual": (None, [G, C.c_int, B, C.c_double, _dp, C.c_double, _dp, pp, _dp, _dp,
+                                             _dp]),
+        "mlr_lin_mg_solve": (C.c_int, [G, C.c_int, B, C.c_double, _dp, C.c_double, _dp, pp, _dp, _dp, C.c_int,
+                                       C.POINTER(Opts), C.POINTER(Stats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -318,4 +326,43 @@ def hc_be_step(g, p, bc, a, b, dt, phi_new, phi_old, src=None, opts=None):
     o = opts or default_opts()
     lib().mlr_hc_be_step(C.byref(g), C.byref(p), C.byref(bc), ptr(a), ptrs(b), dt, ptr(phi_new),
                          ptr(phi_old), ptr(src), C.byref(o), C.byref(st))
+    return st
+
+
+# ------------------------------------------------- linear VC Helmholtz / Poisson
+
+def lin_gsrb(g, ncomp, bc, alpha, a, beta, c, b, phi, rhs, iters=1):
+    lm = lam(g, ncomp, alpha, a, beta, b, c)
+    for _ in range(iters):
+        lib().mlr_lin_gsrb(C.byref(g), ncomp, C.byref(bc), alpha, ptr(a), beta, ptr(c), ptrs(b), ptr(lm),
+                           ptr(phi), ptr(rhs))
+    return phi
+
+
+def lin_residual(g, ncomp, bc, alpha, a, beta, c, b, phi, rhs, homogeneous=0):
+    res = np.zeros(cell_shape(g, ncomp))
+    lib().mlr_lin_residual(C.byref(g), ncomp, C.byref(bc), alpha, ptr(a), beta, ptr(c), ptrs(b), ptr(phi),
+                           ptr(rhs), ptr(res), int(homogeneous))
+    return res
+
+
+def lin_apply_op(g, ncomp, bc, alpha, a, beta, c, b, phi, homogeneous=0):
+    lhs = np.zeros(cell_shape(g, ncomp))
+    lib().mlr_lin_apply_op(C.byref(g), ncomp, C.byref(bc), alpha, ptr(a), beta, ptr(c), ptrs(b), ptr(phi),
+                           ptr(lhs), int(homogeneous))
+    return lhs
+
+
+def lin_restrict_residual(g, ncomp, bc, alpha, a, beta, c, b, phi, rhs):
+    resc = np.zeros(cell_shape(coarsen(g, 2), ncomp))
+    lib().mlr_lin_restrict_residual(C.byref(g), ncomp, C.byref(bc), alpha, ptr(a), beta, ptr(c), ptrs(b),
+                                    ptr(phi), ptr(rhs), ptr(resc))
+    return resc
+
+
+def lin_mg_solve(g, ncomp, bc, alpha, a, beta, c, b, phi, rhs, prolong_linear=0, opts=None):
+    st = Stats()
+    o = opts or default_opts()
+    lib().mlr_lin_mg_solve(C.byref(g), ncomp, C.byref(bc), alpha, ptr(a), beta, ptr(c), ptrs(b), ptr(phi),
+                           ptr(rhs), int(prolong_linear), C.byref(o), C.byref(st))
     return st

@@ -1,2 +1,274 @@
-/* placeholder: linear operators added next */
-#include "mlref.h"
+/*
+ * mlref_lin.c — TEST INFRASTRUCTURE ONLY.  CPU restatement of the linear
+ * variable-coefficient operators on the path:
+ *
+ *   L u = alpha*a*u - beta*c*u - beta*div(b grad u)
+ *
+ *   DarcyBrinkmanOp (src/DarcyBrinkmanOp.cpp, src/DarcyBrinkmanOpF.ChF):
+ *     a = 1, b = -Pr, c = (Pr/Da) chi/Pi, BE alpha = 1, beta = -dt, one
+ *     solve per velocity component;
+ *   AMRProjectionOp over VCAMRPoissonOp2 (src/AMRProjectionOp.cpp:31-98):
+ *     c absent, alpha = 0, beta = 1, b = -S_face; piecewise-constant
+ *     prolongation.
+ *
+ * The operator arithmetic follows the DarcyBrinkmanOp Fortran line by line
+ * (left-to-right evaluation, -ffp-contract=off).  With c == NULL the c term
+ * is omitted, which is VCAMRPoissonOp2's form (Chombo, not in the reference
+ * tree: build spec, parity unpinned against Chombo).  The linear multigrid
+ * driver (Chombo AMRMultiGrid + MultiGrid::cycle) is build spec as well.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mlref_internal.h"
+
+/* Ghost fill of a linear-op field: periodic exchange, then the scalar BC
+ * (DarcyBrinkmanOp::residualI / levelGSRB order: m_bc then exchange,
+ * src/DarcyBrinkmanOp.cpp:40-47, :526-539; the two commute on a single box
+ * since BC ghosts and periodic ghosts are disjoint). */
+static void lin_ghosts(const mlr_grid* g, int ncomp, const mlr_bc* bc,
+                       int homogeneous, double* phi) {
+  mlr_scalar_bc(g, ncomp, bc, homogeneous, phi);
+  mlr_exchange(g, ncomp, phi);
+}
+
+/* lofphi of DBCOMPUTEOP / GSRBHELMHOLTZDB (src/DarcyBrinkmanOpF.ChF:125-147,
+ * :262-286): alpha*a*phi - beta*c*phi - beta*(S)*dxinv, S summed in the
+ * Fortran order, c term omitted when c == NULL. */
+static double lin_lof(const mlr_lay* L, int dim, int n, int i, int j, int k,
+                      double alpha, const double* a, double beta,
+                      const double* c, const double* const* b,
+                      const double* phi, double dxinv) {
+  const long q = mlr_ci(L, i, j, k) + n * L->ncell;
+  const double p0 = phi[q];
+  const double* b0 = b[0] + n * L->nface[0];
+  const double* b1 = b[1] + n * L->nface[1];
+  double s = b0[mlr_fi(L, 0, i + 1, j, k)] * (phi[mlr_ci(L, i + 1, j, k) + n * L->ncell] - p0) -
+             b0[mlr_fi(L, 0, i, j, k)] * (p0 - phi[mlr_ci(L, i - 1, j, k) + n * L->ncell]) +
+             b1[mlr_fi(L, 1, i, j + 1, k)] * (phi[mlr_ci(L, i, j + 1, k) + n * L->ncell] - p0) -
+             b1[mlr_fi(L, 1, i, j, k)] * (p0 - phi[mlr_ci(L, i, j - 1, k) + n * L->ncell]);
+  if (dim == 3) {
+    const double* b2 = b[2] + n * L->nface[2];
+    s = s + b2[mlr_fi(L, 2, i, j, k + 1)] * (phi[mlr_ci(L, i, j, k + 1) + n * L->ncell] - p0) -
+        b2[mlr_fi(L, 2, i, j, k)] * (p0 - phi[mlr_ci(L, i, j, k - 1) + n * L->ncell]);
+  }
+  const double aq = a ? a[q] : 1.0;
+  double lof = alpha * aq * p0;
+  if (c) lof = lof - beta * c[q] * p0;
+  return lof - beta * s * dxinv;
+}
+
+/* DarcyBrinkmanOp::levelGSRB (src/DarcyBrinkmanOp.cpp:503-576): for pass in
+ * {0, 1}: exchange, homogeneous BC, GSRBHELMHOLTZDB on the cells with
+ * (i+j+k+pass) even (src/DarcyBrinkmanOpF.ChF:110-153). */
+void mlr_lin_gsrb(const mlr_grid* g, int ncomp, const mlr_bc* bc, double alpha,
+                  const double* a, double beta, const double* c,
+                  const double* const* b, const double* lambda, double* phi,
+                  const double* rhs) {
+  mlr_lay L;
+  mlr_layout(g, &L);
+  const double dxinv = 1.0 / (g->dx * g->dx);
+  for (int pass = 0; pass <= 1; ++pass) {
+    lin_ghosts(g, ncomp, bc, 1, phi);
+    for (int n = 0; n < ncomp; ++n)
+      for (int k = 0; k < L.nz; ++k)
+        for (int j = 0; j < g->n[1]; ++j) {
+          const int imin = abs((j + k + pass) % 2);
+          for (int i = imin; i < g->n[0]; i += 2) {
+            const long q = mlr_ci(&L, i, j, k) + n * L.ncell;
+            const double lof = lin_lof(&L, g->dim, n, i, j, k, alpha, a, beta, c, b, phi, dxinv);
+            phi[q] = phi[q] - lambda[q] * (lof - rhs[q]);
+          }
+        }
+  }
+}
+
+/* DarcyBrinkmanOp::residualI (src/DarcyBrinkmanOp.cpp:31-83) with
+ * DBCOMPUTERES (src/DarcyBrinkmanOpF.ChF:290-400): res = rhs - L(phi). */
+void mlr_lin_residual(const mlr_grid* g, int ncomp, const mlr_bc* bc,
+                      double alpha, const double* a, double beta,
+                      const double* c, const double* const* b, double* phi,
+                      const double* rhs, double* res, int homogeneous) {
+  mlr_lay L;
+  mlr_layout(g, &L);
+  const double dxinv = 1.0 / (g->dx * g->dx);
+  lin_ghosts(g, ncomp, bc, homogeneous, phi);
+  for (int n = 0; n < ncomp; ++n)
+    for (int k = 0; k < L.nz; ++k)
+      for (int j = 0; j < g->n[1]; ++j)
+        for (int i = 0; i < g->n[0]; ++i) {
+          const long q = mlr_ci(&L, i, j, k) + n * L.ncell;
+          res[q] = rhs[q] - lin_lof(&L, g->dim, n, i, j, k, alpha, a, beta, c, b, phi, dxinv);
+        }
+}
+
+/* DarcyBrinkmanOp::applyOpI (src/DarcyBrinkmanOp.cpp:123-137) with
+ * DBCOMPUTEOP (src/DarcyBrinkmanOpF.ChF:160-288): lhs = L(phi). */
+void mlr_lin_apply_op(const mlr_grid* g, int ncomp, const mlr_bc* bc,
+                      double alpha, const double* a, double beta,
+                      const double* c, const double* const* b, double* phi,
+                      double* lhs, int homogeneous) {
+  mlr_lay L;
+  mlr_layout(g, &L);
+  const double dxinv = 1.0 / (g->dx * g->dx);
+  lin_ghosts(g, ncomp, bc, homogeneous, phi);
+  for (int n = 0; n < ncomp; ++n)
+    for (int k = 0; k < L.nz; ++k)
+      for (int j = 0; j < g->n[1]; ++j)
+        for (int i = 0; i < g->n[0]; ++i) {
+          const long q = mlr_ci(&L, i, j, k) + n * L.ncell;
+          lhs[q] = lin_lof(&L, g->dim, n, i, j, k, alpha, a, beta, c, b, phi, dxinv);
+        }
+}
+
+/* DarcyBrinkmanOp::restrictResidual (src/DarcyBrinkmanOp.cpp:183-240) with
+ * RESTRICTRESDB (src/DarcyBrinkmanOpF.ChF:402-509): homogeneous BC,
+ * res_c zeroed, res_c(i/2) += (rhs - lofphi)/denom, denom = 2^d, fine cells
+ * in Fortran order (i fastest). */
+void mlr_lin_restrict_residual(const mlr_grid* gf, int ncomp,
+                               const mlr_bc* bc, double alpha, const double* a,
+                               double beta, const double* c,
+                               const double* const* b, double* phi,
+                               const double* rhs, double* resc) {
+  mlr_grid gc;
+  mlr_coarsen_grid(gf, 2, &gc);
+  mlr_lay Lf, Lc;
+  mlr_layout(gf, &Lf);
+  mlr_layout(&gc, &Lc);
+  const double dxinv = 1.0 / (gf->dx * gf->dx);
+  const double denom = gf->dim == 3 ? 8.0 : 4.0;
+  lin_ghosts(gf, ncomp, bc, 1, phi);
+  for (long q = 0; q < ncomp * Lc.ncell; ++q) resc[q] = 0.0;
+  for (int n = 0; n < ncomp; ++n)
+    for (int k = 0; k < Lf.nz; ++k)
+      for (int j = 0; j < gf->n[1]; ++j)
+        for (int i = 0; i < gf->n[0]; ++i) {
+          const long q = mlr_ci(&Lf, i, j, k) + n * Lf.ncell;
+          const double lof = lin_lof(&Lf, gf->dim, n, i, j, k, alpha, a, beta, c, b, phi, dxinv);
+          const long qc = mlr_ci(&Lc, i / 2, j / 2, gf->dim == 3 ? k / 2 : 0) + n * Lc.ncell;
+          resc[qc] = resc[qc] + (rhs[q] - lof) / denom;
+        }
+}
+
+/* ---- linear multigrid (build spec for Chombo AMRMultiGrid::solve on one
+ * AMR level and MultiGrid::cycle):
+ *   res = rhs - L(phi) (inhomogeneous), r0 = |res|_inf
+ *   per iteration: e = 0; V-cycle on L e = res with homogeneous BCs; phi += e;
+ *                  res = rhs - L(phi); stop as the FAS solve does
+ *   V-cycle on level l: relax(e, r, nu1); if l is not the coarsest:
+ *     r_c = restrictResidual(e, r); e_c = 0; V-cycle(l+1); e += P e_c;
+ *     relax(e, r, nu2);  else relax(e, r, bottom)
+ *   Coefficients of level l are arithmetic averages of the finest level's
+ *   (DarcyBrinkmanOpFactory::MGnewOp, src/DarcyBrinkmanOp.cpp:977-1008).
+ */
+typedef struct {
+  mlr_grid g;
+  double *a, *c, *b[3], *lam;
+  double *e, *r;
+} lin_level;
+
+static void lin_vcycle(lin_level* lev, int l, int nlev, int ncomp,
+                       const mlr_bc* bc, double alpha, double beta,
+                       int prolong_linear, const mlr_opts* o) {
+  lin_level* L = &lev[l];
+  const double* const* bb = (const double* const*)L->b;
+  if (l == nlev - 1) {
+    for (int it = 0; it < o->bottom; ++it)
+      mlr_lin_gsrb(&L->g, ncomp, bc, alpha, L->a, beta, L->c, bb, L->lam, L->e, L->r);
+    return;
+  }
+  for (int it = 0; it < o->nu1; ++it)
+    mlr_lin_gsrb(&L->g, ncomp, bc, alpha, L->a, beta, L->c, bb, L->lam, L->e, L->r);
+  lin_level* C = &lev[l + 1];
+  mlr_lin_restrict_residual(&L->g, ncomp, bc, alpha, L->a, beta, L->c, bb, L->e, L->r, C->r);
+  memset(C->e, 0, sizeof(double) * ncomp * mlr_ncell(&C->g));
+  lin_vcycle(lev, l + 1, nlev, ncomp, bc, alpha, beta, prolong_linear, o);
+  if (prolong_linear)
+    mlr_prolong_linear(&L->g, ncomp, L->e, C->e);
+  else
+    mlr_prolong_constant(&L->g, ncomp, L->e, C->e);
+  for (int it = 0; it < o->nu2; ++it)
+    mlr_lin_gsrb(&L->g, ncomp, bc, alpha, L->a, beta, L->c, bb, L->lam, L->e, L->r);
+}
+
+int mlr_lin_mg_solve(const mlr_grid* g, int ncomp, const mlr_bc* bc,
+                     double alpha, const double* a, double beta,
+                     const double* c, const double* const* b, double* phi,
+                     const double* rhs, int prolong_linear, const mlr_opts* o,
+                     mlr_stats* st) {
+  const int nlev = mlr_mg_depth(g);
+  lin_level* lev = (lin_level*)calloc((size_t)nlev, sizeof(lin_level));
+  for (int l = 0; l < nlev; ++l) {
+    lin_level* L = &lev[l];
+    if (l == 0)
+      L->g = *g;
+    else
+      mlr_coarsen_grid(g, 1 << l, &L->g);
+    const long nc = ncomp * mlr_ncell(&L->g);
+    L->e = (double*)calloc((size_t)nc, sizeof(double));
+    L->r = (double*)calloc((size_t)nc, sizeof(double));
+    L->lam = (double*)calloc((size_t)nc, sizeof(double));
+    if (l == 0) {
+      L->a = (double*)a;
+      L->c = (double*)c;
+      for (int d = 0; d < 3; ++d) L->b[d] = d < g->dim ? (double*)b[d] : NULL;
+    } else {
+      L->a = a ? (double*)calloc((size_t)nc, sizeof(double)) : NULL;
+      L->c = c ? (double*)calloc((size_t)nc, sizeof(double)) : NULL;
+      if (a) mlr_average_cells(g, ncomp, a, 1 << l, L->a);
+      if (c) mlr_average_cells(g, ncomp, c, 1 << l, L->c);
+      for (int d = 0; d < 3; ++d)
+        L->b[d] = d < g->dim ? (double*)calloc((size_t)(ncomp * mlr_nface(&L->g, d)), sizeof(double)) : NULL;
+      mlr_average_faces(g, ncomp, b, 1 << l, L->b);
+    }
+    mlr_lambda(&L->g, ncomp, alpha, L->a, beta, (const double* const*)L->b, L->c, L->lam);
+  }
+  lin_level* F = &lev[0];
+  const long n0 = ncomp * mlr_ncell(g);
+  mlr_lin_residual(g, ncomp, bc, alpha, a, beta, c, b, phi, rhs, F->r, 0);
+  const double r0 = mlr_norm_max(g, ncomp, F->r);
+  double norm = r0, prev = r0;
+  int iter = 0, status = 0;
+  if (r0 <= o->norm_thresh) status = 8;
+  while (status == 0 && iter < o->max_iter) {
+    memset(F->e, 0, sizeof(double) * n0);
+    lin_vcycle(lev, 0, nlev, ncomp, bc, alpha, beta, prolong_linear, o);
+    mlr_lay L;
+    mlr_layout(g, &L);
+    for (int n = 0; n < ncomp; ++n)
+      for (int k = 0; k < L.nz; ++k)
+        for (int j = 0; j < g->n[1]; ++j)
+          for (int i = 0; i < g->n[0]; ++i) {
+            const long q = mlr_ci(&L, i, j, k) + n * L.ncell;
+            phi[q] = phi[q] + F->e[q];
+          }
+    ++iter;
+    mlr_lin_residual(g, ncomp, bc, alpha, a, beta, c, b, phi, rhs, F->r, 0);
+    norm = mlr_norm_max(g, ncomp, F->r);
+    if (norm <= o->eps * r0) status |= 1;
+    if (norm <= o->norm_thresh) status |= 8;
+    if (status == 0 && norm > (1.0 - o->hang) * prev) status |= 4;
+    prev = norm;
+  }
+  if (status == 0) status = 2;
+  if (st) {
+    st->exit_status = status;
+    st->iterations = iter;
+    st->initial_residual = r0;
+    st->final_residual = norm;
+  }
+  for (int l = 0; l < nlev; ++l) {
+    lin_level* L = &lev[l];
+    free(L->e);
+    free(L->r);
+    free(L->lam);
+    if (l > 0) {
+      free(L->a);
+      free(L->c);
+      for (int d = 0; d < 3; ++d) free(L->b[d]);
+    }
+  }
+  free(lev);
+  return status;
+}
