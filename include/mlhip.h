@@ -51,6 +51,7 @@ enum { ML_PRECISION_FP64 = 0, ML_PRECISION_FP32_REFINED = 1 };
 
 typedef struct ml_ctx_s* ml_ctx;
 typedef struct ml_hc_op_s* ml_hc_op;
+typedef struct ml_vc_op_s* ml_vc_op;
 
 /* Physical parameters (MushyLayerParams, src/MushyLayerParams.h;
  * nondimensional defaults src/MushyLayerParams.cpp:266-270,:340-365,:476-484,
@@ -180,6 +181,50 @@ int ml_hc_solve(ml_hc_op op, double* phi, const double* rhs, const ml_solve_opts
 int ml_hc_solve_be(ml_hc_op op, double* phi_new, const double* phi_old, const double* src,
                    double dt, const ml_solve_opts* opts, ml_solve_stats* stats);
 
+/* ---- linear variable-coefficient operator: DarcyBrinkmanOp and
+ * AMRProjectionOp --------------------------------------------------------
+ * L u = alpha*aCoef*u - beta*cCoef*u - beta*div(bCoef grad u), one
+ * component.  Darcy-Brinkman U* solve (one call per velocity component):
+ * aCoef = 1, bCoef = -Pr, cCoef = (Pr/Da) chi/Pi, backward Euler alpha = 1,
+ * beta = -dt (srcSubcycle/AMRLevelMushyLayerInit.cpp:579-747).  Projection:
+ * cCoef = NULL, alpha = 0, beta = 1, bCoef = -S_face (srcSubcycle/
+ * Projector.cpp:2850-2991).  The BC uses component 0 of `bc` and is always
+ * applied; MG prolongation is piecewise constant. */
+/* DarcyBrinkmanOpFactory::define + MGnewOp for every depth
+ * (src/DarcyBrinkmanOp.cpp:801-1017) */
+int ml_vc_op_create(ml_ctx ctx, int dim, const int lo[], const int hi[], double dx,
+                    const int periodic[], const ml_bc* bc, ml_vc_op* out);
+int ml_vc_op_destroy(ml_vc_op op);
+int ml_vc_num_levels(ml_vc_op op);
+/* DarcyBrinkmanOp::setCoefs (src/DarcyBrinkmanOp.cpp:250-263) plus the MG
+ * coarsening of aCoef, bCoef, cCoef to every depth with arithmetic averaging
+ * (src/DarcyBrinkmanOp.cpp:977-1008).  aCoef NULL: aCoef == 1; cCoef NULL:
+ * no c term (the projection's VCAMRPoissonOp2 form).  Copied. */
+int ml_vc_set_coefs(ml_vc_op op, const double* aCoef, const double* const bCoef[],
+                    const double* cCoef, double alpha, double beta);
+/* setAlphaAndBeta (src/DarcyBrinkmanOp.cpp:242-248) */
+int ml_vc_set_alpha_beta(ml_vc_op op, double alpha, double beta);
+/* relax -> levelGSRB x iters, homogeneous BCs (src/DarcyBrinkmanOp.cpp:503-576) */
+int ml_vc_relax(ml_vc_op op, int depth, double* phi, const double* rhs, int iters);
+/* residualI (src/DarcyBrinkmanOp.cpp:31-83): res = rhs - L(phi) */
+int ml_vc_residual(ml_vc_op op, int depth, double* res, double* phi, const double* rhs,
+                   int homogeneous);
+/* applyOpI (src/DarcyBrinkmanOp.cpp:123-137): lhs = L(phi) */
+int ml_vc_apply_op(ml_vc_op op, int depth, double* lhs, double* phi, int homogeneous);
+/* restrictResidual (src/DarcyBrinkmanOp.cpp:183-240), homogeneous BCs */
+int ml_vc_restrict_residual(ml_vc_op op, int depth, double* resCoarse, double* phiFine,
+                            const double* rhsFine);
+/* prolongIncrement, piecewise constant (AMRProjectionOp::prolongIncrement,
+ * src/AMRProjectionOp.cpp:67-98) */
+int ml_vc_prolong_increment(ml_vc_op op, int depth, double* phiFine, const double* corrCoarse);
+/* AMRMultiGrid::solve on one level with this operator (Chombo driver;
+ * build-owned specification); FP64 only */
+int ml_vc_solve(ml_vc_op op, double* phi, const double* rhs, const ml_solve_opts* opts,
+                ml_solve_stats* stats);
+/* backward-Euler U* step (srcSubcycle/AMRLevelMushyLayerVel.cpp:1326-1403):
+ * alpha = 1, beta = -dt, rhs = aCoef*phi_old + dt*src (src may be NULL) */
+int ml_vc_solve_be(ml_vc_op op, double* phi_new, const double* phi_old, const double* src,
+                   double dt, const ml_solve_opts* opts, ml_solve_stats* stats);
 /* ---- state update and coefficient assembly ----------------------------- */
 /* porosity of every valid cell (CALCULATEPOROSITY_MULTICOMP,
  * src/EnthalpyVariablesF.ChF:56-94, as in updateEnthalpyVariables

@@ -89,6 +89,19 @@ SIGNATURES = {
     "ml_hc_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "ml_hc_solve_be": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.POINTER(SolveOpts),
                                  C.POINTER(SolveStats)]),
+    "ml_vc_op_create": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_double, _ip, C.POINTER(BC), C.POINTER(_vp)]),
+    "ml_vc_op_destroy": (C.c_int, [_vp]),
+    "ml_vc_num_levels": (C.c_int, [_vp]),
+    "ml_vc_set_coefs": (C.c_int, [_vp, _vp, C.POINTER(_vp), _vp, C.c_double, C.c_double]),
+    "ml_vc_set_alpha_beta": (C.c_int, [_vp, C.c_double, C.c_double]),
+    "ml_vc_relax": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
+    "ml_vc_residual": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int]),
+    "ml_vc_apply_op": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
+    "ml_vc_restrict_residual": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
+    "ml_vc_prolong_increment": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "ml_vc_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "ml_vc_solve_be": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.POINTER(SolveOpts),
+                                 C.POINTER(SolveStats)]),
     "ml_hc_porosity": (C.c_int, [_vp, _vp, _vp]),
     "ml_hc_assemble_bcoef": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
     "ml_norm_max": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _vp, _dp]),
@@ -348,3 +361,70 @@ class HCOp:
         if self.h:
             self.ctx.lib.ml_hc_op_destroy(self.h)
             self.h = None
+
+
+class VCOp:
+    """DarcyBrinkmanOp / AMRProjectionOp (+ the factory's MG hierarchy) on the
+    GPU: L u = alpha*a*u - beta*c*u - beta*div(b grad u), one component."""
+
+    def __init__(self, ctx, n, dx, periodic, bc):
+        self.ctx = ctx
+        self.n = list(n)
+        self.dim = len(n)
+        self.dx = dx
+        h = _vp()
+        lo = _iarr([0] * 3)
+        hi = _iarr([x - 1 for x in n])
+        ctx.check(ctx.lib.ml_vc_op_create(ctx.h, self.dim, lo, hi, dx, _iarr(periodic), C.byref(bc), C.byref(h)),
+                  "ml_vc_op_create")
+        self.h = h
+        self.nlev = ctx.lib.ml_vc_num_levels(h)
+
+    def set_coefs(self, a, b, c, alpha, beta):
+        arr = (_vp * 3)()
+        for d in range(self.dim):
+            arr[d] = b[d].ptr
+        self.ctx.check(self.ctx.lib.ml_vc_set_coefs(self.h, a.ptr if a is not None else None, arr,
+                                                    c.ptr if c is not None else None, alpha, beta),
+                       "ml_vc_set_coefs")
+
+    def set_alpha_beta(self, alpha, beta):
+        self.ctx.check(self.ctx.lib.ml_vc_set_alpha_beta(self.h, alpha, beta), "ml_vc_set_alpha_beta")
+
+    def relax(self, depth, phi, rhs, iters):
+        self.ctx.check(self.ctx.lib.ml_vc_relax(self.h, depth, phi.ptr, rhs.ptr, iters), "ml_vc_relax")
+
+    def residual(self, depth, res, phi, rhs, homogeneous=0):
+        self.ctx.check(self.ctx.lib.ml_vc_residual(self.h, depth, res.ptr, phi.ptr, rhs.ptr, homogeneous),
+                       "ml_vc_residual")
+
+    def apply_op(self, depth, lhs, phi, homogeneous=0):
+        self.ctx.check(self.ctx.lib.ml_vc_apply_op(self.h, depth, lhs.ptr, phi.ptr, homogeneous), "ml_vc_apply_op")
+
+    def restrict_residual(self, depth, resc, phi, rhs):
+        self.ctx.check(self.ctx.lib.ml_vc_restrict_residual(self.h, depth, resc.ptr, phi.ptr, rhs.ptr),
+                       "ml_vc_restrict_residual")
+
+    def prolong_increment(self, depth, phif, corrc):
+        self.ctx.check(self.ctx.lib.ml_vc_prolong_increment(self.h, depth, phif.ptr, corrc.ptr),
+                       "ml_vc_prolong_increment")
+
+    def solve(self, phi, rhs, opts=None):
+        st = SolveStats()
+        o = opts or default_opts()
+        self.ctx.check(self.ctx.lib.ml_vc_solve(self.h, phi.ptr, rhs.ptr, C.byref(o), C.byref(st)), "ml_vc_solve")
+        return st
+
+    def solve_be(self, phi_new, phi_old, dt, src=None, opts=None):
+        st = SolveStats()
+        o = opts or default_opts()
+        self.ctx.check(self.ctx.lib.ml_vc_solve_be(self.h, phi_new.ptr, phi_old.ptr,
+                                                   src.ptr if src is not None else None, dt, C.byref(o),
+                                                   C.byref(st)), "ml_vc_solve_be")
+        return st
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.ml_vc_op_destroy(self.h)
+            self.h = None
+

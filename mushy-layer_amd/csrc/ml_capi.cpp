@@ -24,6 +24,16 @@ int fail(ml_ctx ctx, int code, const std::string& msg) {
   return code;
 }
 
+bool valid_grid(int dim, const int n[], const int periodic[], double dx) {
+  if (dim != 2 && dim != 3) return false;
+  if (!(dx > 0)) return false;
+  for (int d = 0; d < dim; ++d) {
+    if (n[d] <= 0) return false;
+    if (periodic && periodic[d] != 0 && periodic[d] != 1) return false;
+  }
+  return true;
+}
+
 struct HostDriver {
   std::vector<std::unique_ptr<mlhost::HipNonLinearMultiCompOp>> owned;
   std::vector<mlhost::HipNonLinearMultiCompOp*> ops;
@@ -36,16 +46,7 @@ struct HostDriver {
 namespace {
 
 using ml::fail;
-
-bool valid_grid(int dim, const int n[], const int periodic[], double dx) {
-  if (dim != 2 && dim != 3) return false;
-  if (!(dx > 0)) return false;
-  for (int d = 0; d < dim; ++d) {
-    if (n[d] <= 0) return false;
-    if (periodic && periodic[d] != 0 && periodic[d] != 1) return false;
-  }
-  return true;
-}
+using ml::valid_grid;
 
 void delete_host_driver(ml::HostDriver* p) { delete p; }
 void delete_engine(ml::HCEngine* p) { delete p; }

@@ -25,6 +25,7 @@ struct ml_ctx_s {
 namespace ml {
 
 int fail(ml_ctx ctx, int code, const std::string& msg);
+bool valid_grid(int dim, const int n[], const int periodic[], double dx);
 
 #define ML_HIP(ctx, expr)                                                                     \
   do {                                                                                        \
@@ -127,3 +128,46 @@ class HCEngine {
 };
 
 }  // namespace ml
+
+// ---- linear variable-coefficient operator (DarcyBrinkmanOp / AMRProjectionOp)
+namespace ml {
+struct VCLevelStore {
+  Geom g;
+  double* a = nullptr;
+  double* c = nullptr;
+  double* b[3] = {nullptr, nullptr, nullptr};
+  double* e[2] = {nullptr, nullptr};   // correction and its ping-pong partner
+  double* r = nullptr;                 // residual right-hand side
+  double* tmp = nullptr;               // partner for the per-level relax entry point (lazy)
+};
+}  // namespace ml
+
+struct ml_vc_op_s {
+  ml_ctx ctx = nullptr;
+  int dim = 2;
+  ml::BCDesc bc{};
+  double alpha = 1.0, beta = 1.0;   // factory defaults (src/DarcyBrinkmanOp.cpp:876-882)
+  bool has_a = false, has_c = false;
+  std::vector<ml::VCLevelStore> lev;
+  int lc = 0;                       // first level handled by the single-workgroup kernel
+  double* alt0 = nullptr;           // level-0 state partner of the solve
+  double* rhs_be = nullptr;
+  unsigned long long* slots = nullptr;
+  unsigned long long* hslots = nullptr;
+  int nslots = 0;
+  ml::VCDesc<double>* desc = nullptr;    // device
+  ml::VCDesc<double>* hdesc = nullptr;   // pinned staging
+
+  ml::VCArgs<double> args(int depth) const {
+    ml::VCArgs<double> A;
+    A.g = lev[depth].g;
+    A.bc = bc;
+    A.alpha = alpha;
+    A.beta = beta;
+    A.a = has_a ? lev[depth].a : nullptr;
+    A.c = has_c ? lev[depth].c : nullptr;
+    for (int d = 0; d < 3; ++d) A.b[d] = lev[depth].b[d];
+    return A;
+  }
+};
+

@@ -219,6 +219,70 @@ hipError_t be_rhs(const Geom& g, int ncomp, T* y, const T* x, const T* acoef, T 
 }  // namespace ml
 
 namespace ml {
+
+// ---- linear variable-coefficient operator (ml_vc_kernels.hip)
+// L u = alpha*a*u - beta*c*u - beta*div(b grad u), one component; a, c
+// nullable (a == 1, no c term); BC component 0 of bc, always applied.
+template <typename T>
+struct VCArgs {
+  Geom g;
+  BCDesc bc;
+  T alpha, beta;
+  const T* a;
+  const T* c;
+  const T* b[3];
+};
+
+template <typename T>
+struct VCTileIO {
+  const T* phi;
+  const T* e;          // residual: u = phi + e (nullable)
+  T* phi_out;          // ... written here when e != nullptr
+  const T* rhs;
+  T* out;
+  unsigned long long* norm;
+  Geom gc;
+  int homog, ntx, nty;
+};
+
+template <typename T>
+struct VCLevel {
+  Geom g;
+  const T* a;
+  const T* c;
+  const T* b[3];
+  T* e;   // correction
+  T* r;   // residual right-hand side
+};
+
+template <typename T>
+struct VCDesc {
+  int l0, nlev, nu1, nu2, bottom;
+  BCDesc bc;
+  T alpha, beta;
+  VCLevel<T> lev[kMaxLevels];
+};
+
+// red/black relaxation of L e = r with homogeneous BCs, ping-ponging between
+// buf[0] and buf[1] (*cur indexes the current state); zero_init starts from
+// e = 0; pc (level gc) is added by piecewise-constant prolongation first
+template <typename T>
+hipError_t vc_relax(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
+                    const Geom* gc, const T* pc, hipStream_t s);
+// res = rhs - L(phi [+ e]) (res nullable), max-norm into *norm (nullable);
+// with e the updated state phi + e is written to phi_out (!= phi)
+template <typename T>
+hipError_t vc_residual(const VCArgs<T>& A, const T* phi, const T* e, T* phi_out, const T* rhs, T* res,
+                       unsigned long long* norm, int homog, hipStream_t s);
+template <typename T>
+hipError_t vc_apply_op(const VCArgs<T>& A, const T* phi, T* lhs, int homog, hipStream_t s);
+template <typename T>
+hipError_t vc_restrict_residual(const VCArgs<T>& A, const Geom& gc, const T* phi, const T* rhs, T* resc,
+                                hipStream_t s);
+template <typename T>
+hipError_t vc_coarse_vcycle(const VCDesc<T>* dev_desc, int dim, hipStream_t s);
+bool vc_small_level(const Geom& g);
+
 hipError_t vec_axby(long long n, double* lhs, const double* x, const double* y, double a, double b, hipStream_t s);
 hipError_t vec_incr(long long n, double* lhs, const double* x, double sc, hipStream_t s);
 hipError_t vec_mult(long long n, double* lhs, const double* x, hipStream_t s);

@@ -133,3 +133,59 @@ def mushy(n=(256, 256), seed=2, St=5.0, CR=2.0, lewis=200.0, corr=0.1, theta_bot
     periodic = [1] * (dim - 1) + [0]
     return dict(n=n, dim=dim, dx=1.0 / n[0], periodic=periodic, bc=bc, phi=phi, dt=dt, steps=steps,
                 params=dict(stefan=St, compositionRatio=CR, lewis=lewis), chi=chi)
+
+
+def face_shape(n, d, ncomp):
+    m = list(n) + [1] * (3 - len(n))
+    m[d] += 1
+    return (ncomp, m[2], m[1], m[0])
+
+
+def geometric_faces(n, periodic, chi):
+    """Face porosity sqrt(|chi_lo chi_hi|) (src/CellToEdge2F.ChF:20-46) from a
+    valid-cell field (shape reversed(n)); wall faces use the interior cell."""
+    dim = len(n)
+    out = []
+    for d in range(dim):
+        ax = dim - 1 - d
+        if periodic[d]:
+            lo = np.concatenate([np.take(chi, [-1], axis=ax), chi], axis=ax)
+            hi = np.concatenate([chi, np.take(chi, [0], axis=ax)], axis=ax)
+        else:
+            lo = np.concatenate([np.take(chi, [0], axis=ax), chi], axis=ax)
+            hi = np.concatenate([chi, np.take(chi, [-1], axis=ax)], axis=ax)
+        f = np.sqrt(np.abs(lo * hi))
+        out.append(np.ascontiguousarray(f.reshape(face_shape(n, d, 1))))
+    return out
+
+
+def darcy_brinkman(n=(256, 256), seed=2, Pr=10.0, Da=1e-3, corr=0.1, dt=1e-3, steps=20, Ra_C=200.0):
+    """Coefficients of the Darcy-Brinkman U* solve and of the projection on a
+    porosity field from the same seeded random field as `mushy`:
+      U*:         a = 1, b = -Pr, c = (Pr/Da) chi/Pi, Pi = chi^3/(1-chi)^2
+                  (Kozeny-Carman, src/MushyLayerParams.cpp:1046-1049;
+                  srcSubcycle/AMRLevelMushyLayerInit.cpp:665-689), no-slip
+                  (homogeneous Dirichlet) walls in the last direction;
+      projection: b = -chi_face, homogeneous Neumann walls.
+    Periodic sideways.  `u0` is a smooth seeded initial velocity component
+    and `src` a buoyancy-like source Ra_C*(chi - mean chi)."""
+    n = list(n)
+    dim = len(n)
+    chi = gaussian_field(n, corr, seed)
+    with np.errstate(divide="ignore"):
+        perm = chi ** 3 / (1.0 - chi) ** 2
+        cval = np.where(np.isinf(perm), 0.0, chi / np.where(perm == 0, 1.0, perm)) * (Pr / Da)
+    c = np.zeros(cell_shape(n, 1))
+    valid(n, c)[0] = cval.reshape(valid(n, c)[0].shape)
+    b_u = [np.full(face_shape(n, d, 1), -Pr) for d in range(dim)]
+    periodic = [1] * (dim - 1) + [0]
+    b_p = [-f for f in geometric_faces(n, periodic, chi)]
+    u0 = np.zeros(cell_shape(n, 1))
+    valid(n, u0)[0] = (0.1 * gaussian_field(n, corr, seed + 2, -1.0, 1.0)).reshape(valid(n, u0)[0].shape)
+    src = np.zeros(cell_shape(n, 1))
+    valid(n, src)[0] = (Ra_C * (chi - chi.mean())).reshape(valid(n, src)[0].shape)
+    vd = dim - 1
+    bc_u = {(vd, 0, 0): (DIRICHLET, 0.0), (vd, 1, 0): (DIRICHLET, 0.0)}
+    bc_p = {(vd, 0, 0): (NEUMANN, 0.0), (vd, 1, 0): (NEUMANN, 0.0)}
+    return dict(n=n, dim=dim, dx=1.0 / n[0], periodic=periodic, chi=chi, c=c, b_u=b_u, b_p=b_p, u0=u0, src=src,
+                bc_u=bc_u, bc_p=bc_p, dt=dt, steps=steps, Pr=Pr, Da=Da)
