@@ -225,6 +225,14 @@ int ml_vc_solve(ml_vc_op op, double* phi, const double* rhs, const ml_solve_opts
  * alpha = 1, beta = -dt, rhs = aCoef*phi_old + dt*src (src may be NULL) */
 int ml_vc_solve_be(ml_vc_op op, double* phi_new, const double* phi_old, const double* src,
                    double dt, const ml_solve_opts* opts, ml_solve_stats* stats);
+/* U* Darcy coefficient on valid cells: cCoef = (chi / Pi(chi)) * darcy_coeff,
+ * Kozeny-Carman Pi = chi^3/(1-chi)^2 (srcSubcycle/AMRLevelMushyLayerInit.cpp
+ * :665-689, src/MushyLayerParams.cpp:1046-1049); darcy_coeff = Pr/Da */
+int ml_vc_darcy_ccoef(ml_vc_op op, const double* chi, double darcy_coeff, double* cCoef);
+/* bCoef[d] = scale * sqrt(|chi_lo chi_hi|) on every face (geometric face
+ * porosity, src/CellToEdge2F.ChF:20-46); scale = -1 gives the projection's
+ * -S_face with S = chi (srcSubcycle/AMRLevelMushyLayerInit.cpp:1141) */
+int ml_vc_porosity_faces(ml_vc_op op, const double* chi, double scale, double* const bCoef[]);
 /* ---- state update and coefficient assembly ----------------------------- */
 /* porosity of every valid cell (CALCULATEPOROSITY_MULTICOMP,
  * src/EnthalpyVariablesF.ChF:56-94, as in updateEnthalpyVariables

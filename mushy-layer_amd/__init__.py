@@ -102,6 +102,8 @@ SIGNATURES = {
     "ml_vc_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "ml_vc_solve_be": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.POINTER(SolveOpts),
                                  C.POINTER(SolveStats)]),
+    "ml_vc_darcy_ccoef": (C.c_int, [_vp, _vp, C.c_double, _vp]),
+    "ml_vc_porosity_faces": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(_vp)]),
     "ml_hc_porosity": (C.c_int, [_vp, _vp, _vp]),
     "ml_hc_assemble_bcoef": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
     "ml_norm_max": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _vp, _dp]),
@@ -408,6 +410,15 @@ class VCOp:
     def prolong_increment(self, depth, phif, corrc):
         self.ctx.check(self.ctx.lib.ml_vc_prolong_increment(self.h, depth, phif.ptr, corrc.ptr),
                        "ml_vc_prolong_increment")
+
+    def darcy_ccoef(self, chi, coeff, c):
+        self.ctx.check(self.ctx.lib.ml_vc_darcy_ccoef(self.h, chi.ptr, coeff, c.ptr), "ml_vc_darcy_ccoef")
+
+    def porosity_faces(self, chi, scale, b):
+        arr = (_vp * 3)()
+        for d in range(self.dim):
+            arr[d] = b[d].ptr
+        self.ctx.check(self.ctx.lib.ml_vc_porosity_faces(self.h, chi.ptr, scale, arr), "ml_vc_porosity_faces")
 
     def solve(self, phi, rhs, opts=None):
         st = SolveStats()

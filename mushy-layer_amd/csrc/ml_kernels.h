@@ -215,6 +215,8 @@ hipError_t hc_bcoef(const Geom& g, double k, double DH, double DC, const double*
                     hipStream_t s);
 template <typename T>
 hipError_t be_rhs(const Geom& g, int ncomp, T* y, const T* x, const T* acoef, T dt, const T* src, hipStream_t s);
+hipError_t darcy_ccoef(const Geom& g, double coeff, const double* chi, double* c, hipStream_t s);
+hipError_t porosity_faces(const Geom& g, double scale, const double* chi, double* const* b, hipStream_t s);
 
 }  // namespace ml
 

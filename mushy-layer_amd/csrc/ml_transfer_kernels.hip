@@ -190,6 +190,47 @@ __global__ void hc_bcoef_kernel(Geom g, int d, double kcond, double DH, double D
   }
 }
 
+// Darcy coefficient of the U* solve: c = (chi / Pi(chi)) * coeff with the
+// Kozeny-Carman permeability Pi = chi^3/(1-chi)^2
+// (srcSubcycle/AMRLevelMushyLayerInit.cpp:665-689: copy porosity, divide by
+// permeability, multiply by m_darcyCoeff; src/MushyLayerParams.cpp:1046-1049).
+__global__ void darcy_ccoef_kernel(Geom g, double coeff, const double* __restrict__ chi, double* __restrict__ c) {
+  const long long N = g.nvalid();
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N;
+       t += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    unflatten(t, g.n[0], g.n[1], i, j, k);
+    const long long q = g.ci(i, j, k);
+    const double liquid = chi[q], solid = 1.0 - liquid;
+    const double perm = pow(liquid, 3.0) / pow(solid, 2.0);
+    double v = liquid;
+    v = v / perm;
+    c[q] = v * coeff;
+  }
+}
+
+// scale * sqrt(|chi_lo chi_hi|) on the faces of direction d
+// (CELLTOEDGEGEOMETRIC, src/CellToEdge2F.ChF:20-46); wall faces use the
+// interior cell.
+template <int DIM>
+__global__ void porosity_faces_kernel(Geom g, int d, double scale, const double* __restrict__ chi,
+                                      double* __restrict__ b) {
+  const long long N = g.nface[d];
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N;
+       c += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    unflatten(c, int(g.fn[d][0]), int(g.fn[d][1]), i, j, k);
+    const int li = wrap_clamp(i - (d == 0), g.n[0], g.periodic[0]);
+    const int lj = wrap_clamp(j - (d == 1), g.n[1], g.periodic[1]);
+    const int lk = DIM == 3 ? wrap_clamp(k - (d == 2), g.n[2], g.periodic[2]) : 0;
+    const int hi = wrap_clamp(i, g.n[0], g.periodic[0]);
+    const int hj = wrap_clamp(j, g.n[1], g.periodic[1]);
+    const int hk = DIM == 3 ? wrap_clamp(k, g.n[2], g.periodic[2]) : 0;
+    const double lo = chi[g.ci(li, lj, lk)], up = chi[g.ci(hi, hj, hk)];
+    b[c] = scale * sqrt(fabs(lo * up));
+  }
+}
+
 // y = x*a + b*z over valid cells (z may be null): backward-Euler rhs
 // a*phi_old + dt*src with a == 1 when the aCoef array is absent.
 template <typename T>
@@ -290,6 +331,22 @@ hipError_t hc_bcoef(const Geom& g, double k, double DH, double DC, const double*
       hipLaunchKernelGGL((hc_bcoef_kernel<2>), dim3(blocks), dim3(256), 0, s, g, d, k, DH, DC, chi, b[d]);
     else
       hipLaunchKernelGGL((hc_bcoef_kernel<3>), dim3(blocks), dim3(256), 0, s, g, d, k, DH, DC, chi, b[d]);
+  }
+  return hipGetLastError();
+}
+
+hipError_t darcy_ccoef(const Geom& g, double coeff, const double* chi, double* c, hipStream_t s) {
+  hipLaunchKernelGGL(darcy_ccoef_kernel, dim3(grid_for(g.nvalid())), dim3(256), 0, s, g, coeff, chi, c);
+  return hipGetLastError();
+}
+
+hipError_t porosity_faces(const Geom& g, double scale, const double* chi, double* const* b, hipStream_t s) {
+  for (int d = 0; d < g.dim; ++d) {
+    const int blocks = grid_for(g.nface[d]);
+    if (g.dim == 2)
+      hipLaunchKernelGGL((porosity_faces_kernel<2>), dim3(blocks), dim3(256), 0, s, g, d, scale, chi, b[d]);
+    else
+      hipLaunchKernelGGL((porosity_faces_kernel<3>), dim3(blocks), dim3(256), 0, s, g, d, scale, chi, b[d]);
   }
   return hipGetLastError();
 }

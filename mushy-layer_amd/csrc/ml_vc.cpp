@@ -357,4 +357,18 @@ int ml_vc_solve_be(ml_vc_op op, double* phi_new, const double* phi_old, const do
   return ml_vc_solve(op, phi_new, op->rhs_be, opts, stats);
 }
 
+int ml_vc_darcy_ccoef(ml_vc_op op, const double* chi, double darcy_coeff, double* cCoef) {
+  if (!op || !chi || !cCoef) return ML_EINVAL;
+  ML_HIP(op->ctx, ml::darcy_ccoef(op->lev[0].g, darcy_coeff, chi, cCoef, op->ctx->stream));
+  return ML_OK;
+}
+
+int ml_vc_porosity_faces(ml_vc_op op, const double* chi, double scale, double* const bCoef[]) {
+  if (!op || !chi || !bCoef) return ML_EINVAL;
+  for (int d = 0; d < op->dim; ++d)
+    if (!bCoef[d]) return ML_EINVAL;
+  ML_HIP(op->ctx, ml::porosity_faces(op->lev[0].g, scale, chi, bCoef, op->ctx->stream));
+  return ML_OK;
+}
+
 }  // extern "C"

@@ -143,3 +143,25 @@ def test_db_be_step_parity(ml, gpu_ctx, name):
     st = op.solve_be(dnew, du0, dt, src=dsrc, opts=ml.default_opts(**kw))
     assert (st.iterations, st.exit_status) == (st_ref.iterations, st_ref.exit_status)
     _check(n, dnew.get(), ref)
+
+
+@pytest.mark.parametrize("n", [(64, 64), (24, 16, 8)])
+def test_darcy_and_projection_coefficients(ml, gpu_ctx, n):
+    """cCoef = (chi/Pi) Pr/Da with Kozeny-Carman Pi, and the projection's
+    -sqrt(|chi_lo chi_hi|) faces, from a device porosity field.  pow() of the
+    device math library may differ from the host's in the last bit, so cCoef
+    is held to 1e-14 relative; the faces are bit-identical."""
+    w = wlm.darcy_brinkman(n=n, seed=3, corr=0.15)
+    chi = np.zeros(O.cell_shape(O.make_grid(n), 1))
+    O.valid(O.make_grid(n), chi)[0] = w["chi"].reshape(O.valid(O.make_grid(n), chi)[0].shape)
+    op = ml.VCOp(gpu_ctx, n, w["dx"], w["periodic"], ml.make_bc(w["bc_p"]))
+    dchi = ml.DeviceArray(gpu_ctx, chi.shape, host=chi)
+    dc = ml.DeviceArray(gpu_ctx, chi.shape)
+    op.darcy_ccoef(dchi, w["Pr"] / w["Da"], dc)
+    c = HC.valid(n, dc.get())
+    ref = HC.valid(n, w["c"])
+    assert np.max(np.abs(c - ref)) <= 1e-14 * np.max(np.abs(ref))
+    db = [ml.DeviceArray(gpu_ctx, x.shape) for x in w["b_p"]]
+    op.porosity_faces(dchi, -1.0, db)
+    for d in range(len(n)):
+        assert HC.bitwise_equal(db[d].get(), w["b_p"][d])
