@@ -78,6 +78,7 @@ __global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_kernel(HCArg
                                                                            double* __restrict__ phi_out,
                                                                            const double* __restrict__ rhs,
                                                                            const float* __restrict__ ecorr,
+                                                                           double omega,
                                                                            float* __restrict__ r32,
                                                                            float* __restrict__ M32,
                                                                            unsigned long long* norm, int ntx, int nty) {
@@ -100,8 +101,8 @@ __global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_kernel(HCArg
     if (r.kind != 1) continue;
     double h = phi[r.q], c = phi[r.q + g.ncell];
     if (ecorr) {
-      h = h + double(ecorr[r.q]);
-      c = c + double(ecorr[r.q + g.ncell]);
+      h = h + omega * double(ecorr[r.q]);
+      c = c + omega * double(ecorr[r.q + g.ncell]);
     }
     sphi[r.e] = h;
     sphi[E + r.e] = c;
@@ -646,7 +647,8 @@ size_t lin_extent(int halo) {
 }  // namespace
 
 hipError_t hc_residual_jacobian(const HCArgs<double>& A, const double* phi, double* phi_out, const double* rhs,
-                                const float* e, float* r32, float* M32, unsigned long long* norm, hipStream_t s) {
+                                const float* e, double omega, float* r32, float* M32, unsigned long long* norm,
+                                hipStream_t s) {
   if (e && (!phi_out || phi_out == phi)) return hipErrorInvalidValue;
   int ntx, nty;
   if (A.g.dim == 2) {
@@ -654,13 +656,13 @@ hipError_t hc_residual_jacobian(const HCArgs<double>& A, const double* phi, doub
     const int nt = lin_tiles<2>(A.g, ntx, nty);
     const size_t lds = 4 * lin_extent<2>(1) * sizeof(double);
     hipLaunchKernelGGL((hc_residual_jacobian_kernel<2, D::TX, D::TY, D::TZ>), dim3(nt), dim3(kLinThreads), lds, s, A,
-                       phi, phi_out, rhs, e, r32, M32, norm, ntx, nty);
+                       phi, phi_out, rhs, e, omega, r32, M32, norm, ntx, nty);
   } else {
     using D = LinTile<3>;
     const int nt = lin_tiles<3>(A.g, ntx, nty);
     const size_t lds = 4 * lin_extent<3>(1) * sizeof(double);
     hipLaunchKernelGGL((hc_residual_jacobian_kernel<3, D::TX, D::TY, D::TZ>), dim3(nt), dim3(kLinThreads), lds, s, A,
-                       phi, phi_out, rhs, e, r32, M32, norm, ntx, nty);
+                       phi, phi_out, rhs, e, omega, r32, M32, norm, ntx, nty);
   }
   return hipGetLastError();
 }

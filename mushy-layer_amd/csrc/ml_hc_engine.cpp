@@ -19,6 +19,7 @@ namespace ml {
 
 namespace {
 constexpr int kSmallCells = 1024;
+constexpr int kMaxHalvings = 6;   // FP32-refined step-length backtracking
 }
 
 HCEngine::HCEngine(ml_hc_op op) : m_op(op) { m_ok = init() == ML_OK; }
@@ -141,7 +142,7 @@ int HCEngine::solve_refined(double* phi, const double* rhs, const ml_solve_opts&
   ML_HIP(ctx, hipMemcpyAsync(m_alt[0], phi, bytes0, hipMemcpyDeviceToDevice, s));
   double* const pb[2] = {phi, m_alt[0]};
   int pc = 0;
-  ML_HIP(ctx, hc_residual_jacobian(A, phi, nullptr, rhs, nullptr, m_lin[0].r, m_lin[0].M, m_slots, s));
+  ML_HIP(ctx, hc_residual_jacobian(A, phi, nullptr, rhs, nullptr, 1.0, m_lin[0].r, m_lin[0].M, m_slots, s));
   double r0 = 0;
   int rc;
   if ((rc = read_norm(0, &r0))) return rc;
@@ -151,10 +152,20 @@ int HCEngine::solve_refined(double* phi, const double* rhs, const ml_solve_opts&
   while (status == 0 && iter < o.max_iter) {
     if ((rc = lin_vcycle(o))) return rc;
     ++iter;
+    // backtracking on the step length: phi + omega*e is accepted once its
+    // FP64 residual max-norm falls below the previous one (or after
+    // kMaxHalvings halvings); the trial overwrites only the partner array
     const float* e0 = m_ecur[0] ? m_ealt[0] : m_lin[0].e;
-    ML_HIP(ctx, hc_residual_jacobian(A, pb[pc], pb[1 - pc], rhs, e0, m_lin[0].r, m_lin[0].M, m_slots + iter, s));
+    double omega = 1.0;
+    for (int k = 0;; ++k) {
+      if (k > 0) ML_HIP(ctx, zero_u64(m_slots + iter, 1, s));
+      ML_HIP(ctx, hc_residual_jacobian(A, pb[pc], pb[1 - pc], rhs, e0, omega, m_lin[0].r, m_lin[0].M,
+                                       m_slots + iter, s));
+      if ((rc = read_norm(iter, &norm))) return rc;
+      if (norm < prev || k == kMaxHalvings) break;
+      omega *= 0.5;
+    }
     pc = 1 - pc;
-    if ((rc = read_norm(iter, &norm))) return rc;
     if (norm <= o.eps * r0) status |= 1;
     if (norm <= o.norm_thresh) status |= 8;
     if (status == 0 && norm > (1.0 - o.hang) * prev) status |= 4;

@@ -143,10 +143,12 @@ struct LinDesc {
   const double* a64;
 };
 
-// fine level: phi_out = phi + e (when e != nullptr; phi_out != phi), then
-// r32 = rhs - L(phi_out), M32 = closure Jacobian, max-norm of r into *norm
+// fine level: phi_out = phi + omega*e (when e != nullptr; phi_out != phi),
+// then r32 = rhs - L(phi_out), M32 = closure Jacobian, max-norm of r into
+// *norm
 hipError_t hc_residual_jacobian(const HCArgs<double>& A, const double* phi, double* phi_out, const double* rhs,
-                                const float* e, float* r32, float* M32, unsigned long long* norm, hipStream_t s);
+                                const float* e, double omega, float* r32, float* M32, unsigned long long* norm,
+                                hipStream_t s);
 // red/black block relaxation of J e = r on a tiled level, ping-ponging
 // between ebuf[0] and ebuf[1] (*cur indexes the current correction; L.e is
 // ignored); zero_init starts from e = 0; pc (coarse correction, level gc) is

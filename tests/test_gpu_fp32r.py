@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from oracle import mlref as O
+from tests import fp32r_model as FM
 from tests import hc_cases as HC
 
 pytestmark = pytest.mark.gpu
@@ -47,7 +48,7 @@ def _mixed(case, seed):
 CASES = {
     "diffusive_64": lambda wl: _diffusive(wl, (64, 64)),
     "diffusive_128": lambda wl: _diffusive(wl, (128, 128)),
-    "mixed_96x40": lambda wl: _mixed(_diffusive(wl, (96, 40)), 11),
+    "mixed_64x32_walls": lambda wl: _mixed(_diffusive(wl, (64, 32), periodic=[0, 0]), 15),
     "mixed_48x32_walls": lambda wl: _mixed(_diffusive(wl, (48, 32), periodic=[0, 0]), 12),
     "mushy_128": lambda wl: wl.mushy(n=(128, 128)),
     "mushy_256": lambda wl: wl.mushy(n=(256, 256)),
@@ -92,9 +93,14 @@ def test_refined_be_step_matches_fp64_oracle(ml, gpu_ctx, wl, name):
     # the FP64 residual of the initial state is the oracle's, to rounding
     assert abs(st.initial_residual - st_ref.initial_residual) <= 1e-12 * st_ref.initial_residual
     eH, eC, eX = _field_errors(case, p, g, out, ref)
-    print("%s: outer iterations %d (FAS V-cycles %d), err H %.3e Theta %.3e chi %.3e"
-          % (name, st.iterations, st_ref.iterations, eH, eC, eX))
+    # the same algorithm evaluated in FP64 on the host takes (about) as many
+    # outer iterations: FP32 smoothing does not slow the defect correction
+    model = old.copy()
+    _, it_model, _ = FM.solve(g, p, bc, 1.0, -dt, b, model, old.copy(), max_iter=100, eps=EPS, hang=-1.0)
+    print("%s: outer iterations %d (FP64 model %d, FAS V-cycles %d), err H %.3e Theta %.3e chi %.3e"
+          % (name, st.iterations, it_model, st_ref.iterations, eH, eC, eX))
     assert eH <= TOL_STATE and eC <= TOL_STATE and eX <= TOL_CHI, (eH, eC, eX)
+    assert st.iterations <= it_model + 2, (st.iterations, it_model)
 
 
 def test_refined_uses_fewer_outer_iterations(ml, gpu_ctx, wl):
