@@ -22,7 +22,7 @@ namespace ml {
 namespace {
 
 constexpr int kLinThreads = 256;
-constexpr int kWGT = 256;      // single-workgroup kernel threads
+constexpr int kWGT = 1024;     // single-workgroup kernel threads (one per cell)
 constexpr int kWGC = 1024;     // cells of the largest level it holds
 constexpr int kWGM = kWGC / kWGT;
 

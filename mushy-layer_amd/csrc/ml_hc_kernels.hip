@@ -17,9 +17,8 @@ namespace ml {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kWGThreads = 256;    // single-workgroup kernels: threads ...
-constexpr int kWGCells = 1024;     // ... and cells of the largest level they hold
-constexpr int kCPT = kWGCells / kWGThreads;
+constexpr int kWGThreads = 1024;   // single-workgroup kernel: one thread per cell ...
+constexpr int kWGCells = 1024;     // ... of the largest level it holds
 
 enum : int { kResid = 1, kApply = 2, kRestrict = 3 };
 
@@ -69,15 +68,13 @@ __device__ inline void tile_diffused_var(const HCArgs<T>& A, const Tile<DIM, TX,
 // ---------------------------------------------------------------------------
 // Red/black relaxation of one tile: `npass` colour passes with halo npass
 // staged in LDS (levelGSRB, src/AMRNonLinearMultiCompOp.cpp:815-949, and
-// NONLINEARSMOOTHING, src/AMRNonLinearMultiCompOpF.ChF:125-195: phi_l
-// recomputed before each pass, frozen within it).  Each thread owns fixed
-// LDS positions whose classification is computed once; the right-hand side
-// and face coefficients of the cells it updates are loaded into registers
-// before the closure of the pass, so their latency hides behind it; a wall
-// ghost's derived value is computed by its own thread from the interior
-// neighbour's state, so a pass needs two barriers.  When io.pc is set, the
-// staged state first receives the linear prolongation of the coarse
-// correction pc - pc0 (prolongIncrement fused with the post-smoother).
+// NONLINEARSMOOTHING, src/AMRNonLinearMultiCompOpF.ChF:125-195: phi_l of
+// every cell as of the start of the pass, frozen within it).  Each thread
+// owns fixed LDS positions whose classification is computed once; a wall
+// ghost's derived value comes from its interior neighbour's state.  When
+// io.pc is set, the staged state first receives the linear prolongation of
+// the coarse correction pc - pc0 (prolongIncrement fused with the
+// post-smoother).  Input and output arrays are distinct.
 template <typename T, int DIM, int TX, int TY, int TZ, int HMAX, int NT>
 __global__ __launch_bounds__(NT) void hc_relax_tile_kernel(HCArgs<T> A, TileIO<T> io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -114,53 +111,77 @@ __global__ __launch_bounds__(NT) void hc_relax_tile_kernel(HCArgs<T> A, TileIO<T
   }
   __syncthreads();
 
+  // closure of every staged cell and wall ghost (computeDiffusedVar,
+  // src/AMRNonLinearMultiCompOp.cpp:57-98); ghosts from their interior
+  // neighbour
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    if (r.kind == 0) continue;
+    const int es = r.kind == 1 ? r.e : r.eint;
+    T th, cl;
+    closure<T>(sphi[es], sphi[E + es], A.cp, th, cl);
+    if (r.kind == 2 && A.bc.apply) {
+      const int isign = r.gs == 0 ? -1 : 1;
+      th = bc_ghost<T>(A.bc.type[r.gd][r.gs][0], T(A.bc.val[r.gd][r.gs][0]), th, isign, T(g.dx));
+      cl = bc_ghost<T>(A.bc.type[r.gd][r.gs][1], T(A.bc.val[r.gd][r.gs][1]), cl, isign, T(g.dx));
+    }
+    sphl[r.e] = th;
+    sphl[E + r.e] = cl;
+  }
+  __syncthreads();
+
+  // Each pass updates one colour and recomputes the closure of exactly the
+  // cells it updated (and of their wall ghosts): the other colour's phi_l is
+  // unchanged, and in this pass a cell reads only its own and the other
+  // colour's phi_l, so one barrier per pass suffices.
   for (int p = 0; p < io.npass; ++p) {
     const int gext = t.H - p;
     const int colour = (io.first + p) & 1;
-    CellFaces<T, DIM> F[KMAX];
-    T rr[KMAX][2], aa[KMAX][2];
-    bool upd[KMAX];
 #pragma unroll
     for (int kk = 0; kk < KMAX; ++kk) {
       const PosRec& r = R[kk];
-      upd[kk] = r.kind == 1 && r.dist <= gext - 1 && !((r.i + r.j + r.k + colour) & 1);
-      if (!upd[kk]) continue;
-      load_faces<T, DIM>(A, r.i, r.j, r.k, F[kk]);
-      rr[kk][0] = io.rhs[r.q];
-      rr[kk][1] = io.rhs[r.q + g.ncell];
-      aa[kk][0] = A.a ? A.a[r.q] : T(1);
-      aa[kk][1] = A.a ? A.a[r.q + g.ncell] : T(1);
-    }
-    // closure of the staged cells; wall ghosts from their interior neighbour
-    // (computeDiffusedVar, src/AMRNonLinearMultiCompOp.cpp:57-98)
-#pragma unroll
-    for (int kk = 0; kk < KMAX; ++kk) {
-      const PosRec& r = R[kk];
-      if (r.kind == 0 || r.dist > gext) continue;
-      const int es = r.kind == 1 ? r.e : r.eint;
-      T th, cl;
-      closure<T>(sphi[es], sphi[E + es], A.cp, th, cl);
-      if (r.kind == 2 && A.bc.apply) {
-        const int isign = r.gs == 0 ? -1 : 1;
-        th = bc_ghost<T>(A.bc.type[r.gd][r.gs][0], T(A.bc.val[r.gd][r.gs][0]), th, isign, T(g.dx));
-        cl = bc_ghost<T>(A.bc.type[r.gd][r.gs][1], T(A.bc.val[r.gd][r.gs][1]), cl, isign, T(g.dx));
-      }
-      sphl[r.e] = th;
-      sphl[E + r.e] = cl;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < KMAX; ++kk) {
-      if (!upd[kk]) continue;
-      const int e = R[kk].e;
+      if (!(r.kind == 1 && r.dist <= gext - 1 && !((r.i + r.j + r.k + colour) & 1))) continue;
+      CellFaces<T, DIM> F;
+      load_faces<T, DIM>(A, r.i, r.j, r.k, F);
+      const int e = r.e;
+      T nv[2];
 #pragma unroll
       for (int n = 0; n < 2; ++n) {
+        const T rr = io.rhs[r.q + n * g.ncell];
+        const T aa = A.a ? A.a[r.q + n * g.ncell] : T(1);
         T st[7];
         gather_lds<T, DIM>(sphl + n * E, e, t.EX, t.EXY, st);
         const T ph = sphi[n * E + e];
-        const T lof = lof_of<T, DIM>(F[kk], n, st, aa[kk][n], ph, A.alpha, A.beta, dxinv);
-        const T lam = lambda_of<T, DIM>(F[kk], n, aa[kk][n], A.alpha, A.beta, scale);
-        sphi[n * E + e] = ph - lam * (lof - rr[kk][n]);
+        const T lof = lof_of<T, DIM>(F, n, st, aa, ph, A.alpha, A.beta, dxinv);
+        const T lam = lambda_of<T, DIM>(F, n, aa, A.alpha, A.beta, scale);
+        nv[n] = ph - lam * (lof - rr);
+      }
+      sphi[e] = nv[0];
+      sphi[E + e] = nv[1];
+      T th, cl;
+      closure<T>(nv[0], nv[1], A.cp, th, cl);
+      sphl[e] = th;
+      sphl[E + e] = cl;
+      // wall ghosts of this cell (it is their interior neighbour)
+      const int iv[3] = {r.i, r.j, r.k};
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        if (g.periodic[d]) continue;
+        const int stride = d == 0 ? 1 : (d == 1 ? t.EX : t.EXY);
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) {
+          if (iv[d] != (sd ? g.n[d] - 1 : 0)) continue;
+          const int ge = sd ? e + stride : e - stride;
+          T gth = th, gcl = cl;
+          if (A.bc.apply) {
+            const int isign = sd ? 1 : -1;
+            gth = bc_ghost<T>(A.bc.type[d][sd][0], T(A.bc.val[d][sd][0]), th, isign, T(g.dx));
+            gcl = bc_ghost<T>(A.bc.type[d][sd][1], T(A.bc.val[d][sd][1]), cl, isign, T(g.dx));
+          }
+          sphl[ge] = gth;
+          sphl[E + ge] = gcl;
+        }
       }
     }
     __syncthreads();
@@ -291,32 +312,33 @@ __global__ __launch_bounds__(kThreads) void hc_tile_kernel(HCArgs<T> A, TileIO<T
 }
 
 // ---------------------------------------------------------------------------
-// Single-workgroup building blocks over a whole (small) level of at most
-// kWGCells cells: thread t owns cells t + m*kWGThreads, phi and phi_l live in
-// LDS, neighbours are found by periodic wrap or the derived-variable wall rule.
+// Single-workgroup V-cycle over the small levels (at most kWGCells cells):
+// thread t owns cell t of the current level; phi and phi_l of that level live
+// in LDS; neighbours are found by periodic wrap or the derived-variable wall
+// rule.  A relaxation pass updates one colour and recomputes the closure of
+// exactly the cells it updated (the other colour's phi_l is unchanged since
+// its own update), so each pass needs a single barrier and half the closures
+// of a full recomputation, with the same values.
 template <typename T, int DIM>
 struct WGLevel {
   const LevelDesc<T>* L;
   int N, nx, nxy;
-  int c[kCPT], i[kCPT], j[kCPT], k[kCPT];
-  long long q[kCPT];
-  bool own[kCPT];
+  int c, i, j, k;
+  long long q;
+  bool own;
   __device__ inline void init(const LevelDesc<T>* lv) {
     L = lv;
     const Geom& g = L->g;
     nx = g.n[0];
     nxy = g.n[0] * g.n[1];
     N = nxy * (DIM == 3 ? g.n[2] : 1);
-#pragma unroll
-    for (int m = 0; m < kCPT; ++m) {
-      c[m] = threadIdx.x + m * kWGThreads;
-      own[m] = c[m] < N;
-      i[m] = c[m] % nx;
-      const int r = c[m] / nx;
-      j[m] = r % g.n[1];
-      k[m] = DIM == 3 ? r / g.n[1] : 0;
-      q[m] = own[m] ? g.ci(i[m], j[m], k[m]) : 0;
-    }
+    c = threadIdx.x;
+    own = c < N;
+    i = c % nx;
+    const int r = c / nx;
+    j = r % g.n[1];
+    k = DIM == 3 ? r / g.n[1] : 0;
+    q = own ? g.ci(i, j, k) : 0;
   }
 };
 
@@ -347,213 +369,240 @@ __device__ inline void wg_stencil(const Geom& g, const BCDesc& bc, const T* pl, 
   }
 }
 
+// closure of the thread's own cell from its LDS state
 template <typename T, int DIM>
-__device__ inline void wg_closure(const ClosureP<T>& cp, const WGLevel<T, DIM>& W, const T* sphi, T* sphl) {
-#pragma unroll
-  for (int m = 0; m < kCPT; ++m) {
-    if (!W.own[m]) continue;
-    T th, cl;
-    closure<T>(sphi[W.c[m]], sphi[W.N + W.c[m]], cp, th, cl);
-    sphl[W.c[m]] = th;
-    sphl[W.N + W.c[m]] = cl;
-  }
+__device__ inline void wg_closure_own(const ClosureP<T>& cp, const WGLevel<T, DIM>& W, const T* sphi, T* sphl) {
+  if (!W.own) return;
+  T th, cl;
+  closure<T>(sphi[W.c], sphi[W.N + W.c], cp, th, cl);
+  sphl[W.c] = th;
+  sphl[W.N + W.c] = cl;
 }
 
 template <typename T, int DIM>
 __device__ inline void wg_faces(const LevelDesc<T>* L, int i, int j, int k, CellFaces<T, DIM>& F, T* aq) {
-  HCArgs<T> A;
-  A.g = L->g;
-  for (int d = 0; d < 3; ++d) A.b[d] = L->b[d];
-  load_faces<T, DIM>(A, i, j, k, F);
+  load_faces<T, DIM>(L->g, L->b, i, j, k, F);
   const long long q = L->g.ci(i, j, k);
   aq[0] = L->a ? L->a[q] : T(1);
   aq[1] = L->a ? L->a[q + L->g.ncell] : T(1);
 }
 
-// L(phi) of owned cell m (both components) from phi/phi_l in LDS.
+// L(phi) of the thread's cell (both components) from phi/phi_l in LDS
 template <typename T, int DIM>
-__device__ inline void wg_lof(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, int m, const CellFaces<T, DIM>& F,
+__device__ inline void wg_lof(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, const CellFaces<T, DIM>& F,
                               const T* aq, const T* sphi, const T* sphl, T* lof) {
   const Geom& g = W.L->g;
   const T dxinv = T(1) / (T(g.dx) * T(g.dx));
 #pragma unroll
   for (int n = 0; n < 2; ++n) {
     T st[7];
-    wg_stencil<T, DIM>(g, D.bc, sphl + n * W.N, W.c[m], W.i[m], W.j[m], W.k[m], n, W.nx, W.nxy, st);
-    lof[n] = lof_of<T, DIM>(F, n, st, aq[n], sphi[n * W.N + W.c[m]], D.alpha, D.beta, dxinv);
+    wg_stencil<T, DIM>(g, D.bc, sphl + n * W.N, W.c, W.i, W.j, W.k, n, W.nx, W.nxy, st);
+    lof[n] = lof_of<T, DIM>(F, n, st, aq[n], sphi[n * W.N + W.c], D.alpha, D.beta, dxinv);
   }
 }
 
-// relax: npass colour passes of the whole level (first colour red).
+// Per-level operator data of the thread's cell, held in registers.
 template <typename T, int DIM>
-__device__ void wg_relax(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, T* sphi, T* sphl, int npass,
-                         bool load) {
-  const Geom& g = W.L->g;
-  CellFaces<T, DIM> F[kCPT];
-  T aq[kCPT][2], rr[kCPT][2], lam[kCPT][2];
-  const T scale = T(1) / (T(g.dx) * T(g.dx));
-#pragma unroll
-  for (int m = 0; m < kCPT; ++m) {
-    if (!W.own[m]) continue;
-    wg_faces<T, DIM>(W.L, W.i[m], W.j[m], W.k[m], F[m], aq[m]);
-    rr[m][0] = W.L->rhs[W.q[m]];
-    rr[m][1] = W.L->rhs[W.q[m] + g.ncell];
-    lam[m][0] = lambda_of<T, DIM>(F[m], 0, aq[m][0], D.alpha, D.beta, scale);
-    lam[m][1] = lambda_of<T, DIM>(F[m], 1, aq[m][1], D.alpha, D.beta, scale);
-    if (load) {
-      sphi[W.c[m]] = W.L->phi[W.q[m]];
-      sphi[W.N + W.c[m]] = W.L->phi[W.q[m] + g.ncell];
-    }
+struct WGCellOp {
+  CellFaces<T, DIM> F;
+  T aq[2], lam[2];
+  __device__ inline void load(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W) {
+    if (!W.own) return;
+    wg_faces<T, DIM>(W.L, W.i, W.j, W.k, F, aq);
+    const T scale = T(1) / (T(W.L->g.dx) * T(W.L->g.dx));
+    lam[0] = lambda_of<T, DIM>(F, 0, aq[0], D.alpha, D.beta, scale);
+    lam[1] = lambda_of<T, DIM>(F, 1, aq[1], D.alpha, D.beta, scale);
   }
-  __syncthreads();
+};
+
+// npass colour passes (first colour red) on the level whose phi is staged in
+// sphi and whose phi_l (closure of sphi) is staged in sphl; rr = rhs of the
+// thread's cell.  Leaves sphi/sphl consistent and writes phi to global memory.
+template <typename T, int DIM>
+__device__ void wg_relax(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, const WGCellOp<T, DIM>& O,
+                         const T* rr, T* sphi, T* sphl, int npass) {
   for (int p = 0; p < npass; ++p) {
-    wg_closure<T, DIM>(D.cp, W, sphi, sphl);
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < kCPT; ++m) {
-      if (!W.own[m] || ((W.i[m] + W.j[m] + W.k[m] + p) & 1)) continue;
+    if (W.own && !((W.i + W.j + W.k + p) & 1)) {
       T lof[2];
-      wg_lof<T, DIM>(D, W, m, F[m], aq[m], sphi, sphl, lof);
-      const int c = W.c[m];
-      sphi[c] = sphi[c] - lam[m][0] * (lof[0] - rr[m][0]);
-      sphi[W.N + c] = sphi[W.N + c] - lam[m][1] * (lof[1] - rr[m][1]);
+      wg_lof<T, DIM>(D, W, O.F, O.aq, sphi, sphl, lof);
+      const int c = W.c;
+      const T h = sphi[c] - O.lam[0] * (lof[0] - rr[0]);
+      const T t = sphi[W.N + c] - O.lam[1] * (lof[1] - rr[1]);
+      sphi[c] = h;
+      sphi[W.N + c] = t;
+      T th, cl;
+      closure<T>(h, t, D.cp, th, cl);
+      sphl[c] = th;
+      sphl[W.N + c] = cl;
     }
     __syncthreads();
   }
-#pragma unroll
-  for (int m = 0; m < kCPT; ++m) {
-    if (!W.own[m]) continue;
-    W.L->phi[W.q[m]] = sphi[W.c[m]];
-    W.L->phi[W.q[m] + g.ncell] = sphi[W.N + W.c[m]];
+  if (W.own) {
+    W.L->phi[W.q] = sphi[W.c];
+    W.L->phi[W.q + W.L->g.ncell] = sphi[W.N + W.c];
   }
-  __syncthreads();
 }
 
-// rhs_l += L_l(phi_l): the FAS coarse right-hand side of level l.
+// stage the thread's (phi) into LDS and compute every closure
 template <typename T, int DIM>
-__device__ void wg_apply_add(const CoarseDesc<T>& D, const LevelDesc<T>* lv, T* sphi, T* sphl) {
-  WGLevel<T, DIM> W;
-  W.init(lv);
-  const Geom& g = W.L->g;
-#pragma unroll
-  for (int m = 0; m < kCPT; ++m) {
-    if (!W.own[m]) continue;
-    sphi[W.c[m]] = W.L->phi[W.q[m]];
-    sphi[W.N + W.c[m]] = W.L->phi[W.q[m] + g.ncell];
+__device__ inline void wg_stage(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, const T* ph, T* sphi, T* sphl) {
+  if (W.own) {
+    sphi[W.c] = ph[0];
+    sphi[W.N + W.c] = ph[1];
   }
   __syncthreads();
-  wg_closure<T, DIM>(D.cp, W, sphi, sphl);
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < kCPT; ++m) {
-    if (!W.own[m]) continue;
-    CellFaces<T, DIM> F;
-    T aq[2], lof[2];
-    wg_faces<T, DIM>(W.L, W.i[m], W.j[m], W.k[m], F, aq);
-    wg_lof<T, DIM>(D, W, m, F, aq, sphi, sphl, lof);
-    W.L->rhs[W.q[m]] = W.L->rhs[W.q[m]] + lof[0];
-    W.L->rhs[W.q[m] + g.ncell] = W.L->rhs[W.q[m] + g.ncell] + lof[1];
-  }
+  wg_closure_own<T, DIM>(D.cp, W, sphi, sphl);
   __syncthreads();
 }
 
 // Coarse part of the FAS V-cycle (levels l0 .. nlev-1) in one workgroup:
-//   rhs_l0 += L(phi_l0)  (FAS coarse rhs; the restriction was done by the
-//   finer level's kernel);  then for l = l0 .. nlev-2: relax nu1; restrict
-//   phi and rhs - L phi; rhs_{l+1} += L(phi_{l+1});  bottom relax;  then back
+//   rhs_l0 += L(phi_l0) when apply_first (the restriction into l0 was done by
+//   the finer level's kernel); then for l = l0 .. nlev-2: relax nu1; restrict
+//   phi and rhs - L phi; rhs_{l+1} += L(phi_{l+1}); bottom relax; then back
 //   up: phi_l += P(phi_{l+1} - phi0_{l+1}); relax nu2.
+// Levels are visited with their state in registers/LDS; global memory keeps
+// phi, rhs and phi0 of every level for the way back up and for the finer
+// tiled levels.
 template <typename T, int DIM>
 __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const CoarseDesc<T>* __restrict__ Dp,
                                                                       int apply_first) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const CoarseDesc<T>& D = *Dp;
-  T* sphi = reinterpret_cast<T*>(smem_raw);
-  T* sphl = sphi + 2 * kWGCells;
+  T* sphi = reinterpret_cast<T*>(smem_raw);   // 2*kWGCells
+  T* sphl = sphi + 2 * kWGCells;              // 2*kWGCells
+  T* se = sphl + 2 * kWGCells;                // coarse correction, 2*kWGCells/4
   const int l0 = D.l0, L = D.nlev;
   const T denom = DIM == 3 ? T(8) : T(4);
 
-  if (apply_first) wg_apply_add<T, DIM>(D, &D.lev[l0], sphi, sphl);
+  // state of the current level's cell in registers
+  T ph[2] = {T(0), T(0)}, rr[2] = {T(0), T(0)};
+  {
+    WGLevel<T, DIM> W;
+    W.init(&D.lev[l0]);
+    if (W.own) {
+      ph[0] = W.L->phi[W.q];
+      ph[1] = W.L->phi[W.q + W.L->g.ncell];
+      rr[0] = W.L->rhs[W.q];
+      rr[1] = W.L->rhs[W.q + W.L->g.ncell];
+    }
+    if (apply_first) {
+      WGCellOp<T, DIM> O;
+      O.load(D, W);
+      wg_stage<T, DIM>(D, W, ph, sphi, sphl);
+      if (W.own) {
+        T lof[2];
+        wg_lof<T, DIM>(D, W, O.F, O.aq, sphi, sphl, lof);
+        rr[0] = rr[0] + lof[0];
+        rr[1] = rr[1] + lof[1];
+        W.L->rhs[W.q] = rr[0];
+        W.L->rhs[W.q + W.L->g.ncell] = rr[1];
+      }
+      __syncthreads();
+    }
+  }
 
   for (int l = l0; l < L - 1; ++l) {
     WGLevel<T, DIM> W;
     W.init(&D.lev[l]);
-    wg_relax<T, DIM>(D, W, sphi, sphl, 2 * D.nu1, true);
-    // restrict phi and rhs - L(phi) (phi is still in LDS)
-    wg_closure<T, DIM>(D.cp, W, sphi, sphl);
-    __syncthreads();
-    {
-      const Geom& g = W.L->g;
-      const LevelDesc<T>& C = D.lev[l + 1];
-      const Geom& gcc = C.g;
-      const int NC = gcc.n[0] * gcc.n[1] * (DIM == 3 ? gcc.n[2] : 1);
-      const int c = threadIdx.x;
-      if (c < NC) {
-        const int I = c % gcc.n[0], r = c / gcc.n[0], J = r % gcc.n[1], K = DIM == 3 ? r / gcc.n[1] : 0;
-        T s[2] = {T(0), T(0)}, sp[2] = {T(0), T(0)};
-        const T dxinv = T(1) / (T(g.dx) * T(g.dx));
-        for (int kk = 0; kk < (DIM == 3 ? 2 : 1); ++kk)
-          for (int jj = 0; jj < 2; ++jj)
-            for (int ii = 0; ii < 2; ++ii) {
-              const int fi = 2 * I + ii, fj = 2 * J + jj, fk = DIM == 3 ? 2 * K + kk : 0;
-              const int fc = fi + W.nx * fj + W.nxy * fk;
-              CellFaces<T, DIM> F;
-              T aq[2];
-              wg_faces<T, DIM>(W.L, fi, fj, fk, F, aq);
-              const long long q = g.ci(fi, fj, fk);
-#pragma unroll
-              for (int n = 0; n < 2; ++n) {
-                T st[7];
-                wg_stencil<T, DIM>(g, D.bc, sphl + n * W.N, fc, fi, fj, fk, n, W.nx, W.nxy, st);
-                const T lof = lof_of<T, DIM>(F, n, st, aq[n], sphi[n * W.N + fc], D.alpha, D.beta, dxinv);
-                s[n] = s[n] + (W.L->rhs[q + n * g.ncell] - lof) / denom;
-                sp[n] = sp[n] + sphi[n * W.N + fc] / denom;
-              }
-            }
-        const long long qc = gcc.ci(I, J, K);
-#pragma unroll
-        for (int n = 0; n < 2; ++n) {
-          C.rhs[qc + n * gcc.ncell] = s[n];
-          C.phi[qc + n * gcc.ncell] = sp[n];
-          C.phi0[qc + n * gcc.ncell] = sp[n];
-        }
-      }
+    WGCellOp<T, DIM> O;
+    O.load(D, W);
+    wg_stage<T, DIM>(D, W, ph, sphi, sphl);
+    wg_relax<T, DIM>(D, W, O, rr, sphi, sphl, 2 * D.nu1);
+    // restriction (RESTRICTRESNLVC / RESTRICT): each fine cell's
+    // (rhs - L phi)/2^d and phi/2^d, then child sums in Fortran order
+    T tr[2] = {T(0), T(0)}, tp[2] = {T(0), T(0)};
+    if (W.own) {
+      T lof[2];
+      wg_lof<T, DIM>(D, W, O.F, O.aq, sphi, sphl, lof);
+      tr[0] = (rr[0] - lof[0]) / denom;
+      tr[1] = (rr[1] - lof[1]) / denom;
+      tp[0] = sphi[W.c] / denom;
+      tp[1] = sphi[W.N + W.c] / denom;
     }
     __syncthreads();
-    wg_apply_add<T, DIM>(D, &D.lev[l + 1], sphi, sphl);
+    if (W.own) {
+      sphl[W.c] = tr[0];
+      sphl[W.N + W.c] = tr[1];
+      sphi[W.c] = tp[0];
+      sphi[W.N + W.c] = tp[1];
+    }
+    __syncthreads();
+    WGLevel<T, DIM> Wc;
+    Wc.init(&D.lev[l + 1]);
+    if (Wc.own) {
+      T s[2] = {T(0), T(0)}, sp[2] = {T(0), T(0)};
+      for (int kk = 0; kk < (DIM == 3 ? 2 : 1); ++kk)
+        for (int jj = 0; jj < 2; ++jj)
+          for (int ii = 0; ii < 2; ++ii) {
+            const int fc = (2 * Wc.i + ii) + W.nx * (2 * Wc.j + jj) + W.nxy * (DIM == 3 ? 2 * Wc.k + kk : 0);
+#pragma unroll
+            for (int n = 0; n < 2; ++n) {
+              s[n] = s[n] + sphl[n * W.N + fc];
+              sp[n] = sp[n] + sphi[n * W.N + fc];
+            }
+          }
+      const long long qc = Wc.q;
+      const long long ncc = Wc.L->g.ncell;
+#pragma unroll
+      for (int n = 0; n < 2; ++n) {
+        Wc.L->phi[qc + n * ncc] = sp[n];
+        Wc.L->phi0[qc + n * ncc] = sp[n];
+      }
+      ph[0] = sp[0];
+      ph[1] = sp[1];
+      rr[0] = s[0];
+      rr[1] = s[1];
+    }
+    __syncthreads();
+    // FAS coarse right-hand side rhs_{l+1} += L(phi_{l+1})
+    {
+      WGCellOp<T, DIM> Oc;
+      Oc.load(D, Wc);
+      wg_stage<T, DIM>(D, Wc, ph, sphi, sphl);
+      if (Wc.own) {
+        T lof[2];
+        wg_lof<T, DIM>(D, Wc, Oc.F, Oc.aq, sphi, sphl, lof);
+        rr[0] = rr[0] + lof[0];
+        rr[1] = rr[1] + lof[1];
+        Wc.L->rhs[Wc.q] = rr[0];
+        Wc.L->rhs[Wc.q + Wc.L->g.ncell] = rr[1];
+      }
+      __syncthreads();
+    }
   }
   {
     WGLevel<T, DIM> W;
     W.init(&D.lev[L - 1]);
-    wg_relax<T, DIM>(D, W, sphi, sphl, 2 * D.bottom, true);
+    WGCellOp<T, DIM> O;
+    O.load(D, W);
+    wg_stage<T, DIM>(D, W, ph, sphi, sphl);
+    wg_relax<T, DIM>(D, W, O, rr, sphi, sphl, 2 * D.bottom);
   }
   for (int l = L - 2; l >= l0; --l) {
-    WGLevel<T, DIM> W;
-    W.init(&D.lev[l]);
-    const LevelDesc<T>& C = D.lev[l + 1];
-#pragma unroll
-    for (int m = 0; m < kCPT; ++m) {
-      if (!W.own[m]) continue;
-      const long long q = W.q[m];
-      sphi[W.c[m]] = prolong_value<T, DIM>(C.g, W.i[m], W.j[m], W.k[m], C.phi, C.phi0, W.L->phi[q]);
-      sphi[W.N + W.c[m]] = prolong_value<T, DIM>(C.g, W.i[m], W.j[m], W.k[m], C.phi + C.g.ncell,
-                                                 C.phi0 + C.g.ncell, W.L->phi[q + W.L->g.ncell]);
+    // coarse correction phi_{l+1} - phi0_{l+1} into LDS (compact layout)
+    WGLevel<T, DIM> Wc;
+    Wc.init(&D.lev[l + 1]);
+    if (Wc.own) {
+      const long long ncc = Wc.L->g.ncell;
+      se[Wc.c] = sphi[Wc.c] - Wc.L->phi0[Wc.q];
+      se[Wc.N + Wc.c] = sphi[Wc.N + Wc.c] - Wc.L->phi0[Wc.q + ncc];
     }
     __syncthreads();
-    wg_relax<T, DIM>(D, W, sphi, sphl, 2 * D.nu2, false);
+    WGLevel<T, DIM> W;
+    W.init(&D.lev[l]);
+    WGCellOp<T, DIM> O;
+    O.load(D, W);
+    if (W.own) {
+      const long long nc = W.L->g.ncell;
+      rr[0] = W.L->rhs[W.q];
+      rr[1] = W.L->rhs[W.q + nc];
+      ph[0] = prolong_value_lds<T, DIM>(Wc.L->g, W.i, W.j, W.k, se, W.L->phi[W.q]);
+      ph[1] = prolong_value_lds<T, DIM>(Wc.L->g, W.i, W.j, W.k, se + Wc.N, W.L->phi[W.q + nc]);
+    }
+    __syncthreads();
+    wg_stage<T, DIM>(D, W, ph, sphi, sphl);
+    wg_relax<T, DIM>(D, W, O, rr, sphi, sphl, 2 * D.nu2);
   }
-}
-
-// Whole level in one workgroup for the per-operator relax entry point.
-template <typename T, int DIM>
-__global__ __launch_bounds__(kWGThreads) void hc_relax_small_kernel(const CoarseDesc<T>* __restrict__ Dp, int l,
-                                                                    int npass) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const CoarseDesc<T>& D = *Dp;
-  T* sphi = reinterpret_cast<T*>(smem_raw);
-  T* sphl = sphi + 2 * kWGCells;
-  WGLevel<T, DIM> W;
-  W.init(&D.lev[l]);
-  wg_relax<T, DIM>(D, W, sphi, sphl, npass, true);
 }
 
 // closure of valid cells and wall ghosts (computeDiffusedVar, LevelData form)
@@ -618,18 +667,6 @@ static size_t tile_lds(int halo, bool restrict_mode) {
   size_t n = 4 * E;
   if (restrict_mode) n += 2 * size_t(D::TX) * D::TY * (DIM == 3 ? D::TZ : 1);
   return n * sizeof(T);
-}
-
-// passes per relax launch: halo 2 (one red/black sweep) in 2D and 3D
-template <typename T>
-int max_tile_passes(int dim) {
-  (void)dim;
-  return 2;
-}
-
-template <typename T>
-bool small_level_fits(const Geom& g) {
-  return g.nvalid() <= kWGCells;
 }
 
 template <typename T, int DIM, int MODE>
@@ -713,18 +750,8 @@ hipError_t hc_relax(const HCArgs<T>& A, T* const buf[2], int* cur, const T* rhs,
 }
 
 template <typename T>
-hipError_t hc_relax_small(const CoarseDesc<T>* dev_desc, int dim, int level, int npass, hipStream_t s) {
-  const size_t lds = size_t(4) * kWGCells * sizeof(T);
-  if (dim == 2)
-    hipLaunchKernelGGL((hc_relax_small_kernel<T, 2>), dim3(1), dim3(kWGThreads), lds, s, dev_desc, level, npass);
-  else
-    hipLaunchKernelGGL((hc_relax_small_kernel<T, 3>), dim3(1), dim3(kWGThreads), lds, s, dev_desc, level, npass);
-  return hipGetLastError();
-}
-
-template <typename T>
 hipError_t hc_coarse_vcycle(const CoarseDesc<T>* dev_desc, int dim, int apply_first, hipStream_t s) {
-  const size_t lds = size_t(4) * kWGCells * sizeof(T);
+  const size_t lds = (size_t(4) * kWGCells + size_t(2) * (kWGCells / 4)) * sizeof(T);
   if (dim == 2)
     hipLaunchKernelGGL((hc_coarse_vcycle_kernel<T, 2>), dim3(1), dim3(kWGThreads), lds, s, dev_desc, apply_first);
   else
@@ -789,7 +816,6 @@ template hipError_t hc_relax<double>(const HCArgs<double>&, double* const[2], in
                                      hipStream_t);
 template hipError_t hc_relax_tiles<double>(const HCArgs<double>&, double* const[2], int*, const double*, int, int,
                                            const Geom*, const double*, const double*, hipStream_t);
-template hipError_t hc_relax_small<double>(const CoarseDesc<double>*, int, int, int, hipStream_t);
 template hipError_t hc_coarse_vcycle<double>(const CoarseDesc<double>*, int, int, hipStream_t);
 template hipError_t hc_residual<double>(const HCArgs<double>&, double*, const double*, double*,
                                         unsigned long long*, int, hipStream_t);
@@ -799,7 +825,5 @@ template hipError_t hc_restrict_residual<double>(const HCArgs<double>&, const Ge
 template hipError_t hc_restrict_all<double>(const HCArgs<double>&, const Geom&, double*, const double*, double*,
                                             double*, double*, int, hipStream_t);
 template hipError_t hc_diffused_var<double>(const HCArgs<double>&, const double*, double*, int, hipStream_t);
-template int max_tile_passes<double>(int);
-template bool small_level_fits<double>(const Geom&);
 
 }  // namespace ml

@@ -120,6 +120,30 @@ __device__ inline T prolong_value(const Geom& gc, int i, int j, int k, const T* 
   return p;
 }
 
+// The same prolongation from a coarse correction held in a compact
+// (ghost-free, x fastest) array e of level gc.
+template <typename T, int DIM>
+__device__ inline T prolong_value_lds(const Geom& gc, int i, int j, int k, const T* e, T p) {
+  const T den = T(1) / (DIM == 3 ? T(64) : T(16));
+  const T fx1 = T(3) * den, fx2 = T(9) * den, fx3 = T(27) * den, f0 = T(1) * den;
+  const int ic = i / 2, jc = j / 2, kc = DIM == 3 ? k / 2 : 0;
+  const int ox = 2 * (i % 2) - 1, oy = 2 * (j % 2) - 1, oz = DIM == 3 ? 2 * (k % 2) - 1 : 0;
+  const int X0 = ic, X1 = wrap_clamp(ic + ox, gc.n[0], gc.periodic[0]);
+  const int Y0 = jc, Y1 = wrap_clamp(jc + oy, gc.n[1], gc.periodic[1]);
+  const int Z0 = kc, Z1 = DIM == 3 ? wrap_clamp(kc + oz, gc.n[2], gc.periodic[2]) : 0;
+  const int nx = gc.n[0], nxy = gc.n[0] * gc.n[1];
+  auto E = [&](int x, int y, int z) -> T { return e[x + nx * y + nxy * z]; };
+  if (DIM == 2) {
+    p = p + fx2 * E(X0, Y0, 0) + f0 * E(X1, Y1, 0);
+    p = p + fx1 * (E(X1, Y0, 0) + E(X0, Y1, 0));
+  } else {
+    p = p + fx3 * E(X0, Y0, Z0) + f0 * E(X1, Y1, Z1);
+    p = p + fx2 * (E(X1, Y0, Z0) + E(X0, Y1, Z0) + E(X0, Y0, Z1));
+    p = p + fx1 * (E(X1, Y1, Z0) + E(X0, Y1, Z1) + E(X1, Y0, Z1));
+  }
+  return p;
+}
+
 // ---- FP32-refined form (ml_fp32r.hip)
 // Linearised (H, Theta) operator J e = alpha*a*e - beta*div(b grad(M e)) on
 // one MG level: M is the per-cell 2x2 Jacobian of the closure
@@ -176,8 +200,6 @@ template <typename T>
 hipError_t hc_relax_tiles(const HCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, int first,
                           const Geom* gc, const T* pc, const T* pc0, hipStream_t s);
 template <typename T>
-hipError_t hc_relax_small(const CoarseDesc<T>* dev_desc, int dim, int level, int npass, hipStream_t s);
-template <typename T>
 hipError_t hc_coarse_vcycle(const CoarseDesc<T>* dev_desc, int dim, int apply_first, hipStream_t s);
 template <typename T>
 hipError_t hc_restrict_all(const HCArgs<T>& A, const Geom& gc, T* phi, const T* rhs, T* resc, T* phic, T* phic0,
@@ -192,10 +214,6 @@ hipError_t hc_restrict_residual(const HCArgs<T>& A, const Geom& gc, T* phi, cons
                                 int homog, hipStream_t s);
 template <typename T>
 hipError_t hc_diffused_var(const HCArgs<T>& A, const T* phi, T* out, int homog, hipStream_t s);
-template <typename T>
-int max_tile_passes(int dim);
-template <typename T>
-bool small_level_fits(const Geom& g);
 
 // ---- transfers, norms, coefficients (ml_transfer_kernels.hip)
 template <typename T>

@@ -148,6 +148,42 @@ __global__ void average_cells_kernel(Geom gf, Geom gc, int ncomp, int ratio, con
   }
 }
 
+// The same average for large ratios: one workgroup per coarse cell stages
+// the fine block through LDS, row by row in Fortran order, and one thread
+// accumulates it in that order (the sum is sequential by specification; the
+// workgroup hides the load latency).
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void average_cells_block_kernel(Geom gf, Geom gc, int ncomp, int ratio,
+                                                                  const T* __restrict__ af, T* __restrict__ ac) {
+  constexpr int kBuf = 2048;
+  __shared__ T buf[kBuf];
+  const int rz = DIM == 3 ? ratio : 1;
+  const T scale = T(1) / T(ratio * ratio * rz);
+  const long long N = gc.nvalid();
+  const long long cidx = blockIdx.x;
+  if (cidx >= N * ncomp) return;
+  const int n = int(cidx / N);
+  int I, J, K;
+  unflatten(cidx % N, gc.n[0], gc.n[1], I, J, K);
+  const int nrows = ratio * rz;                 // rows (jj, kk), jj fastest
+  const int rows_per_chunk = kBuf / ratio > 0 ? kBuf / ratio : 1;
+  T s = T(0);
+  for (int r0 = 0; r0 < nrows; r0 += rows_per_chunk) {
+    const int nr = (nrows - r0) < rows_per_chunk ? (nrows - r0) : rows_per_chunk;
+    const int cnt = nr * ratio;
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+      const int row = r0 + t / ratio, ii = t % ratio;
+      const int jj = row % ratio, kk = row / ratio;
+      buf[t] = af[gf.ci(I * ratio + ii, J * ratio + jj, K * rz + kk) + n * gf.ncell];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int t = 0; t < cnt; ++t) s = s + buf[t];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ac[gc.ci(I, J, K) + n * gc.ncell] = s * scale;
+}
+
 // CALCULATEPOROSITY_MULTICOMP over valid cells (src/EnthalpyVariablesF.ChF:56-94).
 __global__ void porosity_kernel(Geom g, ClosureP<double> cp, const double* __restrict__ phi,
                                 double* __restrict__ chi) {
@@ -309,6 +345,17 @@ hipError_t average_faces(const Geom& gf, int ncomp, const T* const* bf, int rati
 template <typename T>
 hipError_t average_cells(const Geom& gf, int ncomp, const T* af, int ratio, T* ac, hipStream_t s) {
   Geom gc = coarsen_geom(gf, ratio);
+  const long long block = (long long)ratio * ratio * (gf.dim == 3 ? ratio : 1);
+  if (block >= 64) {
+    const long long nb = gc.nvalid() * ncomp;
+    if (gf.dim == 2)
+      hipLaunchKernelGGL((average_cells_block_kernel<T, 2>), dim3(unsigned(nb)), dim3(256), 0, s, gf, gc, ncomp, ratio,
+                         af, ac);
+    else
+      hipLaunchKernelGGL((average_cells_block_kernel<T, 3>), dim3(unsigned(nb)), dim3(256), 0, s, gf, gc, ncomp, ratio,
+                         af, ac);
+    return hipGetLastError();
+  }
   const int b = grid_for(gc.nvalid() * ncomp);
   if (gf.dim == 2)
     hipLaunchKernelGGL((average_cells_kernel<T, 2>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, ratio, af, ac);

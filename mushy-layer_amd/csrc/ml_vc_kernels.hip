@@ -28,7 +28,7 @@ namespace ml {
 namespace {
 
 constexpr int kVCThreads = 256;   // halo-1 tile kernels
-constexpr int kVCWG = 256;        // single-workgroup kernel threads
+constexpr int kVCWG = 512;        // single-workgroup kernel threads
 constexpr int kVCWGCells = 1024;  // largest level it holds
 constexpr int kVCM = kVCWGCells / kVCWG;
 
