@@ -329,12 +329,16 @@ int ml_hc_set_coefs(ml_hc_op op, const double* aCoef, const double* const bCoef[
                                ctx->stream));
   // MG coarsening straight from the finest level (MGnewOp,
   // src/AMRNonLinearMultiCompOp.cpp:1423-1494)
+  const int L = int(op->lev.size());
   const double* bf[3] = {op->lev[0].b[0], op->lev[0].b[1], op->lev[0].b[2]};
-  for (size_t l = 1; l < op->lev.size(); ++l) {
-    double* bc[3] = {op->lev[l].b[0], op->lev[l].b[1], op->lev[l].b[2]};
-    ML_HIP(ctx, ml::average_faces<double>(g0, 2, bf, 1 << l, bc, ctx->stream));
-    if (aCoef) ML_HIP(ctx, ml::average_cells<double>(g0, 2, op->lev[0].a, 1 << l, op->lev[l].a, ctx->stream));
+  std::vector<double* const*> bc(L);
+  std::vector<double*> ac(L, nullptr);
+  for (int l = 0; l < L; ++l) {
+    bc[l] = op->lev[l].b;
+    ac[l] = op->lev[l].a;
   }
+  ML_HIP(ctx, ml::coarsen_hierarchy<double>(g0, 2, L, bf, bc.data(), aCoef ? op->lev[0].a : nullptr, ac.data(),
+                                            ctx->stream));
   return ML_OK;
 }
 

@@ -35,7 +35,7 @@ struct Geom {
   }
 };
 
-inline Geom make_geom(int dim, const int n[3], const int periodic[3], double dx) {
+__host__ __device__ inline Geom make_geom(int dim, const int n[3], const int periodic[3], double dx) {
   Geom g{};
   g.dim = dim;
   for (int d = 0; d < 3; ++d) {
@@ -55,7 +55,7 @@ inline Geom make_geom(int dim, const int n[3], const int periodic[3], double dx)
   return g;
 }
 
-inline Geom coarsen_geom(const Geom& f, int ratio) {
+__host__ __device__ inline Geom coarsen_geom(const Geom& f, int ratio) {
   int n[3] = {f.n[0] / ratio, f.n[1] / ratio, f.dim == 3 ? f.n[2] / ratio : 1};
   return make_geom(f.dim, n, f.periodic, f.dx * ratio);
 }

@@ -229,6 +229,23 @@ template <typename T>
 hipError_t average_faces(const Geom& gf, int ncomp, const T* const* bf, int ratio, T* const* bc, hipStream_t s);
 template <typename T>
 hipError_t average_cells(const Geom& gf, int ncomp, const T* af, int ratio, T* ac, hipStream_t s);
+
+// The MG coefficient hierarchy in one launch: for every level l = 1..nlev-1,
+// the arithmetic average by ratio 2^l of the finest faces bf[d] into bc[l][d]
+// and (when af != nullptr) of the finest cell field af into ac[l] (same sums,
+// in the same order, as average_faces / average_cells).
+constexpr int kMaxCoarsenJobs = 96;
+template <typename T>
+struct CoarsenJobs {
+  int njobs;
+  long long begin[kMaxCoarsenJobs + 1];
+  short level[kMaxCoarsenJobs];
+  short kind[kMaxCoarsenJobs];     // 0..2 face direction, 3 cells
+  T* dst[kMaxCoarsenJobs];
+};
+template <typename T>
+hipError_t coarsen_hierarchy(const Geom& gf, int ncomp, int nlev, const T* const* bf, T* const* const* bc, const T* af,
+                             T* const* ac, hipStream_t s, bool faces = true);
 hipError_t porosity_field(const Geom& g, const ClosureP<double>& cp, const double* phi, double* chi,
                           hipStream_t s);
 hipError_t hc_bcoef(const Geom& g, double k, double DH, double DC, const double* chi, double* const* b,

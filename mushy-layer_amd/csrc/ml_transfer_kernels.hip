@@ -184,6 +184,53 @@ __global__ __launch_bounds__(256) void average_cells_block_kernel(Geom gf, Geom 
   if (threadIdx.x == 0) ac[gc.ci(I, J, K) + n * gc.ncell] = s * scale;
 }
 
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void coarsen_hierarchy_kernel(Geom gf, int ncomp, CoarsenJobs<T> J, const T* bf0,
+                                                                const T* bf1, const T* bf2, const T* af) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= J.begin[J.njobs]) return;
+  int jb = 0;
+  while (t >= J.begin[jb + 1]) ++jb;
+  const long long item = t - J.begin[jb];
+  const int ratio = 1 << J.level[jb];
+  const Geom gc = coarsen_geom(gf, ratio);
+  const int kind = J.kind[jb];
+  if (kind < 3) {
+    const int d = kind;
+    const T* bf = d == 0 ? bf0 : (d == 1 ? bf1 : bf2);
+    int r[3] = {ratio, ratio, DIM == 3 ? ratio : 1};
+    r[d] = 1;
+    const T scale = T(1) / T(r[0] * r[1] * r[2]);
+    const long long N = gc.nface[d];
+    const int n = int(item / N);
+    int I, Jj, K;
+    unflatten(item % N, int(gc.fn[d][0]), int(gc.fn[d][1]), I, Jj, K);
+    const int base[3] = {I * (d == 0 ? ratio : r[0]), Jj * (d == 1 ? ratio : r[1]), K * (d == 2 ? ratio : r[2])};
+    T sum = T(0);
+    for (int kk = 0; kk < r[2]; ++kk)
+      for (int jj = 0; jj < r[1]; ++jj)
+#pragma unroll 8
+        for (int ii = 0; ii < r[0]; ++ii)
+          sum = sum + bf[gf.fi(d, base[0] + ii, base[1] + jj, base[2] + kk) + n * gf.nface[d]];
+    J.dst[jb][gc.fi(d, I, Jj, K) + n * gc.nface[d]] = sum * scale;
+  } else {
+    const int rz = DIM == 3 ? ratio : 1;
+    const T scale = T(1) / T(ratio * ratio * rz);
+    const long long N = gc.nvalid();
+    const int n = int(item / N);
+    int I, Jj, K;
+    unflatten(item % N, gc.n[0], gc.n[1], I, Jj, K);
+    T sum = T(0);
+    for (int kk = 0; kk < rz; ++kk)
+      for (int jj = 0; jj < ratio; ++jj) {
+        const T* row = af + gf.ci(I * ratio, Jj * ratio + jj, K * rz + kk) + n * gf.ncell;
+#pragma unroll 8
+        for (int ii = 0; ii < ratio; ++ii) sum = sum + row[ii];
+      }
+    J.dst[jb][gc.ci(I, Jj, K) + n * gc.ncell] = sum * scale;
+  }
+}
+
 // CALCULATEPOROSITY_MULTICOMP over valid cells (src/EnthalpyVariablesF.ChF:56-94).
 __global__ void porosity_kernel(Geom g, ClosureP<double> cp, const double* __restrict__ phi,
                                 double* __restrict__ chi) {
@@ -343,6 +390,38 @@ hipError_t average_faces(const Geom& gf, int ncomp, const T* const* bf, int rati
 }
 
 template <typename T>
+hipError_t coarsen_hierarchy(const Geom& gf, int ncomp, int nlev, const T* const* bf, T* const* const* bc, const T* af,
+                             T* const* ac, hipStream_t s, bool faces) {
+  CoarsenJobs<T> J{};
+  long long total = 0;
+  // heaviest (coarsest) levels first so their long sums start first
+  for (int l = nlev - 1; l >= 1; --l) {
+    const Geom gc = coarsen_geom(gf, 1 << l);
+    for (int kind = 0; kind < 4; ++kind) {
+      if (kind < 3 && (!faces || kind >= gf.dim)) continue;
+      if (kind == 3 && !af) continue;
+      if (J.njobs >= kMaxCoarsenJobs) return hipErrorInvalidValue;
+      J.begin[J.njobs] = total;
+      J.level[J.njobs] = short(l);
+      J.kind[J.njobs] = short(kind);
+      J.dst[J.njobs] = kind < 3 ? bc[l][kind] : ac[l];
+      total += (kind < 3 ? gc.nface[kind] : gc.nvalid()) * ncomp;
+      ++J.njobs;
+    }
+  }
+  J.begin[J.njobs] = total;
+  if (total == 0) return hipSuccess;
+  const unsigned blocks = unsigned((total + 255) / 256);
+  if (gf.dim == 2)
+    hipLaunchKernelGGL((coarsen_hierarchy_kernel<T, 2>), dim3(blocks), dim3(256), 0, s, gf, ncomp, J, bf[0], bf[1],
+                       bf[2], af);
+  else
+    hipLaunchKernelGGL((coarsen_hierarchy_kernel<T, 3>), dim3(blocks), dim3(256), 0, s, gf, ncomp, J, bf[0], bf[1],
+                       bf[2], af);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t average_cells(const Geom& gf, int ncomp, const T* af, int ratio, T* ac, hipStream_t s) {
   Geom gc = coarsen_geom(gf, ratio);
   const long long block = (long long)ratio * ratio * (gf.dim == 3 ? ratio : 1);
@@ -411,6 +490,8 @@ template hipError_t prolong_constant<double>(const Geom&, int, double*, const do
 template hipError_t norm_max<double>(const Geom&, int, const double*, unsigned long long*, hipStream_t);
 template hipError_t average_faces<double>(const Geom&, int, const double* const*, int, double* const*, hipStream_t);
 template hipError_t average_cells<double>(const Geom&, int, const double*, int, double*, hipStream_t);
+template hipError_t coarsen_hierarchy<double>(const Geom&, int, int, const double* const*, double* const* const*,
+                                              const double*, double* const*, hipStream_t, bool);
 template hipError_t be_rhs<double>(const Geom&, int, double*, const double*, const double*, double, const double*,
                                    hipStream_t);
 

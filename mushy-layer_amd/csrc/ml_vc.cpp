@@ -264,12 +264,20 @@ int ml_vc_set_coefs(ml_vc_op op, const double* aCoef, const double* const bCoef[
   if (cCoef) ML_HIP(ctx, hipMemcpyAsync(op->lev[0].c, cCoef, sizeof(double) * g0.ncell, hipMemcpyDeviceToDevice, s));
   // arithmetic coarsening straight from the finest level
   // (DarcyBrinkmanOpFactory::MGnewOp, src/DarcyBrinkmanOp.cpp:977-1008)
+  const int L = int(op->lev.size());
   const double* bf[3] = {op->lev[0].b[0], op->lev[0].b[1], op->lev[0].b[2]};
-  for (size_t l = 1; l < op->lev.size(); ++l) {
-    double* bc[3] = {op->lev[l].b[0], op->lev[l].b[1], op->lev[l].b[2]};
-    ML_HIP(ctx, ml::average_faces<double>(g0, 1, bf, 1 << l, bc, s));
-    if (aCoef) ML_HIP(ctx, ml::average_cells<double>(g0, 1, op->lev[0].a, 1 << l, op->lev[l].a, s));
-    if (cCoef) ML_HIP(ctx, ml::average_cells<double>(g0, 1, op->lev[0].c, 1 << l, op->lev[l].c, s));
+  std::vector<double* const*> bc(L);
+  std::vector<double*> ac(L, nullptr), cc(L, nullptr);
+  for (int l = 0; l < L; ++l) {
+    bc[l] = op->lev[l].b;
+    ac[l] = op->lev[l].a;
+    cc[l] = op->lev[l].c;
+  }
+  ML_HIP(ctx, ml::coarsen_hierarchy<double>(g0, 1, L, bf, bc.data(), aCoef ? op->lev[0].a : nullptr, ac.data(), s));
+  if (cCoef) {
+    const double* none[3] = {nullptr, nullptr, nullptr};
+    std::vector<double* const*> nob(L, nullptr);
+    ML_HIP(ctx, ml::coarsen_hierarchy<double>(g0, 1, L, none, nob.data(), op->lev[0].c, cc.data(), s, false));
   }
   return ML_OK;
 }

@@ -414,7 +414,7 @@ template <int DIM>
 struct VCRelaxTile;
 template <>
 struct VCRelaxTile<2> {
-  static constexpr int TX = 32, TY = 16, TZ = 1, HMAX = 4, NT = 512;
+  static constexpr int TX = 32, TY = 16, TZ = 1, HMAX = 8, NT = 512;
 };
 template <>
 struct VCRelaxTile<3> {
