@@ -172,27 +172,36 @@ __device__ inline float wall_sign(const BCDesc& bc, int d, int s, int n) {
   return (bc.apply && bc.type[d][s][n] == 0) ? -1.0f : 1.0f;
 }
 
-// (J e)_n, its 2x2 diagonal block, then one block Gauss-Seidel update of e.
+// Inverse of the 2x2 diagonal block of J at a cell:
+// D = alpha*a + beta*sum(b_lo + b_hi)*dxinv * M (per component row).
 template <typename T, int DIM>
-__device__ inline void block_update(const CellFaces<T, DIM>& F, const T* aq, const T* rr, const T* Mc, T alpha,
-                                    T beta, T dxinv, const T* u0s, const T* u1s, T& e0, T& e1) {
-  T je[2], sig[2];
-  const T* us[2] = {u0s, u1s};
-  const T ec[2] = {e0, e1};
+__device__ inline void block_inverse(const CellFaces<T, DIM>& F, const T* aq, const T* Mc, T alpha, T beta, T dxinv,
+                                     T* inv) {
+  T sig[2];
 #pragma unroll
   for (int n = 0; n < 2; ++n) {
-    je[n] = lof_of<T, DIM>(F, n, us[n], aq[n], ec[n], alpha, beta, dxinv);
     T sb = T(0);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) sb = sb + (F.v[n][2 * d + 1] + F.v[n][2 * d]);
     sig[n] = beta * sb * dxinv;
   }
-  const T r0 = rr[0] - je[0], r1 = rr[1] - je[1];
   const T d00 = alpha * aq[0] + sig[0] * Mc[0], d01 = sig[0] * Mc[1];
   const T d10 = sig[1] * Mc[2], d11 = alpha * aq[1] + sig[1] * Mc[3];
-  const T det = d00 * d11 - d01 * d10;
-  e0 = e0 + (d11 * r0 - d01 * r1) / det;
-  e1 = e1 + (d00 * r1 - d10 * r0) / det;
+  const T rdet = T(1) / (d00 * d11 - d01 * d10);
+  inv[0] = d11 * rdet;
+  inv[1] = -d01 * rdet;
+  inv[2] = -d10 * rdet;
+  inv[3] = d00 * rdet;
+}
+
+// (J e)_n from the staged (M e) stencils, then one block Gauss-Seidel update.
+template <typename T, int DIM>
+__device__ inline void block_step(const CellFaces<T, DIM>& F, const T* aq, const T* rr, const T* inv, T alpha, T beta,
+                                  T dxinv, const T* u0s, const T* u1s, T& e0, T& e1) {
+  const T r0 = rr[0] - lof_of<T, DIM>(F, 0, u0s, aq[0], e0, alpha, beta, dxinv);
+  const T r1 = rr[1] - lof_of<T, DIM>(F, 1, u1s, aq[1], e1, alpha, beta, dxinv);
+  e0 = e0 + (inv[0] * r0 + inv[1] * r1);
+  e1 = e1 + (inv[2] * r0 + inv[3] * r1);
 }
 
 // Red/black block relaxation of J e = r over a tile with halo npass.
@@ -230,48 +239,60 @@ __global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, c
     for (int m = 0; m < 4; ++m) sM[m * E + r.e] = L.M[r.q + m * g.ncell];
   }
   __syncthreads();
+  // u = M e at every staged position (wall ghosts by the homogeneous rule)
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    if (r.kind == 0) continue;
+    float u0, u1;
+    mvec<float>(sM, se, E, r.kind == 1 ? r.e : r.eint, u0, u1);
+    if (r.kind == 2) {
+      u0 *= wall_sign(bc, r.gd, r.gs, 0);
+      u1 *= wall_sign(bc, r.gd, r.gs, 1);
+    }
+    su[r.e] = u0;
+    su[E + r.e] = u1;
+  }
+  __syncthreads();
+  // each pass updates one colour and refreshes M e at the updated cells and
+  // their wall ghosts: one barrier per pass
   for (int p = 0; p < npass; ++p) {
     const int gext = t.H - p;
-    CellFaces<float, DIM> F[KMAX];
-    float rr[KMAX][2], aq[KMAX][2];
-    bool upd[KMAX];
 #pragma unroll
     for (int kk = 0; kk < KMAX; ++kk) {
       const PosRec& r = R[kk];
-      upd[kk] = r.kind == 1 && r.dist <= gext - 1 && !((r.i + r.j + r.k + p) & 1);
-      if (!upd[kk]) continue;
-      load_faces<float, DIM>(g, L.b, r.i, r.j, r.k, F[kk]);
-      rr[kk][0] = L.r[r.q];
-      rr[kk][1] = L.r[r.q + g.ncell];
-      aq[kk][0] = L.a ? L.a[r.q] : 1.0f;
-      aq[kk][1] = L.a ? L.a[r.q + g.ncell] : 1.0f;
-    }
-#pragma unroll
-    for (int kk = 0; kk < KMAX; ++kk) {
-      const PosRec& r = R[kk];
-      if (r.kind == 0 || r.dist > gext) continue;
-      float u0, u1;
-      mvec<float>(sM, se, E, r.kind == 1 ? r.e : r.eint, u0, u1);
-      if (r.kind == 2) {
-        u0 *= wall_sign(bc, r.gd, r.gs, 0);
-        u1 *= wall_sign(bc, r.gd, r.gs, 1);
-      }
-      su[r.e] = u0;
-      su[E + r.e] = u1;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < KMAX; ++kk) {
-      if (!upd[kk]) continue;
-      const int e = R[kk].e;
+      if (!(r.kind == 1 && r.dist <= gext - 1 && !((r.i + r.j + r.k + p) & 1))) continue;
+      CellFaces<float, DIM> F;
+      load_faces<float, DIM>(g, L.b, r.i, r.j, r.k, F);
+      const float rr[2] = {L.r[r.q], L.r[r.q + g.ncell]};
+      const float aq[2] = {L.a ? L.a[r.q] : 1.0f, L.a ? L.a[r.q + g.ncell] : 1.0f};
+      const int e = r.e;
+      const float Mc[4] = {sM[e], sM[E + e], sM[2 * E + e], sM[3 * E + e]};
+      float inv[4];
+      block_inverse<float, DIM>(F, aq, Mc, alpha, beta, dxinv, inv);
       float u0s[7], u1s[7];
       gather_lds<float, DIM>(su, e, t.EX, t.EXY, u0s);
       gather_lds<float, DIM>(su + E, e, t.EX, t.EXY, u1s);
-      const float Mc[4] = {sM[e], sM[E + e], sM[2 * E + e], sM[3 * E + e]};
       float e0 = se[e], e1 = se[E + e];
-      block_update<float, DIM>(F[kk], aq[kk], rr[kk], Mc, alpha, beta, dxinv, u0s, u1s, e0, e1);
+      block_step<float, DIM>(F, aq, rr, inv, alpha, beta, dxinv, u0s, u1s, e0, e1);
       se[e] = e0;
       se[E + e] = e1;
+      const float u0 = Mc[0] * e0 + Mc[1] * e1, u1 = Mc[2] * e0 + Mc[3] * e1;
+      su[e] = u0;
+      su[E + e] = u1;
+      const int iv[3] = {r.i, r.j, r.k};
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        if (g.periodic[d]) continue;
+        const int stride = d == 0 ? 1 : (d == 1 ? t.EX : t.EXY);
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) {
+          if (iv[d] != (sd ? g.n[d] - 1 : 0)) continue;
+          const int ge = sd ? e + stride : e - stride;
+          su[ge] = u0 * wall_sign(bc, d, sd, 0);
+          su[E + ge] = u1 * wall_sign(bc, d, sd, 1);
+        }
+      }
     }
     __syncthreads();
   }
@@ -426,33 +447,32 @@ __device__ inline void wg_ustencil(const Geom& g, const BCDesc& bc, const T* su,
 }
 
 // npass red/black block passes on a whole small level held in LDS
-// (se: e, su: M e, sM: Jacobian; all of this level).
+// (se: e, su: M e, sM: Jacobian; all of this level).  M e is refreshed at
+// the updated cells only, so each pass needs one barrier.
 template <typename T, int DIM>
 __device__ void wg_lin_relax(const WGLin<T, DIM>& W, const BCDesc& bc, T alpha, T beta, const T* const* b,
                              const T* a, const T* rg, T* se, T* su, const T* sM, int npass) {
   const Geom& g = W.g;
   const T dxinv = T(1.0 / (g.dx * g.dx));
+  const int N = W.N;
   CellFaces<T, DIM> F[kWGM];
-  T rr[kWGM][2], aq[kWGM][2];
+  T rr[kWGM][2], aq[kWGM][2], inv[kWGM][4];
 #pragma unroll
   for (int m = 0; m < kWGM; ++m) {
     if (!W.own[m]) continue;
+    const int c = W.c[m];
     load_faces<T, DIM>(g, b, W.i[m], W.j[m], W.k[m], F[m]);
-    rr[m][0] = rg[W.c[m]];
-    rr[m][1] = rg[W.N + W.c[m]];
+    rr[m][0] = rg[c];
+    rr[m][1] = rg[N + c];
     aq[m][0] = a ? a[W.q[m]] : T(1);
     aq[m][1] = a ? a[W.q[m] + g.ncell] : T(1);
+    const T Mc[4] = {sM[c], sM[N + c], sM[2 * N + c], sM[3 * N + c]};
+    block_inverse<T, DIM>(F[m], aq[m], Mc, alpha, beta, dxinv, inv[m]);
+    su[c] = Mc[0] * se[c] + Mc[1] * se[N + c];
+    su[N + c] = Mc[2] * se[c] + Mc[3] * se[N + c];
   }
-  const int N = W.N;
+  __syncthreads();
   for (int p = 0; p < npass; ++p) {
-#pragma unroll
-    for (int m = 0; m < kWGM; ++m) {
-      if (!W.own[m]) continue;
-      const int c = W.c[m];
-      su[c] = sM[c] * se[c] + sM[N + c] * se[N + c];
-      su[N + c] = sM[2 * N + c] * se[c] + sM[3 * N + c] * se[N + c];
-    }
-    __syncthreads();
 #pragma unroll
     for (int m = 0; m < kWGM; ++m) {
       if (!W.own[m] || ((W.i[m] + W.j[m] + W.k[m] + p) & 1)) continue;
@@ -460,14 +480,117 @@ __device__ void wg_lin_relax(const WGLin<T, DIM>& W, const BCDesc& bc, T alpha, 
       T u0s[7], u1s[7];
       wg_ustencil<T, DIM>(g, bc, su, c, W.i[m], W.j[m], W.k[m], 0, W.nx, W.nxy, u0s);
       wg_ustencil<T, DIM>(g, bc, su + N, c, W.i[m], W.j[m], W.k[m], 1, W.nx, W.nxy, u1s);
-      const T Mc[4] = {sM[c], sM[N + c], sM[2 * N + c], sM[3 * N + c]};
       T e0 = se[c], e1 = se[N + c];
-      block_update<T, DIM>(F[m], aq[m], rr[m], Mc, alpha, beta, dxinv, u0s, u1s, e0, e1);
+      block_step<T, DIM>(F[m], aq[m], rr[m], inv[m], alpha, beta, dxinv, u0s, u1s, e0, e1);
       se[c] = e0;
       se[N + c] = e1;
+      su[c] = sM[c] * e0 + sM[N + c] * e1;
+      su[N + c] = sM[2 * N + c] * e0 + sM[3 * N + c] * e1;
     }
     __syncthreads();
   }
+}
+
+// Direct FP64 solve of J e = r on a whole tiny level (2N <= kDirect
+// unknowns): each thread t < 2N builds column t of J by applying the
+// operator to the unit vector of unknown t; thread 0 then eliminates with
+// partial pivoting.  Returns false (leaving e untouched) on a zero pivot.
+constexpr int kDirect = 16;
+template <int DIM>
+__device__ bool wg_direct_solve(const Geom& g, const BCDesc& bc, double alpha, double beta, const double* const* b,
+                                const double* a, const double* sMd, const double* rd, double* ed, double* A,
+                                int* flag) {
+  const int N = g.n[0] * g.n[1] * (DIM == 3 ? g.n[2] : 1);
+  const int U = 2 * N;
+  const double dxinv = 1.0 / (g.dx * g.dx);
+  const int nx = g.n[0], nxy = g.n[0] * g.n[1];
+  const int t = threadIdx.x;
+  if (t < U) {
+    const int ct = t % N, nt = t / N;
+    // u = M e for the unit vector: nonzero only at cell ct
+    auto u_at = [&](int cell, int k) -> double {
+      return cell == ct ? sMd[(2 * k + nt) * N + ct] : 0.0;
+    };
+    for (int r = 0; r < N; ++r) {
+      const int i = r % nx, j = (r / nx) % g.n[1], k = DIM == 3 ? r / nxy : 0;
+      CellFaces<double, DIM> F;
+      load_faces<double, DIM>(g, b, i, j, k, F);
+      const int iv[3] = {i, j, k};
+#pragma unroll
+      for (int kc = 0; kc < 2; ++kc) {
+        double p[7];
+        p[0] = u_at(r, kc);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          const int nd = g.n[d];
+          const int st = d == 0 ? 1 : (d == 1 ? nx : nxy);
+#pragma unroll
+          for (int sd = 0; sd < 2; ++sd) {
+            const int v = iv[d] + (sd ? 1 : -1);
+            double val;
+            if (v >= 0 && v < nd)
+              val = u_at(r + (sd ? st : -st), kc);
+            else if (g.periodic[d])
+              val = u_at(r + (sd ? -(nd - 1) * st : (nd - 1) * st), kc);
+            else
+              val = p[0] * double(wall_sign(bc, d, sd, kc));
+            p[1 + 2 * d + (sd ? 0 : 1)] = val;
+          }
+        }
+        const long long q = g.ci(i, j, k) + kc * g.ncell;
+        const double aq = a ? a[q] : 1.0;
+        const double ev = (r == ct && kc == nt) ? 1.0 : 0.0;
+        A[(kc * N + r) * kDirect + t] = lof_of<double, DIM>(F, kc, p, aq, ev, alpha, beta, dxinv);
+      }
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    double x[kDirect];
+    for (int r = 0; r < U; ++r) x[r] = rd[r];
+    int ok = 1;
+    for (int c = 0; c < U && ok; ++c) {
+      int piv = c;
+      double best = fabs(A[c * kDirect + c]);
+      for (int r = c + 1; r < U; ++r)
+        if (fabs(A[r * kDirect + c]) > best) {
+          best = fabs(A[r * kDirect + c]);
+          piv = r;
+        }
+      if (!(best > 0.0)) {
+        ok = 0;
+        break;
+      }
+      if (piv != c) {
+        for (int k = 0; k < U; ++k) {
+          const double tmp = A[c * kDirect + k];
+          A[c * kDirect + k] = A[piv * kDirect + k];
+          A[piv * kDirect + k] = tmp;
+        }
+        const double tmp = x[c];
+        x[c] = x[piv];
+        x[piv] = tmp;
+      }
+      const double rp = 1.0 / A[c * kDirect + c];
+      for (int r = c + 1; r < U; ++r) {
+        const double f = A[r * kDirect + c] * rp;
+        if (f == 0.0) continue;
+        for (int k = c; k < U; ++k) A[r * kDirect + k] -= f * A[c * kDirect + k];
+        x[r] -= f * x[c];
+      }
+    }
+    if (ok)
+      for (int c = U - 1; c >= 0; --c) {
+        double v = x[c];
+        for (int k = c + 1; k < U; ++k) v -= A[c * kDirect + k] * x[k];
+        x[c] = v / A[c * kDirect + c];
+      }
+    if (ok)
+      for (int r = 0; r < U; ++r) ed[r] = x[r];
+    *flag = ok;
+  }
+  __syncthreads();
+  return *flag != 0;
 }
 
 // Linear V-cycle for levels l0.. in one workgroup: per level the state
@@ -562,7 +685,7 @@ __global__ __launch_bounds__(kWGT) void lin_coarse_vcycle_kernel(const LinDesc* 
     __syncthreads();
   }
 
-  // bottom: FP64 block Gauss-Seidel on the coarsest level
+  // bottom: FP64 solve on the coarsest level
   {
     const LinLevel<float>& lv = D.lev[L - 1];
     WGLin<double, DIM> W;
@@ -584,7 +707,14 @@ __global__ __launch_bounds__(kWGT) void lin_coarse_vcycle_kernel(const LinDesc* 
       for (int mm = 0; mm < 4; ++mm) dM[mm * W.N + c] = double(lv.M[q + mm * lv.g.ncell]);
     }
     __syncthreads();
-    wg_lin_relax<double, DIM>(W, D.bc, D.alpha, D.beta, D.b64, D.a64, dr, de, du, dM, 2 * D.bottom);
+    // a tiny coarsest level is solved exactly; otherwise (or on a zero
+    // pivot) `bottom` block Gauss-Seidel iterations
+    __shared__ int direct_ok;
+    bool solved = false;
+    if (2 * W.N <= kDirect)
+      solved = wg_direct_solve<DIM>(lv.g, D.bc, D.alpha, D.beta, D.b64, D.a64, dM, dr, de, dr + 2 * W.N,
+                                    &direct_ok);
+    if (!solved) wg_lin_relax<double, DIM>(W, D.bc, D.alpha, D.beta, D.b64, D.a64, dr, de, du, dM, 2 * D.bottom);
 #pragma unroll
     for (int m = 0; m < kWGM; ++m) {
       if (!W.own[m]) continue;

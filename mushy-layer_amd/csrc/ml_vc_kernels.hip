@@ -298,13 +298,14 @@ __device__ void vcwg_relax(const VCWG<T, DIM>& W, const VCDesc<T>& D, const VCLe
   A.c = lv.c;
   for (int d = 0; d < 3; ++d) A.b[d] = lv.b[d];
   Faces1<T, DIM> F[kVCM];
-  T aq[kVCM], cv[kVCM];
+  T aq[kVCM], cv[kVCM], lam[kVCM];
 #pragma unroll
   for (int m = 0; m < kVCM; ++m) {
     if (!W.own[m]) continue;
     load_faces1<T, DIM>(A, W.i[m], W.j[m], W.k[m], F[m]);
     aq[m] = A.a ? A.a[W.q[m]] : T(1);
     cv[m] = A.c ? A.c[W.q[m]] : T(0);
+    lam[m] = vc_lambda<T, DIM>(F[m], aq[m], A.c != nullptr, cv[m], A.alpha, A.beta, dxinv);
   }
   for (int p = 0; p < npass; ++p) {
 #pragma unroll
@@ -313,8 +314,7 @@ __device__ void vcwg_relax(const VCWG<T, DIM>& W, const VCDesc<T>& D, const VCLe
       T st[7];
       vcwg_stencil<T, DIM>(g, D.bc, su, W.c[m], W.i[m], W.j[m], W.k[m], W.nx, W.nxy, st);
       const T lof = vc_lof<T, DIM>(F[m], st, aq[m], A.c != nullptr, cv[m], A.alpha, A.beta, dxinv);
-      const T lam = vc_lambda<T, DIM>(F[m], aq[m], A.c != nullptr, cv[m], A.alpha, A.beta, dxinv);
-      su[W.c[m]] = st[0] - lam * (lof - sr[W.c[m]]);
+      su[W.c[m]] = st[0] - lam[m] * (lof - sr[W.c[m]]);
     }
     __syncthreads();
   }

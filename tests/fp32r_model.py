@@ -3,8 +3,8 @@ outer defect correction (FP64 nonlinear residual from the oracle, closure
 Jacobian M) and the same inner linear V-cycle on
     J e = alpha e - beta div(b grad(M e)) = r
 (2x2 point-block red/black Gauss-Seidel, restriction of r - J e and of M,
-PROLONG_JP linear prolongation, `bottom` block GS iterations on the coarsest
-level), evaluated here in FP64 (or FP32 with dtype=np.float32), with the
+PROLONG_JP linear prolongation, an exact solve on a coarsest level of at
+most 16 unknowns and `bottom` block GS iterations on a larger one), evaluated here in FP64 (or FP32 with dtype=np.float32), with the
 same step-length backtracking on the residual max-norm.
 
 It documents the algorithm the GPU form implements and lets tests separate
@@ -165,9 +165,24 @@ def _prolong(lev_f, ec):
     return O.valid(gf, f).copy()
 
 
+def direct(L, M, r, alpha, beta):
+    """Exact solve of J e = r on a tiny level (dense assembly from unit
+    vectors)."""
+    shape = r.shape
+    U = r.size
+    A = np.zeros((U, U))
+    for t in range(U):
+        unit = np.zeros(U)
+        unit[t] = 1.0
+        A[:, t] = apply_J(L, M, unit.reshape(shape), alpha, beta).ravel()
+    return np.linalg.solve(A, r.ravel()).reshape(shape)
+
+
 def vcycle(levs, Ms, l, e, r, alpha, beta, nu1, nu2, bottom):
     L = levs[l]
     if l == len(levs) - 1:
+        if r.size <= 16:
+            return direct(L, Ms[l], r, alpha, beta)
         return smooth(L, Ms[l], e, r, alpha, beta, 2 * bottom)
     e = smooth(L, Ms[l], e, r, alpha, beta, 2 * nu1)
     rc = _restrict(r - apply_J(L, Ms[l], e, alpha, beta), L.dim)
