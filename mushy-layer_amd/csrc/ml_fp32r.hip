@@ -82,6 +82,7 @@ __global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_kernel(HCArg
                                                                            float* __restrict__ r32,
                                                                            float* __restrict__ M32,
                                                                            unsigned long long* norm, int ntx, int nty) {
+  if (A.skip && *A.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2) * (TY + 2) * (DIM == 3 ? TZ + 2 : 1);
   constexpr int KMAX = (EMAX + kLinThreads - 1) / kLinThreads;
@@ -210,6 +211,7 @@ __global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, c
                                                             float* __restrict__ eout, BCDesc bc, float alpha,
                                                             float beta, int npass, int zero_init, Geom gc,
                                                             const float* __restrict__ pc, int ntx, int nty) {
+  if (L.skip && *L.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2 * HMAX) * (TY + 2 * HMAX) * (DIM == 3 ? TZ + 2 * HMAX : 1);
   constexpr int KMAX = (EMAX + NT - 1) / NT;
@@ -309,6 +311,7 @@ __global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, c
 template <int DIM, int TX, int TY, int TZ>
 __global__ __launch_bounds__(kLinThreads) void lin_restrict_kernel(LinLevel<float> L, LinLevel<float> C, BCDesc bc,
                                                                    float alpha, float beta, int ntx, int nty) {
+  if (L.skip && *L.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2) * (TY + 2) * (DIM == 3 ? TZ + 2 : 1);
   constexpr int KMAX = (EMAX + kLinThreads - 1) / kLinThreads;
@@ -600,6 +603,7 @@ template <int DIM>
 __global__ __launch_bounds__(kWGT) void lin_coarse_vcycle_kernel(const LinDesc* __restrict__ Dp) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const LinDesc& D = *Dp;
+  if (D.skip && *D.skip) return;
   float* se = reinterpret_cast<float*>(smem_raw);   // 2*kWGC
   float* su = se + 2 * kWGC;                         // 2*kWGC
   float* sM = su + 2 * kWGC;                         // 4*kWGC

@@ -77,6 +77,7 @@ __device__ inline void tile_diffused_var(const HCArgs<T>& A, const Tile<DIM, TX,
 // post-smoother).  Input and output arrays are distinct.
 template <typename T, int DIM, int TX, int TY, int TZ, int HMAX, int NT>
 __global__ __launch_bounds__(NT) void hc_relax_tile_kernel(HCArgs<T> A, TileIO<T> io) {
+  if (A.skip && *A.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2 * HMAX) * (TY + 2 * HMAX) * (DIM == 3 ? TZ + 2 * HMAX : 1);
   constexpr int KMAX = (EMAX + NT - 1) / NT;
@@ -199,6 +200,7 @@ __global__ __launch_bounds__(NT) void hc_relax_tile_kernel(HCArgs<T> A, TileIO<T
 // one tile with a one-cell halo.
 template <typename T, int DIM, int TX, int TY, int TZ, int MODE>
 __global__ __launch_bounds__(kThreads) void hc_tile_kernel(HCArgs<T> A, TileIO<T> io) {
+  if (A.skip && *A.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const Geom& g = A.g;
   Tile<DIM, TX, TY, TZ> t;
@@ -468,6 +470,7 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
                                                                       int apply_first) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const CoarseDesc<T>& D = *Dp;
+  if (D.skip && *D.skip) return;
   T* sphi = reinterpret_cast<T*>(smem_raw);   // 2*kWGCells
   T* sphl = sphi + 2 * kWGCells;              // 2*kWGCells
   T* se = sphl + 2 * kWGCells;                // coarse correction, 2*kWGCells/4

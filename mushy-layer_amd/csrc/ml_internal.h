@@ -88,9 +88,11 @@ class HCEngine {
 
  private:
   int init();
-  int vcycle(double* phi, const double* rhs, const ml_solve_opts& o);
-  int residual_norm(double* phi, const double* rhs, int slot);
+  int vcycle(double* phi, const double* rhs, const ml_solve_opts& o, const int* skip);
+  int residual_norm(double* phi, const double* rhs, int slot, const int* skip);
   int read_norm(int slot, double* out);
+  int read_ctl(SolveCtl* out);
+  int write_ctl(const SolveCtl& c);
   int upload_desc(double* phi0, const double* rhs0, const ml_solve_opts& o);
   // FP32-refined form: FP64 defect correction around FP32 linear V-cycles
   int solve_refined(double* phi, const double* rhs, const ml_solve_opts& o, ml_solve_stats* st);
@@ -124,6 +126,10 @@ class HCEngine {
   int m_nslots = 0;
   CoarseDesc<double>* m_desc = nullptr;        // device
   CoarseDesc<double>* m_hdesc = nullptr;       // pinned host staging
+  SolveCtl* m_ctl = nullptr;                   // device iteration control
+  SolveCtl* m_hctl = nullptr;                  // pinned mirror
+  int m_batch_fas = 4;                         // iterations enqueued per host check
+  int m_batch_refined = 4;
   bool m_ok = false;
 };
 
@@ -157,6 +163,9 @@ struct ml_vc_op_s {
   int nslots = 0;
   ml::VCDesc<double>* desc = nullptr;    // device
   ml::VCDesc<double>* hdesc = nullptr;   // pinned staging
+  ml::SolveCtl* ctl = nullptr;           // device iteration control
+  ml::SolveCtl* hctl = nullptr;          // pinned mirror
+  int batch = 4;                         // iterations enqueued per host check
 
   ml::VCArgs<double> args(int depth) const {
     ml::VCArgs<double> A;

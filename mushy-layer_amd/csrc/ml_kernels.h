@@ -20,6 +20,7 @@ struct HCArgs {
   T alpha, beta;
   const T* a;      // aCoef (2 comps) or nullptr for aCoef == 1
   const T* b[3];   // face coefficients (2 comps each)
+  const int* skip = nullptr;   // device flag: the kernel returns at once when set
 };
 
 // Max-reduction of a non-negative value over the workgroup, then one
@@ -83,6 +84,7 @@ struct CoarseDesc {
   ClosureP<T> cp;
   BCDesc bc;
   T alpha, beta;
+  const int* skip;   // device flag: the kernel returns at once when set
   LevelDesc<T> lev[kMaxLevels];
 };
 
@@ -156,12 +158,14 @@ struct LinLevel {
   T* M;            // Jacobian (4 comps)
   T* e;            // correction (2 comps)
   T* r;            // right-hand side (2 comps)
+  const int* skip;   // device flag: the kernel returns at once when set
 };
 
 struct LinDesc {
   int l0, nlev, nu1, nu2, bottom;
   BCDesc bc;
   double alpha, beta;
+  const int* skip;
   LinLevel<float> lev[kMaxLevels];
   const double* b64[3];   // coarsest level, FP64 faces for the bottom solve
   const double* a64;
@@ -225,6 +229,27 @@ hipError_t prolong_constant(const Geom& gf, int ncomp, T* phif, const T* corr, h
 template <typename T>
 hipError_t norm_max(const Geom& g, int ncomp, const T* x, unsigned long long* out, hipStream_t s);
 hipError_t zero_u64(unsigned long long* p, int n, hipStream_t s);
+
+// Device-side iteration control of a multigrid solve.  The host launches
+// several iterations ahead; every kernel of an iteration returns at once when
+// `done` is set, so the iterations that run, and their results, are exactly
+// those of a host loop that checks the norm after each iteration.
+struct SolveCtl {
+  int done;        // read by every kernel of the loop (skip flag)
+  int iter;        // iterations accepted so far
+  int status;      // exit-status bits 1/2/4/8 (0 while running)
+  int need_host;   // refined form: a trial step was rejected (host backtracks)
+  int max_iter;
+  double eps, hang, thresh;
+  double r0, prev, norm;
+};
+// ctl from the initial residual norm in slots[0]
+hipError_t ctl_init(SolveCtl* ctl, const unsigned long long* slots, int max_iter, double eps, double hang,
+                    double thresh, hipStream_t s);
+// after iteration iter+1 wrote its norm into slots[iter+1]: apply the
+// stopping rule; with refined != 0 a norm that does not decrease is not
+// accepted but handed to the host (need_host, done)
+hipError_t ctl_check(SolveCtl* ctl, const unsigned long long* slots, int refined, hipStream_t s);
 template <typename T>
 hipError_t average_faces(const Geom& gf, int ncomp, const T* const* bf, int ratio, T* const* bc, hipStream_t s);
 template <typename T>
@@ -270,6 +295,7 @@ struct VCArgs {
   const T* a;
   const T* c;
   const T* b[3];
+  const int* skip = nullptr;   // device flag: the kernel returns at once when set
 };
 
 template <typename T>
@@ -299,6 +325,7 @@ struct VCDesc {
   int l0, nlev, nu1, nu2, bottom;
   BCDesc bc;
   T alpha, beta;
+  const int* skip;
   VCLevel<T> lev[kMaxLevels];
 };
 

@@ -103,6 +103,60 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
   if (i < n) p[i] = 0ull;
 }
 
+// stopping rule of the multigrid solves (status bits: 1 |r| <= eps*r0,
+// 2 iteration maximum, 4 hang, 8 |r| <= thresh), evaluated by one thread
+__device__ inline void ctl_finish_iteration(SolveCtl* c, double norm) {
+  int status = 0;
+  if (norm <= c->eps * c->r0) status |= 1;
+  if (norm <= c->thresh) status |= 8;
+  if (status == 0 && norm > (1.0 - c->hang) * c->prev) status |= 4;
+  c->prev = norm;
+  c->norm = norm;
+  if (status != 0 || c->iter >= c->max_iter) {
+    c->status = status != 0 ? status : 2;
+    c->done = 1;
+  }
+}
+
+__global__ void ctl_init_kernel(SolveCtl* c, const unsigned long long* slots, int max_iter, double eps, double hang,
+                                double thresh) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double r0 = __longlong_as_double((long long)slots[0]);
+  c->iter = 0;
+  c->need_host = 0;
+  c->max_iter = max_iter;
+  c->eps = eps;
+  c->hang = hang;
+  c->thresh = thresh;
+  c->r0 = r0;
+  c->prev = r0;
+  c->norm = r0;
+  c->status = 0;
+  c->done = 0;
+  if (r0 <= thresh) {
+    c->status = 8;
+    c->done = 1;
+  } else if (max_iter <= 0) {
+    c->status = 2;
+    c->done = 1;
+  }
+}
+
+__global__ void ctl_check_kernel(SolveCtl* c, const unsigned long long* slots, int refined) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (c->done) return;
+  const int it = c->iter + 1;
+  const double norm = __longlong_as_double((long long)slots[it]);
+  if (refined && !(norm < c->prev)) {
+    c->need_host = 1;
+    c->norm = norm;
+    c->done = 1;
+    return;
+  }
+  c->iter = it;
+  ctl_finish_iteration(c, norm);
+}
+
 // Arithmetic CoarseAverageFace by `ratio` from the finest level
 // (src/AMRNonLinearMultiCompOp.cpp:1450-1469): sub-faces summed lowest
 // transverse index fastest, times 1/ratio^(d-1).
@@ -373,6 +427,17 @@ hipError_t norm_max(const Geom& g, int ncomp, const T* x, unsigned long long* ou
 
 hipError_t zero_u64(unsigned long long* p, int n, hipStream_t s) {
   hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t ctl_init(SolveCtl* ctl, const unsigned long long* slots, int max_iter, double eps, double hang,
+                    double thresh, hipStream_t s) {
+  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(64), 0, s, ctl, slots, max_iter, eps, hang, thresh);
+  return hipGetLastError();
+}
+
+hipError_t ctl_check(SolveCtl* ctl, const unsigned long long* slots, int refined, hipStream_t s) {
+  hipLaunchKernelGGL(ctl_check_kernel, dim3(1), dim3(64), 0, s, ctl, slots, refined);
   return hipGetLastError();
 }
 

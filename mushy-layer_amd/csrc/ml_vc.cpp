@@ -13,6 +13,7 @@
 //   `bottom` times.
 // Tiled kernels serve the levels above `lc`; one single-workgroup kernel
 // runs the V-cycle from `lc` down.
+#include <algorithm>
 #include <cstring>
 #include <memory>
 
@@ -43,6 +44,7 @@ int vc_upload_desc(ml_vc_op op, const ml_solve_opts& o) {
   D.bc = op->bc;
   D.alpha = op->alpha;
   D.beta = op->beta;
+  D.skip = &op->ctl->done;
   for (int l = 0; l < D.nlev; ++l) {
     const ml::VCArgs<double> A = op->args(l);
     ml::VCLevel<double>& L = D.lev[l];
@@ -58,21 +60,26 @@ int vc_upload_desc(ml_vc_op op, const ml_solve_opts& o) {
 }
 
 // one V-cycle of the correction equation; on return the level-0 correction
-// is op->lev[0].e[*ecur0]
-int vc_vcycle(ml_vc_op op, const ml_solve_opts& o, int* ecur) {
+// is op->lev[0].e[ecur[0]]
+int vc_vcycle(ml_vc_op op, const ml_solve_opts& o, int* ecur, const int* skip) {
   ml_ctx ctx = op->ctx;
   hipStream_t s = ctx->stream;
   const int L = int(op->lev.size());
   const int lc = op->lc;
   auto E = [&](int l) { return op->lev[l].e[ecur[l]]; };
+  auto args = [&](int l) {
+    ml::VCArgs<double> A = op->args(l);
+    A.skip = skip;
+    return A;
+  };
   const int ltiles = lc < L ? lc : L - 1;
   if (lc >= L && L == 1) {
-    ML_HIP(ctx, ml::vc_relax<double>(op->args(0), op->lev[0].e, &ecur[0], op->lev[0].r, 2 * o.bottom, true,
-                                      nullptr, nullptr, s));
+    ML_HIP(ctx, ml::vc_relax<double>(args(0), op->lev[0].e, &ecur[0], op->lev[0].r, 2 * o.bottom, true, nullptr,
+                                      nullptr, s));
     return ML_OK;
   }
   for (int l = 0; l < ltiles; ++l) {
-    const ml::VCArgs<double> A = op->args(l);
+    const ml::VCArgs<double> A = args(l);
     ML_HIP(ctx, ml::vc_relax<double>(A, op->lev[l].e, &ecur[l], op->lev[l].r, 2 * o.nu1, true, nullptr, nullptr,
                                       s));
     ML_HIP(ctx, ml::vc_restrict_residual<double>(A, op->lev[l + 1].g, E(l), op->lev[l].r, op->lev[l + 1].r, s));
@@ -80,23 +87,22 @@ int vc_vcycle(ml_vc_op op, const ml_solve_opts& o, int* ecur) {
   if (lc < L) {
     ML_HIP(ctx, ml::vc_coarse_vcycle<double>(op->desc, op->dim, s));
   } else {
-    ML_HIP(ctx, ml::vc_relax<double>(op->args(L - 1), op->lev[L - 1].e, &ecur[L - 1], op->lev[L - 1].r,
-                                      2 * o.bottom, true, nullptr, nullptr, s));
+    ML_HIP(ctx, ml::vc_relax<double>(args(L - 1), op->lev[L - 1].e, &ecur[L - 1], op->lev[L - 1].r, 2 * o.bottom,
+                                      true, nullptr, nullptr, s));
   }
   for (int l = ltiles - 1; l >= 0; --l) {
     const ml::Geom gc = op->lev[l + 1].g;
-    ML_HIP(ctx, ml::vc_relax<double>(op->args(l), op->lev[l].e, &ecur[l], op->lev[l].r, 2 * o.nu2, false, &gc,
+    ML_HIP(ctx, ml::vc_relax<double>(args(l), op->lev[l].e, &ecur[l], op->lev[l].r, 2 * o.nu2, false, &gc,
                                       E(l + 1), s));
   }
   return ML_OK;
 }
 
-int vc_read_norm(ml_vc_op op, int slot, double* out) {
+int vc_read_ctl(ml_vc_op op, ml::SolveCtl* out) {
   ml_ctx ctx = op->ctx;
-  ML_HIP(ctx, hipMemcpyAsync(op->hslots + slot, op->slots + slot, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                             ctx->stream));
+  ML_HIP(ctx, hipMemcpyAsync(op->hctl, op->ctl, sizeof(ml::SolveCtl), hipMemcpyDeviceToHost, ctx->stream));
   ML_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  std::memcpy(out, op->hslots + slot, sizeof(double));
+  *out = *op->hctl;
   return ML_OK;
 }
 
@@ -113,35 +119,37 @@ int vc_solve(ml_vc_op op, double* phi, const double* rhs, const ml_solve_opts& o
   const size_t bytes0 = sizeof(double) * g0.ncell;
   ML_HIP(ctx, hipMemcpyAsync(op->alt0, phi, bytes0, hipMemcpyDeviceToDevice, s));
   double* const pb[2] = {phi, op->alt0};
-  int pcur = 0;
   std::vector<int> ecur(L, 0);
   ML_HIP(ctx, ml::zero_u64(op->slots, o.max_iter + 1, s));
-  const ml::VCArgs<double> A0 = op->args(0);
+  ml::VCArgs<double> A0 = op->args(0);
   ML_HIP(ctx, ml::vc_residual<double>(A0, phi, nullptr, nullptr, rhs, op->lev[0].r, op->slots, 0, s));
-  double r0 = 0;
-  if ((rc = vc_read_norm(op, 0, &r0))) return rc;
-  double norm = r0, prev = r0;
-  int iter = 0, status = 0;
-  if (r0 <= o.norm_thresh) status = 8;
-  while (status == 0 && iter < o.max_iter) {
-    if ((rc = vc_vcycle(op, o, ecur.data()))) return rc;
-    ++iter;
-    ML_HIP(ctx, ml::vc_residual<double>(A0, pb[pcur], op->lev[0].e[ecur[0]], pb[1 - pcur], rhs, op->lev[0].r,
-                                         op->slots + iter, 0, s));
-    pcur = 1 - pcur;
-    if ((rc = vc_read_norm(op, iter, &norm))) return rc;
-    if (norm <= o.eps * r0) status |= 1;
-    if (norm <= o.norm_thresh) status |= 8;
-    if (status == 0 && norm > (1.0 - o.hang) * prev) status |= 4;
-    prev = norm;
+  ML_HIP(ctx, ml::ctl_init(op->ctl, op->slots, o.max_iter, o.eps, o.hang, o.norm_thresh, s));
+  const int* skip = &op->ctl->done;
+  A0.skip = skip;
+  // iterations are enqueued in batches; the device stops them (SolveCtl)
+  int launched = 0;
+  int batch = std::max(1, std::min(op->batch, o.max_iter));
+  ml::SolveCtl c{};
+  for (;;) {
+    for (int b = 0; b < batch && launched < o.max_iter; ++b) {
+      if ((rc = vc_vcycle(op, o, ecur.data(), skip))) return rc;
+      ++launched;
+      const int pcur = (launched - 1) & 1;   // phi += e alternates the state buffer
+      ML_HIP(ctx, ml::vc_residual<double>(A0, pb[pcur], op->lev[0].e[ecur[0]], pb[1 - pcur], rhs, op->lev[0].r,
+                                           op->slots + launched, 0, s));
+      ML_HIP(ctx, ml::ctl_check(op->ctl, op->slots, 0, s));
+    }
+    if ((rc = vc_read_ctl(op, &c))) return rc;
+    if (c.done || launched >= o.max_iter) break;
+    batch = 2;
   }
-  if (status == 0) status = 2;
-  if (pcur) ML_HIP(ctx, hipMemcpyAsync(phi, op->alt0, bytes0, hipMemcpyDeviceToDevice, s));
+  op->batch = std::max(1, c.iter);
+  if (c.iter & 1) ML_HIP(ctx, hipMemcpyAsync(phi, op->alt0, bytes0, hipMemcpyDeviceToDevice, s));
   if (st) {
-    st->exit_status = status;
-    st->iterations = iter;
-    st->initial_residual = r0;
-    st->final_residual = norm;
+    st->exit_status = c.status != 0 ? c.status : 2;
+    st->iterations = c.iter;
+    st->initial_residual = c.r0;
+    st->final_residual = c.norm;
   }
   return ML_OK;
 }
@@ -212,6 +220,10 @@ int ml_vc_op_create(ml_ctx ctx, int dim, const int lo[], const int hi[], double 
   raw->desc = static_cast<ml::VCDesc<double>*>(p);
   if (hipHostMalloc(&p, sizeof(ml::VCDesc<double>), hipHostMallocDefault) != hipSuccess) return bail(ML_ENOMEM);
   raw->hdesc = static_cast<ml::VCDesc<double>*>(p);
+  if (hipMalloc(&p, sizeof(ml::SolveCtl)) != hipSuccess) return bail(ML_ENOMEM);
+  raw->ctl = static_cast<ml::SolveCtl*>(p);
+  if (hipHostMalloc(&p, sizeof(ml::SolveCtl), hipHostMallocDefault) != hipSuccess) return bail(ML_ENOMEM);
+  raw->hctl = static_cast<ml::SolveCtl*>(p);
   *out = op.release();
   return ML_OK;
 }
@@ -235,6 +247,8 @@ int ml_vc_op_destroy(ml_vc_op op) {
   if (op->hslots) hipHostFree(op->hslots);
   if (op->desc) hipFree(op->desc);
   if (op->hdesc) hipHostFree(op->hdesc);
+  if (op->ctl) hipFree(op->ctl);
+  if (op->hctl) hipHostFree(op->hctl);
   delete op;
   return ML_OK;
 }

@@ -79,6 +79,7 @@ __global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T*
                                                            T* __restrict__ out, const T* __restrict__ rhs, int npass,
                                                            int first, int zero_init, Geom gc,
                                                            const T* __restrict__ pc, int ntx, int nty) {
+  if (A.skip && *A.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2 * HMAX) * (TY + 2 * HMAX) * (DIM == 3 ? TZ + 2 * HMAX : 1);
   constexpr int KMAX = (EMAX + NT - 1) / NT;
@@ -139,6 +140,7 @@ __global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T*
 // kVCApply: out = L u;  kVCRestrict: out (coarse) = child sums of (rhs - L u)/2^d.
 template <typename T, int DIM, int TX, int TY, int TZ, int MODE>
 __global__ __launch_bounds__(kVCThreads) void vc_tile_kernel(VCArgs<T> A, VCTileIO<T> io) {
+  if (A.skip && *A.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2) * (TY + 2) * (DIM == 3 ? TZ + 2 : 1);
   constexpr int KMAX = (EMAX + kVCThreads - 1) / kVCThreads;
@@ -324,6 +326,7 @@ template <typename T, int DIM>
 __global__ __launch_bounds__(kVCWG) void vc_coarse_vcycle_kernel(const VCDesc<T>* __restrict__ Dp) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const VCDesc<T>& D = *Dp;
+  if (D.skip && *D.skip) return;
   T* su = reinterpret_cast<T*>(smem_raw);   // e of the level (kVCWGCells)
   T* sr = su + kVCWGCells;                  // r of the level
   const int l0 = D.l0, L = D.nlev;
