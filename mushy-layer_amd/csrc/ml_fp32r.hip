@@ -240,6 +240,21 @@ __global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, c
 #pragma unroll
     for (int m = 0; m < 4; ++m) sM[m * E + r.e] = L.M[r.q + m * g.ncell];
   }
+  // operator data of every position this thread may update, in registers
+  CellFaces<float, DIM> F[KMAX];
+  float rr[KMAX][2], aq[KMAX][2], inv[KMAX][4];
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    if (r.kind != 1 || r.dist > t.H - 1) continue;
+    load_faces<float, DIM>(g, L.b, r.i, r.j, r.k, F[kk]);
+    rr[kk][0] = L.r[r.q];
+    rr[kk][1] = L.r[r.q + g.ncell];
+    aq[kk][0] = L.a ? L.a[r.q] : 1.0f;
+    aq[kk][1] = L.a ? L.a[r.q + g.ncell] : 1.0f;
+    const float Mc[4] = {L.M[r.q], L.M[r.q + g.ncell], L.M[r.q + 2 * g.ncell], L.M[r.q + 3 * g.ncell]};
+    block_inverse<float, DIM>(F[kk], aq[kk], Mc, alpha, beta, dxinv, inv[kk]);
+  }
   __syncthreads();
   // u = M e at every staged position (wall ghosts by the homogeneous rule)
 #pragma unroll
@@ -264,19 +279,13 @@ __global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, c
     for (int kk = 0; kk < KMAX; ++kk) {
       const PosRec& r = R[kk];
       if (!(r.kind == 1 && r.dist <= gext - 1 && !((r.i + r.j + r.k + p) & 1))) continue;
-      CellFaces<float, DIM> F;
-      load_faces<float, DIM>(g, L.b, r.i, r.j, r.k, F);
-      const float rr[2] = {L.r[r.q], L.r[r.q + g.ncell]};
-      const float aq[2] = {L.a ? L.a[r.q] : 1.0f, L.a ? L.a[r.q + g.ncell] : 1.0f};
       const int e = r.e;
       const float Mc[4] = {sM[e], sM[E + e], sM[2 * E + e], sM[3 * E + e]};
-      float inv[4];
-      block_inverse<float, DIM>(F, aq, Mc, alpha, beta, dxinv, inv);
       float u0s[7], u1s[7];
       gather_lds<float, DIM>(su, e, t.EX, t.EXY, u0s);
       gather_lds<float, DIM>(su + E, e, t.EX, t.EXY, u1s);
       float e0 = se[e], e1 = se[E + e];
-      block_step<float, DIM>(F, aq, rr, inv, alpha, beta, dxinv, u0s, u1s, e0, e1);
+      block_step<float, DIM>(F[kk], aq[kk], rr[kk], inv[kk], alpha, beta, dxinv, u0s, u1s, e0, e1);
       se[e] = e0;
       se[E + e] = e1;
       const float u0 = Mc[0] * e0 + Mc[1] * e1, u1 = Mc[2] * e0 + Mc[3] * e1;
@@ -747,6 +756,14 @@ __global__ __launch_bounds__(kWGT) void lin_coarse_vcycle_kernel(const LinDesc* 
   }
 }
 
+__global__ void cast_many_kernel(CastJobs J) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= J.begin[J.n]) return;
+  int j = 0;
+  while (t >= J.begin[j + 1]) ++j;
+  J.dst[j][t - J.begin[j]] = float(J.src[j][t - J.begin[j]]);
+}
+
 __global__ void cast_kernel(long long n, const double* __restrict__ x, float* __restrict__ y) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     y[i] = float(x[i]);
@@ -869,6 +886,13 @@ hipError_t lin_coarse_vcycle(const LinDesc* dev_desc, int dim, hipStream_t s) {
     hipLaunchKernelGGL((lin_coarse_vcycle_kernel<2>), dim3(1), dim3(kWGT), lds, s, dev_desc);
   else
     hipLaunchKernelGGL((lin_coarse_vcycle_kernel<3>), dim3(1), dim3(kWGT), lds, s, dev_desc);
+  return hipGetLastError();
+}
+
+hipError_t cast_many(const CastJobs& J, hipStream_t s) {
+  const long long n = J.begin[J.n];
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(cast_many_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, J);
   return hipGetLastError();
 }
 

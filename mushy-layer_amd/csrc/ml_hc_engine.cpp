@@ -363,13 +363,22 @@ int HCEngine::solve_refined(double* phi, const double* rhs, const ml_solve_opts&
   if (o.max_iter + 1 > m_nslots) return fail(ctx, ML_EINVAL, "max_iter too large");
   if (o.nu1 < 1 || o.nu2 < 1 || o.bottom < 1)
     return fail(ctx, ML_EINVAL, "FP32-refined form needs nu1, nu2, bottom >= 1");
-  // FP32 copies of the face coefficients (and aCoef) of every level
+  // FP32 copies of the face coefficients of every level, in one launch
+  CastJobs J{};
+  long long total = 0;
   for (int l = 0; l < m_L; ++l) {
     const Geom& g = m_op->lev[l].g;
-    for (int d = 0; d < g.dim; ++d)
-      ML_HIP(ctx, cast_f64_f32(2 * g.nface[d], m_op->lev[l].b[d], const_cast<float*>(m_lin[l].b[d]), s));
     if (m_op->lev[l].a) return fail(ctx, ML_ENOTSUP, "FP32-refined form supports aCoef == 1 only");
+    for (int d = 0; d < g.dim; ++d) {
+      J.begin[J.n] = total;
+      J.src[J.n] = m_op->lev[l].b[d];
+      J.dst[J.n] = const_cast<float*>(m_lin[l].b[d]);
+      total += 2 * g.nface[d];
+      ++J.n;
+    }
   }
+  J.begin[J.n] = total;
+  ML_HIP(ctx, cast_many(J, s));
   ML_HIP(ctx, hipStreamSynchronize(s));   // the pinned descriptor may still be in flight
   LinDesc& D = *m_hlindesc;
   std::memset(&D, 0, sizeof(D));

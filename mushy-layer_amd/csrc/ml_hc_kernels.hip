@@ -136,6 +136,21 @@ __global__ __launch_bounds__(NT) void hc_relax_tile_kernel(HCArgs<T> A, TileIO<T
   // cells it updated (and of their wall ghosts): the other colour's phi_l is
   // unchanged, and in this pass a cell reads only its own and the other
   // colour's phi_l, so one barrier per pass suffices.
+  // operator data of every position this thread may update, in registers
+  CellFaces<T, DIM> F[KMAX];
+  T rr[KMAX][2], aa[KMAX][2], lam[KMAX][2];
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    if (r.kind != 1 || r.dist > t.H - 1) continue;
+    load_faces<T, DIM>(A, r.i, r.j, r.k, F[kk]);
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+      rr[kk][n] = io.rhs[r.q + n * g.ncell];
+      aa[kk][n] = A.a ? A.a[r.q + n * g.ncell] : T(1);
+      lam[kk][n] = lambda_of<T, DIM>(F[kk], n, aa[kk][n], A.alpha, A.beta, scale);
+    }
+  }
   for (int p = 0; p < io.npass; ++p) {
     const int gext = t.H - p;
     const int colour = (io.first + p) & 1;
@@ -143,20 +158,15 @@ __global__ __launch_bounds__(NT) void hc_relax_tile_kernel(HCArgs<T> A, TileIO<T
     for (int kk = 0; kk < KMAX; ++kk) {
       const PosRec& r = R[kk];
       if (!(r.kind == 1 && r.dist <= gext - 1 && !((r.i + r.j + r.k + colour) & 1))) continue;
-      CellFaces<T, DIM> F;
-      load_faces<T, DIM>(A, r.i, r.j, r.k, F);
       const int e = r.e;
       T nv[2];
 #pragma unroll
       for (int n = 0; n < 2; ++n) {
-        const T rr = io.rhs[r.q + n * g.ncell];
-        const T aa = A.a ? A.a[r.q + n * g.ncell] : T(1);
         T st[7];
         gather_lds<T, DIM>(sphl + n * E, e, t.EX, t.EXY, st);
         const T ph = sphi[n * E + e];
-        const T lof = lof_of<T, DIM>(F, n, st, aa, ph, A.alpha, A.beta, dxinv);
-        const T lam = lambda_of<T, DIM>(F, n, aa, A.alpha, A.beta, scale);
-        nv[n] = ph - lam * (lof - rr);
+        const T lof = lof_of<T, DIM>(F[kk], n, st, aa[kk][n], ph, A.alpha, A.beta, dxinv);
+        nv[n] = ph - lam[kk][n] * (lof - rr[kk][n]);
       }
       sphi[e] = nv[0];
       sphi[E + e] = nv[1];

@@ -190,6 +190,14 @@ hipError_t lin_restrict(const LinLevel<float>& L, const LinLevel<float>& C, cons
 hipError_t lin_coarse_vcycle(const LinDesc* dev_desc, int dim, hipStream_t s);
 // cast of FP64 face coefficients to FP32
 hipError_t cast_f64_f32(long long n, const double* x, float* y, hipStream_t s);
+// several casts in one launch
+struct CastJobs {
+  int n;
+  long long begin[3 * kMaxLevels + 1];
+  const double* src[3 * kMaxLevels];
+  float* dst[3 * kMaxLevels];
+};
+hipError_t cast_many(const CastJobs& J, hipStream_t s);
 
 // ---- nonlinear (H, Theta) operator (ml_hc_kernels.hip)
 // Tiled red/black relaxation.  Every launch reads buf[*cur] and writes

@@ -285,6 +285,46 @@ __global__ __launch_bounds__(256) void coarsen_hierarchy_kernel(Geom gf, int nco
   }
 }
 
+// Heavy cell jobs of the hierarchy (large ratios): one workgroup per coarse
+// cell stages the fine block through LDS in Fortran order and one thread
+// accumulates it in that order.
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void coarsen_cells_block_kernel(Geom gf, int ncomp, CoarsenJobs<T> J,
+                                                                  const T* __restrict__ af) {
+  constexpr int kBuf = 2048;
+  __shared__ T buf[kBuf];
+  const long long t = blockIdx.x;
+  if (t >= J.begin[J.njobs]) return;
+  int jb = 0;
+  while (t >= J.begin[jb + 1]) ++jb;
+  const long long item = t - J.begin[jb];
+  const int ratio = 1 << J.level[jb];
+  const Geom gc = coarsen_geom(gf, ratio);
+  const int rz = DIM == 3 ? ratio : 1;
+  const T scale = T(1) / T(ratio * ratio * rz);
+  const long long N = gc.nvalid();
+  const int n = int(item / N);
+  int I, Jj, K;
+  unflatten(item % N, gc.n[0], gc.n[1], I, Jj, K);
+  const int nrows = ratio * rz;   // rows (jj, kk), jj fastest
+  const int rows_per_chunk = kBuf / ratio > 0 ? kBuf / ratio : 1;
+  T sum = T(0);
+  for (int r0 = 0; r0 < nrows; r0 += rows_per_chunk) {
+    const int nr = (nrows - r0) < rows_per_chunk ? (nrows - r0) : rows_per_chunk;
+    const int cnt = nr * ratio;
+    for (int u = threadIdx.x; u < cnt; u += blockDim.x) {
+      const int row = r0 + u / ratio, ii = u % ratio;
+      const int jj = row % ratio, kk = row / ratio;
+      buf[u] = af[gf.ci(I * ratio + ii, Jj * ratio + jj, K * rz + kk) + n * gf.ncell];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int u = 0; u < cnt; ++u) sum = sum + buf[u];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) J.dst[jb][gc.ci(I, Jj, K) + n * gc.ncell] = sum * scale;
+}
+
 // CALCULATEPOROSITY_MULTICOMP over valid cells (src/EnthalpyVariablesF.ChF:56-94).
 __global__ void porosity_kernel(Geom g, ClosureP<double> cp, const double* __restrict__ phi,
                                 double* __restrict__ chi) {
@@ -457,14 +497,26 @@ hipError_t average_faces(const Geom& gf, int ncomp, const T* const* bf, int rati
 template <typename T>
 hipError_t coarsen_hierarchy(const Geom& gf, int ncomp, int nlev, const T* const* bf, T* const* const* bc, const T* af,
                              T* const* ac, hipStream_t s, bool faces) {
-  CoarsenJobs<T> J{};
-  long long total = 0;
-  // heaviest (coarsest) levels first so their long sums start first
+  CoarsenJobs<T> J{}, H{};
+  long long total = 0, htotal = 0;
+  // cell averages over blocks of at least kHeavy fine cells go to the
+  // workgroup-per-cell kernel
+  constexpr long long kHeavy = 64;
   for (int l = nlev - 1; l >= 1; --l) {
     const Geom gc = coarsen_geom(gf, 1 << l);
+    const long long block = (1LL << l) * (1LL << l) * (gf.dim == 3 ? (1LL << l) : 1);
+    if (af && block >= kHeavy) {
+      if (H.njobs >= kMaxCoarsenJobs) return hipErrorInvalidValue;
+      H.begin[H.njobs] = htotal;
+      H.level[H.njobs] = short(l);
+      H.kind[H.njobs] = 3;
+      H.dst[H.njobs] = ac[l];
+      htotal += gc.nvalid() * ncomp;
+      ++H.njobs;
+    }
     for (int kind = 0; kind < 4; ++kind) {
       if (kind < 3 && (!faces || kind >= gf.dim)) continue;
-      if (kind == 3 && !af) continue;
+      if (kind == 3 && (!af || block >= kHeavy)) continue;
       if (J.njobs >= kMaxCoarsenJobs) return hipErrorInvalidValue;
       J.begin[J.njobs] = total;
       J.level[J.njobs] = short(l);
@@ -475,6 +527,15 @@ hipError_t coarsen_hierarchy(const Geom& gf, int ncomp, int nlev, const T* const
     }
   }
   J.begin[J.njobs] = total;
+  H.begin[H.njobs] = htotal;
+  if (htotal > 0) {
+    if (gf.dim == 2)
+      hipLaunchKernelGGL((coarsen_cells_block_kernel<T, 2>), dim3(unsigned(htotal)), dim3(256), 0, s, gf, ncomp, H, af);
+    else
+      hipLaunchKernelGGL((coarsen_cells_block_kernel<T, 3>), dim3(unsigned(htotal)), dim3(256), 0, s, gf, ncomp, H, af);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   if (total == 0) return hipSuccess;
   const unsigned blocks = unsigned((total + 255) / 256);
   if (gf.dim == 2)

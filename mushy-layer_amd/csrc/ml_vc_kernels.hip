@@ -73,7 +73,10 @@ __device__ inline T vc_lof(const Faces1<T, DIM>& F, const T* p, T aq, bool hasc,
   return lof - beta * s * dxinv;
 }
 
-// Relaxation of L e = r on one tile with a halo of npass cells.
+// Relaxation of L e = r on one tile with a halo of npass cells.  The
+// coefficients, lambda and rhs of every position a thread may update are
+// loaded once into registers; a pass updates one colour and refreshes the
+// homogeneous wall ghosts of the cells it updated, so one barrier per pass.
 template <typename T, int DIM, int TX, int TY, int TZ, int HMAX, int NT>
 __global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T* __restrict__ in,
                                                            T* __restrict__ out, const T* __restrict__ rhs, int npass,
@@ -91,6 +94,8 @@ __global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T*
   PosRec R[KMAX];
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) pos_record<DIM, TX, TY, TZ>(g, t, threadIdx.x + kk * NT, R[kk]);
+  Faces1<T, DIM> F[KMAX];
+  T aq[KMAX], cv[KMAX], lam[KMAX], rr[KMAX];
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) {
     const PosRec& r = R[kk];
@@ -98,32 +103,46 @@ __global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T*
     T v = zero_init ? T(0) : in[r.q];
     if (pc) v = v + pc[gc.ci(r.i / 2, r.j / 2, DIM == 3 ? r.k / 2 : 0)];
     su[r.e] = v;
+    if (r.dist > t.H - 1) continue;
+    load_faces1<T, DIM>(A, r.i, r.j, r.k, F[kk]);
+    aq[kk] = A.a ? A.a[r.q] : T(1);
+    cv[kk] = A.c ? A.c[r.q] : T(0);
+    rr[kk] = rhs[r.q];
+    lam[kk] = vc_lambda<T, DIM>(F[kk], aq[kk], A.c != nullptr, cv[kk], A.alpha, A.beta, dxinv);
+  }
+  __syncthreads();
+  // homogeneous wall ghosts of the staged state
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    if (r.kind != 2) continue;
+    su[r.e] = bc_ghost<T>(A.bc.type[r.gd][r.gs][0], T(0), su[r.eint], r.gs == 0 ? -1 : 1, T(g.dx));
   }
   __syncthreads();
   for (int p = 0; p < npass; ++p) {
     const int gext = t.H - p;
     const int colour = (first + p) & 1;
-    // homogeneous wall ghosts from the current interior neighbour
-#pragma unroll
-    for (int kk = 0; kk < KMAX; ++kk) {
-      const PosRec& r = R[kk];
-      if (r.kind != 2 || r.dist > gext) continue;
-      su[r.e] = bc_ghost<T>(A.bc.type[r.gd][r.gs][0], T(0), su[r.eint], r.gs == 0 ? -1 : 1, T(g.dx));
-    }
-    __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < KMAX; ++kk) {
       const PosRec& r = R[kk];
       if (!(r.kind == 1 && r.dist <= gext - 1 && !((r.i + r.j + r.k + colour) & 1))) continue;
-      Faces1<T, DIM> F;
-      load_faces1<T, DIM>(A, r.i, r.j, r.k, F);
-      const T aq = A.a ? A.a[r.q] : T(1);
-      const T cv = A.c ? A.c[r.q] : T(0);
       T st[7];
       gather_lds<T, DIM>(su, r.e, t.EX, t.EXY, st);
-      const T lof = vc_lof<T, DIM>(F, st, aq, A.c != nullptr, cv, A.alpha, A.beta, dxinv);
-      const T lam = vc_lambda<T, DIM>(F, aq, A.c != nullptr, cv, A.alpha, A.beta, dxinv);
-      su[r.e] = st[0] - lam * (lof - rhs[r.q]);
+      const T lof = vc_lof<T, DIM>(F[kk], st, aq[kk], A.c != nullptr, cv[kk], A.alpha, A.beta, dxinv);
+      const T nv = st[0] - lam[kk] * (lof - rr[kk]);
+      su[r.e] = nv;
+      // wall ghosts of this cell (it is their interior neighbour)
+      const int iv[3] = {r.i, r.j, r.k};
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        if (g.periodic[d]) continue;
+        const int stride = d == 0 ? 1 : (d == 1 ? t.EX : t.EXY);
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) {
+          if (iv[d] != (sd ? g.n[d] - 1 : 0)) continue;
+          su[sd ? r.e + stride : r.e - stride] = bc_ghost<T>(A.bc.type[d][sd][0], T(0), nv, sd ? 1 : -1, T(g.dx));
+        }
+      }
     }
     __syncthreads();
   }
