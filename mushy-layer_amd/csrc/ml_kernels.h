@@ -258,15 +258,11 @@ hipError_t ctl_init(SolveCtl* ctl, const unsigned long long* slots, int max_iter
 // stopping rule; with refined != 0 a norm that does not decrease is not
 // accepted but handed to the host (need_host, done)
 hipError_t ctl_check(SolveCtl* ctl, const unsigned long long* slots, int refined, hipStream_t s);
-template <typename T>
-hipError_t average_faces(const Geom& gf, int ncomp, const T* const* bf, int ratio, T* const* bc, hipStream_t s);
-template <typename T>
-hipError_t average_cells(const Geom& gf, int ncomp, const T* af, int ratio, T* ac, hipStream_t s);
 
-// The MG coefficient hierarchy in one launch: for every level l = 1..nlev-1,
-// the arithmetic average by ratio 2^l of the finest faces bf[d] into bc[l][d]
-// and (when af != nullptr) of the finest cell field af into ac[l] (same sums,
-// in the same order, as average_faces / average_cells).
+// The MG coefficient hierarchy: for every level l = 1..nlev-1, the
+// arithmetic average by ratio 2^l of the finest faces bf[d] into bc[l][d]
+// and (when af != nullptr) of the finest cell field af into ac[l], in the
+// summation order of the CPU restatement (mlr_average_faces/cells).
 constexpr int kMaxCoarsenJobs = 96;
 template <typename T>
 struct CoarsenJobs {

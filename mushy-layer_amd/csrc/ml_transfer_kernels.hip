@@ -157,87 +157,12 @@ __global__ void ctl_check_kernel(SolveCtl* c, const unsigned long long* slots, i
   ctl_finish_iteration(c, norm);
 }
 
-// Arithmetic CoarseAverageFace by `ratio` from the finest level
-// (src/AMRNonLinearMultiCompOp.cpp:1450-1469): sub-faces summed lowest
-// transverse index fastest, times 1/ratio^(d-1).
-template <typename T, int DIM>
-__global__ void average_faces_kernel(Geom gf, Geom gc, int ncomp, int d, int ratio, const T* __restrict__ bf,
-                                     T* __restrict__ bc) {
-  int r[3] = {ratio, ratio, DIM == 3 ? ratio : 1};
-  r[d] = 1;
-  const T scale = T(1) / T(r[0] * r[1] * r[2]);
-  const long long N = gc.nface[d];
-  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N * ncomp;
-       c += (long long)gridDim.x * blockDim.x) {
-    const int n = int(c / N);
-    int I, J, K;
-    unflatten(c % N, int(gc.fn[d][0]), int(gc.fn[d][1]), I, J, K);
-    const int base[3] = {I * (d == 0 ? ratio : r[0]), J * (d == 1 ? ratio : r[1]), K * (d == 2 ? ratio : r[2])};
-    T s = T(0);
-    for (int kk = 0; kk < r[2]; ++kk)
-      for (int jj = 0; jj < r[1]; ++jj)
-        for (int ii = 0; ii < r[0]; ++ii)
-          s = s + bf[gf.fi(d, base[0] + ii, base[1] + jj, base[2] + kk) + n * gf.nface[d]];
-    bc[gc.fi(d, I, J, K) + n * gc.nface[d]] = s * scale;
-  }
-}
-
-template <typename T, int DIM>
-__global__ void average_cells_kernel(Geom gf, Geom gc, int ncomp, int ratio, const T* __restrict__ af,
-                                     T* __restrict__ ac) {
-  const int rz = DIM == 3 ? ratio : 1;
-  const T scale = T(1) / T(ratio * ratio * rz);
-  const long long N = gc.nvalid();
-  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N * ncomp;
-       c += (long long)gridDim.x * blockDim.x) {
-    const int n = int(c / N);
-    int I, J, K;
-    unflatten(c % N, gc.n[0], gc.n[1], I, J, K);
-    T s = T(0);
-    for (int kk = 0; kk < rz; ++kk)
-      for (int jj = 0; jj < ratio; ++jj)
-        for (int ii = 0; ii < ratio; ++ii)
-          s = s + af[gf.ci(I * ratio + ii, J * ratio + jj, K * rz + kk) + n * gf.ncell];
-    ac[gc.ci(I, J, K) + n * gc.ncell] = s * scale;
-  }
-}
-
-// The same average for large ratios: one workgroup per coarse cell stages
-// the fine block through LDS, row by row in Fortran order, and one thread
-// accumulates it in that order (the sum is sequential by specification; the
-// workgroup hides the load latency).
-template <typename T, int DIM>
-__global__ __launch_bounds__(256) void average_cells_block_kernel(Geom gf, Geom gc, int ncomp, int ratio,
-                                                                  const T* __restrict__ af, T* __restrict__ ac) {
-  constexpr int kBuf = 2048;
-  __shared__ T buf[kBuf];
-  const int rz = DIM == 3 ? ratio : 1;
-  const T scale = T(1) / T(ratio * ratio * rz);
-  const long long N = gc.nvalid();
-  const long long cidx = blockIdx.x;
-  if (cidx >= N * ncomp) return;
-  const int n = int(cidx / N);
-  int I, J, K;
-  unflatten(cidx % N, gc.n[0], gc.n[1], I, J, K);
-  const int nrows = ratio * rz;                 // rows (jj, kk), jj fastest
-  const int rows_per_chunk = kBuf / ratio > 0 ? kBuf / ratio : 1;
-  T s = T(0);
-  for (int r0 = 0; r0 < nrows; r0 += rows_per_chunk) {
-    const int nr = (nrows - r0) < rows_per_chunk ? (nrows - r0) : rows_per_chunk;
-    const int cnt = nr * ratio;
-    for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-      const int row = r0 + t / ratio, ii = t % ratio;
-      const int jj = row % ratio, kk = row / ratio;
-      buf[t] = af[gf.ci(I * ratio + ii, J * ratio + jj, K * rz + kk) + n * gf.ncell];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-      for (int t = 0; t < cnt; ++t) s = s + buf[t];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) ac[gc.ci(I, J, K) + n * gc.ncell] = s * scale;
-}
-
+// MG coefficient hierarchy (CoarseAverage / CoarseAverageFace from the
+// finest level, MGnewOp src/AMRNonLinearMultiCompOp.cpp:1423-1494,
+// src/DarcyBrinkmanOp.cpp:977-1008).  Faces: the ratio^(d-1) fine sub-faces
+// summed lowest transverse index fastest, times 1/ratio^(d-1).  Cells: each
+// fine row (fixed j, k) summed i fastest, the row sums added in (j, k)
+// order, times 1/ratio^d.
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void coarsen_hierarchy_kernel(Geom gf, int ncomp, CoarsenJobs<T> J, const T* bf0,
                                                                 const T* bf1, const T* bf2, const T* af) {
@@ -278,21 +203,23 @@ __global__ __launch_bounds__(256) void coarsen_hierarchy_kernel(Geom gf, int nco
     for (int kk = 0; kk < rz; ++kk)
       for (int jj = 0; jj < ratio; ++jj) {
         const T* row = af + gf.ci(I * ratio, Jj * ratio + jj, K * rz + kk) + n * gf.ncell;
+        T rs = T(0);
 #pragma unroll 8
-        for (int ii = 0; ii < ratio; ++ii) sum = sum + row[ii];
+        for (int ii = 0; ii < ratio; ++ii) rs = rs + row[ii];
+        sum = sum + rs;
       }
     J.dst[jb][gc.ci(I, Jj, K) + n * gc.ncell] = sum * scale;
   }
 }
 
 // Heavy cell jobs of the hierarchy (large ratios): one workgroup per coarse
-// cell stages the fine block through LDS in Fortran order and one thread
-// accumulates it in that order.
+// cell; each thread sums fine rows (i fastest) into LDS, then one thread
+// adds the row sums in (j, k) order.
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void coarsen_cells_block_kernel(Geom gf, int ncomp, CoarsenJobs<T> J,
                                                                   const T* __restrict__ af) {
-  constexpr int kBuf = 2048;
-  __shared__ T buf[kBuf];
+  constexpr int kRows = 2048;
+  __shared__ T rows[kRows];
   const long long t = blockIdx.x;
   if (t >= J.begin[J.njobs]) return;
   int jb = 0;
@@ -307,19 +234,19 @@ __global__ __launch_bounds__(256) void coarsen_cells_block_kernel(Geom gf, int n
   int I, Jj, K;
   unflatten(item % N, gc.n[0], gc.n[1], I, Jj, K);
   const int nrows = ratio * rz;   // rows (jj, kk), jj fastest
-  const int rows_per_chunk = kBuf / ratio > 0 ? kBuf / ratio : 1;
   T sum = T(0);
-  for (int r0 = 0; r0 < nrows; r0 += rows_per_chunk) {
-    const int nr = (nrows - r0) < rows_per_chunk ? (nrows - r0) : rows_per_chunk;
-    const int cnt = nr * ratio;
-    for (int u = threadIdx.x; u < cnt; u += blockDim.x) {
-      const int row = r0 + u / ratio, ii = u % ratio;
-      const int jj = row % ratio, kk = row / ratio;
-      buf[u] = af[gf.ci(I * ratio + ii, Jj * ratio + jj, K * rz + kk) + n * gf.ncell];
+  for (int r0 = 0; r0 < nrows; r0 += kRows) {
+    const int nr = (nrows - r0) < kRows ? (nrows - r0) : kRows;
+    for (int u = threadIdx.x; u < nr; u += blockDim.x) {
+      const int row = r0 + u, jj = row % ratio, kk = row / ratio;
+      const T* src = af + gf.ci(I * ratio, Jj * ratio + jj, K * rz + kk) + n * gf.ncell;
+      T rs = T(0);
+      for (int ii = 0; ii < ratio; ++ii) rs = rs + src[ii];
+      rows[u] = rs;
     }
     __syncthreads();
     if (threadIdx.x == 0)
-      for (int u = 0; u < cnt; ++u) sum = sum + buf[u];
+      for (int u = 0; u < nr; ++u) sum = sum + rows[u];
     __syncthreads();
   }
   if (threadIdx.x == 0) J.dst[jb][gc.ci(I, Jj, K) + n * gc.ncell] = sum * scale;
@@ -482,19 +409,6 @@ hipError_t ctl_check(SolveCtl* ctl, const unsigned long long* slots, int refined
 }
 
 template <typename T>
-hipError_t average_faces(const Geom& gf, int ncomp, const T* const* bf, int ratio, T* const* bc, hipStream_t s) {
-  Geom gc = coarsen_geom(gf, ratio);
-  for (int d = 0; d < gf.dim; ++d) {
-    const int b = grid_for(gc.nface[d] * ncomp);
-    if (gf.dim == 2)
-      hipLaunchKernelGGL((average_faces_kernel<T, 2>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, d, ratio, bf[d], bc[d]);
-    else
-      hipLaunchKernelGGL((average_faces_kernel<T, 3>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, d, ratio, bf[d], bc[d]);
-  }
-  return hipGetLastError();
-}
-
-template <typename T>
 hipError_t coarsen_hierarchy(const Geom& gf, int ncomp, int nlev, const T* const* bf, T* const* const* bc, const T* af,
                              T* const* ac, hipStream_t s, bool faces) {
   CoarsenJobs<T> J{}, H{};
@@ -547,28 +461,6 @@ hipError_t coarsen_hierarchy(const Geom& gf, int ncomp, int nlev, const T* const
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t average_cells(const Geom& gf, int ncomp, const T* af, int ratio, T* ac, hipStream_t s) {
-  Geom gc = coarsen_geom(gf, ratio);
-  const long long block = (long long)ratio * ratio * (gf.dim == 3 ? ratio : 1);
-  if (block >= 64) {
-    const long long nb = gc.nvalid() * ncomp;
-    if (gf.dim == 2)
-      hipLaunchKernelGGL((average_cells_block_kernel<T, 2>), dim3(unsigned(nb)), dim3(256), 0, s, gf, gc, ncomp, ratio,
-                         af, ac);
-    else
-      hipLaunchKernelGGL((average_cells_block_kernel<T, 3>), dim3(unsigned(nb)), dim3(256), 0, s, gf, gc, ncomp, ratio,
-                         af, ac);
-    return hipGetLastError();
-  }
-  const int b = grid_for(gc.nvalid() * ncomp);
-  if (gf.dim == 2)
-    hipLaunchKernelGGL((average_cells_kernel<T, 2>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, ratio, af, ac);
-  else
-    hipLaunchKernelGGL((average_cells_kernel<T, 3>), dim3(b), dim3(256), 0, s, gf, gc, ncomp, ratio, af, ac);
-  return hipGetLastError();
-}
-
 hipError_t porosity_field(const Geom& g, const ClosureP<double>& cp, const double* phi, double* chi,
                           hipStream_t s) {
   hipLaunchKernelGGL(porosity_kernel, dim3(grid_for(g.nvalid())), dim3(256), 0, s, g, cp, phi, chi);
@@ -614,8 +506,6 @@ template hipError_t restrict_r<double>(const Geom&, int, const double*, double*,
 template hipError_t prolong_linear<double>(const Geom&, int, double*, const double*, const double*, hipStream_t);
 template hipError_t prolong_constant<double>(const Geom&, int, double*, const double*, hipStream_t);
 template hipError_t norm_max<double>(const Geom&, int, const double*, unsigned long long*, hipStream_t);
-template hipError_t average_faces<double>(const Geom&, int, const double* const*, int, double* const*, hipStream_t);
-template hipError_t average_cells<double>(const Geom&, int, const double*, int, double*, hipStream_t);
 template hipError_t coarsen_hierarchy<double>(const Geom&, int, int, const double* const*, double* const* const*,
                                               const double*, double* const*, hipStream_t, bool);
 template hipError_t be_rhs<double>(const Geom&, int, double*, const double*, const double*, double, const double*,

@@ -261,8 +261,10 @@ void mlr_prolong_constant(const mlr_grid* gf, int ncomp, double* phif,
 }
 
 /* Arithmetic CoarseAverage of a cell field by `ratio` (used for aCoef at
- * src/AMRNonLinearMultiCompOp.cpp:1448-1458; build spec for the order: fine
- * cells summed i fastest, then multiplied by 1/ratio^d). */
+ * src/AMRNonLinearMultiCompOp.cpp:1448-1458 and for aCoef/cCoef at
+ * src/DarcyBrinkmanOp.cpp:977-1008).  Build spec for the order: each row of
+ * fine cells (fixed j, k) is summed i fastest, the row sums are added in
+ * (j, k) order, then multiplied by 1/ratio^d. */
 void mlr_average_cells(const mlr_grid* gf, int ncomp, const double* af,
                        int ratio, double* ac) {
   mlr_grid gc;
@@ -278,9 +280,12 @@ void mlr_average_cells(const mlr_grid* gf, int ncomp, const double* af,
         for (int I = 0; I < gc.n[0]; ++I) {
           double s = 0.0;
           for (int kk = 0; kk < rz; ++kk)
-            for (int jj = 0; jj < ratio; ++jj)
+            for (int jj = 0; jj < ratio; ++jj) {
+              double rs = 0.0;
               for (int ii = 0; ii < ratio; ++ii)
-                s = s + af[mlr_ci(&Lf, I * ratio + ii, J * ratio + jj, K * rz + kk) + n * Lf.ncell];
+                rs = rs + af[mlr_ci(&Lf, I * ratio + ii, J * ratio + jj, K * rz + kk) + n * Lf.ncell];
+              s = s + rs;
+            }
           ac[mlr_ci(&Lc, I, J, K) + n * Lc.ncell] = s * scale;
         }
 }
