@@ -210,14 +210,17 @@ template <int DIM, int TX, int TY, int TZ, int HMAX, int NT>
 __global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, const float* __restrict__ ein,
                                                             float* __restrict__ eout, BCDesc bc, float alpha,
                                                             float beta, int npass, int zero_init, Geom gc,
-                                                            const float* __restrict__ pc, int ntx, int nty) {
+                                                            const float* __restrict__ pc, int ntx, int nty, Geom gr,
+                                                            float* __restrict__ rc, float* __restrict__ Mc_out) {
   if (L.skip && *L.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2 * HMAX) * (TY + 2 * HMAX) * (DIM == 3 ? TZ + 2 * HMAX : 1);
   constexpr int KMAX = (EMAX + NT - 1) / NT;
   const Geom& g = L.g;
   Tile<DIM, TX, TY, TZ> t;
-  t.init(npass, ntx, nty);
+  // with a restriction to produce, one more halo cell keeps the core's
+  // neighbours exact after the last pass
+  t.init(npass + (rc ? 1 : 0), ntx, nty);
   const int E = t.E;
   float* se = reinterpret_cast<float*>(smem_raw);
   float* su = se + 2 * E;
@@ -313,6 +316,50 @@ __global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, c
     if (!r.owner) continue;
     eout[r.q] = se[r.e];
     eout[r.q + g.ncell] = se[E + r.e];
+  }
+  if (!rc) return;
+  // r_c = R(r - J e), M_c = R(M) of the relaxed correction (the restriction
+  // of lin_restrict_kernel on the tile's own cells)
+  constexpr int NC = TX * TY * (DIM == 3 ? TZ : 1);
+  float* sres = sM + 4 * E;
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    if (!r.owner) continue;
+    int ex, ey, ez;
+    t.coords(r.e, ex, ey, ez);
+    const int c = (ex - t.H) + TX * ((ey - t.H) + TY * (DIM == 3 ? ez - t.H : 0));
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+      float us[7];
+      gather_lds<float, DIM>(su + n * E, r.e, t.EX, t.EXY, us);
+      const float je = lof_of<float, DIM>(F[kk], n, us, aq[kk][n], se[n * E + r.e], alpha, beta, dxinv);
+      sres[n * NC + c] = rr[kk][n] - je;
+    }
+  }
+  __syncthreads();
+  constexpr int CX = TX / 2, CY = TY / 2, CZ = DIM == 3 ? TZ / 2 : 1;
+  const float w = DIM == 3 ? 0.125f : 0.25f;
+  for (int c = threadIdx.x; c < CX * CY * CZ; c += NT) {
+    const int cx = c % CX, cr = c / CX, cy = cr % CY, cz = cr / CY;
+    const int I = t.x0 / 2 + cx, J = t.y0 / 2 + cy, K = DIM == 3 ? t.z0 / 2 + cz : 0;
+    if (I >= gr.n[0] || J >= gr.n[1] || (DIM == 3 && K >= gr.n[2])) continue;
+    const long long qc = gr.ci(I, J, K);
+    float s0 = 0.f, s1 = 0.f, m[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int kk = 0; kk < (DIM == 3 ? 2 : 1); ++kk)
+      for (int jj = 0; jj < 2; ++jj)
+        for (int ii = 0; ii < 2; ++ii) {
+          const int f = (2 * cx + ii) + TX * ((2 * cy + jj) + TY * (DIM == 3 ? 2 * cz + kk : 0));
+          s0 += sres[f];
+          s1 += sres[NC + f];
+          const int e = (2 * cx + ii + t.H) + t.EX * (2 * cy + jj + t.H) + t.EXY * (DIM == 3 ? 2 * cz + kk + t.H : 0);
+#pragma unroll
+          for (int mm = 0; mm < 4; ++mm) m[mm] += sM[mm * E + e];
+        }
+    rc[qc] = s0 * w;
+    rc[qc + gr.ncell] = s1 * w;
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm) Mc_out[qc + mm * gr.ncell] = m[mm] * w;
   }
 }
 
@@ -818,15 +865,24 @@ hipError_t hc_residual_jacobian(const HCArgs<double>& A, const double* phi, doub
   return hipGetLastError();
 }
 
-hipError_t lin_relax(const LinLevel<float>& L, float* const ebuf[2], int* cur, const BCDesc& bc, float alpha,
-                     float beta, int npass, bool zero_init, const Geom* gc, const float* pc, hipStream_t s) {
+namespace {
+
+// fused: the restriction into C (rc != nullptr) needs npass + 1 <= the
+// kernel's halo bound, so that one launch does all passes
+hipError_t lin_relax_impl(const LinLevel<float>& L, float* const ebuf[2], int* cur, const BCDesc& bc, float alpha,
+                          float beta, int npass, bool zero_init, const Geom* gc, const float* pc,
+                          const LinLevel<float>* C, hipStream_t s) {
   int ntx, nty;
   const Geom gcc = gc ? *gc : L.g;
+  const Geom grr = C ? C->g : L.g;
+  float* rc = C ? C->r : nullptr;
+  float* Mc = C ? C->M : nullptr;
+  const int hx = C ? 1 : 0;
   int done = 0;
-  // npass == 0 still runs one launch when there is a zero start or a
-  // prolongation to apply
+  // npass == 0 still runs one launch when there is a zero start, a
+  // prolongation or a restriction to apply
   bool first = true;
-  while (done < npass || (first && npass == 0 && (zero_init || pc))) {
+  while (done < npass || (first && npass == 0 && (zero_init || pc || C))) {
     first = false;
     const float* ein = ebuf[*cur];
     float* eout = ebuf[1 - *cur];
@@ -835,23 +891,25 @@ hipError_t lin_relax(const LinLevel<float>& L, float* const ebuf[2], int* cur, c
     if (L.g.dim == 2) {
       using D = LinTile<2>;
       const int nt = lin_tiles<2>(L.g, ntx, nty);
-      const int pmax = nt <= 1024 ? 8 : 2;
+      const int pmax = (nt <= 1024 || (C && npass + hx > 2)) ? 8 : 2;
       const int p = (npass - done) < pmax ? (npass - done) : pmax;
-      const size_t lds = 8 * lin_extent<2>(p) * sizeof(float);
+      if (C && p + hx > pmax) return hipErrorInvalidValue;
+      const size_t lds = (8 * lin_extent<2>(p + hx) + (C ? 2 * D::TX * D::TY : 0)) * sizeof(float);
       if (pmax == 8)
         hipLaunchKernelGGL((lin_relax_tile_kernel<2, D::TX, D::TY, D::TZ, 8, 512>), dim3(nt), dim3(512), lds, s, L,
-                           ein, eout, bc, alpha, beta, p, zi, gcc, p0, ntx, nty);
+                           ein, eout, bc, alpha, beta, p, zi, gcc, p0, ntx, nty, grr, rc, Mc);
       else
         hipLaunchKernelGGL((lin_relax_tile_kernel<2, D::TX, D::TY, D::TZ, 2, 256>), dim3(nt), dim3(256), lds, s, L,
-                           ein, eout, bc, alpha, beta, p, zi, gcc, p0, ntx, nty);
+                           ein, eout, bc, alpha, beta, p, zi, gcc, p0, ntx, nty, grr, rc, Mc);
       done += p;
     } else {
       using D = LinTile<3>;
       const int nt = lin_tiles<3>(L.g, ntx, nty);
       const int p = (npass - done) < 2 ? (npass - done) : 2;
-      const size_t lds = 8 * lin_extent<3>(p) * sizeof(float);
+      if (C && p + hx > 2) return hipErrorInvalidValue;
+      const size_t lds = (8 * lin_extent<3>(p + hx) + (C ? 2 * D::TX * D::TY * D::TZ : 0)) * sizeof(float);
       hipLaunchKernelGGL((lin_relax_tile_kernel<3, D::TX, D::TY, D::TZ, 2, 512>), dim3(nt), dim3(512), lds, s, L, ein,
-                         eout, bc, alpha, beta, p, zi, gcc, p0, ntx, nty);
+                         eout, bc, alpha, beta, p, zi, gcc, p0, ntx, nty, grr, rc, Mc);
       done += p;
     }
     hipError_t err = hipGetLastError();
@@ -859,6 +917,24 @@ hipError_t lin_relax(const LinLevel<float>& L, float* const ebuf[2], int* cur, c
     *cur = 1 - *cur;
   }
   return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t lin_relax(const LinLevel<float>& L, float* const ebuf[2], int* cur, const BCDesc& bc, float alpha,
+                     float beta, int npass, bool zero_init, const Geom* gc, const float* pc, hipStream_t s) {
+  return lin_relax_impl(L, ebuf, cur, bc, alpha, beta, npass, zero_init, gc, pc, nullptr, s);
+}
+
+hipError_t lin_relax_restrict(const LinLevel<float>& L, float* const ebuf[2], int* cur, const LinLevel<float>& C,
+                              const BCDesc& bc, float alpha, float beta, int npass, bool zero_init, hipStream_t s) {
+  if (npass + 1 <= (L.g.dim == 2 ? 8 : 2))
+    return lin_relax_impl(L, ebuf, cur, bc, alpha, beta, npass, zero_init, nullptr, nullptr, &C, s);
+  hipError_t e = lin_relax_impl(L, ebuf, cur, bc, alpha, beta, npass, zero_init, nullptr, nullptr, nullptr, s);
+  if (e != hipSuccess) return e;
+  LinLevel<float> Lf = L;
+  Lf.e = ebuf[*cur];
+  return lin_restrict(Lf, C, bc, alpha, beta, s);
 }
 
 hipError_t lin_restrict(const LinLevel<float>& L, const LinLevel<float>& C, const BCDesc& bc, float alpha, float beta,

@@ -339,10 +339,8 @@ int HCEngine::lin_vcycle(const ml_solve_opts& o) {
   auto ecur = [&](int l) { return m_ecur[l] ? m_ealt[l] : m_lin[l].e; };
   for (int l = 0; l < m_lc; ++l) {
     float* const eb[2] = {m_lin[l].e, m_ealt[l]};
-    ML_HIP(ctx, lin_relax(m_lin[l], eb, &m_ecur[l], m_op->bc, alpha, beta, 2 * o.nu1, true, nullptr, nullptr, s));
-    LinLevel<float> Lf = m_lin[l];
-    Lf.e = ecur(l);
-    ML_HIP(ctx, lin_restrict(Lf, m_lin[l + 1], m_op->bc, alpha, beta, s));
+    ML_HIP(ctx, lin_relax_restrict(m_lin[l], eb, &m_ecur[l], m_lin[l + 1], m_op->bc, alpha, beta, 2 * o.nu1, true,
+                                   s));
   }
   ML_HIP(ctx, lin_coarse_vcycle(m_lindesc, m_op->dim, s));
   for (int l = m_lc - 1; l >= 0; --l) {

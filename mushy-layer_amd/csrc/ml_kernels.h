@@ -183,6 +183,10 @@ hipError_t hc_residual_jacobian(const HCArgs<double>& A, const double* phi, doub
 // prolonged into e first
 hipError_t lin_relax(const LinLevel<float>& L, float* const ebuf[2], int* cur, const BCDesc& bc, float alpha,
                      float beta, int npass, bool zero_init, const Geom* gc, const float* pc, hipStream_t s);
+// lin_relax from a zero start followed by r_c = R(r - J e), M_c = R(M) into C
+// (one launch when the passes fit the tile halo)
+hipError_t lin_relax_restrict(const LinLevel<float>& L, float* const ebuf[2], int* cur, const LinLevel<float>& C,
+                              const BCDesc& bc, float alpha, float beta, int npass, bool zero_init, hipStream_t s);
 // r_c = R(r - J e), M_c = R(M) on the next coarser level
 hipError_t lin_restrict(const LinLevel<float>& L, const LinLevel<float>& C, const BCDesc& bc, float alpha,
                         float beta, hipStream_t s);
@@ -339,6 +343,12 @@ struct VCDesc {
 template <typename T>
 hipError_t vc_relax(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
                     const Geom* gc, const T* pc, hipStream_t s);
+// relaxation followed by the restricted residual R(rhs - L e) (homogeneous
+// BCs) of the relaxed state into resc on level gr, fused in one launch when
+// the passes fit the tile halo
+template <typename T>
+hipError_t vc_relax_restrict(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
+                             const Geom& gr, T* resc, hipStream_t s);
 // res = rhs - L(phi [+ e]) (res nullable), max-norm into *norm (nullable);
 // with e the updated state phi + e is written to phi_out (!= phi)
 template <typename T>

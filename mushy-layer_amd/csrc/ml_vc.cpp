@@ -79,10 +79,8 @@ int vc_vcycle(ml_vc_op op, const ml_solve_opts& o, int* ecur, const int* skip) {
     return ML_OK;
   }
   for (int l = 0; l < ltiles; ++l) {
-    const ml::VCArgs<double> A = args(l);
-    ML_HIP(ctx, ml::vc_relax<double>(A, op->lev[l].e, &ecur[l], op->lev[l].r, 2 * o.nu1, true, nullptr, nullptr,
-                                      s));
-    ML_HIP(ctx, ml::vc_restrict_residual<double>(A, op->lev[l + 1].g, E(l), op->lev[l].r, op->lev[l + 1].r, s));
+    ML_HIP(ctx, ml::vc_relax_restrict<double>(args(l), op->lev[l].e, &ecur[l], op->lev[l].r, 2 * o.nu1, true,
+                                               op->lev[l + 1].g, op->lev[l + 1].r, s));
   }
   if (lc < L) {
     ML_HIP(ctx, ml::vc_coarse_vcycle<double>(op->desc, op->dim, s));

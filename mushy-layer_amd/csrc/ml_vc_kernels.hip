@@ -81,14 +81,17 @@ template <typename T, int DIM, int TX, int TY, int TZ, int HMAX, int NT>
 __global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T* __restrict__ in,
                                                            T* __restrict__ out, const T* __restrict__ rhs, int npass,
                                                            int first, int zero_init, Geom gc,
-                                                           const T* __restrict__ pc, int ntx, int nty) {
+                                                           const T* __restrict__ pc, int ntx, int nty, Geom gr,
+                                                           T* __restrict__ resc) {
   if (A.skip && *A.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2 * HMAX) * (TY + 2 * HMAX) * (DIM == 3 ? TZ + 2 * HMAX : 1);
   constexpr int KMAX = (EMAX + NT - 1) / NT;
   const Geom& g = A.g;
   Tile<DIM, TX, TY, TZ> t;
-  t.init(npass, ntx, nty);
+  // with a restriction to produce, one more halo cell keeps the core's
+  // neighbours exact after the last pass
+  t.init(npass + (resc ? 1 : 0), ntx, nty);
   T* su = reinterpret_cast<T*>(smem_raw);
   const T dxinv = T(1) / (T(g.dx) * T(g.dx));
   PosRec R[KMAX];
@@ -150,6 +153,37 @@ __global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T*
   for (int kk = 0; kk < KMAX; ++kk) {
     const PosRec& r = R[kk];
     if (r.owner) out[r.q] = su[r.e];
+  }
+  if (!resc) return;
+  // restricted residual of the relaxed correction (RESTRICTRESDB,
+  // src/DarcyBrinkmanOpF.ChF:402-509, homogeneous BCs): (r - L e)/2^d per
+  // fine cell, child sums in Fortran order
+  T* sres = su + t.E;
+  const T denom = DIM == 3 ? T(8) : T(4);
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    if (!r.owner) continue;
+    int ex, ey, ez;
+    t.coords(r.e, ex, ey, ez);
+    const int c = (ex - t.H) + TX * ((ey - t.H) + TY * (DIM == 3 ? ez - t.H : 0));
+    T st[7];
+    gather_lds<T, DIM>(su, r.e, t.EX, t.EXY, st);
+    const T lof = vc_lof<T, DIM>(F[kk], st, aq[kk], A.c != nullptr, cv[kk], A.alpha, A.beta, dxinv);
+    sres[c] = (rr[kk] - lof) / denom;
+  }
+  __syncthreads();
+  constexpr int CX = TX / 2, CY = TY / 2, CZ = DIM == 3 ? TZ / 2 : 1;
+  for (int c = threadIdx.x; c < CX * CY * CZ; c += NT) {
+    const int cx = c % CX, cr = c / CX, cy = cr % CY, cz = cr / CY;
+    const int I = t.x0 / 2 + cx, J = t.y0 / 2 + cy, K = DIM == 3 ? t.z0 / 2 + cz : 0;
+    if (I >= gr.n[0] || J >= gr.n[1] || (DIM == 3 && K >= gr.n[2])) continue;
+    T sum = T(0);
+    for (int kk = 0; kk < (DIM == 3 ? 2 : 1); ++kk)
+      for (int jj = 0; jj < 2; ++jj)
+        for (int ii = 0; ii < 2; ++ii)
+          sum = sum + sres[(2 * cx + ii) + TX * ((2 * cy + jj) + TY * (DIM == 3 ? 2 * cz + kk : 0))];
+    resc[gr.ci(I, J, K)] = sum;
   }
 }
 
@@ -463,20 +497,24 @@ int vc_tiles(const Geom& g, int& ntx, int& nty) {
 
 template <typename T, int DIM>
 hipError_t vc_relax_dim(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
-                        const Geom* gc, const T* pc, hipStream_t s) {
+                        const Geom* gc, const T* pc, hipStream_t s, const Geom* gr = nullptr, T* resc = nullptr) {
   using D = VCRelaxTile<DIM>;
   int ntx, nty;
   const int nt = vc_tiles<D::TX, D::TY, D::TZ>(A.g, ntx, nty);
   const Geom gcc = gc ? *gc : A.g;
+  const Geom grr = gr ? *gr : A.g;
+  if (resc && npass + 1 > D::HMAX) return hipErrorInvalidValue;
   int done = 0;
   bool first_launch = true;
-  while (done < npass || (first_launch && npass == 0 && (zero_init || pc))) {
+  while (done < npass || (first_launch && npass == 0 && (zero_init || pc || resc))) {
     first_launch = false;
     const int p = (npass - done) < D::HMAX ? (npass - done) : D::HMAX;
-    const size_t lds = size_t(D::TX + 2 * p) * (D::TY + 2 * p) * (DIM == 3 ? D::TZ + 2 * p : 1) * sizeof(T);
+    const int h = p + (resc ? 1 : 0);
+    size_t lds = size_t(D::TX + 2 * h) * (D::TY + 2 * h) * (DIM == 3 ? D::TZ + 2 * h : 1);
+    if (resc) lds += size_t(D::TX) * D::TY * D::TZ;
     hipLaunchKernelGGL((vc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, D::HMAX, D::NT>), dim3(nt), dim3(D::NT),
-                       lds, s, A, buf[*cur], buf[1 - *cur], rhs, p, done & 1, (done == 0 && zero_init) ? 1 : 0, gcc,
-                       done == 0 ? pc : nullptr, ntx, nty);
+                       lds * sizeof(T), s, A, buf[*cur], buf[1 - *cur], rhs, p, done & 1,
+                       (done == 0 && zero_init) ? 1 : 0, gcc, done == 0 ? pc : nullptr, ntx, nty, grr, resc);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     *cur = 1 - *cur;
@@ -503,6 +541,18 @@ hipError_t vc_relax(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs,
                     const Geom* gc, const T* pc, hipStream_t s) {
   return A.g.dim == 2 ? vc_relax_dim<T, 2>(A, buf, cur, rhs, npass, zero_init, gc, pc, s)
                       : vc_relax_dim<T, 3>(A, buf, cur, rhs, npass, zero_init, gc, pc, s);
+}
+
+template <typename T>
+hipError_t vc_relax_restrict(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
+                             const Geom& gr, T* resc, hipStream_t s) {
+  const int hmax = A.g.dim == 2 ? VCRelaxTile<2>::HMAX : VCRelaxTile<3>::HMAX;
+  if (npass + 1 <= hmax)
+    return A.g.dim == 2 ? vc_relax_dim<T, 2>(A, buf, cur, rhs, npass, zero_init, nullptr, nullptr, s, &gr, resc)
+                        : vc_relax_dim<T, 3>(A, buf, cur, rhs, npass, zero_init, nullptr, nullptr, s, &gr, resc);
+  hipError_t e = vc_relax<T>(A, buf, cur, rhs, npass, zero_init, nullptr, nullptr, s);
+  if (e != hipSuccess) return e;
+  return vc_restrict_residual<T>(A, gr, buf[*cur], rhs, resc, s);
 }
 
 template <typename T>
@@ -555,6 +605,8 @@ bool vc_small_level(const Geom& g) { return g.nvalid() <= kVCWGCells; }
 
 template hipError_t vc_relax<double>(const VCArgs<double>&, double* const[2], int*, const double*, int, bool,
                                      const Geom*, const double*, hipStream_t);
+template hipError_t vc_relax_restrict<double>(const VCArgs<double>&, double* const[2], int*, const double*, int, bool,
+                                              const Geom&, double*, hipStream_t);
 template hipError_t vc_residual<double>(const VCArgs<double>&, const double*, const double*, double*,
                                         const double*, double*, unsigned long long*, int, hipStream_t);
 template hipError_t vc_apply_op<double>(const VCArgs<double>&, const double*, double*, int, hipStream_t);
