@@ -24,6 +24,13 @@ int fail(ml_ctx ctx, int code, const std::string& msg) {
   return code;
 }
 
+int max_lds(ml_ctx ctx, size_t* out) {
+  int v = 0;
+  ML_HIP(ctx, hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+  *out = size_t(v);
+  return ML_OK;
+}
+
 bool valid_grid(int dim, const int n[], const int periodic[], double dx) {
   if (dim != 2 && dim != 3) return false;
   if (!(dx > 0)) return false;

@@ -22,7 +22,7 @@ namespace ml {
 namespace {
 
 constexpr int kLinThreads = 256;
-constexpr int kWGT = 1024;     // single-workgroup kernel threads (one per cell)
+constexpr int kWGT = 512;      // single-workgroup kernel threads
 constexpr int kWGC = 1024;     // cells of the largest level it holds
 constexpr int kWGM = kWGC / kWGT;
 
@@ -176,7 +176,7 @@ __device__ inline float wall_sign(const BCDesc& bc, int d, int s, int n) {
 // Inverse of the 2x2 diagonal block of J at a cell:
 // D = alpha*a + beta*sum(b_lo + b_hi)*dxinv * M (per component row).
 template <typename T, int DIM>
-__device__ inline void block_inverse(const CellFaces<T, DIM>& F, const T* aq, const T* Mc, T alpha, T beta, T dxinv,
+__device__ __forceinline__ void block_inverse(const CellFaces<T, DIM>& F, const T* aq, const T* Mc, T alpha, T beta, T dxinv,
                                      T* inv) {
   T sig[2];
 #pragma unroll
@@ -197,7 +197,7 @@ __device__ inline void block_inverse(const CellFaces<T, DIM>& F, const T* aq, co
 
 // (J e)_n from the staged (M e) stencils, then one block Gauss-Seidel update.
 template <typename T, int DIM>
-__device__ inline void block_step(const CellFaces<T, DIM>& F, const T* aq, const T* rr, const T* inv, T alpha, T beta,
+__device__ __forceinline__ void block_step(const CellFaces<T, DIM>& F, const T* aq, const T* rr, const T* inv, T alpha, T beta,
                                   T dxinv, const T* u0s, const T* u1s, T& e0, T& e1) {
   const T r0 = rr[0] - lof_of<T, DIM>(F, 0, u0s, aq[0], e0, alpha, beta, dxinv);
   const T r1 = rr[1] - lof_of<T, DIM>(F, 1, u1s, aq[1], e1, alpha, beta, dxinv);
@@ -455,351 +455,498 @@ __global__ __launch_bounds__(kLinThreads) void lin_restrict_kernel(LinLevel<floa
 
 // --------------------------------------------------------------------------
 // Single-workgroup linear V-cycle below the tiled levels.
-template <typename T, int DIM>
-struct WGLin {
-  Geom g;
-  int N, nx, nxy;
-  int c[kWGM], i[kWGM], j[kWGM], k[kWGM];
-  long long q[kWGM];
-  bool own[kWGM];
-  __device__ inline void init(const Geom& gg) {
-    g = gg;
-    nx = g.n[0];
-    nxy = g.n[0] * g.n[1];
-    N = nxy * (DIM == 3 ? g.n[2] : 1);
-#pragma unroll
-    for (int m = 0; m < kWGM; ++m) {
-      c[m] = threadIdx.x + m * kWGT;
-      own[m] = c[m] < N;
-      i[m] = c[m] % nx;
-      const int r = c[m] / nx;
-      j[m] = r % g.n[1];
-      k[m] = DIM == 3 ? r / g.n[1] : 0;
-      q[m] = own[m] ? g.ci(i[m], j[m], k[m]) : 0;
-    }
-  }
+// unknowns of the largest coarsest level solved exactly
+constexpr int kDirect = 16;
+
+// Linear V-cycle for levels l0.. in one workgroup.  e, M e, M, r, the faces,
+// a and the block inverses of every level stay in LDS for the whole cycle
+// (compact, x fastest, levels one after another; the largest level's faces
+// and inverses of this thread's cells in registers).  Each thread owns one
+// cell of each colour of a level (even row length) or the cells t and
+// t + 512; levels whose owners fit one wavefront are relaxed by wave 0
+// alone.  The coarsest level is solved in FP64: exactly (Gauss-Jordan with
+// partial pivoting by one wavefront) when it has at most kDirect unknowns,
+// otherwise by `bottom` block Gauss-Seidel iterations.
+constexpr int kLL = 6;   // levels one launch handles: 1024 cells coarsen to one cell in 6 (2D), 4 (3D)
+template <int DIM>
+struct LinLevels {
+  static constexpr int n = DIM == 3 ? 4 : kLL;
 };
 
-// gather the (M e) stencil of cell (i,j,k) from LDS with wrap / wall rule
-template <typename T, int DIM>
-__device__ inline void wg_ustencil(const Geom& g, const BCDesc& bc, const T* su, int c, int i, int j, int k, int n,
-                                   int nx, int nxy, T* p) {
-  const int iv[3] = {i, j, k};
-  p[0] = su[c];
+template <int DIM>
+struct LinLds {
+  float *se, *su, *sM, *sr;   // level at 2*off (e, M e, r: 2 comps) and 4*off (M)
+  float *sF, *sA, *sI;        // faces [2][2*DIM][tot], a [2][tot], inverses [4][tot]
+  int tot;
+  __device__ float* e(const CLev& L) const { return se + 2 * L.off; }
+  __device__ float* u(const CLev& L) const { return su + 2 * L.off; }
+  __device__ float* m(const CLev& L) const { return sM + 4 * L.off; }
+  __device__ float* r(const CLev& L) const { return sr + 2 * L.off; }
+};
+
+// wall sign of component n's M e (bit 6n + 2d + s of bcw: -1)
+__device__ __forceinline__ float lc_wsign(int bcw, int d, int s, int n) {
+  return ((bcw >> (6 * n + 2 * d + s)) & 1) ? -1.0f : 1.0f;
+}
+
+// A thread's cells of one level for the duration of a level visit:
+// coordinates, colour, neighbour indices, faces, a, M, r and the block
+// inverses in registers.
+template <int DIM>
+struct LCCtx {
+  int pk[2], c[2], col[2], wall[2];
+  int nb[2][2 * DIM];
+  CellFaces<float, DIM> F[2];
+  float aq[2][2], inv[2][4], M[2][4], rr[2][2];
+};
+
+// inverses: compute the block inverses (and keep them in LDS for the
+// level's next visit) or read them back
+template <int DIM>
+__device__ __forceinline__ void lcx_build(LCCtx<DIM>& X, const CLev& L, const int* per, const LinLds<DIM>& S,
+                                          float alpha, float beta, bool inverses) {
+  const float dxinv = float(1.0 / (L.dx * L.dx));
+  const int N = L.N;
+  const float* sM = S.m(L);
+  const float* sr = S.r(L);
+  COwn o;
+  cown_set<kWGT>(L, o);
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    const int nd = g.n[d];
-    const int st = d == 0 ? 1 : (d == 1 ? nx : nxy);
+  for (int m = 0; m < 2; ++m) {
+    X.pk[m] = o.p[m];
+    if (o.p[m] < 0) continue;
+    int i, j, k;
+    cunpack(o.p[m], i, j, k);
+    const int c = i + L.nx * j + L.nxy * k;
+    X.c[m] = c;
+    X.col[m] = (i + j + k) & 1;
+    clev_nbrs<DIM>(L, per, i, j, k, X.nb[m], X.wall[m]);
+    const int q = L.off + c;
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int v = iv[d] + (s ? 1 : -1);
-      T val;
-      if (v >= 0 && v < nd)
-        val = su[c + (s ? st : -st)];
-      else if (g.periodic[d])
-        val = su[c + (s ? -(nd - 1) * st : (nd - 1) * st)];
-      else
-        val = su[c] * T(wall_sign(bc, d, s, n));
-      p[1 + 2 * d + (s ? 0 : 1)] = val;
+    for (int n = 0; n < 2; ++n) {
+#pragma unroll
+      for (int f = 0; f < 2 * DIM; ++f) X.F[m].v[n][f] = S.sF[(n * 2 * DIM + f) * S.tot + q];
+      X.aq[m][n] = S.sA[n * S.tot + q];
+      X.rr[m][n] = sr[n * N + c];
+    }
+#pragma unroll
+    for (int z = 0; z < 4; ++z) X.M[m][z] = sM[z * N + c];
+    if (inverses) {
+      block_inverse<float, DIM>(X.F[m], X.aq[m], X.M[m], alpha, beta, dxinv, X.inv[m]);
+#pragma unroll
+      for (int z = 0; z < 4; ++z) S.sI[z * S.tot + q] = X.inv[m][z];
+    } else {
+#pragma unroll
+      for (int z = 0; z < 4; ++z) X.inv[m][z] = S.sI[z * S.tot + q];
     }
   }
 }
 
-// npass red/black block passes on a whole small level held in LDS
-// (se: e, su: M e, sM: Jacobian; all of this level).  M e is refreshed at
-// the updated cells only, so each pass needs one barrier.
-template <typename T, int DIM>
-__device__ void wg_lin_relax(const WGLin<T, DIM>& W, const BCDesc& bc, T alpha, T beta, const T* const* b,
-                             const T* a, const T* rg, T* se, T* su, const T* sM, int npass) {
-  const Geom& g = W.g;
-  const T dxinv = T(1.0 / (g.dx * g.dx));
-  const int N = W.N;
-  CellFaces<T, DIM> F[kWGM];
-  T rr[kWGM][2], aq[kWGM][2], inv[kWGM][4];
+// (M e) stencil of component n of cell m from the level's su
+template <int DIM>
+__device__ __forceinline__ void lcx_ustencil(const LCCtx<DIM>& X, int m, int n, int bcw, const float* su, float* p) {
+  p[0] = su[X.c[m]];
 #pragma unroll
-  for (int m = 0; m < kWGM; ++m) {
-    if (!W.own[m]) continue;
-    const int c = W.c[m];
-    load_faces<T, DIM>(g, b, W.i[m], W.j[m], W.k[m], F[m]);
-    rr[m][0] = rg[c];
-    rr[m][1] = rg[N + c];
-    aq[m][0] = a ? a[W.q[m]] : T(1);
-    aq[m][1] = a ? a[W.q[m] + g.ncell] : T(1);
-    const T Mc[4] = {sM[c], sM[N + c], sM[2 * N + c], sM[3 * N + c]};
-    block_inverse<T, DIM>(F[m], aq[m], Mc, alpha, beta, dxinv, inv[m]);
-    su[c] = Mc[0] * se[c] + Mc[1] * se[N + c];
-    su[N + c] = Mc[2] * se[c] + Mc[3] * se[N + c];
+  for (int f = 0; f < 2 * DIM; ++f) {
+    const int d = f >> 1, s = f & 1;
+    float v = su[X.nb[m][f]];
+    if ((X.wall[m] >> f) & 1) v = v * lc_wsign(bcw, d, s, n);
+    p[1 + 2 * d + (1 - s)] = v;
+  }
+}
+
+// u = M e at the context's cells
+template <int DIM>
+__device__ __forceinline__ void lcx_mvec(const LCCtx<DIM>& X, const CLev& L, const LinLds<DIM>& S) {
+  const int N = L.N;
+  const float* se = S.e(L);
+  float* su = S.u(L);
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    if (X.pk[m] < 0) continue;
+    const int c = X.c[m];
+    const float e0 = se[c], e1 = se[N + c];
+    su[c] = X.M[m][0] * e0 + X.M[m][1] * e1;
+    su[N + c] = X.M[m][2] * e0 + X.M[m][3] * e1;
   }
   __syncthreads();
-  for (int p = 0; p < npass; ++p) {
+}
+
+template <int DIM>
+__device__ __forceinline__ void lcx_relax(const LCCtx<DIM>& X, const CLev& L, int bcw, const LinLds<DIM>& S,
+                                          float alpha, float beta, int npass) {
+  const float dxinv = float(1.0 / (L.dx * L.dx));
+  const int N = L.N;
+  float* se = S.e(L);
+  float* su = S.u(L);
+  auto pass = [&](int p) {
 #pragma unroll
-    for (int m = 0; m < kWGM; ++m) {
-      if (!W.own[m] || ((W.i[m] + W.j[m] + W.k[m] + p) & 1)) continue;
-      const int c = W.c[m];
-      T u0s[7], u1s[7];
-      wg_ustencil<T, DIM>(g, bc, su, c, W.i[m], W.j[m], W.k[m], 0, W.nx, W.nxy, u0s);
-      wg_ustencil<T, DIM>(g, bc, su + N, c, W.i[m], W.j[m], W.k[m], 1, W.nx, W.nxy, u1s);
-      T e0 = se[c], e1 = se[N + c];
-      block_step<T, DIM>(F[m], aq[m], rr[m], inv[m], alpha, beta, dxinv, u0s, u1s, e0, e1);
+    for (int m = 0; m < 2; ++m) {
+      if (X.pk[m] < 0 || X.col[m] != (p & 1)) continue;
+      const int c = X.c[m];
+      float u0s[7], u1s[7];
+      lcx_ustencil<DIM>(X, m, 0, bcw, su, u0s);
+      lcx_ustencil<DIM>(X, m, 1, bcw, su + N, u1s);
+      float e0 = se[c], e1 = se[N + c];
+      block_step<float, DIM>(X.F[m], X.aq[m], X.rr[m], X.inv[m], alpha, beta, dxinv, u0s, u1s, e0, e1);
       se[c] = e0;
       se[N + c] = e1;
-      su[c] = sM[c] * e0 + sM[N + c] * e1;
-      su[N + c] = sM[2 * N + c] * e0 + sM[3 * N + c] * e1;
+      su[c] = X.M[m][0] * e0 + X.M[m][1] * e1;
+      su[N + c] = X.M[m][2] * e0 + X.M[m][3] * e1;
     }
+  };
+  if (cown_active<kWGT>(L) <= 64) {
+    if (threadIdx.x < 64)
+      for (int p = 0; p < npass; ++p) {
+        pass(p);
+        wave_sync();
+      }
     __syncthreads();
+  } else {
+    for (int p = 0; p < npass; ++p) {
+      pass(p);
+      __syncthreads();
+    }
   }
 }
 
-// Direct FP64 solve of J e = r on a whole tiny level (2N <= kDirect
-// unknowns): each thread t < 2N builds column t of J by applying the
-// operator to the unit vector of unknown t; thread 0 then eliminates with
-// partial pivoting.  Returns false (leaving e untouched) on a zero pivot.
-constexpr int kDirect = 16;
+// r_c = R(r - J e), M_c = R(M); the coarse correction starts at zero.  The
+// fine level's M e array holds the residual between the two phases (the
+// next visit of the level recomputes it).
 template <int DIM>
-__device__ bool wg_direct_solve(const Geom& g, const BCDesc& bc, double alpha, double beta, const double* const* b,
-                                const double* a, const double* sMd, const double* rd, double* ed, double* A,
-                                int* flag) {
-  const int N = g.n[0] * g.n[1] * (DIM == 3 ? g.n[2] : 1);
-  const int U = 2 * N;
-  const double dxinv = 1.0 / (g.dx * g.dx);
-  const int nx = g.n[0], nxy = g.n[0] * g.n[1];
-  const int t = threadIdx.x;
-  if (t < U) {
-    const int ct = t % N, nt = t / N;
-    // u = M e for the unit vector: nonzero only at cell ct
-    auto u_at = [&](int cell, int k) -> double {
-      return cell == ct ? sMd[(2 * k + nt) * N + ct] : 0.0;
-    };
-    for (int r = 0; r < N; ++r) {
-      const int i = r % nx, j = (r / nx) % g.n[1], k = DIM == 3 ? r / nxy : 0;
-      CellFaces<double, DIM> F;
-      load_faces<double, DIM>(g, b, i, j, k, F);
-      const int iv[3] = {i, j, k};
+__device__ __forceinline__ void lcx_restrict(const LCCtx<DIM>& X, const CLev& Lf, const CLev& Lc, int bcw,
+                                             const LinLds<DIM>& S, float alpha, float beta) {
+  const float dxinv = float(1.0 / (Lf.dx * Lf.dx));
+  const int N = Lf.N;
+  const float* se = S.e(Lf);
+  const float* sM = S.m(Lf);
+  float* su = S.u(Lf);
+  float res[2][2];
 #pragma unroll
-      for (int kc = 0; kc < 2; ++kc) {
-        double p[7];
-        p[0] = u_at(r, kc);
+  for (int m = 0; m < 2; ++m) {
+    if (X.pk[m] < 0) continue;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          const int nd = g.n[d];
-          const int st = d == 0 ? 1 : (d == 1 ? nx : nxy);
-#pragma unroll
-          for (int sd = 0; sd < 2; ++sd) {
-            const int v = iv[d] + (sd ? 1 : -1);
-            double val;
-            if (v >= 0 && v < nd)
-              val = u_at(r + (sd ? st : -st), kc);
-            else if (g.periodic[d])
-              val = u_at(r + (sd ? -(nd - 1) * st : (nd - 1) * st), kc);
-            else
-              val = p[0] * double(wall_sign(bc, d, sd, kc));
-            p[1 + 2 * d + (sd ? 0 : 1)] = val;
-          }
-        }
-        const long long q = g.ci(i, j, k) + kc * g.ncell;
-        const double aq = a ? a[q] : 1.0;
-        const double ev = (r == ct && kc == nt) ? 1.0 : 0.0;
-        A[(kc * N + r) * kDirect + t] = lof_of<double, DIM>(F, kc, p, aq, ev, alpha, beta, dxinv);
-      }
+    for (int n = 0; n < 2; ++n) {
+      float us[7];
+      lcx_ustencil<DIM>(X, m, n, bcw, su + n * N, us);
+      res[m][n] = X.rr[m][n] - lof_of<float, DIM>(X.F[m], n, us, X.aq[m][n], se[n * N + X.c[m]], alpha, beta, dxinv);
     }
   }
   __syncthreads();
-  if (t == 0) {
-    double x[kDirect];
-    for (int r = 0; r < U; ++r) x[r] = rd[r];
-    int ok = 1;
-    for (int c = 0; c < U && ok; ++c) {
-      int piv = c;
-      double best = fabs(A[c * kDirect + c]);
-      for (int r = c + 1; r < U; ++r)
-        if (fabs(A[r * kDirect + c]) > best) {
-          best = fabs(A[r * kDirect + c]);
-          piv = r;
-        }
-      if (!(best > 0.0)) {
-        ok = 0;
-        break;
-      }
-      if (piv != c) {
-        for (int k = 0; k < U; ++k) {
-          const double tmp = A[c * kDirect + k];
-          A[c * kDirect + k] = A[piv * kDirect + k];
-          A[piv * kDirect + k] = tmp;
-        }
-        const double tmp = x[c];
-        x[c] = x[piv];
-        x[piv] = tmp;
-      }
-      const double rp = 1.0 / A[c * kDirect + c];
-      for (int r = c + 1; r < U; ++r) {
-        const double f = A[r * kDirect + c] * rp;
-        if (f == 0.0) continue;
-        for (int k = c; k < U; ++k) A[r * kDirect + k] -= f * A[c * kDirect + k];
-        x[r] -= f * x[c];
-      }
-    }
-    if (ok)
-      for (int c = U - 1; c >= 0; --c) {
-        double v = x[c];
-        for (int k = c + 1; k < U; ++k) v -= A[c * kDirect + k] * x[k];
-        x[c] = v / A[c * kDirect + c];
-      }
-    if (ok)
-      for (int r = 0; r < U; ++r) ed[r] = x[r];
-    *flag = ok;
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    if (X.pk[m] < 0) continue;
+    su[X.c[m]] = res[m][0];
+    su[N + X.c[m]] = res[m][1];
   }
   __syncthreads();
-  return *flag != 0;
+  const int NC = Lc.N;
+  float* rc = S.r(Lc);
+  float* Mc = S.m(Lc);
+  float* ec = S.e(Lc);
+  const float w = DIM == 3 ? 0.125f : 0.25f;
+  for (int cc = threadIdx.x; cc < NC; cc += kWGT) {
+    int I, J, K;
+    clev_coords(Lc, cc, I, J, K);
+    float s0 = 0.f, s1 = 0.f, mm4[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int kk = 0; kk < (DIM == 3 ? 2 : 1); ++kk)
+      for (int jj = 0; jj < 2; ++jj)
+        for (int ii = 0; ii < 2; ++ii) {
+          const int fc = (2 * I + ii) + Lf.nx * (2 * J + jj) + Lf.nxy * (DIM == 3 ? 2 * K + kk : 0);
+          s0 += su[fc];
+          s1 += su[N + fc];
+#pragma unroll
+          for (int mm = 0; mm < 4; ++mm) mm4[mm] += sM[mm * N + fc];
+        }
+    rc[cc] = s0 * w;
+    rc[NC + cc] = s1 * w;
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm) Mc[mm * NC + cc] = mm4[mm] * w;
+    ec[cc] = 0.f;
+    ec[NC + cc] = 0.f;
+  }
+  __syncthreads();
 }
 
-// Linear V-cycle for levels l0.. in one workgroup: per level the state
-// (e, M, r) is staged in LDS; the coarsest level is solved in FP64 with
-// `bottom` block Gauss-Seidel iterations.
+// e_f += P(e_c) (linear prolongation from the coarse level in LDS)
+template <int DIM>
+__device__ __forceinline__ void lcx_prolong(const LCCtx<DIM>& X, const CLev& Lf, const CLev& Lc, const int* per,
+                                            const LinLds<DIM>& S) {
+  float* ef = S.e(Lf);
+  const float* ec = S.e(Lc);
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    if (X.pk[m] < 0) continue;
+    int i, j, k;
+    cunpack(X.pk[m], i, j, k);
+    const int c = X.c[m];
+    ef[c] = prolong_value_lds<float, DIM>(Lc.n, per, i, j, k, ec, ef[c]);
+    ef[Lf.N + c] = prolong_value_lds<float, DIM>(Lc.n, per, i, j, k, ec + Lc.N, ef[Lf.N + c]);
+  }
+  __syncthreads();
+}
+
+// Exact FP64 solve of J e = r on a tiny level (U = 2N <= kDirect unknowns)
+// by wave 0: lane R builds row R of J (unknown R = comp*N + cell) from the
+// unit vectors, then Gauss-Jordan elimination with partial pivoting, the
+// rows staying in the lanes' registers.  Returns false (e untouched) on a
+// zero pivot.  bF/bA: the level's FP64 faces and a in LDS; buf and prow:
+// 64 + kDirect + 1 doubles of LDS scratch.
+template <int DIM>
+__device__ __forceinline__ bool lc_direct(const CLev& L, const int* per, int bcw, double alpha, double beta, const double* bF,
+                          const double* bA, const double* dM, const double* dr, double* de, double* buf,
+                          double* prow) {
+  const int N = L.N, U = 2 * N;
+  const int lane = threadIdx.x;
+  const double dxinv = 1.0 / (L.dx * L.dx);
+  double a[kDirect];
+  double b = 0.0;
+#pragma unroll
+  for (int t = 0; t < kDirect; ++t) a[t] = 0.0;
+  if (lane < U) {
+    const int rc = lane % N, kc = lane / N;
+    int i, j, k;
+    clev_coords(L, rc, i, j, k);
+    const int iv[3] = {i, j, k};
+    // faces of component kc (held as component 0 of F)
+    CellFaces<double, DIM> F;
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) {
+      F.v[0][f] = bF[(rc * 2 + kc) * 2 * DIM + f];
+      F.v[1][f] = 0.0;
+    }
+    const double aq = bA[rc * 2 + kc];
+#pragma unroll
+    for (int t = 0; t < kDirect; ++t) {
+      if (t >= U) continue;
+      const int ct = t % N, nt = t / N;
+      // u = M e of the unit vector of unknown t: nonzero only at cell ct
+      auto u_at = [&](int cell) -> double { return cell == ct ? dM[(2 * kc + nt) * N + ct] : 0.0; };
+      double p[7];
+      p[0] = u_at(rc);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const int nd = L.n[d];
+        const int st = d == 0 ? 1 : (d == 1 ? L.nx : L.nxy);
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) {
+          const int v = iv[d] + (sd ? 1 : -1);
+          double val;
+          if (v >= 0 && v < nd)
+            val = u_at(rc + (sd ? st : -st));
+          else if (per[d])
+            val = u_at(rc + (sd ? -(nd - 1) * st : (nd - 1) * st));
+          else
+            val = p[0] * double(lc_wsign(bcw, d, sd, kc));
+          p[1 + 2 * d + (sd ? 0 : 1)] = val;
+        }
+      }
+      const double ev = (rc == ct && kc == nt) ? 1.0 : 0.0;
+      a[t] = lof_of<double, DIM>(F, 0, p, aq, ev, alpha, beta, dxinv);
+    }
+    b = dr[lane];
+  }
+  bool used = false;
+  int mycol = -1;
+#pragma unroll
+  for (int c = 0; c < kDirect; ++c) {
+    if (c >= U) break;
+    buf[lane] = (lane < U && !used) ? fabs(a[c]) : -1.0;
+    wave_sync();
+    int piv = 0;
+    double best = -1.0;
+    for (int r = 0; r < U; ++r) {
+      const double v = buf[r];
+      if (v > best) {
+        best = v;
+        piv = r;
+      }
+    }
+    if (!(best > 0.0)) return false;
+    if (lane == piv) {
+#pragma unroll
+      for (int t = 0; t < kDirect; ++t) prow[t] = a[t];
+      prow[kDirect] = b;
+      used = true;
+      mycol = c;
+    }
+    wave_sync();
+    if (lane < U && lane != piv) {
+      const double f = a[c] / prow[c];
+#pragma unroll
+      for (int t = 0; t < kDirect; ++t) a[t] = a[t] - f * prow[t];
+      b = b - f * prow[kDirect];
+    }
+    wave_sync();
+  }
+  if (lane < U) {
+    double dcol = 1.0;
+#pragma unroll
+    for (int t = 0; t < kDirect; ++t)
+      if (t == mycol) dcol = a[t];
+    de[mycol] = b / dcol;
+  }
+  wave_sync();
+  return true;
+}
+
 template <int DIM>
 __global__ __launch_bounds__(kWGT) void lin_coarse_vcycle_kernel(const LinDesc* __restrict__ Dp) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int LL = LinLevels<DIM>::n;
   const LinDesc& D = *Dp;
   if (D.skip && *D.skip) return;
-  float* se = reinterpret_cast<float*>(smem_raw);   // 2*kWGC
-  float* su = se + 2 * kWGC;                         // 2*kWGC
-  float* sM = su + 2 * kWGC;                         // 4*kWGC
-  float* sr = sM + 4 * kWGC;                         // 2*kWGC
-  const int l0 = D.l0, L = D.nlev;
+  const int l0 = D.l0, nl = D.nlev - D.l0;
   const float alpha = float(D.alpha), beta = float(D.beta);
-
-  auto stage = [&](const LinLevel<float>& lv, const WGLin<float, DIM>& W, bool zero_e) {
-#pragma unroll
-    for (int m = 0; m < kWGM; ++m) {
-      if (!W.own[m]) continue;
-      const int c = W.c[m];
-      const long long q = W.q[m];
-      se[c] = zero_e ? 0.0f : lv.e[q];
-      se[W.N + c] = zero_e ? 0.0f : lv.e[q + lv.g.ncell];
-      sr[c] = lv.r[q];
-      sr[W.N + c] = lv.r[q + lv.g.ncell];
-#pragma unroll
-      for (int mm = 0; mm < 4; ++mm) sM[mm * W.N + c] = lv.M[q + mm * lv.g.ncell];
+  int per[3];
+  for (int d = 0; d < 3; ++d) per[d] = D.lev[l0].g.periodic[d];
+  int bcw = 0;
+  for (int n = 0; n < 2; ++n)
+    for (int d = 0; d < DIM; ++d)
+      for (int sd = 0; sd < 2; ++sd)
+        bcw |= ((D.bc.apply && D.bc.type[d][sd][n] == 0) ? 1 : 0) << (6 * n + 2 * d + sd);
+  // level geometry in LDS (uniform reads) rather than registers
+  __shared__ CLev G[LL];
+  int tot = 0;
+  for (int l = 0; l < nl && l < LL; ++l) tot += D.lev[l0 + l].g.n[0] * D.lev[l0 + l].g.n[1] * D.lev[l0 + l].g.n[2];
+  if (threadIdx.x == 0) {
+    int off = 0;
+    for (int l = 0; l < nl && l < LL; ++l) {
+      clev_set(G[l], D.lev[l0 + l].g, off);
+      off += G[l].N;
     }
-    __syncthreads();
-  };
-  auto store_e = [&](const LinLevel<float>& lv, const WGLin<float, DIM>& W) {
+  }
+  __syncthreads();
+  LinLds<DIM> S;
+  S.tot = tot;
+  S.se = reinterpret_cast<float*>(smem_raw);
+  S.su = S.se + 2 * tot;
+  S.sM = S.su + 2 * tot;
+  S.sr = S.sM + 4 * tot;
+  S.sF = S.sr + 2 * tot;
+  S.sA = S.sF + 4 * DIM * tot;
+  S.sI = S.sA + 2 * tot;
+  // FP64 arrays of the coarsest level after the FP32 ones
+  const int lb = nl - 1;
+  const CLev& Lb = G[lb];
+  const int Nb = Lb.N;
+  const bool direct = 2 * Nb <= kDirect;
+  // exact solve only: FP64 e, M, r, faces [cell][2][2*DIM], a [cell][2] and
+  // 64 + kDirect + 1 scratch
+  double* de = reinterpret_cast<double*>(smem_raw + ((size_t(16 + 4 * DIM) * tot * sizeof(float) + 15) & ~size_t(15)));
+  double* dM = de + 2 * Nb;
+  double* dr = dM + 4 * Nb;
+  double* bF = dr + 2 * Nb;
+  double* bA = bF + 4 * DIM * Nb;
+  double* dbuf = bA + 2 * Nb;
+  // operator data of every level (and the coarsest level's FP64 copy for
+  // the exact solve), level l0's state
 #pragma unroll
-    for (int m = 0; m < kWGM; ++m) {
-      if (!W.own[m]) continue;
-      lv.e[W.q[m]] = se[W.c[m]];
-      lv.e[W.q[m] + lv.g.ncell] = se[W.N + W.c[m]];
-    }
-    __syncthreads();
-  };
-
-  for (int l = l0; l < L - 1; ++l) {
-    const LinLevel<float>& lv = D.lev[l];
-    const LinLevel<float>& cv = D.lev[l + 1];
-    WGLin<float, DIM> W;
-    W.init(lv.g);
-    stage(lv, W, true);
-    wg_lin_relax<float, DIM>(W, D.bc, alpha, beta, lv.b, lv.a, sr, se, su, sM, 2 * D.nu1);
-    store_e(lv, W);
-    // r_c = R(r - J e), M_c = R(M)
+  for (int l = 0; l < LL; ++l) {
+    if (l >= nl) continue;
+    const LinLevel<float>& lv = D.lev[l0 + l];
+    for (int c = threadIdx.x; c < G[l].N; c += kWGT) {
+      int i, j, k;
+      clev_coords(G[l], c, i, j, k);
+      const int q = G[l].off + c;
+      CellFaces<float, DIM> F;
+      load_faces<float, DIM>(lv.g, lv.b, i, j, k, F);
+      const long long g = lv.g.ci(i, j, k);
 #pragma unroll
-    for (int m = 0; m < kWGM; ++m) {
-      if (!W.own[m]) continue;
-      const int c = W.c[m];
-      su[c] = sM[c] * se[c] + sM[W.N + c] * se[W.N + c];
-      su[W.N + c] = sM[2 * W.N + c] * se[c] + sM[3 * W.N + c] * se[W.N + c];
-    }
-    __syncthreads();
-    {
-      const Geom& g = lv.g;
-      const Geom& gcc = cv.g;
-      const int NC = gcc.n[0] * gcc.n[1] * (DIM == 3 ? gcc.n[2] : 1);
-      const float dxinv = float(1.0 / (g.dx * g.dx));
-      const float w = DIM == 3 ? 0.125f : 0.25f;
-      for (int cc = threadIdx.x; cc < NC; cc += kWGT) {
-        const int I = cc % gcc.n[0], rq = cc / gcc.n[0], J = rq % gcc.n[1], K = DIM == 3 ? rq / gcc.n[1] : 0;
-        float s0 = 0.f, s1 = 0.f, mm4[4] = {0.f, 0.f, 0.f, 0.f};
-        for (int kk = 0; kk < (DIM == 3 ? 2 : 1); ++kk)
-          for (int jj = 0; jj < 2; ++jj)
-            for (int ii = 0; ii < 2; ++ii) {
-              const int fi = 2 * I + ii, fj = 2 * J + jj, fk = DIM == 3 ? 2 * K + kk : 0;
-              const int fc = fi + W.nx * fj + W.nxy * fk;
-              CellFaces<float, DIM> F;
-              load_faces<float, DIM>(g, lv.b, fi, fj, fk, F);
-              const long long q = g.ci(fi, fj, fk);
-              float u0s[7], u1s[7];
-              wg_ustencil<float, DIM>(g, D.bc, su, fc, fi, fj, fk, 0, W.nx, W.nxy, u0s);
-              wg_ustencil<float, DIM>(g, D.bc, su + W.N, fc, fi, fj, fk, 1, W.nx, W.nxy, u1s);
-              const float a0 = lv.a ? lv.a[q] : 1.0f, a1 = lv.a ? lv.a[q + g.ncell] : 1.0f;
-              s0 += sr[fc] - lof_of<float, DIM>(F, 0, u0s, a0, se[fc], alpha, beta, dxinv);
-              s1 += sr[W.N + fc] - lof_of<float, DIM>(F, 1, u1s, a1, se[W.N + fc], alpha, beta, dxinv);
+      for (int n = 0; n < 2; ++n) {
 #pragma unroll
-              for (int mm = 0; mm < 4; ++mm) mm4[mm] += sM[mm * W.N + fc];
-            }
-        const long long qc = gcc.ci(I, J, K);
-        cv.r[qc] = s0 * w;
-        cv.r[qc + gcc.ncell] = s1 * w;
+        for (int f = 0; f < 2 * DIM; ++f) S.sF[(n * 2 * DIM + f) * tot + q] = F.v[n][f];
+        S.sA[n * tot + q] = lv.a ? lv.a[g + n * lv.g.ncell] : 1.0f;
+      }
+      if (l == 0) {
+        const int N = G[0].N;
+        S.se[c] = 0.f;
+        S.se[N + c] = 0.f;
+        S.sr[c] = lv.r[g];
+        S.sr[N + c] = lv.r[g + lv.g.ncell];
 #pragma unroll
-        for (int mm = 0; mm < 4; ++mm) cv.M[qc + mm * gcc.ncell] = mm4[mm] * w;
+        for (int mm = 0; mm < 4; ++mm) S.sM[mm * N + c] = lv.M[g + mm * lv.g.ncell];
+      }
+      if (direct && l == lb) {
+        CellFaces<double, DIM> Fd;
+        load_faces<double, DIM>(lv.g, D.b64, i, j, k, Fd);
+#pragma unroll
+        for (int n = 0; n < 2; ++n) {
+#pragma unroll
+          for (int f = 0; f < 2 * DIM; ++f) bF[(c * 2 + n) * 2 * DIM + f] = Fd.v[n][f];
+          bA[c * 2 + n] = D.a64 ? D.a64[g + n * lv.g.ncell] : 1.0;
+        }
       }
     }
-    __syncthreads();
   }
-
-  // bottom: FP64 solve on the coarsest level
+  __syncthreads();
+  // down: pre-smoothing and restriction to the coarsest level
+#pragma unroll
+  for (int l = 0; l < LL; ++l) {
+    if (l + 1 >= nl) continue;
+    LCCtx<DIM> X;
+    lcx_build<DIM>(X, G[l], per, S, alpha, beta, true);
+    lcx_mvec<DIM>(X, G[l], S);
+    lcx_relax<DIM>(X, G[l], bcw, S, alpha, beta, 2 * D.nu1);
+    lcx_restrict<DIM>(X, G[l], G[l + 1], bcw, S, alpha, beta);
+  }
+  // bottom: exact FP64 solve of a tiny coarsest level; otherwise (or on a
+  // zero pivot) `bottom` block Gauss-Seidel iterations
   {
-    const LinLevel<float>& lv = D.lev[L - 1];
-    WGLin<double, DIM> W;
-    W.init(lv.g);
-    double* de = reinterpret_cast<double*>(smem_raw);   // reuse LDS: 2N + 2N + 4N + 2N doubles
-    double* du = de + 2 * W.N;
-    double* dM = du + 2 * W.N;
-    double* dr = dM + 4 * W.N;
-#pragma unroll
-    for (int m = 0; m < kWGM; ++m) {
-      if (!W.own[m]) continue;
-      const int c = W.c[m];
-      const long long q = W.q[m];
-      de[c] = 0.0;
-      de[W.N + c] = 0.0;
-      dr[c] = double(lv.r[q]);
-      dr[W.N + c] = double(lv.r[q + lv.g.ncell]);
-#pragma unroll
-      for (int mm = 0; mm < 4; ++mm) dM[mm * W.N + c] = double(lv.M[q + mm * lv.g.ncell]);
-    }
-    __syncthreads();
-    // a tiny coarsest level is solved exactly; otherwise (or on a zero
-    // pivot) `bottom` block Gauss-Seidel iterations
     __shared__ int direct_ok;
     bool solved = false;
-    if (2 * W.N <= kDirect)
-      solved = wg_direct_solve<DIM>(lv.g, D.bc, D.alpha, D.beta, D.b64, D.a64, dM, dr, de, dr + 2 * W.N,
-                                    &direct_ok);
-    if (!solved) wg_lin_relax<double, DIM>(W, D.bc, D.alpha, D.beta, D.b64, D.a64, dr, de, du, dM, 2 * D.bottom);
+    if (direct) {
+      const float* fr = S.r(Lb);
+      const float* fM = S.m(Lb);
+      for (int c = threadIdx.x; c < Nb; c += kWGT) {
+        de[c] = 0.0;
+        de[Nb + c] = 0.0;
+        dr[c] = double(fr[c]);
+        dr[Nb + c] = double(fr[Nb + c]);
 #pragma unroll
-    for (int m = 0; m < kWGM; ++m) {
-      if (!W.own[m]) continue;
-      lv.e[W.q[m]] = float(de[W.c[m]]);
-      lv.e[W.q[m] + lv.g.ncell] = float(de[W.N + W.c[m]]);
+        for (int mm = 0; mm < 4; ++mm) dM[mm * Nb + c] = double(fM[mm * Nb + c]);
+      }
+      __syncthreads();
+      if (threadIdx.x < 64) {
+        const bool ok = lc_direct<DIM>(Lb, per, bcw, D.alpha, D.beta, bF, bA, dM, dr, de, dbuf, dbuf + 64);
+        if (threadIdx.x == 0) direct_ok = ok ? 1 : 0;
+      }
+      __syncthreads();
+      solved = direct_ok != 0;
+      if (solved) {
+        float* fe = S.e(Lb);
+        for (int c = threadIdx.x; c < Nb; c += kWGT) {
+          fe[c] = float(de[c]);
+          fe[Nb + c] = float(de[Nb + c]);
+        }
+        __syncthreads();
+      }
     }
-    __syncthreads();
+    if (!solved) {
+      LCCtx<DIM> X;
+      lcx_build<DIM>(X, Lb, per, S, alpha, beta, true);
+      lcx_mvec<DIM>(X, Lb, S);
+      lcx_relax<DIM>(X, Lb, bcw, S, alpha, beta, 2 * D.bottom);
+    }
   }
-
-  for (int l = L - 2; l >= l0; --l) {
-    const LinLevel<float>& lv = D.lev[l];
-    const LinLevel<float>& cv = D.lev[l + 1];
-    WGLin<float, DIM> W;
-    W.init(lv.g);
-    stage(lv, W, false);
+  // up: prolongation and post-smoothing
 #pragma unroll
-    for (int m = 0; m < kWGM; ++m) {
-      if (!W.own[m]) continue;
-      const int c = W.c[m];
-      se[c] = prolong_value<float, DIM>(cv.g, W.i[m], W.j[m], W.k[m], cv.e, nullptr, se[c]);
-      se[W.N + c] = prolong_value<float, DIM>(cv.g, W.i[m], W.j[m], W.k[m], cv.e + cv.g.ncell, nullptr, se[W.N + c]);
-    }
-    __syncthreads();
-    wg_lin_relax<float, DIM>(W, D.bc, alpha, beta, lv.b, lv.a, sr, se, su, sM, 2 * D.nu2);
-    store_e(lv, W);
+  for (int l = LL - 2; l >= 0; --l) {
+    if (l + 1 >= nl) continue;
+    LCCtx<DIM> X;
+    lcx_build<DIM>(X, G[l], per, S, alpha, beta, false);
+    lcx_prolong<DIM>(X, G[l], G[l + 1], per, S);
+    lcx_mvec<DIM>(X, G[l], S);
+    lcx_relax<DIM>(X, G[l], bcw, S, alpha, beta, 2 * D.nu2);
+  }
+  const LinLevel<float>& lv = D.lev[l0];
+  const float* se = S.e(G[0]);
+  for (int c = threadIdx.x; c < G[0].N; c += kWGT) {
+    int i, j, k;
+    clev_coords(G[0], c, i, j, k);
+    const long long q = lv.g.ci(i, j, k);
+    lv.e[q] = se[c];
+    lv.e[q + lv.g.ncell] = se[G[0].N + c];
   }
 }
 
@@ -956,8 +1103,23 @@ hipError_t lin_restrict(const LinLevel<float>& L, const LinLevel<float>& C, cons
   return hipGetLastError();
 }
 
-hipError_t lin_coarse_vcycle(const LinDesc* dev_desc, int dim, hipStream_t s) {
-  const size_t lds = 10 * size_t(kWGC) * sizeof(float);
+size_t lin_coarse_lds_bytes(const Geom* g, int l0, int nlev) {
+  long long tot = 0;
+  for (int l = l0; l < nlev; ++l) tot += g[l].nvalid();
+  const long long nb = g[nlev - 1].nvalid();
+  const int dim = g[l0].dim;
+  const bool direct = 2 * nb <= kDirect;
+  const size_t dbl = direct ? size_t((8 + 4 * dim + 2) * nb + 64 + kDirect + 1) : 0;
+  return ((size_t(16 + 4 * dim) * tot * sizeof(float) + 15) & ~size_t(15)) + dbl * sizeof(double);
+}
+
+size_t lin_coarse_static_lds() {
+  return 1024;   // bound on the kernel's static LDS (level geometry, flags)
+}
+
+int lin_coarse_levels_max(int dim) { return dim == 3 ? LinLevels<3>::n : LinLevels<2>::n; }
+
+hipError_t lin_coarse_vcycle(const LinDesc* dev_desc, int dim, size_t lds, hipStream_t s) {
   if (dim == 2)
     hipLaunchKernelGGL((lin_coarse_vcycle_kernel<2>), dim3(1), dim3(kWGT), lds, s, dev_desc);
   else

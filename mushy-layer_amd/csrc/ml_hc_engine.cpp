@@ -284,6 +284,18 @@ int HCEngine::init_refined() {
   if (m_lc >= m_L) return fail(ctx, ML_ENOTSUP, "FP32-refined form needs an MG level of at most 1024 cells");
   if (m_op->lev[m_L - 1].g.nvalid() > 512)
     return fail(ctx, ML_ENOTSUP, "FP32-refined form needs a coarsest level of at most 512 cells");
+  if (m_L - m_lc > lin_coarse_levels_max(m_op->dim))
+    return fail(ctx, ML_ENOTSUP, "FP32-refined form: too many levels below the single-workgroup level");
+  {
+    std::vector<Geom> gs(m_L);
+    for (int l = 0; l < m_L; ++l) gs[l] = m_op->lev[l].g;
+    m_lin_lds = lin_coarse_lds_bytes(gs.data(), m_lc, m_L);
+    size_t maxlds = 0;
+    int rc = max_lds(ctx, &maxlds);
+    if (rc) return rc;
+    if (m_lin_lds + lin_coarse_static_lds() > maxlds)
+      return fail(ctx, ML_ENOTSUP, "FP32-refined form: the coarse levels exceed the workgroup's LDS");
+  }
   m_lin.assign(m_L, LinLevel<float>{});
   auto alloc = [&](size_t n, float** p) -> int {
     void* v = nullptr;
@@ -342,7 +354,7 @@ int HCEngine::lin_vcycle(const ml_solve_opts& o) {
     ML_HIP(ctx, lin_relax_restrict(m_lin[l], eb, &m_ecur[l], m_lin[l + 1], m_op->bc, alpha, beta, 2 * o.nu1, true,
                                    s));
   }
-  ML_HIP(ctx, lin_coarse_vcycle(m_lindesc, m_op->dim, s));
+  ML_HIP(ctx, lin_coarse_vcycle(m_lindesc, m_op->dim, m_lin_lds, s));
   for (int l = m_lc - 1; l >= 0; --l) {
     const Geom gc = m_lin[l + 1].g;
     float* const eb[2] = {m_lin[l].e, m_ealt[l]};

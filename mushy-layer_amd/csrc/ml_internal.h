@@ -25,6 +25,8 @@ struct ml_ctx_s {
 namespace ml {
 
 int fail(ml_ctx ctx, int code, const std::string& msg);
+// largest dynamic LDS of one workgroup on the context's device (bytes)
+int max_lds(ml_ctx ctx, size_t* out);
 bool valid_grid(int dim, const int n[], const int periodic[], double dx);
 
 #define ML_HIP(ctx, expr)                                                                     \
@@ -114,6 +116,7 @@ class HCEngine {
   LinDesc* m_lindesc = nullptr;      // device
   LinDesc* m_hlindesc = nullptr;     // pinned host staging
   bool m_refined_ok = false;
+  size_t m_lin_lds = 0;               // dynamic LDS of the coarse linear V-cycle
 
   ml_hc_op m_op;
   int m_L = 0, m_lc = 0;
@@ -166,6 +169,7 @@ struct ml_vc_op_s {
   ml::SolveCtl* ctl = nullptr;           // device iteration control
   ml::SolveCtl* hctl = nullptr;          // pinned mirror
   int batch = 4;                         // iterations enqueued per host check
+  size_t coarse_lds = 0;                 // dynamic LDS of the single-workgroup V-cycle
 
   ml::VCArgs<double> args(int depth) const {
     ml::VCArgs<double> A;

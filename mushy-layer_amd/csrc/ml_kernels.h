@@ -125,15 +125,15 @@ __device__ inline T prolong_value(const Geom& gc, int i, int j, int k, const T* 
 // The same prolongation from a coarse correction held in a compact
 // (ghost-free, x fastest) array e of level gc.
 template <typename T, int DIM>
-__device__ inline T prolong_value_lds(const Geom& gc, int i, int j, int k, const T* e, T p) {
+__device__ inline T prolong_value_lds(const int* n, const int* per, int i, int j, int k, const T* e, T p) {
   const T den = T(1) / (DIM == 3 ? T(64) : T(16));
   const T fx1 = T(3) * den, fx2 = T(9) * den, fx3 = T(27) * den, f0 = T(1) * den;
   const int ic = i / 2, jc = j / 2, kc = DIM == 3 ? k / 2 : 0;
   const int ox = 2 * (i % 2) - 1, oy = 2 * (j % 2) - 1, oz = DIM == 3 ? 2 * (k % 2) - 1 : 0;
-  const int X0 = ic, X1 = wrap_clamp(ic + ox, gc.n[0], gc.periodic[0]);
-  const int Y0 = jc, Y1 = wrap_clamp(jc + oy, gc.n[1], gc.periodic[1]);
-  const int Z0 = kc, Z1 = DIM == 3 ? wrap_clamp(kc + oz, gc.n[2], gc.periodic[2]) : 0;
-  const int nx = gc.n[0], nxy = gc.n[0] * gc.n[1];
+  const int X0 = ic, X1 = wrap_clamp(ic + ox, n[0], per[0]);
+  const int Y0 = jc, Y1 = wrap_clamp(jc + oy, n[1], per[1]);
+  const int Z0 = kc, Z1 = DIM == 3 ? wrap_clamp(kc + oz, n[2], per[2]) : 0;
+  const int nx = n[0], nxy = n[0] * n[1];
   auto E = [&](int x, int y, int z) -> T { return e[x + nx * y + nxy * z]; };
   if (DIM == 2) {
     p = p + fx2 * E(X0, Y0, 0) + f0 * E(X1, Y1, 0);
@@ -144,6 +144,11 @@ __device__ inline T prolong_value_lds(const Geom& gc, int i, int j, int k, const
     p = p + fx1 * (E(X1, Y1, Z0) + E(X0, Y1, Z1) + E(X1, Y0, Z1));
   }
   return p;
+}
+
+template <typename T, int DIM>
+__device__ inline T prolong_value_lds(const Geom& gc, int i, int j, int k, const T* e, T p) {
+  return prolong_value_lds<T, DIM>(gc.n, gc.periodic, i, j, k, e, p);
 }
 
 // ---- FP32-refined form (ml_fp32r.hip)
@@ -191,7 +196,12 @@ hipError_t lin_relax_restrict(const LinLevel<float>& L, float* const ebuf[2], in
 hipError_t lin_restrict(const LinLevel<float>& L, const LinLevel<float>& C, const BCDesc& bc, float alpha,
                         float beta, hipStream_t s);
 // single-workgroup linear V-cycle over levels l0.. of the device descriptor
-hipError_t lin_coarse_vcycle(const LinDesc* dev_desc, int dim, hipStream_t s);
+hipError_t lin_coarse_vcycle(const LinDesc* dev_desc, int dim, size_t lds, hipStream_t s);
+// dynamic LDS of lin_coarse_vcycle for levels l0..nlev-1 of g, and the
+// largest number of levels it handles
+size_t lin_coarse_lds_bytes(const Geom* g, int l0, int nlev);
+size_t lin_coarse_static_lds();
+int lin_coarse_levels_max(int dim);
 // cast of FP64 face coefficients to FP32
 hipError_t cast_f64_f32(long long n, const double* x, float* y, hipStream_t s);
 // several casts in one launch
@@ -360,8 +370,13 @@ template <typename T>
 hipError_t vc_restrict_residual(const VCArgs<T>& A, const Geom& gc, const T* phi, const T* rhs, T* resc,
                                 hipStream_t s);
 template <typename T>
-hipError_t vc_coarse_vcycle(const VCDesc<T>* dev_desc, int dim, hipStream_t s);
+hipError_t vc_coarse_vcycle(const VCDesc<T>* dev_desc, int dim, size_t lds, hipStream_t s);
+// dynamic LDS of vc_coarse_vcycle for levels l0..nlev-1 of g
+template <typename T>
+size_t vc_coarse_lds_bytes(const Geom* g, int l0, int nlev);
 bool vc_small_level(const Geom& g);
+// levels (the small level and all coarser) one vc_coarse_vcycle launch handles
+int vc_coarse_levels_max(int dim);
 
 hipError_t vec_axby(long long n, double* lhs, const double* x, const double* y, double a, double b, hipStream_t s);
 hipError_t vec_incr(long long n, double* lhs, const double* x, double sc, hipStream_t s);

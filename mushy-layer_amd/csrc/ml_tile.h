@@ -185,4 +185,106 @@ __device__ inline void pos_record(const Geom& g, const Tile<DIM, TX, TY, TZ>& t,
   r.owner = r.dist == 0 && P.kind == 1 && gi < g.n[0] && gj < g.n[1] && (DIM == 2 || gk < g.n[2]);
 }
 
+// ---- single-workgroup kernels: one level held compactly in LDS (x fastest)
+struct CLev {
+  int n[3];
+  int nx, nxy, N, off;   // off: the level's offset in the workgroup's LDS arrays
+  double dx;
+};
+
+__device__ inline void clev_coords(const CLev& L, int c, int& i, int& j, int& k) {
+  i = c % L.nx;
+  const int r = c / L.nx;
+  j = r % L.n[1];
+  k = r / L.n[1];
+}
+
+__device__ inline void clev_set(CLev& L, const Geom& g, int off) {
+  for (int d = 0; d < 3; ++d) L.n[d] = g.n[d];
+  L.nx = g.n[0];
+  L.nxy = g.n[0] * g.n[1];
+  L.N = L.nxy * g.n[2];
+  L.off = off;
+  L.dx = g.dx;
+}
+
+// wavefront-scope barrier: orders the LDS accesses of the lanes of one wave
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// cells of a level owned by this thread: packed (i | j << 10 | k << 20), -1
+// when none.  Even row length: the pair (2s + par, j, k), (2s + 1 - par, j, k)
+// of opposite colours; otherwise cells t and t + NT.
+struct COwn {
+  int p[2];
+};
+
+__device__ inline int cpack(int i, int j, int k) { return i | (j << 10) | (k << 20); }
+__device__ inline void cunpack(int v, int& i, int& j, int& k) {
+  i = v & 1023;
+  j = (v >> 10) & 1023;
+  k = v >> 20;
+}
+
+// threads owning cells of level L
+template <int NT>
+__device__ inline int cown_active(const CLev& L) {
+  return (L.nx & 1) == 0 ? (L.N >> 1) : (L.N < NT ? L.N : NT);
+}
+
+template <int NT>
+__device__ inline void cown_set(const CLev& L, COwn& o) {
+  const int t = threadIdx.x;
+  o.p[0] = o.p[1] = -1;
+  if ((L.nx & 1) == 0) {
+    const int h = L.nx >> 1;
+    if (t < (L.N >> 1)) {
+      const int s = t % h, r = t / h, j = r % L.n[1], k = r / L.n[1];
+      const int par = (j + k) & 1;
+      o.p[0] = cpack(2 * s + par, j, k);
+      o.p[1] = cpack(2 * s + 1 - par, j, k);
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int c = t + m * NT;
+      if (c >= L.N) continue;
+      int i, j, k;
+      clev_coords(L, c, i, j, k);
+      o.p[m] = cpack(i, j, k);
+    }
+  }
+}
+
+
+// neighbour compact indices of cell (i,j,k) of level L, face f = 2d + s
+// (s = 1: +e_d): periodic images wrap; across a wall the cell itself, with
+// bit f of `wall` set
+template <int DIM>
+__device__ __forceinline__ void clev_nbrs(const CLev& L, const int* per, int i, int j, int k, int* nb, int& wall) {
+  const int iv[3] = {i, j, k};
+  const int c = i + L.nx * j + L.nxy * k;
+  wall = 0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const int nd = L.n[d];
+    const int st = d == 0 ? 1 : (d == 1 ? L.nx : L.nxy);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int v = iv[d] + (s ? 1 : -1);
+      if (v >= 0 && v < nd) {
+        nb[2 * d + s] = c + (s ? st : -st);
+      } else if (per[d]) {
+        nb[2 * d + s] = c + (s ? -(nd - 1) * st : (nd - 1) * st);
+      } else {
+        nb[2 * d + s] = c;
+        wall |= 1 << (2 * d + s);
+      }
+    }
+  }
+}
+
 }  // namespace ml

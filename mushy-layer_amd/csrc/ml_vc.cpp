@@ -83,7 +83,7 @@ int vc_vcycle(ml_vc_op op, const ml_solve_opts& o, int* ecur, const int* skip) {
                                                op->lev[l + 1].g, op->lev[l + 1].r, s));
   }
   if (lc < L) {
-    ML_HIP(ctx, ml::vc_coarse_vcycle<double>(op->desc, op->dim, s));
+    ML_HIP(ctx, ml::vc_coarse_vcycle<double>(op->desc, op->dim, op->coarse_lds, s));
   } else {
     ML_HIP(ctx, ml::vc_relax<double>(args(L - 1), op->lev[L - 1].e, &ecur[L - 1], op->lev[L - 1].r, 2 * o.bottom,
                                       true, nullptr, nullptr, s));
@@ -187,6 +187,8 @@ int ml_vc_op_create(ml_ctx ctx, int dim, const int lo[], const int hi[], double 
       op->lc = l;
       break;
     }
+  if (op->lc < depth && depth - op->lc > ml::vc_coarse_levels_max(dim))
+    return fail(ctx, ML_ENOTSUP, "too many levels below the single-workgroup level");
   ml_vc_op raw = op.get();
   auto bail = [&](int rc) {
     ml_vc_op_destroy(op.release());
@@ -205,6 +207,17 @@ int ml_vc_op_create(ml_ctx ctx, int dim, const int lo[], const int hi[], double 
       if ((rc = vc_alloc(ctx, L.g.ncell, &L.e[1]))) return bail(rc);
   }
   int rc;
+  if (raw->lc < depth) {
+    std::vector<ml::Geom> gs(depth);
+    for (int l = 0; l < depth; ++l) gs[l] = raw->lev[l].g;
+    raw->coarse_lds = ml::vc_coarse_lds_bytes<double>(gs.data(), raw->lc, depth);
+    size_t mx = 0;
+    if ((rc = ml::max_lds(ctx, &mx))) return bail(rc);
+    if (raw->coarse_lds > mx) {
+      ml::fail(ctx, ML_ENOTSUP, "the coarse MG levels exceed the workgroup's LDS");
+      return bail(ML_ENOTSUP);
+    }
+  }
   if ((rc = vc_alloc(ctx, g0.ncell, &raw->alt0))) return bail(rc);
   if ((rc = vc_alloc(ctx, g0.ncell, &raw->rhs_be))) return bail(rc);
   raw->nslots = 1024;

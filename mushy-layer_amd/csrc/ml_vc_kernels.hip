@@ -287,182 +287,253 @@ __global__ __launch_bounds__(kVCThreads) void vc_tile_kernel(VCArgs<T> A, VCTile
 }
 
 // ---------------------------------------------------------------------------
-// Single-workgroup V-cycle over levels l0.. of a device descriptor.
+// Single-workgroup V-cycle over levels l0.. of a device descriptor.  The
+// correction, right-hand side and operator data (faces, a, c, lambda) of
+// every level live in LDS for the whole cycle (compact, x fastest, levels
+// one after another); the operator data are loaded once at the start.  Each
+// thread owns one cell of each colour of a level (even row length) or the
+// cells t and t + 512, with their coordinates held in registers; levels
+// whose owners fit one wavefront are relaxed by wave 0 alone.
+
+constexpr int kVCL = 6;   // levels one launch handles: 1024 cells coarsen to one cell in 6 (2D), 4 (3D)
+template <int DIM>
+struct VCLevels {
+  static constexpr int n = DIM == 3 ? 4 : kVCL;
+};
+
+// operator data of all levels in LDS; q = level offset + compact cell index
 template <typename T, int DIM>
-struct VCWG {
-  Geom g;
-  int N, nx, nxy;
-  int c[kVCM], i[kVCM], j[kVCM], k[kVCM];
-  long long q[kVCM];
-  bool own[kVCM];
-  __device__ inline void init(const Geom& gg) {
-    g = gg;
-    nx = g.n[0];
-    nxy = g.n[0] * g.n[1];
-    N = nxy * (DIM == 3 ? g.n[2] : 1);
+struct VCLds {
+  T *se, *sr, *sres;   // corrections, right-hand sides, restriction scratch
+  T *sF, *sA, *sC, *sL;   // faces [2*DIM][tot], a, c, lambda
+  int tot;
+  __device__ inline void faces(int q, Faces1<T, DIM>& F) const {
 #pragma unroll
-    for (int m = 0; m < kVCM; ++m) {
-      c[m] = threadIdx.x + m * kVCWG;
-      own[m] = c[m] < N;
-      i[m] = c[m] % nx;
-      const int r = c[m] / nx;
-      j[m] = r % g.n[1];
-      k[m] = DIM == 3 ? r / g.n[1] : 0;
-      q[m] = own[m] ? g.ci(i[m], j[m], k[m]) : 0;
-    }
+    for (int f = 0; f < 2 * DIM; ++f) F.v[f] = sF[f * tot + q];
   }
 };
 
-// stencil of cell (i,j,k) of a whole level held in LDS (su), with periodic
-// wrap or the homogeneous wall rule
+template <typename T>
+size_t vc_coarse_lds_elems(int dim, long long tot) {
+  return size_t(tot) * (2 + 2 * dim + 3) + kVCWGCells;
+}
+
+// homogeneous wall ghost of component 0 (bit 2d+s of bcm: type != 0)
+template <typename T>
+__device__ __forceinline__ T vc_wall(int bcm, int d, int s, T interior, T dx) {
+  return bc_ghost<T>((bcm >> (2 * d + s)) & 1, T(0), interior, s ? 1 : -1, dx);
+}
+
+// A thread's cells of one level for the duration of a level visit:
+// coordinates, colour, neighbour indices and operator data in registers.
 template <typename T, int DIM>
-__device__ inline void vcwg_stencil(const Geom& g, const BCDesc& bc, const T* su, int c, int i, int j, int k,
-                                    int nx, int nxy, T* p) {
-  const int iv[3] = {i, j, k};
-  p[0] = su[c];
+struct VCCtx {
+  int pk[2], c[2], col[2], wall[2];
+  int nb[2][2 * DIM];
+  Faces1<T, DIM> F[2];
+  T aq[2], cv[2], lam[2], rr[2];
+};
+
+template <typename T, int DIM>
+__device__ __forceinline__ void vcx_build(VCCtx<T, DIM>& X, const CLev& L, const int* per, const VCLds<T, DIM>& S) {
+  COwn o;
+  cown_set<kVCWG>(L, o);
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    const int nd = g.n[d];
-    const int st = d == 0 ? 1 : (d == 1 ? nx : nxy);
+  for (int m = 0; m < 2; ++m) {
+    X.pk[m] = o.p[m];
+    if (o.p[m] < 0) continue;
+    int i, j, k;
+    cunpack(o.p[m], i, j, k);
+    const int c = i + L.nx * j + L.nxy * k;
+    X.c[m] = c;
+    X.col[m] = (i + j + k) & 1;
+    clev_nbrs<DIM>(L, per, i, j, k, X.nb[m], X.wall[m]);
+    const int q = L.off + c;
+    S.faces(q, X.F[m]);
+    X.aq[m] = S.sA[q];
+    X.cv[m] = S.sC[q];
+    X.lam[m] = S.sL[q];
+    X.rr[m] = S.sr[q];
+  }
+}
+
+// stencil of cell m of the context from the level's corrections su
+template <typename T, int DIM>
+__device__ __forceinline__ void vcx_stencil(const VCCtx<T, DIM>& X, int m, int bcm, T dx, const T* su, T* p) {
+  p[0] = su[X.c[m]];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int v = iv[d] + (s ? 1 : -1);
-      T val;
-      if (v >= 0 && v < nd)
-        val = su[c + (s ? st : -st)];
-      else if (g.periodic[d])
-        val = su[c + (s ? -(nd - 1) * st : (nd - 1) * st)];
-      else
-        val = bc_ghost<T>(bc.type[d][s][0], T(0), su[c], s ? 1 : -1, T(g.dx));
-      p[1 + 2 * d + (s ? 0 : 1)] = val;
-    }
+  for (int f = 0; f < 2 * DIM; ++f) {
+    const int d = f >> 1, s = f & 1;
+    T v = su[X.nb[m][f]];
+    if ((X.wall[m] >> f) & 1) v = vc_wall<T>(bcm, d, s, v, dx);
+    p[1 + 2 * d + (1 - s)] = v;
   }
 }
 
 template <typename T, int DIM>
-__device__ void vcwg_relax(const VCWG<T, DIM>& W, const VCDesc<T>& D, const VCLevel<T>& lv, const T* sr, T* su,
-                           int npass) {
-  const Geom& g = W.g;
-  const T dxinv = T(1) / (T(g.dx) * T(g.dx));
-  VCArgs<T> A;
-  A.g = g;
-  A.bc = D.bc;
-  A.alpha = D.alpha;
-  A.beta = D.beta;
-  A.a = lv.a;
-  A.c = lv.c;
-  for (int d = 0; d < 3; ++d) A.b[d] = lv.b[d];
-  Faces1<T, DIM> F[kVCM];
-  T aq[kVCM], cv[kVCM], lam[kVCM];
+__device__ __forceinline__ void vcx_relax(const VCCtx<T, DIM>& X, const CLev& L, int bcm, const VCLds<T, DIM>& S,
+                                          bool hasc, T alpha, T beta, int npass) {
+  const T dx = T(L.dx);
+  const T dxinv = T(1) / (dx * dx);
+  T* se = S.se + L.off;
+  auto pass = [&](int p) {
 #pragma unroll
-  for (int m = 0; m < kVCM; ++m) {
-    if (!W.own[m]) continue;
-    load_faces1<T, DIM>(A, W.i[m], W.j[m], W.k[m], F[m]);
-    aq[m] = A.a ? A.a[W.q[m]] : T(1);
-    cv[m] = A.c ? A.c[W.q[m]] : T(0);
-    lam[m] = vc_lambda<T, DIM>(F[m], aq[m], A.c != nullptr, cv[m], A.alpha, A.beta, dxinv);
-  }
-  for (int p = 0; p < npass; ++p) {
-#pragma unroll
-    for (int m = 0; m < kVCM; ++m) {
-      if (!W.own[m] || ((W.i[m] + W.j[m] + W.k[m] + p) & 1)) continue;
+    for (int m = 0; m < 2; ++m) {
+      if (X.pk[m] < 0 || X.col[m] != (p & 1)) continue;
       T st[7];
-      vcwg_stencil<T, DIM>(g, D.bc, su, W.c[m], W.i[m], W.j[m], W.k[m], W.nx, W.nxy, st);
-      const T lof = vc_lof<T, DIM>(F[m], st, aq[m], A.c != nullptr, cv[m], A.alpha, A.beta, dxinv);
-      su[W.c[m]] = st[0] - lam[m] * (lof - sr[W.c[m]]);
+      vcx_stencil<T, DIM>(X, m, bcm, dx, se, st);
+      const T lof = vc_lof<T, DIM>(X.F[m], st, X.aq[m], hasc, X.cv[m], alpha, beta, dxinv);
+      se[X.c[m]] = st[0] - X.lam[m] * (lof - X.rr[m]);
     }
+  };
+  if (cown_active<kVCWG>(L) <= 64) {
+    if (threadIdx.x < 64)
+      for (int p = 0; p < npass; ++p) {
+        pass(p);
+        wave_sync();
+      }
     __syncthreads();
+  } else {
+    for (int p = 0; p < npass; ++p) {
+      pass(p);
+      __syncthreads();
+    }
   }
+}
+
+// restricted residual (homogeneous BCs, RESTRICTRESDB order) of level Lf into
+// the right-hand side of level Lc, whose correction starts at zero
+template <typename T, int DIM>
+__device__ __forceinline__ void vcx_restrict(const VCCtx<T, DIM>& X, const CLev& Lf, const CLev& Lc, int bcm,
+                                             const VCLds<T, DIM>& S, bool hasc, T alpha, T beta) {
+  const T dx = T(Lf.dx);
+  const T dxinv = T(1) / (dx * dx);
+  const T denom = DIM == 3 ? T(8) : T(4);
+  const T* se = S.se + Lf.off;
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    if (X.pk[m] < 0) continue;
+    T st[7];
+    vcx_stencil<T, DIM>(X, m, bcm, dx, se, st);
+    const T lof = vc_lof<T, DIM>(X.F[m], st, X.aq[m], hasc, X.cv[m], alpha, beta, dxinv);
+    S.sres[X.c[m]] = (X.rr[m] - lof) / denom;
+  }
+  __syncthreads();
+  for (int cc = threadIdx.x; cc < Lc.N; cc += kVCWG) {
+    int I, J, K;
+    clev_coords(Lc, cc, I, J, K);
+    T s = T(0);
+    for (int kk = 0; kk < (DIM == 3 ? 2 : 1); ++kk)
+      for (int jj = 0; jj < 2; ++jj)
+        for (int ii = 0; ii < 2; ++ii)
+          s = s + S.sres[(2 * I + ii) + Lf.nx * (2 * J + jj) + Lf.nxy * (DIM == 3 ? 2 * K + kk : 0)];
+    S.sr[Lc.off + cc] = s;
+    S.se[Lc.off + cc] = T(0);
+  }
+  __syncthreads();
+}
+
+// e_f += e_c(parent) (piecewise-constant prolongation)
+template <typename T, int DIM>
+__device__ __forceinline__ void vcx_prolong(const VCCtx<T, DIM>& X, const CLev& Lf, const CLev& Lc, T* se) {
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    if (X.pk[m] < 0) continue;
+    int i, j, k;
+    cunpack(X.pk[m], i, j, k);
+    const int c = Lf.off + X.c[m];
+    se[c] = se[c] + se[Lc.off + (i / 2) + Lc.nx * (j / 2) + Lc.nxy * (DIM == 3 ? k / 2 : 0)];
+  }
+  __syncthreads();
 }
 
 template <typename T, int DIM>
 __global__ __launch_bounds__(kVCWG) void vc_coarse_vcycle_kernel(const VCDesc<T>* __restrict__ Dp) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int LL = VCLevels<DIM>::n;
   const VCDesc<T>& D = *Dp;
   if (D.skip && *D.skip) return;
-  T* su = reinterpret_cast<T*>(smem_raw);   // e of the level (kVCWGCells)
-  T* sr = su + kVCWGCells;                  // r of the level
-  const int l0 = D.l0, L = D.nlev;
-  auto stage = [&](const VCLevel<T>& lv, const VCWG<T, DIM>& W, bool zero_e) {
+  const int l0 = D.l0, nl = D.nlev - D.l0;
+  const bool hasc = D.lev[l0].c != nullptr;
+  const T alpha = D.alpha, beta = D.beta;
+  int per[3];
+  for (int d = 0; d < 3; ++d) per[d] = D.lev[l0].g.periodic[d];
+  int bcm = 0;
+  for (int d = 0; d < DIM; ++d)
+    for (int sd = 0; sd < 2; ++sd) bcm |= (D.bc.type[d][sd][0] != 0 ? 1 : 0) << (2 * d + sd);
+  CLev G[LL];
+  int tot = 0;
 #pragma unroll
-    for (int m = 0; m < kVCM; ++m) {
-      if (!W.own[m]) continue;
-      su[W.c[m]] = zero_e ? T(0) : lv.e[W.q[m]];
-      sr[W.c[m]] = lv.r[W.q[m]];
-    }
-    __syncthreads();
-  };
-  auto store = [&](const VCLevel<T>& lv, const VCWG<T, DIM>& W) {
+  for (int l = 0; l < LL; ++l) {
+    if (l >= nl) continue;
+    clev_set(G[l], D.lev[l0 + l].g, tot);
+    tot += G[l].N;
+  }
+  VCLds<T, DIM> S;
+  S.tot = tot;
+  S.se = reinterpret_cast<T*>(smem_raw);
+  S.sr = S.se + tot;
+  S.sF = S.sr + tot;
+  S.sA = S.sF + 2 * DIM * tot;
+  S.sC = S.sA + tot;
+  S.sL = S.sC + tot;
+  S.sres = S.sL + tot;
+  // operator data of every level, and level l0's right-hand side
 #pragma unroll
-    for (int m = 0; m < kVCM; ++m)
-      if (W.own[m]) lv.e[W.q[m]] = su[W.c[m]];
-    __syncthreads();
-  };
-  for (int l = l0; l < L - 1; ++l) {
-    const VCLevel<T>& lv = D.lev[l];
-    const VCLevel<T>& cv = D.lev[l + 1];
-    VCWG<T, DIM> W;
-    W.init(lv.g);
-    stage(lv, W, true);
-    vcwg_relax<T, DIM>(W, D, lv, sr, su, 2 * D.nu1);
-    store(lv, W);
-    // restricted residual (homogeneous), children in Fortran order
-    const Geom& g = lv.g;
-    const Geom& gcc = cv.g;
-    const T dxinv = T(1) / (T(g.dx) * T(g.dx));
-    const T denom = DIM == 3 ? T(8) : T(4);
+  for (int l = 0; l < LL; ++l) {
+    if (l >= nl) continue;
+    const VCLevel<T>& lv = D.lev[l0 + l];
     VCArgs<T> A;
-    A.g = g;
-    A.bc = D.bc;
-    A.alpha = D.alpha;
-    A.beta = D.beta;
+    A.g = lv.g;
     A.a = lv.a;
     A.c = lv.c;
     for (int d = 0; d < 3; ++d) A.b[d] = lv.b[d];
-    const int NC = gcc.n[0] * gcc.n[1] * (DIM == 3 ? gcc.n[2] : 1);
-    for (int cc = threadIdx.x; cc < NC; cc += kVCWG) {
-      const int I = cc % gcc.n[0], rq = cc / gcc.n[0], J = rq % gcc.n[1], K = DIM == 3 ? rq / gcc.n[1] : 0;
-      T s = T(0);
-      for (int kk = 0; kk < (DIM == 3 ? 2 : 1); ++kk)
-        for (int jj = 0; jj < 2; ++jj)
-          for (int ii = 0; ii < 2; ++ii) {
-            const int fi = 2 * I + ii, fj = 2 * J + jj, fk = DIM == 3 ? 2 * K + kk : 0;
-            const int fc = fi + W.nx * fj + W.nxy * fk;
-            Faces1<T, DIM> F;
-            load_faces1<T, DIM>(A, fi, fj, fk, F);
-            const long long q = g.ci(fi, fj, fk);
-            const T aq = A.a ? A.a[q] : T(1);
-            const T cvv = A.c ? A.c[q] : T(0);
-            T st[7];
-            vcwg_stencil<T, DIM>(g, D.bc, su, fc, fi, fj, fk, W.nx, W.nxy, st);
-            s = s + (sr[fc] - vc_lof<T, DIM>(F, st, aq, A.c != nullptr, cvv, A.alpha, A.beta, dxinv)) / denom;
-          }
-      cv.r[gcc.ci(I, J, K)] = s;
-    }
-    __syncthreads();
-  }
-  {
-    const VCLevel<T>& lv = D.lev[L - 1];
-    VCWG<T, DIM> W;
-    W.init(lv.g);
-    stage(lv, W, true);
-    vcwg_relax<T, DIM>(W, D, lv, sr, su, 2 * D.bottom);
-    store(lv, W);
-  }
-  for (int l = L - 2; l >= l0; --l) {
-    const VCLevel<T>& lv = D.lev[l];
-    const VCLevel<T>& cv = D.lev[l + 1];
-    VCWG<T, DIM> W;
-    W.init(lv.g);
-    stage(lv, W, false);
+    const T dxinv = T(1) / (T(G[l].dx) * T(G[l].dx));
+    for (int c = threadIdx.x; c < G[l].N; c += kVCWG) {
+      int i, j, k;
+      clev_coords(G[l], c, i, j, k);
+      const int q = G[l].off + c;
+      Faces1<T, DIM> F;
+      load_faces1<T, DIM>(A, i, j, k, F);
 #pragma unroll
-    for (int m = 0; m < kVCM; ++m) {
-      if (!W.own[m]) continue;
-      su[W.c[m]] = su[W.c[m]] + cv.e[cv.g.ci(W.i[m] / 2, W.j[m] / 2, DIM == 3 ? W.k[m] / 2 : 0)];
+      for (int f = 0; f < 2 * DIM; ++f) S.sF[f * tot + q] = F.v[f];
+      const long long g = lv.g.ci(i, j, k);
+      const T aq = A.a ? A.a[g] : T(1);
+      const T cv = A.c ? A.c[g] : T(0);
+      S.sA[q] = aq;
+      S.sC[q] = cv;
+      S.sL[q] = vc_lambda<T, DIM>(F, aq, hasc, cv, alpha, beta, dxinv);
+      if (l == 0) {
+        S.se[q] = T(0);
+        S.sr[q] = lv.r[g];
+      }
     }
-    __syncthreads();
-    vcwg_relax<T, DIM>(W, D, lv, sr, su, 2 * D.nu2);
-    store(lv, W);
+  }
+  __syncthreads();
+  // down: pre-smoothing and restriction; the coarsest level gets the bottom
+  // smoothing
+#pragma unroll
+  for (int l = 0; l < LL; ++l) {
+    if (l >= nl) continue;
+    VCCtx<T, DIM> X;
+    vcx_build<T, DIM>(X, G[l], per, S);
+    vcx_relax<T, DIM>(X, G[l], bcm, S, hasc, alpha, beta, 2 * (l == nl - 1 ? D.bottom : D.nu1));
+    if (l + 1 < nl) vcx_restrict<T, DIM>(X, G[l], G[l + 1], bcm, S, hasc, alpha, beta);
+  }
+  // up: prolongation and post-smoothing
+#pragma unroll
+  for (int l = LL - 2; l >= 0; --l) {
+    if (l + 1 >= nl) continue;
+    VCCtx<T, DIM> X;
+    vcx_build<T, DIM>(X, G[l], per, S);
+    vcx_prolong<T, DIM>(X, G[l], G[l + 1], S.se);
+    vcx_relax<T, DIM>(X, G[l], bcm, S, hasc, alpha, beta, 2 * D.nu2);
+  }
+  const VCLevel<T>& lv = D.lev[l0];
+  for (int c = threadIdx.x; c < G[0].N; c += kVCWG) {
+    int i, j, k;
+    clev_coords(G[0], c, i, j, k);
+    lv.e[lv.g.ci(i, j, k)] = S.se[c];
   }
 }
 
@@ -592,8 +663,14 @@ hipError_t vc_restrict_residual(const VCArgs<T>& A, const Geom& gc, const T* phi
 }
 
 template <typename T>
-hipError_t vc_coarse_vcycle(const VCDesc<T>* dev_desc, int dim, hipStream_t s) {
-  const size_t lds = size_t(2) * kVCWGCells * sizeof(T);
+size_t vc_coarse_lds_bytes(const Geom* g, int l0, int nlev) {
+  long long tot = 0;
+  for (int l = l0; l < nlev; ++l) tot += g[l].nvalid();
+  return vc_coarse_lds_elems<T>(g[l0].dim, tot) * sizeof(T);
+}
+
+template <typename T>
+hipError_t vc_coarse_vcycle(const VCDesc<T>* dev_desc, int dim, size_t lds, hipStream_t s) {
   if (dim == 2)
     hipLaunchKernelGGL((vc_coarse_vcycle_kernel<T, 2>), dim3(1), dim3(kVCWG), lds, s, dev_desc);
   else
@@ -602,6 +679,7 @@ hipError_t vc_coarse_vcycle(const VCDesc<T>* dev_desc, int dim, hipStream_t s) {
 }
 
 bool vc_small_level(const Geom& g) { return g.nvalid() <= kVCWGCells; }
+int vc_coarse_levels_max(int dim) { return dim == 3 ? VCLevels<3>::n : VCLevels<2>::n; }
 
 template hipError_t vc_relax<double>(const VCArgs<double>&, double* const[2], int*, const double*, int, bool,
                                      const Geom*, const double*, hipStream_t);
@@ -612,6 +690,7 @@ template hipError_t vc_residual<double>(const VCArgs<double>&, const double*, co
 template hipError_t vc_apply_op<double>(const VCArgs<double>&, const double*, double*, int, hipStream_t);
 template hipError_t vc_restrict_residual<double>(const VCArgs<double>&, const Geom&, const double*, const double*,
                                                  double*, hipStream_t);
-template hipError_t vc_coarse_vcycle<double>(const VCDesc<double>*, int, hipStream_t);
+template hipError_t vc_coarse_vcycle<double>(const VCDesc<double>*, int, size_t, hipStream_t);
+template size_t vc_coarse_lds_bytes<double>(const Geom*, int, int);
 
 }  // namespace ml
