@@ -229,23 +229,25 @@ __global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, c
   PosRec R[KMAX];
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) pos_record<DIM, TX, TY, TZ>(g, t, threadIdx.x + kk * NT, R[kk]);
+  // every global load of the tile is issued before the first use
+  float ev[KMAX][2], Mv[KMAX][4];
+  ProlongCorners<float, DIM> P0[KMAX], P1[KMAX];
+  CellFaces<float, DIM> F[KMAX];
+  float rr[KMAX][2], aq[KMAX][2], inv[KMAX][4];
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) {
     const PosRec& r = R[kk];
     if (r.kind != 1) continue;
-    float e0 = zero_init ? 0.0f : ein[r.q], e1 = zero_init ? 0.0f : ein[r.q + g.ncell];
+    ev[kk][0] = zero_init ? 0.0f : ein[r.q];
+    ev[kk][1] = zero_init ? 0.0f : ein[r.q + g.ncell];
     if (pc) {
-      e0 = prolong_value<float, DIM>(gc, r.i, r.j, r.k, pc, nullptr, e0);
-      e1 = prolong_value<float, DIM>(gc, r.i, r.j, r.k, pc + gc.ncell, nullptr, e1);
+      prolong_fetch<float, DIM>(gc, r.i, r.j, r.k, pc, P0[kk]);
+      prolong_fetch<float, DIM>(gc, r.i, r.j, r.k, pc + gc.ncell, P1[kk]);
     }
-    se[r.e] = e0;
-    se[E + r.e] = e1;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) sM[m * E + r.e] = L.M[r.q + m * g.ncell];
+    for (int m = 0; m < 4; ++m) Mv[kk][m] = L.M[r.q + m * g.ncell];
   }
   // operator data of every position this thread may update, in registers
-  CellFaces<float, DIM> F[KMAX];
-  float rr[KMAX][2], aq[KMAX][2], inv[KMAX][4];
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) {
     const PosRec& r = R[kk];
@@ -255,8 +257,22 @@ __global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, c
     rr[kk][1] = L.r[r.q + g.ncell];
     aq[kk][0] = L.a ? L.a[r.q] : 1.0f;
     aq[kk][1] = L.a ? L.a[r.q + g.ncell] : 1.0f;
-    const float Mc[4] = {L.M[r.q], L.M[r.q + g.ncell], L.M[r.q + 2 * g.ncell], L.M[r.q + 3 * g.ncell]};
-    block_inverse<float, DIM>(F[kk], aq[kk], Mc, alpha, beta, dxinv, inv[kk]);
+  }
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    if (r.kind != 1) continue;
+    float e0 = ev[kk][0], e1 = ev[kk][1];
+    if (pc) {
+      e0 = prolong_combine<float, DIM>(P0[kk], e0);
+      e1 = prolong_combine<float, DIM>(P1[kk], e1);
+    }
+    se[r.e] = e0;
+    se[E + r.e] = e1;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) sM[m * E + r.e] = Mv[kk][m];
+    if (r.dist > t.H - 1) continue;
+    block_inverse<float, DIM>(F[kk], aq[kk], Mv[kk], alpha, beta, dxinv, inv[kk]);
   }
   __syncthreads();
   // u = M e at every staged position (wall ghosts by the homogeneous rule)

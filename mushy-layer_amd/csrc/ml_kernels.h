@@ -122,6 +122,53 @@ __device__ inline T prolong_value(const Geom& gc, int i, int j, int k, const T* 
   return p;
 }
 
+// The same prolongation in two steps, so that the loads of several cells can
+// be in flight together: the coarse values (pc alone) in the order the
+// combination uses them, then the weighted sum added to p.
+template <typename T, int DIM>
+struct ProlongCorners {
+  T v[DIM == 3 ? 8 : 4];
+};
+
+template <typename T, int DIM>
+__device__ inline void prolong_fetch(const Geom& gc, int i, int j, int k, const T* pc, ProlongCorners<T, DIM>& P) {
+  const int ic = i / 2, jc = j / 2, kc = DIM == 3 ? k / 2 : 0;
+  const int ox = 2 * (i % 2) - 1, oy = 2 * (j % 2) - 1, oz = DIM == 3 ? 2 * (k % 2) - 1 : 0;
+  const int X0 = ic, X1 = wrap_clamp(ic + ox, gc.n[0], gc.periodic[0]);
+  const int Y0 = jc, Y1 = wrap_clamp(jc + oy, gc.n[1], gc.periodic[1]);
+  const int Z0 = kc, Z1 = DIM == 3 ? wrap_clamp(kc + oz, gc.n[2], gc.periodic[2]) : 0;
+  if (DIM == 2) {
+    P.v[0] = pc[gc.ci(X0, Y0, 0)];
+    P.v[1] = pc[gc.ci(X1, Y1, 0)];
+    P.v[2] = pc[gc.ci(X1, Y0, 0)];
+    P.v[3] = pc[gc.ci(X0, Y1, 0)];
+  } else {
+    P.v[0] = pc[gc.ci(X0, Y0, Z0)];
+    P.v[1] = pc[gc.ci(X1, Y1, Z1)];
+    P.v[2] = pc[gc.ci(X1, Y0, Z0)];
+    P.v[3] = pc[gc.ci(X0, Y1, Z0)];
+    P.v[4] = pc[gc.ci(X0, Y0, Z1)];
+    P.v[5] = pc[gc.ci(X1, Y1, Z0)];
+    P.v[6] = pc[gc.ci(X0, Y1, Z1)];
+    P.v[7] = pc[gc.ci(X1, Y0, Z1)];
+  }
+}
+
+template <typename T, int DIM>
+__device__ inline T prolong_combine(const ProlongCorners<T, DIM>& P, T p) {
+  const T den = T(1) / (DIM == 3 ? T(64) : T(16));
+  const T fx1 = T(3) * den, fx2 = T(9) * den, fx3 = T(27) * den, f0 = T(1) * den;
+  if (DIM == 2) {
+    p = p + fx2 * P.v[0] + f0 * P.v[1];
+    p = p + fx1 * (P.v[2] + P.v[3]);
+  } else {
+    p = p + fx3 * P.v[0] + f0 * P.v[1];
+    p = p + fx2 * (P.v[2] + P.v[3] + P.v[4]);
+    p = p + fx1 * (P.v[5] + P.v[6] + P.v[7]);
+  }
+  return p;
+}
+
 // The same prolongation from a coarse correction held in a compact
 // (ghost-free, x fastest) array e of level gc.
 template <typename T, int DIM>

@@ -98,19 +98,34 @@ __global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T*
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) pos_record<DIM, TX, TY, TZ>(g, t, threadIdx.x + kk * NT, R[kk]);
   Faces1<T, DIM> F[KMAX];
-  T aq[KMAX], cv[KMAX], lam[KMAX], rr[KMAX];
+  T aq[KMAX], cv[KMAX], lam[KMAX], rr[KMAX], v[KMAX], pv[KMAX];
+  // every global load of the tile is issued before the first use
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) {
     const PosRec& r = R[kk];
+    v[kk] = T(0);
+    pv[kk] = T(0);
     if (r.kind != 1) continue;
-    T v = zero_init ? T(0) : in[r.q];
-    if (pc) v = v + pc[gc.ci(r.i / 2, r.j / 2, DIM == 3 ? r.k / 2 : 0)];
-    su[r.e] = v;
-    if (r.dist > t.H - 1) continue;
+    if (!zero_init) v[kk] = in[r.q];
+    if (pc) pv[kk] = pc[gc.ci(r.i / 2, r.j / 2, DIM == 3 ? r.k / 2 : 0)];
+  }
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    if (r.kind != 1 || r.dist > t.H - 1) continue;
     load_faces1<T, DIM>(A, r.i, r.j, r.k, F[kk]);
     aq[kk] = A.a ? A.a[r.q] : T(1);
     cv[kk] = A.c ? A.c[r.q] : T(0);
     rr[kk] = rhs[r.q];
+  }
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    if (r.kind != 1) continue;
+    T val = v[kk];
+    if (pc) val = val + pv[kk];
+    su[r.e] = val;
+    if (r.dist > t.H - 1) continue;
     lam[kk] = vc_lambda<T, DIM>(F[kk], aq[kk], A.c != nullptr, cv[kk], A.alpha, A.beta, dxinv);
   }
   __syncthreads();
@@ -205,13 +220,36 @@ __global__ __launch_bounds__(kVCThreads) void vc_tile_kernel(VCArgs<T> A, VCTile
   PosRec R[KMAX];
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) pos_record<DIM, TX, TY, TZ>(g, t, threadIdx.x + kk * kVCThreads, R[kk]);
+  // owner data of the residual / operator modes, loaded with the state
+  Faces1<T, DIM> OF[KMAX];
+  T oaq[KMAX], ocv[KMAX], orhs[KMAX], v[KMAX], ev[KMAX];
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    v[kk] = T(0);
+    ev[kk] = T(0);
+    if (r.kind != 1) continue;
+    v[kk] = io.phi[r.q];
+    if (io.e) ev[kk] = io.e[r.q];
+  }
+  if (MODE == kVCResid || MODE == kVCApply) {
+#pragma unroll
+    for (int kk = 0; kk < KMAX; ++kk) {
+      const PosRec& r = R[kk];
+      if (!r.owner) continue;
+      load_faces1<T, DIM>(A, r.i, r.j, r.k, OF[kk]);
+      oaq[kk] = A.a ? A.a[r.q] : T(1);
+      ocv[kk] = A.c ? A.c[r.q] : T(0);
+      if (MODE == kVCResid) orhs[kk] = io.rhs[r.q];
+    }
+  }
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) {
     const PosRec& r = R[kk];
     if (r.kind != 1) continue;
-    T v = io.phi[r.q];
-    if (io.e) v = v + io.e[r.q];
-    su[r.e] = v;
+    T val = v[kk];
+    if (io.e) val = val + ev[kk];
+    su[r.e] = val;
   }
   __syncthreads();
 #pragma unroll
@@ -230,15 +268,11 @@ __global__ __launch_bounds__(kVCThreads) void vc_tile_kernel(VCArgs<T> A, VCTile
       const PosRec& r = R[kk];
       if (!r.owner) continue;
       if (MODE == kVCResid && io.e) io.phi_out[r.q] = su[r.e];
-      Faces1<T, DIM> F;
-      load_faces1<T, DIM>(A, r.i, r.j, r.k, F);
-      const T aq = A.a ? A.a[r.q] : T(1);
-      const T cv = A.c ? A.c[r.q] : T(0);
       T st[7];
       gather_lds<T, DIM>(su, r.e, t.EX, t.EXY, st);
-      const T lof = vc_lof<T, DIM>(F, st, aq, A.c != nullptr, cv, A.alpha, A.beta, dxinv);
+      const T lof = vc_lof<T, DIM>(OF[kk], st, oaq[kk], A.c != nullptr, ocv[kk], A.alpha, A.beta, dxinv);
       if (MODE == kVCResid) {
-        const T res = io.rhs[r.q] - lof;
+        const T res = orhs[kk] - lof;
         if (io.out) io.out[r.q] = res;
         const T ar = fabs(res);
         if (ar != ar) nan = 1;
