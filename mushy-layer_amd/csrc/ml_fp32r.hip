@@ -81,7 +81,8 @@ __global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_kernel(HCArg
                                                                            double omega,
                                                                            float* __restrict__ r32,
                                                                            float* __restrict__ M32,
-                                                                           unsigned long long* norm, int ntx, int nty) {
+                                                                           unsigned long long* norm, int ntx, int nty,
+                                                                           CtlFuse fuse) {
   if (A.skip && *A.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2) * (TY + 2) * (DIM == 3 ? TZ + 2 : 1);
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_kernel(HCArg
 #pragma unroll
     for (int m = 0; m < 4; ++m) M32[r.q + m * g.ncell] = float(M[m]);
   }
-  block_max_atomic<double>(mx, nan, norm);
+  block_max_atomic<double>(mx, nan, norm, fuse);
 }
 
 // --------------------------------------------------------------------------
@@ -806,6 +807,71 @@ __device__ __forceinline__ bool lc_direct(const CLev& L, const int* per, int bcw
   return true;
 }
 
+// one cell's operator data (level l0 also: state and Jacobian; the
+// coarsest level of an exact solve also: FP64 faces and a) read from global
+// memory
+template <int DIM>
+struct LinPre {
+  CellFaces<float, DIM> F;
+  float aq[2], r[2], M[4];
+  CellFaces<double, DIM> Fd;
+  double ad[2];
+};
+
+template <int DIM>
+__device__ __forceinline__ void linpre_load(LinPre<DIM>& P, const LinLevel<float>& lv, const CLev& L, int c,
+                                            bool state, bool fp64, const LinDesc& D) {
+  if (c >= L.N) return;
+  int i, j, k;
+  clev_coords(L, c, i, j, k);
+  load_faces<float, DIM>(lv.g, lv.b, i, j, k, P.F);
+  const long long g = lv.g.ci(i, j, k);
+#pragma unroll
+  for (int n = 0; n < 2; ++n) P.aq[n] = lv.a ? lv.a[g + n * lv.g.ncell] : 1.0f;
+  if (state) {
+#pragma unroll
+    for (int n = 0; n < 2; ++n) P.r[n] = lv.r[g + n * lv.g.ncell];
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm) P.M[mm] = lv.M[g + mm * lv.g.ncell];
+  }
+  if (fp64) {
+    load_faces<double, DIM>(lv.g, D.b64, i, j, k, P.Fd);
+#pragma unroll
+    for (int n = 0; n < 2; ++n) P.ad[n] = D.a64 ? D.a64[g + n * lv.g.ncell] : 1.0;
+  }
+}
+
+template <int DIM>
+__device__ __forceinline__ void linpre_store(const LinPre<DIM>& P, const LinLds<DIM>& S, const CLev& L, int c,
+                                             bool state, bool fp64, double* bF, double* bA) {
+  if (c >= L.N) return;
+  const int q = L.off + c;
+  const int tot = S.tot;
+#pragma unroll
+  for (int n = 0; n < 2; ++n) {
+#pragma unroll
+    for (int f = 0; f < 2 * DIM; ++f) S.sF[(n * 2 * DIM + f) * tot + q] = P.F.v[n][f];
+    S.sA[n * tot + q] = P.aq[n];
+  }
+  if (state) {
+    const int N = L.N;
+    S.se[c] = 0.f;
+    S.se[N + c] = 0.f;
+    S.sr[c] = P.r[0];
+    S.sr[N + c] = P.r[1];
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm) S.sM[mm * N + c] = P.M[mm];
+  }
+  if (fp64) {
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+#pragma unroll
+      for (int f = 0; f < 2 * DIM; ++f) bF[(c * 2 + n) * 2 * DIM + f] = P.Fd.v[n][f];
+      bA[c * 2 + n] = P.ad[n];
+    }
+  }
+}
+
 template <int DIM>
 __global__ __launch_bounds__(kWGT) void lin_coarse_vcycle_kernel(const LinDesc* __restrict__ Dp) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -856,45 +922,21 @@ __global__ __launch_bounds__(kWGT) void lin_coarse_vcycle_kernel(const LinDesc* 
   double* bA = bF + 4 * DIM * Nb;
   double* dbuf = bA + 2 * Nb;
   // operator data of every level (and the coarsest level's FP64 copy for
-  // the exact solve), level l0's state
+  // the exact solve), level l0's state: all global loads first (level l0:
+  // cells t and t + 512, coarser levels: cell t), then the LDS stores
+  LinPre<DIM> P0[2], P1[LL - 1];
 #pragma unroll
-  for (int l = 0; l < LL; ++l) {
-    if (l >= nl) continue;
-    const LinLevel<float>& lv = D.lev[l0 + l];
-    for (int c = threadIdx.x; c < G[l].N; c += kWGT) {
-      int i, j, k;
-      clev_coords(G[l], c, i, j, k);
-      const int q = G[l].off + c;
-      CellFaces<float, DIM> F;
-      load_faces<float, DIM>(lv.g, lv.b, i, j, k, F);
-      const long long g = lv.g.ci(i, j, k);
+  for (int m = 0; m < 2; ++m)
+    linpre_load<DIM>(P0[m], D.lev[l0], G[0], threadIdx.x + m * kWGT, true, direct && lb == 0, D);
 #pragma unroll
-      for (int n = 0; n < 2; ++n) {
+  for (int l = 1; l < LL; ++l)
+    if (l < nl) linpre_load<DIM>(P1[l - 1], D.lev[l0 + l], G[l], threadIdx.x, false, direct && lb == l, D);
 #pragma unroll
-        for (int f = 0; f < 2 * DIM; ++f) S.sF[(n * 2 * DIM + f) * tot + q] = F.v[n][f];
-        S.sA[n * tot + q] = lv.a ? lv.a[g + n * lv.g.ncell] : 1.0f;
-      }
-      if (l == 0) {
-        const int N = G[0].N;
-        S.se[c] = 0.f;
-        S.se[N + c] = 0.f;
-        S.sr[c] = lv.r[g];
-        S.sr[N + c] = lv.r[g + lv.g.ncell];
+  for (int m = 0; m < 2; ++m)
+    linpre_store<DIM>(P0[m], S, G[0], threadIdx.x + m * kWGT, true, direct && lb == 0, bF, bA);
 #pragma unroll
-        for (int mm = 0; mm < 4; ++mm) S.sM[mm * N + c] = lv.M[g + mm * lv.g.ncell];
-      }
-      if (direct && l == lb) {
-        CellFaces<double, DIM> Fd;
-        load_faces<double, DIM>(lv.g, D.b64, i, j, k, Fd);
-#pragma unroll
-        for (int n = 0; n < 2; ++n) {
-#pragma unroll
-          for (int f = 0; f < 2 * DIM; ++f) bF[(c * 2 + n) * 2 * DIM + f] = Fd.v[n][f];
-          bA[c * 2 + n] = D.a64 ? D.a64[g + n * lv.g.ncell] : 1.0;
-        }
-      }
-    }
-  }
+  for (int l = 1; l < LL; ++l)
+    if (l < nl) linpre_store<DIM>(P1[l - 1], S, G[l], threadIdx.x, false, direct && lb == l, bF, bA);
   __syncthreads();
   // down: pre-smoothing and restriction to the coarsest level
 #pragma unroll
@@ -1009,7 +1051,7 @@ size_t lin_extent(int halo) {
 
 hipError_t hc_residual_jacobian(const HCArgs<double>& A, const double* phi, double* phi_out, const double* rhs,
                                 const float* e, double omega, float* r32, float* M32, unsigned long long* norm,
-                                hipStream_t s) {
+                                hipStream_t s, const CtlFuse& fuse) {
   if (e && (!phi_out || phi_out == phi)) return hipErrorInvalidValue;
   int ntx, nty;
   if (A.g.dim == 2) {
@@ -1017,13 +1059,13 @@ hipError_t hc_residual_jacobian(const HCArgs<double>& A, const double* phi, doub
     const int nt = lin_tiles<2>(A.g, ntx, nty);
     const size_t lds = 4 * lin_extent<2>(1) * sizeof(double);
     hipLaunchKernelGGL((hc_residual_jacobian_kernel<2, D::TX, D::TY, D::TZ>), dim3(nt), dim3(kLinThreads), lds, s, A,
-                       phi, phi_out, rhs, e, omega, r32, M32, norm, ntx, nty);
+                       phi, phi_out, rhs, e, omega, r32, M32, norm, ntx, nty, fuse);
   } else {
     using D = LinTile<3>;
     const int nt = lin_tiles<3>(A.g, ntx, nty);
     const size_t lds = 4 * lin_extent<3>(1) * sizeof(double);
     hipLaunchKernelGGL((hc_residual_jacobian_kernel<3, D::TX, D::TY, D::TZ>), dim3(nt), dim3(kLinThreads), lds, s, A,
-                       phi, phi_out, rhs, e, omega, r32, M32, norm, ntx, nty);
+                       phi, phi_out, rhs, e, omega, r32, M32, norm, ntx, nty, fuse);
   }
   return hipGetLastError();
 }

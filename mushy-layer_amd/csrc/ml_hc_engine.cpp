@@ -47,6 +47,7 @@ HCEngine::~HCEngine() {
   if (m_desc) hipFree(m_desc);
   if (m_hdesc) hipHostFree(m_hdesc);
   if (m_ctl) hipFree(m_ctl);
+  if (m_arrive) hipFree(m_arrive);
   if (m_hctl) hipHostFree(m_hctl);
 }
 
@@ -100,6 +101,9 @@ int HCEngine::init() {
   m_hdesc = static_cast<CoarseDesc<double>*>(p);
   ML_HIP(ctx, hipMalloc(&p, sizeof(SolveCtl)));
   m_ctl = static_cast<SolveCtl*>(p);
+  ML_HIP(ctx, hipMalloc(&p, sizeof(unsigned int)));
+  m_arrive = static_cast<unsigned int*>(p);
+  ML_HIP(ctx, hipMemsetAsync(m_arrive, 0, sizeof(unsigned int), ctx->stream));
   ML_HIP(ctx, hipHostMalloc(&p, sizeof(SolveCtl), hipHostMallocDefault));
   m_hctl = static_cast<SolveCtl*>(p);
   return ML_OK;
@@ -204,11 +208,12 @@ int HCEngine::vcycle(double* phi, const double* rhs, const ml_solve_opts& o, con
   return ML_OK;
 }
 
-int HCEngine::residual_norm(double* phi, const double* rhs, int slot, const int* skip) {
+int HCEngine::residual_norm(double* phi, const double* rhs, int slot, const int* skip, bool check) {
   ml_ctx ctx = m_op->ctx;
   HCArgs<double> A = m_op->args(0);
   A.skip = skip;
-  ML_HIP(ctx, hc_residual<double>(A, phi, rhs, nullptr, m_slots + slot, 0, ctx->stream));
+  const CtlFuse f = check ? CtlFuse{m_ctl, m_slots, m_arrive, 0} : CtlFuse{};
+  ML_HIP(ctx, hc_residual<double>(A, phi, rhs, nullptr, m_slots + slot, 0, ctx->stream, f));
   return ML_OK;
 }
 
@@ -227,7 +232,7 @@ int HCEngine::solve(double* phi, const double* rhs, const ml_solve_opts& o, ml_s
   ML_HIP(ctx, hipMemcpyAsync(m_alt[0], phi, bytes0, hipMemcpyDeviceToDevice, s));
   std::fill(m_cur.begin(), m_cur.end(), 0);
   ML_HIP(ctx, zero_u64(m_slots, o.max_iter + 1, s));
-  if ((rc = residual_norm(phi, rhs, 0, nullptr))) return rc;
+  if ((rc = residual_norm(phi, rhs, 0, nullptr, false))) return rc;
   ML_HIP(ctx, ctl_init(m_ctl, m_slots, o.max_iter, o.eps, o.hang, o.norm_thresh, s));
   const int* skip = &m_ctl->done;
   int flip0 = -1;   // level-0 buffer flips per V-cycle (fixed by the launch sequence)
@@ -240,8 +245,8 @@ int HCEngine::solve(double* phi, const double* rhs, const ml_solve_opts& o, ml_s
       if ((rc = vcycle(phi, rhs, o, skip))) return rc;
       if (flip0 < 0) flip0 = m_cur[0] != before ? 1 : 0;
       ++launched;
-      if ((rc = residual_norm(cur(0, phi), rhs, launched, skip))) return rc;
-      ML_HIP(ctx, ctl_check(m_ctl, m_slots, 0, s));
+      // the residual kernel's last workgroup applies the stopping rule
+      if ((rc = residual_norm(cur(0, phi), rhs, launched, skip, true))) return rc;
     }
     if ((rc = read_ctl(&c))) return rc;
     if (c.done || launched >= o.max_iter) break;
@@ -433,9 +438,9 @@ int HCEngine::solve_refined(double* phi, const double* rhs, const ml_solve_opts&
       const int it = accepted0 + launched + 1;
       const int pc = (it - 1) & 1;   // each earlier iteration accepted one step
       const float* e0 = m_ecur[0] ? m_ealt[0] : m_lin[0].e;
+      // the residual kernel's last workgroup applies the stopping rule
       ML_HIP(ctx, hc_residual_jacobian(A, pb[pc], pb[1 - pc], rhs, e0, 1.0, m_lin[0].r, m_lin[0].M,
-                                       m_slots + it, s));
-      ML_HIP(ctx, ctl_check(m_ctl, m_slots, 1, s));
+                                       m_slots + it, s, CtlFuse{m_ctl, m_slots, m_arrive, 1}));
       ++launched;
     }
     if ((rc = read_ctl(&c))) return rc;

@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kThreads) void hc_tile_kernel(HCArgs<T> A, TileIO<T
         }
       }
     }
-    if (MODE == kResid && io.norm) block_max_atomic<T>(mx, nan, io.norm);
+    if (MODE == kResid && io.norm) block_max_atomic<T>(mx, nan, io.norm, io.fuse);
     return;
   }
 
@@ -774,8 +774,10 @@ hipError_t hc_coarse_vcycle(const CoarseDesc<T>* dev_desc, int dim, int apply_fi
 
 template <typename T>
 hipError_t hc_residual(const HCArgs<T>& A, T* phi, const T* rhs, T* res, unsigned long long* norm,
-                       int homog, hipStream_t s) {
+                       int homog, hipStream_t s, const CtlFuse& fuse) {
+  if (fuse.ctl && !norm) return hipErrorInvalidValue;
   TileIO<T> io{};
+  io.fuse = fuse;
   io.phi = phi;
   io.rhs = rhs;
   io.out = res;
@@ -831,7 +833,7 @@ template hipError_t hc_relax_tiles<double>(const HCArgs<double>&, double* const[
                                            const Geom*, const double*, const double*, hipStream_t);
 template hipError_t hc_coarse_vcycle<double>(const CoarseDesc<double>*, int, int, hipStream_t);
 template hipError_t hc_residual<double>(const HCArgs<double>&, double*, const double*, double*,
-                                        unsigned long long*, int, hipStream_t);
+                                        unsigned long long*, int, hipStream_t, const CtlFuse&);
 template hipError_t hc_apply_op<double>(const HCArgs<double>&, double*, double*, const double*, int, hipStream_t);
 template hipError_t hc_restrict_residual<double>(const HCArgs<double>&, const Geom&, double*, const double*,
                                                  double*, int, hipStream_t);

@@ -91,7 +91,7 @@ class HCEngine {
  private:
   int init();
   int vcycle(double* phi, const double* rhs, const ml_solve_opts& o, const int* skip);
-  int residual_norm(double* phi, const double* rhs, int slot, const int* skip);
+  int residual_norm(double* phi, const double* rhs, int slot, const int* skip, bool check);
   int read_norm(int slot, double* out);
   int read_ctl(SolveCtl* out);
   int write_ctl(const SolveCtl& c);
@@ -130,6 +130,7 @@ class HCEngine {
   CoarseDesc<double>* m_desc = nullptr;        // device
   CoarseDesc<double>* m_hdesc = nullptr;       // pinned host staging
   SolveCtl* m_ctl = nullptr;                   // device iteration control
+  unsigned int* m_arrive = nullptr;            // workgroup arrival counter of the fused stopping rule
   SolveCtl* m_hctl = nullptr;                  // pinned mirror
   int m_batch_fas = 4;                         // iterations enqueued per host check
   int m_batch_refined = 4;
@@ -167,6 +168,7 @@ struct ml_vc_op_s {
   ml::VCDesc<double>* desc = nullptr;    // device
   ml::VCDesc<double>* hdesc = nullptr;   // pinned staging
   ml::SolveCtl* ctl = nullptr;           // device iteration control
+  unsigned int* arrive = nullptr;        // workgroup arrival counter of the fused stopping rule
   ml::SolveCtl* hctl = nullptr;          // pinned mirror
   int batch = 4;                         // iterations enqueued per host check
   size_t coarse_lds = 0;                 // dynamic LDS of the single-workgroup V-cycle

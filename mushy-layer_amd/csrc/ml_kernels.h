@@ -23,11 +23,69 @@ struct HCArgs {
   const int* skip = nullptr;   // device flag: the kernel returns at once when set
 };
 
+// Device-side iteration control of a multigrid solve.  The host launches
+// several iterations ahead; every kernel of an iteration returns at once when
+// `done` is set, so the iterations that run, and their results, are exactly
+// those of a host loop that checks the norm after each iteration.
+struct SolveCtl {
+  int done;        // read by every kernel of the loop (skip flag)
+  int iter;        // iterations accepted so far
+  int status;      // exit-status bits 1/2/4/8 (0 while running)
+  int need_host;   // refined form: a trial step was rejected (host backtracks)
+  int max_iter;
+  double eps, hang, thresh;
+  double r0, prev, norm;
+};
+
+// Stopping rule after an accepted iteration with residual max-norm `norm`.
+__device__ inline void ctl_finish_iteration(SolveCtl* c, double norm) {
+  int status = 0;
+  if (norm <= c->eps * c->r0) status |= 1;
+  if (norm <= c->thresh) status |= 8;
+  if (status == 0 && norm > (1.0 - c->hang) * c->prev) status |= 4;
+  c->prev = norm;
+  c->norm = norm;
+  if (status != 0 || c->iter >= c->max_iter) {
+    c->status = status != 0 ? status : 2;
+    c->done = 1;
+  }
+}
+
+// Iteration iter+1 wrote its norm into slots[iter+1]: apply the stopping
+// rule; with refined != 0 a norm that does not decrease is not accepted but
+// handed to the host (need_host, done).  One thread.
+__device__ inline void ctl_check_step(SolveCtl* c, const unsigned long long* slots, int refined) {
+  if (c->done) return;
+  const int it = c->iter + 1;
+  const unsigned long long bits =
+      __hip_atomic_load(const_cast<unsigned long long*>(slots + it), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const double norm = __longlong_as_double((long long)bits);
+  if (refined && !(norm < c->prev)) {
+    c->need_host = 1;
+    c->norm = norm;
+    c->done = 1;
+    return;
+  }
+  c->iter = it;
+  ctl_finish_iteration(c, norm);
+}
+
+// The stopping rule applied by the last workgroup of the kernel that
+// computes the residual norm (ctl == nullptr: none).  `arrive` counts the
+// workgroups that have added their norm; the last one resets it.
+struct CtlFuse {
+  SolveCtl* ctl;
+  const unsigned long long* slots;
+  unsigned int* arrive;
+  int refined;
+};
+
 // Max-reduction of a non-negative value over the workgroup, then one
 // atomicMax on its IEEE bit pattern (monotone for non-negative doubles; a
-// NaN anywhere wins).
+// NaN anywhere wins); with f.ctl set, the grid's last workgroup then applies
+// the stopping rule.
 template <typename T>
-__device__ inline void block_max_atomic(T v, int nan, unsigned long long* out) {
+__device__ inline void block_max_atomic(T v, int nan, unsigned long long* out, CtlFuse f = CtlFuse{}) {
   double d = nan ? __longlong_as_double(0x7ff8000000000000LL) : double(v);
   for (int off = 32; off > 0; off >>= 1) {
     double o = __shfl_xor(d, off, 64);
@@ -45,6 +103,15 @@ __device__ inline void block_max_atomic(T v, int nan, unsigned long long* out) {
       m = (o != o || m != m) ? __longlong_as_double(0x7ff8000000000000LL) : (o > m ? o : m);
     }
     atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    if (f.ctl) {
+      __threadfence();
+      const unsigned int total = gridDim.x * gridDim.y * gridDim.z;
+      if (atomicAdd(f.arrive, 1u) == total - 1) {
+        __threadfence();
+        atomicExch(f.arrive, 0u);
+        ctl_check_step(f.ctl, f.slots, f.refined);
+      }
+    }
   }
 }
 
@@ -57,6 +124,7 @@ struct TileIO {
   T* out;
   const T* add;
   unsigned long long* norm;
+  CtlFuse fuse;   // residual: stopping rule applied by the last workgroup
   Geom gc;        // coarse level (restriction outputs / prolongation inputs)
   T* phic;        // restricted state (optional)
   T* phic0;       // copy of the restricted state (FAS initial coarse state)
@@ -228,7 +296,7 @@ struct LinDesc {
 // *norm
 hipError_t hc_residual_jacobian(const HCArgs<double>& A, const double* phi, double* phi_out, const double* rhs,
                                 const float* e, double omega, float* r32, float* M32, unsigned long long* norm,
-                                hipStream_t s);
+                                hipStream_t s, const CtlFuse& fuse = CtlFuse{});
 // red/black block relaxation of J e = r on a tiled level, ping-ponging
 // between ebuf[0] and ebuf[1] (*cur indexes the current correction; L.e is
 // ignored); zero_init starts from e = 0; pc (coarse correction, level gc) is
@@ -279,7 +347,7 @@ hipError_t hc_restrict_all(const HCArgs<T>& A, const Geom& gc, T* phi, const T* 
                            int homog, hipStream_t s);
 template <typename T>
 hipError_t hc_residual(const HCArgs<T>& A, T* phi, const T* rhs, T* res, unsigned long long* norm,
-                       int homog, hipStream_t s);
+                       int homog, hipStream_t s, const CtlFuse& fuse = CtlFuse{});
 template <typename T>
 hipError_t hc_apply_op(const HCArgs<T>& A, T* phi, T* lhs, const T* add, int homog, hipStream_t s);
 template <typename T>
@@ -299,19 +367,6 @@ template <typename T>
 hipError_t norm_max(const Geom& g, int ncomp, const T* x, unsigned long long* out, hipStream_t s);
 hipError_t zero_u64(unsigned long long* p, int n, hipStream_t s);
 
-// Device-side iteration control of a multigrid solve.  The host launches
-// several iterations ahead; every kernel of an iteration returns at once when
-// `done` is set, so the iterations that run, and their results, are exactly
-// those of a host loop that checks the norm after each iteration.
-struct SolveCtl {
-  int done;        // read by every kernel of the loop (skip flag)
-  int iter;        // iterations accepted so far
-  int status;      // exit-status bits 1/2/4/8 (0 while running)
-  int need_host;   // refined form: a trial step was rejected (host backtracks)
-  int max_iter;
-  double eps, hang, thresh;
-  double r0, prev, norm;
-};
 // ctl from the initial residual norm in slots[0]
 hipError_t ctl_init(SolveCtl* ctl, const unsigned long long* slots, int max_iter, double eps, double hang,
                     double thresh, hipStream_t s);
@@ -371,6 +426,7 @@ struct VCTileIO {
   const T* rhs;
   T* out;
   unsigned long long* norm;
+  CtlFuse fuse;        // residual: stopping rule applied by the last workgroup
   Geom gc;
   int homog, ntx, nty;
 };
@@ -410,7 +466,7 @@ hipError_t vc_relax_restrict(const VCArgs<T>& A, T* const buf[2], int* cur, cons
 // with e the updated state phi + e is written to phi_out (!= phi)
 template <typename T>
 hipError_t vc_residual(const VCArgs<T>& A, const T* phi, const T* e, T* phi_out, const T* rhs, T* res,
-                       unsigned long long* norm, int homog, hipStream_t s);
+                       unsigned long long* norm, int homog, hipStream_t s, const CtlFuse& fuse = CtlFuse{});
 template <typename T>
 hipError_t vc_apply_op(const VCArgs<T>& A, const T* phi, T* lhs, int homog, hipStream_t s);
 template <typename T>

@@ -105,18 +105,6 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
 
 // stopping rule of the multigrid solves (status bits: 1 |r| <= eps*r0,
 // 2 iteration maximum, 4 hang, 8 |r| <= thresh), evaluated by one thread
-__device__ inline void ctl_finish_iteration(SolveCtl* c, double norm) {
-  int status = 0;
-  if (norm <= c->eps * c->r0) status |= 1;
-  if (norm <= c->thresh) status |= 8;
-  if (status == 0 && norm > (1.0 - c->hang) * c->prev) status |= 4;
-  c->prev = norm;
-  c->norm = norm;
-  if (status != 0 || c->iter >= c->max_iter) {
-    c->status = status != 0 ? status : 2;
-    c->done = 1;
-  }
-}
 
 __global__ void ctl_init_kernel(SolveCtl* c, const unsigned long long* slots, int max_iter, double eps, double hang,
                                 double thresh) {
@@ -144,17 +132,7 @@ __global__ void ctl_init_kernel(SolveCtl* c, const unsigned long long* slots, in
 
 __global__ void ctl_check_kernel(SolveCtl* c, const unsigned long long* slots, int refined) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (c->done) return;
-  const int it = c->iter + 1;
-  const double norm = __longlong_as_double((long long)slots[it]);
-  if (refined && !(norm < c->prev)) {
-    c->need_host = 1;
-    c->norm = norm;
-    c->done = 1;
-    return;
-  }
-  c->iter = it;
-  ctl_finish_iteration(c, norm);
+  ctl_check_step(c, slots, refined);
 }
 
 // MG coefficient hierarchy (CoarseAverage / CoarseAverageFace from the

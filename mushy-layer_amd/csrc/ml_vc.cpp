@@ -133,9 +133,9 @@ int vc_solve(ml_vc_op op, double* phi, const double* rhs, const ml_solve_opts& o
       if ((rc = vc_vcycle(op, o, ecur.data(), skip))) return rc;
       ++launched;
       const int pcur = (launched - 1) & 1;   // phi += e alternates the state buffer
+      // the residual kernel's last workgroup applies the stopping rule
       ML_HIP(ctx, ml::vc_residual<double>(A0, pb[pcur], op->lev[0].e[ecur[0]], pb[1 - pcur], rhs, op->lev[0].r,
-                                           op->slots + launched, 0, s));
-      ML_HIP(ctx, ml::ctl_check(op->ctl, op->slots, 0, s));
+                                           op->slots + launched, 0, s, ml::CtlFuse{op->ctl, op->slots, op->arrive, 0}));
     }
     if ((rc = vc_read_ctl(op, &c))) return rc;
     if (c.done || launched >= o.max_iter) break;
@@ -233,6 +233,9 @@ int ml_vc_op_create(ml_ctx ctx, int dim, const int lo[], const int hi[], double 
   raw->hdesc = static_cast<ml::VCDesc<double>*>(p);
   if (hipMalloc(&p, sizeof(ml::SolveCtl)) != hipSuccess) return bail(ML_ENOMEM);
   raw->ctl = static_cast<ml::SolveCtl*>(p);
+  if (hipMalloc(&p, sizeof(unsigned int)) != hipSuccess) return bail(ML_ENOMEM);
+  raw->arrive = static_cast<unsigned int*>(p);
+  if (hipMemsetAsync(raw->arrive, 0, sizeof(unsigned int), ctx->stream) != hipSuccess) return bail(ML_EHIP);
   if (hipHostMalloc(&p, sizeof(ml::SolveCtl), hipHostMallocDefault) != hipSuccess) return bail(ML_ENOMEM);
   raw->hctl = static_cast<ml::SolveCtl*>(p);
   *out = op.release();
@@ -259,6 +262,7 @@ int ml_vc_op_destroy(ml_vc_op op) {
   if (op->desc) hipFree(op->desc);
   if (op->hdesc) hipHostFree(op->hdesc);
   if (op->ctl) hipFree(op->ctl);
+  if (op->arrive) hipFree(op->arrive);
   if (op->hctl) hipHostFree(op->hctl);
   delete op;
   return ML_OK;

@@ -281,7 +281,7 @@ __global__ __launch_bounds__(kVCThreads) void vc_tile_kernel(VCArgs<T> A, VCTile
         io.out[r.q] = lof;
       }
     }
-    if (MODE == kVCResid && io.norm) block_max_atomic<T>(mx, nan, io.norm);
+    if (MODE == kVCResid && io.norm) block_max_atomic<T>(mx, nan, io.norm, io.fuse);
     return;
   }
   // kVCRestrict (RESTRICTRESDB, src/DarcyBrinkmanOpF.ChF:402-509)
@@ -481,6 +481,45 @@ __device__ __forceinline__ void vcx_prolong(const VCCtx<T, DIM>& X, const CLev& 
   __syncthreads();
 }
 
+// one cell's operator data (and right-hand side) read from global memory
+template <typename T, int DIM>
+struct VCPre {
+  Faces1<T, DIM> F;
+  T aq, cv, r;
+};
+
+template <typename T, int DIM>
+__device__ __forceinline__ void vcpre_load(VCPre<T, DIM>& P, const VCLevel<T>& lv, const CLev& L, int c, bool rhs) {
+  if (c >= L.N) return;
+  int i, j, k;
+  clev_coords(L, c, i, j, k);
+  VCArgs<T> A;
+  A.g = lv.g;
+  for (int d = 0; d < 3; ++d) A.b[d] = lv.b[d];
+  load_faces1<T, DIM>(A, i, j, k, P.F);
+  const long long g = lv.g.ci(i, j, k);
+  P.aq = lv.a ? lv.a[g] : T(1);
+  P.cv = lv.c ? lv.c[g] : T(0);
+  if (rhs) P.r = lv.r[g];
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ void vcpre_store(const VCPre<T, DIM>& P, const VCLds<T, DIM>& S, const CLev& L, int c,
+                                            bool hasc, T alpha, T beta, bool rhs) {
+  if (c >= L.N) return;
+  const int q = L.off + c;
+#pragma unroll
+  for (int f = 0; f < 2 * DIM; ++f) S.sF[f * S.tot + q] = P.F.v[f];
+  S.sA[q] = P.aq;
+  S.sC[q] = P.cv;
+  const T dxinv = T(1) / (T(L.dx) * T(L.dx));
+  S.sL[q] = vc_lambda<T, DIM>(P.F, P.aq, hasc, P.cv, alpha, beta, dxinv);
+  if (rhs) {
+    S.se[q] = T(0);
+    S.sr[q] = P.r;
+  }
+}
+
 template <typename T, int DIM>
 __global__ __launch_bounds__(kVCWG) void vc_coarse_vcycle_kernel(const VCDesc<T>* __restrict__ Dp) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -512,37 +551,20 @@ __global__ __launch_bounds__(kVCWG) void vc_coarse_vcycle_kernel(const VCDesc<T>
   S.sC = S.sA + tot;
   S.sL = S.sC + tot;
   S.sres = S.sL + tot;
-  // operator data of every level, and level l0's right-hand side
+  // operator data of every level, and level l0's right-hand side: all
+  // global loads first (level l0: cells t and t + 512, coarser levels: cell
+  // t), then the LDS stores
+  VCPre<T, DIM> P0[2], P1[LL - 1];
 #pragma unroll
-  for (int l = 0; l < LL; ++l) {
-    if (l >= nl) continue;
-    const VCLevel<T>& lv = D.lev[l0 + l];
-    VCArgs<T> A;
-    A.g = lv.g;
-    A.a = lv.a;
-    A.c = lv.c;
-    for (int d = 0; d < 3; ++d) A.b[d] = lv.b[d];
-    const T dxinv = T(1) / (T(G[l].dx) * T(G[l].dx));
-    for (int c = threadIdx.x; c < G[l].N; c += kVCWG) {
-      int i, j, k;
-      clev_coords(G[l], c, i, j, k);
-      const int q = G[l].off + c;
-      Faces1<T, DIM> F;
-      load_faces1<T, DIM>(A, i, j, k, F);
+  for (int m = 0; m < 2; ++m) vcpre_load<T, DIM>(P0[m], D.lev[l0], G[0], threadIdx.x + m * kVCWG, true);
 #pragma unroll
-      for (int f = 0; f < 2 * DIM; ++f) S.sF[f * tot + q] = F.v[f];
-      const long long g = lv.g.ci(i, j, k);
-      const T aq = A.a ? A.a[g] : T(1);
-      const T cv = A.c ? A.c[g] : T(0);
-      S.sA[q] = aq;
-      S.sC[q] = cv;
-      S.sL[q] = vc_lambda<T, DIM>(F, aq, hasc, cv, alpha, beta, dxinv);
-      if (l == 0) {
-        S.se[q] = T(0);
-        S.sr[q] = lv.r[g];
-      }
-    }
-  }
+  for (int l = 1; l < LL; ++l)
+    if (l < nl) vcpre_load<T, DIM>(P1[l - 1], D.lev[l0 + l], G[l], threadIdx.x, false);
+#pragma unroll
+  for (int m = 0; m < 2; ++m) vcpre_store<T, DIM>(P0[m], S, G[0], threadIdx.x + m * kVCWG, hasc, alpha, beta, true);
+#pragma unroll
+  for (int l = 1; l < LL; ++l)
+    if (l < nl) vcpre_store<T, DIM>(P1[l - 1], S, G[l], threadIdx.x, hasc, alpha, beta, false);
   __syncthreads();
   // down: pre-smoothing and restriction; the coarsest level gets the bottom
   // smoothing
@@ -662,7 +684,8 @@ hipError_t vc_relax_restrict(const VCArgs<T>& A, T* const buf[2], int* cur, cons
 
 template <typename T>
 hipError_t vc_residual(const VCArgs<T>& A, const T* phi, const T* e, T* phi_out, const T* rhs, T* res,
-                       unsigned long long* norm, int homog, hipStream_t s) {
+                       unsigned long long* norm, int homog, hipStream_t s, const CtlFuse& fuse) {
+  if (fuse.ctl && !norm) return hipErrorInvalidValue;
   if (e && (!phi_out || phi_out == phi)) return hipErrorInvalidValue;
   VCTileIO<T> io{};
   io.phi = phi;
@@ -671,6 +694,7 @@ hipError_t vc_residual(const VCArgs<T>& A, const T* phi, const T* e, T* phi_out,
   io.rhs = rhs;
   io.out = res;
   io.norm = norm;
+  io.fuse = fuse;
   io.homog = homog;
   return A.g.dim == 2 ? vc_tile_dim<T, 2, kVCResid>(A, io, s) : vc_tile_dim<T, 3, kVCResid>(A, io, s);
 }
@@ -720,7 +744,8 @@ template hipError_t vc_relax<double>(const VCArgs<double>&, double* const[2], in
 template hipError_t vc_relax_restrict<double>(const VCArgs<double>&, double* const[2], int*, const double*, int, bool,
                                               const Geom&, double*, hipStream_t);
 template hipError_t vc_residual<double>(const VCArgs<double>&, const double*, const double*, double*,
-                                        const double*, double*, unsigned long long*, int, hipStream_t);
+                                        const double*, double*, unsigned long long*, int, hipStream_t,
+                                        const CtlFuse&);
 template hipError_t vc_apply_op<double>(const VCArgs<double>&, const double*, double*, int, hipStream_t);
 template hipError_t vc_restrict_residual<double>(const VCArgs<double>&, const Geom&, const double*, const double*,
                                                  double*, hipStream_t);
