@@ -14,9 +14,14 @@ correction).  `value` is the FP32-refined form's rate; the FP64 rate, the
 ratio and the per-field difference of the two final states are reported
 beside it.
 
-Also reported: the roofline of the dominant kernel (fused closure + red/black
-smoother on the finest level, timed with HIP events on the library's stream)
-and a CPU baseline (the C oracle on this host, 1 thread, bounded sample).
+Also reported: the roofline of the dominant kernel of the FP32-refined step,
+the tiled red/black relaxation of the Darcy-Brinkman / projection operator
+(vc_relax_tile_kernel, all its launches in the step): every launch of it in a
+second timed pass of the same steps is bracketed by HIP events on the
+library's stream, and achieved = summed algorithmic bytes / summed launch
+time; `traffic` (HBM bytes per launch) comes from the rocprofv3 --pmc
+FETCH_SIZE / WRITE_SIZE runs recorded in profiles/ (null when absent).  And a
+CPU baseline (the C oracle on this host, 1 thread, bounded sample).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -47,6 +52,8 @@ def parse():
     ap.add_argument("--max-iter", type=int, default=100)
     ap.add_argument("--form", choices=["both", "fp64", "fp32r"], default="both")
     ap.add_argument("--hc-only", action="store_true", help="time the (H, Theta) solve alone")
+    ap.add_argument("--no-lookahead", action="store_true",
+                    help="solves enqueue one iteration at a time (no launches cut short; for counter runs)")
     return ap.parse_args()
 
 
@@ -71,6 +78,9 @@ class HCStepper:
     def reset(self):
         self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, self.phi.ptr, self.init.ptr, self.phi.nbytes), "copy")
         self.stats = []
+
+    def set_flags(self, flags):
+        self.opts.flags = flags
 
     def step(self):
         op = self.op
@@ -127,6 +137,11 @@ class CoupledStepper(HCStepper):
             self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, u.ptr, u0.ptr, u.nbytes), "copy")
         self.vstats = []
 
+    def set_flags(self, flags):
+        super().set_flags(flags)
+        self.uopts.flags = flags
+        self.popts.flags = flags
+
     def step(self):
         super().step()
         self.op.porosity(self.phi, self.chi)
@@ -169,6 +184,18 @@ def kernel_roofline(ml, ctx, stepper, reps=200):
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
             "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": bytes_per_cell * cells}
+
+
+def calibration_copy(ml, ctx, mib=64):
+    """One device-to-device copy of a known size: the reference dispatch that
+    the rocprofv3 FETCH_SIZE / WRITE_SIZE counters are scaled by."""
+    nbytes = mib << 20
+    a = ml.DeviceArray(ctx, (nbytes // 8,))
+    b = ml.DeviceArray(ctx, (nbytes // 8,))
+    ctx.check(ctx.lib.ml_buf_copy(ctx.h, b.ptr, a.ptr, nbytes), "copy")
+    ctx.sync()
+    a.free()
+    b.free()
 
 
 def field_errors(ml, ctx, case, a, b):
@@ -219,20 +246,53 @@ def cpu_baseline(case, seconds, max_iter):
                       "(gcc -O2 -ffp-contract=off), 1 thread, %.1f s" % (steps, case["n"][0], case["n"][1], dt)}
 
 
-def run_form(ml, ctx, case, precision, steps, warmup, max_iter, db=None):
+def run_form(ml, ctx, case, precision, steps, warmup, max_iter, db=None, no_lookahead=False):
     opts = ml.default_opts(max_iter=max_iter, hang=-1.0, precision=precision)
     st = HCStepper(ml, ctx, case, opts) if db is None else CoupledStepper(ml, ctx, case, opts, db)
+    if no_lookahead:
+        st.set_flags(ml.SOLVE_NO_LOOKAHEAD)
     for _ in range(warmup):
         st.step()
     ctx.sync()
+    return st, timed_steps(ctx, st, steps)
+
+
+def timed_steps(ctx, st, steps):
     st.reset()
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
         st.step()
     ctx.sync()
-    wall = time.perf_counter() - t0
-    return st, wall
+    return time.perf_counter() - t0
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vc_relax_tile_256_traffic.json")
+
+
+def vc_relax_roofline(ml, ctx, stepper, steps):
+    """Second timed pass of the same steps with every vc_relax_tile_kernel
+    launch carrying HIP start/stop events (library stream): average launch
+    time and algorithmic bytes per launch.  The solves enqueue one iteration
+    at a time in this pass, so every timed launch does its work."""
+    stepper.set_flags(ml.SOLVE_NO_LOOKAHEAD)
+    ctx.timing_start(ml.KT_VC_RELAX, 400 * steps)
+    wall = timed_steps(ctx, stepper, steps)
+    n, ms, nbytes, missed = ctx.timing_stop()
+    stepper.set_flags(0)
+    avg_s = ms * 1e-3 / n
+    per_launch = nbytes / n
+    achieved = per_launch / avg_s / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    return {"kernel": "vc_relax_tile_kernel<double,2,32,16,1,8,512> (every launch of the step)", "bound": "hbm",
+            "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": round(per_launch, 1),
+            "launches_timed": n, "launches_untimed": missed,
+            "ms_per_step_instrumented": round(wall / steps * 1e3, 4)}
 
 
 def main():
@@ -252,14 +312,18 @@ def main():
     todo = ["fp64", "fp32r"] if args.form == "both" else [args.form]
     res = {}
     for f in todo:
-        st, wall = run_form(ml, ctx, case, forms[f], args.steps, args.warmup, args.max_iter, db)
+        st, wall = run_form(ml, ctx, case, forms[f], args.steps, args.warmup, args.max_iter, db, args.no_lookahead)
         res[f] = dict(stepper=st, wall=wall, value=cells * args.steps / wall,
                       iters=[s[1] for s in st.stats], status=sorted(set(s[0] for s in st.stats)),
                       vstats=getattr(st, "vstats", None))
     head = "fp32r" if "fp32r" in res else "fp64"
     h = res[head]
 
-    roof = kernel_roofline(ml, ctx, res[todo[0]]["stepper"])
+    if args.hc_only:
+        roof = kernel_roofline(ml, ctx, res[todo[0]]["stepper"])
+    else:
+        roof = vc_relax_roofline(ml, ctx, h["stepper"], args.steps)
+    calibration_copy(ml, ctx)
     cpu = None if args.no_cpu else cpu_baseline(case, args.cpu_seconds, args.max_iter)
 
     what = ("HC backward-Euler solve (porosity + coefficient assembly + solve)" if args.hc_only else

@@ -85,7 +85,11 @@ typedef struct {
  * nu1 = 1, nu2 = 4, bottom = 40, max_iter = 10, eps = hang = norm_thresh =
  * 1e-10, precision FP64).  flags: ML_SOLVE_HOST_DRIVER runs the solve through
  * the per-operator entry points with the host C++ driver (host/FASMultiGrid.H)
- * instead of the fused engine; both give identical FP64 results. */
+ * instead of the fused engine; both give identical FP64 results.
+ * ML_SOLVE_NO_LOOKAHEAD enqueues one iteration at a time (the host reads the
+ * stopping rule after each) instead of batches that the device cuts short:
+ * same results, no launches that return at once (for per-launch
+ * measurements). */
 typedef struct {
   int nu1, nu2, bottom, max_iter;
   double eps, hang, norm_thresh;
@@ -93,7 +97,7 @@ typedef struct {
   int flags;
 } ml_solve_opts;
 
-enum { ML_SOLVE_HOST_DRIVER = 1 };
+enum { ML_SOLVE_HOST_DRIVER = 1, ML_SOLVE_NO_LOOKAHEAD = 2 };
 
 typedef struct {
   int exit_status;   /* 1 |r|<=eps|r0|, 2 iteration max, 4 hang, 8 |r|<=norm_thresh */
@@ -114,6 +118,16 @@ void* ml_ctx_stream(ml_ctx ctx);
  * slots (blocks until slot_b has completed). */
 int ml_ctx_event_record(ml_ctx ctx, int slot);
 int ml_ctx_event_elapsed(ml_ctx ctx, int slot_a, int slot_b, float* ms);
+/* Launch timing of one kernel family (measurement; no counterpart in the
+ * reference).  While started, every launch of `kernel` on the context is
+ * bracketed by HIP events on the context stream (the first max_launches
+ * launches) and its algorithmic bytes (every input read once, every output
+ * written once) are summed.  ml_ctx_timing_stop synchronises the stream,
+ * returns the timed launches, their summed duration in ms and bytes, and the
+ * launches beyond max_launches, and ends the timing. */
+enum { ML_KT_VC_RELAX = 1 /* tiled red/black relaxation of the linear VC operator */ };
+int ml_ctx_timing_start(ml_ctx ctx, int kernel, int max_launches);
+int ml_ctx_timing_stop(ml_ctx ctx, long long* launches, double* total_ms, double* bytes, long long* missed);
 int ml_buf_alloc(ml_ctx ctx, size_t bytes, void** dptr);
 int ml_buf_free(ml_ctx ctx, void* dptr);
 int ml_buf_upload(ml_ctx ctx, void* dst_dev, const void* src_host, size_t bytes);

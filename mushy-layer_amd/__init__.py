@@ -41,7 +41,9 @@ class SolveOpts(C.Structure):
 
 
 SOLVE_HOST_DRIVER = 1
+SOLVE_NO_LOOKAHEAD = 2   # one iteration enqueued at a time (per-launch measurements)
 PRECISION_FP64, PRECISION_FP32_REFINED = 0, 1
+KT_VC_RELAX = 1   # ml_ctx_timing_start kernel family: tiled relaxation of the linear VC operator
 
 
 class SolveStats(C.Structure):
@@ -62,6 +64,9 @@ SIGNATURES = {
     "ml_ctx_stream": (_vp, [_vp]),
     "ml_ctx_event_record": (C.c_int, [_vp, C.c_int]),
     "ml_ctx_event_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    "ml_ctx_timing_start": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "ml_ctx_timing_stop": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_longlong)]),
     "ml_buf_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
     "ml_buf_free": (C.c_int, [_vp, _vp]),
     "ml_buf_upload": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
@@ -218,6 +223,18 @@ class Context:
         ms = C.c_float()
         self.check(self.lib.ml_ctx_event_elapsed(self.h, a, b, C.byref(ms)), "ml_ctx_event_elapsed")
         return ms.value
+
+    def timing_start(self, kernel, max_launches):
+        """Bracket the launches of a kernel family (KT_*) with HIP events."""
+        self.check(self.lib.ml_ctx_timing_start(self.h, kernel, max_launches), "ml_ctx_timing_start")
+
+    def timing_stop(self):
+        """(timed launches, summed ms, summed algorithmic bytes, untimed
+        launches) since timing_start."""
+        n, ms, b, missed = C.c_longlong(), C.c_double(), C.c_double(), C.c_longlong()
+        self.check(self.lib.ml_ctx_timing_stop(self.h, C.byref(n), C.byref(ms), C.byref(b), C.byref(missed)),
+                   "ml_ctx_timing_stop")
+        return n.value, ms.value, b.value, missed.value
 
     def close(self):
         if self.h:

@@ -90,6 +90,8 @@ int ml_ctx_destroy(ml_ctx ctx) {
   if (ctx->h_norm) hipHostFree(ctx->h_norm);
   for (auto& e : ctx->events)
     if (e) hipEventDestroy(e);
+  for (auto& e : ctx->timing_start) hipEventDestroy(e);
+  for (auto& e : ctx->timing_stop) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return ML_OK;
@@ -117,6 +119,42 @@ int ml_ctx_event_elapsed(ml_ctx ctx, int a, int b, float* ms) {
     return ML_EINVAL;
   ML_HIP(ctx, hipEventSynchronize(ctx->events[b]));
   ML_HIP(ctx, hipEventElapsedTime(ms, ctx->events[a], ctx->events[b]));
+  return ML_OK;
+}
+
+int ml_ctx_timing_start(ml_ctx ctx, int kernel, int max_launches) {
+  if (!ctx || kernel != ML_KT_VC_RELAX || max_launches <= 0) return ML_EINVAL;
+  while (int(ctx->timing_start.size()) < max_launches) {
+    hipEvent_t a, b;
+    ML_HIP(ctx, hipEventCreate(&a));
+    ML_HIP(ctx, hipEventCreate(&b));
+    ctx->timing_start.push_back(a);
+    ctx->timing_stop.push_back(b);
+  }
+  ctx->timing_start.resize(max_launches);
+  ctx->timing_stop.resize(max_launches);
+  ctx->timing_kernel = kernel;
+  ctx->timing_n = 0;
+  ctx->timing_missed = 0;
+  ctx->timing_bytes = 0;
+  return ML_OK;
+}
+
+int ml_ctx_timing_stop(ml_ctx ctx, long long* launches, double* total_ms, double* bytes, long long* missed) {
+  if (!ctx || !launches || !total_ms || !bytes || !missed) return ML_EINVAL;
+  ML_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double ms = 0;
+  for (int i = 0; i < ctx->timing_n; ++i) {
+    float t = 0;
+    ML_HIP(ctx, hipEventElapsedTime(&t, ctx->timing_start[i], ctx->timing_stop[i]));
+    ms += t;
+  }
+  *launches = ctx->timing_n;
+  *total_ms = ms;
+  *bytes = ctx->timing_bytes;
+  *missed = ctx->timing_missed;
+  ctx->timing_kernel = 0;
+  ctx->timing_n = 0;
   return ML_OK;
 }
 

@@ -237,7 +237,8 @@ int HCEngine::solve(double* phi, const double* rhs, const ml_solve_opts& o, ml_s
   const int* skip = &m_ctl->done;
   int flip0 = -1;   // level-0 buffer flips per V-cycle (fixed by the launch sequence)
   int launched = 0;
-  int batch = std::max(1, std::min(m_batch_fas, o.max_iter));
+  const bool lookahead = !(o.flags & ML_SOLVE_NO_LOOKAHEAD);
+  int batch = lookahead ? std::max(1, std::min(m_batch_fas, o.max_iter)) : 1;
   SolveCtl c{};
   for (;;) {
     for (int b = 0; b < batch && launched < o.max_iter; ++b) {
@@ -250,9 +251,9 @@ int HCEngine::solve(double* phi, const double* rhs, const ml_solve_opts& o, ml_s
     }
     if ((rc = read_ctl(&c))) return rc;
     if (c.done || launched >= o.max_iter) break;
-    batch = 2;
+    batch = lookahead ? 2 : 1;
   }
-  m_batch_fas = std::max(1, c.iter);
+  if (lookahead) m_batch_fas = std::max(1, c.iter);
   // the state lies in the buffer reached after c.iter V-cycles
   const int cur0 = flip0 > 0 ? (c.iter & 1) : 0;
   if (cur0) ML_HIP(ctx, hipMemcpyAsync(phi, m_alt[0], bytes0, hipMemcpyDeviceToDevice, s));
@@ -424,7 +425,8 @@ int HCEngine::solve_refined(double* phi, const double* rhs, const ml_solve_opts&
   int executed = 0;         // linear V-cycles that ran
   int rc;
   SolveCtl c{};
-  int batch = std::max(1, std::min(m_batch_refined, o.max_iter));
+  const bool lookahead = !(o.flags & ML_SOLVE_NO_LOOKAHEAD);
+  int batch = lookahead ? std::max(1, std::min(m_batch_refined, o.max_iter)) : 1;
   for (;;) {
     const int accepted0 = c.iter;   // accepted iterations before this batch
     int launched = 0;
@@ -480,9 +482,9 @@ int HCEngine::solve_refined(double* phi, const double* rhs, const ml_solve_opts&
       if ((rc = write_ctl(c))) return rc;
     }
     if (c.done || c.iter >= o.max_iter) break;
-    batch = 2;
+    batch = lookahead ? 2 : 1;
   }
-  m_batch_refined = std::max(1, c.iter);
+  if (lookahead) m_batch_refined = std::max(1, c.iter);
   if (c.iter & 1) ML_HIP(ctx, hipMemcpyAsync(phi, m_alt[0], bytes0, hipMemcpyDeviceToDevice, s));
   std::fill(m_ecur.begin(), m_ecur.end(), 0);
   if (st) {

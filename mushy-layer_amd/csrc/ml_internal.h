@@ -20,6 +20,17 @@ struct ml_ctx_s {
   unsigned long long* d_norm = nullptr;   // device scratch for max-norms
   unsigned long long* h_norm = nullptr;   // pinned host mirror
   hipEvent_t events[16] = {};
+  // launch timing of one kernel family (ml_ctx_timing_start/stop)
+  int timing_kernel = 0;
+  int timing_n = 0;
+  long long timing_missed = 0;
+  double timing_bytes = 0;
+  std::vector<hipEvent_t> timing_start, timing_stop;
+  ml::LaunchTimer timer(int kernel) {
+    if (timing_kernel != kernel || timing_start.empty()) return ml::LaunchTimer{};
+    return ml::LaunchTimer{timing_start.data(), timing_stop.data(), int(timing_start.size()), &timing_n,
+                           &timing_bytes, &timing_missed};
+  }
 };
 
 namespace ml {

@@ -115,6 +115,19 @@ __device__ inline void block_max_atomic(T v, int nan, unsigned long long* out, C
   }
 }
 
+// HIP events around the launches of one kernel family (measurement):
+// launch n is bracketed by start[n], stop[n] while n < cap, and its
+// algorithmic bytes (every input read once, every output written once) are
+// added to *bytes.
+struct LaunchTimer {
+  hipEvent_t* start;
+  hipEvent_t* stop;
+  int cap;
+  int* n;
+  double* bytes;
+  long long* missed;   // launches beyond cap (not timed)
+};
+
 // Inputs/outputs of the tiled kernels.
 template <typename T>
 struct TileIO {
@@ -455,13 +468,13 @@ struct VCDesc {
 // e = 0; pc (level gc) is added by piecewise-constant prolongation first
 template <typename T>
 hipError_t vc_relax(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
-                    const Geom* gc, const T* pc, hipStream_t s);
+                    const Geom* gc, const T* pc, hipStream_t s, const LaunchTimer* tm = nullptr);
 // relaxation followed by the restricted residual R(rhs - L e) (homogeneous
 // BCs) of the relaxed state into resc on level gr, fused in one launch when
 // the passes fit the tile halo
 template <typename T>
 hipError_t vc_relax_restrict(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
-                             const Geom& gr, T* resc, hipStream_t s);
+                             const Geom& gr, T* resc, hipStream_t s, const LaunchTimer* tm = nullptr);
 // res = rhs - L(phi [+ e]) (res nullable), max-norm into *norm (nullable);
 // with e the updated state phi + e is written to phi_out (!= phi)
 template <typename T>

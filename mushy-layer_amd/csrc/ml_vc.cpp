@@ -73,25 +73,27 @@ int vc_vcycle(ml_vc_op op, const ml_solve_opts& o, int* ecur, const int* skip) {
     return A;
   };
   const int ltiles = lc < L ? lc : L - 1;
+  const ml::LaunchTimer T = ctx->timer(ML_KT_VC_RELAX);
+  const ml::LaunchTimer* tm = T.start ? &T : nullptr;
   if (lc >= L && L == 1) {
     ML_HIP(ctx, ml::vc_relax<double>(args(0), op->lev[0].e, &ecur[0], op->lev[0].r, 2 * o.bottom, true, nullptr,
-                                      nullptr, s));
+                                      nullptr, s, tm));
     return ML_OK;
   }
   for (int l = 0; l < ltiles; ++l) {
     ML_HIP(ctx, ml::vc_relax_restrict<double>(args(l), op->lev[l].e, &ecur[l], op->lev[l].r, 2 * o.nu1, true,
-                                               op->lev[l + 1].g, op->lev[l + 1].r, s));
+                                               op->lev[l + 1].g, op->lev[l + 1].r, s, tm));
   }
   if (lc < L) {
     ML_HIP(ctx, ml::vc_coarse_vcycle<double>(op->desc, op->dim, op->coarse_lds, s));
   } else {
     ML_HIP(ctx, ml::vc_relax<double>(args(L - 1), op->lev[L - 1].e, &ecur[L - 1], op->lev[L - 1].r, 2 * o.bottom,
-                                      true, nullptr, nullptr, s));
+                                      true, nullptr, nullptr, s, tm));
   }
   for (int l = ltiles - 1; l >= 0; --l) {
     const ml::Geom gc = op->lev[l + 1].g;
     ML_HIP(ctx, ml::vc_relax<double>(args(l), op->lev[l].e, &ecur[l], op->lev[l].r, 2 * o.nu2, false, &gc,
-                                      E(l + 1), s));
+                                      E(l + 1), s, tm));
   }
   return ML_OK;
 }
@@ -126,7 +128,8 @@ int vc_solve(ml_vc_op op, double* phi, const double* rhs, const ml_solve_opts& o
   A0.skip = skip;
   // iterations are enqueued in batches; the device stops them (SolveCtl)
   int launched = 0;
-  int batch = std::max(1, std::min(op->batch, o.max_iter));
+  const bool lookahead = !(o.flags & ML_SOLVE_NO_LOOKAHEAD);
+  int batch = lookahead ? std::max(1, std::min(op->batch, o.max_iter)) : 1;
   ml::SolveCtl c{};
   for (;;) {
     for (int b = 0; b < batch && launched < o.max_iter; ++b) {
@@ -139,9 +142,9 @@ int vc_solve(ml_vc_op op, double* phi, const double* rhs, const ml_solve_opts& o
     }
     if ((rc = vc_read_ctl(op, &c))) return rc;
     if (c.done || launched >= o.max_iter) break;
-    batch = 2;
+    batch = lookahead ? 2 : 1;
   }
-  op->batch = std::max(1, c.iter);
+  if (lookahead) op->batch = std::max(1, c.iter);
   if (c.iter & 1) ML_HIP(ctx, hipMemcpyAsync(phi, op->alt0, bytes0, hipMemcpyDeviceToDevice, s));
   if (st) {
     st->exit_status = c.status != 0 ? c.status : 2;

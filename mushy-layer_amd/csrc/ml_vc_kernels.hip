@@ -19,6 +19,8 @@
 //                          operator application, restricted residual
 //   vc_coarse_vcycle_kernel  the V-cycle below the tiled levels in one
 //                          workgroup (levels of at most 1024 cells)
+#include <hip/hip_ext.h>
+
 #include "ml_common.h"
 #include "ml_kernels.h"
 #include "ml_tile.h"
@@ -624,7 +626,8 @@ int vc_tiles(const Geom& g, int& ntx, int& nty) {
 
 template <typename T, int DIM>
 hipError_t vc_relax_dim(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
-                        const Geom* gc, const T* pc, hipStream_t s, const Geom* gr = nullptr, T* resc = nullptr) {
+                        const Geom* gc, const T* pc, hipStream_t s, const Geom* gr = nullptr, T* resc = nullptr,
+                        const LaunchTimer* tm = nullptr) {
   using D = VCRelaxTile<DIM>;
   int ntx, nty;
   const int nt = vc_tiles<D::TX, D::TY, D::TZ>(A.g, ntx, nty);
@@ -639,11 +642,29 @@ hipError_t vc_relax_dim(const VCArgs<T>& A, T* const buf[2], int* cur, const T* 
     const int h = p + (resc ? 1 : 0);
     size_t lds = size_t(D::TX + 2 * h) * (D::TY + 2 * h) * (DIM == 3 ? D::TZ + 2 * h : 1);
     if (resc) lds += size_t(D::TX) * D::TY * D::TZ;
-    hipLaunchKernelGGL((vc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, D::HMAX, D::NT>), dim3(nt), dim3(D::NT),
-                       lds * sizeof(T), s, A, buf[*cur], buf[1 - *cur], rhs, p, done & 1,
-                       (done == 0 && zero_init) ? 1 : 0, gcc, done == 0 ? pc : nullptr, ntx, nty, grr, resc);
+    const bool timed = tm && *tm->n < tm->cap;
+    if (tm && !timed) ++*tm->missed;
+    const bool zi = done == 0 && zero_init;
+    const T* pcl = done == 0 ? pc : nullptr;
+    // a timed launch carries its start and stop events in the dispatch
+    hipExtLaunchKernelGGL((vc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, D::HMAX, D::NT>), dim3(nt),
+                          dim3(D::NT), lds * sizeof(T), s, timed ? tm->start[*tm->n] : nullptr,
+                          timed ? tm->stop[*tm->n] : nullptr, 0, A, buf[*cur], buf[1 - *cur], rhs, p, done & 1,
+                          zi ? 1 : 0, gcc, pcl, ntx, nty, grr, resc);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (timed) {
+      // algorithmic bytes: state in (unless zero start), coarse correction,
+      // rhs, a, c, faces once each; state out and restricted residual once
+      const double cells = double(A.g.nvalid());
+      const double coarse = cells / double(DIM == 3 ? 8 : 4);
+      double faces = 0;
+      for (int d = 0; d < DIM; ++d) faces += double(A.g.nface[d]);
+      double b = cells * (zi ? 0 : 1) + (pcl ? coarse : 0) + cells + (A.a ? cells : 0) + (A.c ? cells : 0) + faces;
+      b += cells + (resc ? coarse : 0);
+      *tm->bytes += b * sizeof(T);
+      ++*tm->n;
+    }
     *cur = 1 - *cur;
     done += p;
   }
@@ -665,19 +686,19 @@ hipError_t vc_tile_dim(const VCArgs<T>& A, VCTileIO<T> io, hipStream_t s) {
 
 template <typename T>
 hipError_t vc_relax(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
-                    const Geom* gc, const T* pc, hipStream_t s) {
-  return A.g.dim == 2 ? vc_relax_dim<T, 2>(A, buf, cur, rhs, npass, zero_init, gc, pc, s)
-                      : vc_relax_dim<T, 3>(A, buf, cur, rhs, npass, zero_init, gc, pc, s);
+                    const Geom* gc, const T* pc, hipStream_t s, const LaunchTimer* tm) {
+  return A.g.dim == 2 ? vc_relax_dim<T, 2>(A, buf, cur, rhs, npass, zero_init, gc, pc, s, nullptr, nullptr, tm)
+                      : vc_relax_dim<T, 3>(A, buf, cur, rhs, npass, zero_init, gc, pc, s, nullptr, nullptr, tm);
 }
 
 template <typename T>
 hipError_t vc_relax_restrict(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
-                             const Geom& gr, T* resc, hipStream_t s) {
+                             const Geom& gr, T* resc, hipStream_t s, const LaunchTimer* tm) {
   const int hmax = A.g.dim == 2 ? VCRelaxTile<2>::HMAX : VCRelaxTile<3>::HMAX;
   if (npass + 1 <= hmax)
-    return A.g.dim == 2 ? vc_relax_dim<T, 2>(A, buf, cur, rhs, npass, zero_init, nullptr, nullptr, s, &gr, resc)
-                        : vc_relax_dim<T, 3>(A, buf, cur, rhs, npass, zero_init, nullptr, nullptr, s, &gr, resc);
-  hipError_t e = vc_relax<T>(A, buf, cur, rhs, npass, zero_init, nullptr, nullptr, s);
+    return A.g.dim == 2 ? vc_relax_dim<T, 2>(A, buf, cur, rhs, npass, zero_init, nullptr, nullptr, s, &gr, resc, tm)
+                        : vc_relax_dim<T, 3>(A, buf, cur, rhs, npass, zero_init, nullptr, nullptr, s, &gr, resc, tm);
+  hipError_t e = vc_relax<T>(A, buf, cur, rhs, npass, zero_init, nullptr, nullptr, s, tm);
   if (e != hipSuccess) return e;
   return vc_restrict_residual<T>(A, gr, buf[*cur], rhs, resc, s);
 }
@@ -740,9 +761,9 @@ bool vc_small_level(const Geom& g) { return g.nvalid() <= kVCWGCells; }
 int vc_coarse_levels_max(int dim) { return dim == 3 ? VCLevels<3>::n : VCLevels<2>::n; }
 
 template hipError_t vc_relax<double>(const VCArgs<double>&, double* const[2], int*, const double*, int, bool,
-                                     const Geom*, const double*, hipStream_t);
+                                     const Geom*, const double*, hipStream_t, const LaunchTimer*);
 template hipError_t vc_relax_restrict<double>(const VCArgs<double>&, double* const[2], int*, const double*, int, bool,
-                                              const Geom&, double*, hipStream_t);
+                                              const Geom&, double*, hipStream_t, const LaunchTimer*);
 template hipError_t vc_residual<double>(const VCArgs<double>&, const double*, const double*, double*,
                                         const double*, double*, unsigned long long*, int, hipStream_t,
                                         const CtlFuse&);
