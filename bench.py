@@ -115,7 +115,7 @@ class CoupledStepper(HCStepper):
         shp1 = (1,) + case["phi"].shape[1:]
         self.u0 = [ml.DeviceArray(ctx, shp1, host=db["u0"]) for _ in range(dim)]
         self.u = [ml.DeviceArray(ctx, shp1, host=db["u0"]) for _ in range(dim)]
-        self.uold = ml.DeviceArray(ctx, shp1)
+        self.uold = [ml.DeviceArray(ctx, shp1) for _ in range(dim)]
         zero = np.zeros(shp1)
         self.src = [ml.DeviceArray(ctx, shp1, host=(db["src"] if d == dim - 1 else zero)) for d in range(dim)]
         self.c = ml.DeviceArray(ctx, shp1)
@@ -147,11 +147,12 @@ class CoupledStepper(HCStepper):
         self.op.porosity(self.phi, self.chi)
         self.uop.darcy_ccoef(self.chi, self.db["Pr"] / self.db["Da"], self.c)
         self.uop.set_coefs(None, self.bu, self.c, 1.0, -self.dt)
-        vs = []
-        for u, src in zip(self.u, self.src):
-            self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, self.uold.ptr, u.ptr, u.nbytes), "copy")
-            st = self.uop.solve_be(u, self.uold, self.dt, src=src, opts=self.uopts)
-            vs.append((st.exit_status, st.iterations))
+        # the velocity components' U* solves share the operator and run
+        # concurrently
+        for u, uo in zip(self.u, self.uold):
+            self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, uo.ptr, u.ptr, u.nbytes), "copy")
+        sts = self.uop.solve_be_many(self.u, self.uold, self.dt, src=self.src, opts=self.uopts)
+        vs = [(st.exit_status, st.iterations) for st in sts]
         self.pop.porosity_faces(self.chi, -1.0, self.bp)
         self.pop.set_coefs(None, self.bp, None, 0.0, 1.0)
         st = self.pop.solve(self.p, self.prhs, opts=self.popts)

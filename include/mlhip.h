@@ -89,7 +89,9 @@ typedef struct {
  * ML_SOLVE_NO_LOOKAHEAD enqueues one iteration at a time (the host reads the
  * stopping rule after each) instead of batches that the device cuts short:
  * same results, no launches that return at once (for per-launch
- * measurements). */
+ * measurements).  ML_SOLVE_NO_GRAPH launches every kernel from the host
+ * instead of replaying the iterations captured as a HIP graph: same
+ * results. */
 typedef struct {
   int nu1, nu2, bottom, max_iter;
   double eps, hang, norm_thresh;
@@ -97,7 +99,7 @@ typedef struct {
   int flags;
 } ml_solve_opts;
 
-enum { ML_SOLVE_HOST_DRIVER = 1, ML_SOLVE_NO_LOOKAHEAD = 2 };
+enum { ML_SOLVE_HOST_DRIVER = 1, ML_SOLVE_NO_LOOKAHEAD = 2, ML_SOLVE_NO_GRAPH = 4 };
 
 typedef struct {
   int exit_status;   /* 1 |r|<=eps|r0|, 2 iteration max, 4 hang, 8 |r|<=norm_thresh */
@@ -239,6 +241,17 @@ int ml_vc_solve(ml_vc_op op, double* phi, const double* rhs, const ml_solve_opts
  * alpha = 1, beta = -dt, rhs = aCoef*phi_old + dt*src (src may be NULL) */
 int ml_vc_solve_be(ml_vc_op op, double* phi_new, const double* phi_old, const double* src,
                    double dt, const ml_solve_opts* opts, ml_solve_stats* stats);
+/* n (<= ML_VC_MAX_SOLVES) independent backward-Euler solves with the same
+ * coefficients, as the reference runs them one velocity component at a time
+ * (UstarBE[comp]->updateSoln, srcSubcycle/AMRLevelMushyLayerVel.cpp:1368-1373);
+ * here they run concurrently on separate streams (each after the work
+ * already enqueued on the context, the context continuing after all of
+ * them).  Solve i gives exactly the result of ml_vc_solve_be on
+ * (phi_new[i], phi_old[i], src ? src[i] : NULL); stats[i] its statistics. */
+enum { ML_VC_MAX_SOLVES = 4 };
+int ml_vc_solve_be_many(ml_vc_op op, int n, double* const phi_new[], const double* const phi_old[],
+                        const double* const src[], double dt, const ml_solve_opts* opts,
+                        ml_solve_stats* stats);
 /* U* Darcy coefficient on valid cells: cCoef = (chi / Pi(chi)) * darcy_coeff,
  * Kozeny-Carman Pi = chi^3/(1-chi)^2 (srcSubcycle/AMRLevelMushyLayerInit.cpp
  * :665-689, src/MushyLayerParams.cpp:1046-1049); darcy_coeff = Pr/Da */

@@ -42,6 +42,7 @@ class SolveOpts(C.Structure):
 
 SOLVE_HOST_DRIVER = 1
 SOLVE_NO_LOOKAHEAD = 2   # one iteration enqueued at a time (per-launch measurements)
+SOLVE_NO_GRAPH = 4   # launch every kernel from the host (no captured iterations)
 PRECISION_FP64, PRECISION_FP32_REFINED = 0, 1
 KT_VC_RELAX = 1   # ml_ctx_timing_start kernel family: tiled relaxation of the linear VC operator
 
@@ -107,6 +108,8 @@ SIGNATURES = {
     "ml_vc_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "ml_vc_solve_be": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.POINTER(SolveOpts),
                                  C.POINTER(SolveStats)]),
+    "ml_vc_solve_be_many": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.c_double,
+                                      C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "ml_vc_darcy_ccoef": (C.c_int, [_vp, _vp, C.c_double, _vp]),
     "ml_vc_porosity_faces": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(_vp)]),
     "ml_hc_porosity": (C.c_int, [_vp, _vp, _vp]),
@@ -450,6 +453,18 @@ class VCOp:
                                                    src.ptr if src is not None else None, dt, C.byref(o),
                                                    C.byref(st)), "ml_vc_solve_be")
         return st
+
+    def solve_be_many(self, phi_new, phi_old, dt, src=None, opts=None):
+        """Independent backward-Euler solves (lists of DeviceArray), run
+        concurrently; returns their SolveStats."""
+        n = len(phi_new)
+        st = (SolveStats * n)()
+        o = opts or default_opts()
+        arr = lambda xs: (_vp * n)(*[x.ptr for x in xs])
+        self.ctx.check(self.ctx.lib.ml_vc_solve_be_many(self.h, n, arr(phi_new), arr(phi_old),
+                                                        arr(src) if src is not None else None, dt, C.byref(o), st),
+                       "ml_vc_solve_be_many")
+        return list(st)
 
     def close(self):
         if self.h:

@@ -1032,6 +1032,18 @@ struct LinTile<3> {
   static constexpr int TX = 16, TY = 8, TZ = 4;
 };
 
+// tiles of the deep (up to 8 passes) 2D relaxation launch
+struct LinRelaxTile2 {
+  static constexpr int TX = 16, TY = 16, TZ = 1, NT = 1024;   // measured best of 7 shapes at 256^2
+};
+
+template <int TX, int TY>
+int tiles2(const Geom& g, int& ntx, int& nty) {
+  ntx = (g.n[0] + TX - 1) / TX;
+  nty = (g.n[1] + TY - 1) / TY;
+  return ntx * nty;
+}
+
 template <int DIM>
 int lin_tiles(const Geom& g, int& ntx, int& nty) {
   using D = LinTile<DIM>;
@@ -1095,17 +1107,22 @@ hipError_t lin_relax_impl(const LinLevel<float>& L, float* const ebuf[2], int* c
     const float* p0 = done == 0 ? pc : nullptr;
     if (L.g.dim == 2) {
       using D = LinTile<2>;
+      using DR = LinRelaxTile2;
       const int nt = lin_tiles<2>(L.g, ntx, nty);
       const int pmax = (nt <= 1024 || (C && npass + hx > 2)) ? 8 : 2;
       const int p = (npass - done) < pmax ? (npass - done) : pmax;
       if (C && p + hx > pmax) return hipErrorInvalidValue;
-      const size_t lds = (8 * lin_extent<2>(p + hx) + (C ? 2 * D::TX * D::TY : 0)) * sizeof(float);
-      if (pmax == 8)
-        hipLaunchKernelGGL((lin_relax_tile_kernel<2, D::TX, D::TY, D::TZ, 8, 512>), dim3(nt), dim3(512), lds, s, L,
-                           ein, eout, bc, alpha, beta, p, zi, gcc, p0, ntx, nty, grr, rc, Mc);
-      else
+      if (pmax == 8) {
+        const int ntr = tiles2<DR::TX, DR::TY>(L.g, ntx, nty);
+        const size_t lds = (8 * size_t(DR::TX + 2 * (p + hx)) * (DR::TY + 2 * (p + hx)) +
+                            (C ? 2 * DR::TX * DR::TY : 0)) * sizeof(float);
+        hipLaunchKernelGGL((lin_relax_tile_kernel<2, DR::TX, DR::TY, DR::TZ, 8, DR::NT>), dim3(ntr), dim3(DR::NT), lds,
+                           s, L, ein, eout, bc, alpha, beta, p, zi, gcc, p0, ntx, nty, grr, rc, Mc);
+      } else {
+        const size_t lds = (8 * lin_extent<2>(p + hx) + (C ? 2 * D::TX * D::TY : 0)) * sizeof(float);
         hipLaunchKernelGGL((lin_relax_tile_kernel<2, D::TX, D::TY, D::TZ, 2, 256>), dim3(nt), dim3(256), lds, s, L,
                            ein, eout, bc, alpha, beta, p, zi, gcc, p0, ntx, nty, grr, rc, Mc);
+      }
       done += p;
     } else {
       using D = LinTile<3>;

@@ -157,9 +157,36 @@ struct VCLevelStore {
   double* a = nullptr;
   double* c = nullptr;
   double* b[3] = {nullptr, nullptr, nullptr};
-  double* e[2] = {nullptr, nullptr};   // correction and its ping-pong partner
-  double* r = nullptr;                 // residual right-hand side
   double* tmp = nullptr;               // partner for the per-level relax entry point (lazy)
+};
+
+// Workspace of one solve with an ml_vc_op's operator: per level the
+// correction (with its ping-pong partner on tiled levels) and the residual
+// right-hand side, the level-0 state partner, the backward-Euler rhs, norm
+// slots, the coarse-kernel descriptor and the iteration control; and the
+// stream the solve runs on (the context's for workspace 0, an own stream of
+// default priority for the others, ordered after the context stream).
+struct VCWork {
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  hipEvent_t joined = nullptr;          // recorded on `stream` when a solve ends
+  std::vector<double*> e0, e1, r;       // per level
+  double* alt0 = nullptr;
+  double* rhs_be = nullptr;
+  unsigned long long* slots = nullptr;
+  VCDesc<double>* desc = nullptr;       // device
+  VCDesc<double>* hdesc = nullptr;      // pinned staging
+  SolveCtl* ctl = nullptr;              // device iteration control
+  SolveCtl* hctl = nullptr;             // pinned mirror
+  unsigned int* arrive = nullptr;       // workgroup arrival counter of the fused stopping rule
+  int batch = 4;                        // iterations enqueued per host check
+  // two iterations (V-cycle + residual, even then odd) captured as a HIP
+  // graph, and what the capture depends on
+  hipGraphExec_t gexec = nullptr;
+  const double* g_phi = nullptr;
+  const double* g_rhs = nullptr;
+  int g_nu1 = -1, g_nu2 = -1, g_bottom = -1;
+  double g_alpha = 0, g_beta = 0;
 };
 }  // namespace ml
 
@@ -171,18 +198,9 @@ struct ml_vc_op_s {
   bool has_a = false, has_c = false;
   std::vector<ml::VCLevelStore> lev;
   int lc = 0;                       // first level handled by the single-workgroup kernel
-  double* alt0 = nullptr;           // level-0 state partner of the solve
-  double* rhs_be = nullptr;
-  unsigned long long* slots = nullptr;
-  unsigned long long* hslots = nullptr;
-  int nslots = 0;
-  ml::VCDesc<double>* desc = nullptr;    // device
-  ml::VCDesc<double>* hdesc = nullptr;   // pinned staging
-  ml::SolveCtl* ctl = nullptr;           // device iteration control
-  unsigned int* arrive = nullptr;        // workgroup arrival counter of the fused stopping rule
-  ml::SolveCtl* hctl = nullptr;          // pinned mirror
-  int batch = 4;                         // iterations enqueued per host check
-  size_t coarse_lds = 0;                 // dynamic LDS of the single-workgroup V-cycle
+  int nslots = 1024;                // norm slots per workspace (max_iter + 1)
+  std::vector<ml::VCWork> work;     // solve workspaces; [0] runs on the context stream
+  size_t coarse_lds = 0;            // dynamic LDS of the single-workgroup V-cycle
 
   ml::VCArgs<double> args(int depth) const {
     ml::VCArgs<double> A;

@@ -72,7 +72,9 @@ __device__ inline void ctl_check_step(SolveCtl* c, const unsigned long long* slo
 
 // The stopping rule applied by the last workgroup of the kernel that
 // computes the residual norm (ctl == nullptr: none).  `arrive` counts the
-// workgroups that have added their norm; the last one resets it.
+// workgroups that have added their norm; the last one resets it.  A kernel
+// given no norm pointer writes the norm into slots[ctl->iter + 1] (the
+// iteration it completes), so the same launch serves every iteration.
 struct CtlFuse {
   SolveCtl* ctl;
   const unsigned long long* slots;
@@ -84,6 +86,11 @@ struct CtlFuse {
 // atomicMax on its IEEE bit pattern (monotone for non-negative doubles; a
 // NaN anywhere wins); with f.ctl set, the grid's last workgroup then applies
 // the stopping rule.
+
+// the norm slot of a residual kernel: `norm`, or the iteration's slot
+__device__ inline unsigned long long* ctl_slot(unsigned long long* norm, const CtlFuse& f) {
+  return norm ? norm : const_cast<unsigned long long*>(f.slots) + f.ctl->iter + 1;
+}
 template <typename T>
 __device__ inline void block_max_atomic(T v, int nan, unsigned long long* out, CtlFuse f = CtlFuse{}) {
   double d = nan ? __longlong_as_double(0x7ff8000000000000LL) : double(v);

@@ -31,10 +31,90 @@ int vc_alloc(ml_ctx ctx, size_t n, double** p) {
   return ML_OK;
 }
 
-int vc_upload_desc(ml_vc_op op, const ml_solve_opts& o) {
+void vc_work_free(ml::VCWork& W) {
+  auto f = [](double* q) {
+    if (q) hipFree(q);
+  };
+  for (double* q : W.e0) f(q);
+  for (double* q : W.e1) f(q);
+  for (double* q : W.r) f(q);
+  f(W.alt0);
+  f(W.rhs_be);
+  if (W.slots) hipFree(W.slots);
+  if (W.desc) hipFree(W.desc);
+  if (W.hdesc) hipHostFree(W.hdesc);
+  if (W.ctl) hipFree(W.ctl);
+  if (W.hctl) hipHostFree(W.hctl);
+  if (W.arrive) hipFree(W.arrive);
+  if (W.joined) hipEventDestroy(W.joined);
+  if (W.gexec) hipGraphExecDestroy(W.gexec);
+  if (W.own_stream && W.stream) hipStreamDestroy(W.stream);
+  W = ml::VCWork{};
+}
+
+// allocate workspace W (own_stream: a new stream of default priority)
+int vc_work_init(ml_vc_op op, ml::VCWork& W, bool own_stream) {
   ml_ctx ctx = op->ctx;
-  ML_HIP(ctx, hipStreamSynchronize(ctx->stream));   // the staging buffer may still be in flight
-  ml::VCDesc<double>& D = *op->hdesc;
+  const int L = int(op->lev.size());
+  W.e0.assign(L, nullptr);
+  W.e1.assign(L, nullptr);
+  W.r.assign(L, nullptr);
+  int rc;
+  for (int l = 0; l < L; ++l) {
+    const ml::Geom& g = op->lev[l].g;
+    if ((rc = vc_alloc(ctx, g.ncell, &W.e0[l]))) return rc;
+    if ((rc = vc_alloc(ctx, g.ncell, &W.r[l]))) return rc;
+    // tiled levels ping-pong their correction
+    if (l < op->lc || op->lc >= L)
+      if ((rc = vc_alloc(ctx, g.ncell, &W.e1[l]))) return rc;
+  }
+  const ml::Geom& g0 = op->lev[0].g;
+  if ((rc = vc_alloc(ctx, g0.ncell, &W.alt0))) return rc;
+  if ((rc = vc_alloc(ctx, g0.ncell, &W.rhs_be))) return rc;
+  void* p = nullptr;
+  ML_HIP(ctx, hipMalloc(&p, sizeof(unsigned long long) * op->nslots));
+  W.slots = static_cast<unsigned long long*>(p);
+  ML_HIP(ctx, hipMalloc(&p, sizeof(ml::VCDesc<double>)));
+  W.desc = static_cast<ml::VCDesc<double>*>(p);
+  ML_HIP(ctx, hipHostMalloc(&p, sizeof(ml::VCDesc<double>), hipHostMallocDefault));
+  W.hdesc = static_cast<ml::VCDesc<double>*>(p);
+  ML_HIP(ctx, hipMalloc(&p, sizeof(ml::SolveCtl)));
+  W.ctl = static_cast<ml::SolveCtl*>(p);
+  ML_HIP(ctx, hipHostMalloc(&p, sizeof(ml::SolveCtl), hipHostMallocDefault));
+  W.hctl = static_cast<ml::SolveCtl*>(p);
+  ML_HIP(ctx, hipMalloc(&p, sizeof(unsigned int)));
+  W.arrive = static_cast<unsigned int*>(p);
+  ML_HIP(ctx, hipMemsetAsync(W.arrive, 0, sizeof(unsigned int), ctx->stream));
+  ML_HIP(ctx, hipEventCreateWithFlags(&W.joined, hipEventDisableTiming));
+  if (own_stream) {
+    ML_HIP(ctx, hipStreamCreateWithFlags(&W.stream, hipStreamNonBlocking));
+    W.own_stream = true;
+  } else {
+    W.stream = ctx->stream;
+  }
+  // the allocations' zero fills run on the context stream
+  ML_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ML_OK;
+}
+
+// workspaces 0..n-1, creating the missing ones
+int vc_work_ensure(ml_vc_op op, int n) {
+  while (int(op->work.size()) < n) {
+    op->work.emplace_back();
+    const int rc = vc_work_init(op, op->work.back(), op->work.size() > 1);
+    if (rc) {
+      vc_work_free(op->work.back());
+      op->work.pop_back();
+      return rc;
+    }
+  }
+  return ML_OK;
+}
+
+int vc_upload_desc(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o) {
+  ml_ctx ctx = op->ctx;
+  ML_HIP(ctx, hipStreamSynchronize(W.stream));   // the staging buffer may still be in flight
+  ml::VCDesc<double>& D = *W.hdesc;
   std::memset(&D, 0, sizeof(D));
   D.l0 = op->lc;
   D.nlev = int(op->lev.size());
@@ -44,7 +124,7 @@ int vc_upload_desc(ml_vc_op op, const ml_solve_opts& o) {
   D.bc = op->bc;
   D.alpha = op->alpha;
   D.beta = op->beta;
-  D.skip = &op->ctl->done;
+  D.skip = &W.ctl->done;
   for (int l = 0; l < D.nlev; ++l) {
     const ml::VCArgs<double> A = op->args(l);
     ml::VCLevel<double>& L = D.lev[l];
@@ -52,21 +132,25 @@ int vc_upload_desc(ml_vc_op op, const ml_solve_opts& o) {
     L.a = A.a;
     L.c = A.c;
     for (int d = 0; d < 3; ++d) L.b[d] = A.b[d];
-    L.e = op->lev[l].e[0];
-    L.r = op->lev[l].r;
+    L.e = W.e0[l];
+    L.r = W.r[l];
   }
-  ML_HIP(ctx, hipMemcpyAsync(op->desc, op->hdesc, sizeof(D), hipMemcpyHostToDevice, ctx->stream));
+  ML_HIP(ctx, hipMemcpyAsync(W.desc, W.hdesc, sizeof(D), hipMemcpyHostToDevice, W.stream));
   return ML_OK;
 }
 
 // one V-cycle of the correction equation; on return the level-0 correction
-// is op->lev[0].e[ecur[0]]
-int vc_vcycle(ml_vc_op op, const ml_solve_opts& o, int* ecur, const int* skip) {
+// is W's level-0 buffer ecur[0]
+int vc_vcycle(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o, int* ecur, const int* skip) {
   ml_ctx ctx = op->ctx;
-  hipStream_t s = ctx->stream;
+  hipStream_t s = W.stream;
   const int L = int(op->lev.size());
   const int lc = op->lc;
-  auto E = [&](int l) { return op->lev[l].e[ecur[l]]; };
+  auto buf = [&](int l, double** b) {
+    b[0] = W.e0[l];
+    b[1] = W.e1[l];
+  };
+  auto E = [&](int l) { return ecur[l] ? W.e1[l] : W.e0[l]; };
   auto args = [&](int l) {
     ml::VCArgs<double> A = op->args(l);
     A.skip = skip;
@@ -75,82 +159,216 @@ int vc_vcycle(ml_vc_op op, const ml_solve_opts& o, int* ecur, const int* skip) {
   const int ltiles = lc < L ? lc : L - 1;
   const ml::LaunchTimer T = ctx->timer(ML_KT_VC_RELAX);
   const ml::LaunchTimer* tm = T.start ? &T : nullptr;
+  double* B[2];
   if (lc >= L && L == 1) {
-    ML_HIP(ctx, ml::vc_relax<double>(args(0), op->lev[0].e, &ecur[0], op->lev[0].r, 2 * o.bottom, true, nullptr,
-                                      nullptr, s, tm));
+    buf(0, B);
+    ML_HIP(ctx, ml::vc_relax<double>(args(0), B, &ecur[0], W.r[0], 2 * o.bottom, true, nullptr, nullptr, s, tm));
     return ML_OK;
   }
   for (int l = 0; l < ltiles; ++l) {
-    ML_HIP(ctx, ml::vc_relax_restrict<double>(args(l), op->lev[l].e, &ecur[l], op->lev[l].r, 2 * o.nu1, true,
-                                               op->lev[l + 1].g, op->lev[l + 1].r, s, tm));
+    buf(l, B);
+    ML_HIP(ctx, ml::vc_relax_restrict<double>(args(l), B, &ecur[l], W.r[l], 2 * o.nu1, true, op->lev[l + 1].g,
+                                               W.r[l + 1], s, tm));
   }
   if (lc < L) {
-    ML_HIP(ctx, ml::vc_coarse_vcycle<double>(op->desc, op->dim, op->coarse_lds, s));
+    ML_HIP(ctx, ml::vc_coarse_vcycle<double>(W.desc, op->dim, op->coarse_lds, s));
   } else {
-    ML_HIP(ctx, ml::vc_relax<double>(args(L - 1), op->lev[L - 1].e, &ecur[L - 1], op->lev[L - 1].r, 2 * o.bottom,
-                                      true, nullptr, nullptr, s, tm));
+    buf(L - 1, B);
+    ML_HIP(ctx, ml::vc_relax<double>(args(L - 1), B, &ecur[L - 1], W.r[L - 1], 2 * o.bottom, true, nullptr, nullptr,
+                                      s, tm));
   }
   for (int l = ltiles - 1; l >= 0; --l) {
     const ml::Geom gc = op->lev[l + 1].g;
-    ML_HIP(ctx, ml::vc_relax<double>(args(l), op->lev[l].e, &ecur[l], op->lev[l].r, 2 * o.nu2, false, &gc,
-                                      E(l + 1), s, tm));
+    buf(l, B);
+    ML_HIP(ctx, ml::vc_relax<double>(args(l), B, &ecur[l], W.r[l], 2 * o.nu2, false, &gc, E(l + 1), s, tm));
   }
   return ML_OK;
 }
 
-int vc_read_ctl(ml_vc_op op, ml::SolveCtl* out) {
+int vc_read_ctl(ml_vc_op op, ml::VCWork& W, ml::SolveCtl* out) {
   ml_ctx ctx = op->ctx;
-  ML_HIP(ctx, hipMemcpyAsync(op->hctl, op->ctl, sizeof(ml::SolveCtl), hipMemcpyDeviceToHost, ctx->stream));
-  ML_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  *out = *op->hctl;
+  ML_HIP(ctx, hipMemcpyAsync(W.hctl, W.ctl, sizeof(ml::SolveCtl), hipMemcpyDeviceToHost, W.stream));
+  ML_HIP(ctx, hipStreamSynchronize(W.stream));
+  *out = *W.hctl;
   return ML_OK;
 }
 
-int vc_solve(ml_vc_op op, double* phi, const double* rhs, const ml_solve_opts& o, ml_solve_stats* st) {
+// One solve on a workspace, split so that several can be interleaved:
+// begin (initial residual, control block), batches of iterations, end.
+struct VCSolve {
+  ml::VCWork* W = nullptr;
+  double* phi = nullptr;
+  const double* rhs = nullptr;
+  std::vector<int> ecur;
+  int launched = 0, batch = 1;
+  bool lookahead = true, finished = false, graph = false;
+  ml::SolveCtl c{};
+};
+
+// one iteration: V-cycle, then phi(out) = phi(in) + e and the residual
+// (its norm into the slot of the iteration it completes, the stopping rule
+// in the kernel's last workgroup)
+int vc_iteration(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o, int* ecur, double* phi_in, double* phi_out,
+                 const double* rhs) {
   ml_ctx ctx = op->ctx;
-  hipStream_t s = ctx->stream;
+  const int* skip = &W.ctl->done;
+  int rc = vc_vcycle(op, W, o, ecur, skip);
+  if (rc) return rc;
+  ml::VCArgs<double> A0 = op->args(0);
+  A0.skip = skip;
+  double* e0 = ecur[0] ? W.e1[0] : W.e0[0];
+  ML_HIP(ctx, ml::vc_residual<double>(A0, phi_in, e0, phi_out, rhs, W.r[0], nullptr, 0, W.stream,
+                                       ml::CtlFuse{W.ctl, W.slots, W.arrive, 0}));
+  return ML_OK;
+}
+
+// Capture two iterations (state phi -> partner -> phi) as a HIP graph on W's
+// stream; every pointer and value they use is fixed for the solve (the norm
+// slot comes from the control block).  Returns false when the capture is
+// not usable (the V-cycle's ping-pong state does not return to its start).
+bool vc_capture(ml_vc_op op, ml::VCWork& W, VCSolve& S, const ml_solve_opts& o) {
+  if (W.gexec && W.g_phi == S.phi && W.g_rhs == S.rhs && W.g_nu1 == o.nu1 && W.g_nu2 == o.nu2 &&
+      W.g_bottom == o.bottom && W.g_alpha == op->alpha && W.g_beta == op->beta)
+    return true;
+  if (W.gexec) hipGraphExecDestroy(W.gexec);
+  W.gexec = nullptr;
+  if (hipStreamBeginCapture(W.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+  std::vector<int> ec(op->lev.size(), 0);
+  int rc = vc_iteration(op, W, o, ec.data(), S.phi, W.alt0, S.rhs);
+  if (!rc) rc = vc_iteration(op, W, o, ec.data(), W.alt0, S.phi, S.rhs);
+  hipGraph_t graph = nullptr;
+  const hipError_t ee = hipStreamEndCapture(W.stream, &graph);
+  bool ok = !rc && ee == hipSuccess && graph;
+  for (int v : ec) ok = ok && v == 0;
+  if (ok) ok = hipGraphInstantiate(&W.gexec, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (graph) hipGraphDestroy(graph);
+  if (!ok) {
+    W.gexec = nullptr;
+    return false;
+  }
+  W.g_phi = S.phi;
+  W.g_rhs = S.rhs;
+  W.g_nu1 = o.nu1;
+  W.g_nu2 = o.nu2;
+  W.g_bottom = o.bottom;
+  W.g_alpha = op->alpha;
+  W.g_beta = op->beta;
+  return true;
+}
+
+int vc_solve_check(ml_vc_op op, const ml_solve_opts& o) {
+  ml_ctx ctx = op->ctx;
   if (o.max_iter + 1 > op->nslots) return fail(ctx, ML_EINVAL, "max_iter too large");
   if (o.nu1 < 0 || o.nu2 < 0 || o.bottom < 0) return fail(ctx, ML_EINVAL, "negative smoothing count");
   if (o.precision != ML_PRECISION_FP64) return fail(ctx, ML_ENOTSUP, "the linear operator solves in FP64 only");
-  int rc = vc_upload_desc(op, o);
+  if (o.flags & ML_SOLVE_HOST_DRIVER) return fail(ctx, ML_ENOTSUP, "the linear solve runs on the fused engine");
+  return ML_OK;
+}
+
+int vc_solve_begin(ml_vc_op op, VCSolve& S, const ml_solve_opts& o) {
+  ml_ctx ctx = op->ctx;
+  ml::VCWork& W = *S.W;
+  hipStream_t s = W.stream;
+  int rc = vc_upload_desc(op, W, o);
   if (rc) return rc;
-  const int L = int(op->lev.size());
   const ml::Geom& g0 = op->lev[0].g;
-  const size_t bytes0 = sizeof(double) * g0.ncell;
-  ML_HIP(ctx, hipMemcpyAsync(op->alt0, phi, bytes0, hipMemcpyDeviceToDevice, s));
-  double* const pb[2] = {phi, op->alt0};
-  std::vector<int> ecur(L, 0);
-  ML_HIP(ctx, ml::zero_u64(op->slots, o.max_iter + 1, s));
-  ml::VCArgs<double> A0 = op->args(0);
-  ML_HIP(ctx, ml::vc_residual<double>(A0, phi, nullptr, nullptr, rhs, op->lev[0].r, op->slots, 0, s));
-  ML_HIP(ctx, ml::ctl_init(op->ctl, op->slots, o.max_iter, o.eps, o.hang, o.norm_thresh, s));
-  const int* skip = &op->ctl->done;
-  A0.skip = skip;
-  // iterations are enqueued in batches; the device stops them (SolveCtl)
-  int launched = 0;
-  const bool lookahead = !(o.flags & ML_SOLVE_NO_LOOKAHEAD);
-  int batch = lookahead ? std::max(1, std::min(op->batch, o.max_iter)) : 1;
-  ml::SolveCtl c{};
-  for (;;) {
-    for (int b = 0; b < batch && launched < o.max_iter; ++b) {
-      if ((rc = vc_vcycle(op, o, ecur.data(), skip))) return rc;
-      ++launched;
-      const int pcur = (launched - 1) & 1;   // phi += e alternates the state buffer
-      // the residual kernel's last workgroup applies the stopping rule
-      ML_HIP(ctx, ml::vc_residual<double>(A0, pb[pcur], op->lev[0].e[ecur[0]], pb[1 - pcur], rhs, op->lev[0].r,
-                                           op->slots + launched, 0, s, ml::CtlFuse{op->ctl, op->slots, op->arrive, 0}));
+  ML_HIP(ctx, hipMemcpyAsync(W.alt0, S.phi, sizeof(double) * g0.ncell, hipMemcpyDeviceToDevice, s));
+  S.ecur.assign(op->lev.size(), 0);
+  ML_HIP(ctx, ml::zero_u64(W.slots, o.max_iter + 1, s));
+  ML_HIP(ctx, ml::vc_residual<double>(op->args(0), S.phi, nullptr, nullptr, S.rhs, W.r[0], W.slots, 0, s));
+  ML_HIP(ctx, ml::ctl_init(W.ctl, W.slots, o.max_iter, o.eps, o.hang, o.norm_thresh, s));
+  S.lookahead = !(o.flags & ML_SOLVE_NO_LOOKAHEAD);
+  S.batch = S.lookahead ? std::max(1, std::min(W.batch, o.max_iter)) : 1;
+  S.launched = 0;
+  S.finished = false;
+  // graphs replay two iterations at a time; not with one-at-a-time
+  // iterations or while launches are being timed
+  S.graph = !(o.flags & ML_SOLVE_NO_GRAPH) && S.lookahead && ctx->timing_kernel == 0 && vc_capture(op, W, S, o);
+  return ML_OK;
+}
+
+// iterations are enqueued in batches; the device stops them (SolveCtl)
+int vc_solve_enqueue(ml_vc_op op, VCSolve& S, const ml_solve_opts& o) {
+  ml_ctx ctx = op->ctx;
+  ml::VCWork& W = *S.W;
+  if (S.graph) {
+    // iterations past the stopping point return at once (done flag)
+    for (int b = 0; b < S.batch && S.launched < o.max_iter; b += 2) {
+      ML_HIP(ctx, hipGraphLaunch(W.gexec, W.stream));
+      S.launched += 2;
     }
-    if ((rc = vc_read_ctl(op, &c))) return rc;
-    if (c.done || launched >= o.max_iter) break;
-    batch = lookahead ? 2 : 1;
+    return ML_OK;
   }
-  if (lookahead) op->batch = std::max(1, c.iter);
-  if (c.iter & 1) ML_HIP(ctx, hipMemcpyAsync(phi, op->alt0, bytes0, hipMemcpyDeviceToDevice, s));
+  double* const pb[2] = {S.phi, W.alt0};
+  for (int b = 0; b < S.batch && S.launched < o.max_iter; ++b) {
+    ++S.launched;
+    const int pcur = (S.launched - 1) & 1;   // phi += e alternates the state buffer
+    int rc = vc_iteration(op, W, o, S.ecur.data(), pb[pcur], pb[1 - pcur], S.rhs);
+    if (rc) return rc;
+  }
+  return ML_OK;
+}
+
+int vc_solve_poll(ml_vc_op op, VCSolve& S, const ml_solve_opts& o) {
+  int rc = vc_read_ctl(op, *S.W, &S.c);
+  if (rc) return rc;
+  if (S.c.done || S.launched >= o.max_iter) S.finished = true;
+  S.batch = S.lookahead ? 2 : 1;
+  return ML_OK;
+}
+
+int vc_solve_end(ml_vc_op op, VCSolve& S, ml_solve_stats* st) {
+  ml_ctx ctx = op->ctx;
+  ml::VCWork& W = *S.W;
+  if (S.lookahead) W.batch = std::max(1, S.c.iter);
+  const size_t bytes0 = sizeof(double) * op->lev[0].g.ncell;
+  if (S.c.iter & 1) ML_HIP(ctx, hipMemcpyAsync(S.phi, W.alt0, bytes0, hipMemcpyDeviceToDevice, W.stream));
   if (st) {
-    st->exit_status = c.status != 0 ? c.status : 2;
-    st->iterations = c.iter;
-    st->initial_residual = c.r0;
-    st->final_residual = c.norm;
+    st->exit_status = S.c.status != 0 ? S.c.status : 2;
+    st->iterations = S.c.iter;
+    st->initial_residual = S.c.r0;
+    st->final_residual = S.c.norm;
+  }
+  return ML_OK;
+}
+
+// n solves on workspaces 0..n-1 (phi[i] updated in place, rhs[i]): the
+// workspaces' streams start after the work already on the context stream,
+// and the context stream continues after all of them
+int vc_solve_many(ml_vc_op op, int n, double* const phi[], const double* const rhs[], const ml_solve_opts& o,
+                  ml_solve_stats* st) {
+  ml_ctx ctx = op->ctx;
+  int rc = vc_solve_check(op, o);
+  if (rc) return rc;
+  if ((rc = vc_work_ensure(op, n))) return rc;
+  std::vector<VCSolve> S(n);
+  for (int i = 0; i < n; ++i) {
+    S[i].W = &op->work[i];
+    S[i].phi = phi[i];
+    S[i].rhs = rhs[i];
+  }
+  if (n > 1) {
+    ML_HIP(ctx, hipEventRecord(op->work[0].joined, ctx->stream));
+    for (int i = 1; i < n; ++i) ML_HIP(ctx, hipStreamWaitEvent(op->work[i].stream, op->work[0].joined, 0));
+  }
+  for (int i = 0; i < n; ++i)
+    if ((rc = vc_solve_begin(op, S[i], o))) return rc;
+  for (;;) {
+    bool any = false;
+    for (int i = 0; i < n; ++i)
+      if (!S[i].finished && (rc = vc_solve_enqueue(op, S[i], o))) return rc;
+    for (int i = 0; i < n; ++i) {
+      if (S[i].finished) continue;
+      if ((rc = vc_solve_poll(op, S[i], o))) return rc;
+      any = any || !S[i].finished;
+    }
+    if (!any) break;
+  }
+  for (int i = 0; i < n; ++i)
+    if ((rc = vc_solve_end(op, S[i], st ? st + i : nullptr))) return rc;
+  for (int i = 1; i < n; ++i) {
+    ML_HIP(ctx, hipEventRecord(op->work[i].joined, op->work[i].stream));
+    ML_HIP(ctx, hipStreamWaitEvent(ctx->stream, op->work[i].joined, 0));
   }
   return ML_OK;
 }
@@ -203,11 +421,6 @@ int ml_vc_op_create(ml_ctx ctx, int dim, const int lo[], const int hi[], double 
     int rc;
     for (int d = 0; d < dim; ++d)
       if ((rc = vc_alloc(ctx, L.g.nface[d], &L.b[d]))) return bail(rc);
-    if ((rc = vc_alloc(ctx, L.g.ncell, &L.e[0]))) return bail(rc);
-    if ((rc = vc_alloc(ctx, L.g.ncell, &L.r))) return bail(rc);
-    // tiled levels ping-pong their correction
-    if (l < raw->lc || raw->lc >= depth)
-      if ((rc = vc_alloc(ctx, L.g.ncell, &L.e[1]))) return bail(rc);
   }
   int rc;
   if (raw->lc < depth) {
@@ -221,26 +434,7 @@ int ml_vc_op_create(ml_ctx ctx, int dim, const int lo[], const int hi[], double 
       return bail(ML_ENOTSUP);
     }
   }
-  if ((rc = vc_alloc(ctx, g0.ncell, &raw->alt0))) return bail(rc);
-  if ((rc = vc_alloc(ctx, g0.ncell, &raw->rhs_be))) return bail(rc);
-  raw->nslots = 1024;
-  void* p = nullptr;
-  if (hipMalloc(&p, sizeof(unsigned long long) * raw->nslots) != hipSuccess) return bail(ML_ENOMEM);
-  raw->slots = static_cast<unsigned long long*>(p);
-  if (hipHostMalloc(&p, sizeof(unsigned long long) * raw->nslots, hipHostMallocDefault) != hipSuccess)
-    return bail(ML_ENOMEM);
-  raw->hslots = static_cast<unsigned long long*>(p);
-  if (hipMalloc(&p, sizeof(ml::VCDesc<double>)) != hipSuccess) return bail(ML_ENOMEM);
-  raw->desc = static_cast<ml::VCDesc<double>*>(p);
-  if (hipHostMalloc(&p, sizeof(ml::VCDesc<double>), hipHostMallocDefault) != hipSuccess) return bail(ML_ENOMEM);
-  raw->hdesc = static_cast<ml::VCDesc<double>*>(p);
-  if (hipMalloc(&p, sizeof(ml::SolveCtl)) != hipSuccess) return bail(ML_ENOMEM);
-  raw->ctl = static_cast<ml::SolveCtl*>(p);
-  if (hipMalloc(&p, sizeof(unsigned int)) != hipSuccess) return bail(ML_ENOMEM);
-  raw->arrive = static_cast<unsigned int*>(p);
-  if (hipMemsetAsync(raw->arrive, 0, sizeof(unsigned int), ctx->stream) != hipSuccess) return bail(ML_EHIP);
-  if (hipHostMalloc(&p, sizeof(ml::SolveCtl), hipHostMallocDefault) != hipSuccess) return bail(ML_ENOMEM);
-  raw->hctl = static_cast<ml::SolveCtl*>(p);
+  if ((rc = vc_work_ensure(raw, 1))) return bail(rc);
   *out = op.release();
   return ML_OK;
 }
@@ -248,25 +442,17 @@ int ml_vc_op_create(ml_ctx ctx, int dim, const int lo[], const int hi[], double 
 int ml_vc_op_destroy(ml_vc_op op) {
   if (!op) return ML_EINVAL;
   if (op->ctx) hipStreamSynchronize(op->ctx->stream);
+  for (auto& W : op->work) {
+    if (W.stream) hipStreamSynchronize(W.stream);
+    vc_work_free(W);
+  }
   for (auto& L : op->lev) {
     for (int d = 0; d < 3; ++d)
       if (L.b[d]) hipFree(L.b[d]);
     if (L.a) hipFree(L.a);
     if (L.c) hipFree(L.c);
-    if (L.e[0]) hipFree(L.e[0]);
-    if (L.e[1]) hipFree(L.e[1]);
-    if (L.r) hipFree(L.r);
     if (L.tmp) hipFree(L.tmp);
   }
-  if (op->alt0) hipFree(op->alt0);
-  if (op->rhs_be) hipFree(op->rhs_be);
-  if (op->slots) hipFree(op->slots);
-  if (op->hslots) hipHostFree(op->hslots);
-  if (op->desc) hipFree(op->desc);
-  if (op->hdesc) hipHostFree(op->hdesc);
-  if (op->ctl) hipFree(op->ctl);
-  if (op->arrive) hipFree(op->arrive);
-  if (op->hctl) hipHostFree(op->hctl);
   delete op;
   return ML_OK;
 }
@@ -383,18 +569,42 @@ int ml_vc_solve(ml_vc_op op, double* phi, const double* rhs, const ml_solve_opts
     o = *opts;
   else
     ml_default_opts(&o);
-  if (o.flags & ML_SOLVE_HOST_DRIVER) return fail(op->ctx, ML_ENOTSUP, "the linear solve runs on the fused engine");
-  return vc_solve(op, phi, rhs, o, stats);
+  double* const p[1] = {phi};
+  const double* const r[1] = {rhs};
+  return vc_solve_many(op, 1, p, r, o, stats);
 }
 
 int ml_vc_solve_be(ml_vc_op op, double* phi_new, const double* phi_old, const double* src, double dt,
                    const ml_solve_opts* opts, ml_solve_stats* stats) {
-  if (!op || !phi_new || !phi_old) return ML_EINVAL;
+  double* const pn[1] = {phi_new};
+  const double* const po[1] = {phi_old};
+  const double* const ps[1] = {src};
+  return ml_vc_solve_be_many(op, 1, pn, po, src ? ps : nullptr, dt, opts, stats);
+}
+
+int ml_vc_solve_be_many(ml_vc_op op, int n, double* const phi_new[], const double* const phi_old[],
+                        const double* const src[], double dt, const ml_solve_opts* opts, ml_solve_stats* stats) {
+  if (!op || n < 1 || n > ML_VC_MAX_SOLVES || !phi_new || !phi_old) return ML_EINVAL;
+  for (int i = 0; i < n; ++i)
+    if (!phi_new[i] || !phi_old[i]) return ML_EINVAL;
+  ml_solve_opts o;
+  if (opts)
+    o = *opts;
+  else
+    ml_default_opts(&o);
+  int rc = vc_solve_check(op, o);
+  if (rc) return rc;
+  if ((rc = vc_work_ensure(op, n))) return rc;
   op->alpha = 1.0;
   op->beta = -dt;
-  ML_HIP(op->ctx, ml::be_rhs<double>(op->lev[0].g, 1, op->rhs_be, phi_old, op->has_a ? op->lev[0].a : nullptr, dt,
-                                     src, op->ctx->stream));
-  return ml_vc_solve(op, phi_new, op->rhs_be, opts, stats);
+  std::vector<const double*> rhs(n);
+  for (int i = 0; i < n; ++i) {
+    ML_HIP(op->ctx, ml::be_rhs<double>(op->lev[0].g, 1, op->work[i].rhs_be, phi_old[i],
+                                       op->has_a ? op->lev[0].a : nullptr, dt, src ? src[i] : nullptr,
+                                       op->ctx->stream));
+    rhs[i] = op->work[i].rhs_be;
+  }
+  return vc_solve_many(op, n, phi_new, rhs.data(), o, stats);
 }
 
 int ml_vc_darcy_ccoef(ml_vc_op op, const double* chi, double darcy_coeff, double* cCoef) {

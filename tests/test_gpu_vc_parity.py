@@ -145,6 +145,66 @@ def test_db_be_step_parity(ml, gpu_ctx, name):
     _check(n, dnew.get(), ref)
 
 
+@pytest.mark.parametrize("name", ["db_2d_64", "db_2d_512", "db_3d_16"])
+def test_db_be_many_matches_oracle_per_component(ml, gpu_ctx, name):
+    """Concurrent U* solves (one per velocity component, sharing the
+    operator): each is bit-identical to the oracle's backward-Euler solve of
+    its own component, with the same iterations and exit status; a second
+    call with the workspaces already created gives the same results."""
+    C, w, g, bc, op, keep = _setup(ml, gpu_ctx, name)
+    n = w["n"]
+    dt = w["dt"]
+    dim = len(n)
+    kw = dict(nu1=2, nu2=2, bottom=8, max_iter=10, eps=1e-10, hang=1e-10, norm_thresh=1e-10)
+    olds, srcs = [], []
+    for comp in range(dim):
+        u = _rand(g, 20 + comp) + (w["u0"] if comp == dim - 1 else 0.0)
+        olds.append(u)
+        srcs.append(w["src"] if comp == dim - 1 else _rand(g, 30 + comp))
+    refs, st_refs = [], []
+    for u, src in zip(olds, srcs):
+        rhs = np.zeros_like(u)
+        O.valid(g, rhs)[...] = O.valid(g, u) * 1.0 + dt * O.valid(g, src)
+        ref = u.copy()
+        st_refs.append(O.lin_mg_solve(g, 1, bc, 1.0, None, -dt, C["c"], C["b"], ref, rhs,
+                                      opts=O.default_opts(**kw)))
+        refs.append(ref)
+    dold = [ml.DeviceArray(gpu_ctx, u.shape, host=u) for u in olds]
+    dsrc = [ml.DeviceArray(gpu_ctx, x.shape, host=x) for x in srcs]
+    for _ in range(2):
+        dnew = [ml.DeviceArray(gpu_ctx, u.shape, host=u) for u in olds]
+        sts = op.solve_be_many(dnew, dold, dt, src=dsrc, opts=ml.default_opts(**kw))
+        for st, st_ref, d, ref in zip(sts, st_refs, dnew, refs):
+            assert (st.iterations, st.exit_status) == (st_ref.iterations, st_ref.exit_status)
+            _check(n, d.get(), ref)
+
+
+@pytest.mark.parametrize("name", ["db_2d_64", "proj_2d_48x32", "db_3d_16"])
+def test_graph_replays_match_direct_launches(ml, gpu_ctx, name):
+    """The solve replays two captured iterations as a HIP graph; three solves
+    in a row, with other eager work and synchronisations between them, each
+    give exactly the state of the solve launched kernel by kernel."""
+    C, w, g, bc, op, keep = _setup(ml, gpu_ctx, name)
+    rhs = _rand(g, 8)
+    if C["c"] is None:
+        O.valid(g, rhs)[...] -= O.valid(g, rhs).mean()
+    kw = dict(nu1=2, nu2=4, bottom=8, max_iter=9, eps=1e-12, hang=1e-12, norm_thresh=1e-14)
+    drhs = ml.DeviceArray(gpu_ctx, rhs.shape, host=rhs)
+    dref = ml.DeviceArray(gpu_ctx, rhs.shape, host=np.zeros_like(rhs))
+    st_ref = op.solve(dref, drhs, opts=ml.default_opts(flags=ml.SOLVE_NO_GRAPH, **kw))
+    ref = dref.get()
+    scratch = ml.DeviceArray(gpu_ctx, rhs.shape, host=rhs)
+    d = ml.DeviceArray(gpu_ctx, rhs.shape)
+    for rep in range(3):
+        d.set(np.zeros_like(rhs))   # same buffers every time: the captured graph is replayed
+        st = op.solve(d, drhs, opts=ml.default_opts(**kw))
+        assert (st.iterations, st.exit_status, st.final_residual) == \
+            (st_ref.iterations, st_ref.exit_status, st_ref.final_residual), rep
+        _check(w["n"], d.get(), ref)
+        op.relax(0, scratch, drhs, 1)   # eager work between the replays
+        gpu_ctx.sync()
+
+
 @pytest.mark.parametrize("n", [(64, 64), (24, 16, 8)])
 def test_darcy_and_projection_coefficients(ml, gpu_ctx, n):
     """cCoef = (chi/Pi) Pr/Da with Kozeny-Carman Pi, and the projection's
