@@ -288,7 +288,7 @@ def vc_relax_roofline(ml, ctx, stepper, steps):
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bytes_per_launch")
-    return {"kernel": "vc_relax_tile_kernel<double,2,32,16,1,8,512> (every launch of the step)", "bound": "hbm",
+    return {"kernel": "vc_relax_tile_kernel<double,2,16,16,1,9,1024> (every launch of the step)", "bound": "hbm",
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": round(per_launch, 1),

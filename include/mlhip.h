@@ -89,9 +89,10 @@ typedef struct {
  * ML_SOLVE_NO_LOOKAHEAD enqueues one iteration at a time (the host reads the
  * stopping rule after each) instead of batches that the device cuts short:
  * same results, no launches that return at once (for per-launch
- * measurements).  ML_SOLVE_NO_GRAPH launches every kernel from the host
- * instead of replaying the iterations captured as a HIP graph: same
- * results. */
+ * measurements).  ML_SOLVE_NO_GRAPH launches every kernel from the host, each
+ * step of an iteration as its own kernel, instead of replaying the
+ * iterations captured as a HIP graph with the residual fused into the last
+ * relaxation: same results. */
 typedef struct {
   int nu1, nu2, bottom, max_iter;
   double eps, hang, norm_thresh;

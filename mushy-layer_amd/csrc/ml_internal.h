@@ -171,6 +171,7 @@ struct VCWork {
   bool own_stream = false;
   hipEvent_t joined = nullptr;          // recorded on `stream` when a solve ends
   std::vector<double*> e0, e1, r;       // per level
+  double* r0alt = nullptr;              // level-0 residual partner (the fused residual writes it)
   double* alt0 = nullptr;
   double* rhs_be = nullptr;
   unsigned long long* slots = nullptr;

@@ -451,6 +451,19 @@ struct VCTileIO {
   int homog, ntx, nty;
 };
 
+// The residual of the full equation after the last relaxation of an
+// iteration, fused into that launch: phi_out = phi + e, res = rhs - L(phi_out)
+// (inhomogeneous BCs), its max-norm into slots[ctl->iter + 1] with the
+// stopping rule (phi == nullptr: none).
+template <typename T>
+struct VCFinal {
+  const T* phi;
+  T* phi_out;
+  const T* rhs;
+  T* res;
+  CtlFuse fuse;
+};
+
 template <typename T>
 struct VCLevel {
   Geom g;
@@ -476,6 +489,13 @@ struct VCDesc {
 template <typename T>
 hipError_t vc_relax(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
                     const Geom* gc, const T* pc, hipStream_t s, const LaunchTimer* tm = nullptr);
+// vc_relax followed in the same launch by the residual of the full equation
+// (VCFinal); false in *fused when the passes do not fit one launch (then
+// nothing was launched)
+template <typename T>
+hipError_t vc_relax_final(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, const Geom* gc,
+                          const T* pc, const VCFinal<T>& fin, bool* fused, hipStream_t s,
+                          const LaunchTimer* tm = nullptr);
 // relaxation followed by the restricted residual R(rhs - L e) (homogeneous
 // BCs) of the relaxed state into resc on level gr, fused in one launch when
 // the passes fit the tile halo

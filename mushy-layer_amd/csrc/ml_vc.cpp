@@ -38,6 +38,7 @@ void vc_work_free(ml::VCWork& W) {
   for (double* q : W.e0) f(q);
   for (double* q : W.e1) f(q);
   for (double* q : W.r) f(q);
+  f(W.r0alt);
   f(W.alt0);
   f(W.rhs_be);
   if (W.slots) hipFree(W.slots);
@@ -69,6 +70,7 @@ int vc_work_init(ml_vc_op op, ml::VCWork& W, bool own_stream) {
       if ((rc = vc_alloc(ctx, g.ncell, &W.e1[l]))) return rc;
   }
   const ml::Geom& g0 = op->lev[0].g;
+  if ((rc = vc_alloc(ctx, g0.ncell, &W.r0alt))) return rc;
   if ((rc = vc_alloc(ctx, g0.ncell, &W.alt0))) return rc;
   if ((rc = vc_alloc(ctx, g0.ncell, &W.rhs_be))) return rc;
   void* p = nullptr;
@@ -139,9 +141,12 @@ int vc_upload_desc(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o) {
   return ML_OK;
 }
 
-// one V-cycle of the correction equation; on return the level-0 correction
-// is W's level-0 buffer ecur[0]
-int vc_vcycle(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o, int* ecur, const int* skip) {
+// one V-cycle of the correction equation with level-0 right-hand side r0;
+// on return the level-0 correction is W's level-0 buffer ecur[0].  With fin,
+// the last launch on a tiled level 0 also forms phi + e and the full
+// residual (*fused = true).
+int vc_vcycle(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o, int* ecur, const int* skip, double* r0,
+              const ml::VCFinal<double>* fin = nullptr, bool* fused = nullptr) {
   ml_ctx ctx = op->ctx;
   hipStream_t s = W.stream;
   const int L = int(op->lev.size());
@@ -151,6 +156,7 @@ int vc_vcycle(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o, int* ecur, con
     b[1] = W.e1[l];
   };
   auto E = [&](int l) { return ecur[l] ? W.e1[l] : W.e0[l]; };
+  auto R = [&](int l) { return l == 0 ? r0 : W.r[l]; };
   auto args = [&](int l) {
     ml::VCArgs<double> A = op->args(l);
     A.skip = skip;
@@ -162,25 +168,31 @@ int vc_vcycle(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o, int* ecur, con
   double* B[2];
   if (lc >= L && L == 1) {
     buf(0, B);
-    ML_HIP(ctx, ml::vc_relax<double>(args(0), B, &ecur[0], W.r[0], 2 * o.bottom, true, nullptr, nullptr, s, tm));
+    ML_HIP(ctx, ml::vc_relax<double>(args(0), B, &ecur[0], R(0), 2 * o.bottom, true, nullptr, nullptr, s, tm));
     return ML_OK;
   }
   for (int l = 0; l < ltiles; ++l) {
     buf(l, B);
-    ML_HIP(ctx, ml::vc_relax_restrict<double>(args(l), B, &ecur[l], W.r[l], 2 * o.nu1, true, op->lev[l + 1].g,
+    ML_HIP(ctx, ml::vc_relax_restrict<double>(args(l), B, &ecur[l], R(l), 2 * o.nu1, true, op->lev[l + 1].g,
                                                W.r[l + 1], s, tm));
   }
   if (lc < L) {
     ML_HIP(ctx, ml::vc_coarse_vcycle<double>(W.desc, op->dim, op->coarse_lds, s));
   } else {
     buf(L - 1, B);
-    ML_HIP(ctx, ml::vc_relax<double>(args(L - 1), B, &ecur[L - 1], W.r[L - 1], 2 * o.bottom, true, nullptr, nullptr,
+    ML_HIP(ctx, ml::vc_relax<double>(args(L - 1), B, &ecur[L - 1], R(L - 1), 2 * o.bottom, true, nullptr, nullptr,
                                       s, tm));
   }
+  if (fused) *fused = false;
   for (int l = ltiles - 1; l >= 0; --l) {
     const ml::Geom gc = op->lev[l + 1].g;
     buf(l, B);
-    ML_HIP(ctx, ml::vc_relax<double>(args(l), B, &ecur[l], W.r[l], 2 * o.nu2, false, &gc, E(l + 1), s, tm));
+    if (l == 0 && fin) {
+      ML_HIP(ctx, ml::vc_relax_final<double>(args(l), B, &ecur[l], R(l), 2 * o.nu2, &gc, E(l + 1), *fin, fused, s,
+                                              tm));
+      if (*fused) continue;
+    }
+    ML_HIP(ctx, ml::vc_relax<double>(args(l), B, &ecur[l], R(l), 2 * o.nu2, false, &gc, E(l + 1), s, tm));
   }
   return ML_OK;
 }
@@ -200,25 +212,38 @@ struct VCSolve {
   double* phi = nullptr;
   const double* rhs = nullptr;
   std::vector<int> ecur;
+  double* r = nullptr;   // level-0 residual the next iteration corrects
   int launched = 0, batch = 1;
   bool lookahead = true, finished = false, graph = false;
   ml::SolveCtl c{};
 };
 
-// one iteration: V-cycle, then phi(out) = phi(in) + e and the residual
-// (its norm into the slot of the iteration it completes, the stopping rule
-// in the kernel's last workgroup)
+// one iteration: V-cycle on the correction equation with rhs *r (the
+// previous residual), then phi(out) = phi(in) + e and the new residual (its
+// norm into the slot of the iteration it completes, the stopping rule in the
+// kernel's last workgroup).  The new residual goes to the other level-0
+// buffer when it is formed inside the last relaxation launch (which still
+// reads *r in other workgroups); *r then points to it.
 int vc_iteration(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o, int* ecur, double* phi_in, double* phi_out,
-                 const double* rhs) {
+                 const double* rhs, double** r) {
   ml_ctx ctx = op->ctx;
   const int* skip = &W.ctl->done;
-  int rc = vc_vcycle(op, W, o, ecur, skip);
+  const ml::CtlFuse fuse{W.ctl, W.slots, W.arrive, 0};
+  // the residual, phi_out = phi_in + e and the stopping rule ride on the last
+  // relaxation launch when it fits, else run as their own kernel
+  double* rnext = *r == W.r[0] ? W.r0alt : W.r[0];
+  const ml::VCFinal<double> fin{phi_in, phi_out, rhs, rnext, fuse};
+  bool fused = false;
+  int rc = vc_vcycle(op, W, o, ecur, skip, *r, (o.flags & ML_SOLVE_NO_GRAPH) ? nullptr : &fin, &fused);
   if (rc) return rc;
+  if (fused) {
+    *r = rnext;
+    return ML_OK;
+  }
   ml::VCArgs<double> A0 = op->args(0);
   A0.skip = skip;
   double* e0 = ecur[0] ? W.e1[0] : W.e0[0];
-  ML_HIP(ctx, ml::vc_residual<double>(A0, phi_in, e0, phi_out, rhs, W.r[0], nullptr, 0, W.stream,
-                                       ml::CtlFuse{W.ctl, W.slots, W.arrive, 0}));
+  ML_HIP(ctx, ml::vc_residual<double>(A0, phi_in, e0, phi_out, rhs, *r, nullptr, 0, W.stream, fuse));
   return ML_OK;
 }
 
@@ -234,11 +259,12 @@ bool vc_capture(ml_vc_op op, ml::VCWork& W, VCSolve& S, const ml_solve_opts& o) 
   W.gexec = nullptr;
   if (hipStreamBeginCapture(W.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
   std::vector<int> ec(op->lev.size(), 0);
-  int rc = vc_iteration(op, W, o, ec.data(), S.phi, W.alt0, S.rhs);
-  if (!rc) rc = vc_iteration(op, W, o, ec.data(), W.alt0, S.phi, S.rhs);
+  double* r = W.r[0];
+  int rc = vc_iteration(op, W, o, ec.data(), S.phi, W.alt0, S.rhs, &r);
+  if (!rc) rc = vc_iteration(op, W, o, ec.data(), W.alt0, S.phi, S.rhs, &r);
   hipGraph_t graph = nullptr;
   const hipError_t ee = hipStreamEndCapture(W.stream, &graph);
-  bool ok = !rc && ee == hipSuccess && graph;
+  bool ok = !rc && ee == hipSuccess && graph && r == W.r[0];
   for (int v : ec) ok = ok && v == 0;
   if (ok) ok = hipGraphInstantiate(&W.gexec, graph, nullptr, nullptr, 0) == hipSuccess;
   if (graph) hipGraphDestroy(graph);
@@ -274,6 +300,7 @@ int vc_solve_begin(ml_vc_op op, VCSolve& S, const ml_solve_opts& o) {
   const ml::Geom& g0 = op->lev[0].g;
   ML_HIP(ctx, hipMemcpyAsync(W.alt0, S.phi, sizeof(double) * g0.ncell, hipMemcpyDeviceToDevice, s));
   S.ecur.assign(op->lev.size(), 0);
+  S.r = W.r[0];
   ML_HIP(ctx, ml::zero_u64(W.slots, o.max_iter + 1, s));
   ML_HIP(ctx, ml::vc_residual<double>(op->args(0), S.phi, nullptr, nullptr, S.rhs, W.r[0], W.slots, 0, s));
   ML_HIP(ctx, ml::ctl_init(W.ctl, W.slots, o.max_iter, o.eps, o.hang, o.norm_thresh, s));
@@ -303,7 +330,7 @@ int vc_solve_enqueue(ml_vc_op op, VCSolve& S, const ml_solve_opts& o) {
   for (int b = 0; b < S.batch && S.launched < o.max_iter; ++b) {
     ++S.launched;
     const int pcur = (S.launched - 1) & 1;   // phi += e alternates the state buffer
-    int rc = vc_iteration(op, W, o, S.ecur.data(), pb[pcur], pb[1 - pcur], S.rhs);
+    int rc = vc_iteration(op, W, o, S.ecur.data(), pb[pcur], pb[1 - pcur], S.rhs, &S.r);
     if (rc) return rc;
   }
   return ML_OK;

@@ -84,32 +84,36 @@ __global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T*
                                                            T* __restrict__ out, const T* __restrict__ rhs, int npass,
                                                            int first, int zero_init, Geom gc,
                                                            const T* __restrict__ pc, int ntx, int nty, Geom gr,
-                                                           T* __restrict__ resc) {
+                                                           T* __restrict__ resc, VCFinal<T> fin) {
   if (A.skip && *A.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2 * HMAX) * (TY + 2 * HMAX) * (DIM == 3 ? TZ + 2 * HMAX : 1);
   constexpr int KMAX = (EMAX + NT - 1) / NT;
   const Geom& g = A.g;
   Tile<DIM, TX, TY, TZ> t;
-  // with a restriction to produce, one more halo cell keeps the core's
-  // neighbours exact after the last pass
-  t.init(npass + (resc ? 1 : 0), ntx, nty);
+  // with a restriction or a residual to produce, one more halo cell keeps
+  // the core's neighbours exact after the last pass
+  const bool final = fin.phi != nullptr;
+  t.init(npass + ((resc || final) ? 1 : 0), ntx, nty);
   T* su = reinterpret_cast<T*>(smem_raw);
   const T dxinv = T(1) / (T(g.dx) * T(g.dx));
   PosRec R[KMAX];
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) pos_record<DIM, TX, TY, TZ>(g, t, threadIdx.x + kk * NT, R[kk]);
   Faces1<T, DIM> F[KMAX];
-  T aq[KMAX], cv[KMAX], lam[KMAX], rr[KMAX], v[KMAX], pv[KMAX];
+  T aq[KMAX], cv[KMAX], lam[KMAX], rr[KMAX], v[KMAX], pv[KMAX], fv[KMAX], frhs[KMAX];
   // every global load of the tile is issued before the first use
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) {
     const PosRec& r = R[kk];
     v[kk] = T(0);
     pv[kk] = T(0);
+    // phi near the core (for a wall ghost: of its interior neighbour)
+    if (final && r.kind != 0 && r.dist <= 1) fv[kk] = fin.phi[r.q];
     if (r.kind != 1) continue;
     if (!zero_init) v[kk] = in[r.q];
     if (pc) pv[kk] = pc[gc.ci(r.i / 2, r.j / 2, DIM == 3 ? r.k / 2 : 0)];
+    if (final && r.owner) frhs[kk] = fin.rhs[r.q];
   }
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) {
@@ -170,6 +174,44 @@ __global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T*
   for (int kk = 0; kk < KMAX; ++kk) {
     const PosRec& r = R[kk];
     if (r.owner) out[r.q] = su[r.e];
+  }
+  if (final) {
+    // u = phi + e near the core, its inhomogeneous wall ghosts, then the
+    // residual of the full equation at the owned cells (DBCOMPUTERES order,
+    // as vc_tile_kernel<kVCResid> with e)
+    T* sv = su + t.E;
+#pragma unroll
+    for (int kk = 0; kk < KMAX; ++kk) {
+      const PosRec& r = R[kk];
+      if (r.dist > 1) continue;
+      if (r.kind == 1) {
+        sv[r.e] = fv[kk] + su[r.e];
+      } else if (r.kind == 2) {
+        // the interior value u = phi + e of the ghost's neighbour
+        const T ui = fv[kk] + su[r.eint];
+        sv[r.e] = bc_ghost<T>(A.bc.type[r.gd][r.gs][0], T(A.bc.val[r.gd][r.gs][0]), ui, r.gs == 0 ? -1 : 1,
+                              T(g.dx));
+      }
+    }
+    __syncthreads();
+    T mx = T(0);
+    int nan = 0;
+#pragma unroll
+    for (int kk = 0; kk < KMAX; ++kk) {
+      const PosRec& r = R[kk];
+      if (!r.owner) continue;
+      fin.phi_out[r.q] = sv[r.e];
+      T st[7];
+      gather_lds<T, DIM>(sv, r.e, t.EX, t.EXY, st);
+      const T lof = vc_lof<T, DIM>(F[kk], st, aq[kk], A.c != nullptr, cv[kk], A.alpha, A.beta, dxinv);
+      const T res = frhs[kk] - lof;
+      fin.res[r.q] = res;
+      const T ar = fabs(res);
+      if (ar != ar) nan = 1;
+      else if (ar > mx) mx = ar;
+    }
+    block_max_atomic<T>(mx, nan, ctl_slot(nullptr, fin.fuse), fin.fuse);
+    return;
   }
   if (!resc) return;
   // restricted residual of the relaxed correction (RESTRICTRESDB,
@@ -600,7 +642,7 @@ template <int DIM>
 struct VCRelaxTile;
 template <>
 struct VCRelaxTile<2> {
-  static constexpr int TX = 16, TY = 16, TZ = 1, HMAX = 8, NT = 1024;   // measured best of 11 shapes at 256^2
+  static constexpr int TX = 16, TY = 16, TZ = 1, HMAX = 9, NT = 1024;   // measured best of 11 shapes at 256^2
 };
 template <>
 struct VCRelaxTile<3> {
@@ -628,21 +670,23 @@ int vc_tiles(const Geom& g, int& ntx, int& nty) {
 template <typename T, int DIM>
 hipError_t vc_relax_dim(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
                         const Geom* gc, const T* pc, hipStream_t s, const Geom* gr = nullptr, T* resc = nullptr,
-                        const LaunchTimer* tm = nullptr) {
+                        const LaunchTimer* tm = nullptr, const VCFinal<T>* fin = nullptr) {
   using D = VCRelaxTile<DIM>;
   int ntx, nty;
   const int nt = vc_tiles<D::TX, D::TY, D::TZ>(A.g, ntx, nty);
   const Geom gcc = gc ? *gc : A.g;
   const Geom grr = gr ? *gr : A.g;
-  if (resc && npass + 1 > D::HMAX) return hipErrorInvalidValue;
+  if ((resc || fin) && npass + 1 > D::HMAX) return hipErrorInvalidValue;
+  const VCFinal<T> none{};
   int done = 0;
   bool first_launch = true;
   while (done < npass || (first_launch && npass == 0 && (zero_init || pc || resc))) {
     first_launch = false;
     const int p = (npass - done) < D::HMAX ? (npass - done) : D::HMAX;
-    const int h = p + (resc ? 1 : 0);
+    const int h = p + ((resc || fin) ? 1 : 0);
     size_t lds = size_t(D::TX + 2 * h) * (D::TY + 2 * h) * (DIM == 3 ? D::TZ + 2 * h : 1);
     if (resc) lds += size_t(D::TX) * D::TY * D::TZ;
+    if (fin) lds *= 2;   // e and u = phi + e
     const bool timed = tm && *tm->n < tm->cap;
     if (tm && !timed) ++*tm->missed;
     const bool zi = done == 0 && zero_init;
@@ -651,7 +695,7 @@ hipError_t vc_relax_dim(const VCArgs<T>& A, T* const buf[2], int* cur, const T* 
     hipExtLaunchKernelGGL((vc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, D::HMAX, D::NT>), dim3(nt),
                           dim3(D::NT), lds * sizeof(T), s, timed ? tm->start[*tm->n] : nullptr,
                           timed ? tm->stop[*tm->n] : nullptr, 0, A, buf[*cur], buf[1 - *cur], rhs, p, done & 1,
-                          zi ? 1 : 0, gcc, pcl, ntx, nty, grr, resc);
+                          zi ? 1 : 0, gcc, pcl, ntx, nty, grr, resc, fin ? *fin : none);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (timed) {
@@ -663,6 +707,7 @@ hipError_t vc_relax_dim(const VCArgs<T>& A, T* const buf[2], int* cur, const T* 
       for (int d = 0; d < DIM; ++d) faces += double(A.g.nface[d]);
       double b = cells * (zi ? 0 : 1) + (pcl ? coarse : 0) + cells + (A.a ? cells : 0) + (A.c ? cells : 0) + faces;
       b += cells + (resc ? coarse : 0);
+      if (fin) b += 2 * cells + 2 * cells;   // phi and the full rhs in; phi + e and the residual out
       *tm->bytes += b * sizeof(T);
       ++*tm->n;
     }
@@ -702,6 +747,16 @@ hipError_t vc_relax_restrict(const VCArgs<T>& A, T* const buf[2], int* cur, cons
   hipError_t e = vc_relax<T>(A, buf, cur, rhs, npass, zero_init, nullptr, nullptr, s, tm);
   if (e != hipSuccess) return e;
   return vc_restrict_residual<T>(A, gr, buf[*cur], rhs, resc, s);
+}
+
+template <typename T>
+hipError_t vc_relax_final(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, const Geom* gc,
+                          const T* pc, const VCFinal<T>& fin, bool* fused, hipStream_t s, const LaunchTimer* tm) {
+  const int hmax = A.g.dim == 2 ? VCRelaxTile<2>::HMAX : VCRelaxTile<3>::HMAX;
+  *fused = npass >= 1 && npass + 1 <= hmax && fin.phi && fin.phi_out != fin.phi;
+  if (!*fused) return hipSuccess;
+  return A.g.dim == 2 ? vc_relax_dim<T, 2>(A, buf, cur, rhs, npass, false, gc, pc, s, nullptr, nullptr, tm, &fin)
+                      : vc_relax_dim<T, 3>(A, buf, cur, rhs, npass, false, gc, pc, s, nullptr, nullptr, tm, &fin);
 }
 
 template <typename T>
@@ -764,6 +819,9 @@ template hipError_t vc_relax<double>(const VCArgs<double>&, double* const[2], in
                                      const Geom*, const double*, hipStream_t, const LaunchTimer*);
 template hipError_t vc_relax_restrict<double>(const VCArgs<double>&, double* const[2], int*, const double*, int, bool,
                                               const Geom&, double*, hipStream_t, const LaunchTimer*);
+template hipError_t vc_relax_final<double>(const VCArgs<double>&, double* const[2], int*, const double*, int,
+                                           const Geom*, const double*, const VCFinal<double>&, bool*, hipStream_t,
+                                           const LaunchTimer*);
 template hipError_t vc_residual<double>(const VCArgs<double>&, const double*, const double*, double*,
                                         const double*, double*, unsigned long long*, int, hipStream_t,
                                         const CtlFuse&);
