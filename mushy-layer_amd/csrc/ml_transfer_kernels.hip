@@ -195,7 +195,8 @@ __global__ __launch_bounds__(256) void coarsen_hierarchy_kernel(Geom gf, int nco
 // adds the row sums in (j, k) order.
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void coarsen_cells_block_kernel(Geom gf, int ncomp, CoarsenJobs<T> J,
-                                                                  const T* __restrict__ af) {
+                                                                  const T* __restrict__ af, const T* bf0,
+                                                                  const T* bf1, const T* bf2) {
   constexpr int kRows = 2048;
   __shared__ T rows[kRows];
   const long long t = blockIdx.x;
@@ -205,6 +206,35 @@ __global__ __launch_bounds__(256) void coarsen_cells_block_kernel(Geom gf, int n
   const long long item = t - J.begin[jb];
   const int ratio = 1 << J.level[jb];
   const Geom gc = coarsen_geom(gf, ratio);
+  if (J.kind[jb] < 3) {
+    // one coarse face: its fine sub-faces loaded in parallel, summed in the
+    // light kernel's order (lowest transverse index fastest) by one thread
+    const int d = J.kind[jb];
+    const T* bf = d == 0 ? bf0 : (d == 1 ? bf1 : bf2);
+    int r[3] = {ratio, ratio, DIM == 3 ? ratio : 1};
+    r[d] = 1;
+    const T scale = T(1) / T(r[0] * r[1] * r[2]);
+    const long long NF = gc.nface[d];
+    const int n = int(item / NF);
+    int I, Jj, K;
+    unflatten(item % NF, int(gc.fn[d][0]), int(gc.fn[d][1]), I, Jj, K);
+    const int base[3] = {I * (d == 0 ? ratio : r[0]), Jj * (d == 1 ? ratio : r[1]), K * (d == 2 ? ratio : r[2])};
+    const int count = r[0] * r[1] * r[2];
+    T sum = T(0);
+    for (int u0 = 0; u0 < count; u0 += kRows) {
+      const int nu = (count - u0) < kRows ? (count - u0) : kRows;
+      for (int u = threadIdx.x; u < nu; u += blockDim.x) {
+        const int w = u0 + u, ii = w % r[0], rest = w / r[0], jj = rest % r[1], kk = rest / r[1];
+        rows[u] = bf[gf.fi(d, base[0] + ii, base[1] + jj, base[2] + kk) + n * gf.nface[d]];
+      }
+      __syncthreads();
+      if (threadIdx.x == 0)
+        for (int u = 0; u < nu; ++u) sum = sum + rows[u];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) J.dst[jb][gc.fi(d, I, Jj, K) + n * gc.nface[d]] = sum * scale;
+    return;
+  }
   const int rz = DIM == 3 ? ratio : 1;
   const T scale = T(1) / T(ratio * ratio * rz);
   const long long N = gc.nvalid();
@@ -391,12 +421,23 @@ hipError_t coarsen_hierarchy(const Geom& gf, int ncomp, int nlev, const T* const
                              T* const* ac, hipStream_t s, bool faces) {
   CoarsenJobs<T> J{}, H{};
   long long total = 0, htotal = 0;
-  // cell averages over blocks of at least kHeavy fine cells go to the
-  // workgroup-per-cell kernel
+  // averages over at least kHeavy fine cells or faces go to the
+  // workgroup-per-item kernel
   constexpr long long kHeavy = 64;
   for (int l = nlev - 1; l >= 1; --l) {
     const Geom gc = coarsen_geom(gf, 1 << l);
     const long long block = (1LL << l) * (1LL << l) * (gf.dim == 3 ? (1LL << l) : 1);
+    const long long fblock = block >> l;   // fine faces per coarse face
+    if (faces && fblock >= kHeavy)
+      for (int d = 0; d < gf.dim; ++d) {
+        if (H.njobs >= kMaxCoarsenJobs) return hipErrorInvalidValue;
+        H.begin[H.njobs] = htotal;
+        H.level[H.njobs] = short(l);
+        H.kind[H.njobs] = short(d);
+        H.dst[H.njobs] = bc[l][d];
+        htotal += gc.nface[d] * ncomp;
+        ++H.njobs;
+      }
     if (af && block >= kHeavy) {
       if (H.njobs >= kMaxCoarsenJobs) return hipErrorInvalidValue;
       H.begin[H.njobs] = htotal;
@@ -407,7 +448,7 @@ hipError_t coarsen_hierarchy(const Geom& gf, int ncomp, int nlev, const T* const
       ++H.njobs;
     }
     for (int kind = 0; kind < 4; ++kind) {
-      if (kind < 3 && (!faces || kind >= gf.dim)) continue;
+      if (kind < 3 && (!faces || kind >= gf.dim || fblock >= kHeavy)) continue;
       if (kind == 3 && (!af || block >= kHeavy)) continue;
       if (J.njobs >= kMaxCoarsenJobs) return hipErrorInvalidValue;
       J.begin[J.njobs] = total;
@@ -422,9 +463,11 @@ hipError_t coarsen_hierarchy(const Geom& gf, int ncomp, int nlev, const T* const
   H.begin[H.njobs] = htotal;
   if (htotal > 0) {
     if (gf.dim == 2)
-      hipLaunchKernelGGL((coarsen_cells_block_kernel<T, 2>), dim3(unsigned(htotal)), dim3(256), 0, s, gf, ncomp, H, af);
+      hipLaunchKernelGGL((coarsen_cells_block_kernel<T, 2>), dim3(unsigned(htotal)), dim3(256), 0, s, gf, ncomp, H, af,
+                         bf[0], bf[1], bf[2]);
     else
-      hipLaunchKernelGGL((coarsen_cells_block_kernel<T, 3>), dim3(unsigned(htotal)), dim3(256), 0, s, gf, ncomp, H, af);
+      hipLaunchKernelGGL((coarsen_cells_block_kernel<T, 3>), dim3(unsigned(htotal)), dim3(256), 0, s, gf, ncomp, H, af,
+                         bf[0], bf[1], bf[2]);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
