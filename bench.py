@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--max-iter", type=int, default=100)
     ap.add_argument("--form", choices=["both", "fp64", "fp32r"], default="both")
     ap.add_argument("--hc-only", action="store_true", help="time the (H, Theta) solve alone")
+    ap.add_argument("--large-n", type=int, default=2048,
+                    help="grid of the smoother bandwidth measurement (0: skip)")
     ap.add_argument("--no-lookahead", action="store_true",
                     help="solves enqueue one iteration at a time (no launches cut short; for counter runs)")
     return ap.parse_args()
@@ -185,6 +187,58 @@ def kernel_roofline(ml, ctx, stepper, reps=200):
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
             "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": bytes_per_cell * cells}
+
+
+def smoother_bandwidth(ml, wl, ctx, n, launches=20):
+    """HBM bandwidth of the FP64 smoothers on an n x n level, where the
+    working set is far larger than the caches: the (H, Theta) relaxation with
+    the closure (one red+black sweep per launch, as the solver runs it at
+    this size) and the variable-coefficient relaxation (Darcy-Brinkman
+    coefficients, up to 8 passes per launch).  Every launch carries HIP
+    events; achieved = algorithmic bytes / kernel time."""
+    out = {"n": [n, n]}
+    case = wl.mushy(n=(n, n))
+    st = HCStepper(ml, ctx, case, ml.default_opts())
+    st.op.porosity(st.phi, st.chi)
+    st.op.assemble_bcoef(st.chi, st.b)
+    st.op.set_coefs(None, st.b, 1.0, -st.dt)
+    ctx.check(ctx.lib.ml_buf_copy(ctx.h, st.old.ptr, st.phi.ptr, st.phi.nbytes), "copy")
+    st.op.relax(0, st.phi, st.old, 1)
+    ctx.timing_start(ml.KT_HC_RELAX, launches)
+    for _ in range(launches):
+        st.op.relax(0, st.phi, st.old, 1)
+    n1, ms, nb, _ = ctx.timing_stop()
+    gbs = nb / (ms * 1e-3) / 1e9
+    out["hc_relax"] = {"kernel": "hc_relax_tile_kernel<double,2,32,8,1,2,256> (1 sweep per launch)",
+                       "avg_launch_us": round(ms / n1 * 1e3, 2), "bytes_per_launch": nb / n1,
+                       "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    st.op.close()
+    db = wl.darcy_brinkman(n=(n, n), Pr=10.0, Da=1e-3, Ra_C=200.0)
+    op = ml.VCOp(ctx, [n, n], case["dx"], case["periodic"], ml.make_bc(db["bc_u"]))
+    shp1 = (1,) + case["phi"].shape[1:]
+    bu = [ml.DeviceArray(ctx, x.shape, host=x) for x in db["b_u"]]
+    c = ml.DeviceArray(ctx, shp1, host=db["c"])
+    op.set_coefs(None, bu, c, 1.0, -db["dt"])
+    u = ml.DeviceArray(ctx, shp1, host=db["u0"])
+    r = ml.DeviceArray(ctx, shp1, host=db["src"])
+    # levels above 512^2 cells run the wide short-halo tiles (2 sweeps per
+    # launch at most), smaller ones the 16x16 deep-halo tiles
+    wide = n * n > 512 * 512
+    tile, hmax = ("64,8,1,2,256", 2) if wide else ("16,16,1,9,1024", 9)
+    for sweeps in (1, 4):
+        op.relax(0, u, r, sweeps)
+        ctx.timing_start(ml.KT_VC_RELAX, launches)
+        for _ in range(launches):
+            op.relax(0, u, r, sweeps)
+        n1, ms, nb, _ = ctx.timing_stop()
+        gbs = nb / (ms * 1e-3) / 1e9
+        out["vc_relax_%d_sweep" % sweeps] = {
+            "kernel": "vc_relax_tile_kernel<double,2,%s> (%d red+black sweep%s per launch)"
+                      % (tile, min(sweeps, hmax), "" if min(sweeps, hmax) == 1 else "s"),
+            "avg_launch_us": round(ms / n1 * 1e3, 2), "bytes_per_launch": nb / n1,
+            "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    op.close()
+    return out
 
 
 def calibration_copy(ml, ctx, mib=64):
@@ -324,6 +378,7 @@ def main():
         roof = kernel_roofline(ml, ctx, res[todo[0]]["stepper"])
     else:
         roof = vc_relax_roofline(ml, ctx, h["stepper"], args.steps)
+    large = smoother_bandwidth(ml, wl, ctx, args.large_n) if args.large_n > 0 else None
     calibration_copy(ml, ctx)
     cpu = None if args.no_cpu else cpu_baseline(case, args.cpu_seconds, args.max_iter)
 
@@ -340,6 +395,8 @@ def main():
         cfg["ustar_iterations_per_solve"] = round(float(np.mean([x[1] for v in vs for x in v[:-1]])), 2)
         cfg["projection_iterations"] = round(float(np.mean([v[-1][1] for v in vs])), 2)
         cfg["ustar_projection_status"] = sorted(set(x[0] for v in vs for x in v))
+    if large:
+        cfg["smoother_hbm"] = large
     if "fp64" in res and "fp32r" in res:
         cfg["fp64_value"] = round(res["fp64"]["value"], 2)
         cfg["fp64_vcycles_per_step"] = round(float(np.mean(res["fp64"]["iters"])), 2)

@@ -128,7 +128,10 @@ int ml_ctx_event_elapsed(ml_ctx ctx, int slot_a, int slot_b, float* ms);
  * written once) are summed.  ml_ctx_timing_stop synchronises the stream,
  * returns the timed launches, their summed duration in ms and bytes, and the
  * launches beyond max_launches, and ends the timing. */
-enum { ML_KT_VC_RELAX = 1 /* tiled red/black relaxation of the linear VC operator */ };
+enum {
+  ML_KT_VC_RELAX = 1, /* tiled red/black relaxation of the linear VC operator */
+  ML_KT_HC_RELAX = 2  /* tiled red/black relaxation of the (H, Theta) operator with the closure */
+};
 int ml_ctx_timing_start(ml_ctx ctx, int kernel, int max_launches);
 int ml_ctx_timing_stop(ml_ctx ctx, long long* launches, double* total_ms, double* bytes, long long* missed);
 int ml_buf_alloc(ml_ctx ctx, size_t bytes, void** dptr);

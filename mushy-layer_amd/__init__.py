@@ -45,6 +45,7 @@ SOLVE_NO_LOOKAHEAD = 2   # one iteration enqueued at a time (per-launch measurem
 SOLVE_NO_GRAPH = 4   # launch every kernel from the host (no captured iterations)
 PRECISION_FP64, PRECISION_FP32_REFINED = 0, 1
 KT_VC_RELAX = 1   # ml_ctx_timing_start kernel family: tiled relaxation of the linear VC operator
+KT_HC_RELAX = 2   # ... of the (H, Theta) operator with the closure
 
 
 class SolveStats(C.Structure):

@@ -123,7 +123,7 @@ int ml_ctx_event_elapsed(ml_ctx ctx, int a, int b, float* ms) {
 }
 
 int ml_ctx_timing_start(ml_ctx ctx, int kernel, int max_launches) {
-  if (!ctx || kernel != ML_KT_VC_RELAX || max_launches <= 0) return ML_EINVAL;
+  if (!ctx || (kernel != ML_KT_VC_RELAX && kernel != ML_KT_HC_RELAX) || max_launches <= 0) return ML_EINVAL;
   while (int(ctx->timing_start.size()) < max_launches) {
     hipEvent_t a, b;
     ML_HIP(ctx, hipEventCreate(&a));
@@ -428,7 +428,9 @@ int ml_hc_relax(ml_hc_op op, int depth, double* phi, const double* rhs, int iter
   ML_HIP(op->ctx, hipMemcpyAsync(L.tmp, phi, bytes, hipMemcpyDeviceToDevice, op->ctx->stream));
   double* const buf[2] = {phi, L.tmp};
   int cur = 0;
-  ML_HIP(op->ctx, ml::hc_relax<double>(op->args(depth), buf, &cur, rhs, 2 * iters, 0, op->ctx->stream));
+  const ml::LaunchTimer T = op->ctx->timer(ML_KT_HC_RELAX);
+  ML_HIP(op->ctx, ml::hc_relax<double>(op->args(depth), buf, &cur, rhs, 2 * iters, 0, op->ctx->stream,
+                                       T.start ? &T : nullptr));
   if (cur) ML_HIP(op->ctx, hipMemcpyAsync(phi, L.tmp, bytes, hipMemcpyDeviceToDevice, op->ctx->stream));
   return ML_OK;
 }

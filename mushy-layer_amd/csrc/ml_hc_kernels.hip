@@ -8,6 +8,8 @@
 // Arithmetic follows the reference's evaluation order with contraction off
 // (-ffp-contract=off), so the FP64 instantiation reproduces the CPU oracle
 // bit for bit.
+#include <hip/hip_ext.h>
+
 #include "ml_common.h"
 #include "ml_kernels.h"
 #include "ml_tile.h"
@@ -704,31 +706,60 @@ static int relax_passes_per_launch(const Geom& g) {
   return (DIM == 2 && nt <= kTilesForDeepHalo) ? 8 : 2;
 }
 
+// tiles of the one-sweep 2D relaxation launch on large levels (measured
+// best of 5 shapes at 2048^2)
+struct HCRelaxWide2 {
+  static constexpr int TX = 32, TY = 8, TZ = 1, NT = 256;
+};
+
 template <typename T, int DIM>
-static hipError_t launch_relax_tile(const HCArgs<T>& A, TileIO<T> io, hipStream_t s) {
+static hipError_t launch_relax_tile(const HCArgs<T>& A, TileIO<T> io, hipStream_t s, const LaunchTimer* tm) {
   using D = TileDims<T, DIM>;
-  const int nt = tiles_of<T, DIM>(A.g, io.ntx, io.nty);
+  using DW = HCRelaxWide2;
+  int nt = tiles_of<T, DIM>(A.g, io.ntx, io.nty);
   const int pmax = relax_passes_per_launch<T, DIM>(A.g);
   if (io.npass < 1 || io.npass > pmax) return hipErrorInvalidValue;
-  const size_t lds = tile_lds<T, DIM>(io.npass, false);
+  size_t lds = tile_lds<T, DIM>(io.npass, false);
+  if (DIM == 2 && pmax == 2) {
+    io.ntx = (A.g.n[0] + DW::TX - 1) / DW::TX;
+    io.nty = (A.g.n[1] + DW::TY - 1) / DW::TY;
+    nt = io.ntx * io.nty;
+    lds = 4 * size_t(DW::TX + 2 * io.npass) * (DW::TY + 2 * io.npass) * sizeof(T);
+  }
   if (lds > kMaxLds) return hipErrorInvalidValue;
+  const bool timed = tm && *tm->n < tm->cap;
+  if (tm && !timed) ++*tm->missed;
+  hipEvent_t e0 = timed ? tm->start[*tm->n] : nullptr, e1 = timed ? tm->stop[*tm->n] : nullptr;
   if constexpr (DIM == 2) {
     if (pmax == 8)
-      hipLaunchKernelGGL((hc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, 8, 512>), dim3(nt), dim3(512), lds, s,
-                         A, io);
+      hipExtLaunchKernelGGL((hc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, 8, 512>), dim3(nt), dim3(512), lds,
+                            s, e0, e1, 0, A, io);
     else
-      hipLaunchKernelGGL((hc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, 2, 256>), dim3(nt), dim3(256), lds, s,
-                         A, io);
+      hipExtLaunchKernelGGL((hc_relax_tile_kernel<T, DIM, DW::TX, DW::TY, DW::TZ, 2, DW::NT>), dim3(nt), dim3(DW::NT),
+                            lds, s, e0, e1, 0, A, io);
   } else {
-    hipLaunchKernelGGL((hc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, 2, 512>), dim3(nt), dim3(512), lds, s,
-                       A, io);
+    hipExtLaunchKernelGGL((hc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, 2, 512>), dim3(nt), dim3(512), lds, s,
+                          e0, e1, 0, A, io);
   }
-  return hipGetLastError();
+  const hipError_t err = hipGetLastError();
+  if (err == hipSuccess && timed) {
+    // algorithmic bytes: state (2 comps) in and out, rhs, a, faces (2 comps)
+    // once each; the coarse state (and its initial copy) when prolonged on
+    // load
+    const double cells = double(A.g.nvalid());
+    double faces = 0;
+    for (int d = 0; d < DIM; ++d) faces += double(A.g.nface[d]);
+    double b = 2 * cells + 2 * cells + (A.a ? 2 * cells : 0) + 2 * faces + 2 * cells;
+    if (io.pc) b += 2 * io.gc.nvalid() * (io.pc0 ? 2 : 1);
+    *tm->bytes += b * sizeof(T);
+    ++*tm->n;
+  }
+  return err;
 }
 
 template <typename T>
 hipError_t hc_relax_tiles(const HCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, int first,
-                          const Geom* gc, const T* pc, const T* pc0, hipStream_t s) {
+                          const Geom* gc, const T* pc, const T* pc0, hipStream_t s, const LaunchTimer* tm) {
   const int pmax = A.g.dim == 2 ? relax_passes_per_launch<T, 2>(A.g) : relax_passes_per_launch<T, 3>(A.g);
   int done = 0;
   while (done < npass) {
@@ -743,7 +774,7 @@ hipError_t hc_relax_tiles(const HCArgs<T>& A, T* const buf[2], int* cur, const T
       io.pc = pc;
       io.pc0 = pc0;
     }
-    hipError_t e = A.g.dim == 2 ? launch_relax_tile<T, 2>(A, io, s) : launch_relax_tile<T, 3>(A, io, s);
+    hipError_t e = A.g.dim == 2 ? launch_relax_tile<T, 2>(A, io, s, tm) : launch_relax_tile<T, 3>(A, io, s, tm);
     if (e != hipSuccess) return e;
     *cur = 1 - *cur;
     done += io.npass;
@@ -757,9 +788,9 @@ hipError_t hc_relax_tiles(const HCArgs<T>& A, T* const buf[2], int* cur, const T
 
 template <typename T>
 hipError_t hc_relax(const HCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, int first,
-                    hipStream_t s) {
+                    hipStream_t s, const LaunchTimer* tm) {
   if (npass <= 0) return hipSuccess;
-  return hc_relax_tiles<T>(A, buf, cur, rhs, npass, first, nullptr, nullptr, nullptr, s);
+  return hc_relax_tiles<T>(A, buf, cur, rhs, npass, first, nullptr, nullptr, nullptr, s, tm);
 }
 
 template <typename T>
@@ -828,9 +859,10 @@ hipError_t hc_diffused_var(const HCArgs<T>& A, const T* phi, T* out, int homog, 
 }
 
 template hipError_t hc_relax<double>(const HCArgs<double>&, double* const[2], int*, const double*, int, int,
-                                     hipStream_t);
+                                     hipStream_t, const LaunchTimer*);
 template hipError_t hc_relax_tiles<double>(const HCArgs<double>&, double* const[2], int*, const double*, int, int,
-                                           const Geom*, const double*, const double*, hipStream_t);
+                                           const Geom*, const double*, const double*, hipStream_t,
+                                           const LaunchTimer*);
 template hipError_t hc_coarse_vcycle<double>(const CoarseDesc<double>*, int, int, hipStream_t);
 template hipError_t hc_residual<double>(const HCArgs<double>&, double*, const double*, double*,
                                         unsigned long long*, int, hipStream_t, const CtlFuse&);

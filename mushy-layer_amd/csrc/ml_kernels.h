@@ -354,12 +354,12 @@ hipError_t cast_many(const CastJobs& J, hipStream_t s);
 // store), then flips *cur.
 template <typename T>
 hipError_t hc_relax(const HCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, int first,
-                    hipStream_t s);
+                    hipStream_t s, const LaunchTimer* tm = nullptr);
 // relax tiles with the linear prolongation of pc - pc0 (level gc) fused on
 // load; npass == 0 prolongs in place into buf[*cur]
 template <typename T>
 hipError_t hc_relax_tiles(const HCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, int first,
-                          const Geom* gc, const T* pc, const T* pc0, hipStream_t s);
+                          const Geom* gc, const T* pc, const T* pc0, hipStream_t s, const LaunchTimer* tm = nullptr);
 template <typename T>
 hipError_t hc_coarse_vcycle(const CoarseDesc<T>* dev_desc, int dim, int apply_first, hipStream_t s);
 template <typename T>

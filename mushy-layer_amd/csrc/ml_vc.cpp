@@ -551,8 +551,9 @@ int ml_vc_relax(ml_vc_op op, int depth, double* phi, const double* rhs, int iter
   ML_HIP(ctx, hipMemcpyAsync(L.tmp, phi, bytes, hipMemcpyDeviceToDevice, ctx->stream));
   double* const buf[2] = {phi, L.tmp};
   int cur = 0;
+  const ml::LaunchTimer T = ctx->timer(ML_KT_VC_RELAX);
   ML_HIP(ctx, ml::vc_relax<double>(op->args(depth), buf, &cur, rhs, 2 * iters, false, nullptr, nullptr,
-                                   ctx->stream));
+                                   ctx->stream, T.start ? &T : nullptr));
   if (cur) ML_HIP(ctx, hipMemcpyAsync(phi, L.tmp, bytes, hipMemcpyDeviceToDevice, ctx->stream));
   return ML_OK;
 }

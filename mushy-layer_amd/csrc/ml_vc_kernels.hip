@@ -667,11 +667,10 @@ int vc_tiles(const Geom& g, int& ntx, int& nty) {
   return ntx * nty * ntz;
 }
 
-template <typename T, int DIM>
-hipError_t vc_relax_dim(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
-                        const Geom* gc, const T* pc, hipStream_t s, const Geom* gr = nullptr, T* resc = nullptr,
-                        const LaunchTimer* tm = nullptr, const VCFinal<T>* fin = nullptr) {
-  using D = VCRelaxTile<DIM>;
+template <typename T, int DIM, typename D>
+hipError_t vc_relax_cfg(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
+                        const Geom* gc, const T* pc, hipStream_t s, const Geom* gr, T* resc, const LaunchTimer* tm,
+                        const VCFinal<T>* fin) {
   int ntx, nty;
   const int nt = vc_tiles<D::TX, D::TY, D::TZ>(A.g, ntx, nty);
   const Geom gcc = gc ? *gc : A.g;
@@ -717,6 +716,31 @@ hipError_t vc_relax_dim(const VCArgs<T>& A, T* const buf[2], int* cur, const T* 
   return hipSuccess;
 }
 
+// tiles of the 2D relaxation on large levels (short halo, wide rows;
+// measured best of 6 shapes at 2048^2)
+struct VCRelaxWide2 {
+  static constexpr int TX = 64, TY = 8, TZ = 1, HMAX = 2, NT = 256;
+};
+constexpr long long kWideCells = 512 * 512;   // levels above this use the wide tiles
+
+inline bool vc_wide(const Geom& g) { return g.dim == 2 && g.nvalid() > kWideCells; }
+
+// passes one relaxation launch can do on level g
+inline int vc_hmax(const Geom& g) {
+  return vc_wide(g) ? VCRelaxWide2::HMAX : (g.dim == 2 ? VCRelaxTile<2>::HMAX : VCRelaxTile<3>::HMAX);
+}
+
+template <typename T, int DIM>
+hipError_t vc_relax_dim(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
+                        const Geom* gc, const T* pc, hipStream_t s, const Geom* gr = nullptr, T* resc = nullptr,
+                        const LaunchTimer* tm = nullptr, const VCFinal<T>* fin = nullptr) {
+  if constexpr (DIM == 2) {
+    if (vc_wide(A.g))
+      return vc_relax_cfg<T, 2, VCRelaxWide2>(A, buf, cur, rhs, npass, zero_init, gc, pc, s, gr, resc, tm, fin);
+  }
+  return vc_relax_cfg<T, DIM, VCRelaxTile<DIM>>(A, buf, cur, rhs, npass, zero_init, gc, pc, s, gr, resc, tm, fin);
+}
+
 template <typename T, int DIM, int MODE>
 hipError_t vc_tile_dim(const VCArgs<T>& A, VCTileIO<T> io, hipStream_t s) {
   using D = VCTile<DIM>;
@@ -740,7 +764,7 @@ hipError_t vc_relax(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs,
 template <typename T>
 hipError_t vc_relax_restrict(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
                              const Geom& gr, T* resc, hipStream_t s, const LaunchTimer* tm) {
-  const int hmax = A.g.dim == 2 ? VCRelaxTile<2>::HMAX : VCRelaxTile<3>::HMAX;
+  const int hmax = vc_hmax(A.g);
   if (npass + 1 <= hmax)
     return A.g.dim == 2 ? vc_relax_dim<T, 2>(A, buf, cur, rhs, npass, zero_init, nullptr, nullptr, s, &gr, resc, tm)
                         : vc_relax_dim<T, 3>(A, buf, cur, rhs, npass, zero_init, nullptr, nullptr, s, &gr, resc, tm);
@@ -752,7 +776,7 @@ hipError_t vc_relax_restrict(const VCArgs<T>& A, T* const buf[2], int* cur, cons
 template <typename T>
 hipError_t vc_relax_final(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, const Geom* gc,
                           const T* pc, const VCFinal<T>& fin, bool* fused, hipStream_t s, const LaunchTimer* tm) {
-  const int hmax = A.g.dim == 2 ? VCRelaxTile<2>::HMAX : VCRelaxTile<3>::HMAX;
+  const int hmax = vc_hmax(A.g);
   *fused = npass >= 1 && npass + 1 <= hmax && fin.phi && fin.phi_out != fin.phi;
   if (!*fused) return hipSuccess;
   return A.g.dim == 2 ? vc_relax_dim<T, 2>(A, buf, cur, rhs, npass, false, gc, pc, s, nullptr, nullptr, tm, &fin)

@@ -187,6 +187,8 @@ def test_be_step_parity(ml, gpu_ctx, wl, name, max_iter, driver):
 LARGE = {
     "2d_512": dict(n=(512, 512)),
     "2d_1024x256": dict(n=(1024, 256), periodic=[0, 0]),
+    # more than 1024 tiles: one sweep per launch on the wide tiles
+    "2d_1024x512": dict(n=(1024, 512)),
     "3d_64": dict(n=(64, 64, 64)),
 }
 

@@ -34,8 +34,12 @@ CASES = {
     "db_3d_16": ((16, 16, 16), "db"),
     "proj_3d_24x16x8": ((24, 16, 8), "proj"),
     "db_3d_64": ((64, 64, 64), "db"),
+    # above 512^2 cells: the wide tiles (short halo) of large levels
+    "db_2d_1024x512": ((1024, 512), "db"),
+    "proj_2d_768x512": ((768, 512), "proj"),
 }
-SMALL = [k for k in CASES if k not in ("db_2d_512", "db_3d_64")]
+SMALL = [k for k in CASES if k not in ("db_2d_512", "db_3d_64", "db_2d_1024x512", "proj_2d_768x512")]
+WIDE = ["db_2d_1024x512", "proj_2d_768x512"]
 
 
 def _setup(ml, ctx, name):
