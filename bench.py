@@ -46,7 +46,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--dim", type=int, choices=[2, 3], default=2,
+                    help="3: the 128^3 Darcy-Brinkman mushy layer (correlation length 0.08, Ra_C 300, Da 5e-4)")
+    ap.add_argument("--n", type=int, default=None, help="cells per direction (default 256 in 2D, 128 in 3D)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-iter", type=int, default=100)
@@ -183,7 +185,7 @@ def kernel_roofline(ml, ctx, stepper, reps=200):
     ms = ctx.elapsed_ms(0, 1) / reps
     phi.free()
     achieved = bytes_per_cell * cells / (ms * 1e-3) / 1e9
-    return {"kernel": "hc_tile_kernel<double,2,...,kRelax> (1 sweep, finest level)",
+    return {"kernel": "hc_relax_tile_kernel<double,%d,...> (1 sweep, finest level)" % dim,
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
             "avg_launch_us": round(ms * 1e3, 3), "bytes_per_launch": bytes_per_cell * cells}
@@ -297,8 +299,8 @@ def cpu_baseline(case, seconds, max_iter):
             break
     dt = time.perf_counter() - t0
     return {"value": round(cells * steps / dt, 2), "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": "%d implicit step(s) of the same %dx%d workload from its initial state, C oracle "
-                      "(gcc -O2 -ffp-contract=off), 1 thread, %.1f s" % (steps, case["n"][0], case["n"][1], dt)}
+            "sample": "%d implicit step(s) of the same %s workload from its initial state, C oracle "
+                      "(gcc -O2 -ffp-contract=off), 1 thread, %.1f s" % (steps, "x".join(map(str, case["n"])), dt)}
 
 
 def run_form(ml, ctx, case, precision, steps, warmup, max_iter, db=None, no_lookahead=False):
@@ -322,7 +324,9 @@ def timed_steps(ctx, st, steps):
     return time.perf_counter() - t0
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vc_relax_tile_256_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vc_relax_tile_256_traffic.json")   # 256x256 workload
+# the tiled relaxation of the finest level of the two workloads
+VC_TILE = {2: "vc_relax_tile_kernel<double,2,16,16,1,9,1024>", 3: "vc_relax_tile_kernel<double,3,16,8,4,2,512>"}
 
 
 def vc_relax_roofline(ml, ctx, stepper, steps):
@@ -331,7 +335,7 @@ def vc_relax_roofline(ml, ctx, stepper, steps):
     time and algorithmic bytes per launch.  The solves enqueue one iteration
     at a time in this pass, so every timed launch does its work."""
     stepper.set_flags(ml.SOLVE_NO_LOOKAHEAD)
-    ctx.timing_start(ml.KT_VC_RELAX, 400 * steps)
+    ctx.timing_start(ml.KT_VC_RELAX, 1200 * steps)
     wall = timed_steps(ctx, stepper, steps)
     n, ms, nbytes, missed = ctx.timing_stop()
     stepper.set_flags(0)
@@ -339,10 +343,10 @@ def vc_relax_roofline(ml, ctx, stepper, steps):
     per_launch = nbytes / n
     achieved = per_launch / avg_s / 1e9
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
+    if list(stepper.n) == [256, 256] and os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bytes_per_launch")
-    return {"kernel": "vc_relax_tile_kernel<double,2,16,16,1,9,1024> (every launch of the step)", "bound": "hbm",
+    return {"kernel": "%s (every launch of the step)" % VC_TILE[len(stepper.n)], "bound": "hbm",
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": round(per_launch, 1),
@@ -359,8 +363,12 @@ def main():
         sys.exit(2)
     ml = importlib.import_module("mushy-layer_amd")
     wl = importlib.import_module("mushy-layer_amd.workloads")
-    case = wl.mushy(n=(args.n, args.n))
-    db = None if args.hc_only else wl.darcy_brinkman(n=(args.n, args.n), Pr=10.0, Da=1e-3, Ra_C=200.0)
+    dim = args.dim
+    nn = args.n or (256 if dim == 2 else 128)
+    shape = (nn,) * dim
+    corr, Ra_C, Da = (0.1, 200.0, 1e-3) if dim == 2 else (0.08, 300.0, 5e-4)
+    case = wl.mushy(n=shape, corr=corr)
+    db = None if args.hc_only else wl.darcy_brinkman(n=shape, Pr=10.0, Da=Da, corr=corr, Ra_C=Ra_C)
     ctx = ml.Context(0)
     cells = int(np.prod(case["n"]))
     forms = {"fp64": ml.PRECISION_FP64, "fp32r": ml.PRECISION_FP32_REFINED}
@@ -384,10 +392,10 @@ def main():
 
     what = ("HC backward-Euler solve (porosity + coefficient assembly + solve)" if args.hc_only else
             "Darcy-Brinkman step: HC backward-Euler solve + U* Helmholtz solve per velocity component "
-            "(Pr 10, Da 1e-3, Kozeny-Carman) + variable-coefficient projection, coefficients assembled on "
-            "the device from the porosity")
-    cfg = {"workload": "mushy layer %dx%d, %s, GRF porosity corr 0.1, St 5, CR 2, Le 200, dt 1e-3"
-                       % (args.n, args.n, what),
+            "(Pr 10, Da %g, Ra_C %g, Kozeny-Carman) + variable-coefficient projection, coefficients assembled "
+            "on the device from the porosity" % (Da, Ra_C))
+    cfg = {"workload": "mushy layer %s, %s, GRF porosity corr %g, St 5, CR 2, Le 200, dt 1e-3"
+                       % ("x".join(map(str, shape)), what, corr),
            "form": head, "tolerance": "1e-10 rel or abs, max-norm", "max_iter": args.max_iter,
            "iterations_per_step": round(float(np.mean(h["iters"])), 2), "exit_status": h["status"]}
     if h["vstats"]:
