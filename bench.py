@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; 5 with --batch)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--dim", type=int, choices=[2, 3], default=2,
                     help="3: the 128^3 Darcy-Brinkman mushy layer (correlation length 0.08, Ra_C 300, Da 5e-4)")
@@ -56,9 +56,15 @@ def parse():
     ap.add_argument("--hc-only", action="store_true", help="time the (H, Theta) solve alone")
     ap.add_argument("--large-n", type=int, default=2048,
                     help="grid of the smoother bandwidth measurement (0: skip)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="> 0: the batched form on this many independent instances (64^2 .. 512x256, 8:4:2:1)")
     ap.add_argument("--no-lookahead", action="store_true",
                     help="solves enqueue one iteration at a time (no launches cut short; for counter runs)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    args.steps_given = args.steps is not None
+    if args.steps is None:
+        args.steps = 20
+    return args
 
 
 class HCStepper:
@@ -354,6 +360,139 @@ def vc_relax_roofline(ml, ctx, stepper, steps):
             "ms_per_step_instrumented": round(wall / steps * 1e3, 4)}
 
 
+def run_batch(ml, wl, ctx, args):
+    """The batched form: `args.batch` independent mushy-layer instances
+    (grid sizes 64^2, 128^2, 256^2, 512x256 drawn 8:4:2:1 by seed; St, CR,
+    Ra_C, Da drawn per instance) advance `args.steps` FP32-refined implicit
+    (H, Theta) steps together.  Reported beside it: the same instances
+    stepped one at a time through the single-instance path (one step), the
+    per-field difference of the batched step from the plain FP64 GPU step of
+    every instance, and the HIP-event roofline of the batched relaxation."""
+    steps = args.steps if args.steps_given else 5
+    cases = wl.batch_instances(count=args.batch, seed=5)
+    cells = int(sum(np.prod(c["n"]) for c in cases))
+    ops = [ml.HCOp(ctx, c["n"], c["dx"], c["periodic"], ml.make_params(**c["params"]), ml.make_bc(c["bc"]))
+           for c in cases]
+    batch = ml.HCBatch(ctx, ops)
+    init = [ml.DeviceArray(ctx, c["phi"].shape, host=c["phi"]) for c in cases]
+    phis = [ml.DeviceArray(ctx, c["phi"].shape) for c in cases]
+    # the reference's full stopping rule, hang included: on the finest
+    # 512x256 grids with large CR, rounding near the eutectic boundary holds
+    # some residuals near 1e-10, and a solve that stops decreasing ends
+    opts = ml.default_opts(max_iter=args.max_iter, precision=ml.PRECISION_FP32_REFINED)
+
+    def reset():
+        for a, b in zip(phis, init):
+            ctx.check(ctx.lib.ml_buf_copy(ctx.h, a.ptr, b.ptr, a.nbytes), "copy")
+
+    reset()
+    for _ in range(args.warmup):
+        batch.step(phis, cases[0]["dt"], opts)
+    reset()
+    ctx.sync()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(steps):
+        stats.append(batch.step(phis, cases[0]["dt"], opts))
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    iters = [s.iterations for st in stats for s in st]
+    status = sorted(set(s.exit_status for st in stats for s in st))
+
+    # one step of every instance through the single-instance path, FP32-
+    # refined (timed) and FP64 (the reference state of the error report)
+    reset()
+    ctx.sync()
+    batch.step(phis, cases[0]["dt"], opts)
+    after = [x.get() for x in phis]
+    single = {}
+    work = [(ml.DeviceArray(ctx, (1,) + x.shape[1:]), [ml.DeviceArray(ctx, ml.face_shape(c["n"], d, 2)) for d in range(2)],
+             ml.DeviceArray(ctx, x.shape)) for c, x in zip(cases, phis)]
+    for prec, name in ((ml.PRECISION_FP32_REFINED, "fp32r"), (ml.PRECISION_FP64, "fp64"),
+                       (ml.PRECISION_FP32_REFINED, "fp32r"), (ml.PRECISION_FP64, "fp64")):
+        # (each form twice: the first pass sets up the operators' engines)
+        reset()
+        ctx.sync()
+        o = ml.default_opts(max_iter=args.max_iter, precision=prec)
+        t0 = time.perf_counter()
+        sts = []
+        for op, c, x, (chi, b, old) in zip(ops, cases, phis, work):
+            op.porosity(x, chi)
+            op.assemble_bcoef(chi, b)
+            op.set_coefs(None, b, 1.0, -c["dt"])
+            ctx.check(ctx.lib.ml_buf_copy(ctx.h, old.ptr, x.ptr, x.nbytes), "copy")
+            sts.append(op.solve_be(x, old, c["dt"], opts=o).exit_status)
+        ctx.sync()
+        single[name] = (time.perf_counter() - t0, [x.get() for x in phis], sts)
+    same = all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(after, single["fp32r"][1]))
+    # per-field difference from the FP64 step, over the instances whose two
+    # solves both met the tolerance (status 1 or 8)
+    ok = [bool(s1 & 9) and bool(s2 & 9) for s1, s2 in zip(single["fp32r"][2], single["fp64"][2])]
+    err = {"H": 0.0, "Theta": 0.0}
+    for c, a, r, good in zip(cases, after, single["fp64"][1], ok):
+        if not good:
+            continue
+        va, vr = ml.valid(c["n"], a), ml.valid(c["n"], r)
+        for k, comp in (("H", 0), ("Theta", 1)):
+            err[k] = max(err[k], float(np.max(np.abs(va[comp] - vr[comp])) / np.max(np.abs(vr[comp]))))
+    chi_err = 0.0
+    for op, c, a, r, good in zip(ops, cases, after, single["fp64"][1], ok):
+        if not good:
+            continue
+        xs = []
+        for s_ in (a, r):
+            d = ml.DeviceArray(ctx, s_.shape, host=s_)
+            chi = ml.DeviceArray(ctx, (1,) + s_.shape[1:])
+            op.porosity(d, chi)
+            xs.append(ml.valid(c["n"], chi.get()))
+        chi_err = max(chi_err, float(np.max(np.abs(xs[0] - xs[1]))))
+    err["porosity_abs"] = chi_err
+    err["instances_compared"] = int(sum(ok))
+
+    # roofline of the batched relaxation: a pass with every launch timed
+    reset()
+    ctx.sync()
+    tflags = ml.SOLVE_NO_GRAPH | ml.SOLVE_NO_LOOKAHEAD
+    ot = ml.default_opts(max_iter=args.max_iter, precision=ml.PRECISION_FP32_REFINED, flags=tflags)
+    ctx.timing_start(ml.KT_BATCH_RELAX, 4000)
+    batch.step(phis, cases[0]["dt"], ot)
+    n1, ms, nb, missed = ctx.timing_stop()
+    achieved = nb / (ms * 1e-3) / 1e9
+    roof = {"kernel": "lin_relax_batch_kernel<2,16,16,1,8,1024> (every launch of one batched step)", "bound": "hbm",
+            "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "avg_launch_us": round(ms / n1 * 1e3, 3), "bytes_per_launch": round(nb / n1, 1),
+            "launches_timed": n1, "launches_untimed": missed}
+    sizes = {}
+    for c in cases:
+        k = "x".join(map(str, c["n"]))
+        sizes[k] = sizes.get(k, 0) + 1
+    batch.close()
+    for op in ops:
+        op.close()
+    cfg = {"workload": "batched form: %d independent mushy layers (%s), St ~ U[1,10], CR ~ U[1,20], Ra_C ~ U[0,400], "
+                       "Da ~ U[1e-4,1e-2] per instance, GRF porosity corr 0.1, dt 1e-3; %d FP32-refined (H, Theta) "
+                       "backward-Euler steps per instance (porosity + coefficient assembly + solve), every launch "
+                       "covering all instances" % (args.batch, ", ".join("%d x %s" % (v, k) for k, v in sizes.items()),
+                                                   steps),
+           "form": "fp32r-batched", "tolerance": "1e-10 rel or abs, max-norm; hang 1e-10", "max_iter": args.max_iter,
+           "status_counts": {str(k): sum(1 for st in stats for x in st if x.exit_status == k) for k in status},
+           "cells": cells, "iterations_per_solve": round(float(np.mean(iters)), 2),
+           "max_iterations": int(max(iters)), "exit_status": status,
+           "single_instance_fp32r_value": round(cells / single["fp32r"][0], 2),
+           "single_instance_fp64_value": round(cells / single["fp64"][0], 2),
+           "batched_over_single_fp32r": round((cells * steps / wall) / (cells / single["fp32r"][0]), 3),
+           "batched_equals_single_bitwise": bool(same),
+           "error_vs_fp64": err}
+    out = {
+        "metric": METRIC, "value": round(cells * steps / wall, 2), "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(wall / steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
+        "data": "synthetic", "config": cfg, "roofline": roof, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -363,6 +502,9 @@ def main():
         sys.exit(2)
     ml = importlib.import_module("mushy-layer_amd")
     wl = importlib.import_module("mushy-layer_amd.workloads")
+    if args.batch > 0:
+        run_batch(ml, wl, ml.Context(0), args)
+        return
     dim = args.dim
     nn = args.n or (256 if dim == 2 else 128)
     shape = (nn,) * dim

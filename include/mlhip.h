@@ -130,7 +130,10 @@ int ml_ctx_event_elapsed(ml_ctx ctx, int slot_a, int slot_b, float* ms);
  * launches beyond max_launches, and ends the timing. */
 enum {
   ML_KT_VC_RELAX = 1, /* tiled red/black relaxation of the linear VC operator */
-  ML_KT_HC_RELAX = 2  /* tiled red/black relaxation of the (H, Theta) operator with the closure */
+  ML_KT_HC_RELAX = 2, /* tiled red/black relaxation of the (H, Theta) operator with the closure */
+  ML_KT_BATCH_RELAX = 3 /* batched form: FP32 block relaxation of the linearised (H, Theta) correction
+                         * (every instance's level in one launch; launches are timed when the step runs
+                         * with ML_SOLVE_NO_GRAPH) */
 };
 int ml_ctx_timing_start(ml_ctx ctx, int kernel, int max_launches);
 int ml_ctx_timing_stop(ml_ctx ctx, long long* launches, double* total_ms, double* bytes, long long* missed);
@@ -264,6 +267,30 @@ int ml_vc_darcy_ccoef(ml_vc_op op, const double* chi, double darcy_coeff, double
  * porosity, src/CellToEdge2F.ChF:20-46); scale = -1 gives the projection's
  * -S_face with S = chi (srcSubcycle/AMRLevelMushyLayerInit.cpp:1141) */
 int ml_vc_porosity_faces(ml_vc_op op, const double* chi, double scale, double* const bCoef[]);
+/* ---- batched form (no counterpart in the reference: its runs advance one
+ * mushy layer per process) ------------------------------------------------
+ * `count` independent (H, Theta) instances, each an ml_hc_op of its own grid
+ * size, parameters and BCs (2D; finest level at most 512x512 cells, aCoef
+ * == 1), advance one implicit step together.  For every instance i the step
+ * is exactly the single-instance sequence
+ *   ml_hc_porosity(op_i, phi[i], chi) -> ml_hc_assemble_bcoef(op_i, chi, b)
+ *   -> ml_hc_set_coefs(op_i, NULL, b, 1, -dt)
+ *   -> ml_hc_solve_be(op_i, phi[i], copy of phi[i], NULL, dt, opts)
+ * with opts.precision == ML_PRECISION_FP32_REFINED, and gives bit-identical
+ * states and statistics; but every kernel launch covers all instances (2K + 2
+ * launches per outer iteration for the whole batch, K the deepest tiled-level
+ * count, replayed from a HIP graph unless ML_SOLVE_NO_GRAPH).  The operators
+ * are borrowed and must outlive the batch; their coefficient arrays are
+ * overwritten by each step.  Requires nu1 <= 3, nu2 <= 4 and max_iter <= 127. */
+typedef struct ml_hc_batch_s* ml_hc_batch;
+int ml_hc_batch_create(ml_ctx ctx, int count, const ml_hc_op ops[], ml_hc_batch* out);
+int ml_hc_batch_destroy(ml_hc_batch b);
+int ml_hc_batch_size(ml_hc_batch b);
+/* phi[i]: instance i's state (device, FAB layout, 2 components), advanced
+ * in place; stats[i] (may be NULL) its solve statistics */
+int ml_hc_batch_step(ml_hc_batch b, double* const phi[], double dt, const ml_solve_opts* opts,
+                     ml_solve_stats stats[]);
+
 /* ---- state update and coefficient assembly ----------------------------- */
 /* porosity of every valid cell (CALCULATEPOROSITY_MULTICOMP,
  * src/EnthalpyVariablesF.ChF:56-94, as in updateEnthalpyVariables

@@ -46,6 +46,7 @@ SOLVE_NO_GRAPH = 4   # launch every kernel from the host (no captured iterations
 PRECISION_FP64, PRECISION_FP32_REFINED = 0, 1
 KT_VC_RELAX = 1   # ml_ctx_timing_start kernel family: tiled relaxation of the linear VC operator
 KT_HC_RELAX = 2   # ... of the (H, Theta) operator with the closure
+KT_BATCH_RELAX = 3   # batched form: FP32 relaxation of every instance's level
 
 
 class SolveStats(C.Structure):
@@ -113,6 +114,10 @@ SIGNATURES = {
                                       C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "ml_vc_darcy_ccoef": (C.c_int, [_vp, _vp, C.c_double, _vp]),
     "ml_vc_porosity_faces": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(_vp)]),
+    "ml_hc_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
+    "ml_hc_batch_destroy": (C.c_int, [_vp]),
+    "ml_hc_batch_size": (C.c_int, [_vp]),
+    "ml_hc_batch_step": (C.c_int, [_vp, C.POINTER(_vp), C.c_double, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "ml_hc_porosity": (C.c_int, [_vp, _vp, _vp]),
     "ml_hc_assemble_bcoef": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
     "ml_norm_max": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _vp, _dp]),
@@ -383,6 +388,38 @@ class HCOp:
     def close(self):
         if self.h:
             self.ctx.lib.ml_hc_op_destroy(self.h)
+            self.h = None
+
+
+class HCBatch:
+    """The batched form: independent (H, Theta) instances (HCOp each, own
+    grid, parameters, BCs) that advance one FP32-refined implicit step
+    together, every launch covering all of them.  Instance i's step equals
+    porosity -> assemble_bcoef -> set_coefs(None, b, 1, -dt) ->
+    solve_be(phi_i, copy, dt, FP32-refined) on its HCOp, bit for bit."""
+
+    def __init__(self, ctx, ops):
+        self.ctx = ctx
+        self.ops = list(ops)
+        n = len(self.ops)
+        h = _vp()
+        ctx.check(ctx.lib.ml_hc_batch_create(ctx.h, n, (_vp * n)(*[op.h for op in self.ops]), C.byref(h)),
+                  "ml_hc_batch_create")
+        self.h = h
+
+    def step(self, phis, dt, opts=None):
+        """Advance every state (list of DeviceArray, one per instance) in
+        place; returns the per-instance SolveStats."""
+        n = len(self.ops)
+        st = (SolveStats * n)()
+        o = opts or default_opts(precision=PRECISION_FP32_REFINED)
+        self.ctx.check(self.ctx.lib.ml_hc_batch_step(self.h, (_vp * n)(*[x.ptr for x in phis]), dt, C.byref(o), st),
+                       "ml_hc_batch_step")
+        return list(st)
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.ml_hc_batch_destroy(self.h)
             self.h = None
 
 

@@ -123,7 +123,9 @@ int ml_ctx_event_elapsed(ml_ctx ctx, int a, int b, float* ms) {
 }
 
 int ml_ctx_timing_start(ml_ctx ctx, int kernel, int max_launches) {
-  if (!ctx || (kernel != ML_KT_VC_RELAX && kernel != ML_KT_HC_RELAX) || max_launches <= 0) return ML_EINVAL;
+  if (!ctx || (kernel != ML_KT_VC_RELAX && kernel != ML_KT_HC_RELAX && kernel != ML_KT_BATCH_RELAX) ||
+      max_launches <= 0)
+    return ML_EINVAL;
   while (int(ctx->timing_start.size()) < max_launches) {
     hipEvent_t a, b;
     ML_HIP(ctx, hipEventCreate(&a));

@@ -13,6 +13,8 @@
 // src/AMRNonLinearMultiCompOpF.ChF:125-160); the Jacobian differentiates
 // its closure branch by branch (CALCULATE_T_CL / CALCULATEPOROSITYMUSH,
 // src/EnthalpyVariablesF.ChF:419-467, :103-150).
+#include <hip/hip_ext.h>
+
 #include "ml_common.h"
 #include "ml_kernels.h"
 #include "ml_tile.h"
@@ -72,24 +74,22 @@ __device__ inline void closure_jacobian(double H, double C, const ClosureP<doubl
 // --------------------------------------------------------------------------
 // FP64 residual of the nonlinear operator, fused with phi += e and the
 // Jacobian, over a tile with a one-cell halo.
+// Tile `blk` of the level.
 template <int DIM, int TX, int TY, int TZ>
-__global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_kernel(HCArgs<double> A,
-                                                                           const double* __restrict__ phi,
-                                                                           double* __restrict__ phi_out,
-                                                                           const double* __restrict__ rhs,
-                                                                           const float* __restrict__ ecorr,
-                                                                           double omega,
-                                                                           float* __restrict__ r32,
-                                                                           float* __restrict__ M32,
-                                                                           unsigned long long* norm, int ntx, int nty,
-                                                                           CtlFuse fuse) {
+__device__ __forceinline__ void hc_residual_jacobian_body(const HCArgs<double>& A, const double* __restrict__ phi,
+                                                          double* __restrict__ phi_out,
+                                                          const double* __restrict__ rhs,
+                                                          const float* __restrict__ ecorr, double omega,
+                                                          float* __restrict__ r32, float* __restrict__ M32,
+                                                          unsigned long long* norm, int ntx, int nty,
+                                                          const CtlFuse& fuse, int blk) {
   if (A.skip && *A.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2) * (TY + 2) * (DIM == 3 ? TZ + 2 : 1);
   constexpr int KMAX = (EMAX + kLinThreads - 1) / kLinThreads;
   const Geom& g = A.g;
   Tile<DIM, TX, TY, TZ> t;
-  t.init(1, ntx, nty);
+  t.init_at(1, ntx, nty, blk);
   const int E = t.E;
   double* sphi = reinterpret_cast<double*>(smem_raw);
   double* sphl = sphi + 2 * E;
@@ -159,6 +159,21 @@ __global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_kernel(HCArg
   block_max_atomic<double>(mx, nan, norm, fuse);
 }
 
+template <int DIM, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_kernel(HCArgs<double> A,
+                                                                           const double* __restrict__ phi,
+                                                                           double* __restrict__ phi_out,
+                                                                           const double* __restrict__ rhs,
+                                                                           const float* __restrict__ ecorr,
+                                                                           double omega,
+                                                                           float* __restrict__ r32,
+                                                                           float* __restrict__ M32,
+                                                                           unsigned long long* norm, int ntx, int nty,
+                                                                           CtlFuse fuse) {
+  hc_residual_jacobian_body<DIM, TX, TY, TZ>(A, phi, phi_out, rhs, ecorr, omega, r32, M32, norm, ntx, nty, fuse,
+                                             blockIdx.x);
+}
+
 // --------------------------------------------------------------------------
 // Linear operator pieces (any precision T).
 
@@ -206,13 +221,14 @@ __device__ __forceinline__ void block_step(const CellFaces<T, DIM>& F, const T* 
   e1 = e1 + (inv[2] * r0 + inv[3] * r1);
 }
 
-// Red/black block relaxation of J e = r over a tile with halo npass.
+// Red/black block relaxation of J e = r over tile `blk` of the level with
+// halo npass.
 template <int DIM, int TX, int TY, int TZ, int HMAX, int NT>
-__global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, const float* __restrict__ ein,
-                                                            float* __restrict__ eout, BCDesc bc, float alpha,
-                                                            float beta, int npass, int zero_init, Geom gc,
-                                                            const float* __restrict__ pc, int ntx, int nty, Geom gr,
-                                                            float* __restrict__ rc, float* __restrict__ Mc_out) {
+__device__ __forceinline__ void lin_relax_tile_body(const LinLevel<float>& L, const float* __restrict__ ein,
+                                                    float* __restrict__ eout, const BCDesc& bc, float alpha,
+                                                    float beta, int npass, int zero_init, const Geom& gc,
+                                                    const float* __restrict__ pc, int ntx, int nty, const Geom& gr,
+                                                    float* __restrict__ rc, float* __restrict__ Mc_out, int blk) {
   if (L.skip && *L.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2 * HMAX) * (TY + 2 * HMAX) * (DIM == 3 ? TZ + 2 * HMAX : 1);
@@ -221,7 +237,7 @@ __global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, c
   Tile<DIM, TX, TY, TZ> t;
   // with a restriction to produce, one more halo cell keeps the core's
   // neighbours exact after the last pass
-  t.init(npass + (rc ? 1 : 0), ntx, nty);
+  t.init_at(npass + (rc ? 1 : 0), ntx, nty, blk);
   const int E = t.E;
   float* se = reinterpret_cast<float*>(smem_raw);
   float* su = se + 2 * E;
@@ -378,6 +394,16 @@ __global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, c
 #pragma unroll
     for (int mm = 0; mm < 4; ++mm) Mc_out[qc + mm * gr.ncell] = m[mm] * w;
   }
+}
+
+template <int DIM, int TX, int TY, int TZ, int HMAX, int NT>
+__global__ __launch_bounds__(NT) void lin_relax_tile_kernel(LinLevel<float> L, const float* __restrict__ ein,
+                                                            float* __restrict__ eout, BCDesc bc, float alpha,
+                                                            float beta, int npass, int zero_init, Geom gc,
+                                                            const float* __restrict__ pc, int ntx, int nty, Geom gr,
+                                                            float* __restrict__ rc, float* __restrict__ Mc_out) {
+  lin_relax_tile_body<DIM, TX, TY, TZ, HMAX, NT>(L, ein, eout, bc, alpha, beta, npass, zero_init, gc, pc, ntx, nty,
+                                                 gr, rc, Mc_out, blockIdx.x);
 }
 
 // r_c = R(r - J e), M_c = R(M) over a tile with a one-cell halo.
@@ -876,7 +902,9 @@ template <int DIM>
 __global__ __launch_bounds__(kWGT) void lin_coarse_vcycle_kernel(const LinDesc* __restrict__ Dp) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int LL = LinLevels<DIM>::n;
-  const LinDesc& D = *Dp;
+  // one workgroup per descriptor: the instances of the batched form share
+  // the launch (a single solve launches one workgroup)
+  const LinDesc& D = Dp[blockIdx.x];
   if (D.skip && *D.skip) return;
   const int l0 = D.l0, nl = D.nlev - D.l0;
   const float alpha = float(D.alpha), beta = float(D.beta);
@@ -1213,6 +1241,85 @@ hipError_t cast_f64_f32(long long n, const double* x, float* y, hipStream_t s) {
   long long b = (n + 255) / 256;
   if (b > 4096) b = 4096;
   hipLaunchKernelGGL(cast_kernel, dim3(int(b < 1 ? 1 : b)), dim3(256), 0, s, n, x, y);
+  return hipGetLastError();
+}
+
+
+// ---- batched form -----------------------------------------------------------
+namespace {
+
+template <int DIM, int TX, int TY, int TZ, int HMAX, int NT>
+__global__ __launch_bounds__(NT) void lin_relax_batch_kernel(const BatchLinJob* __restrict__ jobs,
+                                                             const unsigned* __restrict__ map, int npass,
+                                                             int zero_init) {
+  const unsigned w = map[blockIdx.x];
+  const BatchLinJob& J = jobs[w >> kBatchTileBits];
+  lin_relax_tile_body<DIM, TX, TY, TZ, HMAX, NT>(J.L, J.ein, J.eout, J.bc, J.alpha, J.beta, npass, zero_init, J.gc,
+                                                 J.pc, J.ntx, J.nty, J.gr, J.rc, J.Mc, int(w & kBatchTileMask));
+}
+
+template <int DIM, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_batch_kernel(const BatchResJob* __restrict__ jobs,
+                                                                                 const unsigned* __restrict__ map,
+                                                                                 int initial) {
+  const unsigned w = map[blockIdx.x];
+  const BatchResJob& J = jobs[w >> kBatchTileBits];
+  const int blk = int(w & kBatchTileMask);
+  if (initial) {
+    hc_residual_jacobian_body<DIM, TX, TY, TZ>(J.A, J.pb[0], nullptr, J.rhs, nullptr, 1.0, J.r32, J.M32, J.slots,
+                                               J.ntx, J.nty, CtlFuse{}, blk);
+    return;
+  }
+  if (*J.A.skip) return;
+  // the instance's own iteration: the last workgroup of this launch moves
+  // ctl->iter on only after every workgroup has read it
+  const int it = J.ctl->iter + 1;
+  const int pc = (it - 1) & 1;
+  const CtlFuse f{J.ctl, J.slots, J.arrive, 1, J.ntiles};
+  hc_residual_jacobian_body<DIM, TX, TY, TZ>(J.A, J.pb[pc], J.pb[1 - pc], J.rhs, J.e, 1.0, J.r32, J.M32,
+                                             J.slots + it, J.ntx, J.nty, f, blk);
+}
+
+}  // namespace
+
+int lin_relax_tiles2(const Geom& g, int& ntx, int& nty) { return tiles2<LinRelaxTile2::TX, LinRelaxTile2::TY>(g, ntx, nty); }
+int lin_residual_tiles2(const Geom& g, int& ntx, int& nty) { return lin_tiles<2>(g, ntx, nty); }
+long long lin_deep_tiles_max2() { return 1024; }
+
+hipError_t lin_relax_batch(const BatchLinJob* jobs, const unsigned* map, int nblocks, int npass, bool zero_init,
+                           bool restrict_, hipStream_t s, const LaunchTimer* tm, double bytes) {
+  using DR = LinRelaxTile2;
+  const int h = npass + (restrict_ ? 1 : 0);
+  if (npass < 1 || h > 8) return hipErrorInvalidValue;
+  if (nblocks == 0) return hipSuccess;
+  const size_t lds =
+      (8 * size_t(DR::TX + 2 * h) * (DR::TY + 2 * h) + (restrict_ ? 2 * DR::TX * DR::TY : 0)) * sizeof(float);
+  const bool timed = tm && tm->start && *tm->n < tm->cap;
+  if (tm && tm->start && !timed) ++*tm->missed;
+  hipExtLaunchKernelGGL((lin_relax_batch_kernel<2, DR::TX, DR::TY, DR::TZ, 8, DR::NT>), dim3(nblocks), dim3(DR::NT),
+                        lds, s, timed ? tm->start[*tm->n] : nullptr, timed ? tm->stop[*tm->n] : nullptr, 0, jobs, map,
+                        npass, zero_init ? 1 : 0);
+  const hipError_t e = hipGetLastError();
+  if (e == hipSuccess && timed) {
+    *tm->bytes += bytes;
+    ++*tm->n;
+  }
+  return e;
+}
+
+hipError_t hc_residual_jacobian_batch(const BatchResJob* jobs, const unsigned* map, int nblocks, bool initial,
+                                      hipStream_t s) {
+  using D = LinTile<2>;
+  if (nblocks == 0) return hipSuccess;
+  const size_t lds = 4 * lin_extent<2>(1) * sizeof(double);
+  hipLaunchKernelGGL((hc_residual_jacobian_batch_kernel<2, D::TX, D::TY, D::TZ>), dim3(nblocks), dim3(kLinThreads), lds,
+                     s, jobs, map, initial ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t lin_coarse_vcycle_batch(const LinDesc* descs, int n, size_t lds, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL((lin_coarse_vcycle_kernel<2>), dim3(n), dim3(kWGT), lds, s, descs);
   return hipGetLastError();
 }
 

@@ -80,6 +80,7 @@ struct CtlFuse {
   const unsigned long long* slots;
   unsigned int* arrive;
   int refined;
+  unsigned int total;   // workgroups that report (0: the whole grid)
 };
 
 // Max-reduction of a non-negative value over the workgroup, then one
@@ -112,7 +113,7 @@ __device__ inline void block_max_atomic(T v, int nan, unsigned long long* out, C
     atomicMax(out, (unsigned long long)__double_as_longlong(m));
     if (f.ctl) {
       __threadfence();
-      const unsigned int total = gridDim.x * gridDim.y * gridDim.z;
+      const unsigned int total = f.total ? f.total : gridDim.x * gridDim.y * gridDim.z;
       if (atomicAdd(f.arrive, 1u) == total - 1) {
         __threadfence();
         atomicExch(f.arrive, 0u);
@@ -347,6 +348,94 @@ struct CastJobs {
   float* dst[3 * kMaxLevels];
 };
 hipError_t cast_many(const CastJobs& J, hipStream_t s);
+
+// ---- batched form: independent instances share every launch ------------
+// A batched tiled launch covers one level of each of several instances;
+// workgroup w works on tile (map[w] & kBatchTileMask) of job
+// (map[w] >> kBatchTileBits).
+constexpr int kBatchTileBits = 20;
+constexpr unsigned kBatchTileMask = (1u << kBatchTileBits) - 1;
+
+// one instance's level in a batched relaxation launch of J e = r
+struct BatchLinJob {
+  LinLevel<float> L;   // L.skip: the instance's done flag
+  const float* ein;
+  float* eout;
+  BCDesc bc;
+  float alpha, beta;
+  Geom gc;             // prolongation source level (pc != nullptr)
+  const float* pc;
+  Geom gr;             // restriction target level (rc != nullptr)
+  float* rc;
+  float* Mc;
+  int ntx, nty;
+};
+
+// one instance in a batched FP64 residual + Jacobian launch
+struct BatchResJob {
+  HCArgs<double> A;    // level 0; A.skip == &ctl->done
+  double* pb[2];       // state pair: iteration k reads pb[(k - 1) & 1] and writes the other
+  const double* rhs;
+  const float* e;      // level-0 correction
+  float* r32;
+  float* M32;
+  unsigned long long* slots;   // residual norm of iteration k in slots[k]
+  SolveCtl* ctl;
+  unsigned int* arrive;
+  unsigned int ntiles;
+  int ntx, nty;
+};
+
+// one instance in the step set-up / wrap-up launches (ml_batch.hip)
+struct BatchSetupJob {
+  Geom g;                 // level 0
+  ClosureP<double> cp;
+  double kcond, DH, DC;   // bCoef parameters
+  double* phi;            // state (in; the wrap-up copies the final state back)
+  double* alt;            // its partner, a full copy
+  double* rhs;            // backward-Euler rhs (aCoef == 1, no source)
+  double* b64[3];         // level-0 faces
+  float* b32[3];
+  unsigned long long* slots;   // norm slots, zeroed
+  int nslots;
+  SolveCtl* ctl;
+};
+
+// one instance in the coefficient-hierarchy launch: job jb covers the faces
+// of direction dir[jb] on level level[jb] >= 1
+struct BatchCoarsenJob {
+  Geom g0;
+  int njobs;
+  long long begin[3 * kMaxLevels + 1];
+  short level[3 * kMaxLevels];
+  short dir[3 * kMaxLevels];
+  const double* b0[3];
+  double* b64[3 * kMaxLevels];
+  float* b32[3 * kMaxLevels];
+};
+
+long long batch_chunk_items();   // items of one workgroup of the set-up launches
+long long batch_setup_items(const BatchSetupJob& J);
+hipError_t batch_setup(const BatchSetupJob* jobs, const unsigned* map, int nblocks, hipStream_t s);
+hipError_t batch_coarsen(const BatchCoarsenJob* jobs, const unsigned* map, int nblocks, hipStream_t s);
+hipError_t batch_ctl_init(SolveCtl* ctl, const unsigned long long* slots, int nslots, int n, int max_iter, double eps,
+                          double hang, double thresh, hipStream_t s);
+hipError_t batch_finish(const BatchSetupJob* jobs, const unsigned* map, int nblocks, hipStream_t s);
+
+// tiles of the relaxation / residual launches of a 2D level (batched form)
+int lin_relax_tiles2(const Geom& g, int& ntx, int& nty);
+int lin_residual_tiles2(const Geom& g, int& ntx, int& nty);
+// level-0 tiles above which a single solve changes its relaxation tiles
+long long lin_deep_tiles_max2();
+// tm (may be null): launch timing, `bytes` the launch's algorithmic bytes
+hipError_t lin_relax_batch(const BatchLinJob* jobs, const unsigned* map, int nblocks, int npass, bool zero_init,
+                           bool restrict_, hipStream_t s, const LaunchTimer* tm = nullptr, double bytes = 0);
+// initial: residual of the state pb[0] into slots[0], no correction, no
+// stopping rule; else iteration ctl->iter + 1 with the stopping rule
+hipError_t hc_residual_jacobian_batch(const BatchResJob* jobs, const unsigned* map, int nblocks, bool initial,
+                                      hipStream_t s);
+// the coarse V-cycles of n instances, one workgroup each
+hipError_t lin_coarse_vcycle_batch(const LinDesc* descs, int n, size_t lds, hipStream_t s);
 
 // ---- nonlinear (H, Theta) operator (ml_hc_kernels.hip)
 // Tiled red/black relaxation.  Every launch reads buf[*cur] and writes

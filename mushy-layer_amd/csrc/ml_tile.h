@@ -98,14 +98,15 @@ template <int DIM, int TX, int TY, int TZ>
 struct Tile {
   int H, EX, EY, EZ, EXY, E;
   int x0, y0, z0;
-  __device__ inline void init(int halo, int ntx, int nty) {
+  __device__ inline void init(int halo, int ntx, int nty) { init_at(halo, ntx, nty, blockIdx.x); }
+  // tile b of the level (b != blockIdx.x when several grids share a launch)
+  __device__ inline void init_at(int halo, int ntx, int nty, int b) {
     H = halo;
     EX = TX + 2 * H;
     EY = TY + 2 * H;
     EZ = DIM == 3 ? TZ + 2 * H : 1;
     EXY = EX * EY;
     E = EXY * EZ;
-    const int b = blockIdx.x;
     const int tx = b % ntx, r = b / ntx, ty = r % nty, tz = r / nty;
     x0 = tx * TX;
     y0 = ty * TY;

@@ -12,6 +12,10 @@ Workloads (named by what they are):
   * ``mushy_256`` — 256x256 mushy layer with porosity from a smooth seeded
     Gaussian random field (correlation length 0.1) in [0.05, 1], St = 5,
     CR = 2, Kozeny-Carman permeability, Da = 1e-3, Ra_C = 200.
+  * ``batch_instances`` — independent mushy layers for the batched form:
+    grid sizes drawn by seed in ratio 8:4:2:1 from 64^2, 128^2, 256^2,
+    512x256; St in [1, 10], CR in [1, 20], Ra_C in [0, 400], Da in
+    [1e-4, 1e-2], uniformly.
 """
 import numpy as np
 
@@ -189,3 +193,27 @@ def darcy_brinkman(n=(256, 256), seed=2, Pr=10.0, Da=1e-3, corr=0.1, dt=1e-3, st
     bc_p = {(vd, 0, 0): (NEUMANN, 0.0), (vd, 1, 0): (NEUMANN, 0.0)}
     return dict(n=n, dim=dim, dx=1.0 / n[0], periodic=periodic, chi=chi, c=c, b_u=b_u, b_p=b_p, u0=u0, src=src,
                 bc_u=bc_u, bc_p=bc_p, dt=dt, steps=steps, Pr=Pr, Da=Da)
+
+
+BATCH_SIZES = ((64, 64), (128, 128), (256, 256), (512, 256))
+
+
+def batch_instances(count=512, seed=5, sizes=BATCH_SIZES, weights=(8, 4, 2, 1), corr=0.1, dt=1e-3):
+    """Independent mushy-layer instances: each a `mushy` case of its own seed,
+    grid size (drawn with probabilities proportional to `weights`) and
+    parameters St ~ U[1, 10], CR ~ U[1, 20]; Ra_C ~ U[0, 400] and
+    Da ~ U[1e-4, 1e-2] are drawn too and stored with the case (they enter
+    only the velocity solves)."""
+    rng = np.random.default_rng(seed)
+    w = np.asarray(weights, dtype=float)
+    pick = rng.choice(len(sizes), size=count, p=w / w.sum())
+    out = []
+    for i, k in enumerate(pick):
+        St = float(rng.uniform(1.0, 10.0))
+        CR = float(rng.uniform(1.0, 20.0))
+        Ra_C = float(rng.uniform(0.0, 400.0))
+        Da = float(rng.uniform(1e-4, 1e-2))
+        c = mushy(n=sizes[k], seed=seed * 100003 + i, St=St, CR=CR, corr=corr, dt=dt)
+        c["Ra_C"], c["Da"] = Ra_C, Da
+        out.append(c)
+    return out
