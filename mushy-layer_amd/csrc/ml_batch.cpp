@@ -30,7 +30,6 @@ namespace {
 constexpr int kBatchSlots = 128;   // norm slots per instance (max_iter + 1)
 constexpr int kSmallCells = 1024;  // levels of at most this many cells go to the single-workgroup kernel
 constexpr int kMaxHalvings = 6;    // step-length backtracking, as the single-instance solve
-constexpr int kCompactFraction = 8;   // compact the launches once at most 1/8 of the instances run
 
 struct BatchInst {
   ml_hc_op op = nullptr;
@@ -64,7 +63,7 @@ struct ml_hc_batch_s {
   ml::SolveCtl* ctl = nullptr;
   ml::SolveCtl* hctl = nullptr;           // pinned mirror
   unsigned long long* slots = nullptr;
-  unsigned long long* hslot = nullptr;    // pinned, one slot
+  unsigned long long* hslot = nullptr;    // pinned, one slot per instance
   unsigned int* arrive = nullptr;
   // job tables and workgroup maps: one device arena, one pinned mirror
   unsigned char* arena = nullptr;
@@ -554,43 +553,57 @@ int capture(ml_hc_batch b, const ml_solve_opts& o) {
   return ML_OK;
 }
 
-// host step-length backtracking of instance i after a rejected step (the
-// single-instance rule)
-int backtrack(ml_hc_batch b, int i, SolveCtl& c) {
+// host step-length backtracking after a rejected step, the single-instance
+// rule for each instance in `idx`: the trials of one step length run for
+// all of them, then one synchronisation
+int backtrack(ml_hc_batch b, const std::vector<int>& idx) {
   ml_ctx ctx = b->ctx;
   hipStream_t s = ctx->stream;
-  BatchInst& I = b->inst[i];
-  unsigned long long* slots = b->slots + size_t(i) * kBatchSlots;
-  const int it = c.iter + 1;
-  const int pc = c.iter & 1;
-  double* const pb[2] = {b->key_phi[i], I.alt0};
-  const HCArgs<double> An = I.op->args(0);
-  double norm = c.norm, omega = 1.0;
-  for (int k = 1; k <= kMaxHalvings; ++k) {
+  std::vector<int> trying = idx;
+  std::vector<double> norm(b->inst.size());
+  for (int i : idx) norm[i] = b->hctl[i].norm;
+  double omega = 1.0;
+  for (int k = 1; k <= kMaxHalvings && !trying.empty(); ++k) {
     omega *= 0.5;
-    ML_HIP(ctx, zero_u64(slots + it, 1, s));
-    ML_HIP(ctx, hc_residual_jacobian(An, pb[pc], pb[1 - pc], I.rhs, I.lin[0].e, omega, I.lin[0].r, I.lin[0].M,
-                                     slots + it, s));
-    ML_HIP(ctx, hipMemcpyAsync(b->hslot, slots + it, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    for (int i : trying) {
+      BatchInst& I = b->inst[i];
+      const SolveCtl& c = b->hctl[i];
+      unsigned long long* slots = b->slots + size_t(i) * kBatchSlots;
+      const int it = c.iter + 1;
+      const int pc = c.iter & 1;
+      double* const pb[2] = {b->key_phi[i], I.alt0};
+      const HCArgs<double> An = I.op->args(0);
+      ML_HIP(ctx, zero_u64(slots + it, 1, s));
+      ML_HIP(ctx, hc_residual_jacobian(An, pb[pc], pb[1 - pc], I.rhs, I.lin[0].e, omega, I.lin[0].r, I.lin[0].M,
+                                       slots + it, s));
+      ML_HIP(ctx, hipMemcpyAsync(b->hslot + i, slots + it, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    }
     ML_HIP(ctx, hipStreamSynchronize(s));
-    std::memcpy(&norm, b->hslot, sizeof(double));
-    if (norm < c.prev) break;
+    std::vector<int> left;
+    for (int i : trying) {
+      std::memcpy(&norm[i], b->hslot + i, sizeof(double));
+      if (!(norm[i] < b->hctl[i].prev)) left.push_back(i);
+    }
+    trying.swap(left);
   }
-  c.iter = it;
-  int status = 0;
-  if (norm <= c.eps * c.r0) status |= 1;
-  if (norm <= c.thresh) status |= 8;
-  if (status == 0 && norm > (1.0 - c.hang) * c.prev) status |= 4;
-  c.prev = norm;
-  c.norm = norm;
-  c.need_host = 0;
-  c.done = 0;
-  if (status != 0 || c.iter >= c.max_iter) {
-    c.status = status != 0 ? status : 2;
-    c.done = 1;
+  for (int i : idx) {
+    SolveCtl& c = b->hctl[i];
+    const double nm = norm[i];
+    c.iter = c.iter + 1;
+    int status = 0;
+    if (nm <= c.eps * c.r0) status |= 1;
+    if (nm <= c.thresh) status |= 8;
+    if (status == 0 && nm > (1.0 - c.hang) * c.prev) status |= 4;
+    c.prev = nm;
+    c.norm = nm;
+    c.need_host = 0;
+    c.done = 0;
+    if (status != 0 || c.iter >= c.max_iter) {
+      c.status = status != 0 ? status : 2;
+      c.done = 1;
+    }
+    ML_HIP(ctx, hipMemcpyAsync(b->ctl + i, b->hctl + i, sizeof(SolveCtl), hipMemcpyHostToDevice, s));
   }
-  b->hctl[i] = c;
-  ML_HIP(ctx, hipMemcpyAsync(b->ctl + i, b->hctl + i, sizeof(SolveCtl), hipMemcpyHostToDevice, s));
   ML_HIP(ctx, hipStreamSynchronize(s));
   return ML_OK;
 }
@@ -631,7 +644,7 @@ int ml_hc_batch_create(ml_ctx ctx, int count, const ml_hc_op ops[], ml_hc_batch*
       b->hctl = static_cast<SolveCtl*>(v);
   }
   if (rc == ML_OK) {
-    if (hipHostMalloc(&v, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc(&v, sizeof(unsigned long long) * count, hipHostMallocDefault) != hipSuccess)
       rc = fail(ctx, ML_ENOMEM, "hipHostMalloc");
     else
       b->hslot = static_cast<unsigned long long*>(v);
@@ -697,7 +710,8 @@ int ml_hc_batch_step(ml_hc_batch b, double* const phi[], double dt, const ml_sol
   const bool lookahead = !(o.flags & ML_SOLVE_NO_LOOKAHEAD);
   int batch = lookahead ? std::max(1, std::min(b->batch_iters, o.max_iter)) : 1;
   int low = 0;   // fewest accepted iterations among the running instances
-  int most = 0;
+  int fewest = o.max_iter;   // iterations of the instance that finished first
+  int compacted = n;         // running instances when the maps were last compacted
   bool compact = false;
   std::vector<char> run(n, 1);
   for (;;) {
@@ -712,28 +726,36 @@ int ml_hc_batch_step(ml_hc_batch b, double* const phi[], double dt, const ml_sol
     }
     ML_HIP(ctx, hipMemcpyAsync(b->hctl, b->ctl, sizeof(SolveCtl) * n, hipMemcpyDeviceToHost, s));
     ML_HIP(ctx, hipStreamSynchronize(s));
+    std::vector<int> rejected;
+    for (int i = 0; i < n; ++i)
+      if (b->hctl[i].need_host) rejected.push_back(i);
+    if (!rejected.empty() && (rc = backtrack(b, rejected))) return rc;
     int running = 0;
     low = o.max_iter;
     for (int i = 0; i < n; ++i) {
-      SolveCtl c = b->hctl[i];
-      if (c.need_host && (rc = backtrack(b, i, c))) return rc;
+      const SolveCtl& c = b->hctl[i];
       run[i] = !c.done && c.iter < o.max_iter;
       if (run[i]) {
         ++running;
         low = std::min(low, c.iter);
+      } else {
+        fewest = std::min(fewest, c.iter);
       }
-      most = std::max(most, c.iter);
     }
     if (!running) break;
-    // few instances left: later iterations launch only their tiles (the
-    // finished ones would return at once from every workgroup)
-    if (!compact && running * kCompactFraction <= n) {
+    // once the running instances have halved, later iterations launch only
+    // their tiles (the finished ones would return at once from every
+    // workgroup)
+    if (2 * running <= compacted) {
       if ((rc = build_compact(b, run))) return rc;
       compact = true;
+      compacted = running;
     }
     batch = lookahead ? 2 : 1;
   }
-  if (lookahead) b->batch_iters = std::max(1, most);
+  // the next step's first host check comes when the first instance is
+  // expected to finish
+  if (lookahead) b->batch_iters = std::max(1, std::min(fewest, o.max_iter));
   ML_HIP(ctx, batch_finish(b->at<BatchSetupJob>(b->setup_off), b->at<unsigned>(b->setup_map_off), b->setup_blocks, s));
   if (stats) {
     ML_HIP(ctx, hipMemcpyAsync(b->hctl, b->ctl, sizeof(SolveCtl) * n, hipMemcpyDeviceToHost, s));

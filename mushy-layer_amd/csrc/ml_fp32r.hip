@@ -97,6 +97,19 @@ __device__ __forceinline__ void hc_residual_jacobian_body(const HCArgs<double>& 
   PosRec R[KMAX];
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) pos_record<DIM, TX, TY, TZ>(g, t, threadIdx.x + kk * kLinThreads, R[kk]);
+  // 2D: the owners' face coefficients and rhs are loaded before anything
+  // waits (3D keeps them for later: too many registers)
+  constexpr bool kPre = DIM == 2;
+  CellFaces<double, DIM> F[KMAX];
+  double rq[KMAX][2];
+#pragma unroll
+  for (int kk = 0; kk < KMAX; ++kk) {
+    const PosRec& r = R[kk];
+    if (!kPre || !r.owner) continue;
+    load_faces<double, DIM>(A, r.i, r.j, r.k, F[kk]);
+    rq[kk][0] = rhs[r.q];
+    rq[kk][1] = rhs[r.q + g.ncell];
+  }
 #pragma unroll
   for (int kk = 0; kk < KMAX; ++kk) {
     const PosRec& r = R[kk];
@@ -136,16 +149,19 @@ __device__ __forceinline__ void hc_residual_jacobian_body(const HCArgs<double>& 
       phi_out[r.q] = sphi[r.e];
       phi_out[r.q + g.ncell] = sphi[E + r.e];
     }
-    CellFaces<double, DIM> F;
-    load_faces<double, DIM>(A, r.i, r.j, r.k, F);
+    if (!kPre) {
+      load_faces<double, DIM>(A, r.i, r.j, r.k, F[kk]);
+      rq[kk][0] = rhs[r.q];
+      rq[kk][1] = rhs[r.q + g.ncell];
+    }
 #pragma unroll
     for (int n = 0; n < 2; ++n) {
       const long long q = r.q + n * g.ncell;
       double st[7];
       gather_lds<double, DIM>(sphl + n * E, r.e, t.EX, t.EXY, st);
       const double aq = A.a ? A.a[q] : 1.0;
-      const double lof = lof_of<double, DIM>(F, n, st, aq, sphi[n * E + r.e], A.alpha, A.beta, dxinv);
-      const double res = rhs[q] - lof;
+      const double lof = lof_of<double, DIM>(F[kk], n, st, aq, sphi[n * E + r.e], A.alpha, A.beta, dxinv);
+      const double res = rq[kk][n] - lof;
       r32[q] = float(res);
       const double ar = fabs(res);
       if (ar != ar) nan = 1;

@@ -110,13 +110,22 @@ __device__ inline void block_max_atomic(T v, int nan, unsigned long long* out, C
       const double o = wmax[i];
       m = (o != o || m != m) ? __longlong_as_double(0x7ff8000000000000LL) : (o > m ? o : m);
     }
-    atomicMax(out, (unsigned long long)__double_as_longlong(m));
-    if (f.ctl) {
-      __threadfence();
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+    if (!f.ctl) {
+      atomicMax(out, bits);
+    } else {
+      // Device-scope atomics are performed at the device's coherence point.
+      // The arrival is counted only once this workgroup's maximum has been
+      // performed there: the increment depends on the value the max returns
+      // (the sign bit of a stored norm, always 0), so the wave waits for it.
+      // The last arrival then reads every maximum with device-scope atomic
+      // loads; no fence (and no L2 write-back) is needed in any workgroup.
+      const unsigned long long before =
+          __hip_atomic_fetch_max(out, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned int inc = 1u + unsigned(before >> 63);
       const unsigned int total = f.total ? f.total : gridDim.x * gridDim.y * gridDim.z;
-      if (atomicAdd(f.arrive, 1u) == total - 1) {
-        __threadfence();
-        atomicExch(f.arrive, 0u);
+      if (__hip_atomic_fetch_add(f.arrive, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == total - 1) {
+        __hip_atomic_store(f.arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ctl_check_step(f.ctl, f.slots, f.refined);
       }
     }
