@@ -396,6 +396,7 @@ def run_batch(ml, wl, ctx, args):
         stats.append(batch.step(phis, cases[0]["dt"], opts))
     ctx.sync()
     wall = time.perf_counter() - t0
+    tv, tshape, tms = batch.tile()
     iters = [s.iterations for st in stats for s in st]
     status = sorted(set(s.exit_status for st in stats for s in st))
 
@@ -458,7 +459,8 @@ def run_batch(ml, wl, ctx, args):
     batch.step(phis, cases[0]["dt"], ot)
     n1, ms, nb, missed = ctx.timing_stop()
     achieved = nb / (ms * 1e-3) / 1e9
-    roof = {"kernel": "lin_relax_batch_kernel<2,16,16,1,8,1024> (every launch of one batched step)", "bound": "hbm",
+    roof = {"kernel": "lin_relax_batch_kernel<2,%d,%d,1,8,%d> (every launch of one batched step)" % tshape,
+            "bound": "hbm",
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
             "avg_launch_us": round(ms / n1 * 1e3, 3), "bytes_per_launch": round(nb / n1, 1),
@@ -483,6 +485,8 @@ def run_batch(ml, wl, ctx, args):
            "single_instance_fp64_value": round(cells / single["fp64"][0], 2),
            "batched_over_single_fp32r": round((cells * steps / wall) / (cells / single["fp32r"][0]), 3),
            "batched_equals_single_bitwise": bool(same),
+           "relax_tile_autotune": {"chosen": "%dx%d/%d" % tshape,
+                                   "ms_per_sweep_set": {"shape%d" % v: round(t, 4) for v, t in enumerate(tms)}},
            "error_vs_fp64": err}
     out = {
         "metric": METRIC, "value": round(cells * steps / wall, 2), "unit": "cell-updates/s", "n_gpus": 1,

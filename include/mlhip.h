@@ -286,6 +286,15 @@ typedef struct ml_hc_batch_s* ml_hc_batch;
 int ml_hc_batch_create(ml_ctx ctx, int count, const ml_hc_op ops[], ml_hc_batch* out);
 int ml_hc_batch_destroy(ml_hc_batch b);
 int ml_hc_batch_size(ml_hc_batch b);
+/* Tile shape of the batched FP32 relaxation (results do not depend on it).
+ * By default the first step times every compiled shape on the batch's own
+ * launches and keeps the fastest (the autotuner); set_tile fixes shape
+ * `variant` (-1: tune again at the next step).  ml_hc_batch_tile reports the
+ * shape in use and, when tune_ms is not NULL, the tuning time of each shape
+ * (up to max_shapes); it returns the number of shapes. */
+int ml_hc_batch_set_tile(ml_hc_batch b, int variant);
+int ml_hc_batch_tile(ml_hc_batch b, int* variant, int* tile_x, int* tile_y, int* threads, float* tune_ms,
+                     int max_shapes);
 /* phi[i]: instance i's state (device, FAB layout, 2 components), advanced
  * in place; stats[i] (may be NULL) its solve statistics */
 int ml_hc_batch_step(ml_hc_batch b, double* const phi[], double dt, const ml_solve_opts* opts,

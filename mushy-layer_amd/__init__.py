@@ -117,6 +117,8 @@ SIGNATURES = {
     "ml_hc_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
     "ml_hc_batch_destroy": (C.c_int, [_vp]),
     "ml_hc_batch_size": (C.c_int, [_vp]),
+    "ml_hc_batch_set_tile": (C.c_int, [_vp, C.c_int]),
+    "ml_hc_batch_tile": (C.c_int, [_vp, _ip, _ip, _ip, _ip, C.POINTER(C.c_float), C.c_int]),
     "ml_hc_batch_step": (C.c_int, [_vp, C.POINTER(_vp), C.c_double, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "ml_hc_porosity": (C.c_int, [_vp, _vp, _vp]),
     "ml_hc_assemble_bcoef": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
@@ -416,6 +418,18 @@ class HCBatch:
         self.ctx.check(self.ctx.lib.ml_hc_batch_step(self.h, (_vp * n)(*[x.ptr for x in phis]), dt, C.byref(o), st),
                        "ml_hc_batch_step")
         return list(st)
+
+    def set_tile(self, variant):
+        """Fix the relaxation tile shape (-1: tune again at the next step)."""
+        self.ctx.check(self.ctx.lib.ml_hc_batch_set_tile(self.h, variant), "ml_hc_batch_set_tile")
+
+    def tile(self):
+        """(variant, (tile_x, tile_y, threads), tuning ms per shape)."""
+        v, tx, ty, nt = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        ms = (C.c_float * 16)()
+        n = self.ctx.lib.ml_hc_batch_tile(self.h, C.byref(v), C.byref(tx), C.byref(ty), C.byref(nt), ms, 16)
+        self.ctx.check(n if n < 0 else 0, "ml_hc_batch_tile")
+        return v.value, (tx.value, ty.value, nt.value), [ms[i] for i in range(n)]
 
     def close(self):
         if self.h:

@@ -44,7 +44,8 @@ struct BatchInst {
 // one tiled relaxation launch: the instances taking part and their levels
 struct LinLaunch {
   std::vector<int> inst, level;
-  int blocks = 0;
+  std::vector<int> blocks_v;   // workgroups under each tile shape
+  int blocks = 0;              // ... under the shape in use
   size_t job_off = 0, map_off = 0;
   double bytes = 0;   // algorithmic bytes (every input read once, every output written once)
 };
@@ -88,6 +89,11 @@ struct ml_hc_batch_s {
   std::vector<size_t> cdown_off, cup_off;
   size_t cres_off = 0;
   int cres_blocks = 0;
+  // tile shape of the batched relaxation: picked by timing every shape on
+  // the batch's own launches at the first step, unless set
+  int variant = 0;
+  bool tuned = false;
+  std::vector<double> tune_ms;
   // one iteration's launches captured as a HIP graph
   hipGraphExec_t gexec = nullptr;
   int batch_iters = 4;   // iterations enqueued before the first host check
@@ -257,19 +263,24 @@ int layout(ml_hc_batch b) {
   for (const BatchInst& I : b->inst) b->K = std::max(b->K, I.lc);
   b->down.assign(b->K, LinLaunch{});
   b->up.assign(b->K, LinLaunch{});
-  for (int k = 0; k < b->K; ++k)
+  const int nv = lin_relax_batch_tiles();
+  for (int k = 0; k < b->K; ++k) {
+    for (LinLaunch* Lh : {&b->down[k], &b->up[k]}) Lh->blocks_v.assign(nv, 0);
     for (int i = 0; i < n; ++i) {
       const BatchInst& I = b->inst[i];
       const int l = k - (b->K - I.lc);
       if (l < 0) continue;
-      int ntx, nty;
-      const int nt = lin_relax_tiles2(I.op->lev[l].g, ntx, nty);
       for (LinLaunch* Lh : {&b->down[k], &b->up[k]}) {
         Lh->inst.push_back(i);
         Lh->level.push_back(l);
-        Lh->blocks += nt;
+        for (int v = 0; v < nv; ++v) {
+          int ntx, nty;
+          Lh->blocks_v[v] += lin_relax_batch_tiles_of(v, I.op->lev[l].g, ntx, nty);
+        }
       }
     }
+    for (LinLaunch* Lh : {&b->down[k], &b->up[k]}) Lh->blocks = Lh->blocks_v[b->variant];
+  }
   for (int k = 0; k < b->K; ++k) {
     b->down[k].bytes = launch_bytes(b, b->down[k], true, nullptr);
     b->up[k].bytes = launch_bytes(b, b->up[k], false, nullptr);
@@ -281,16 +292,17 @@ int layout(ml_hc_batch b) {
   b->setup_map_off = take(sizeof(unsigned) * b->setup_blocks);
   b->coarsen_map_off = take(sizeof(unsigned) * b->coarsen_blocks);
   b->res_map_off = take(sizeof(unsigned) * b->res_blocks);
+  auto most = [](const LinLaunch& Lh) { return *std::max_element(Lh.blocks_v.begin(), Lh.blocks_v.end()); };
   for (int k = 0; k < b->K; ++k) {
-    b->down[k].map_off = take(sizeof(unsigned) * b->down[k].blocks);
-    b->up[k].map_off = take(sizeof(unsigned) * b->up[k].blocks);
+    b->down[k].map_off = take(sizeof(unsigned) * most(b->down[k]));
+    b->up[k].map_off = take(sizeof(unsigned) * most(b->up[k]));
   }
   b->arena_bytes = off;
   ml_ctx ctx = b->ctx;
   void* v = nullptr;
   int rc;
   b->cmap_cap = size_t(b->res_blocks);
-  for (int k = 0; k < b->K; ++k) b->cmap_cap += size_t(b->down[k].blocks) + size_t(b->up[k].blocks);
+  for (int k = 0; k < b->K; ++k) b->cmap_cap += size_t(most(b->down[k])) + size_t(most(b->up[k]));
   if ((rc = dalloc(b, sizeof(unsigned) * b->cmap_cap, &v))) return rc;
   b->cmap = static_cast<unsigned*>(v);
   ML_HIP(ctx, hipHostMalloc(&v, sizeof(unsigned) * b->cmap_cap, hipHostMallocDefault));
@@ -314,16 +326,22 @@ int layout(ml_hc_batch b) {
   for (int i = 0; i < n; ++i) fill(b->coarsen_map_off, i, coarsen_chunks[i], p);
   p = 0;
   for (int i = 0; i < n; ++i) fill(b->res_map_off, i, b->inst[i].nres, p);
+  return ML_OK;
+}
+
+// relaxation maps for the tile shape in use (into the pinned mirror)
+void fill_lin_maps(ml_hc_batch b) {
   for (int k = 0; k < b->K; ++k)
     for (LinLaunch* Lh : {&b->down[k], &b->up[k]}) {
-      p = 0;
+      Lh->blocks = Lh->blocks_v[b->variant];
+      unsigned* m = b->hat<unsigned>(Lh->map_off);
+      int p = 0;
       for (size_t j = 0; j < Lh->inst.size(); ++j) {
         int ntx, nty;
-        const int nt = lin_relax_tiles2(b->inst[Lh->inst[j]].op->lev[Lh->level[j]].g, ntx, nty);
-        fill(Lh->map_off, int(j), nt, p);
+        const int nt = lin_relax_batch_tiles_of(b->variant, b->inst[Lh->inst[j]].op->lev[Lh->level[j]].g, ntx, nty);
+        for (int t = 0; t < nt; ++t) m[p++] = (unsigned(j) << kBatchTileBits) | unsigned(t);
       }
     }
-  return ML_OK;
 }
 
 // job tables for the given states, dt and cycle parameters
@@ -424,7 +442,7 @@ int build_tables(ml_hc_batch b, double* const* phi, double dt, const ml_solve_op
         J.bc = I.op->bc;
         J.alpha = float(I.op->alpha);
         J.beta = float(I.op->beta);
-        lin_relax_tiles2(J.L.g, J.ntx, J.nty);
+        lin_relax_batch_tiles_of(b->variant, J.L.g, J.ntx, J.nty);
         if (pass == 0) {
           J.ein = I.lin[l].e;
           J.eout = I.ealt[l];
@@ -445,6 +463,7 @@ int build_tables(ml_hc_batch b, double* const* phi, double dt, const ml_solve_op
       }
     }
   }
+  fill_lin_maps(b);
   ML_HIP(ctx, hipMemcpyAsync(b->arena, b->harena, b->arena_bytes, hipMemcpyHostToDevice, ctx->stream));
   b->key_phi.assign(phi, phi + n);
   b->key_dt = dt;
@@ -465,7 +484,7 @@ int build_compact(ml_hc_batch b, const std::vector<char>& run) {
     for (size_t j = 0; j < Lh.inst.size(); ++j) {
       if (!run[Lh.inst[j]]) continue;
       int ntx, nty;
-      const int nt = lin_relax_tiles2(b->inst[Lh.inst[j]].op->lev[Lh.level[j]].g, ntx, nty);
+      const int nt = lin_relax_batch_tiles_of(b->variant, b->inst[Lh.inst[j]].op->lev[Lh.level[j]].g, ntx, nty);
       for (int t = 0; t < nt; ++t) b->hcmap[pos++] = (unsigned(j) << kBatchTileBits) | unsigned(t);
       blocks += nt;
     }
@@ -498,14 +517,14 @@ hipError_t enqueue_compact(ml_hc_batch b, const ml_solve_opts& o, hipStream_t s,
   hipError_t e;
   for (int k = 0; k < b->K; ++k)
     if ((e = lin_relax_batch(b->at<BatchLinJob>(b->down[k].job_off), b->cmap + b->cdown_off[k], b->cdown_blocks[k],
-                             2 * o.nu1, true, true, s, tm, b->cdown_bytes[k])) != hipSuccess)
+                             2 * o.nu1, true, true, b->variant, s, tm, b->cdown_bytes[k])) != hipSuccess)
       return e;
   if ((e = lin_coarse_vcycle_batch(b->at<LinDesc>(b->desc_off), int(b->inst.size()), b->coarse_lds, s)) !=
       hipSuccess)
     return e;
   for (int k = b->K - 1; k >= 0; --k)
     if ((e = lin_relax_batch(b->at<BatchLinJob>(b->up[k].job_off), b->cmap + b->cup_off[k], b->cup_blocks[k],
-                             2 * o.nu2, false, false, s, tm, b->cup_bytes[k])) != hipSuccess)
+                             2 * o.nu2, false, false, b->variant, s, tm, b->cup_bytes[k])) != hipSuccess)
       return e;
   return hc_residual_jacobian_batch(b->at<BatchResJob>(b->res_off), b->cmap + b->cres_off, b->cres_blocks, false, s);
 }
@@ -517,7 +536,7 @@ hipError_t enqueue_iteration(ml_hc_batch b, const ml_solve_opts& o, hipStream_t 
   for (int k = 0; k < b->K; ++k) {
     const LinLaunch& Lh = b->down[k];
     if ((e = lin_relax_batch(b->at<BatchLinJob>(Lh.job_off), b->at<unsigned>(Lh.map_off), Lh.blocks, 2 * o.nu1, true,
-                             true, s, tm, Lh.bytes)) != hipSuccess)
+                             true, b->variant, s, tm, Lh.bytes)) != hipSuccess)
       return e;
   }
   if ((e = lin_coarse_vcycle_batch(b->at<LinDesc>(b->desc_off), int(b->inst.size()), b->coarse_lds, s)) !=
@@ -526,11 +545,59 @@ hipError_t enqueue_iteration(ml_hc_batch b, const ml_solve_opts& o, hipStream_t 
   for (int k = b->K - 1; k >= 0; --k) {
     const LinLaunch& Lh = b->up[k];
     if ((e = lin_relax_batch(b->at<BatchLinJob>(Lh.job_off), b->at<unsigned>(Lh.map_off), Lh.blocks, 2 * o.nu2, false,
-                             false, s, tm, Lh.bytes)) != hipSuccess)
+                             false, b->variant, s, tm, Lh.bytes)) != hipSuccess)
       return e;
   }
   return hc_residual_jacobian_batch(b->at<BatchResJob>(b->res_off), b->at<unsigned>(b->res_map_off), b->res_blocks,
                                     false, s);
+}
+
+// The autotuner: every tile shape of the batched relaxation runs the
+// batch's own down- and up-sweep launches (one warm-up, then kTuneReps
+// timed with HIP events); the fastest shape is kept.  The relaxations only
+// read and write correction buffers that every step recomputes from
+// scratch, so the trial runs leave no trace in the results.
+constexpr int kTuneReps = 3;
+
+int autotune(ml_hc_batch b, double* const* phi, double dt, const ml_solve_opts& o) {
+  ml_ctx ctx = b->ctx;
+  hipStream_t s = ctx->stream;
+  const int nv = lin_relax_batch_tiles();
+  hipEvent_t e0, e1;
+  ML_HIP(ctx, hipEventCreate(&e0));
+  ML_HIP(ctx, hipEventCreate(&e1));
+  b->tune_ms.assign(nv, 0.0);
+  int best = 0, rc = ML_OK;
+  for (int v = 0; v < nv && rc == ML_OK; ++v) {
+    b->variant = v;
+    if ((rc = build_tables(b, phi, dt, o))) break;
+    for (int rep = 0; rep <= kTuneReps && rc == ML_OK; ++rep) {
+      if (rep == 1 && hipEventRecord(e0, s) != hipSuccess) rc = fail(ctx, ML_EHIP, "autotune event");
+      for (int k = 0; k < b->K && rc == ML_OK; ++k)
+        if (lin_relax_batch(b->at<BatchLinJob>(b->down[k].job_off), b->at<unsigned>(b->down[k].map_off),
+                            b->down[k].blocks, 2 * o.nu1, true, true, v, s) != hipSuccess)
+          rc = fail(ctx, ML_EHIP, "autotune launch");
+      for (int k = b->K - 1; k >= 0 && rc == ML_OK; --k)
+        if (lin_relax_batch(b->at<BatchLinJob>(b->up[k].job_off), b->at<unsigned>(b->up[k].map_off), b->up[k].blocks,
+                            2 * o.nu2, false, false, v, s) != hipSuccess)
+          rc = fail(ctx, ML_EHIP, "autotune launch");
+    }
+    if (rc != ML_OK) break;
+    float ms = 0;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = fail(ctx, ML_EHIP, "autotune timing");
+      break;
+    }
+    b->tune_ms[v] = ms / kTuneReps;
+    if (b->tune_ms[v] < b->tune_ms[best]) best = v;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != ML_OK) return rc;
+  b->variant = best;
+  b->tuned = true;
+  return build_tables(b, phi, dt, o);
 }
 
 int capture(ml_hc_batch b, const ml_solve_opts& o) {
@@ -668,6 +735,29 @@ int ml_hc_batch_destroy(ml_hc_batch b) {
 
 int ml_hc_batch_size(ml_hc_batch b) { return b ? int(b->inst.size()) : ML_EINVAL; }
 
+int ml_hc_batch_set_tile(ml_hc_batch b, int variant) {
+  if (!b || variant < -1 || variant >= lin_relax_batch_tiles()) return ML_EINVAL;
+  if (variant < 0) {
+    b->tuned = false;
+  } else {
+    b->variant = variant;
+    b->tuned = true;
+  }
+  b->tables_ok = false;
+  return ML_OK;
+}
+
+int ml_hc_batch_tile(ml_hc_batch b, int* variant, int* tile_x, int* tile_y, int* threads, float* tune_ms,
+                     int max_shapes) {
+  if (!b || !variant || !tile_x || !tile_y || !threads) return ML_EINVAL;
+  *variant = b->variant;
+  lin_relax_batch_tile_shape(b->variant, tile_x, tile_y, threads);
+  if (tune_ms)
+    for (int v = 0; v < max_shapes && v < lin_relax_batch_tiles(); ++v)
+      tune_ms[v] = v < int(b->tune_ms.size()) ? float(b->tune_ms[v]) : 0.0f;
+  return lin_relax_batch_tiles();
+}
+
 int ml_hc_batch_step(ml_hc_batch b, double* const phi[], double dt, const ml_solve_opts* opts, ml_solve_stats stats[]) {
   if (!b || !phi) return ML_EINVAL;
   ml_ctx ctx = b->ctx;
@@ -692,7 +782,11 @@ int ml_hc_batch_step(ml_hc_batch b, double* const phi[], double dt, const ml_sol
   bool same = b->tables_ok && b->key_dt == dt && b->key_nu1 == o.nu1 && b->key_nu2 == o.nu2 &&
               b->key_bottom == o.bottom;
   for (int i = 0; same && i < n; ++i) same = b->key_phi[i] == phi[i];
-  if (!same && (rc = build_tables(b, phi, dt, o))) return rc;
+  if (!b->tuned) {
+    if ((rc = autotune(b, phi, dt, o))) return rc;
+  } else if (!same && (rc = build_tables(b, phi, dt, o))) {
+    return rc;
+  }
   for (BatchInst& I : b->inst) {
     I.op->alpha = 1.0;
     I.op->beta = -dt;

@@ -1299,23 +1299,65 @@ __global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_batch_kernel
 }  // namespace
 
 int lin_relax_tiles2(const Geom& g, int& ntx, int& nty) { return tiles2<LinRelaxTile2::TX, LinRelaxTile2::TY>(g, ntx, nty); }
+
+// Tile shapes of the batched relaxation (TX, TY, threads); every shape
+// stages a halo of up to 8 cells, so a launch takes up to 8 passes.  The
+// results do not depend on the shape.
+namespace {
+struct BatchTile {
+  int TX, TY, NT;
+};
+constexpr BatchTile kBatchTiles[] = {{16, 16, 1024}, {32, 16, 512}, {32, 32, 1024}, {64, 16, 1024}, {32, 8, 256}};
+constexpr int kNumBatchTiles = int(sizeof(kBatchTiles) / sizeof(kBatchTiles[0]));
+
+template <int V>
+hipError_t launch_batch_variant(const BatchLinJob* jobs, const unsigned* map, int nblocks, int npass, bool zero_init,
+                                size_t lds, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  constexpr BatchTile T = kBatchTiles[V];
+  hipExtLaunchKernelGGL((lin_relax_batch_kernel<2, T.TX, T.TY, 1, 8, T.NT>), dim3(nblocks), dim3(T.NT), lds, s, e0, e1,
+                        0, jobs, map, npass, zero_init ? 1 : 0);
+  return hipGetLastError();
+}
+}  // namespace
+
+int lin_relax_batch_tiles() { return kNumBatchTiles; }
+
+int lin_relax_batch_tile_shape(int v, int* tx, int* ty, int* nt) {
+  if (v < 0 || v >= kNumBatchTiles) return -1;
+  *tx = kBatchTiles[v].TX;
+  *ty = kBatchTiles[v].TY;
+  *nt = kBatchTiles[v].NT;
+  return 0;
+}
+
+int lin_relax_batch_tiles_of(int v, const Geom& g, int& ntx, int& nty) {
+  const BatchTile& T = kBatchTiles[v];
+  ntx = (g.n[0] + T.TX - 1) / T.TX;
+  nty = (g.n[1] + T.TY - 1) / T.TY;
+  return ntx * nty;
+}
 int lin_residual_tiles2(const Geom& g, int& ntx, int& nty) { return lin_tiles<2>(g, ntx, nty); }
 long long lin_deep_tiles_max2() { return 1024; }
 
 hipError_t lin_relax_batch(const BatchLinJob* jobs, const unsigned* map, int nblocks, int npass, bool zero_init,
-                           bool restrict_, hipStream_t s, const LaunchTimer* tm, double bytes) {
-  using DR = LinRelaxTile2;
+                           bool restrict_, int variant, hipStream_t s, const LaunchTimer* tm, double bytes) {
   const int h = npass + (restrict_ ? 1 : 0);
-  if (npass < 1 || h > 8) return hipErrorInvalidValue;
+  if (npass < 1 || h > 8 || variant < 0 || variant >= kNumBatchTiles) return hipErrorInvalidValue;
   if (nblocks == 0) return hipSuccess;
+  const BatchTile& T = kBatchTiles[variant];
   const size_t lds =
-      (8 * size_t(DR::TX + 2 * h) * (DR::TY + 2 * h) + (restrict_ ? 2 * DR::TX * DR::TY : 0)) * sizeof(float);
+      (8 * size_t(T.TX + 2 * h) * (T.TY + 2 * h) + (restrict_ ? 2 * T.TX * T.TY : 0)) * sizeof(float);
   const bool timed = tm && tm->start && *tm->n < tm->cap;
   if (tm && tm->start && !timed) ++*tm->missed;
-  hipExtLaunchKernelGGL((lin_relax_batch_kernel<2, DR::TX, DR::TY, DR::TZ, 8, DR::NT>), dim3(nblocks), dim3(DR::NT),
-                        lds, s, timed ? tm->start[*tm->n] : nullptr, timed ? tm->stop[*tm->n] : nullptr, 0, jobs, map,
-                        npass, zero_init ? 1 : 0);
-  const hipError_t e = hipGetLastError();
+  hipEvent_t e0 = timed ? tm->start[*tm->n] : nullptr, e1 = timed ? tm->stop[*tm->n] : nullptr;
+  hipError_t e;
+  switch (variant) {
+    case 0: e = launch_batch_variant<0>(jobs, map, nblocks, npass, zero_init, lds, s, e0, e1); break;
+    case 1: e = launch_batch_variant<1>(jobs, map, nblocks, npass, zero_init, lds, s, e0, e1); break;
+    case 2: e = launch_batch_variant<2>(jobs, map, nblocks, npass, zero_init, lds, s, e0, e1); break;
+    case 3: e = launch_batch_variant<3>(jobs, map, nblocks, npass, zero_init, lds, s, e0, e1); break;
+    default: e = launch_batch_variant<4>(jobs, map, nblocks, npass, zero_init, lds, s, e0, e1); break;
+  }
   if (e == hipSuccess && timed) {
     *tm->bytes += bytes;
     ++*tm->n;

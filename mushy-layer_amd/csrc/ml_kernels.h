@@ -436,9 +436,15 @@ int lin_relax_tiles2(const Geom& g, int& ntx, int& nty);
 int lin_residual_tiles2(const Geom& g, int& ntx, int& nty);
 // level-0 tiles above which a single solve changes its relaxation tiles
 long long lin_deep_tiles_max2();
+// tile shapes of the batched relaxation: count, shape of variant v, and
+// the tiles of level g under it
+int lin_relax_batch_tiles();
+int lin_relax_batch_tile_shape(int v, int* tx, int* ty, int* nt);
+int lin_relax_batch_tiles_of(int v, const Geom& g, int& ntx, int& nty);
 // tm (may be null): launch timing, `bytes` the launch's algorithmic bytes
 hipError_t lin_relax_batch(const BatchLinJob* jobs, const unsigned* map, int nblocks, int npass, bool zero_init,
-                           bool restrict_, hipStream_t s, const LaunchTimer* tm = nullptr, double bytes = 0);
+                           bool restrict_, int variant, hipStream_t s, const LaunchTimer* tm = nullptr,
+                           double bytes = 0);
 // initial: residual of the state pb[0] into slots[0], no correction, no
 // stopping rule; else iteration ctl->iter + 1 with the stopping rule
 hipError_t hc_residual_jacobian_batch(const BatchResJob* jobs, const unsigned* map, int nblocks, bool initial,

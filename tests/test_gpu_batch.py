@@ -79,6 +79,36 @@ def test_batch_step_matches_single_instance_steps(ml, gpu_ctx, wl, flags):
         op.close()
 
 
+def test_batch_tile_shapes_give_identical_steps(ml, gpu_ctx, wl):
+    """Every tile shape of the batched relaxation gives the single-instance
+    steps bit for bit; the autotuner times them all and keeps the fastest."""
+    cases = _instances(wl)
+    dt = cases[0]["dt"]
+    opts = ml.default_opts(max_iter=100, eps=EPS, hang=-1.0, precision=ml.PRECISION_FP32_REFINED)
+    ops = [HC.gpu_setup(ml, gpu_ctx, c) for c in cases]
+    batch = ml.HCBatch(gpu_ctx, ops)
+    ref = []
+    for c in cases:
+        x = ml.DeviceArray(gpu_ctx, c["phi"].shape, host=c["phi"])
+        _single_step(ml, gpu_ctx, c, x, opts)
+        ref.append(x.get())
+    nshapes = len(batch.tile()[2])
+    assert nshapes >= 2
+    for v in list(range(nshapes)) + [-1]:
+        batch.set_tile(v)
+        bphi = [ml.DeviceArray(gpu_ctx, c["phi"].shape, host=c["phi"]) for c in cases]
+        print("tile shape %d" % v, flush=True)
+        batch.step(bphi, dt, opts)
+        for i in range(len(cases)):
+            assert HC.bitwise_equal(bphi[i].get(), ref[i]), (v, i)
+    chosen, shape, ms = batch.tile()
+    print("autotuned shape %d %s, ms per shape %s" % (chosen, shape, ms))
+    assert all(t > 0 for t in ms) and ms[chosen] == min(ms)
+    batch.close()
+    for op in ops:
+        op.close()
+
+
 def test_batch_compacted_stragglers_match_single_instance_steps(ml, gpu_ctx, wl):
     """15 small instances converge well before one 512x256 instance with a
     large concentration ratio; once at most 1/8 of the batch runs, later
