@@ -58,6 +58,8 @@ def parse():
                     help="grid of the smoother bandwidth measurement (0: skip)")
     ap.add_argument("--batch", type=int, default=0,
                     help="> 0: the batched form on this many independent instances (64^2 .. 512x256, 8:4:2:1)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="--batch: skip the single-instance comparison passes (for profiler runs)")
     ap.add_argument("--no-lookahead", action="store_true",
                     help="solves enqueue one iteration at a time (no launches cut short; for counter runs)")
     args = ap.parse_args()
@@ -360,6 +362,63 @@ def vc_relax_roofline(ml, ctx, stepper, steps):
             "ms_per_step_instrumented": round(wall / steps * 1e3, 4)}
 
 
+def batch_compare(ml, ctx, cases, ops, batch, phis, reset, opts, args):
+    """One step of every instance through the batch, then through the
+    single-instance path (FP32-refined, timed; FP64, the reference state of
+    the error report)."""
+    cells = int(sum(np.prod(c["n"]) for c in cases))
+    reset()
+    ctx.sync()
+    batch.step(phis, cases[0]["dt"], opts)
+    after = [x.get() for x in phis]
+    single = {}
+    work = [(ml.DeviceArray(ctx, (1,) + x.shape[1:]), [ml.DeviceArray(ctx, ml.face_shape(c["n"], d, 2)) for d in range(2)],
+             ml.DeviceArray(ctx, x.shape)) for c, x in zip(cases, phis)]
+    for prec, name in ((ml.PRECISION_FP32_REFINED, "fp32r"), (ml.PRECISION_FP64, "fp64"),
+                       (ml.PRECISION_FP32_REFINED, "fp32r"), (ml.PRECISION_FP64, "fp64")):
+        # (each form twice: the first pass sets up the operators' engines)
+        reset()
+        ctx.sync()
+        o = ml.default_opts(max_iter=args.max_iter, precision=prec)
+        t0 = time.perf_counter()
+        sts = []
+        for op, c, x, (chi, b, old) in zip(ops, cases, phis, work):
+            op.porosity(x, chi)
+            op.assemble_bcoef(chi, b)
+            op.set_coefs(None, b, 1.0, -c["dt"])
+            ctx.check(ctx.lib.ml_buf_copy(ctx.h, old.ptr, x.ptr, x.nbytes), "copy")
+            sts.append(op.solve_be(x, old, c["dt"], opts=o).exit_status)
+        ctx.sync()
+        single[name] = (time.perf_counter() - t0, [x.get() for x in phis], sts)
+    # per-field difference from the FP64 step, over the instances whose two
+    # solves both met the tolerance (status 1 or 8)
+    ok = [bool(s1 & 9) and bool(s2 & 9) for s1, s2 in zip(single["fp32r"][2], single["fp64"][2])]
+    err = {"H": 0.0, "Theta": 0.0}
+    for c, a, r, good in zip(cases, after, single["fp64"][1], ok):
+        if not good:
+            continue
+        va, vr = ml.valid(c["n"], a), ml.valid(c["n"], r)
+        for k, comp in (("H", 0), ("Theta", 1)):
+            err[k] = max(err[k], float(np.max(np.abs(va[comp] - vr[comp])) / np.max(np.abs(vr[comp]))))
+    chi_err = 0.0
+    for op, c, a, r, good in zip(ops, cases, after, single["fp64"][1], ok):
+        if not good:
+            continue
+        xs = []
+        for s_ in (a, r):
+            d = ml.DeviceArray(ctx, s_.shape, host=s_)
+            chi = ml.DeviceArray(ctx, (1,) + s_.shape[1:])
+            op.porosity(d, chi)
+            xs.append(ml.valid(c["n"], chi.get()))
+        chi_err = max(chi_err, float(np.max(np.abs(xs[0] - xs[1]))))
+    err["porosity_abs"] = chi_err
+    err["instances_compared"] = int(sum(ok))
+    same = all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(after, single["fp32r"][1]))
+    return {"single_instance_fp32r_value": round(cells / single["fp32r"][0], 2),
+            "single_instance_fp64_value": round(cells / single["fp64"][0], 2),
+            "batched_equals_single_bitwise": bool(same), "error_vs_fp64": err}
+
+
 def run_batch(ml, wl, ctx, args):
     """The batched form: `args.batch` independent mushy-layer instances
     (grid sizes 64^2, 128^2, 256^2, 512x256 drawn 8:4:2:1 by seed; St, CR,
@@ -402,53 +461,8 @@ def run_batch(ml, wl, ctx, args):
 
     # one step of every instance through the single-instance path, FP32-
     # refined (timed) and FP64 (the reference state of the error report)
-    reset()
-    ctx.sync()
-    batch.step(phis, cases[0]["dt"], opts)
-    after = [x.get() for x in phis]
-    single = {}
-    work = [(ml.DeviceArray(ctx, (1,) + x.shape[1:]), [ml.DeviceArray(ctx, ml.face_shape(c["n"], d, 2)) for d in range(2)],
-             ml.DeviceArray(ctx, x.shape)) for c, x in zip(cases, phis)]
-    for prec, name in ((ml.PRECISION_FP32_REFINED, "fp32r"), (ml.PRECISION_FP64, "fp64"),
-                       (ml.PRECISION_FP32_REFINED, "fp32r"), (ml.PRECISION_FP64, "fp64")):
-        # (each form twice: the first pass sets up the operators' engines)
-        reset()
-        ctx.sync()
-        o = ml.default_opts(max_iter=args.max_iter, precision=prec)
-        t0 = time.perf_counter()
-        sts = []
-        for op, c, x, (chi, b, old) in zip(ops, cases, phis, work):
-            op.porosity(x, chi)
-            op.assemble_bcoef(chi, b)
-            op.set_coefs(None, b, 1.0, -c["dt"])
-            ctx.check(ctx.lib.ml_buf_copy(ctx.h, old.ptr, x.ptr, x.nbytes), "copy")
-            sts.append(op.solve_be(x, old, c["dt"], opts=o).exit_status)
-        ctx.sync()
-        single[name] = (time.perf_counter() - t0, [x.get() for x in phis], sts)
-    same = all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(after, single["fp32r"][1]))
-    # per-field difference from the FP64 step, over the instances whose two
-    # solves both met the tolerance (status 1 or 8)
-    ok = [bool(s1 & 9) and bool(s2 & 9) for s1, s2 in zip(single["fp32r"][2], single["fp64"][2])]
-    err = {"H": 0.0, "Theta": 0.0}
-    for c, a, r, good in zip(cases, after, single["fp64"][1], ok):
-        if not good:
-            continue
-        va, vr = ml.valid(c["n"], a), ml.valid(c["n"], r)
-        for k, comp in (("H", 0), ("Theta", 1)):
-            err[k] = max(err[k], float(np.max(np.abs(va[comp] - vr[comp])) / np.max(np.abs(vr[comp]))))
-    chi_err = 0.0
-    for op, c, a, r, good in zip(ops, cases, after, single["fp64"][1], ok):
-        if not good:
-            continue
-        xs = []
-        for s_ in (a, r):
-            d = ml.DeviceArray(ctx, s_.shape, host=s_)
-            chi = ml.DeviceArray(ctx, (1,) + s_.shape[1:])
-            op.porosity(d, chi)
-            xs.append(ml.valid(c["n"], chi.get()))
-        chi_err = max(chi_err, float(np.max(np.abs(xs[0] - xs[1]))))
-    err["porosity_abs"] = chi_err
-    err["instances_compared"] = int(sum(ok))
+    compare = None if args.no_compare else batch_compare(ml, ctx, cases, ops, batch, phis, reset, opts, args)
+
 
     # roofline of the batched relaxation: a pass with every launch timed
     reset()
@@ -465,6 +479,7 @@ def run_batch(ml, wl, ctx, args):
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
             "avg_launch_us": round(ms / n1 * 1e3, 3), "bytes_per_launch": round(nb / n1, 1),
             "launches_timed": n1, "launches_untimed": missed}
+    calibration_copy(ml, ctx)
     sizes = {}
     for c in cases:
         k = "x".join(map(str, c["n"]))
@@ -481,13 +496,14 @@ def run_batch(ml, wl, ctx, args):
            "status_counts": {str(k): sum(1 for st in stats for x in st if x.exit_status == k) for k in status},
            "cells": cells, "iterations_per_solve": round(float(np.mean(iters)), 2),
            "max_iterations": int(max(iters)), "exit_status": status,
-           "single_instance_fp32r_value": round(cells / single["fp32r"][0], 2),
-           "single_instance_fp64_value": round(cells / single["fp64"][0], 2),
-           "batched_over_single_fp32r": round((cells * steps / wall) / (cells / single["fp32r"][0]), 3),
-           "batched_equals_single_bitwise": bool(same),
+           "single_instance_fp32r_value": compare and compare["single_instance_fp32r_value"],
+           "single_instance_fp64_value": compare and compare["single_instance_fp64_value"],
+           "batched_over_single_fp32r": compare and round((cells * steps / wall) /
+                                                          compare["single_instance_fp32r_value"], 3),
+           "batched_equals_single_bitwise": compare and compare["batched_equals_single_bitwise"],
            "relax_tile_autotune": {"chosen": "%dx%d/%d" % tshape,
                                    "ms_per_sweep_set": {"shape%d" % v: round(t, 4) for v, t in enumerate(tms)}},
-           "error_vs_fp64": err}
+           "error_vs_fp64": compare and compare["error_vs_fp64"]}
     out = {
         "metric": METRIC, "value": round(cells * steps / wall, 2), "unit": "cell-updates/s", "n_gpus": 1,
         "steps": steps, "warmup": args.warmup, "ms_per_step": round(wall / steps * 1e3, 4),
