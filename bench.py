@@ -58,6 +58,9 @@ def parse():
                     help="grid of the smoother bandwidth measurement (0: skip)")
     ap.add_argument("--batch", type=int, default=0,
                     help="> 0: the batched form on this many independent instances (64^2 .. 512x256, 8:4:2:1)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="the adjoint workload: 128^2 heat-flux solidification, 50 steps, gradients and a fit")
+    ap.add_argument("--fit-iters", type=int, default=6)
     ap.add_argument("--no-compare", action="store_true",
                     help="--batch: skip the single-instance comparison passes (for profiler runs)")
     ap.add_argument("--no-lookahead", action="store_true",
@@ -513,6 +516,106 @@ def run_batch(ml, wl, ctx, args):
     print(json.dumps(out))
 
 
+def run_adjoint(ml, wl, ctx, args):
+    """The adjoint workload: 128x128 diffusive solidification cooled through
+    the bottom by a heat-flux profile, 50 backward-Euler steps of dt = 5e-4
+    (FP64 forward solves), the loss the mean squared misfit of the final
+    Theta and porosity fields against a target made by a seeded parameter
+    set.  Reported: forward and forward+adjoint times, the adjoint's
+    directional derivatives against central differences (h = 1e-6 relative),
+    and a few steepest-descent steps of a fit of the flux profile and St."""
+    import torch
+    adj = importlib.import_module("mushy-layer_amd.adjoint")
+    case = wl.heat_flux(n=(128, 128), steps=50, dt=5e-4)
+    n = case["n"]
+    nx = n[0]
+    model = adj.HeatFluxAdjoint(ctx, case, checkpoint=10)
+    F_true = wl.flux_profile(nx, F0=12.0, modes=((1, 0.3, 0.0), (2, 0.1, 1.0)))
+    St_true = 5.5
+    tphi, tchi, cps = model.forward(F_true, St_true)
+    for x in cps.values():
+        x.free()
+
+    def loss_only(F, St):
+        phi, chi, c = model.forward(F, St)
+        for x in c.values():
+            x.free()
+        return float(np.mean((phi[1, :, 1:-1, 1:-1] - tphi[1, :, 1:-1, 1:-1]) ** 2) +
+                     np.mean((chi[0, :, 1:-1, 1:-1] - tchi[0, :, 1:-1, 1:-1]) ** 2))
+
+    def grads(F, St):
+        flux = torch.tensor(F, dtype=torch.float64, requires_grad=True)
+        perm = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+        st = torch.tensor(St, dtype=torch.float64, requires_grad=True)
+        phi, chi = adj.ImplicitSteps.apply(flux, perm, st, model)
+        L = adj.field_loss(n, phi, chi, tphi, tchi)
+        L.backward()
+        return float(L.detach()), flux.grad.numpy().copy(), float(perm.grad), float(st.grad)
+
+    F = wl.flux_profile(nx, F0=10.0, modes=())
+    St = 5.0
+    loss_only(F, St)
+    ctx.sync()
+    t0 = time.perf_counter()
+    L0 = loss_only(F, St)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    L, gF, gperm, gSt = grads(F, St)
+    t_fb = time.perf_counter() - t0
+    its = [a for a, _ in model.adj_stats]
+    rng = np.random.default_rng(11)
+    v = rng.standard_normal(nx)
+    v /= np.linalg.norm(v)
+    h = 1e-6 * np.max(np.abs(F))
+    fd_v = (loss_only(F + h * v, St) - loss_only(F - h * v, St)) / (2 * h)
+    h = 1e-6 * St
+    fd_s = (loss_only(F, St + h) - loss_only(F, St - h)) / (2 * h)
+    check = {"flux_direction": {"adjoint": float(gF @ v), "central_difference": fd_v,
+                                "rel_err": abs(float(gF @ v) - fd_v) / abs(fd_v)},
+             "stefan": {"adjoint": gSt, "central_difference": fd_s, "rel_err": abs(gSt - fd_s) / abs(fd_s)},
+             "permeability_prefactor": gperm}
+    # steepest descent on (F / 10, St / 5) with a halving line search
+    # (steps along the normalised gradient, the length halved until the loss
+    # decreases and doubled after each accepted step)
+    hist = [L]
+    x = np.concatenate([F / 10.0, [St / 5.0]])
+    step = 0.05
+    for _ in range(args.fit_iters):
+        Lc, gF, _, gS = grads(x[:-1] * 10.0, x[-1] * 5.0)
+        g = np.concatenate([gF * 10.0, [gS * 5.0]])
+        d = g / max(1e-300, float(np.linalg.norm(g)))
+        for _k in range(12):
+            xt = x - step * d
+            Lt = loss_only(xt[:-1] * 10.0, xt[-1] * 5.0)
+            if Lt < Lc:
+                x = xt
+                hist.append(Lt)
+                step *= 2.0
+                break
+            step *= 0.5
+        else:
+            break
+    F_fit, St_fit = x[:-1] * 10.0, x[-1] * 5.0
+    model.close()
+    cells = nx * n[1]
+    cfg = {"workload": "adjoint: 128x128 diffusive solidification cooled by a bottom heat-flux profile, 50 FP64 "
+                       "backward-Euler steps (dt 5e-4), checkpoints every 10 steps; loss = mean squared misfit of the "
+                       "final Theta and porosity fields against a seeded target (flux 12(1 + 0.3 sin 2 pi x + "
+                       "0.1 sin(4 pi x + 1)), St 5.5); gradients in the 128-point flux, St and the permeability "
+                       "prefactor",
+           "form": "fp64 forward + hand-written adjoint", "forward_s": round(t_fwd, 4),
+           "forward_and_adjoint_s": round(t_fb, 4), "adjoint_solve_iterations_mean": round(float(np.mean(its)), 1),
+           "gradient_check": check, "fit_loss_history": hist,
+           "fit_flux_rel_err": float(np.linalg.norm(F_fit - F_true) / np.linalg.norm(F_true)),
+           "fit_stefan": float(St_fit), "initial_loss": L0}
+    out = {"metric": "cell-updates/s per implicit step with its adjoint (forward + backward, FP64)",
+           "value": round(cells * case["steps"] / t_fb, 2), "unit": "cell-updates/s", "n_gpus": 1,
+           "steps": case["steps"], "warmup": 1, "ms_per_step": round(t_fb / case["steps"] * 1e3, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic", "config": cfg, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -524,6 +627,9 @@ def main():
     wl = importlib.import_module("mushy-layer_amd.workloads")
     if args.batch > 0:
         run_batch(ml, wl, ml.Context(0), args)
+        return
+    if args.adjoint:
+        run_adjoint(ml, wl, ml.Context(0), args)
         return
     dim = args.dim
     nn = args.n or (256 if dim == 2 else 128)

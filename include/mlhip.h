@@ -300,6 +300,35 @@ int ml_hc_batch_tile(ml_hc_batch b, int* variant, int* tile_x, int* tile_y, int*
 int ml_hc_batch_step(ml_hc_batch b, double* const phi[], double dt, const ml_solve_opts* opts,
                      ml_solve_stats stats[]);
 
+/* ---- adjoint of the implicit step (no counterpart in the reference) ----
+ * Differentiable time stepping of the no-flow (H, Theta) problem with a
+ * bottom heat-flux profile F (one value per bottom face, n[0] values; 2D,
+ * conductivity ratio 1, aCoef == 1).  The flux enters the forward step as
+ * the source that is exactly the Neumann ghost theta_g = theta_1 - dx F
+ * (BCutil/BCFuncF.ChF:80-88) on theta, the bottom BC of theta itself being
+ * homogeneous Neumann:
+ *   ml_hc_flux_source: src = 0 except src_H = b_H,bottom face F / dx in the
+ *   bottom cells, from the operator's current level-0 bCoef; then
+ *   ml_hc_solve_be(op, phi, phi_old, src, dt, ...) is the forward step.
+ * ml_hc_adjoint_step: given g_new = dL/dphi^n of a loss L of the state phi^n
+ * reached by the forward step from phi^{n-1} = phi_old (with bCoef from the
+ * porosity of phi_old, as the step assembled it), writes g_old = dL/dphi^{n-1}
+ * through this step (valid cells; ghosts untouched), adds dL/dF to g_flux
+ * (may be NULL) and dL/dSt to *g_stefan (may be NULL).  Solves J^T lambda =
+ * g_new (J the step's Jacobian with the linearised closure) by BiCGStab with
+ * the 2x2 block-diagonal preconditioner to opts->eps relative (max_iter
+ * iterations); stats: status 1 converged / 2 iteration limit / 4 breakdown,
+ * iterations, |g_new|_2, final residual.  Overwrites the operator's level-0
+ * bCoef.
+ * ml_hc_porosity_vjp: g_phi += g_chi dchi/dphi (valid cells) and, when
+ * g_stefan is not NULL, *g_stefan += sum g_chi dchi/dSt (the porosity's own
+ * dependence on St), for losses on the porosity. */
+int ml_hc_flux_source(ml_hc_op op, const double* flux, double* src);
+int ml_hc_adjoint_step(ml_hc_op op, const double* phi_new, const double* phi_old, double dt,
+                       const double* g_new, double* g_old, double* g_flux, double* g_stefan,
+                       const ml_solve_opts* opts, ml_solve_stats* stats);
+int ml_hc_porosity_vjp(ml_hc_op op, const double* phi, const double* g_chi, double* g_phi, double* g_stefan);
+
 /* ---- state update and coefficient assembly ----------------------------- */
 /* porosity of every valid cell (CALCULATEPOROSITY_MULTICOMP,
  * src/EnthalpyVariablesF.ChF:56-94, as in updateEnthalpyVariables

@@ -120,6 +120,10 @@ SIGNATURES = {
     "ml_hc_batch_set_tile": (C.c_int, [_vp, C.c_int]),
     "ml_hc_batch_tile": (C.c_int, [_vp, _ip, _ip, _ip, _ip, C.POINTER(C.c_float), C.c_int]),
     "ml_hc_batch_step": (C.c_int, [_vp, C.POINTER(_vp), C.c_double, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "ml_hc_flux_source": (C.c_int, [_vp, _vp, _vp]),
+    "ml_hc_adjoint_step": (C.c_int, [_vp, _vp, _vp, C.c_double, _vp, _vp, _vp, _dp, C.POINTER(SolveOpts),
+                                     C.POINTER(SolveStats)]),
+    "ml_hc_porosity_vjp": (C.c_int, [_vp, _vp, _vp, _vp, _dp]),
     "ml_hc_porosity": (C.c_int, [_vp, _vp, _vp]),
     "ml_hc_assemble_bcoef": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
     "ml_norm_max": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _vp, _dp]),
@@ -378,6 +382,29 @@ class HCOp:
         o = opts or default_opts()
         self.ctx.check(self.ctx.lib.ml_hc_solve(self.h, phi.ptr, rhs.ptr, C.byref(o), C.byref(st)), "ml_hc_solve")
         return st
+
+    def flux_source(self, flux, src):
+        """src = the bottom heat flux's source (theta component of the bottom
+        cells), from the current level-0 bCoef."""
+        self.ctx.check(self.ctx.lib.ml_hc_flux_source(self.h, flux.ptr, src.ptr), "ml_hc_flux_source")
+
+    def adjoint_step(self, phi_new, phi_old, dt, g_new, g_old, g_flux=None, opts=None):
+        """Adjoint of the backward-Euler step phi_old -> phi_new: g_old, adds to
+        g_flux; returns (dL/dSt of this step, SolveStats of the J^T solve)."""
+        st = SolveStats()
+        gst = C.c_double(0.0)
+        o = opts or default_opts(eps=1e-13, max_iter=500)
+        self.ctx.check(self.ctx.lib.ml_hc_adjoint_step(self.h, phi_new.ptr, phi_old.ptr, dt, g_new.ptr, g_old.ptr,
+                                                       g_flux.ptr if g_flux is not None else None, C.byref(gst),
+                                                       C.byref(o), C.byref(st)), "ml_hc_adjoint_step")
+        return gst.value, st
+
+    def porosity_vjp(self, phi, g_chi, g_phi):
+        """g_phi += g_chi dchi/dphi; returns sum g_chi dchi/dSt."""
+        gst = C.c_double(0.0)
+        self.ctx.check(self.ctx.lib.ml_hc_porosity_vjp(self.h, phi.ptr, g_chi.ptr, g_phi.ptr, C.byref(gst)),
+                       "ml_hc_porosity_vjp")
+        return gst.value
 
     def solve_be(self, phi_new, phi_old, dt, src=None, opts=None):
         st = SolveStats()

@@ -28,49 +28,6 @@ constexpr int kWGT = 512;      // single-workgroup kernel threads
 constexpr int kWGC = 1024;     // cells of the largest level it holds
 constexpr int kWGM = kWGC / kWGT;
 
-// Jacobian of the closure at (H, C), regime decided exactly as the closure
-// decides it.  M[0] = dtheta/dH, M[1] = dtheta/dTheta, M[2] = dTheta_l/dH,
-// M[3] = dTheta_l/dTheta.
-__device__ inline void closure_jacobian(double H, double C, const ClosureP<double>& p, double* M) {
-  double hs;
-  if (C < -p.CR)
-    hs = 0.0;
-  else
-    hs = p.c * ((-C - p.CR) / p.pc);
-  if (hs < 0.0) hs = 0.0;
-  hs = hs + p.te * p.c;
-  const double chie = (p.CR + C) / (p.CR + p.Te);
-  const double he = chie * (p.St + p.te * (1.0 - p.c)) + p.c * p.te;
-  const double hl = p.St - C + p.te + p.Te;
-  M[0] = M[1] = M[2] = M[3] = 0.0;
-  if (H <= hs) {
-    M[0] = 1.0 / p.c;
-  } else if (H > hs && H <= he) {
-    // eutectic plateau: theta and Theta_l constant
-  } else if (H > he && H < hl) {
-    const double chi = porosity_mush<double>(C, H, p);
-    double chiH = 0.0, chiC = 0.0;
-    if (p.St != 0.0) {
-      const double Cv = (C >= -p.CR) ? C : -p.CR;
-      const double a = p.CR * (p.c - 1.0) + p.St * (p.pc - 1.0);
-      const double b = p.CR * (1.0 - 2.0 * p.c) + H * (1.0 - p.pc) - Cv * (p.c - 1.0) - p.pc * p.St;
-      const double Fchi = 2.0 * a * chi + b;
-      chiH = -(chi * (1.0 - p.pc) + p.pc) / Fchi;
-      chiC = (C >= -p.CR) ? -(p.c - chi * (p.c - 1.0)) / Fchi : 0.0;
-    }
-    const double D = chi + p.pc * (1.0 - chi);
-    const double Tl = (C + p.CR * (1.0 - chi)) / D;
-    const double dTl = -(p.CR + Tl * (1.0 - p.pc)) / D;
-    M[2] = dTl * chiH;
-    M[3] = 1.0 / D + dTl * chiC;
-    M[0] = -M[2];
-    M[1] = -M[3];
-  } else if (H >= hl) {
-    M[0] = 1.0;
-    M[3] = 1.0;
-  }
-}
-
 // --------------------------------------------------------------------------
 // FP64 residual of the nonlinear operator, fused with phi += e and the
 // Jacobian, over a tile with a one-cell halo.

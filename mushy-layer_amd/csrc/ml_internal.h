@@ -56,6 +56,7 @@ struct HCLevel {
 
 struct HostDriver;   // reference-API FAS driver state (ml_capi.cpp)
 class HCEngine;      // fused FAS engine (ml_hc_engine.cpp)
+struct AdjWork;      // adjoint-step workspace (ml_adjoint.hip)
 
 }  // namespace ml
 
@@ -69,6 +70,7 @@ struct ml_hc_op_s {
   std::vector<ml::HCLevel> lev;
   std::unique_ptr<ml::HostDriver, void (*)(ml::HostDriver*)> host_driver{nullptr, nullptr};
   std::unique_ptr<ml::HCEngine, void (*)(ml::HCEngine*)> engine{nullptr, nullptr};
+  std::unique_ptr<ml::AdjWork, void (*)(ml::AdjWork*)> adj{nullptr, nullptr};
 
   ml::HCArgs<double> args(int depth) const {
     ml::HCArgs<double> A;

@@ -12,6 +12,8 @@ Workloads (named by what they are):
   * ``mushy_256`` — 256x256 mushy layer with porosity from a smooth seeded
     Gaussian random field (correlation length 0.1) in [0.05, 1], St = 5,
     CR = 2, Kozeny-Carman permeability, Da = 1e-3, Ra_C = 200.
+  * ``heat_flux`` — 128x128 diffusive solidification driven by a bottom
+    heat-flux profile (the adjoint workload): dt = 5e-4, 50 steps.
   * ``batch_instances`` — independent mushy layers for the batched form:
     grid sizes drawn by seed in ratio 8:4:2:1 from 64^2, 128^2, 256^2,
     512x256; St in [1, 10], CR in [1, 20], Ra_C in [0, 400], Da in
@@ -217,3 +219,26 @@ def batch_instances(count=512, seed=5, sizes=BATCH_SIZES, weights=(8, 4, 2, 1), 
         c["Ra_C"], c["Da"] = Ra_C, Da
         out.append(c)
     return out
+
+
+def heat_flux(n=(128, 128), seed=4, St=5.0, CR=2.0, lewis=200.0, theta_top=1.2, dt=5e-4, steps=50, noise=1e-3):
+    """Diffusive solidification cooled through the bottom by a heat-flux
+    profile: the `diffusive` liquid start and top BC, with the bottom theta
+    BC Neumann (its value per face is the flux profile, applied as the
+    equivalent source of the step) and the bottom Theta_l BC Neumann 0.
+    Periodic in x."""
+    c = diffusive(n=n, seed=seed, St=St, CR=CR, lewis=lewis, theta_top=theta_top, dt=dt, steps=steps, noise=noise)
+    vd = len(n) - 1
+    c["bc"][(vd, 0, 0)] = (NEUMANN, 0.0)
+    c["bc"][(vd, 0, 1)] = (NEUMANN, 0.0)
+    return c
+
+
+def flux_profile(nx, F0=10.0, modes=((1, 0.3, 0.0),)):
+    """F(x) = F0 (1 + sum_k a_k sin(2 pi k x + p_k)) at the bottom face
+    centres x = (i + 1/2)/nx."""
+    x = (np.arange(nx) + 0.5) / nx
+    f = np.ones(nx)
+    for k, a, ph in modes:
+        f = f + a * np.sin(2 * np.pi * k * x + ph)
+    return F0 * f

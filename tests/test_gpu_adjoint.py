@@ -1,0 +1,138 @@
+"""Adjoint of the implicit (H, Theta) steps: the hand-written GPU adjoint
+(ml_hc_adjoint_step behind a torch.autograd.Function, states recomputed from
+checkpoints) against central finite differences of the same GPU forward model
+and of the FP64 oracle's forward model, on a 32x32 heat-flux case (12 steps,
+checkpoints every 5, mush forming at the cooled bottom).
+
+Bars: the forward state equals the oracle's to 1e-12 relative; directional
+derivatives agree with central differences (h = 1e-6 relative to the
+parameter's scale) to FD_TOL relative; the permeability prefactor's gradient
+is exactly zero (no flow: the step does not use the permeability)."""
+import importlib
+
+import numpy as np
+import pytest
+
+from oracle import mlref as O
+from tests import hc_cases as HC
+
+pytestmark = pytest.mark.gpu
+
+FD_TOL = 2e-5
+H_REL = 1e-6
+
+
+def _setup(ml, gpu_ctx, wl):
+    torch = pytest.importorskip("torch")
+    adj = importlib.import_module("mushy-layer_amd.adjoint")
+    case = wl.heat_flux(n=(32, 32), steps=12, seed=9)
+    model = adj.HeatFluxAdjoint(gpu_ctx, case, checkpoint=5)
+    F_true = wl.flux_profile(32, F0=12.0, modes=((1, 0.3, 0.0), (2, 0.1, 1.0)))
+    tphi, tchi, cps = model.forward(F_true, 5.5)
+    for x in cps.values():
+        x.free()
+    return torch, adj, case, model, tphi, tchi
+
+
+def _loss_np(n, phi, chi, tphi, tchi):
+    v = lambda a: a[..., 1:-1, 1:-1]
+    return float(np.mean((v(phi)[1] - v(tphi)[1]) ** 2) + np.mean((v(chi)[0] - v(tchi)[0]) ** 2))
+
+
+def _gpu_loss(model, F, St, tphi, tchi):
+    phi, chi, cps = model.forward(F, St)
+    for x in cps.values():
+        x.free()
+    return _loss_np(model.n, phi, chi, tphi, tchi)
+
+
+def _adjoint_grads(torch, adj, model, F, St, tphi, tchi):
+    flux = torch.tensor(F, dtype=torch.float64, requires_grad=True)
+    perm = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    st = torch.tensor(St, dtype=torch.float64, requires_grad=True)
+    phi, chi = adj.ImplicitSteps.apply(flux, perm, st, model)
+    loss = adj.field_loss(model.n, phi, chi, tphi, tchi)
+    loss.backward()
+    return float(loss.detach()), flux.grad.numpy(), float(perm.grad), float(st.grad)
+
+
+def test_adjoint_matches_gpu_finite_differences(ml, gpu_ctx, wl):
+    torch, adj, case, model, tphi, tchi = _setup(ml, gpu_ctx, wl)
+    F = wl.flux_profile(32, F0=10.0)
+    St = 5.0
+    L, gF, gperm, gSt = _adjoint_grads(torch, adj, model, F, St, tphi, tchi)
+    print("loss %.6e, |dL/dF| %.3e, dL/dSt %.6e, adjoint solves (iterations, rel residual) %s"
+          % (L, np.linalg.norm(gF), gSt, model.adj_stats[:3]))
+    assert L > 0
+    assert gperm == 0.0
+    assert all(r <= 1e-12 for _, r in model.adj_stats)
+    rng = np.random.default_rng(3)
+    for k in range(2):
+        v = rng.standard_normal(F.shape)
+        v /= np.linalg.norm(v)
+        h = H_REL * np.max(np.abs(F))
+        fd = (_gpu_loss(model, F + h * v, St, tphi, tchi) - _gpu_loss(model, F - h * v, St, tphi, tchi)) / (2 * h)
+        ad = float(gF @ v)
+        print("direction %d: adjoint %.9e, central difference %.9e" % (k, ad, fd))
+        assert abs(ad - fd) <= FD_TOL * abs(fd), (k, ad, fd)
+    h = H_REL * St
+    fd = (_gpu_loss(model, F, St + h, tphi, tchi) - _gpu_loss(model, F, St - h, tphi, tchi)) / (2 * h)
+    print("St: adjoint %.9e, central difference %.9e" % (gSt, fd))
+    assert abs(gSt - fd) <= FD_TOL * abs(fd), (gSt, fd)
+    model.close()
+
+
+def _oracle_forward(case, F, St):
+    n = case["n"]
+    c = dict(case)
+    c["params"] = dict(case["params"], stefan=St)
+    c["apply_bc"] = 1
+    g, p, bc = HC.oracle_setup(c)
+    phi = case["phi"].copy()
+    # the forward step of the adjoint model: to 1e-10 without the hang rule,
+    # then on to the rounding floor
+    opts = O.default_opts(max_iter=400, hang=-1.0)
+    polish = O.default_opts(max_iter=200, eps=1e-15, norm_thresh=1e-15, hang=1e-10)
+    for _ in range(case["steps"]):
+        chi = O.porosity_field(g, p, phi)
+        b = O.hc_bcoef(g, p, chi)
+        src = np.zeros_like(phi)
+        src[0, 0, 1, 1:-1] = b[1][0, 0, 0, :] * F / case["dx"]
+        old = phi.copy()
+        st = O.hc_be_step(g, p, bc, None, b, case["dt"], phi, old, src=src, opts=opts)
+        assert st.exit_status & (1 | 8)
+        O.hc_be_step(g, p, bc, None, b, case["dt"], phi, old, src=src, opts=polish)
+    return phi, O.porosity_field(g, p, phi)
+
+
+def test_adjoint_matches_oracle_finite_differences(ml, gpu_ctx, wl):
+    """The reference FP64 gradient: central differences of the oracle's
+    forward model (C, FP64) against the GPU adjoint."""
+    torch, adj, case, model, tphi, tchi = _setup(ml, gpu_ctx, wl)
+    F = wl.flux_profile(32, F0=10.0)
+    St = 5.0
+    phi_gpu, _, cps = model.forward(F, St)
+    for x in cps.values():
+        x.free()
+    phi_ref, _ = _oracle_forward(case, F, St)
+    n = case["n"]
+    err = HC.rel_err(HC.valid(n, phi_gpu), HC.valid(n, phi_ref))
+    print("forward state, GPU vs oracle: %.3e" % err)
+    assert err <= 1e-12
+    L, gF, _, gSt = _adjoint_grads(torch, adj, model, F, St, tphi, tchi)
+
+    def loss(FF, SS):
+        ph, ch = _oracle_forward(case, FF, SS)
+        return _loss_np(n, ph, ch, tphi, tchi)
+
+    v = np.cos(2 * np.pi * (np.arange(32) + 0.5) / 32)
+    v /= np.linalg.norm(v)
+    h = H_REL * np.max(np.abs(F))
+    fd = (loss(F + h * v, St) - loss(F - h * v, St)) / (2 * h)
+    print("oracle direction: adjoint %.9e, central difference %.9e" % (float(gF @ v), fd))
+    assert abs(float(gF @ v) - fd) <= FD_TOL * abs(fd)
+    h = H_REL * St
+    fd = (loss(F, St + h) - loss(F, St - h)) / (2 * h)
+    print("oracle St: adjoint %.9e, central difference %.9e" % (gSt, fd))
+    assert abs(gSt - fd) <= FD_TOL * abs(fd)
+    model.close()
