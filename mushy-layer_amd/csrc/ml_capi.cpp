@@ -133,8 +133,7 @@ int ml_ctx_timing_start(ml_ctx ctx, int kernel, int max_launches) {
     ctx->timing_start.push_back(a);
     ctx->timing_stop.push_back(b);
   }
-  ctx->timing_start.resize(max_launches);
-  ctx->timing_stop.resize(max_launches);
+  ctx->timing_cap = max_launches;   // the pool keeps its events (destroyed with the context)
   ctx->timing_kernel = kernel;
   ctx->timing_n = 0;
   ctx->timing_missed = 0;

@@ -25,10 +25,11 @@ struct ml_ctx_s {
   int timing_n = 0;
   long long timing_missed = 0;
   double timing_bytes = 0;
+  int timing_cap = 0;                       // launches timed (<= the pool's size)
   std::vector<hipEvent_t> timing_start, timing_stop;
   ml::LaunchTimer timer(int kernel) {
-    if (timing_kernel != kernel || timing_start.empty()) return ml::LaunchTimer{};
-    return ml::LaunchTimer{timing_start.data(), timing_stop.data(), int(timing_start.size()), &timing_n,
+    if (timing_kernel != kernel || timing_cap == 0) return ml::LaunchTimer{};
+    return ml::LaunchTimer{timing_start.data(), timing_stop.data(), timing_cap, &timing_n,
                            &timing_bytes, &timing_missed};
   }
 };
