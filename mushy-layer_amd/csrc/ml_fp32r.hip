@@ -1264,7 +1264,8 @@ namespace {
 struct BatchTile {
   int TX, TY, NT;
 };
-constexpr BatchTile kBatchTiles[] = {{16, 16, 1024}, {32, 16, 512}, {32, 32, 1024}, {64, 16, 1024}, {32, 8, 256}};
+constexpr BatchTile kBatchTiles[] = {{16, 16, 1024}, {32, 16, 512}, {32, 32, 1024}, {64, 16, 1024},
+                                     {32, 8, 256},   {64, 8, 512},  {32, 16, 256},  {16, 32, 512}};
 constexpr int kNumBatchTiles = int(sizeof(kBatchTiles) / sizeof(kBatchTiles[0]));
 
 template <int V>
@@ -1313,7 +1314,10 @@ hipError_t lin_relax_batch(const BatchLinJob* jobs, const unsigned* map, int nbl
     case 1: e = launch_batch_variant<1>(jobs, map, nblocks, npass, zero_init, lds, s, e0, e1); break;
     case 2: e = launch_batch_variant<2>(jobs, map, nblocks, npass, zero_init, lds, s, e0, e1); break;
     case 3: e = launch_batch_variant<3>(jobs, map, nblocks, npass, zero_init, lds, s, e0, e1); break;
-    default: e = launch_batch_variant<4>(jobs, map, nblocks, npass, zero_init, lds, s, e0, e1); break;
+    case 4: e = launch_batch_variant<4>(jobs, map, nblocks, npass, zero_init, lds, s, e0, e1); break;
+    case 5: e = launch_batch_variant<5>(jobs, map, nblocks, npass, zero_init, lds, s, e0, e1); break;
+    case 6: e = launch_batch_variant<6>(jobs, map, nblocks, npass, zero_init, lds, s, e0, e1); break;
+    default: e = launch_batch_variant<7>(jobs, map, nblocks, npass, zero_init, lds, s, e0, e1); break;
   }
   if (e == hipSuccess && timed) {
     *tm->bytes += bytes;
