@@ -337,7 +337,7 @@ def timed_steps(ctx, st, steps):
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vc_relax_tile_256_traffic.json")   # 256x256 workload
 # the tiled relaxation of the finest level of the two workloads
-VC_TILE = {2: "vc_relax_tile_kernel<double,2,16,16,1,9,1024>", 3: "vc_relax_tile_kernel<double,3,16,8,4,2,512>"}
+VC_TILE = {2: "vc_relax_tile_kernel<double,2,16,16,1,9,1024>", 3: "vc_relax_tile_kernel<double,3,16,8,4,2,1024>"}
 
 
 def vc_relax_roofline(ml, ctx, stepper, steps):

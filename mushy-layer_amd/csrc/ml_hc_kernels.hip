@@ -712,10 +712,17 @@ struct HCRelaxWide2 {
   static constexpr int TX = 32, TY = 8, TZ = 1, NT = 256;
 };
 
+// tiles of the 3D relaxation launch (measured best of 5 shapes at 128^3;
+// the others spill registers)
+struct HCRelaxTile3 {
+  static constexpr int TX = 16, TY = 8, TZ = 4, NT = 512;
+};
+
 template <typename T, int DIM>
 static hipError_t launch_relax_tile(const HCArgs<T>& A, TileIO<T> io, hipStream_t s, const LaunchTimer* tm) {
   using D = TileDims<T, DIM>;
   using DW = HCRelaxWide2;
+  using D3 = HCRelaxTile3;
   int nt = tiles_of<T, DIM>(A.g, io.ntx, io.nty);
   const int pmax = relax_passes_per_launch<T, DIM>(A.g);
   if (io.npass < 1 || io.npass > pmax) return hipErrorInvalidValue;
@@ -725,6 +732,12 @@ static hipError_t launch_relax_tile(const HCArgs<T>& A, TileIO<T> io, hipStream_
     io.nty = (A.g.n[1] + DW::TY - 1) / DW::TY;
     nt = io.ntx * io.nty;
     lds = 4 * size_t(DW::TX + 2 * io.npass) * (DW::TY + 2 * io.npass) * sizeof(T);
+  }
+  if (DIM == 3) {
+    io.ntx = (A.g.n[0] + D3::TX - 1) / D3::TX;
+    io.nty = (A.g.n[1] + D3::TY - 1) / D3::TY;
+    nt = io.ntx * io.nty * ((A.g.n[2] + D3::TZ - 1) / D3::TZ);
+    lds = 4 * size_t(D3::TX + 2 * io.npass) * (D3::TY + 2 * io.npass) * (D3::TZ + 2 * io.npass) * sizeof(T);
   }
   if (lds > kMaxLds) return hipErrorInvalidValue;
   const bool timed = tm && *tm->n < tm->cap;
@@ -738,8 +751,8 @@ static hipError_t launch_relax_tile(const HCArgs<T>& A, TileIO<T> io, hipStream_
       hipExtLaunchKernelGGL((hc_relax_tile_kernel<T, DIM, DW::TX, DW::TY, DW::TZ, 2, DW::NT>), dim3(nt), dim3(DW::NT),
                             lds, s, e0, e1, 0, A, io);
   } else {
-    hipExtLaunchKernelGGL((hc_relax_tile_kernel<T, DIM, D::TX, D::TY, D::TZ, 2, 512>), dim3(nt), dim3(512), lds, s,
-                          e0, e1, 0, A, io);
+    hipExtLaunchKernelGGL((hc_relax_tile_kernel<T, DIM, D3::TX, D3::TY, D3::TZ, 2, D3::NT>), dim3(nt), dim3(D3::NT),
+                          lds, s, e0, e1, 0, A, io);
   }
   const hipError_t err = hipGetLastError();
   if (err == hipSuccess && timed) {
