@@ -1131,7 +1131,7 @@ hipError_t lin_relax_impl(const LinLevel<float>& L, float* const ebuf[2], int* c
       const int p = (npass - done) < 2 ? (npass - done) : 2;
       if (C && p + hx > 2) return hipErrorInvalidValue;
       const size_t lds = (8 * lin_extent<3>(p + hx) + (C ? 2 * D::TX * D::TY * D::TZ : 0)) * sizeof(float);
-      hipLaunchKernelGGL((lin_relax_tile_kernel<3, D::TX, D::TY, D::TZ, 2, 512>), dim3(nt), dim3(512), lds, s, L, ein,
+      hipLaunchKernelGGL((lin_relax_tile_kernel<3, D::TX, D::TY, D::TZ, 2, 1024>), dim3(nt), dim3(1024), lds, s, L, ein,
                          eout, bc, alpha, beta, p, zi, gcc, p0, ntx, nty, grr, rc, Mc);
       done += p;
     }
