@@ -435,7 +435,7 @@ struct WGCellOp {
 template <typename T, int DIM>
 __device__ void wg_relax(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, const WGCellOp<T, DIM>& O,
                          const T* rr, T* sphi, T* sphl, int npass) {
-  for (int p = 0; p < npass; ++p) {
+  auto pass = [&](int p) {
     if (W.own && !((W.i + W.j + W.k + p) & 1)) {
       T lof[2];
       wg_lof<T, DIM>(D, W, O.F, O.aq, sphi, sphl, lof);
@@ -449,7 +449,21 @@ __device__ void wg_relax(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, const
       sphl[c] = th;
       sphl[W.N + c] = cl;
     }
+  };
+  if (W.N <= 64) {
+    // every cell is owned by wave 0: it runs the passes alone, synchronising
+    // at wavefront scope; the other waves wait once
+    if (threadIdx.x < 64)
+      for (int p = 0; p < npass; ++p) {
+        pass(p);
+        wave_sync();
+      }
     __syncthreads();
+  } else {
+    for (int p = 0; p < npass; ++p) {
+      pass(p);
+      __syncthreads();
+    }
   }
   if (W.own) {
     W.L->phi[W.q] = sphi[W.c];
