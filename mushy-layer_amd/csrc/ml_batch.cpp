@@ -64,7 +64,6 @@ struct ml_hc_batch_s {
   ml::SolveCtl* ctl = nullptr;
   ml::SolveCtl* hctl = nullptr;           // pinned mirror
   unsigned long long* slots = nullptr;
-  unsigned long long* hslot = nullptr;    // pinned, one slot per instance
   unsigned int* arrive = nullptr;
   // job tables and workgroup maps: one device arena, one pinned mirror
   unsigned char* arena = nullptr;
@@ -89,6 +88,13 @@ struct ml_hc_batch_s {
   std::vector<size_t> cdown_off, cup_off;
   size_t cres_off = 0;
   int cres_blocks = 0;
+  // backtracking trials: instance list, workgroup map, gathered norms
+  int* tlist = nullptr;
+  unsigned* tmap = nullptr;
+  unsigned long long* tnorm = nullptr;
+  int* htlist = nullptr;
+  unsigned* htmap = nullptr;
+  unsigned long long* htnorm = nullptr;
   // tile shape of the batched relaxation: picked by timing every shape on
   // the batch's own launches at the first step, unless set
   int variant = 0;
@@ -127,9 +133,11 @@ void batch_free(ml_hc_batch b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   for (void* p : b->owned) (void)hipFree(p);
   if (b->hctl) (void)hipHostFree(b->hctl);
-  if (b->hslot) (void)hipHostFree(b->hslot);
   if (b->harena) (void)hipHostFree(b->harena);
   if (b->hcmap) (void)hipHostFree(b->hcmap);
+  if (b->htlist) (void)hipHostFree(b->htlist);
+  if (b->htmap) (void)hipHostFree(b->htmap);
+  if (b->htnorm) (void)hipHostFree(b->htnorm);
   delete b;
 }
 
@@ -307,6 +315,18 @@ int layout(ml_hc_batch b) {
   b->cmap = static_cast<unsigned*>(v);
   ML_HIP(ctx, hipHostMalloc(&v, sizeof(unsigned) * b->cmap_cap, hipHostMallocDefault));
   b->hcmap = static_cast<unsigned*>(v);
+  if ((rc = dalloc(b, sizeof(int) * n, &v))) return rc;
+  b->tlist = static_cast<int*>(v);
+  if ((rc = dalloc(b, sizeof(unsigned) * size_t(b->res_blocks), &v))) return rc;
+  b->tmap = static_cast<unsigned*>(v);
+  if ((rc = dalloc(b, sizeof(unsigned long long) * n, &v))) return rc;
+  b->tnorm = static_cast<unsigned long long*>(v);
+  ML_HIP(ctx, hipHostMalloc(&v, sizeof(int) * n, hipHostMallocDefault));
+  b->htlist = static_cast<int*>(v);
+  ML_HIP(ctx, hipHostMalloc(&v, sizeof(unsigned) * size_t(b->res_blocks), hipHostMallocDefault));
+  b->htmap = static_cast<unsigned*>(v);
+  ML_HIP(ctx, hipHostMalloc(&v, sizeof(unsigned long long) * n, hipHostMallocDefault));
+  b->htnorm = static_cast<unsigned long long*>(v);
   if ((rc = dalloc(b, off, &v))) return rc;
   b->arena = static_cast<unsigned char*>(v);
   ML_HIP(ctx, hipHostMalloc(&v, off, hipHostMallocDefault));
@@ -621,34 +641,37 @@ int capture(ml_hc_batch b, const ml_solve_opts& o) {
 }
 
 // host step-length backtracking after a rejected step, the single-instance
-// rule for each instance in `idx`: the trials of one step length run for
-// all of them, then one synchronisation
+// rule for each instance in `idx`: each step length's trials run for all of
+// them in one launch (the residual kernel's tile body at step length omega),
+// then one synchronisation
 int backtrack(ml_hc_batch b, const std::vector<int>& idx) {
   ml_ctx ctx = b->ctx;
   hipStream_t s = ctx->stream;
+  const BatchResJob* jobs = b->at<BatchResJob>(b->res_off);
   std::vector<int> trying = idx;
   std::vector<double> norm(b->inst.size());
   for (int i : idx) norm[i] = b->hctl[i].norm;
   double omega = 1.0;
   for (int k = 1; k <= kMaxHalvings && !trying.empty(); ++k) {
     omega *= 0.5;
-    for (int i : trying) {
-      BatchInst& I = b->inst[i];
-      const SolveCtl& c = b->hctl[i];
-      unsigned long long* slots = b->slots + size_t(i) * kBatchSlots;
-      const int it = c.iter + 1;
-      const int pc = c.iter & 1;
-      double* const pb[2] = {b->key_phi[i], I.alt0};
-      const HCArgs<double> An = I.op->args(0);
-      ML_HIP(ctx, zero_u64(slots + it, 1, s));
-      ML_HIP(ctx, hc_residual_jacobian(An, pb[pc], pb[1 - pc], I.rhs, I.lin[0].e, omega, I.lin[0].r, I.lin[0].M,
-                                       slots + it, s));
-      ML_HIP(ctx, hipMemcpyAsync(b->hslot + i, slots + it, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    const int nt = int(trying.size());
+    int pos = 0;
+    for (int j = 0; j < nt; ++j) {
+      const int i = trying[j];
+      b->htlist[j] = i;
+      for (int t = 0; t < b->inst[i].nres; ++t) b->htmap[pos++] = (unsigned(i) << kBatchTileBits) | unsigned(t);
     }
+    ML_HIP(ctx, hipMemcpyAsync(b->tlist, b->htlist, sizeof(int) * nt, hipMemcpyHostToDevice, s));
+    ML_HIP(ctx, hipMemcpyAsync(b->tmap, b->htmap, sizeof(unsigned) * pos, hipMemcpyHostToDevice, s));
+    ML_HIP(ctx, batch_trial_slots(jobs, b->tlist, nt, false, nullptr, s));
+    ML_HIP(ctx, hc_residual_jacobian_batch_trial(jobs, b->tmap, pos, omega, s));
+    ML_HIP(ctx, batch_trial_slots(jobs, b->tlist, nt, true, b->tnorm, s));
+    ML_HIP(ctx, hipMemcpyAsync(b->htnorm, b->tnorm, sizeof(unsigned long long) * nt, hipMemcpyDeviceToHost, s));
     ML_HIP(ctx, hipStreamSynchronize(s));
     std::vector<int> left;
-    for (int i : trying) {
-      std::memcpy(&norm[i], b->hslot + i, sizeof(double));
+    for (int j = 0; j < nt; ++j) {
+      const int i = trying[j];
+      std::memcpy(&norm[i], b->htnorm + j, sizeof(double));
       if (!(norm[i] < b->hctl[i].prev)) left.push_back(i);
     }
     trying.swap(left);
@@ -669,8 +692,9 @@ int backtrack(ml_hc_batch b, const std::vector<int>& idx) {
       c.status = status != 0 ? status : 2;
       c.done = 1;
     }
-    ML_HIP(ctx, hipMemcpyAsync(b->ctl + i, b->hctl + i, sizeof(SolveCtl), hipMemcpyHostToDevice, s));
   }
+  // no kernel has changed a control block since the host read them
+  ML_HIP(ctx, hipMemcpyAsync(b->ctl, b->hctl, sizeof(SolveCtl) * b->inst.size(), hipMemcpyHostToDevice, s));
   ML_HIP(ctx, hipStreamSynchronize(s));
   return ML_OK;
 }
@@ -709,12 +733,6 @@ int ml_hc_batch_create(ml_ctx ctx, int count, const ml_hc_op ops[], ml_hc_batch*
       rc = fail(ctx, ML_ENOMEM, "hipHostMalloc");
     else
       b->hctl = static_cast<SolveCtl*>(v);
-  }
-  if (rc == ML_OK) {
-    if (hipHostMalloc(&v, sizeof(unsigned long long) * count, hipHostMallocDefault) != hipSuccess)
-      rc = fail(ctx, ML_ENOMEM, "hipHostMalloc");
-    else
-      b->hslot = static_cast<unsigned long long*>(v);
   }
   for (int i = 0; i < count && rc == ML_OK; ++i) rc = inst_init(b, i, ops[i], maxlds);
   if (rc == ML_OK) rc = layout(b);

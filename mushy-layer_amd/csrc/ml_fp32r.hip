@@ -39,8 +39,8 @@ __device__ __forceinline__ void hc_residual_jacobian_body(const HCArgs<double>& 
                                                           const float* __restrict__ ecorr, double omega,
                                                           float* __restrict__ r32, float* __restrict__ M32,
                                                           unsigned long long* norm, int ntx, int nty,
-                                                          const CtlFuse& fuse, int blk) {
-  if (A.skip && *A.skip) return;
+                                                          const CtlFuse& fuse, int blk, const int* skip) {
+  if (skip && *skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2) * (TY + 2) * (DIM == 3 ? TZ + 2 : 1);
   constexpr int KMAX = (EMAX + kLinThreads - 1) / kLinThreads;
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_kernel(HCArg
                                                                            unsigned long long* norm, int ntx, int nty,
                                                                            CtlFuse fuse) {
   hc_residual_jacobian_body<DIM, TX, TY, TZ>(A, phi, phi_out, rhs, ecorr, omega, r32, M32, norm, ntx, nty, fuse,
-                                             blockIdx.x);
+                                             blockIdx.x, A.skip);
 }
 
 // --------------------------------------------------------------------------
@@ -1231,16 +1231,27 @@ __global__ __launch_bounds__(NT) void lin_relax_batch_kernel(const BatchLinJob* 
                                                  J.pc, J.ntx, J.nty, J.gr, J.rc, J.Mc, int(w & kBatchTileMask));
 }
 
+// mode 0: iteration ctl->iter + 1 with the stopping rule; 1: initial
+// residual of pb[0] into slots[0]; 2: a backtracking trial of step length
+// omega for an instance whose step was rejected (its done flag is set; the
+// norm goes to slots[ctl->iter + 1], which the caller has zeroed)
 template <int DIM, int TX, int TY, int TZ>
 __global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_batch_kernel(const BatchResJob* __restrict__ jobs,
                                                                                  const unsigned* __restrict__ map,
-                                                                                 int initial) {
+                                                                                 int mode, double omega) {
   const unsigned w = map[blockIdx.x];
   const BatchResJob& J = jobs[w >> kBatchTileBits];
   const int blk = int(w & kBatchTileMask);
-  if (initial) {
+  if (mode == 1) {
     hc_residual_jacobian_body<DIM, TX, TY, TZ>(J.A, J.pb[0], nullptr, J.rhs, nullptr, 1.0, J.r32, J.M32, J.slots,
-                                               J.ntx, J.nty, CtlFuse{}, blk);
+                                               J.ntx, J.nty, CtlFuse{}, blk, J.A.skip);
+    return;
+  }
+  if (mode == 2) {
+    const int it = J.ctl->iter + 1;
+    const int pc = (it - 1) & 1;
+    hc_residual_jacobian_body<DIM, TX, TY, TZ>(J.A, J.pb[pc], J.pb[1 - pc], J.rhs, J.e, omega, J.r32, J.M32,
+                                               J.slots + it, J.ntx, J.nty, CtlFuse{}, blk, nullptr);
     return;
   }
   if (*J.A.skip) return;
@@ -1250,7 +1261,21 @@ __global__ __launch_bounds__(kLinThreads) void hc_residual_jacobian_batch_kernel
   const int pc = (it - 1) & 1;
   const CtlFuse f{J.ctl, J.slots, J.arrive, 1, J.ntiles};
   hc_residual_jacobian_body<DIM, TX, TY, TZ>(J.A, J.pb[pc], J.pb[1 - pc], J.rhs, J.e, 1.0, J.r32, J.M32,
-                                             J.slots + it, J.ntx, J.nty, f, blk);
+                                             J.slots + it, J.ntx, J.nty, f, blk, J.A.skip);
+}
+
+// slots[i][ctl_i->iter + 1] of the listed instances: zeroed (gather == 0)
+// or copied to out[j] (gather == 1)
+__global__ void batch_slots_kernel(const BatchResJob* __restrict__ jobs, const int* __restrict__ list, int n,
+                                   int gather, unsigned long long* out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const BatchResJob& J = jobs[list[j]];
+  unsigned long long* slot = J.slots + J.ctl->iter + 1;
+  if (gather)
+    out[j] = *slot;
+  else
+    *slot = 0ull;
 }
 
 }  // namespace
@@ -1332,7 +1357,24 @@ hipError_t hc_residual_jacobian_batch(const BatchResJob* jobs, const unsigned* m
   if (nblocks == 0) return hipSuccess;
   const size_t lds = 4 * lin_extent<2>(1) * sizeof(double);
   hipLaunchKernelGGL((hc_residual_jacobian_batch_kernel<2, D::TX, D::TY, D::TZ>), dim3(nblocks), dim3(kLinThreads), lds,
-                     s, jobs, map, initial ? 1 : 0);
+                     s, jobs, map, initial ? 1 : 0, 1.0);
+  return hipGetLastError();
+}
+
+hipError_t hc_residual_jacobian_batch_trial(const BatchResJob* jobs, const unsigned* map, int nblocks, double omega,
+                                            hipStream_t s) {
+  using D = LinTile<2>;
+  if (nblocks == 0) return hipSuccess;
+  const size_t lds = 4 * lin_extent<2>(1) * sizeof(double);
+  hipLaunchKernelGGL((hc_residual_jacobian_batch_kernel<2, D::TX, D::TY, D::TZ>), dim3(nblocks), dim3(kLinThreads), lds,
+                     s, jobs, map, 2, omega);
+  return hipGetLastError();
+}
+
+hipError_t batch_trial_slots(const BatchResJob* jobs, const int* list, int n, bool gather, unsigned long long* out,
+                             hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_slots_kernel, dim3((n + 255) / 256), dim3(256), 0, s, jobs, list, n, gather ? 1 : 0, out);
   return hipGetLastError();
 }
 

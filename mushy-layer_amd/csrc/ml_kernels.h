@@ -449,6 +449,13 @@ hipError_t lin_relax_batch(const BatchLinJob* jobs, const unsigned* map, int nbl
 // stopping rule; else iteration ctl->iter + 1 with the stopping rule
 hipError_t hc_residual_jacobian_batch(const BatchResJob* jobs, const unsigned* map, int nblocks, bool initial,
                                       hipStream_t s);
+// backtracking of the listed instances (their steps were rejected): the
+// residual at step length omega into slots[ctl->iter + 1]; zero / gather
+// those slots (out[j] for list[j])
+hipError_t hc_residual_jacobian_batch_trial(const BatchResJob* jobs, const unsigned* map, int nblocks, double omega,
+                                            hipStream_t s);
+hipError_t batch_trial_slots(const BatchResJob* jobs, const int* list, int n, bool gather, unsigned long long* out,
+                             hipStream_t s);
 // the coarse V-cycles of n instances, one workgroup each
 hipError_t lin_coarse_vcycle_batch(const LinDesc* descs, int n, size_t lds, hipStream_t s);
 
