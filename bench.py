@@ -289,29 +289,100 @@ def field_errors(ml, ctx, case, a, b):
             "porosity_abs": float(np.max(np.abs(chi[0] - chi[1])))}
 
 
-def cpu_baseline(case, seconds, max_iter):
-    from oracle import mlref as O
+class CPUStep:
+    """The step of the GPU steppers restated with the C oracle: the (H, Theta)
+    backward-Euler solve and, with `db`, the U* Helmholtz solve of each
+    velocity component and the projection solve from the new porosity (the
+    Darcy coefficients formed from the porosity on the host)."""
 
-    g = O.make_grid(case["n"], case["dx"], case["periodic"] + [0])
-    p = O.default_params(**case["params"])
-    bc = O.make_bc(case["bc"])
-    phi = case["phi"].copy()
-    cells = int(np.prod(case["n"]))
-    opts = O.default_opts(max_iter=max_iter, hang=-1.0)
+    def __init__(self, O, wl, case, db, max_iter):
+        self.O, self.wl, self.db = O, wl, db
+        self.n = list(case["n"])
+        self.g = O.make_grid(self.n, case["dx"], case["periodic"] + [0] * (3 - len(case["periodic"])))
+        self.p = O.default_params(**case["params"])
+        self.bc = O.make_bc(case["bc"])
+        self.dt = case["dt"]
+        self.periodic = case["periodic"]
+        self.phi = case["phi"].copy()
+        self.opts = O.default_opts(max_iter=max_iter, hang=-1.0)
+        if db is not None:
+            dim = len(self.n)
+            self.bcu = O.make_bc(db["bc_u"])
+            self.bcp = O.make_bc(db["bc_p"])
+            self.u = [db["u0"].copy() for _ in range(dim)]
+            self.src = [db["src"] if d == dim - 1 else np.zeros_like(db["src"]) for d in range(dim)]
+            self.prhs = np.zeros_like(db["u0"])
+            v = O.valid(self.g, self.prhs)
+            v[0] = O.valid(self.g, db["u0"])[0]
+            v[0] -= v[0].mean()
+            self.pres = np.zeros_like(db["u0"])
+            self.uopts = O.default_opts(**CoupledStepper.U_OPTS)
+            self.popts = O.default_opts(**CoupledStepper.P_OPTS)
+
+    def step(self):
+        O, g = self.O, self.g
+        b = O.hc_bcoef(g, self.p, O.porosity_field(g, self.p, self.phi))
+        old = self.phi.copy()
+        O.hc_be_step(g, self.p, self.bc, None, b, self.dt, self.phi, old, opts=self.opts)
+        if self.db is None:
+            return
+        db = self.db
+        chi = O.porosity_field(g, self.p, self.phi)
+        cv = O.valid(g, chi)[0].reshape(tuple(reversed(self.n)))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            perm = cv ** 3 / (1.0 - cv) ** 2
+            cval = np.where(np.isinf(perm), 0.0, cv / np.where(perm == 0, 1.0, perm)) * (db["Pr"] / db["Da"])
+        c = np.zeros_like(chi)
+        O.valid(g, c)[0] = cval
+        for u, src in zip(self.u, self.src):
+            rhs = np.zeros_like(u)
+            O.valid(g, rhs)[...] = O.valid(g, u) * 1.0 + self.dt * O.valid(g, src)
+            O.lin_mg_solve(g, 1, self.bcu, 1.0, None, -self.dt, c, db["b_u"], u, rhs, opts=self.uopts)
+        bp = [-f for f in self.wl.geometric_faces(self.n, self.periodic, cv)]
+        O.lin_mg_solve(g, 1, self.bcp, 0.0, None, 1.0, None, bp, self.pres, self.prhs, opts=self.popts)
+
+
+def cpu_sample(O, wl, case, db, seconds, max_iter):
+    """Steps of the oracle from the workload's initial state until `seconds`
+    have passed: (steps, wall seconds)."""
+    st = CPUStep(O, wl, case, db, max_iter)
     t0 = time.perf_counter()
     steps = 0
     while True:
-        chi = O.porosity_field(g, p, phi)
-        b = O.hc_bcoef(g, p, chi)
-        old = phi.copy()
-        O.hc_be_step(g, p, bc, None, b, case["dt"], phi, old, opts=opts)
+        st.step()
         steps += 1
         if time.perf_counter() - t0 >= seconds:
             break
-    dt = time.perf_counter() - t0
-    return {"value": round(cells * steps / dt, 2), "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": "%d implicit step(s) of the same %s workload from its initial state, C oracle "
-                      "(gcc -O2 -ffp-contract=off), 1 thread, %.1f s" % (steps, "x".join(map(str, case["n"])), dt)}
+    return steps, time.perf_counter() - t0
+
+
+def host_cores():
+    """Threads for the all-cores sample: OMP_NUM_THREADS when set (the host's
+    share of cores), else the CPU count."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
+def cpu_baseline(wl, case, db, seconds, max_iter):
+    """The C oracle timed on a bounded sample of the same workload, on one
+    OpenMP thread (the reported value) and on all host cores (`all_cores`).
+    Cells within a red/black colour are independent, so both runs compute the
+    same states."""
+    from oracle import mlref as O
+
+    cells = int(np.prod(case["n"]))
+    shape = "x".join(map(str, case["n"]))
+    what = "step(s)" if db is None else "Darcy-Brinkman step(s) (HC solve, U* solves, projection)"
+    O.set_threads(1)
+    steps1, dt1 = cpu_sample(O, wl, case, db, seconds, max_iter)
+    nthr = O.set_threads(host_cores())
+    stepsn, dtn = cpu_sample(O, wl, case, db, seconds, max_iter)
+    O.set_threads(1)
+    return {"value": round(cells * steps1 / dt1, 2), "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d implicit %s of the same %s workload from its initial state, C oracle "
+                      "(gcc -O2 -ffp-contract=off -fopenmp), 1 thread, %.1f s" % (steps1, what, shape, dt1),
+            "all_cores": {"value": round(cells * stepsn / dtn, 2), "cores": nthr,
+                          "sample": "%d step(s), same oracle, %d OpenMP threads, %.1f s" % (stepsn, nthr, dtn)}}
 
 
 def run_form(ml, ctx, case, precision, steps, warmup, max_iter, db=None, no_lookahead=False):
@@ -656,7 +727,7 @@ def main():
         roof = vc_relax_roofline(ml, ctx, h["stepper"], args.steps)
     large = smoother_bandwidth(ml, wl, ctx, args.large_n) if args.large_n > 0 else None
     calibration_copy(ml, ctx)
-    cpu = None if args.no_cpu else cpu_baseline(case, args.cpu_seconds, args.max_iter)
+    cpu = None if args.no_cpu else cpu_baseline(wl, case, db, args.cpu_seconds, args.max_iter)
 
     what = ("HC backward-Euler solve (porosity + coefficient assembly + solve)" if args.hc_only else
             "Darcy-Brinkman step: HC backward-Euler solve + U* Helmholtz solve per velocity component "

@@ -180,6 +180,10 @@ int mlr_lin_mg_solve(const mlr_grid* g, int ncomp, const mlr_bc* bc,
                      const double* rhs, int prolong_linear, const mlr_opts* o,
                      mlr_stats* st);
 
+/* OpenMP threads of the oracle's parallel loops (0: unchanged); returns the
+ * count in use.  Results do not depend on it. */
+int mlr_set_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,7 @@ def _declare(L):
         "mlr_geometric_faces": (None, [G, _dp, pp]),
         "mlr_hc_bcoef": (None, [G, P, _dp, pp]),
         "mlr_default_opts": (None, [C.POINTER(Opts)]),
+        "mlr_set_threads": (C.c_int, [C.c_int]),
         "mlr_hc_fas_solve": (C.c_int, [G, P, B, C.c_double, _dp, C.c_double, pp, _dp, _dp,
                                        C.POINTER(Opts), C.POINTER(Stats)]),
         "mlr_hc_be_step": (C.c_int, [G, P, B, _dp, pp, C.c_double, _dp, _dp, _dp,
@@ -366,3 +367,8 @@ def lin_mg_solve(g, ncomp, bc, alpha, a, beta, c, b, phi, rhs, prolong_linear=0,
     lib().mlr_lin_mg_solve(C.byref(g), ncomp, C.byref(bc), alpha, ptr(a), beta, ptr(c), ptrs(b), ptr(phi),
                            ptr(rhs), int(prolong_linear), C.byref(o), C.byref(st))
     return st
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops; returns the count in use."""
+    return lib().mlr_set_threads(int(n))

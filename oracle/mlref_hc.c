@@ -93,6 +93,7 @@ static void closure_box(const mlr_grid* g, const mlr_params* p,
                         const double* phi, double* phil) {
   mlr_lay L;
   mlr_layout(g, &L);
+#pragma omp parallel for schedule(static) if (L.ncell > 4096)
   for (long q = 0; q < L.ncell; ++q) {
     double t, cl;
     mlr_closure_point(phi[q], phi[q + L.ncell], p, &t, &cl);
@@ -165,8 +166,11 @@ void mlr_hc_gsrb(const mlr_grid* g, const mlr_params* p, const mlr_bc* bc,
     mlr_exchange(g, 2, phi);
     mlr_noflux_bc(g, 2, phi);
     diffused_var_fab(g, p, bc, phi, scratch, 0);
+    /* the cells of one colour are independent: rows in parallel (identical
+       results for any thread count) */
     for (int n = 0; n < 2; ++n)
       for (int k = 0; k < L.nz; ++k)
+#pragma omp parallel for schedule(static) if (g->n[0] * g->n[1] > 4096)
         for (int j = 0; j < g->n[1]; ++j) {
           int imin = abs((j + k + pass) % 2);
           for (int i = imin; i < g->n[0]; i += 2) {
@@ -192,6 +196,7 @@ void mlr_hc_residual(const mlr_grid* g, const mlr_params* p, const mlr_bc* bc,
   diffused_var_fab(g, p, bc, phi, scratch, 0);
   for (int n = 0; n < 2; ++n)
     for (int k = 0; k < L.nz; ++k)
+#pragma omp parallel for schedule(static) if (g->n[0] * g->n[1] > 4096)
       for (int j = 0; j < g->n[1]; ++j)
         for (int i = 0; i < g->n[0]; ++i) {
           long q = mlr_ci(&L, i, j, k) + n * L.ncell;
@@ -213,6 +218,7 @@ void mlr_hc_apply_op(const mlr_grid* g, const mlr_params* p, const mlr_bc* bc,
   mlr_hc_diffused_var(g, p, bc, phi, scratch, 0);
   for (int n = 0; n < 2; ++n)
     for (int k = 0; k < L.nz; ++k)
+#pragma omp parallel for schedule(static) if (g->n[0] * g->n[1] > 4096)
       for (int j = 0; j < g->n[1]; ++j)
         for (int i = 0; i < g->n[0]; ++i) {
           long q = mlr_ci(&L, i, j, k) + n * L.ncell;
@@ -240,9 +246,13 @@ void mlr_hc_restrict_residual(const mlr_grid* gf, const mlr_params* p,
   mlr_exchange(gf, 2, phi);
   mlr_hc_diffused_var(gf, p, bc, phi, scratch, 0);
   for (long q = 0; q < 2 * Lc.ncell; ++q) resc[q] = 0.0;
+  /* a coarse row gathers fine rows 2jc and 2jc+1 only: coarse rows in
+     parallel keep every coarse cell's summation order */
   for (int n = 0; n < 2; ++n)
     for (int k = 0; k < L.nz; ++k)
-      for (int j = 0; j < gf->n[1]; ++j)
+#pragma omp parallel for schedule(static) if (gf->n[0] * gf->n[1] > 4096)
+      for (int jc = 0; jc < (gf->n[1] + 1) / 2; ++jc)
+      for (int j = 2 * jc; j < 2 * jc + 2 && j < gf->n[1]; ++j)
         for (int i = 0; i < gf->n[0]; ++i) {
           long q = mlr_ci(&L, i, j, k) + n * L.ncell;
           double lof = hc_lofphi(&L, gf->dim, n, i, j, k, alpha, a, beta, b, phi, scratch, dxinv);
@@ -497,4 +507,19 @@ int mlr_hc_be_step(const mlr_grid* g, const mlr_params* p, const mlr_bc* bc,
   int s = mlr_hc_fas_solve(g, p, bc, 1.0, a, -dt, b, phi_new, rhs, o, st);
   free(rhs);
   return s;
+}
+
+/* OpenMP threads of the parallel loops above (test/baseline control; the
+ * results do not depend on it).  0 leaves the current setting. */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+int mlr_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
 }

@@ -71,8 +71,10 @@ void mlr_lin_gsrb(const mlr_grid* g, int ncomp, const mlr_bc* bc, double alpha,
   const double dxinv = 1.0 / (g->dx * g->dx);
   for (int pass = 0; pass <= 1; ++pass) {
     lin_ghosts(g, ncomp, bc, 1, phi);
+    /* the cells of one colour are independent: rows in parallel */
     for (int n = 0; n < ncomp; ++n)
       for (int k = 0; k < L.nz; ++k)
+#pragma omp parallel for schedule(static) if (g->n[0] * g->n[1] > 4096)
         for (int j = 0; j < g->n[1]; ++j) {
           const int imin = abs((j + k + pass) % 2);
           for (int i = imin; i < g->n[0]; i += 2) {
@@ -96,6 +98,7 @@ void mlr_lin_residual(const mlr_grid* g, int ncomp, const mlr_bc* bc,
   lin_ghosts(g, ncomp, bc, homogeneous, phi);
   for (int n = 0; n < ncomp; ++n)
     for (int k = 0; k < L.nz; ++k)
+#pragma omp parallel for schedule(static) if (g->n[0] * g->n[1] > 4096)
       for (int j = 0; j < g->n[1]; ++j)
         for (int i = 0; i < g->n[0]; ++i) {
           const long q = mlr_ci(&L, i, j, k) + n * L.ncell;
@@ -115,6 +118,7 @@ void mlr_lin_apply_op(const mlr_grid* g, int ncomp, const mlr_bc* bc,
   lin_ghosts(g, ncomp, bc, homogeneous, phi);
   for (int n = 0; n < ncomp; ++n)
     for (int k = 0; k < L.nz; ++k)
+#pragma omp parallel for schedule(static) if (g->n[0] * g->n[1] > 4096)
       for (int j = 0; j < g->n[1]; ++j)
         for (int i = 0; i < g->n[0]; ++i) {
           const long q = mlr_ci(&L, i, j, k) + n * L.ncell;
@@ -140,9 +144,13 @@ void mlr_lin_restrict_residual(const mlr_grid* gf, int ncomp,
   const double denom = gf->dim == 3 ? 8.0 : 4.0;
   lin_ghosts(gf, ncomp, bc, 1, phi);
   for (long q = 0; q < ncomp * Lc.ncell; ++q) resc[q] = 0.0;
+  /* a coarse row gathers fine rows 2jc and 2jc+1 only: coarse rows in
+     parallel keep every coarse cell's summation order */
   for (int n = 0; n < ncomp; ++n)
     for (int k = 0; k < Lf.nz; ++k)
-      for (int j = 0; j < gf->n[1]; ++j)
+#pragma omp parallel for schedule(static) if (gf->n[0] * gf->n[1] > 4096)
+      for (int jc = 0; jc < (gf->n[1] + 1) / 2; ++jc)
+      for (int j = 2 * jc; j < 2 * jc + 2 && j < gf->n[1]; ++j)
         for (int i = 0; i < gf->n[0]; ++i) {
           const long q = mlr_ci(&Lf, i, j, k) + n * Lf.ncell;
           const double lof = lin_lof(&Lf, gf->dim, n, i, j, k, alpha, a, beta, c, b, phi, dxinv);

@@ -408,3 +408,31 @@ def test_hc_coefficients_from_porosity():
     np.testing.assert_allclose(b[1][1, 0, 1:4, :], -np.sqrt(cv[:-1] * cv[1:]) / 200.0, rtol=1e-14)
     np.testing.assert_allclose(b[1][1, 0, 0, :], -cv[0] / 200.0, rtol=1e-14)
     np.testing.assert_allclose(b[1][1, 0, 4, :], -cv[-1] / 200.0, rtol=1e-14)
+
+
+# ------------------------------------------------------------- threading
+
+def test_step_independent_of_thread_count():
+    """The oracle's parallel loops run over the cells of one red/black colour
+    (and over independent cells of the residual and closure), so a backward
+    Euler step on 1 thread and on 4 threads gives the same bits."""
+    import importlib
+    W = importlib.import_module("mushy-layer_amd.workloads")
+    case = W.mushy(n=(96, 96), seed=3)
+    g = O.make_grid(case["n"], case["dx"], case["periodic"] + [0])
+    p = O.default_params(**case["params"])
+    bc = O.make_bc(case["bc"])
+    out = []
+    try:
+        for nthr in (1, 4):
+            O.set_threads(nthr)
+            phi = case["phi"].copy()
+            old = phi.copy()
+            b = O.hc_bcoef(g, p, O.porosity_field(g, p, phi))
+            st = O.hc_be_step(g, p, bc, None, b, case["dt"], phi, old,
+                              opts=O.default_opts(max_iter=20, hang=-1.0))
+            out.append((phi, st.iterations))
+    finally:
+        O.set_threads(1)
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0].view(np.uint64), out[1][0].view(np.uint64))
