@@ -150,6 +150,7 @@ class CoupledStepper(HCStepper):
         super().reset()
         for u, u0 in zip(self.u, self.u0):
             self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, u.ptr, u0.ptr, u.nbytes), "copy")
+        self.ctx.check(self.ctx.lib.ml_vec_set(self.ctx.h, self.p.nbytes // 8, self.p.ptr, 0.0), "ml_vec_set")
         self.vstats = []
 
     def set_flags(self, flags):
@@ -266,17 +267,34 @@ def calibration_copy(ml, ctx, mib=64):
     b.free()
 
 
-def field_errors(ml, ctx, case, a, b):
-    """Relative max-norm differences of H and Theta and the absolute
-    porosity difference between two device states."""
+def log(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def rel_max(x, y):
+    """max|x - y| / max|y| (the absolute difference when y is zero)."""
+    d = float(np.max(np.abs(y)))
+    return float(np.max(np.abs(x - y))) / (d if d > 0 else 1.0)
+
+
+def flow_fields(st):
+    """Host copies of a stepper's velocity components and pressure (valid
+    cells), empty without the flow solves."""
+    if not hasattr(st, "u"):
+        return {}
+    out = {"u%d" % d: st.ml.valid(st.n, u.get())[0] for d, u in enumerate(st.u)}
+    out["p"] = st.ml.valid(st.n, st.p.get())[0]
+    return out
+
+
+def field_errors(ml, ctx, case, sa, sb):
+    """Relative max-norm differences of H, Theta, the velocity components and
+    the pressure, and the absolute porosity difference, between the final
+    states of two steppers."""
     n = case["n"]
-    A, B = a.get(), b.get()
-    va, vb = ml.valid(n, A), ml.valid(n, B)
-
-    def rel(x, y):
-        d = float(np.max(np.abs(y)))
-        return float(np.max(np.abs(x - y))) / (d if d > 0 else 1.0)
-
+    a, b = sa.phi, sb.phi
+    va, vb = ml.valid(n, a.get()), ml.valid(n, b.get())
     op = ml.HCOp(ctx, n, case["dx"], case["periodic"], ml.make_params(**case["params"]), ml.make_bc(case["bc"]))
     chi = []
     for s in (a, b):
@@ -285,8 +303,39 @@ def field_errors(ml, ctx, case, a, b):
         chi.append(ml.valid(n, c.get()))
         c.free()
     op.close()
-    return {"H": rel(va[0], vb[0]), "Theta": rel(va[1], vb[1]),
-            "porosity_abs": float(np.max(np.abs(chi[0] - chi[1])))}
+    out = {"H": rel_max(va[0], vb[0]), "Theta": rel_max(va[1], vb[1]),
+           "porosity_abs": float(np.max(np.abs(chi[0] - chi[1])))}
+    fa, fb = flow_fields(sa), flow_fields(sb)
+    for k in fa:
+        out[k] = rel_max(fa[k], fb[k])
+    return out
+
+
+def oracle_parity(ml, ctx, st, cpu):
+    """The FP64 GPU stepper re-run from the initial state for the steps the
+    oracle took in the CPU sample, then compared field by field with the
+    oracle's state: relative max-norm differences of H, Theta, velocity and
+    pressure, the absolute porosity difference (porosity of both states by
+    the oracle's closure), and whether every (H, Theta) solve took the same
+    iterations with the same exit status."""
+    O = cpu.O
+    st.reset()
+    for _ in range(cpu.steps):
+        st.step()
+    ctx.sync()
+    n = st.n
+    g = cpu.g
+    gphi = st.phi.get()
+    vg, vc = ml.valid(n, gphi), ml.valid(n, cpu.phi)
+    chig = O.valid(g, O.porosity_field(g, cpu.p, gphi))
+    chic = O.valid(g, O.porosity_field(g, cpu.p, cpu.phi))
+    out = {"steps": cpu.steps, "H": rel_max(vg[0], vc[0]), "Theta": rel_max(vg[1], vc[1]),
+           "porosity_abs": float(np.max(np.abs(chig - chic)))}
+    for k, v in flow_fields(st).items():
+        ref = cpu.pres if k == "p" else cpu.u[int(k[1:])]
+        out[k] = rel_max(v, ml.valid(n, ref)[0])
+    out["hc_iterations_equal"] = [(x[0], x[1]) for x in st.stats] == cpu.hc_stats
+    return out
 
 
 class CPUStep:
@@ -305,6 +354,8 @@ class CPUStep:
         self.periodic = case["periodic"]
         self.phi = case["phi"].copy()
         self.opts = O.default_opts(max_iter=max_iter, hang=-1.0)
+        self.steps = 0
+        self.hc_stats = []
         if db is not None:
             dim = len(self.n)
             self.bcu = O.make_bc(db["bc_u"])
@@ -323,7 +374,9 @@ class CPUStep:
         O, g = self.O, self.g
         b = O.hc_bcoef(g, self.p, O.porosity_field(g, self.p, self.phi))
         old = self.phi.copy()
-        O.hc_be_step(g, self.p, self.bc, None, b, self.dt, self.phi, old, opts=self.opts)
+        r = O.hc_be_step(g, self.p, self.bc, None, b, self.dt, self.phi, old, opts=self.opts)
+        self.hc_stats.append((r.exit_status, r.iterations))
+        self.steps += 1
         if self.db is None:
             return
         db = self.db
@@ -344,16 +397,14 @@ class CPUStep:
 
 def cpu_sample(O, wl, case, db, seconds, max_iter):
     """Steps of the oracle from the workload's initial state until `seconds`
-    have passed: (steps, wall seconds)."""
+    have passed: (the stepped CPUStep, wall seconds)."""
     st = CPUStep(O, wl, case, db, max_iter)
     t0 = time.perf_counter()
-    steps = 0
     while True:
         st.step()
-        steps += 1
         if time.perf_counter() - t0 >= seconds:
             break
-    return steps, time.perf_counter() - t0
+    return st, time.perf_counter() - t0
 
 
 def host_cores():
@@ -374,15 +425,17 @@ def cpu_baseline(wl, case, db, seconds, max_iter):
     shape = "x".join(map(str, case["n"]))
     what = "step(s)" if db is None else "Darcy-Brinkman step(s) (HC solve, U* solves, projection)"
     O.set_threads(1)
-    steps1, dt1 = cpu_sample(O, wl, case, db, seconds, max_iter)
+    one, dt1 = cpu_sample(O, wl, case, db, seconds, max_iter)
     nthr = O.set_threads(host_cores())
-    stepsn, dtn = cpu_sample(O, wl, case, db, seconds, max_iter)
+    alls, dtn = cpu_sample(O, wl, case, db, seconds, max_iter)
     O.set_threads(1)
-    return {"value": round(cells * steps1 / dt1, 2), "unit": "cell-updates/s", "cores": 1, "kind": "port",
+    steps1, stepsn = one.steps, alls.steps
+    return one, {"value": round(cells * steps1 / dt1, 2), "unit": "cell-updates/s", "cores": 1, "kind": "port",
             "sample": "%d implicit %s of the same %s workload from its initial state, C oracle "
                       "(gcc -O2 -ffp-contract=off -fopenmp), 1 thread, %.1f s" % (steps1, what, shape, dt1),
             "all_cores": {"value": round(cells * stepsn / dtn, 2), "cores": nthr,
-                          "sample": "%d step(s), same oracle, %d OpenMP threads, %.1f s" % (stepsn, nthr, dtn)}}
+                          "sample": "%d step(s), same oracle, %d OpenMP threads, %.1f s" % (stepsn, nthr, dtn)},
+            "processes": 1}
 
 
 def run_form(ml, ctx, case, precision, steps, warmup, max_iter, db=None, no_lookahead=False):
@@ -715,6 +768,7 @@ def main():
     res = {}
     for f in todo:
         st, wall = run_form(ml, ctx, case, forms[f], args.steps, args.warmup, args.max_iter, db, args.no_lookahead)
+        log("%s form: %d steps in %.3f s" % (f, args.steps, wall))
         res[f] = dict(stepper=st, wall=wall, value=cells * args.steps / wall,
                       iters=[s[1] for s in st.stats], status=sorted(set(s[0] for s in st.stats)),
                       vstats=getattr(st, "vstats", None))
@@ -725,9 +779,14 @@ def main():
         roof = kernel_roofline(ml, ctx, res[todo[0]]["stepper"])
     else:
         roof = vc_relax_roofline(ml, ctx, h["stepper"], args.steps)
+    log("roofline pass done")
     large = smoother_bandwidth(ml, wl, ctx, args.large_n) if args.large_n > 0 else None
     calibration_copy(ml, ctx)
-    cpu = None if args.no_cpu else cpu_baseline(wl, case, db, args.cpu_seconds, args.max_iter)
+    cpu, cpu_state = None, None
+    if not args.no_cpu:
+        log("CPU sample (C oracle, 1 thread then all cores)")
+        cpu_state, cpu = cpu_baseline(wl, case, db, args.cpu_seconds, args.max_iter)
+        log("CPU sample done: %d step(s) on 1 thread" % cpu_state.steps)
 
     what = ("HC backward-Euler solve (porosity + coefficient assembly + solve)" if args.hc_only else
             "Darcy-Brinkman step: HC backward-Euler solve + U* Helmholtz solve per velocity component "
@@ -748,8 +807,9 @@ def main():
         cfg["fp64_value"] = round(res["fp64"]["value"], 2)
         cfg["fp64_vcycles_per_step"] = round(float(np.mean(res["fp64"]["iters"])), 2)
         cfg["fp32r_over_fp64"] = round(res["fp32r"]["value"] / res["fp64"]["value"], 3)
-        cfg["fp32r_error_vs_fp64"] = field_errors(ml, ctx, case, res["fp32r"]["stepper"].phi,
-                                                  res["fp64"]["stepper"].phi)
+        cfg["fp32r_error_vs_fp64"] = field_errors(ml, ctx, case, res["fp32r"]["stepper"], res["fp64"]["stepper"])
+    if cpu_state is not None and "fp64" in res:
+        cfg["fp64_vs_oracle"] = oracle_parity(ml, ctx, res["fp64"]["stepper"], cpu_state)
     out = {
         "metric": METRIC, "value": round(h["value"], 2), "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(h["wall"] / args.steps * 1e3, 4),
