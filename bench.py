@@ -435,7 +435,7 @@ def cpu_baseline(wl, case, db, seconds, max_iter):
                       "(gcc -O2 -ffp-contract=off -fopenmp), 1 thread, %.1f s" % (steps1, what, shape, dt1),
             "all_cores": {"value": round(cells * stepsn / dtn, 2), "cores": nthr,
                           "sample": "%d step(s), same oracle, %d OpenMP threads, %.1f s" % (stepsn, nthr, dtn)},
-            "processes": 1}
+            "mpi_ranks": 1}
 
 
 def run_form(ml, ctx, case, precision, steps, warmup, max_iter, db=None, no_lookahead=False):
@@ -774,6 +774,19 @@ def main():
                       vstats=getattr(st, "vstats", None))
     head = "fp32r" if "fp32r" in res else "fp64"
     h = res[head]
+    hc_only = None
+    if db is not None:
+        # the same steps without the flow solves: the (H, Theta) solve alone
+        hc_only = {}
+        for f in todo:
+            st1, wall1 = run_form(ml, ctx, case, forms[f], args.steps, args.warmup, args.max_iter, None,
+                                  args.no_lookahead)
+            hc_only[f + "_value"] = round(cells * args.steps / wall1, 2)
+            hc_only[f + "_ms_per_step"] = round(wall1 / args.steps * 1e3, 4)
+            st1.op.close()
+        if len(todo) == 2:
+            hc_only["fp32r_over_fp64"] = round(hc_only["fp32r_value"] / hc_only["fp64_value"], 3)
+        log("HC-only steps done")
 
     if args.hc_only:
         roof = kernel_roofline(ml, ctx, res[todo[0]]["stepper"])
@@ -796,6 +809,8 @@ def main():
                        % ("x".join(map(str, shape)), what, corr),
            "form": head, "tolerance": "1e-10 rel or abs, max-norm", "max_iter": args.max_iter,
            "iterations_per_step": round(float(np.mean(h["iters"])), 2), "exit_status": h["status"]}
+    if hc_only:
+        cfg["hc_solve_only"] = hc_only
     if h["vstats"]:
         vs = h["vstats"]
         cfg["ustar_iterations_per_solve"] = round(float(np.mean([x[1] for v in vs for x in v[:-1]])), 2)

@@ -34,6 +34,8 @@ HCEngine::~HCEngine() {
   for (float* p : m_lin_owned) hipFree(p);
   if (m_lindesc) hipFree(m_lindesc);
   if (m_hlindesc) hipHostFree(m_hlindesc);
+  m_stage_lin.release();
+  m_stage_fas.release();
   for (size_t l = 1; l < m_phi.size(); ++l) {
     if (m_phi[l]) hipFree(m_phi[l]);
     if (m_rhs[l]) hipFree(m_rhs[l]);
@@ -138,8 +140,7 @@ int HCEngine::write_ctl(const SolveCtl& c) {
 // FP64 FAS.
 int HCEngine::upload_desc(double* phi0, const double* rhs0, const ml_solve_opts& o) {
   ml_ctx ctx = m_op->ctx;
-  ML_HIP(ctx, hipStreamSynchronize(ctx->stream));   // the staging buffer may still be in flight
-  CoarseDesc<double>& D = *m_hdesc;
+  CoarseDesc<double> D;
   std::memset(&D, 0, sizeof(D));
   D.l0 = m_lc;
   D.nlev = m_L;
@@ -161,7 +162,7 @@ int HCEngine::upload_desc(double* phi0, const double* rhs0, const ml_solve_opts&
     L.a = m_op->lev[l].a;
     for (int d = 0; d < 3; ++d) L.b[d] = m_op->lev[l].b[d];
   }
-  ML_HIP(ctx, hipMemcpyAsync(m_desc, m_hdesc, sizeof(D), hipMemcpyHostToDevice, ctx->stream));
+  ML_HIP(ctx, m_stage_fas.put(m_desc, m_hdesc, &D, sizeof(D), ctx->stream));
   return ML_OK;
 }
 
@@ -395,8 +396,7 @@ int HCEngine::solve_refined(double* phi, const double* rhs, const ml_solve_opts&
   }
   J.begin[J.n] = total;
   ML_HIP(ctx, cast_many(J, s));
-  ML_HIP(ctx, hipStreamSynchronize(s));   // the pinned descriptor may still be in flight
-  LinDesc& D = *m_hlindesc;
+  LinDesc D;
   std::memset(&D, 0, sizeof(D));
   D.l0 = m_lc;
   D.nlev = m_L;
@@ -410,7 +410,7 @@ int HCEngine::solve_refined(double* phi, const double* rhs, const ml_solve_opts&
   for (int l = 0; l < m_L; ++l) D.lev[l] = m_lin[l];
   for (int d = 0; d < 3; ++d) D.b64[d] = m_op->lev[m_L - 1].b[d];
   D.a64 = nullptr;
-  ML_HIP(ctx, hipMemcpyAsync(m_lindesc, m_hlindesc, sizeof(D), hipMemcpyHostToDevice, s));
+  ML_HIP(ctx, m_stage_lin.put(m_lindesc, m_hlindesc, &D, sizeof(D), s));
   ML_HIP(ctx, zero_u64(m_slots, o.max_iter + 1, s));
 
   HCArgs<double> A = m_op->args(0);

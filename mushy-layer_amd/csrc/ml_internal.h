@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <memory>
 #include <string>
 #include <vector>
@@ -35,6 +37,32 @@ struct ml_ctx_s {
 };
 
 namespace ml {
+
+// A kernel descriptor sent to the device through pinned staging memory.  The
+// copy is skipped when the descriptor equals the last one sent (the device
+// copy still holds it); otherwise it waits only for the previous copy out of
+// the staging buffer (an event), not for the whole stream.
+struct StagedDesc {
+  std::vector<unsigned char> last;
+  hipEvent_t ev = nullptr;
+  hipError_t put(void* dev, void* pinned, const void* d, size_t bytes, hipStream_t s) {
+    if (last.size() == bytes && std::memcmp(last.data(), d, bytes) == 0) return hipSuccess;
+    last.clear();
+    hipError_t e = ev ? hipEventSynchronize(ev) : hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+    std::memcpy(pinned, d, bytes);
+    if ((e = hipMemcpyAsync(dev, pinned, bytes, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    if ((e = hipEventRecord(ev, s)) != hipSuccess) return e;
+    const unsigned char* b = static_cast<const unsigned char*>(d);
+    last.assign(b, b + bytes);
+    return hipSuccess;
+  }
+  void release() {
+    if (ev) hipEventDestroy(ev);
+    ev = nullptr;
+    last.clear();
+  }
+};
 
 int fail(ml_ctx ctx, int code, const std::string& msg);
 // largest dynamic LDS of one workgroup on the context's device (bytes)
@@ -129,6 +157,7 @@ class HCEngine {
   std::vector<float*> m_lin_owned;
   LinDesc* m_lindesc = nullptr;      // device
   LinDesc* m_hlindesc = nullptr;     // pinned host staging
+  StagedDesc m_stage_lin;
   bool m_refined_ok = false;
   size_t m_lin_lds = 0;               // dynamic LDS of the coarse linear V-cycle
 
@@ -143,6 +172,7 @@ class HCEngine {
   int m_nslots = 0;
   CoarseDesc<double>* m_desc = nullptr;        // device
   CoarseDesc<double>* m_hdesc = nullptr;       // pinned host staging
+  StagedDesc m_stage_fas;
   SolveCtl* m_ctl = nullptr;                   // device iteration control
   unsigned int* m_arrive = nullptr;            // workgroup arrival counter of the fused stopping rule
   SolveCtl* m_hctl = nullptr;                  // pinned mirror
@@ -180,6 +210,7 @@ struct VCWork {
   unsigned long long* slots = nullptr;
   VCDesc<double>* desc = nullptr;       // device
   VCDesc<double>* hdesc = nullptr;      // pinned staging
+  StagedDesc stage;
   SolveCtl* ctl = nullptr;              // device iteration control
   SolveCtl* hctl = nullptr;             // pinned mirror
   unsigned int* arrive = nullptr;       // workgroup arrival counter of the fused stopping rule

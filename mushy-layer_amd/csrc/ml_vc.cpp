@@ -44,6 +44,7 @@ void vc_work_free(ml::VCWork& W) {
   if (W.slots) hipFree(W.slots);
   if (W.desc) hipFree(W.desc);
   if (W.hdesc) hipHostFree(W.hdesc);
+  W.stage.release();
   if (W.ctl) hipFree(W.ctl);
   if (W.hctl) hipHostFree(W.hctl);
   if (W.arrive) hipFree(W.arrive);
@@ -115,8 +116,7 @@ int vc_work_ensure(ml_vc_op op, int n) {
 
 int vc_upload_desc(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o) {
   ml_ctx ctx = op->ctx;
-  ML_HIP(ctx, hipStreamSynchronize(W.stream));   // the staging buffer may still be in flight
-  ml::VCDesc<double>& D = *W.hdesc;
+  ml::VCDesc<double> D;
   std::memset(&D, 0, sizeof(D));
   D.l0 = op->lc;
   D.nlev = int(op->lev.size());
@@ -137,7 +137,7 @@ int vc_upload_desc(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o) {
     L.e = W.e0[l];
     L.r = W.r[l];
   }
-  ML_HIP(ctx, hipMemcpyAsync(W.desc, W.hdesc, sizeof(D), hipMemcpyHostToDevice, W.stream));
+  ML_HIP(ctx, W.stage.put(W.desc, W.hdesc, &D, sizeof(D), W.stream));
   return ML_OK;
 }
 
