@@ -333,50 +333,77 @@ __global__ __launch_bounds__(kThreads) void hc_tile_kernel(HCArgs<T> A, TileIO<T
 // exactly the cells it updated (the other colour's phi_l is unchanged since
 // its own update), so each pass needs a single barrier and half the closures
 // of a full recomputation, with the same values.
+// The level's geometry is copied into registers: reads through the
+// descriptor pointer would be repeated after every barrier or wavefront fence
+// of the relaxation passes.
 template <typename T, int DIM>
 struct WGLevel {
   const LevelDesc<T>* L;
   int N, nx, nxy;
+  int n[3], per[3];
+  T dx;
+  long long ncell;
   int c, i, j, k;
   long long q;
   bool own;
   __device__ inline void init(const LevelDesc<T>* lv) {
     L = lv;
     const Geom& g = L->g;
-    nx = g.n[0];
-    nxy = g.n[0] * g.n[1];
-    N = nxy * (DIM == 3 ? g.n[2] : 1);
+    for (int d = 0; d < 3; ++d) {
+      n[d] = g.n[d];
+      per[d] = g.periodic[d];
+    }
+    dx = T(g.dx);
+    ncell = g.ncell;
+    nx = n[0];
+    nxy = n[0] * n[1];
+    N = nxy * (DIM == 3 ? n[2] : 1);
     c = threadIdx.x;
     own = c < N;
     i = c % nx;
     const int r = c / nx;
-    j = r % g.n[1];
-    k = DIM == 3 ? r / g.n[1] : 0;
+    j = r % n[1];
+    k = DIM == 3 ? r / n[1] : 0;
     q = own ? g.ci(i, j, k) : 0;
   }
 };
 
+// The descriptor's scalars that every pass uses, in registers for the same
+// reason.
+template <typename T>
+struct WGConst {
+  const BCDesc* bc;   // read at wall cells only
+  ClosureP<T> cp;
+  T alpha, beta;
+  __device__ inline void init(const CoarseDesc<T>& D) {
+    bc = &D.bc;
+    cp = D.cp;
+    alpha = D.alpha;
+    beta = D.beta;
+  }
+};
+
 template <typename T, int DIM>
-__device__ inline void wg_stencil(const Geom& g, const BCDesc& bc, const T* pl, int c, int i, int j, int k, int n,
-                                  int nx, int nxy, T* p) {
-  const int iv[3] = {i, j, k};
+__device__ inline void wg_stencil(const WGLevel<T, DIM>& W, const BCDesc& bc, const T* pl, int n, T* p) {
+  const int c = W.c;
+  const int iv[3] = {W.i, W.j, W.k};
   p[0] = pl[c];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    const int nd = g.n[d];
-    const int st = d == 0 ? 1 : (d == 1 ? nx : nxy);
+    const int nd = W.n[d];
+    const int st = d == 0 ? 1 : (d == 1 ? W.nx : W.nxy);
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       const int v = iv[d] + (s ? 1 : -1);
       T val;
       if (v >= 0 && v < nd) {
         val = pl[c + (s ? st : -st)];
-      } else if (g.periodic[d]) {
+      } else if (W.per[d]) {
         val = pl[c + (s ? -(nd - 1) * st : (nd - 1) * st)];
       } else {
         const T x1 = pl[c];
         val = x1;
-        if (bc.apply) val = bc_ghost<T>(bc.type[d][s][n], T(bc.val[d][s][n]), x1, s ? 1 : -1, T(g.dx));
+        if (bc.apply) val = bc_ghost<T>(bc.type[d][s][n], T(bc.val[d][s][n]), x1, s ? 1 : -1, W.dx);
       }
       p[1 + 2 * d + (s ? 0 : 1)] = val;
     }
@@ -403,15 +430,14 @@ __device__ inline void wg_faces(const LevelDesc<T>* L, int i, int j, int k, Cell
 
 // L(phi) of the thread's cell (both components) from phi/phi_l in LDS
 template <typename T, int DIM>
-__device__ inline void wg_lof(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, const CellFaces<T, DIM>& F,
+__device__ inline void wg_lof(const WGConst<T>& K, const WGLevel<T, DIM>& W, const CellFaces<T, DIM>& F,
                               const T* aq, const T* sphi, const T* sphl, T* lof) {
-  const Geom& g = W.L->g;
-  const T dxinv = T(1) / (T(g.dx) * T(g.dx));
+  const T dxinv = T(1) / (W.dx * W.dx);
 #pragma unroll
   for (int n = 0; n < 2; ++n) {
     T st[7];
-    wg_stencil<T, DIM>(g, D.bc, sphl + n * W.N, W.c, W.i, W.j, W.k, n, W.nx, W.nxy, st);
-    lof[n] = lof_of<T, DIM>(F, n, st, aq[n], sphi[n * W.N + W.c], D.alpha, D.beta, dxinv);
+    wg_stencil<T, DIM>(W, *K.bc, sphl + n * W.N, n, st);
+    lof[n] = lof_of<T, DIM>(F, n, st, aq[n], sphi[n * W.N + W.c], K.alpha, K.beta, dxinv);
   }
 }
 
@@ -420,12 +446,12 @@ template <typename T, int DIM>
 struct WGCellOp {
   CellFaces<T, DIM> F;
   T aq[2], lam[2];
-  __device__ inline void load(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W) {
+  __device__ inline void load(const WGConst<T>& K, const WGLevel<T, DIM>& W) {
     if (!W.own) return;
     wg_faces<T, DIM>(W.L, W.i, W.j, W.k, F, aq);
-    const T scale = T(1) / (T(W.L->g.dx) * T(W.L->g.dx));
-    lam[0] = lambda_of<T, DIM>(F, 0, aq[0], D.alpha, D.beta, scale);
-    lam[1] = lambda_of<T, DIM>(F, 1, aq[1], D.alpha, D.beta, scale);
+    const T scale = T(1) / (W.dx * W.dx);
+    lam[0] = lambda_of<T, DIM>(F, 0, aq[0], K.alpha, K.beta, scale);
+    lam[1] = lambda_of<T, DIM>(F, 1, aq[1], K.alpha, K.beta, scale);
   }
 };
 
@@ -433,19 +459,19 @@ struct WGCellOp {
 // sphi and whose phi_l (closure of sphi) is staged in sphl; rr = rhs of the
 // thread's cell.  Leaves sphi/sphl consistent and writes phi to global memory.
 template <typename T, int DIM>
-__device__ void wg_relax(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, const WGCellOp<T, DIM>& O,
+__device__ void wg_relax(const WGConst<T>& K, const WGLevel<T, DIM>& W, const WGCellOp<T, DIM>& O,
                          const T* rr, T* sphi, T* sphl, int npass) {
   auto pass = [&](int p) {
     if (W.own && !((W.i + W.j + W.k + p) & 1)) {
       T lof[2];
-      wg_lof<T, DIM>(D, W, O.F, O.aq, sphi, sphl, lof);
+      wg_lof<T, DIM>(K, W, O.F, O.aq, sphi, sphl, lof);
       const int c = W.c;
       const T h = sphi[c] - O.lam[0] * (lof[0] - rr[0]);
       const T t = sphi[W.N + c] - O.lam[1] * (lof[1] - rr[1]);
       sphi[c] = h;
       sphi[W.N + c] = t;
       T th, cl;
-      closure<T>(h, t, D.cp, th, cl);
+      closure<T>(h, t, K.cp, th, cl);
       sphl[c] = th;
       sphl[W.N + c] = cl;
     }
@@ -467,19 +493,19 @@ __device__ void wg_relax(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, const
   }
   if (W.own) {
     W.L->phi[W.q] = sphi[W.c];
-    W.L->phi[W.q + W.L->g.ncell] = sphi[W.N + W.c];
+    W.L->phi[W.q + W.ncell] = sphi[W.N + W.c];
   }
 }
 
 // stage the thread's (phi) into LDS and compute every closure
 template <typename T, int DIM>
-__device__ inline void wg_stage(const CoarseDesc<T>& D, const WGLevel<T, DIM>& W, const T* ph, T* sphi, T* sphl) {
+__device__ inline void wg_stage(const WGConst<T>& K, const WGLevel<T, DIM>& W, const T* ph, T* sphi, T* sphl) {
   if (W.own) {
     sphi[W.c] = ph[0];
     sphi[W.N + W.c] = ph[1];
   }
   __syncthreads();
-  wg_closure_own<T, DIM>(D.cp, W, sphi, sphl);
+  wg_closure_own<T, DIM>(K.cp, W, sphi, sphl);
   __syncthreads();
 }
 
@@ -497,6 +523,9 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const CoarseDesc<T>& D = *Dp;
   if (D.skip && *D.skip) return;
+  WGConst<T> K;
+  K.init(D);
+  const int nu1 = D.nu1, nu2 = D.nu2, bottom = D.bottom;
   T* sphi = reinterpret_cast<T*>(smem_raw);   // 2*kWGCells
   T* sphl = sphi + 2 * kWGCells;              // 2*kWGCells
   T* se = sphl + 2 * kWGCells;                // coarse correction, 2*kWGCells/4
@@ -516,11 +545,11 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
     }
     if (apply_first) {
       WGCellOp<T, DIM> O;
-      O.load(D, W);
-      wg_stage<T, DIM>(D, W, ph, sphi, sphl);
+      O.load(K, W);
+      wg_stage<T, DIM>(K, W, ph, sphi, sphl);
       if (W.own) {
         T lof[2];
-        wg_lof<T, DIM>(D, W, O.F, O.aq, sphi, sphl, lof);
+        wg_lof<T, DIM>(K, W, O.F, O.aq, sphi, sphl, lof);
         rr[0] = rr[0] + lof[0];
         rr[1] = rr[1] + lof[1];
         W.L->rhs[W.q] = rr[0];
@@ -534,15 +563,15 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
     WGLevel<T, DIM> W;
     W.init(&D.lev[l]);
     WGCellOp<T, DIM> O;
-    O.load(D, W);
-    wg_stage<T, DIM>(D, W, ph, sphi, sphl);
-    wg_relax<T, DIM>(D, W, O, rr, sphi, sphl, 2 * D.nu1);
+    O.load(K, W);
+    wg_stage<T, DIM>(K, W, ph, sphi, sphl);
+    wg_relax<T, DIM>(K, W, O, rr, sphi, sphl, 2 * nu1);
     // restriction (RESTRICTRESNLVC / RESTRICT): each fine cell's
     // (rhs - L phi)/2^d and phi/2^d, then child sums in Fortran order
     T tr[2] = {T(0), T(0)}, tp[2] = {T(0), T(0)};
     if (W.own) {
       T lof[2];
-      wg_lof<T, DIM>(D, W, O.F, O.aq, sphi, sphl, lof);
+      wg_lof<T, DIM>(K, W, O.F, O.aq, sphi, sphl, lof);
       tr[0] = (rr[0] - lof[0]) / denom;
       tr[1] = (rr[1] - lof[1]) / denom;
       tp[0] = sphi[W.c] / denom;
@@ -586,11 +615,11 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
     // FAS coarse right-hand side rhs_{l+1} += L(phi_{l+1})
     {
       WGCellOp<T, DIM> Oc;
-      Oc.load(D, Wc);
-      wg_stage<T, DIM>(D, Wc, ph, sphi, sphl);
+      Oc.load(K, Wc);
+      wg_stage<T, DIM>(K, Wc, ph, sphi, sphl);
       if (Wc.own) {
         T lof[2];
-        wg_lof<T, DIM>(D, Wc, Oc.F, Oc.aq, sphi, sphl, lof);
+        wg_lof<T, DIM>(K, Wc, Oc.F, Oc.aq, sphi, sphl, lof);
         rr[0] = rr[0] + lof[0];
         rr[1] = rr[1] + lof[1];
         Wc.L->rhs[Wc.q] = rr[0];
@@ -603,9 +632,9 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
     WGLevel<T, DIM> W;
     W.init(&D.lev[L - 1]);
     WGCellOp<T, DIM> O;
-    O.load(D, W);
-    wg_stage<T, DIM>(D, W, ph, sphi, sphl);
-    wg_relax<T, DIM>(D, W, O, rr, sphi, sphl, 2 * D.bottom);
+    O.load(K, W);
+    wg_stage<T, DIM>(K, W, ph, sphi, sphl);
+    wg_relax<T, DIM>(K, W, O, rr, sphi, sphl, 2 * bottom);
   }
   for (int l = L - 2; l >= l0; --l) {
     // coarse correction phi_{l+1} - phi0_{l+1} into LDS (compact layout)
@@ -620,7 +649,7 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
     WGLevel<T, DIM> W;
     W.init(&D.lev[l]);
     WGCellOp<T, DIM> O;
-    O.load(D, W);
+    O.load(K, W);
     if (W.own) {
       const long long nc = W.L->g.ncell;
       rr[0] = W.L->rhs[W.q];
@@ -629,8 +658,8 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
       ph[1] = prolong_value_lds<T, DIM>(Wc.L->g, W.i, W.j, W.k, se + Wc.N, W.L->phi[W.q + nc]);
     }
     __syncthreads();
-    wg_stage<T, DIM>(D, W, ph, sphi, sphl);
-    wg_relax<T, DIM>(D, W, O, rr, sphi, sphl, 2 * D.nu2);
+    wg_stage<T, DIM>(K, W, ph, sphi, sphl);
+    wg_relax<T, DIM>(K, W, O, rr, sphi, sphl, 2 * nu2);
   }
 }
 
