@@ -744,7 +744,7 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build measures one GPU only (single-GPU solve, no collectives); "
+        print("bench.py: the solve runs on one GPU (single-GPU solve, no collectives); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     ml = importlib.import_module("mushy-layer_amd")
