@@ -216,3 +216,45 @@ def test_relax_parity_many_tiles(ml, gpu_ctx, wl, name, iters):
     op.relax(0, dphi, drhs, iters)
     out = dphi.get()
     assert HC.bitwise_equal(HC.valid(n, out), HC.valid(n, ref))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32r"])
+def test_steps_follow_changed_options_and_dt(ml, gpu_ctx, wl, precision):
+    """Consecutive backward-Euler steps on one operator with different
+    smoothing counts and then a different dt: each solve uses its own settings
+    (the device descriptor is resent when they change and reused when they do
+    not).  FP64: bit-identical to the oracle with the same iterations;
+    FP32-refined: within 1e-9 of the oracle's FP64 solution."""
+    if precision == "fp64":
+        case, g, p, bc, b, op, db = _setup(ml, gpu_ctx, wl, "2d_96x40")
+        max_iter = 30
+    else:
+        # a case whose FP64 solves converge, so that both forms can be held
+        # to the tolerance
+        case = wl.mushy(n=(64, 64))
+        g, p, bc = HC.oracle_setup(case)
+        b = HC.bcoef_from_state(case)
+        op = HC.gpu_setup(ml, gpu_ctx, case)
+        db = [ml.DeviceArray(gpu_ctx, x.shape, host=x) for x in b]
+        max_iter = 100
+    n = case["n"]
+    prec = ml.PRECISION_FP64 if precision == "fp64" else ml.PRECISION_FP32_REFINED
+    old = case["phi"].copy()
+    dold = ml.DeviceArray(gpu_ctx, old.shape, host=old)
+    runs = [(dict(nu1=1, nu2=4, bottom=40), case["dt"]), (dict(nu1=2, nu2=2, bottom=10), case["dt"]),
+            (dict(nu1=2, nu2=2, bottom=10), case["dt"]), (dict(nu1=1, nu2=4, bottom=40), 2.0 * case["dt"])]
+    for k, (sm, dt) in enumerate(runs):
+        kw = dict(max_iter=max_iter, hang=-1.0, **sm)
+        new = old.copy()
+        st_ref = O.hc_be_step(g, p, bc, None, b, dt, new, old, opts=O.default_opts(**kw))
+        op.set_coefs(None, db, 1.0, -dt)
+        dnew = ml.DeviceArray(gpu_ctx, old.shape, host=old)
+        st = op.solve_be(dnew, dold, dt, opts=ml.default_opts(precision=prec, **kw))
+        out = dnew.get()
+        if precision == "fp64":
+            assert (st.iterations, st.exit_status) == (st_ref.iterations, st_ref.exit_status), k
+            assert HC.bitwise_equal(HC.valid(n, out), HC.valid(n, new)), k
+        else:
+            assert st_ref.exit_status & (1 | 8) and st.exit_status & (1 | 8), (k, st_ref.exit_status, st.exit_status)
+            e = HC.rel_err(HC.valid(n, out), HC.valid(n, new))
+            assert e <= 1e-9, (k, e)

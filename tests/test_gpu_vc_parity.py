@@ -229,3 +229,27 @@ def test_darcy_and_projection_coefficients(ml, gpu_ctx, n):
     op.porosity_faces(dchi, -1.0, db)
     for d in range(len(n)):
         assert HC.bitwise_equal(db[d].get(), w["b_p"][d])
+
+
+def test_solves_follow_changed_options_and_coefficients(ml, gpu_ctx):
+    """Consecutive solves on one operator with different smoothing counts and
+    then a different beta: each solve uses its own settings (the device
+    descriptor is resent when they change and reused when they do not) and
+    matches the oracle bit for bit."""
+    C, w, g, bc, op, keep = _setup(ml, gpu_ctx, "db_2d_64")
+    n = w["n"]
+    a, c, b = C["alpha"], C["c"], C["b"]
+    rhs = _rand(g, 11)
+    drhs = ml.DeviceArray(gpu_ctx, rhs.shape, host=rhs)
+    runs = [(dict(nu1=2, nu2=4, bottom=8), C["beta"]), (dict(nu1=1, nu2=2, bottom=4), C["beta"]),
+            (dict(nu1=1, nu2=2, bottom=4), C["beta"]), (dict(nu1=2, nu2=4, bottom=8), 2.0 * C["beta"])]
+    for k, (sm, be) in enumerate(runs):
+        kw = dict(max_iter=6, eps=1e-10, hang=1e-10, norm_thresh=1e-10, **sm)
+        if be != C["beta"]:
+            op.set_coefs(None, keep[0], keep[1], a, be)
+        phi = np.zeros(O.cell_shape(g, 1))
+        st_ref = O.lin_mg_solve(g, 1, bc, a, None, be, c, b, phi, rhs, opts=O.default_opts(**kw))
+        dphi = ml.DeviceArray(gpu_ctx, phi.shape, host=np.zeros_like(phi))
+        st = op.solve(dphi, drhs, opts=ml.default_opts(**kw))
+        assert (st.iterations, st.exit_status) == (st_ref.iterations, st_ref.exit_status), k
+        _check(n, dphi.get(), phi)
