@@ -360,10 +360,10 @@ struct WGLevel {
     N = nxy * (DIM == 3 ? n[2] : 1);
     c = threadIdx.x;
     own = c < N;
-    i = c % nx;
-    const int r = c / nx;
-    j = r % n[1];
-    k = DIM == 3 ? r / n[1] : 0;
+    int r, kk;
+    divmod_pos(c, nx, r, i);
+    divmod_pos(r, n[1], kk, j);
+    k = DIM == 3 ? kk : 0;
     q = own ? g.ci(i, j, k) : 0;
   }
 };

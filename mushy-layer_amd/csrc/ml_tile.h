@@ -193,11 +193,24 @@ struct CLev {
   double dx;
 };
 
+// q = x / d, r = x % d for x >= 0, d > 0: a shift and a mask when d is a
+// power of two (every level of a power-of-two grid), integer division
+// otherwise.  d is uniform across the workgroup, so the branch is too.
+__device__ __forceinline__ void divmod_pos(int x, int d, int& q, int& r) {
+  if ((d & (d - 1)) == 0) {
+    const int s = __builtin_ctz(d);
+    q = x >> s;
+    r = x & (d - 1);
+  } else {
+    q = x / d;
+    r = x - q * d;
+  }
+}
+
 __device__ inline void clev_coords(const CLev& L, int c, int& i, int& j, int& k) {
-  i = c % L.nx;
-  const int r = c / L.nx;
-  j = r % L.n[1];
-  k = r / L.n[1];
+  int r;
+  divmod_pos(c, L.nx, r, i);
+  divmod_pos(r, L.n[1], k, j);
 }
 
 __device__ inline void clev_set(CLev& L, const Geom& g, int off) {
@@ -243,7 +256,9 @@ __device__ inline void cown_set(const CLev& L, COwn& o) {
   if ((L.nx & 1) == 0) {
     const int h = L.nx >> 1;
     if (t < (L.N >> 1)) {
-      const int s = t % h, r = t / h, j = r % L.n[1], k = r / L.n[1];
+      int s, r, j, k;
+      divmod_pos(t, h, r, s);
+      divmod_pos(r, L.n[1], k, j);
       const int par = (j + k) & 1;
       o.p[0] = cpack(2 * s + par, j, k);
       o.p[1] = cpack(2 * s + 1 - par, j, k);
