@@ -385,9 +385,14 @@ struct WGConst {
 
 template <typename T, int DIM>
 __device__ inline void wg_stencil(const WGLevel<T, DIM>& W, const BCDesc& bc, const T* pl, int n, T* p) {
+  // without branches on the cell's position: the cells of one colour lie on
+  // different sides of the walls, so branches would diverge in a wavefront.
+  // Across a wall the load reads the cell itself and the ghost formula's
+  // result is selected; the values are those of the branching form.
   const int c = W.c;
   const int iv[3] = {W.i, W.j, W.k};
-  p[0] = pl[c];
+  const T x0 = pl[c];
+  p[0] = x0;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     const int nd = W.n[d];
@@ -395,17 +400,12 @@ __device__ inline void wg_stencil(const WGLevel<T, DIM>& W, const BCDesc& bc, co
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       const int v = iv[d] + (s ? 1 : -1);
-      T val;
-      if (v >= 0 && v < nd) {
-        val = pl[c + (s ? st : -st)];
-      } else if (W.per[d]) {
-        val = pl[c + (s ? -(nd - 1) * st : (nd - 1) * st)];
-      } else {
-        const T x1 = pl[c];
-        val = x1;
-        if (bc.apply) val = bc_ghost<T>(bc.type[d][s][n], T(bc.val[d][s][n]), x1, s ? 1 : -1, W.dx);
-      }
-      p[1 + 2 * d + (s ? 0 : 1)] = val;
+      const bool inside = v >= 0 && v < nd;
+      const bool wall = !inside && !W.per[d];
+      const int off = inside ? (s ? st : -st) : (wall ? 0 : (s ? -(nd - 1) * st : (nd - 1) * st));
+      const T nb = pl[c + off];
+      const T gh = bc_ghost<T>(bc.type[d][s][n], T(bc.val[d][s][n]), x0, s ? 1 : -1, W.dx);
+      p[1 + 2 * d + (s ? 0 : 1)] = (wall && bc.apply) ? gh : nb;
     }
   }
 }
