@@ -108,8 +108,9 @@ __host__ __device__ inline void closure(T H, T C, const ClosureP<T>& p, T& theta
     Cl = p.Te;
   } else if (H > he && H < hl) {
     T chi = porosity_mush(C, H, p);
-    theta = -(C + p.CR * (T(1) - chi)) / (chi + p.pc * (T(1) - chi));
+    // one division: (-x)/y and -(x/y) are the same IEEE value
     Cl = (C + p.CR * (T(1) - chi)) / (chi + p.pc * (T(1) - chi));
+    theta = -Cl;
   } else if (H >= hl) {
     theta = H - p.St;
     Cl = C;
