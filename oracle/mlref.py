@@ -303,6 +303,13 @@ def prolong_constant(g, ncomp, phif, corrc):
     return phif
 
 
+def average_cells(g, ncomp, af, ratio):
+    gc = coarsen(g, ratio)
+    out = np.zeros(cell_shape(gc, ncomp))
+    lib().mlr_average_cells(C.byref(g), ncomp, ptr(np.ascontiguousarray(af)), ratio, ptr(out))
+    return out
+
+
 def average_faces(g, ncomp, bf, ratio):
     gc = coarsen(g, ratio)
     out = [np.zeros(face_shape(gc, d, ncomp)) for d in range(g.dim)]

@@ -436,3 +436,90 @@ def test_step_independent_of_thread_count():
         O.set_threads(1)
     assert out[0][1] == out[1][1]
     assert np.array_equal(out[0][0].view(np.uint64), out[1][0].view(np.uint64))
+
+
+# ------------------------------------------- FAS restriction, cell coarsening
+
+def _mushy_state(g, rng):
+    """A state with all four closure regimes present (solid, eutectic, mush,
+    liquid), so the restricted residual goes through the nonlinear closure."""
+    phi = np.zeros(O.cell_shape(g, 2))
+    v = O.valid(g, phi)
+    v[0] = rng.uniform(-0.5, 7.5, v[0].shape)
+    v[1] = rng.uniform(-1.9, -0.2, v[1].shape)
+    return phi
+
+
+@pytest.mark.parametrize("n,periodic,spec", [
+    ((8, 8), (1, 0, 0), {(1, 0, 0): (0, -0.5), (1, 1, 0): (0, 1.2), (1, 0, 1): (1, 0.3), (1, 1, 1): (0, -1.0)}),
+    ((8, 6), (0, 0, 0), {(0, 0, 0): (1, 0.2), (0, 1, 1): (0, -0.7), (1, 0, 0): (0, -1.0), (1, 1, 1): (1, -0.1)}),
+    ((6, 8), (1, 1, 0), {}),
+    ((4, 6, 8), (1, 0, 0), {(2, 0, 0): (0, -0.4), (2, 1, 0): (0, 1.1), (1, 0, 1): (1, 0.2), (1, 1, 1): (0, -0.9)}),
+    ((8, 4, 4), (0, 1, 0), {(0, 0, 0): (0, 0.3), (0, 1, 1): (1, 0.5), (2, 0, 0): (1, -0.2), (2, 1, 1): (0, -1.2)}),
+])
+def test_fas_restricted_residual_is_child_average_of_residual(n, periodic, spec):
+    """restrictResidual (RESTRICTRESNLVC, src/AMRNonLinearMultiCompOpF.ChF:452-542)
+    gives every coarse cell the mean of the 2^d fine residuals rhs - L(phi)
+    it covers, with the same (inhomogeneous) boundary data as residualI.
+    Compared with an average taken here of the oracle's full residual, on a
+    state with every closure regime, random aCoef and coefficients, walls
+    of both kinds and periodic sides, in 2D and 3D."""
+    g = O.make_grid(list(n), 1.0 / n[0], list(periodic))
+    p = params()
+    bc = O.make_bc(spec)
+    rng = np.random.default_rng(sum(n) + 7 * sum(periodic))
+    phi = _mushy_state(g, rng)
+    rhs = np.zeros(O.cell_shape(g, 2))
+    O.valid(g, rhs)[...] = O.valid(g, phi) + 1e-2 * rng.standard_normal(O.valid(g, rhs).shape)
+    a = np.zeros(O.cell_shape(g, 2))
+    O.valid(g, a)[...] = rng.uniform(0.5, 2.0, O.valid(g, a).shape)
+    b = [-rng.uniform(0.1, 1.0, O.face_shape(g, d, 2)) for d in range(g.dim)]
+    alpha, beta = 1.3, -2e-3
+    res = O.hc_residual(g, p, bc, alpha, a, beta, b, phi.copy(), rhs)
+    resc = O.hc_restrict_residual(g, p, bc, alpha, a, beta, b, phi.copy(), rhs)
+    gc = O.coarsen(g, 2)
+    v = O.valid(g, res)
+    if g.dim == 2:
+        nx, ny = n
+        avg = v.reshape(2, 1, ny // 2, 2, nx // 2, 2).mean(axis=(3, 5))
+    else:
+        nx, ny, nz = n
+        avg = v.reshape(2, nz // 2, 2, ny // 2, 2, nx // 2, 2).mean(axis=(2, 4, 6))
+    scale = np.abs(v).max()
+    assert scale > 1e-3    # a non-trivial residual
+    np.testing.assert_allclose(O.valid(gc, resc), avg, rtol=0, atol=1e-13 * scale)
+    # the ghost cells of the coarse residual are untouched (zero)
+    assert np.count_nonzero(resc) == np.count_nonzero(O.valid(gc, resc))
+
+
+@pytest.mark.parametrize("n", [(16, 16), (32, 8), (8, 16, 16), (16, 8, 8)])
+@pytest.mark.parametrize("ratio", [2, 4, 8])
+def test_cell_average_is_brute_force_mean(n, ratio):
+    """CoarseAverage of a cell field (aCoef / cCoef coarsening,
+    src/AMRNonLinearMultiCompOp.cpp:1448-1458, src/DarcyBrinkmanOp.cpp:977-1008):
+    each coarse cell is the mean of the ratio^d fine cells it covers,
+    computed here by explicit loops over the fine cells."""
+    g = O.make_grid(list(n), 1.0 / n[0], [0, 0, 0])
+    rng = np.random.default_rng(ratio * 100 + len(n))
+    ncomp = 2
+    af = rng.standard_normal(O.cell_shape(g, ncomp))
+    out = O.average_cells(g, ncomp, af, ratio)
+    gc = O.coarsen(g, ratio)
+    fv = O.valid(g, af)
+    nz = n[2] if len(n) == 3 else 1
+    rz = ratio if len(n) == 3 else 1
+    expect = np.zeros((ncomp, nz // rz, n[1] // ratio, n[0] // ratio))
+    for c in range(ncomp):
+        for K in range(nz // rz):
+            for J in range(n[1] // ratio):
+                for I in range(n[0] // ratio):
+                    s = 0.0
+                    for k in range(K * rz, K * rz + rz):
+                        for j in range(J * ratio, J * ratio + ratio):
+                            for i in range(I * ratio, I * ratio + ratio):
+                                s += fv[c, k, j, i]
+                    expect[c, K, J, I] = s / (ratio * ratio * rz)
+    np.testing.assert_allclose(O.valid(gc, out), expect, rtol=1e-14, atol=1e-15)
+    # a constant field is reproduced exactly
+    one = np.full(O.cell_shape(g, 1), 0.7)
+    np.testing.assert_allclose(O.valid(gc, O.average_cells(g, 1, one, ratio)), 0.7, rtol=1e-15)
