@@ -286,6 +286,11 @@ typedef struct ml_hc_batch_s* ml_hc_batch;
 int ml_hc_batch_create(ml_ctx ctx, int count, const ml_hc_op ops[], ml_hc_batch* out);
 int ml_hc_batch_destroy(ml_hc_batch b);
 int ml_hc_batch_size(ml_hc_batch b);
+/* Device memory of the batch's set-up: the bytes of its one device
+ * allocation, and how many device allocations and fill launches set-up made
+ * (one each: every buffer of the batch is carved out of that allocation and
+ * zeroed by one fill). */
+int ml_hc_batch_memory(ml_hc_batch b, size_t* bytes, int* allocs, int* fills);
 /* Tile shape of the batched FP32 relaxation (results do not depend on it).
  * By default the first step times every compiled shape on the batch's own
  * launches and keeps the fastest (the autotuner); set_tile fixes shape

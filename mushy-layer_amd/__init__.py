@@ -117,6 +117,7 @@ SIGNATURES = {
     "ml_hc_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp)]),
     "ml_hc_batch_destroy": (C.c_int, [_vp]),
     "ml_hc_batch_size": (C.c_int, [_vp]),
+    "ml_hc_batch_memory": (C.c_int, [_vp, C.POINTER(C.c_size_t), _ip, _ip]),
     "ml_hc_batch_set_tile": (C.c_int, [_vp, C.c_int]),
     "ml_hc_batch_tile": (C.c_int, [_vp, _ip, _ip, _ip, _ip, C.POINTER(C.c_float), C.c_int]),
     "ml_hc_batch_step": (C.c_int, [_vp, C.POINTER(_vp), C.c_double, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
@@ -445,6 +446,13 @@ class HCBatch:
         self.ctx.check(self.ctx.lib.ml_hc_batch_step(self.h, (_vp * n)(*[x.ptr for x in phis]), dt, C.byref(o), st),
                        "ml_hc_batch_step")
         return list(st)
+
+    def memory(self):
+        """(device bytes, device allocations, fill launches) of the batch's set-up."""
+        nb, na, nf = C.c_size_t(), C.c_int(), C.c_int()
+        self.ctx.check(self.ctx.lib.ml_hc_batch_memory(self.h, C.byref(nb), C.byref(na), C.byref(nf)),
+                       "ml_hc_batch_memory")
+        return nb.value, na.value, nf.value
 
     def set_tile(self, variant):
         """Fix the relaxation tile shape (-1: tune again at the next step)."""

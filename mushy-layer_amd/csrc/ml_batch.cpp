@@ -20,6 +20,7 @@
 // converged returns at once from every later launch (its done flag).
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "ml_internal.h"
@@ -60,7 +61,12 @@ struct ml_hc_batch_s {
   std::vector<ml::BatchInst> inst;
   int K = 0;
   size_t coarse_lds = 0;
-  std::vector<void*> owned;
+  // device memory: the buffers requested during set-up are carved out of
+  // one allocation and zeroed by one fill launch (counts kept on the host)
+  unsigned char* dev_block = nullptr;
+  size_t dev_bytes = 0;
+  std::vector<std::function<void(unsigned char*)>> dev_req;   // sets one buffer's pointer from the base
+  int dev_allocs = 0, dev_fills = 0;
   ml::SolveCtl* ctl = nullptr;
   ml::SolveCtl* hctl = nullptr;           // pinned mirror
   unsigned long long* slots = nullptr;
@@ -119,11 +125,31 @@ namespace {
 
 int batch_fail(ml_hc_batch b, int code, const std::string& msg) { return fail(b->ctx, code, msg); }
 
-int dalloc(ml_hc_batch b, size_t bytes, void** p) {
+// Requests `bytes` of zeroed device memory for *field; the pointer is set
+// by commit_device() once every buffer of the batch has been requested.
+template <typename T>
+int dalloc(ml_hc_batch b, size_t bytes, T** field) {
+  *field = nullptr;
+  const size_t off = b->dev_bytes;
+  b->dev_req.push_back([field, off](unsigned char* base) { *field = reinterpret_cast<T*>(base + off); });
+  b->dev_bytes = align256(b->dev_bytes + bytes);
+  return ML_OK;
+}
+
+// one hipMalloc for every requested buffer, one fill launch to zero it;
+// then the pointers that set-up derived from them
+int commit_device(ml_hc_batch b) {
   ml_ctx ctx = b->ctx;
-  ML_HIP(ctx, hipMalloc(p, bytes));
-  b->owned.push_back(*p);
-  ML_HIP(ctx, hipMemsetAsync(*p, 0, bytes, ctx->stream));
+  void* v = nullptr;
+  ML_HIP(ctx, hipMalloc(&v, b->dev_bytes));
+  b->dev_block = static_cast<unsigned char*>(v);
+  ++b->dev_allocs;
+  for (const auto& set : b->dev_req) set(b->dev_block);
+  ML_HIP(ctx, vec_set((long long)(b->dev_bytes / sizeof(double)), reinterpret_cast<double*>(b->dev_block), 0.0,
+                      ctx->stream));
+  ++b->dev_fills;
+  for (size_t i = 0; i < b->inst.size(); ++i)
+    for (LinLevel<float>& Lv : b->inst[i].lin) Lv.skip = &b->ctl[i].done;
   return ML_OK;
 }
 
@@ -131,7 +157,7 @@ void batch_free(ml_hc_batch b) {
   if (!b) return;
   if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  for (void* p : b->owned) (void)hipFree(p);
+  if (b->dev_block) (void)hipFree(b->dev_block);
   if (b->hctl) (void)hipHostFree(b->hctl);
   if (b->harena) (void)hipHostFree(b->harena);
   if (b->hcmap) (void)hipHostFree(b->hcmap);
@@ -176,44 +202,26 @@ int inst_init(ml_hc_batch b, int i, ml_hc_op op, size_t maxlds) {
   b->coarse_lds = std::max(b->coarse_lds, lds);
   I.lin.assign(I.L, LinLevel<float>{});
   I.ealt.assign(I.L, nullptr);
-  auto fl = [&](size_t n, float** p) -> int {
-    void* v = nullptr;
-    int rc = dalloc(b, n * sizeof(float), &v);
-    *p = static_cast<float*>(v);
-    return rc;
-  };
+  auto fl = [&](size_t n, auto** field) -> int { return dalloc(b, n * sizeof(float), field); };
   int rc;
   for (int l = 0; l < I.L; ++l) {
     LinLevel<float>& Lv = I.lin[l];
     const Geom& g = op->lev[l].g;
     Lv.g = g;
-    float* p = nullptr;
-    if ((rc = fl(2 * g.ncell, &p))) return rc;
-    Lv.e = p;
-    if ((rc = fl(2 * g.ncell, &p))) return rc;
-    Lv.r = p;
-    if ((rc = fl(4 * g.ncell, &p))) return rc;
-    Lv.M = p;
+    if ((rc = fl(2 * g.ncell, &Lv.e))) return rc;
+    if ((rc = fl(2 * g.ncell, &Lv.r))) return rc;
+    if ((rc = fl(4 * g.ncell, &Lv.M))) return rc;
     for (int d = 0; d < 3; ++d) {
       Lv.b[d] = nullptr;
-      if (d < g.dim) {
-        if ((rc = fl(2 * g.nface[d], &p))) return rc;
-        Lv.b[d] = p;
-      }
+      if (d < g.dim && (rc = fl(2 * g.nface[d], &Lv.b[d]))) return rc;
     }
     Lv.a = nullptr;
-    Lv.skip = &b->ctl[i].done;
-    if (l < I.lc) {
-      if ((rc = fl(2 * g.ncell, &p))) return rc;
-      I.ealt[l] = p;
-    }
+    Lv.skip = nullptr;   // &ctl[i].done, set once the device memory exists
+    if (l < I.lc && (rc = fl(2 * g.ncell, &I.ealt[l]))) return rc;
   }
-  void* v = nullptr;
   const size_t bytes0 = sizeof(double) * 2 * op->lev[0].g.ncell;
-  if ((rc = dalloc(b, bytes0, &v))) return rc;
-  I.alt0 = static_cast<double*>(v);
-  if ((rc = dalloc(b, bytes0, &v))) return rc;
-  I.rhs = static_cast<double*>(v);
+  if ((rc = dalloc(b, bytes0, &I.alt0))) return rc;
+  if ((rc = dalloc(b, bytes0, &I.rhs))) return rc;
   I.nres = lin_residual_tiles2(op->lev[0].g, I.res_ntx, I.res_nty);
   return ML_OK;
 }
@@ -311,24 +319,19 @@ int layout(ml_hc_batch b) {
   int rc;
   b->cmap_cap = size_t(b->res_blocks);
   for (int k = 0; k < b->K; ++k) b->cmap_cap += size_t(most(b->down[k])) + size_t(most(b->up[k]));
-  if ((rc = dalloc(b, sizeof(unsigned) * b->cmap_cap, &v))) return rc;
-  b->cmap = static_cast<unsigned*>(v);
+  if ((rc = dalloc(b, sizeof(unsigned) * b->cmap_cap, &b->cmap))) return rc;
   ML_HIP(ctx, hipHostMalloc(&v, sizeof(unsigned) * b->cmap_cap, hipHostMallocDefault));
   b->hcmap = static_cast<unsigned*>(v);
-  if ((rc = dalloc(b, sizeof(int) * n, &v))) return rc;
-  b->tlist = static_cast<int*>(v);
-  if ((rc = dalloc(b, sizeof(unsigned) * size_t(b->res_blocks), &v))) return rc;
-  b->tmap = static_cast<unsigned*>(v);
-  if ((rc = dalloc(b, sizeof(unsigned long long) * n, &v))) return rc;
-  b->tnorm = static_cast<unsigned long long*>(v);
+  if ((rc = dalloc(b, sizeof(int) * n, &b->tlist))) return rc;
+  if ((rc = dalloc(b, sizeof(unsigned) * size_t(b->res_blocks), &b->tmap))) return rc;
+  if ((rc = dalloc(b, sizeof(unsigned long long) * n, &b->tnorm))) return rc;
   ML_HIP(ctx, hipHostMalloc(&v, sizeof(int) * n, hipHostMallocDefault));
   b->htlist = static_cast<int*>(v);
   ML_HIP(ctx, hipHostMalloc(&v, sizeof(unsigned) * size_t(b->res_blocks), hipHostMallocDefault));
   b->htmap = static_cast<unsigned*>(v);
   ML_HIP(ctx, hipHostMalloc(&v, sizeof(unsigned long long) * n, hipHostMallocDefault));
   b->htnorm = static_cast<unsigned long long*>(v);
-  if ((rc = dalloc(b, off, &v))) return rc;
-  b->arena = static_cast<unsigned char*>(v);
+  if ((rc = dalloc(b, off, &b->arena))) return rc;
   ML_HIP(ctx, hipHostMalloc(&v, off, hipHostMallocDefault));
   b->harena = static_cast<unsigned char*>(v);
   std::memset(b->harena, 0, off);
@@ -718,17 +721,10 @@ int ml_hc_batch_create(ml_ctx ctx, int count, const ml_hc_op ops[], ml_hc_batch*
   for (int i = 0; i < count && rc == ML_OK; ++i)
     if (!ops[i] || ops[i]->ctx != ctx) rc = fail(ctx, ML_EINVAL, "ml_hc_batch_create: null operator or another context");
   if (rc == ML_OK) rc = max_lds(ctx, &maxlds);
-  if (rc == ML_OK) rc = dalloc(b, sizeof(SolveCtl) * count, &v);
+  if (rc == ML_OK) rc = dalloc(b, sizeof(SolveCtl) * count, &b->ctl);
+  if (rc == ML_OK) rc = dalloc(b, sizeof(unsigned long long) * kBatchSlots * count, &b->slots);
+  if (rc == ML_OK) rc = dalloc(b, sizeof(unsigned int) * count, &b->arrive);
   if (rc == ML_OK) {
-    b->ctl = static_cast<SolveCtl*>(v);
-    rc = dalloc(b, sizeof(unsigned long long) * kBatchSlots * count, &v);
-  }
-  if (rc == ML_OK) {
-    b->slots = static_cast<unsigned long long*>(v);
-    rc = dalloc(b, sizeof(unsigned int) * count, &v);
-  }
-  if (rc == ML_OK) {
-    b->arrive = static_cast<unsigned int*>(v);
     if (hipHostMalloc(&v, sizeof(SolveCtl) * count, hipHostMallocDefault) != hipSuccess)
       rc = fail(ctx, ML_ENOMEM, "hipHostMalloc");
     else
@@ -736,6 +732,7 @@ int ml_hc_batch_create(ml_ctx ctx, int count, const ml_hc_op ops[], ml_hc_batch*
   }
   for (int i = 0; i < count && rc == ML_OK; ++i) rc = inst_init(b, i, ops[i], maxlds);
   if (rc == ML_OK) rc = layout(b);
+  if (rc == ML_OK) rc = commit_device(b);
   if (rc == ML_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, ML_EHIP, "batch set-up");
   if (rc != ML_OK) {
     batch_free(b);
@@ -752,6 +749,14 @@ int ml_hc_batch_destroy(ml_hc_batch b) {
 }
 
 int ml_hc_batch_size(ml_hc_batch b) { return b ? int(b->inst.size()) : ML_EINVAL; }
+
+int ml_hc_batch_memory(ml_hc_batch b, size_t* bytes, int* allocs, int* fills) {
+  if (!b) return ML_EINVAL;
+  if (bytes) *bytes = b->dev_bytes;
+  if (allocs) *allocs = b->dev_allocs;
+  if (fills) *fills = b->dev_fills;
+  return ML_OK;
+}
 
 int ml_hc_batch_set_tile(ml_hc_batch b, int variant) {
   if (!b || variant < -1 || variant >= lin_relax_batch_tiles()) return ML_EINVAL;
