@@ -179,6 +179,17 @@ int ml_hc_compute_diffused_var(ml_hc_op op, int depth, const double* phi, double
                                int homogeneous);
 /* relax -> levelGSRB x iters (src/AMRNonLinearMultiCompOp.cpp:815-949) */
 int ml_hc_relax(ml_hc_op op, int depth, double* phi, const double* rhs, int iters);
+/* No counterpart in the reference: ml_hc_relax for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels and same results as
+ * ml_hc_relax, but the smoother's ping-pong partner array (2 components of
+ * the level's cells, FP64) is taken from a static device array of
+ * ML_NOALLOC_SCRATCH_BYTES per GPU instead of being allocated; no
+ * allocating call is made.  A level whose state exceeds that size is
+ * rejected (ML_ENOTSUP).  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (other contexts or host threads) race. */
+#define ML_NOALLOC_SCRATCH_BYTES (8u << 20)
+int ml_hc_relax_noalloc(ml_hc_op op, int depth, double* phi, const double* rhs, int iters);
 /* residualI (src/AMRNonLinearMultiCompOp.cpp:100-166): res = rhs - L(phi) */
 int ml_hc_residual(ml_hc_op op, int depth, double* res, double* phi, const double* rhs,
                    int homogeneous);

@@ -89,6 +89,7 @@ SIGNATURES = {
     "ml_hc_diagonal_scale": (C.c_int, [_vp, C.c_int, _vp]),
     "ml_hc_compute_diffused_var": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
     "ml_hc_relax": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
+    "ml_hc_relax_noalloc": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
     "ml_hc_residual": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int]),
     "ml_hc_apply_op": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
     "ml_hc_restrict_residual": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
@@ -345,6 +346,12 @@ class HCOp:
 
     def relax(self, depth, phi, rhs, iters):
         self.ctx.check(self.ctx.lib.ml_hc_relax(self.h, depth, phi.ptr, rhs.ptr, iters), "ml_hc_relax")
+
+    def relax_noalloc(self, depth, phi, rhs, iters):
+        """relax() with the partner array in the library's static scratch
+        (no device allocation)."""
+        self.ctx.check(self.ctx.lib.ml_hc_relax_noalloc(self.h, depth, phi.ptr, rhs.ptr, iters),
+                       "ml_hc_relax_noalloc")
 
     def residual(self, depth, res, phi, rhs, homogeneous=0):
         self.ctx.check(self.ctx.lib.ml_hc_residual(self.h, depth, res.ptr, phi.ptr, rhs.ptr, homogeneous),

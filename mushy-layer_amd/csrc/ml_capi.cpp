@@ -78,6 +78,8 @@ int ml_ctx_create(int device, unsigned flags, ml_ctx* out) {
   if (hipMalloc(&c->d_norm, 64 * sizeof(unsigned long long)) != hipSuccess) return ML_ENOMEM;
   if (hipHostMalloc(&c->h_norm, 64 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
     return ML_ENOMEM;
+  // the no-allocation entry point's static scratch: its address on this device
+  if (ml::scratch_init(c.get(), device) != ML_OK) return ML_EHIP;
   *out = c.release();
   return ML_OK;
 }

@@ -65,6 +65,9 @@ struct StagedDesc {
 };
 
 int fail(ml_ctx ctx, int code, const std::string& msg);
+// device address of the no-allocation scratch array on `device` (the current
+// device), looked up once per device (ml_noalloc.cpp)
+int scratch_init(ml_ctx ctx, int device);
 // largest dynamic LDS of one workgroup on the context's device (bytes)
 int max_lds(ml_ctx ctx, size_t* out);
 bool valid_grid(int dim, const int n[], const int periodic[], double dx);
