@@ -1,18 +1,25 @@
 """Benchmark: cell-updates/s per implicit step to a 1e-10 residual on one
 MI355X.
 
-Workload (one GPU): 256x256 mushy layer (porosity from a seeded smooth
-Gaussian random field in [0.05, 1], correlation length 0.1; St = 5, CR = 2,
-Le = 200, dt = 1e-3).  One step = porosity update -> HC face-coefficient
-assembly -> backward-Euler solve of (H, Theta) to |r| <= 1e-10 |r0| (or
-1e-10 absolute), all on the device.  cell-updates/s = cells * steps / wall
-time of the timed steps; inputs are resident in HBM before timing.
+Workload (one GPU): one Darcy-Brinkman mushy-layer step on a 256x256 grid
+(porosity from a seeded smooth Gaussian random field in [0.05, 1],
+correlation length 0.1; St = 5, CR = 2, Le = 200, Pr = 10, Da = 1e-3,
+Ra_C = 200, Kozeny-Carman permeability, dt = 1e-3; velocity and pressure
+start at 0): enthalpy variables -> HC face coefficients -> backward-Euler
+solve of (H, Theta) to |r| <= 1e-10 |r0| (or 1e-10 absolute) -> enthalpy
+variables at the new time -> U* source (buoyancy - porosity grad P) and
+Darcy coefficient -> U* backward-Euler solve per velocity component ->
+projection rhs div(u)/dt -> variable-coefficient projection solve ->
+projection correction, all on the device (mushy-layer_amd/steps.py).
+cell-updates/s = cells * steps / wall time of the timed steps; inputs are
+resident in HBM before timing.  --dim 3: the 128^3 layer (correlation length
+0.08, Ra_C 300, Da 5e-4).
 
-Two forms are timed on the same steps: the plain FP64 FAS solve and the
-FP32-refined form (FP32 smoothing and transfers inside FP64 defect
-correction).  `value` is the FP32-refined form's rate; the FP64 rate, the
-ratio and the per-field difference of the two final states are reported
-beside it.
+Two forms are timed on the same steps: the (H, Theta) solve in plain FP64
+FAS and in the FP32-refined form (FP32 smoothing and transfers inside FP64
+defect correction); the velocity solves are FP64 in both.  `value` is the
+FP32-refined form's rate; the FP64 rate, the ratio and the per-field
+difference of the two final states are reported beside it.
 
 Also reported: the roofline of the dominant kernel of the FP32-refined step,
 the tiled red/black relaxation of the Darcy-Brinkman / projection operator
@@ -21,7 +28,8 @@ second timed pass of the same steps is bracketed by HIP events on the
 library's stream, and achieved = summed algorithmic bytes / summed launch
 time; `traffic` (HBM bytes per launch) comes from the rocprofv3 --pmc
 FETCH_SIZE / WRITE_SIZE runs recorded in profiles/ (null when absent).  And a
-CPU baseline (the C oracle on this host, 1 thread, bounded sample).
+CPU baseline (the C oracle on this host, 1 thread and all cores, bounded
+sample of the same step).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -72,110 +80,6 @@ def parse():
     return args
 
 
-class HCStepper:
-    """Device-resident state and one implicit step of the (H, Theta) solve."""
-
-    def __init__(self, ml, ctx, case, opts):
-        self.ml, self.ctx, self.case, self.opts = ml, ctx, case, opts
-        n = case["n"]
-        self.n = n
-        self.dt = case["dt"]
-        self.op = ml.HCOp(ctx, n, case["dx"], case["periodic"], ml.make_params(**case["params"]),
-                          ml.make_bc(case["bc"]))
-        shp = case["phi"].shape
-        self.init = ml.DeviceArray(ctx, shp, host=case["phi"])
-        self.phi = ml.DeviceArray(ctx, shp, host=case["phi"])
-        self.old = ml.DeviceArray(ctx, shp)
-        self.chi = ml.DeviceArray(ctx, (1,) + shp[1:])
-        self.b = [ml.DeviceArray(ctx, ml.face_shape(n, d, 2)) for d in range(len(n))]
-        self.stats = []
-
-    def reset(self):
-        self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, self.phi.ptr, self.init.ptr, self.phi.nbytes), "copy")
-        self.stats = []
-
-    def set_flags(self, flags):
-        self.opts.flags = flags
-
-    def step(self):
-        op = self.op
-        op.porosity(self.phi, self.chi)
-        op.assemble_bcoef(self.chi, self.b)
-        op.set_coefs(None, self.b, 1.0, -self.dt)
-        self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, self.old.ptr, self.phi.ptr, self.phi.nbytes), "copy")
-        st = op.solve_be(self.phi, self.old, self.dt, opts=self.opts)
-        self.stats.append((st.exit_status, st.iterations, st.final_residual))
-
-
-class CoupledStepper(HCStepper):
-    """One Darcy-Brinkman mushy-layer step: the (H, Theta) backward-Euler
-    solve, then from the new porosity the U* backward-Euler Helmholtz solve
-    of each velocity component (a = 1, b = -Pr, c = (Pr/Da) chi/Pi) and the
-    variable-coefficient projection solve (b = -chi_face), all on the device.
-    The U* source (buoyancy-like Ra_C (chi - mean chi) on the vertical
-    component) and the projection right-hand side (a zero-mean smooth field)
-    are synthetic: the advective and velocity-source terms that would produce
-    them are outside the solve."""
-
-    U_OPTS = dict(nu1=2, nu2=2, bottom=8, max_iter=10, eps=1e-10, hang=1e-10, norm_thresh=1e-10)
-    P_OPTS = dict(nu1=2, nu2=4, bottom=4, max_iter=20, eps=1e-15, hang=1e-15, norm_thresh=1e-15)
-
-    def __init__(self, ml, ctx, case, opts, db):
-        super().__init__(ml, ctx, case, opts)
-        n = self.n
-        dim = len(n)
-        self.db = db
-        self.uop = ml.VCOp(ctx, n, case["dx"], case["periodic"], ml.make_bc(db["bc_u"]))
-        self.pop = ml.VCOp(ctx, n, case["dx"], case["periodic"], ml.make_bc(db["bc_p"]))
-        shp1 = (1,) + case["phi"].shape[1:]
-        self.u0 = [ml.DeviceArray(ctx, shp1, host=db["u0"]) for _ in range(dim)]
-        self.u = [ml.DeviceArray(ctx, shp1, host=db["u0"]) for _ in range(dim)]
-        self.uold = [ml.DeviceArray(ctx, shp1) for _ in range(dim)]
-        zero = np.zeros(shp1)
-        self.src = [ml.DeviceArray(ctx, shp1, host=(db["src"] if d == dim - 1 else zero)) for d in range(dim)]
-        self.c = ml.DeviceArray(ctx, shp1)
-        self.bu = [ml.DeviceArray(ctx, x.shape, host=x) for x in db["b_u"]]
-        self.bp = [ml.DeviceArray(ctx, ml.face_shape(n, d, 1)) for d in range(dim)]
-        rp = np.zeros(shp1)
-        v = ml.valid(n, rp)
-        v[0] = ml.valid(n, db["u0"])[0]
-        v[0] -= v[0].mean()
-        self.prhs = ml.DeviceArray(ctx, shp1, host=rp)
-        self.p = ml.DeviceArray(ctx, shp1)
-        self.uopts = ml.default_opts(**self.U_OPTS)
-        self.popts = ml.default_opts(**self.P_OPTS)
-        self.vstats = []
-
-    def reset(self):
-        super().reset()
-        for u, u0 in zip(self.u, self.u0):
-            self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, u.ptr, u0.ptr, u.nbytes), "copy")
-        self.ctx.check(self.ctx.lib.ml_vec_set(self.ctx.h, self.p.nbytes // 8, self.p.ptr, 0.0), "ml_vec_set")
-        self.vstats = []
-
-    def set_flags(self, flags):
-        super().set_flags(flags)
-        self.uopts.flags = flags
-        self.popts.flags = flags
-
-    def step(self):
-        super().step()
-        self.op.porosity(self.phi, self.chi)
-        self.uop.darcy_ccoef(self.chi, self.db["Pr"] / self.db["Da"], self.c)
-        self.uop.set_coefs(None, self.bu, self.c, 1.0, -self.dt)
-        # the velocity components' U* solves share the operator and run
-        # concurrently
-        for u, uo in zip(self.u, self.uold):
-            self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, uo.ptr, u.ptr, u.nbytes), "copy")
-        sts = self.uop.solve_be_many(self.u, self.uold, self.dt, src=self.src, opts=self.uopts)
-        vs = [(st.exit_status, st.iterations) for st in sts]
-        self.pop.porosity_faces(self.chi, -1.0, self.bp)
-        self.pop.set_coefs(None, self.bp, None, 0.0, 1.0)
-        st = self.pop.solve(self.p, self.prhs, opts=self.popts)
-        vs.append((st.exit_status, st.iterations))
-        self.vstats.append(vs)
-
-
 def kernel_roofline(ml, ctx, stepper, reps=200):
     """Average duration of the fused closure + smoother sweep on the finest
     level (HIP events on the library stream) and its algorithmic bandwidth:
@@ -212,9 +116,10 @@ def smoother_bandwidth(ml, wl, ctx, n, launches=20):
     events; achieved = algorithmic bytes / kernel time."""
     out = {"n": [n, n]}
     case = wl.mushy(n=(n, n))
-    st = HCStepper(ml, ctx, case, ml.default_opts())
-    st.op.porosity(st.phi, st.chi)
-    st.op.assemble_bcoef(st.chi, st.b)
+    stp = importlib.import_module("mushy-layer_amd.steps")
+    st = stp.HCStepper(ml, ctx, case, ml.default_opts())
+    st.op.enthalpy_vars(st.phi, st.vold)
+    st.op.assemble_bcoef(st.vold.comp_ptr(3), st.b)
     st.op.set_coefs(None, st.b, 1.0, -st.dt)
     ctx.check(ctx.lib.ml_buf_copy(ctx.h, st.old.ptr, st.phi.ptr, st.phi.nbytes), "copy")
     st.op.relax(0, st.phi, st.old, 1)
@@ -281,11 +186,7 @@ def rel_max(x, y):
 def flow_fields(st):
     """Host copies of a stepper's velocity components and pressure (valid
     cells), empty without the flow solves."""
-    if not hasattr(st, "u"):
-        return {}
-    out = {"u%d" % d: st.ml.valid(st.n, u.get())[0] for d, u in enumerate(st.u)}
-    out["p"] = st.ml.valid(st.n, st.p.get())[0]
-    return out
+    return st.flow_fields() if hasattr(st, "flow_fields") else {}
 
 
 def field_errors(ml, ctx, case, sa, sb):
@@ -293,18 +194,9 @@ def field_errors(ml, ctx, case, sa, sb):
     the pressure, and the absolute porosity difference, between the final
     states of two steppers."""
     n = case["n"]
-    a, b = sa.phi, sb.phi
-    va, vb = ml.valid(n, a.get()), ml.valid(n, b.get())
-    op = ml.HCOp(ctx, n, case["dx"], case["periodic"], ml.make_params(**case["params"]), ml.make_bc(case["bc"]))
-    chi = []
-    for s in (a, b):
-        c = ml.DeviceArray(ctx, (1,) + s.shape[1:])
-        op.porosity(s, c)
-        chi.append(ml.valid(n, c.get()))
-        c.free()
-    op.close()
+    va, vb = ml.valid(n, sa.phi.get()), ml.valid(n, sb.phi.get())
     out = {"H": rel_max(va[0], vb[0]), "Theta": rel_max(va[1], vb[1]),
-           "porosity_abs": float(np.max(np.abs(chi[0] - chi[1])))}
+           "porosity_abs": float(np.max(np.abs(sa.porosity() - sb.porosity())))}
     fa, fb = flow_fields(sa), flow_fields(sb)
     for k in fa:
         out[k] = rel_max(fa[k], fb[k])
@@ -316,8 +208,8 @@ def oracle_parity(ml, ctx, st, cpu):
     oracle took in the CPU sample, then compared field by field with the
     oracle's state: relative max-norm differences of H, Theta, velocity and
     pressure, the absolute porosity difference (porosity of both states by
-    the oracle's closure), and whether every (H, Theta) solve took the same
-    iterations with the same exit status."""
+    the oracle's closure), and whether every solve took the same iterations
+    with the same exit status."""
     O = cpu.O
     st.reset()
     for _ in range(cpu.steps):
@@ -335,70 +227,17 @@ def oracle_parity(ml, ctx, st, cpu):
         ref = cpu.pres if k == "p" else cpu.u[int(k[1:])]
         out[k] = rel_max(v, ml.valid(n, ref)[0])
     out["hc_iterations_equal"] = [(x[0], x[1]) for x in st.stats] == cpu.hc_stats
+    if hasattr(cpu, "vstats"):
+        out["velocity_iterations_equal"] = st.vstats == cpu.vstats
     return out
 
 
-class CPUStep:
-    """The step of the GPU steppers restated with the C oracle: the (H, Theta)
-    backward-Euler solve and, with `db`, the U* Helmholtz solve of each
-    velocity component and the projection solve from the new porosity (the
-    Darcy coefficients formed from the porosity on the host)."""
-
-    def __init__(self, O, wl, case, db, max_iter):
-        self.O, self.wl, self.db = O, wl, db
-        self.n = list(case["n"])
-        self.g = O.make_grid(self.n, case["dx"], case["periodic"] + [0] * (3 - len(case["periodic"])))
-        self.p = O.default_params(**case["params"])
-        self.bc = O.make_bc(case["bc"])
-        self.dt = case["dt"]
-        self.periodic = case["periodic"]
-        self.phi = case["phi"].copy()
-        self.opts = O.default_opts(max_iter=max_iter, hang=-1.0)
-        self.steps = 0
-        self.hc_stats = []
-        if db is not None:
-            dim = len(self.n)
-            self.bcu = O.make_bc(db["bc_u"])
-            self.bcp = O.make_bc(db["bc_p"])
-            self.u = [db["u0"].copy() for _ in range(dim)]
-            self.src = [db["src"] if d == dim - 1 else np.zeros_like(db["src"]) for d in range(dim)]
-            self.prhs = np.zeros_like(db["u0"])
-            v = O.valid(self.g, self.prhs)
-            v[0] = O.valid(self.g, db["u0"])[0]
-            v[0] -= v[0].mean()
-            self.pres = np.zeros_like(db["u0"])
-            self.uopts = O.default_opts(**CoupledStepper.U_OPTS)
-            self.popts = O.default_opts(**CoupledStepper.P_OPTS)
-
-    def step(self):
-        O, g = self.O, self.g
-        b = O.hc_bcoef(g, self.p, O.porosity_field(g, self.p, self.phi))
-        old = self.phi.copy()
-        r = O.hc_be_step(g, self.p, self.bc, None, b, self.dt, self.phi, old, opts=self.opts)
-        self.hc_stats.append((r.exit_status, r.iterations))
-        self.steps += 1
-        if self.db is None:
-            return
-        db = self.db
-        chi = O.porosity_field(g, self.p, self.phi)
-        cv = O.valid(g, chi)[0].reshape(tuple(reversed(self.n)))
-        with np.errstate(divide="ignore", invalid="ignore"):
-            perm = cv ** 3 / (1.0 - cv) ** 2
-            cval = np.where(np.isinf(perm), 0.0, cv / np.where(perm == 0, 1.0, perm)) * (db["Pr"] / db["Da"])
-        c = np.zeros_like(chi)
-        O.valid(g, c)[0] = cval
-        for u, src in zip(self.u, self.src):
-            rhs = np.zeros_like(u)
-            O.valid(g, rhs)[...] = O.valid(g, u) * 1.0 + self.dt * O.valid(g, src)
-            O.lin_mg_solve(g, 1, self.bcu, 1.0, None, -self.dt, c, db["b_u"], u, rhs, opts=self.uopts)
-        bp = [-f for f in self.wl.geometric_faces(self.n, self.periodic, cv)]
-        O.lin_mg_solve(g, 1, self.bcp, 0.0, None, 1.0, None, bp, self.pres, self.prhs, opts=self.popts)
-
-
-def cpu_sample(O, wl, case, db, seconds, max_iter):
+def cpu_sample(case, flow, seconds, max_iter):
     """Steps of the oracle from the workload's initial state until `seconds`
-    have passed: (the stepped CPUStep, wall seconds)."""
-    st = CPUStep(O, wl, case, db, max_iter)
+    have passed: (the stepped oracle step, wall seconds)."""
+    from oracle import steps as S
+    st = S.HCStep(case, max_iter) if flow is None else S.DarcyBrinkmanStep(case, flow, max_iter)
+    st.O = S.O
     t0 = time.perf_counter()
     while True:
         st.step()
@@ -414,7 +253,7 @@ def host_cores():
     return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
 
 
-def cpu_baseline(wl, case, db, seconds, max_iter):
+def cpu_baseline(case, flow, seconds, max_iter):
     """The C oracle timed on a bounded sample of the same workload, on one
     OpenMP thread (the reported value) and on all host cores (`all_cores`).
     Cells within a red/black colour are independent, so both runs compute the
@@ -423,11 +262,11 @@ def cpu_baseline(wl, case, db, seconds, max_iter):
 
     cells = int(np.prod(case["n"]))
     shape = "x".join(map(str, case["n"]))
-    what = "step(s)" if db is None else "Darcy-Brinkman step(s) (HC solve, U* solves, projection)"
+    what = "step(s)" if flow is None else "Darcy-Brinkman step(s) (HC solve, U* solves, projection)"
     O.set_threads(1)
-    one, dt1 = cpu_sample(O, wl, case, db, seconds, max_iter)
+    one, dt1 = cpu_sample(case, flow, seconds, max_iter)
     nthr = O.set_threads(host_cores())
-    alls, dtn = cpu_sample(O, wl, case, db, seconds, max_iter)
+    alls, dtn = cpu_sample(case, flow, seconds, max_iter)
     O.set_threads(1)
     steps1, stepsn = one.steps, alls.steps
     return one, {"value": round(cells * steps1 / dt1, 2), "unit": "cell-updates/s", "cores": 1, "kind": "port",
@@ -438,9 +277,10 @@ def cpu_baseline(wl, case, db, seconds, max_iter):
             "mpi_ranks": 1}
 
 
-def run_form(ml, ctx, case, precision, steps, warmup, max_iter, db=None, no_lookahead=False):
+def run_form(ml, ctx, case, precision, steps, warmup, max_iter, flow=None, no_lookahead=False):
+    stp = importlib.import_module("mushy-layer_amd.steps")
     opts = ml.default_opts(max_iter=max_iter, hang=-1.0, precision=precision)
-    st = HCStepper(ml, ctx, case, opts) if db is None else CoupledStepper(ml, ctx, case, opts, db)
+    st = stp.HCStepper(ml, ctx, case, opts) if flow is None else stp.DarcyBrinkmanStepper(ml, ctx, case, opts, flow)
     if no_lookahead:
         st.set_flags(ml.SOLVE_NO_LOOKAHEAD)
     for _ in range(warmup):
@@ -760,14 +600,14 @@ def main():
     shape = (nn,) * dim
     corr, Ra_C, Da = (0.1, 200.0, 1e-3) if dim == 2 else (0.08, 300.0, 5e-4)
     case = wl.mushy(n=shape, corr=corr)
-    db = None if args.hc_only else wl.darcy_brinkman(n=shape, Pr=10.0, Da=Da, corr=corr, Ra_C=Ra_C)
+    flow = None if args.hc_only else wl.flow(dim=dim, Pr=10.0, Da=Da, Ra_C=Ra_C)
     ctx = ml.Context(0)
     cells = int(np.prod(case["n"]))
     forms = {"fp64": ml.PRECISION_FP64, "fp32r": ml.PRECISION_FP32_REFINED}
     todo = ["fp64", "fp32r"] if args.form == "both" else [args.form]
     res = {}
     for f in todo:
-        st, wall = run_form(ml, ctx, case, forms[f], args.steps, args.warmup, args.max_iter, db, args.no_lookahead)
+        st, wall = run_form(ml, ctx, case, forms[f], args.steps, args.warmup, args.max_iter, flow, args.no_lookahead)
         log("%s form: %d steps in %.3f s" % (f, args.steps, wall))
         res[f] = dict(stepper=st, wall=wall, value=cells * args.steps / wall,
                       iters=[s[1] for s in st.stats], status=sorted(set(s[0] for s in st.stats)),
@@ -775,7 +615,7 @@ def main():
     head = "fp32r" if "fp32r" in res else "fp64"
     h = res[head]
     hc_only = None
-    if db is not None:
+    if flow is not None:
         # the same steps without the flow solves: the (H, Theta) solve alone
         hc_only = {}
         for f in todo:
@@ -783,7 +623,7 @@ def main():
                                   args.no_lookahead)
             hc_only[f + "_value"] = round(cells * args.steps / wall1, 2)
             hc_only[f + "_ms_per_step"] = round(wall1 / args.steps * 1e3, 4)
-            st1.op.close()
+            st1.close()
         if len(todo) == 2:
             hc_only["fp32r_over_fp64"] = round(hc_only["fp32r_value"] / hc_only["fp64_value"], 3)
         log("HC-only steps done")
@@ -798,16 +638,21 @@ def main():
     cpu, cpu_state = None, None
     if not args.no_cpu:
         log("CPU sample (C oracle, 1 thread then all cores)")
-        cpu_state, cpu = cpu_baseline(wl, case, db, args.cpu_seconds, args.max_iter)
+        cpu_state, cpu = cpu_baseline(case, flow, args.cpu_seconds, args.max_iter)
         log("CPU sample done: %d step(s) on 1 thread" % cpu_state.steps)
 
-    what = ("HC backward-Euler solve (porosity + coefficient assembly + solve)" if args.hc_only else
-            "Darcy-Brinkman step: HC backward-Euler solve + U* Helmholtz solve per velocity component "
-            "(Pr 10, Da %g, Ra_C %g, Kozeny-Carman) + variable-coefficient projection, coefficients assembled "
-            "on the device from the porosity" % (Da, Ra_C))
-    cfg = {"workload": "mushy layer %s, %s, GRF porosity corr %g, St 5, CR 2, Le 200, dt 1e-3"
-                       % ("x".join(map(str, shape)), what, corr),
-           "form": head, "tolerance": "1e-10 rel or abs, max-norm", "max_iter": args.max_iter,
+    what = ("HC backward-Euler solve (enthalpy variables + coefficient assembly + solve)" if args.hc_only else
+            "Darcy-Brinkman step: HC backward-Euler solve, enthalpy variables, U* source (buoyancy - porosity "
+            "grad P^{n-1/2}) and Darcy coefficient (Kozeny-Carman, Pr 10, Da %g, Ra_C %g), U* Helmholtz solve per "
+            "velocity component, projection rhs div(u)/dt, variable-coefficient projection from pi = 0, "
+            "projection correction, all on the device" % (Da, Ra_C))
+    cfg = {"workload": "mushy layer %s, %s, GRF porosity corr %g, St 5, CR 2, Le 200, dt 1e-3, velocity and "
+                       "pressure start at 0" % ("x".join(map(str, shape)), what, corr),
+           "form": head, "tolerance": "1e-10 rel or abs, max-norm",
+           "max_iter": args.max_iter, "hang": "disabled (a solve stops on eps/norm_thresh or max_iter only)",
+           "velocity_solves": {"ustar": "V(2,2), bottom 8, max 10, eps = hang = 1e-10, from the previous U*",
+                               "projection": "V(2,4), bottom 4, max 20, eps = hang = 1e-15, from pi = 0 each step"},
+           "advection": "none (no advective source on this path)",
            "iterations_per_step": round(float(np.mean(h["iters"])), 2), "exit_status": h["status"]}
     if hc_only:
         cfg["hc_solve_only"] = hc_only
@@ -830,7 +675,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(h["wall"] / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32/f64" if head == "fp32r" else "f64",
-        "data": "synthetic",
+        "data": "synthetic (seeded initial porosity field; no synthetic sources)",
         "config": cfg,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -354,6 +354,59 @@ int ml_hc_porosity(ml_hc_op op, const double* phi, double* chi);
  * geometric faces src/CellToEdge2F.ChF:20-46) */
 int ml_hc_assemble_bcoef(ml_hc_op op, const double* chi, double* const bCoef[]);
 
+/* updateEnthalpyVariables (src/phaseDiagram.cpp:92-177): from (H, Theta)
+ * the 4-component cell field vars = (theta, Theta_l, Theta_s, porosity) on
+ * the valid cells of level 0 (CALCULATET, CALCULATECL, CALCULATECS,
+ * CALCULATEPOROSITY, src/EnthalpyVariablesF.ChF:13-287) */
+int ml_hc_enthalpy_vars(ml_hc_op op, const double* phi, double* vars);
+
+/* ---- cell-centred velocity of a Darcy-Brinkman step --------------------
+ * (computeCCvelocity, srcSubcycle/AMRLevelMushyLayerVel.cpp:673-917), on
+ * level 0 of a VC operator of the grid.  vars_old / vars_new are the
+ * enthalpy variables (ml_hc_enthalpy_vars) at t^n and t^{n+1}; fields at
+ * the half time are 0.5*x^n + 0.5*x^{n+1}.  pi is the pressure of the last
+ * projection (1 component; ghosts are not read: beyond a wall the pressure
+ * is extrapolated to second order, BCutil/PhysBCUtil.cpp:2524-2536, 2628-2633).
+ * Velocity components, sources and the projection rhs are separate
+ * 1-component cell arrays; only valid cells are written.  Velocity walls
+ * are no-slip (BCutil/PhysBCUtil.cpp:1577-1611); periodic sides wrap. */
+typedef struct {
+  double buoyancyTCoeff;  /* m_buoyancyTCoeff: Pr*Ra_T (diffusive time, advective velocity) */
+  double buoyancySCoeff;  /* m_buoyancySCoeff: Pr*Ra_C */
+  double porosityLimit;   /* velocity set to zero where the new porosity <= this
+                           * (ccVelPorosityLimit = solidPorosity = 0.05) */
+  int scalePressure;      /* scaleP_CC: pressure gradients multiplied by the porosity (default 1) */
+} ml_flow_params;
+/* computeUstarSrc (srcSubcycle/AMRLevelMushyLayerVel.cpp:1724-1853):
+ * src_d = buoyancy_d - porosity*grad(pi)_d, buoyancy along the last direction
+ * porosity*(buoyancyTCoeff*theta - buoyancySCoeff*Theta_l) (fillBuoyancy,
+ * srcSubcycle/AMRLevelMushyLayerFill.cpp:870-885; fillPressureSrcTerm :787-807) */
+int ml_vc_ustar_src(ml_vc_op op, const ml_flow_params* fp, const double* vars_old, const double* vars_new,
+                    const double* pi, double* const src[]);
+/* u = ustar + dt*porosity*grad(pi) (removeGradPressure,
+ * srcSubcycle/AMRLevelMushyLayerVel.cpp:654-671), u = 0 where the new
+ * porosity <= limit (setVelZero), then rhs = div(u)/dt, the cell-centred
+ * divergence through arithmetic face values (Projector::levelProject,
+ * srcSubcycle/Projector.cpp:1379-1397; util/Divergence.cpp:349-390) */
+int ml_vc_cc_project_rhs(ml_vc_op op, const ml_flow_params* fp, double dt, const double* vars_old,
+                         const double* vars_new, const double* pi, const double* const ustar[], double* const u[],
+                         double* rhs);
+/* the projection's face coefficient bCoef[d] = -(porosity at the half time,
+ * arithmetic face mean; srcSubcycle/AMRLevelMushyLayerVel.cpp:717-731),
+ * wall faces the adjacent cell's value */
+int ml_vc_pressure_faces(ml_vc_op op, const double* vars_old, const double* vars_new, double* const bCoef[]);
+/* applyCCcorrection (srcSubcycle/Projector.cpp:1632-1683):
+ * u -= (grad(pi)*porosity)*(1/(1/dt)), then u = 0 where the new porosity <= limit */
+int ml_vc_cc_correct(ml_vc_op op, const ml_flow_params* fp, double dt, const double* vars_old,
+                     const double* vars_new, const double* pi, double* const u[]);
+/* Permeability laws (PermeabilityFunctions, src/MushyLayerParams.h:108-130;
+ * MushyLayerParams::calculatePermeability, src/MushyLayerParams.cpp:1031-1090) */
+enum { ML_PERM_PURE_FLUID = 0, ML_PERM_CUBIC = 1, ML_PERM_KOZENY_CARMAN = 2, ML_PERM_LOG = 3, ML_PERM_POROSITY = 4 };
+/* ml_vc_darcy_ccoef with any permeability law; hele_shaw > 0 combines the law
+ * harmonically with the Hele-Shaw permeability 1/(1/hele_shaw + 1/Pi) */
+int ml_vc_darcy_ccoef_law(ml_vc_op op, const double* chi, double darcy_coeff, int law, double hele_shaw,
+                          double* cCoef);
+
 /* ---- norms and LevelData arithmetic ------------------------------------ */
 /* max |x| over valid cells and ncomp components (AMRPoissonOp::norm, type 0) */
 int ml_norm_max(ml_ctx ctx, int dim, const int n[], int ncomp, const double* x, double* out);

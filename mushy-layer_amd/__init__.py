@@ -49,6 +49,25 @@ KT_HC_RELAX = 2   # ... of the (H, Theta) operator with the closure
 KT_BATCH_RELAX = 3   # batched form: FP32 relaxation of every instance's level
 
 
+class FlowParams(C.Structure):
+    _fields_ = [("buoyancyTCoeff", C.c_double), ("buoyancySCoeff", C.c_double), ("porosityLimit", C.c_double),
+                ("scalePressure", C.c_int)]
+
+
+PERM_PURE_FLUID, PERM_CUBIC, PERM_KOZENY_CARMAN, PERM_LOG, PERM_POROSITY = 0, 1, 2, 3, 4
+PERM_LAWS = {"pure_fluid": PERM_PURE_FLUID, "cubic": PERM_CUBIC, "kozeny_carman": PERM_KOZENY_CARMAN,
+             "log": PERM_LOG, "porosity": PERM_POROSITY}
+
+
+def make_flow_params(buoyancyTCoeff=0.0, buoyancySCoeff=0.0, porosityLimit=0.05, scalePressure=1):
+    f = FlowParams()
+    f.buoyancyTCoeff = buoyancyTCoeff
+    f.buoyancySCoeff = buoyancySCoeff
+    f.porosityLimit = porosityLimit
+    f.scalePressure = scalePressure
+    return f
+
+
 class SolveStats(C.Structure):
     _fields_ = [("exit_status", C.c_int), ("iterations", C.c_int),
                 ("initial_residual", C.c_double), ("final_residual", C.c_double)]
@@ -128,6 +147,13 @@ SIGNATURES = {
     "ml_hc_porosity_vjp": (C.c_int, [_vp, _vp, _vp, _vp, _dp]),
     "ml_hc_porosity": (C.c_int, [_vp, _vp, _vp]),
     "ml_hc_assemble_bcoef": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+    "ml_hc_enthalpy_vars": (C.c_int, [_vp, _vp, _vp]),
+    "ml_vc_ustar_src": (C.c_int, [_vp, C.POINTER(FlowParams), _vp, _vp, _vp, C.POINTER(_vp)]),
+    "ml_vc_cc_project_rhs": (C.c_int, [_vp, C.POINTER(FlowParams), C.c_double, _vp, _vp, _vp, C.POINTER(_vp),
+                                       C.POINTER(_vp), _vp]),
+    "ml_vc_pressure_faces": (C.c_int, [_vp, _vp, _vp, C.POINTER(_vp)]),
+    "ml_vc_cc_correct": (C.c_int, [_vp, C.POINTER(FlowParams), C.c_double, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "ml_vc_darcy_ccoef_law": (C.c_int, [_vp, _vp, C.c_double, C.c_int, C.c_double, _vp]),
     "ml_norm_max": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _vp, _dp]),
     "ml_vec_axby": (C.c_int, [_vp, C.c_size_t, _vp, _vp, _vp, C.c_double, C.c_double]),
     "ml_vec_incr": (C.c_int, [_vp, C.c_size_t, _vp, _vp, C.c_double]),
@@ -287,6 +313,10 @@ class DeviceArray:
         out = np.empty(self.shape, self.dtype)
         return self.ctx.download(self.ptr, out)
 
+    def comp_ptr(self, c):
+        """Device address of component c of a cell-layout array."""
+        return self.ptr + c * (self.nbytes // self.shape[0])
+
     def free(self):
         if self.ptr:
             self.ctx.free(self.ptr)
@@ -376,14 +406,21 @@ class HCOp:
         self.ctx.check(self.ctx.lib.ml_hc_prolong_increment(self.h, depth, phif.ptr, corrc.ptr),
                        "ml_hc_prolong_increment")
 
+    def enthalpy_vars(self, phi, out):
+        """out (4 components) = (theta, Theta_l, Theta_s, porosity) of phi."""
+        self.ctx.check(self.ctx.lib.ml_hc_enthalpy_vars(self.h, phi.ptr, out.ptr), "ml_hc_enthalpy_vars")
+
     def porosity(self, phi, chi):
         self.ctx.check(self.ctx.lib.ml_hc_porosity(self.h, phi.ptr, chi.ptr), "ml_hc_porosity")
 
     def assemble_bcoef(self, chi, b):
+        """b (face arrays) from the cell porosity chi (a DeviceArray or a
+        device address, e.g. component 3 of the enthalpy variables)."""
         arr = (_vp * 3)()
         for d in range(self.dim):
             arr[d] = b[d].ptr
-        self.ctx.check(self.ctx.lib.ml_hc_assemble_bcoef(self.h, chi.ptr, arr), "ml_hc_assemble_bcoef")
+        p = chi.ptr if hasattr(chi, "ptr") else chi
+        self.ctx.check(self.ctx.lib.ml_hc_assemble_bcoef(self.h, p, arr), "ml_hc_assemble_bcoef")
 
     def solve(self, phi, rhs, opts=None):
         st = SolveStats()
@@ -527,6 +564,38 @@ class VCOp:
 
     def darcy_ccoef(self, chi, coeff, c):
         self.ctx.check(self.ctx.lib.ml_vc_darcy_ccoef(self.h, chi.ptr, coeff, c.ptr), "ml_vc_darcy_ccoef")
+
+    def darcy_ccoef_law(self, chi_ptr, coeff, law, c, hele_shaw=0.0):
+        """c = (chi/Pi(chi))*coeff for a permeability law (PERM_* or its name);
+        chi_ptr: a DeviceArray or a device address."""
+        law = PERM_LAWS.get(law, law)
+        p = chi_ptr.ptr if hasattr(chi_ptr, "ptr") else chi_ptr
+        self.ctx.check(self.ctx.lib.ml_vc_darcy_ccoef_law(self.h, p, coeff, law, hele_shaw, c.ptr),
+                       "ml_vc_darcy_ccoef_law")
+
+    @staticmethod
+    def _arr(xs):
+        arr = (_vp * 3)()
+        for d, x in enumerate(xs):
+            arr[d] = x.ptr
+        return arr
+
+    def ustar_src(self, fp, vold, vnew, pi, src):
+        self.ctx.check(self.ctx.lib.ml_vc_ustar_src(self.h, C.byref(fp), vold.ptr, vnew.ptr, pi.ptr, self._arr(src)),
+                       "ml_vc_ustar_src")
+
+    def cc_project_rhs(self, fp, dt, vold, vnew, pi, ustar, u, rhs):
+        self.ctx.check(self.ctx.lib.ml_vc_cc_project_rhs(self.h, C.byref(fp), dt, vold.ptr, vnew.ptr, pi.ptr,
+                                                         self._arr(ustar), self._arr(u), rhs.ptr),
+                       "ml_vc_cc_project_rhs")
+
+    def pressure_faces(self, vold, vnew, b):
+        self.ctx.check(self.ctx.lib.ml_vc_pressure_faces(self.h, vold.ptr, vnew.ptr, self._arr(b)),
+                       "ml_vc_pressure_faces")
+
+    def cc_correct(self, fp, dt, vold, vnew, pi, u):
+        self.ctx.check(self.ctx.lib.ml_vc_cc_correct(self.h, C.byref(fp), dt, vold.ptr, vnew.ptr, pi.ptr,
+                                                     self._arr(u)), "ml_vc_cc_correct")
 
     def porosity_faces(self, chi, scale, b):
         arr = (_vp * 3)()

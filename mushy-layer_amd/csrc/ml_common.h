@@ -183,6 +183,20 @@ __host__ __device__ inline T porosity(T H, T C, const ClosureP<T>& p) {
   return T(NAN);
 }
 
+// x^3 rounded once: x*x = p + e and p*x = h + l exactly (FMA remainders), so
+// x^3 = h + (l + e*x) to within a tiny fraction of an ulp before the final
+// rounding.  Host and device (and the oracle) evaluate the same expression,
+// so they agree bit for bit; it is the value of a correctly rounded
+// pow(x, 3.0) (MushyLayerParams::calculatePermeability,
+// src/MushyLayerParams.cpp:1031-1090).
+__host__ __device__ inline double cube_rn(double x) {
+  const double p = x * x;
+  const double e = fma(x, x, -p);
+  const double h = p * x;
+  const double l = fma(p, x, -h);
+  return h + (l + e * x);
+}
+
 // Boundary ghost value from the adjacent interior value x1:
 // Dirichlet g = 2v - x1 (BCutil/BCFuncF.ChF:40-46),
 // Neumann  g = x1 + isign*dx*v (BCutil/BCFuncF.ChF:80-88).

@@ -531,6 +531,18 @@ hipError_t be_rhs(const Geom& g, int ncomp, T* y, const T* x, const T* acoef, T 
 hipError_t darcy_ccoef(const Geom& g, double coeff, const double* chi, double* c, hipStream_t s);
 hipError_t porosity_faces(const Geom& g, double scale, const double* chi, double* const* b, hipStream_t s);
 
+// ---- state update and velocity closure (ml_flow_kernels.hip) ----
+hipError_t enthalpy_vars(const Geom& g, const ClosureP<double>& cp, const double* phi, double* vars, hipStream_t s);
+hipError_t ustar_src(const Geom& g, double buoyT, double buoyS, int scaleP, const double* vold, const double* vnew,
+                     const double* pi, double* const* src, hipStream_t s);
+hipError_t cc_project_rhs(const Geom& g, double dt, double limit, int scaleP, const double* vold, const double* vnew,
+                          const double* pi, const double* const* ustar, double* const* u, double* rhs, hipStream_t s);
+hipError_t pressure_faces(const Geom& g, const double* vold, const double* vnew, double* const* b, hipStream_t s);
+hipError_t cc_correct(const Geom& g, double dt, double limit, int scaleP, const double* vold, const double* vnew,
+                      const double* pi, double* const* u, hipStream_t s);
+hipError_t darcy_ccoef_law(const Geom& g, double coeff, int law, double hele_shaw, const double* chi, double* c,
+                           hipStream_t s);
+
 }  // namespace ml
 
 namespace ml {

@@ -305,7 +305,8 @@ __global__ void hc_bcoef_kernel(Geom g, int d, double kcond, double DH, double D
 // Darcy coefficient of the U* solve: c = (chi / Pi(chi)) * coeff with the
 // Kozeny-Carman permeability Pi = chi^3/(1-chi)^2
 // (srcSubcycle/AMRLevelMushyLayerInit.cpp:665-689: copy porosity, divide by
-// permeability, multiply by m_darcyCoeff; src/MushyLayerParams.cpp:1046-1049).
+// permeability, multiply by m_darcyCoeff; src/MushyLayerParams.cpp:1046-1049),
+// pow(x, 3.0) and pow(x, 2.0) as the once-rounded cube and square.
 __global__ void darcy_ccoef_kernel(Geom g, double coeff, const double* __restrict__ chi, double* __restrict__ c) {
   const long long N = g.nvalid();
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N;
@@ -314,7 +315,7 @@ __global__ void darcy_ccoef_kernel(Geom g, double coeff, const double* __restric
     unflatten(t, g.n[0], g.n[1], i, j, k);
     const long long q = g.ci(i, j, k);
     const double liquid = chi[q], solid = 1.0 - liquid;
-    const double perm = pow(liquid, 3.0) / pow(solid, 2.0);
+    const double perm = cube_rn(liquid) / (solid * solid);
     double v = liquid;
     v = v / perm;
     c[q] = v * coeff;

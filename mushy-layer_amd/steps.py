@@ -1,0 +1,167 @@
+"""Whole implicit steps on the GPU, composed from the C-ABI entry points.
+
+HCStepper: one backward-Euler step of (H, Theta) — enthalpy variables at
+t^n -> HC face coefficients from the porosity at t^n -> FAS solve
+(multiCompAdvectDiffuse, srcSubcycle/AMRLevelMushyLayer.cpp:1645-1831).
+
+DarcyBrinkmanStepper: one Darcy-Brinkman mushy-layer step in the order of
+AMRLevelMushyLayer::advance (srcSubcycle/AMRLevelMushyLayer.cpp:451-787):
+the (H, Theta) step, the enthalpy variables at t^{n+1}
+(updateEnthalpyVariables), then the cell-centred velocity
+(computeCCvelocity, srcSubcycle/AMRLevelMushyLayerVel.cpp:673-917):
+  U* source = buoyancy - porosity * grad P^{n-1/2} at the half time
+    (computeUstarSrc, :1724-1853);
+  U* coefficients a = 1, b = -Pr, c = (Pr/Da) chi^n / Pi(chi^n)
+    (defineUstarMultigrid, srcSubcycle/AMRLevelMushyLayerInit.cpp:579-747);
+  one backward-Euler solve per velocity component from the previous U*
+    (:1326-1403), run concurrently;
+  u = U* + dt chi grad P^{n-1/2}, zero where chi^{n+1} <= 0.05, projection
+    rhs div(u)/dt, faces -(porosity at the half time), solve from pi = 0
+    (Projector::levelProject, srcSubcycle/Projector.cpp:1340-1486,
+    solveMGlevel :3164-3173), u -= dt chi grad pi, zero where solid.
+Everything stays on the device; no advective source enters (the solve path
+has no advection).
+"""
+import numpy as np
+
+from . import (DeviceArray, HCOp, VCOp, default_opts, face_shape, make_bc, make_flow_params, make_params, valid)
+
+# multigrid options of the velocity solves: U* (VelocityMultigrid: smoothing
+# 2, 10 iterations, tolerance 1e-10) and the projection
+# (srcSubcycle/Projector.cpp:45-64: down 2, up 4, tolerance and hang 1e-15,
+# 20 iterations); relaxation bottoms
+U_OPTS = dict(nu1=2, nu2=2, bottom=8, max_iter=10, eps=1e-10, hang=1e-10, norm_thresh=1e-10)
+P_OPTS = dict(nu1=2, nu2=4, bottom=4, max_iter=20, eps=1e-15, hang=1e-15, norm_thresh=1e-15)
+POROSITY_LIMIT = 0.05   # solidPorosity (srcSubcycle/MushyLayerSubcycleUtils.cpp:223-233)
+
+
+def flow_coeffs(Pr, Da, Ra_C, Ra_T=0.0):
+    """Diffusive time scale, advective velocity scale
+    (src/MushyLayerParams.cpp:736-753)."""
+    return dict(viscosity=Pr, buoyT=Pr * Ra_T, buoyS=Pr * Ra_C, darcy=Pr / Da)
+
+
+class HCStepper:
+    """Device-resident (H, Theta) state and its implicit step."""
+
+    def __init__(self, ml, ctx, case, opts):
+        self.ml, self.ctx, self.case, self.opts = ml, ctx, case, opts
+        n = case["n"]
+        self.n = list(n)
+        self.dt = case["dt"]
+        self.op = HCOp(ctx, n, case["dx"], case["periodic"], make_params(**case["params"]),
+                       make_bc(case["bc"], apply=case.get("apply_bc", 1)))
+        shp = case["phi"].shape
+        self.init = DeviceArray(ctx, shp, host=case["phi"])
+        self.phi = DeviceArray(ctx, shp, host=case["phi"])
+        self.old = DeviceArray(ctx, shp)
+        self.vold = DeviceArray(ctx, (4,) + shp[1:])
+        self.vnew = DeviceArray(ctx, (4,) + shp[1:])
+        self.b = [DeviceArray(ctx, face_shape(n, d, 2)) for d in range(len(n))]
+        self.stats = []
+
+    def copy(self, dst, src):
+        self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, dst.ptr, src.ptr, src.nbytes), "ml_buf_copy")
+
+    def reset(self):
+        self.copy(self.phi, self.init)
+        self.stats = []
+
+    def set_flags(self, flags):
+        self.opts.flags = flags
+
+    def hc_step(self):
+        op = self.op
+        op.enthalpy_vars(self.phi, self.vold)
+        op.assemble_bcoef(self.vold.comp_ptr(3), self.b)
+        op.set_coefs(None, self.b, 1.0, -self.dt)
+        self.copy(self.old, self.phi)
+        st = op.solve_be(self.phi, self.old, self.dt, opts=self.opts)
+        self.stats.append((st.exit_status, st.iterations, st.final_residual))
+
+    def step(self):
+        self.hc_step()
+
+    def porosity(self):
+        """Host copy of the porosity of the current state (valid cells)."""
+        self.op.enthalpy_vars(self.phi, self.vnew)
+        return valid(self.n, self.vnew.get())[3]
+
+    def close(self):
+        self.op.close()
+
+
+class DarcyBrinkmanStepper(HCStepper):
+    """The Darcy-Brinkman step (module docstring).  `flow`: Pr, Da, Ra_C
+    (Ra_T), the velocity and pressure BC specs, the permeability law."""
+
+    def __init__(self, ml, ctx, case, opts, flow):
+        super().__init__(ml, ctx, case, opts)
+        n = self.n
+        dim = len(n)
+        self.flow = flow
+        self.coef = flow_coeffs(flow["Pr"], flow["Da"], flow["Ra_C"], flow.get("Ra_T", 0.0))
+        self.law = flow.get("permeability", "kozeny_carman")
+        self.fp = make_flow_params(self.coef["buoyT"], self.coef["buoyS"], POROSITY_LIMIT, 1)
+        self.uop = VCOp(ctx, n, case["dx"], case["periodic"], make_bc(flow["bc_u"]))
+        self.pop = VCOp(ctx, n, case["dx"], case["periodic"], make_bc(flow["bc_p"]))
+        shp1 = (1,) + case["phi"].shape[1:]
+        self.u = [DeviceArray(ctx, shp1) for _ in range(dim)]
+        self.ustar = [DeviceArray(ctx, shp1) for _ in range(dim)]
+        self.uold = [DeviceArray(ctx, shp1) for _ in range(dim)]
+        self.src = [DeviceArray(ctx, shp1) for _ in range(dim)]
+        self.c = DeviceArray(ctx, shp1)
+        self.bu = [DeviceArray(ctx, face_shape(n, d, 1), host=np.full(face_shape(n, d, 1), -self.coef["viscosity"]))
+                   for d in range(dim)]
+        self.bp = [DeviceArray(ctx, face_shape(n, d, 1)) for d in range(dim)]
+        self.prhs = DeviceArray(ctx, shp1)
+        self.pi = DeviceArray(ctx, shp1)
+        self.uopts = default_opts(**U_OPTS)
+        self.popts = default_opts(**P_OPTS)
+        self.vstats = []
+
+    def _zero(self, x):
+        self.ctx.check(self.ctx.lib.ml_vec_set(self.ctx.h, x.nbytes // 8, x.ptr, 0.0), "ml_vec_set")
+
+    def reset(self):
+        super().reset()
+        for x in self.u + self.ustar + [self.pi]:
+            self._zero(x)
+        self.vstats = []
+
+    def set_flags(self, flags):
+        super().set_flags(flags)
+        self.uopts.flags = flags
+        self.popts.flags = flags
+
+    def step(self):
+        self.hc_step()
+        dt = self.dt
+        self.op.enthalpy_vars(self.phi, self.vnew)
+        uop, pop = self.uop, self.pop
+        uop.darcy_ccoef_law(self.vold.comp_ptr(3), self.coef["darcy"], self.law, self.c)
+        uop.set_coefs(None, self.bu, self.c, 1.0, -dt)
+        uop.ustar_src(self.fp, self.vold, self.vnew, self.pi, self.src)
+        for u, uo in zip(self.u, self.uold):
+            self.copy(uo, u)
+        sts = uop.solve_be_many(self.ustar, self.uold, dt, src=self.src, opts=self.uopts)
+        vs = [(st.exit_status, st.iterations) for st in sts]
+        uop.cc_project_rhs(self.fp, dt, self.vold, self.vnew, self.pi, self.ustar, self.u, self.prhs)
+        pop.pressure_faces(self.vold, self.vnew, self.bp)
+        pop.set_coefs(None, self.bp, None, 0.0, 1.0)
+        self._zero(self.pi)
+        st = pop.solve(self.pi, self.prhs, opts=self.popts)
+        vs.append((st.exit_status, st.iterations))
+        uop.cc_correct(self.fp, dt, self.vold, self.vnew, self.pi, self.u)
+        self.vstats.append(vs)
+
+    def flow_fields(self):
+        """Host copies of the velocity components and the pressure (valid cells)."""
+        out = {"u%d" % d: valid(self.n, u.get())[0] for d, u in enumerate(self.u)}
+        out["p"] = valid(self.n, self.pi.get())[0]
+        return out
+
+    def close(self):
+        super().close()
+        self.uop.close()
+        self.pop.close()

@@ -242,3 +242,16 @@ def flux_profile(nx, F0=10.0, modes=((1, 0.3, 0.0),)):
     for k, a, ph in modes:
         f = f + a * np.sin(2 * np.pi * k * x + ph)
     return F0 * f
+
+
+def flow(dim=2, Pr=10.0, Da=1e-3, Ra_C=200.0, Ra_T=0.0, permeability="kozeny_carman"):
+    """Flow parameters of the Darcy-Brinkman step on a `mushy` case: Prandtl
+    number (the reference's Darcy-Brinkman example uses Pr = 10,
+    examples/mushylayer_darcybrinkman/inputs), Darcy number, solutal and
+    thermal Rayleigh numbers, the permeability law; no-slip walls in the last
+    direction for the velocity, homogeneous Neumann for the pressure, periodic
+    sideways (as the case).  The velocity and pressure start at zero."""
+    vd = dim - 1
+    return dict(Pr=Pr, Da=Da, Ra_C=Ra_C, Ra_T=Ra_T, permeability=permeability,
+                bc_u={(vd, 0, 0): (DIRICHLET, 0.0), (vd, 1, 0): (DIRICHLET, 0.0)},
+                bc_p={(vd, 0, 0): (NEUMANN, 0.0), (vd, 1, 0): (NEUMANN, 0.0)})

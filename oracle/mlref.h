@@ -184,6 +184,20 @@ int mlr_lin_mg_solve(const mlr_grid* g, int ncomp, const mlr_bc* bc,
  * count in use.  Results do not depend on it. */
 int mlr_set_threads(int n);
 
+/* ---- state update and velocity closure of a Darcy-Brinkman step
+ * (mlref_flow.c) ---------------------------------------------------------- */
+void mlr_enthalpy_vars_point(double H, double C, const mlr_params* p, double* out);
+void mlr_enthalpy_vars(const mlr_grid* g, const mlr_params* p, const double* phi, double* vars);
+void mlr_ustar_src(const mlr_grid* g, double buoyT, double buoyS, int scaleP, const double* vold,
+                   const double* vnew, double* pi, double* const* src);
+void mlr_cc_project_rhs(const mlr_grid* g, double dt, double limit, int scaleP, const double* vold,
+                        const double* vnew, double* pi, const double* const* ustar, double* const* u,
+                        double* rhs);
+void mlr_pressure_faces(const mlr_grid* g, const double* vold, const double* vnew, double* const* b);
+void mlr_cc_correct(const mlr_grid* g, double dt, double limit, int scaleP, const double* vold,
+                    const double* vnew, double* pi, double* const* u);
+void mlr_darcy_ccoef(const mlr_grid* g, const double* chi, double coeff, int law, double hele_shaw, double* c);
+
 #ifdef __cplusplus
 }
 #endif

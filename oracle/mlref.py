@@ -106,6 +106,13 @@ def _declare(L):
         "mlr_lin_apply_op": (None, [G, C.c_int, B, C.c_double, _dp, C.c_double, _dp, pp, _dp, _dp, C.c_int]),
         "mlr_lin_restrict_residual": (None, [G, C.c_int, B, C.c_double, _dp, C.c_double, _dp, pp, _dp, _dp,
                                              _dp]),
+        "mlr_enthalpy_vars_point": (None, [C.c_double, C.c_double, P, _dp]),
+        "mlr_enthalpy_vars": (None, [G, P, _dp, _dp]),
+        "mlr_ustar_src": (None, [G, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, pp]),
+        "mlr_cc_project_rhs": (None, [G, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, pp, pp, _dp]),
+        "mlr_pressure_faces": (None, [G, _dp, _dp, pp]),
+        "mlr_cc_correct": (None, [G, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, pp]),
+        "mlr_darcy_ccoef": (None, [G, _dp, C.c_double, C.c_int, C.c_double, _dp]),
         "mlr_lin_mg_solve": (C.c_int, [G, C.c_int, B, C.c_double, _dp, C.c_double, _dp, pp, _dp, _dp, C.c_int,
                                        C.POINTER(Opts), C.POINTER(Stats)]),
     }
@@ -374,6 +381,59 @@ def lin_mg_solve(g, ncomp, bc, alpha, a, beta, c, b, phi, rhs, prolong_linear=0,
     lib().mlr_lin_mg_solve(C.byref(g), ncomp, C.byref(bc), alpha, ptr(a), beta, ptr(c), ptrs(b), ptr(phi),
                            ptr(rhs), int(prolong_linear), C.byref(o), C.byref(st))
     return st
+
+
+def enthalpy_vars_point(H, C_, p):
+    out = (C.c_double * 4)()
+    lib().mlr_enthalpy_vars_point(H, C_, C.byref(p), out)
+    return tuple(out)
+
+
+def enthalpy_vars(g, p, phi):
+    """(theta, Theta_l, Theta_s, porosity) of every valid cell (4 components)."""
+    out = np.zeros(cell_shape(g, 4))
+    lib().mlr_enthalpy_vars(C.byref(g), C.byref(p), ptr(phi), ptr(out))
+    return out
+
+
+def ustar_src(g, buoyT, buoyS, scaleP, vold, vnew, pi):
+    """U* source per velocity component (list of 1-component arrays); fills
+    the ghosts of pi."""
+    src = [np.zeros(cell_shape(g, 1)) for _ in range(g.dim)]
+    lib().mlr_ustar_src(C.byref(g), buoyT, buoyS, int(scaleP), ptr(vold), ptr(vnew), ptr(pi), ptrs(src))
+    return src
+
+
+def cc_project_rhs(g, dt, limit, scaleP, vold, vnew, pi, ustar):
+    """(velocity before the projection, projection right-hand side)."""
+    u = [np.zeros(cell_shape(g, 1)) for _ in range(g.dim)]
+    rhs = np.zeros(cell_shape(g, 1))
+    lib().mlr_cc_project_rhs(C.byref(g), dt, limit, int(scaleP), ptr(vold), ptr(vnew), ptr(pi), ptrs(ustar),
+                             ptrs(u), ptr(rhs))
+    return u, rhs
+
+
+def pressure_faces(g, vold, vnew):
+    b = [np.zeros(face_shape(g, d, 1)) for d in range(g.dim)]
+    lib().mlr_pressure_faces(C.byref(g), ptr(vold), ptr(vnew), ptrs(b))
+    return b
+
+
+def cc_correct(g, dt, limit, scaleP, vold, vnew, pi, u):
+    """Projection correction of the velocity components u (in place)."""
+    lib().mlr_cc_correct(C.byref(g), dt, limit, int(scaleP), ptr(vold), ptr(vnew), ptr(pi), ptrs(u))
+    return u
+
+
+PERM_LAWS = {"pure_fluid": 0, "cubic": 1, "kozeny_carman": 2, "log": 3, "porosity": 4}
+
+
+def darcy_ccoef(g, chi, coeff, law="kozeny_carman", hele_shaw=0.0):
+    """U* Darcy coefficient (chi/Pi(chi))*coeff of a 1-component cell field."""
+    c = np.zeros(cell_shape(g, 1))
+    lib().mlr_darcy_ccoef(C.byref(g), ptr(np.ascontiguousarray(chi)), coeff, PERM_LAWS.get(law, law), hele_shaw,
+                          ptr(c))
+    return c
 
 
 def set_threads(n):
