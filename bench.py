@@ -300,6 +300,7 @@ def timed_steps(ctx, st, steps):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vc_relax_tile_256_traffic.json")   # 256x256 workload
+BATCH_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "batch_relax_512_traffic.json")   # 512-instance batch
 # the tiled relaxation of the finest level of the two workloads
 VC_TILE = {2: "vc_relax_tile_kernel<double,2,16,16,1,9,1024>", 3: "vc_relax_tile_kernel<double,3,16,8,4,2,1024>"}
 
@@ -440,10 +441,17 @@ def run_batch(ml, wl, ctx, args):
     batch.step(phis, cases[0]["dt"], ot)
     n1, ms, nb, missed = ctx.timing_stop()
     achieved = nb / (ms * 1e-3) / 1e9
-    roof = {"kernel": "lin_relax_batch_kernel<2,%d,%d,1,8,%d> (every launch of one batched step)" % tshape,
+    traffic = None
+    kname = "lin_relax_batch_kernel<2,%d,%d,1,8,%d>" % tshape
+    if args.batch == 512 and os.path.exists(BATCH_TRAFFIC_FILE):
+        with open(BATCH_TRAFFIC_FILE) as f:
+            tf = json.load(f)
+        if tf.get("kernel") == kname:
+            traffic = tf.get("bytes_per_launch")
+    roof = {"kernel": "%s (every launch of one batched step)" % kname,
             "bound": "hbm",
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "avg_launch_us": round(ms / n1 * 1e3, 3), "bytes_per_launch": round(nb / n1, 1),
             "launches_timed": n1, "launches_untimed": missed}
     calibration_copy(ml, ctx)

@@ -205,6 +205,19 @@ int ml_hc_restrict_r(ml_hc_op op, int depth, double* phiCoarse, const double* ph
  * phiFine += P corrCoarse (linear, ratio 2) */
 int ml_hc_prolong_increment(ml_hc_op op, int depth, double* phiFine, const double* corrCoarse);
 
+/* levelJacobi (src/AMRNonLinearMultiCompOp.cpp:1077-1101), `iters` sweeps
+ * (relax mode 4): res = rhs - L(phi); phi += 0.5 * lambda * res */
+int ml_hc_jacobi(ml_hc_op op, int depth, double* phi, const double* rhs, int iters);
+/* preCond (src/AMRNonLinearMultiCompOp.cpp:172-206): phi = lambda * rhs, then
+ * two levelGSRB iterations */
+int ml_hc_precond(ml_hc_op op, int depth, double* phi, const double* rhs);
+/* getFlux (src/AMRNonLinearMultiCompOp.cpp:1103-1162) on every face of level
+ * `depth`: flux[d] (2 components, face layout) = -bCoef * beta *
+ * (phi_l(i) - phi_l(i - e_d)) / dx, phi_l the diffused variable with the
+ * derived-variable BC.  (reflux, :601-693, needs a finer AMR level; a
+ * single-level solve has none.) */
+int ml_hc_get_flux(ml_hc_op op, int depth, const double* phi, double* const flux[]);
+
 /* AMRFASMultiGrid::solve with this operator (Chombo driver; build-owned
  * specification).  phi is the initial guess and the result. */
 int ml_hc_solve(ml_hc_op op, double* phi, const double* rhs, const ml_solve_opts* opts,
@@ -406,6 +419,31 @@ enum { ML_PERM_PURE_FLUID = 0, ML_PERM_CUBIC = 1, ML_PERM_KOZENY_CARMAN = 2, ML_
  * harmonically with the Hele-Shaw permeability 1/(1/hele_shaw + 1/Pi) */
 int ml_vc_darcy_ccoef_law(ml_vc_op op, const double* chi, double darcy_coeff, int law, double hele_shaw,
                           double* cCoef);
+
+/* ---- multi-box levels ----------------------------------------------------
+ * A level of the reference is a DisjointBoxLayout (the default
+ * max_grid_size 128, srcSubcycle/MushyLayerSubcycleUtils.cpp:1011-1012,
+ * cuts a 256^2 domain into four boxes), each box with its own FAB of one
+ * ghost layer (LevelData<FArrayBox>, component outermost).  The solves run
+ * on the whole domain as one array (the layout described at the top); these
+ * copy between the boxes and that array on the device, and synchronise.
+ * box_lo / box_hi: nbox*3 ints, the inclusive valid-cell bounds of each box
+ * inside the domain [0, n-1] (the boxes must be disjoint and cover it);
+ * fabs[b]: box b's FAB.
+ *   gather:  the domain array's valid cells from the boxes' valid cells;
+ *   scatter: the boxes' valid cells from the domain array, and their ghost
+ *            cells that lie inside the domain or across a periodic side
+ *            (what LevelData::exchange fills); ghosts on a physical wall are
+ *            left to the caller's boundary conditions;
+ *   gather_faces: the domain's face array of direction dir (FluxBox
+ *            component dir, surroundingNodes(box, dir) per box); a face
+ *            shared by two boxes takes the later box's value. */
+int ml_level_gather(ml_ctx ctx, int dim, const int n[], const int periodic[], int ncomp, int nbox,
+                    const int box_lo[], const int box_hi[], const double* const fabs[], double* level);
+int ml_level_scatter(ml_ctx ctx, int dim, const int n[], const int periodic[], int ncomp, int nbox,
+                     const int box_lo[], const int box_hi[], const double* level, double* const fabs[]);
+int ml_level_gather_faces(ml_ctx ctx, int dim, const int n[], int ncomp, int nbox, const int box_lo[],
+                          const int box_hi[], int dir, const double* const fabs[], double* level_faces);
 
 /* ---- norms and LevelData arithmetic ------------------------------------ */
 /* max |x| over valid cells and ncomp components (AMRPoissonOp::norm, type 0) */

@@ -109,6 +109,9 @@ SIGNATURES = {
     "ml_hc_compute_diffused_var": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
     "ml_hc_relax": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
     "ml_hc_relax_noalloc": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
+    "ml_hc_jacobi": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
+    "ml_hc_precond": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "ml_hc_get_flux": (C.c_int, [_vp, C.c_int, _vp, C.POINTER(_vp)]),
     "ml_hc_residual": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int]),
     "ml_hc_apply_op": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
     "ml_hc_restrict_residual": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
@@ -154,6 +157,10 @@ SIGNATURES = {
     "ml_vc_pressure_faces": (C.c_int, [_vp, _vp, _vp, C.POINTER(_vp)]),
     "ml_vc_cc_correct": (C.c_int, [_vp, C.POINTER(FlowParams), C.c_double, _vp, _vp, _vp, C.POINTER(_vp)]),
     "ml_vc_darcy_ccoef_law": (C.c_int, [_vp, _vp, C.c_double, C.c_int, C.c_double, _vp]),
+    "ml_level_gather": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_int, C.c_int, _ip, _ip, C.POINTER(_vp), _vp]),
+    "ml_level_scatter": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_int, C.c_int, _ip, _ip, _vp, C.POINTER(_vp)]),
+    "ml_level_gather_faces": (C.c_int, [_vp, C.c_int, _ip, C.c_int, C.c_int, _ip, _ip, C.c_int, C.POINTER(_vp),
+                                        _vp]),
     "ml_norm_max": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _vp, _dp]),
     "ml_vec_axby": (C.c_int, [_vp, C.c_size_t, _vp, _vp, _vp, C.c_double, C.c_double]),
     "ml_vec_incr": (C.c_int, [_vp, C.c_size_t, _vp, _vp, C.c_double]),
@@ -342,6 +349,39 @@ def valid(n, x):
     return x[:, :, 1:-1, 1:-1]
 
 
+def _boxes(boxes):
+    nb = len(boxes)
+    lo = (C.c_int * (3 * nb))()
+    hi = (C.c_int * (3 * nb))()
+    for b, (l, h) in enumerate(boxes):
+        for d in range(3):
+            lo[3 * b + d] = l[d] if d < len(l) else 0
+            hi[3 * b + d] = h[d] if d < len(h) else 0
+    return nb, lo, hi
+
+
+def level_gather(ctx, n, periodic, ncomp, boxes, fabs, level):
+    """The domain array `level` from the boxes' FABs; boxes: [(lo, hi), ...]
+    inclusive valid-cell bounds (a DisjointBoxLayout)."""
+    nb, lo, hi = _boxes(boxes)
+    ctx.check(ctx.lib.ml_level_gather(ctx.h, len(n), _iarr(n), _iarr(periodic), ncomp, nb, lo, hi,
+                                      (_vp * nb)(*[f.ptr for f in fabs]), level.ptr), "ml_level_gather")
+
+
+def level_scatter(ctx, n, periodic, ncomp, boxes, level, fabs):
+    """The boxes' FABs (valid cells and exchange ghosts) from `level`."""
+    nb, lo, hi = _boxes(boxes)
+    ctx.check(ctx.lib.ml_level_scatter(ctx.h, len(n), _iarr(n), _iarr(periodic), ncomp, nb, lo, hi, level.ptr,
+                                       (_vp * nb)(*[f.ptr for f in fabs])), "ml_level_scatter")
+
+
+def level_gather_faces(ctx, n, ncomp, boxes, d, fabs, level_faces):
+    nb, lo, hi = _boxes(boxes)
+    ctx.check(ctx.lib.ml_level_gather_faces(ctx.h, len(n), _iarr(n), ncomp, nb, lo, hi, d,
+                                            (_vp * nb)(*[f.ptr for f in fabs]), level_faces.ptr),
+              "ml_level_gather_faces")
+
+
 class HCOp:
     """AMRNonLinearMultiCompOp (+ its factory's MG hierarchy) on the GPU."""
 
@@ -376,6 +416,18 @@ class HCOp:
 
     def relax(self, depth, phi, rhs, iters):
         self.ctx.check(self.ctx.lib.ml_hc_relax(self.h, depth, phi.ptr, rhs.ptr, iters), "ml_hc_relax")
+
+    def jacobi(self, depth, phi, rhs, iters):
+        self.ctx.check(self.ctx.lib.ml_hc_jacobi(self.h, depth, phi.ptr, rhs.ptr, iters), "ml_hc_jacobi")
+
+    def precond(self, depth, phi, rhs):
+        self.ctx.check(self.ctx.lib.ml_hc_precond(self.h, depth, phi.ptr, rhs.ptr), "ml_hc_precond")
+
+    def get_flux(self, depth, phi, flux):
+        arr = (_vp * 3)()
+        for d in range(self.dim):
+            arr[d] = flux[d].ptr
+        self.ctx.check(self.ctx.lib.ml_hc_get_flux(self.h, depth, phi.ptr, arr), "ml_hc_get_flux")
 
     def relax_noalloc(self, depth, phi, rhs, iters):
         """relax() with the partner array in the library's static scratch

@@ -87,6 +87,7 @@ struct HCLevel {
 };
 
 struct HostDriver;   // reference-API FAS driver state (ml_capi.cpp)
+struct VCHostDriver; // reference-API linear MG driver state (ml_vc.cpp)
 class HCEngine;      // fused FAS engine (ml_hc_engine.cpp)
 struct AdjWork;      // adjoint-step workspace (ml_adjoint.hip)
 
@@ -239,6 +240,9 @@ struct ml_vc_op_s {
   int nslots = 1024;                // norm slots per workspace (max_iter + 1)
   std::vector<ml::VCWork> work;     // solve workspaces; [0] runs on the context stream
   size_t coarse_lds = 0;            // dynamic LDS of the single-workgroup V-cycle
+  // the reference-API operator objects and linear MG driver behind
+  // ML_SOLVE_HOST_DRIVER (built at its first use)
+  std::unique_ptr<ml::VCHostDriver, void (*)(ml::VCHostDriver*)> host_driver{nullptr, nullptr};
 
   ml::VCArgs<double> args(int depth) const {
     ml::VCArgs<double> A;

@@ -17,11 +17,77 @@
 #include <cstring>
 #include <memory>
 
+#include "../host/HipVCOps.H"
+#include "../host/LinearMultiGrid.H"
 #include "ml_internal.h"
+
+namespace ml {
+
+// ML_SOLVE_HOST_DRIVER: one reference-API operator object per MG depth
+// (host/HipVCOps.H) and the linear MG driver (host/LinearMultiGrid.H), every
+// operator method one C-ABI call; bit-identical to the fused engine
+struct VCHostDriver {
+  std::vector<std::unique_ptr<mlhost::HipVCLevelOp>> owned;
+  std::vector<mlhost::HipVCLevelOp*> ops;
+  mlhost::LinearMultiGrid<mlhost::HipVCLevelOp> mg;
+};
+
+}  // namespace ml
 
 namespace {
 
 using ml::fail;
+
+void delete_vc_host_driver(ml::VCHostDriver* p) { delete p; }
+
+ml::VCHostDriver& vc_host_driver_of(ml_vc_op op) {
+  if (!op->host_driver) {
+    std::unique_ptr<ml::VCHostDriver> S(new ml::VCHostDriver());
+    const ml::Geom& g = op->lev[0].g;
+    for (int l = 0; l < int(op->lev.size()); ++l) {
+      S->owned.emplace_back(new mlhost::HipDarcyBrinkmanOp(op, op->ctx, l, g.dim, g.n));
+      S->ops.push_back(S->owned.back().get());
+    }
+    mlhost::LevelData proto(op->ctx, g.dim, g.n, 1);
+    S->mg.define(S->ops, proto);
+    op->host_driver = std::unique_ptr<ml::VCHostDriver, void (*)(ml::VCHostDriver*)>(S.release(), delete_vc_host_driver);
+  }
+  return *op->host_driver;
+}
+
+// the solves of ml_vc_solve_many through the host driver, one after another
+// on the context stream
+int vc_solve_host(ml_vc_op op, int n, double* const phi[], const double* const rhs[], const ml_solve_opts& o,
+                  ml_solve_stats* st) {
+  try {
+    ml::VCHostDriver& S = vc_host_driver_of(op);
+    mlhost::FASParams& p = S.mg.params();
+    p.nu1 = o.nu1;
+    p.nu2 = o.nu2;
+    p.bottom = o.bottom;
+    p.max_iter = o.max_iter;
+    p.eps = o.eps;
+    p.hang = o.hang;
+    p.norm_thresh = o.norm_thresh;
+    const ml::Geom& g = op->lev[0].g;
+    for (int i = 0; i < n; ++i) {
+      mlhost::LevelData vphi, vrhs;
+      vphi.alias(op->ctx, g.dim, g.n, 1, phi[i]);
+      vrhs.alias(op->ctx, g.dim, g.n, 1, const_cast<double*>(rhs[i]));
+      mlhost::SolveStats s;
+      S.mg.solve(vphi, vrhs, &s);
+      if (st) {
+        st[i].exit_status = s.exit_status;
+        st[i].iterations = s.iterations;
+        st[i].initial_residual = s.initial_residual;
+        st[i].final_residual = s.final_residual;
+      }
+    }
+  } catch (const std::exception& e) {
+    return fail(op->ctx, ML_EHIP, e.what());
+  }
+  return ML_OK;
+}
 
 int vc_alloc(ml_ctx ctx, size_t n, double** p) {
   void* v = nullptr;
@@ -287,7 +353,6 @@ int vc_solve_check(ml_vc_op op, const ml_solve_opts& o) {
   if (o.max_iter + 1 > op->nslots) return fail(ctx, ML_EINVAL, "max_iter too large");
   if (o.nu1 < 0 || o.nu2 < 0 || o.bottom < 0) return fail(ctx, ML_EINVAL, "negative smoothing count");
   if (o.precision != ML_PRECISION_FP64) return fail(ctx, ML_ENOTSUP, "the linear operator solves in FP64 only");
-  if (o.flags & ML_SOLVE_HOST_DRIVER) return fail(ctx, ML_ENOTSUP, "the linear solve runs on the fused engine");
   return ML_OK;
 }
 
@@ -367,6 +432,7 @@ int vc_solve_many(ml_vc_op op, int n, double* const phi[], const double* const r
   ml_ctx ctx = op->ctx;
   int rc = vc_solve_check(op, o);
   if (rc) return rc;
+  if (o.flags & ML_SOLVE_HOST_DRIVER) return vc_solve_host(op, n, phi, rhs, o, st);
   if ((rc = vc_work_ensure(op, n))) return rc;
   std::vector<VCSolve> S(n);
   for (int i = 0; i < n; ++i) {
@@ -473,6 +539,7 @@ int ml_vc_op_destroy(ml_vc_op op) {
     if (W.stream) hipStreamSynchronize(W.stream);
     vc_work_free(W);
   }
+  op->host_driver.reset();
   for (auto& L : op->lev) {
     for (int d = 0; d < 3; ++d)
       if (L.b[d]) hipFree(L.b[d]);

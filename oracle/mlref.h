@@ -184,6 +184,16 @@ int mlr_lin_mg_solve(const mlr_grid* g, int ncomp, const mlr_bc* bc,
  * count in use.  Results do not depend on it. */
 int mlr_set_threads(int n);
 
+/* levelJacobi, preCond, getFlux of the (H, Theta) operator (mlref_hc.c) */
+void mlr_hc_jacobi(const mlr_grid* g, const mlr_params* p, const mlr_bc* bc, double alpha, const double* a,
+                   double beta, const double* const* b, const double* lambda, double* phi, const double* rhs,
+                   double* res, double* scratch);
+void mlr_hc_precond(const mlr_grid* g, const mlr_params* p, const mlr_bc* bc, double alpha, const double* a,
+                    double beta, const double* const* b, const double* lambda, double* phi, const double* rhs,
+                    double* scratch);
+void mlr_hc_get_flux(const mlr_grid* g, const mlr_params* p, const mlr_bc* bc, double beta,
+                     const double* const* b, double* phi, double* const* flux, double* scratch);
+
 /* ---- state update and velocity closure of a Darcy-Brinkman step
  * (mlref_flow.c) ---------------------------------------------------------- */
 void mlr_enthalpy_vars_point(double H, double C, const mlr_params* p, double* out);

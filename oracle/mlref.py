@@ -112,6 +112,9 @@ def _declare(L):
         "mlr_cc_project_rhs": (None, [G, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, pp, pp, _dp]),
         "mlr_pressure_faces": (None, [G, _dp, _dp, pp]),
         "mlr_cc_correct": (None, [G, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, pp]),
+        "mlr_hc_jacobi": (None, [G, P, B, C.c_double, _dp, C.c_double, pp, _dp, _dp, _dp, _dp, _dp]),
+        "mlr_hc_precond": (None, [G, P, B, C.c_double, _dp, C.c_double, pp, _dp, _dp, _dp, _dp]),
+        "mlr_hc_get_flux": (None, [G, P, B, C.c_double, pp, _dp, pp, _dp]),
         "mlr_darcy_ccoef": (None, [G, _dp, C.c_double, C.c_int, C.c_double, _dp]),
         "mlr_lin_mg_solve": (C.c_int, [G, C.c_int, B, C.c_double, _dp, C.c_double, _dp, pp, _dp, _dp, C.c_int,
                                        C.POINTER(Opts), C.POINTER(Stats)]),
@@ -269,6 +272,35 @@ def hc_apply_op(g, p, bc, alpha, a, beta, b, phi):
     lib().mlr_hc_apply_op(C.byref(g), C.byref(p), C.byref(bc), alpha, ptr(a), beta, ptrs(b),
                           ptr(phi), ptr(lhs), ptr(scratch))
     return lhs
+
+
+def hc_jacobi(g, p, bc, alpha, a, beta, b, phi, rhs, iters=1):
+    """levelJacobi x iters (phi in place)."""
+    lm = lam(g, 2, alpha, a, beta, b)
+    res = np.zeros(cell_shape(g, 2))
+    scratch = np.zeros(cell_shape(g, 2))
+    for _ in range(iters):
+        lib().mlr_hc_jacobi(C.byref(g), C.byref(p), C.byref(bc), alpha, ptr(a), beta, ptrs(b), ptr(lm), ptr(phi),
+                            ptr(rhs), ptr(res), ptr(scratch))
+    return phi
+
+
+def hc_precond(g, p, bc, alpha, a, beta, b, rhs):
+    """preCond: phi = lambda*rhs then two GSRB iterations."""
+    lm = lam(g, 2, alpha, a, beta, b)
+    phi = np.zeros(cell_shape(g, 2))
+    scratch = np.zeros(cell_shape(g, 2))
+    lib().mlr_hc_precond(C.byref(g), C.byref(p), C.byref(bc), alpha, ptr(a), beta, ptrs(b), ptr(lm), ptr(phi),
+                         ptr(rhs), ptr(scratch))
+    return phi
+
+
+def hc_get_flux(g, p, bc, beta, b, phi):
+    """getFlux on every face: list of face arrays (2 components)."""
+    flux = [np.zeros(face_shape(g, d, 2)) for d in range(g.dim)]
+    scratch = np.zeros(cell_shape(g, 2))
+    lib().mlr_hc_get_flux(C.byref(g), C.byref(p), C.byref(bc), beta, ptrs(b), ptr(phi), ptrs(flux), ptr(scratch))
+    return flux
 
 
 def hc_diffused_var(g, p, bc, phi, homogeneous=0):

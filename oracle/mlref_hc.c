@@ -261,6 +261,69 @@ void mlr_hc_restrict_residual(const mlr_grid* gf, const mlr_params* p,
         }
 }
 
+/* levelJacobi (src/AMRNonLinearMultiCompOp.cpp:1077-1101), one sweep:
+ * resid = rhs - L(phi) (residualI, inhomogeneous); resid *= lambda;
+ * phi += 0.5 * resid (incr with weight 0.5). */
+void mlr_hc_jacobi(const mlr_grid* g, const mlr_params* p, const mlr_bc* bc, double alpha, const double* a,
+                   double beta, const double* const* b, const double* lambda, double* phi, const double* rhs,
+                   double* res, double* scratch) {
+  mlr_lay L;
+  mlr_layout(g, &L);
+  mlr_hc_residual(g, p, bc, alpha, a, beta, b, phi, rhs, res, scratch);
+  for (int n = 0; n < 2; ++n)
+    for (int k = 0; k < L.nz; ++k)
+      for (int j = 0; j < g->n[1]; ++j)
+        for (int i = 0; i < g->n[0]; ++i) {
+          long q = mlr_ci(&L, i, j, k) + n * L.ncell;
+          double r = res[q] * lambda[q];
+          phi[q] = phi[q] + 0.5 * r;
+        }
+}
+
+/* preCond (src/AMRNonLinearMultiCompOp.cpp:172-206): phi = rhs * lambda,
+ * then relax(phi, rhs, 2) (two levelGSRB iterations). */
+void mlr_hc_precond(const mlr_grid* g, const mlr_params* p, const mlr_bc* bc, double alpha, const double* a,
+                    double beta, const double* const* b, const double* lambda, double* phi, const double* rhs,
+                    double* scratch) {
+  mlr_lay L;
+  mlr_layout(g, &L);
+  for (int n = 0; n < 2; ++n)
+    for (int k = 0; k < L.nz; ++k)
+      for (int j = 0; j < g->n[1]; ++j)
+        for (int i = 0; i < g->n[0]; ++i) {
+          long q = mlr_ci(&L, i, j, k) + n * L.ncell;
+          phi[q] = rhs[q] * lambda[q];
+        }
+  for (int it = 0; it < 2; ++it) mlr_hc_gsrb(g, p, bc, alpha, a, beta, b, lambda, phi, rhs, scratch);
+}
+
+/* getFlux (src/AMRNonLinearMultiCompOp.cpp:1103-1162) on every face of the
+ * level, direction d, both components: flux = -b * ((phil_hi - phil_lo) * scale)
+ * with scale = beta * ref / dx (ref = 1) and phil the diffused variable
+ * (computeDiffusedVar with the inhomogeneous derived-variable BC, periodic
+ * images across periodic sides). */
+void mlr_hc_get_flux(const mlr_grid* g, const mlr_params* p, const mlr_bc* bc, double beta,
+                     const double* const* b, double* phi, double* const* flux, double* scratch) {
+  mlr_lay L;
+  mlr_layout(g, &L);
+  mlr_noflux_bc(g, 2, phi);
+  mlr_exchange(g, 2, phi);
+  mlr_hc_diffused_var(g, p, bc, phi, scratch, 0);
+  const double scale = beta * 1 / g->dx;
+  for (int d = 0; d < g->dim; ++d)
+    for (int n = 0; n < 2; ++n)
+      for (int k = 0; k < (int)L.fn[d][2]; ++k)
+        for (int j = 0; j < (int)L.fn[d][1]; ++j)
+          for (int i = 0; i < (int)L.fn[d][0]; ++i) {
+            int lo[3] = {i - (d == 0), j - (d == 1), k - (d == 2)};
+            double phihi = scratch[mlr_ci(&L, i, j, k) + n * L.ncell];
+            double philo = scratch[mlr_ci(&L, lo[0], lo[1], lo[2]) + n * L.ncell];
+            double gradphi = (phihi - philo) * scale;
+            long f = mlr_fi(&L, d, i, j, k) + n * L.nface[d];
+            flux[d][f] = -b[d][f] * gradphi;
+          }
+}
+
 /* ---- coefficient assembly ---------------------------------------------- */
 
 /* Porosity of every valid cell (CALCULATEPOROSITY_MULTICOMP as used by

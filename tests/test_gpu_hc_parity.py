@@ -258,3 +258,32 @@ def test_steps_follow_changed_options_and_dt(ml, gpu_ctx, wl, precision):
             assert st_ref.exit_status & (1 | 8) and st.exit_status & (1 | 8), (k, st_ref.exit_status, st.exit_status)
             e = HC.rel_err(HC.valid(n, out), HC.valid(n, new))
             assert e <= 1e-9, (k, e)
+
+
+@pytest.mark.parametrize("name", ["2d_64_periodic_x", "2d_48x32_walls", "2d_32_no_derived_bc", "3d_24x16x8_walls"])
+def test_jacobi_precond_flux_bitwise(ml, gpu_ctx, wl, name):
+    """levelJacobi (2 sweeps), preCond and getFlux against the oracle, bit for
+    bit, on a state spanning every phase regime."""
+    case, g, p, bc, b, op, db = _setup(ml, gpu_ctx, wl, name)
+    n = case["n"]
+    dt = case["dt"]
+    op.set_coefs(None, db, 1.0, -dt)
+    phi = case["phi"]
+    rng = np.random.default_rng(9)
+    rhs = phi + 1e-3 * rng.standard_normal(phi.shape)
+    ref = O.hc_jacobi(g, p, bc, 1.0, None, -dt, b, phi.copy(), rhs, iters=2)
+    dphi = ml.DeviceArray(gpu_ctx, phi.shape, host=phi)
+    drhs = ml.DeviceArray(gpu_ctx, rhs.shape, host=rhs)
+    op.jacobi(0, dphi, drhs, 2)
+    assert HC.bitwise_equal(HC.valid(n, dphi.get()), HC.valid(n, ref)), "jacobi"
+    ref = O.hc_precond(g, p, bc, 1.0, None, -dt, b, rhs)
+    dpc = ml.DeviceArray(gpu_ctx, phi.shape)
+    op.precond(0, dpc, drhs)
+    assert HC.bitwise_equal(HC.valid(n, dpc.get()), HC.valid(n, ref)), "precond"
+    fref = O.hc_get_flux(g, p, bc, -dt, b, phi.copy())
+    dphi = ml.DeviceArray(gpu_ctx, phi.shape, host=phi)
+    df = [ml.DeviceArray(gpu_ctx, x.shape) for x in fref]
+    op.get_flux(0, dphi, df)
+    for d in range(len(n)):
+        assert HC.bitwise_equal(df[d].get(), fref[d]), ("flux", d)
+    op.close()

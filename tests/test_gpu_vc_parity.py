@@ -253,3 +253,42 @@ def test_solves_follow_changed_options_and_coefficients(ml, gpu_ctx):
         st = op.solve(dphi, drhs, opts=ml.default_opts(**kw))
         assert (st.iterations, st.exit_status) == (st_ref.iterations, st_ref.exit_status), k
         _check(n, dphi.get(), phi)
+
+
+@pytest.mark.parametrize("name", ["db_2d_64", "proj_2d_48x32", "db_3d_16", "proj_3d_24x16x8", "db_2d_1024x512"])
+def test_host_driver_equals_engine(ml, gpu_ctx, name):
+    """ML_SOLVE_HOST_DRIVER runs the linear MG through the reference-API
+    operator objects (host/HipVCOps.H, HipDarcyBrinkmanOp per MG depth) and
+    the host driver (host/LinearMultiGrid.H), one C-ABI call per operator
+    method: the same iterations, statistics and bits as the fused engine,
+    for the plain solve and the backward-Euler form of two components."""
+    C, w, g, bc, op, keep = _setup(ml, gpu_ctx, name)
+    rhs = _rand(g, 21)
+    if C["c"] is None:
+        O.valid(g, rhs)[...] -= O.valid(g, rhs).mean()
+    drhs = ml.DeviceArray(gpu_ctx, rhs.shape, host=rhs)
+    out = {}
+    for flags in (0, ml.SOLVE_HOST_DRIVER):
+        o = ml.default_opts(nu1=2, nu2=2, bottom=8, max_iter=12, eps=1e-12, hang=1e-14, flags=flags)
+        dphi = ml.DeviceArray(gpu_ctx, rhs.shape)
+        st = op.solve(dphi, drhs, opts=o)
+        out[flags] = (dphi.get(), (st.exit_status, st.iterations, st.initial_residual, st.final_residual))
+    e, h = out[0], out[ml.SOLVE_HOST_DRIVER]
+    print("%s: engine %s, host driver %s" % (name, e[1], h[1]))
+    assert e[1] == h[1]
+    assert HC.bitwise_equal(e[0], h[0])
+    if C["c"] is None:
+        return
+    # backward-Euler solves of two components (the U* form)
+    old = [_rand(g, 30 + i) for i in range(2)]
+    src = [_rand(g, 40 + i) for i in range(2)]
+    res = {}
+    for flags in (0, ml.SOLVE_HOST_DRIVER):
+        o = ml.default_opts(nu1=2, nu2=2, bottom=8, max_iter=10, eps=1e-10, hang=1e-10, flags=flags)
+        dn = [ml.DeviceArray(gpu_ctx, x.shape, host=x) for x in old]
+        do = [ml.DeviceArray(gpu_ctx, x.shape, host=x) for x in old]
+        ds = [ml.DeviceArray(gpu_ctx, x.shape, host=x) for x in src]
+        sts = op.solve_be_many(dn, do, C["w"]["dt"], src=ds, opts=o)
+        res[flags] = ([x.get() for x in dn], [(s.exit_status, s.iterations) for s in sts])
+    assert res[0][1] == res[ml.SOLVE_HOST_DRIVER][1]
+    assert all(HC.bitwise_equal(a, b) for a, b in zip(res[0][0], res[ml.SOLVE_HOST_DRIVER][0]))

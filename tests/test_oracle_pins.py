@@ -523,3 +523,84 @@ def test_cell_average_is_brute_force_mean(n, ratio):
     # a constant field is reproduced exactly
     one = np.full(O.cell_shape(g, 1), 0.7)
     np.testing.assert_allclose(O.valid(gc, O.average_cells(g, 1, one, ratio)), 0.7, rtol=1e-15)
+
+
+# ------------------------------------------ levelJacobi, preCond, getFlux
+
+def test_jacobi_is_weighted_jacobi_in_liquid():
+    """levelJacobi in the liquid regime (linear operator A x + f): one sweep
+    is x + 0.5 D^-1 (r - A x - f) with D the diagonal of the assembled
+    matrix (doubly periodic, so lambda is exactly 1/D)."""
+    g, p, bc, spec, phi, b = _liquid_case(periodic_y=True)
+    n = (g.n[0], g.n[1])
+    alpha, beta = 1.0, -1e-3
+    A, f = _dense_liquid_matrix(g, n, alpha, beta, b, spec, 5.0, periodic_y=True)
+    rng = np.random.default_rng(15)
+    rhs = phi + 1e-3 * rng.standard_normal(phi.shape)
+    r = np.concatenate([O.valid(g, rhs)[c].ravel() for c in range(2)])
+    x = np.concatenate([O.valid(g, phi)[c].ravel() for c in range(2)])
+    expect = x + 0.5 * (r - f - A @ x) / np.diag(A)
+    out = O.hc_jacobi(g, p, bc, alpha, None, beta, b, phi.copy(), rhs, iters=1)
+    xs = np.concatenate([O.valid(g, out)[c].ravel() for c in range(2)])
+    np.testing.assert_allclose(xs, expect, rtol=1e-13, atol=1e-13)
+
+
+def test_precond_is_diagonal_start_then_two_gauss_seidel_sweeps():
+    """preCond in the liquid regime: x0 = D^-1 rhs, then two red/black
+    Gauss-Seidel sweeps of the assembled system (a small beta keeps the
+    diagonal start D^-1 rhs in the liquid regime, where the operator is
+    linear)."""
+    g, p, bc, spec, phi, b = _liquid_case(periodic_y=True)
+    n = (g.n[0], g.n[1])
+    alpha, beta = 1.0, -1e-5
+    A, f = _dense_liquid_matrix(g, n, alpha, beta, b, spec, 5.0, periodic_y=True)
+    rng = np.random.default_rng(16)
+    rhs = phi + 1e-3 * rng.standard_normal(phi.shape)
+    r = np.concatenate([O.valid(g, rhs)[c].ravel() for c in range(2)])
+    x = r / np.diag(A)
+    N = n[0] * n[1]
+    for _ in range(2):
+        for colour in (0, 1):
+            xo = x.copy()
+            for comp in range(2):
+                for j in range(n[1]):
+                    for i in range(n[0]):
+                        if (i + j + colour) % 2:
+                            continue
+                        q = comp * N + i + n[0] * j
+                        x[q] = xo[q] + (r[q] - f[q] - A[q] @ xo) / A[q, q]
+    out = O.hc_precond(g, p, bc, alpha, None, beta, b, rhs)
+    xs = np.concatenate([O.valid(g, out)[c].ravel() for c in range(2)])
+    np.testing.assert_allclose(xs, x, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,periodic", [((8, 6), (1, 0, 0)), ((6, 8), (0, 0, 0)), ((4, 6, 8), (1, 0, 0))])
+def test_flux_divergence_is_the_operator(n, periodic):
+    """getFlux: the discrete divergence of the face fluxes is the operator
+    without its alpha a phi term, sum_d (F(i+1/2) - F(i-1/2))/dx =
+    L(phi) - alpha a phi, on a state with every closure regime, walls of
+    both kinds and periodic sides."""
+    g = O.make_grid(list(n), 1.0 / n[0], list(periodic))
+    p = params()
+    spec = {(g.dim - 1, 0, 0): (0, -0.4), (g.dim - 1, 1, 0): (0, 1.1), (g.dim - 1, 0, 1): (1, 0.2),
+            (g.dim - 1, 1, 1): (0, -0.9)}
+    if not periodic[0]:
+        spec[(0, 0, 0)] = (1, 0.3)
+    bc = O.make_bc(spec)
+    rng = np.random.default_rng(17)
+    phi = _mushy_state(g, rng)
+    b = [-rng.uniform(0.1, 1.0, O.face_shape(g, d, 2)) for d in range(g.dim)]
+    alpha, beta = 1.0, -2e-3
+    flux = O.hc_get_flux(g, p, bc, beta, b, phi.copy())
+    Lphi = O.hc_apply_op(g, p, bc, alpha, None, beta, b, phi.copy())
+    div = np.zeros(O.valid(g, Lphi).shape)
+    for d in range(g.dim):
+        f = flux[d]
+        ax = 3 - d
+        hi = [slice(None)] * 4
+        lo = [slice(None)] * 4
+        hi[ax] = slice(1, None)
+        lo[ax] = slice(0, -1)
+        div += (f[tuple(hi)] - f[tuple(lo)]) / g.dx
+    expect = O.valid(g, Lphi) - alpha * O.valid(g, phi)
+    np.testing.assert_allclose(div, expect, rtol=1e-11, atol=1e-9 * np.abs(expect).max())
