@@ -5,8 +5,10 @@ Workload (one GPU): one Darcy-Brinkman mushy-layer step on a 256x256 grid
 (porosity from a seeded smooth Gaussian random field in [0.05, 1],
 correlation length 0.1; St = 5, CR = 2, Le = 200, Pr = 10, Da = 1e-3,
 Ra_C = 200, Kozeny-Carman permeability, dt = 1e-3; velocity and pressure
-start at 0): enthalpy variables -> HC face coefficients -> backward-Euler
-solve of (H, Theta) to |r| <= 1e-10 |r0| (or 1e-10 absolute) -> enthalpy
+start at 0): enthalpy variables -> HC face coefficients -> advective source
+of (H, Theta) (diffusive source, face velocities of u^n, Godunov fluxes of
+(theta, Theta_l)) -> backward-Euler solve of (H, Theta) to
+|r| <= 1e-10 |r0| (or 1e-10 absolute; no hang rule, max 100 V-cycles) -> enthalpy
 variables at the new time -> U* source (buoyancy - porosity grad P) and
 Darcy coefficient -> U* backward-Euler solve per velocity component ->
 projection rhs div(u)/dt -> variable-coefficient projection solve ->
@@ -262,7 +264,7 @@ def cpu_baseline(case, flow, seconds, max_iter):
 
     cells = int(np.prod(case["n"]))
     shape = "x".join(map(str, case["n"]))
-    what = "step(s)" if flow is None else "Darcy-Brinkman step(s) (HC solve, U* solves, projection)"
+    what = "step(s)" if flow is None else "Darcy-Brinkman step(s) (advective source, HC solve, U* solves, projection)"
     O.set_threads(1)
     one, dt1 = cpu_sample(case, flow, seconds, max_iter)
     nthr = O.set_threads(host_cores())
@@ -279,7 +281,7 @@ def cpu_baseline(case, flow, seconds, max_iter):
 
 def run_form(ml, ctx, case, precision, steps, warmup, max_iter, flow=None, no_lookahead=False):
     stp = importlib.import_module("mushy-layer_amd.steps")
-    opts = ml.default_opts(max_iter=max_iter, hang=-1.0, precision=precision)
+    opts = ml.default_opts(max_iter=max_iter, hang=stp.NO_HANG, precision=precision)
     st = stp.HCStepper(ml, ctx, case, opts) if flow is None else stp.DarcyBrinkmanStepper(ml, ctx, case, opts, flow)
     if no_lookahead:
         st.set_flags(ml.SOLVE_NO_LOOKAHEAD)
@@ -650,17 +652,21 @@ def main():
         log("CPU sample done: %d step(s) on 1 thread" % cpu_state.steps)
 
     what = ("HC backward-Euler solve (enthalpy variables + coefficient assembly + solve)" if args.hc_only else
-            "Darcy-Brinkman step: HC backward-Euler solve, enthalpy variables, U* source (buoyancy - porosity "
+            "Darcy-Brinkman step: advective source of (H, Theta) (diffusive source, face velocities of u^n, "
+            "unsplit PLM Godunov fluxes of (theta, Theta_l)), HC backward-Euler solve, enthalpy variables, U* "
+            "source (buoyancy - porosity "
             "grad P^{n-1/2}) and Darcy coefficient (Kozeny-Carman, Pr 10, Da %g, Ra_C %g), U* Helmholtz solve per "
             "velocity component, projection rhs div(u)/dt, variable-coefficient projection from pi = 0, "
             "projection correction, all on the device" % (Da, Ra_C))
     cfg = {"workload": "mushy layer %s, %s, GRF porosity corr %g, St 5, CR 2, Le 200, dt 1e-3, velocity and "
                        "pressure start at 0" % ("x".join(map(str, shape)), what, corr),
            "form": head, "tolerance": "1e-10 rel or abs, max-norm",
-           "max_iter": args.max_iter, "hang": "disabled (a solve stops on eps/norm_thresh or max_iter only)",
+           "max_iter": args.max_iter, "hang": "disabled (hang = -inf: a solve stops on eps/norm_thresh or max_iter only)",
            "velocity_solves": {"ustar": "V(2,2), bottom 8, max 10, eps = hang = 1e-10, from the previous U*",
                                "projection": "V(2,4), bottom 4, max 20, eps = hang = 1e-15, from pi = 0 each step"},
-           "advection": "none (no advective source on this path)",
+           "advection": "HC advective source each step (computeScalarAdvectiveSrcHC): PLM Godunov predictor with "
+                        "van Leer limiting and corner-transport corrections, advection velocity = face average of "
+                        "the projected cell velocity u^n (the reference's MAC-projected predictor is not built)",
            "iterations_per_step": round(float(np.mean(h["iters"])), 2), "exit_status": h["status"]}
     if hc_only:
         cfg["hc_solve_only"] = hc_only

@@ -420,6 +420,44 @@ enum { ML_PERM_PURE_FLUID = 0, ML_PERM_CUBIC = 1, ML_PERM_KOZENY_CARMAN = 2, ML_
 int ml_vc_darcy_ccoef_law(ml_vc_op op, const double* chi, double darcy_coeff, int law, double hele_shaw,
                           double* cCoef);
 
+/* ---- the advective source of the (H, Theta) equations ---------------------
+ * multiCompAdvectDiffuse -> computeScalarAdvectiveSrcHC
+ * (srcSubcycle/AMRLevelMushyLayer.cpp:1645-1672, 2074-2104): face values of
+ * W = (theta, Theta_l) at t^n + dt/2 by the unsplit PLM Godunov predictor
+ * (PatchGodunov with AdvectionPhysics, srcSubcycle/AdvectionPhysics.cpp;
+ * HC options srcSubcycle/AMRLevelMushyLayerInit.cpp:497-537), fluxes
+ * W_half*u_face*advectionCoeff (computeScalarAdvectiveFlux, :1165-1198),
+ * src = src - div(flux) (util/Divergence.cpp:19-60).
+ * Boundary types per [dir][side][comp] in AdvectIBC's numbering
+ * (BCutil/AdvectIBC.H:137-144; primBC BCutil/AdvectIBC.cpp:130-260); the
+ * plume inflow is not supported (ML_ENOTSUP). */
+enum { ML_ADV_DIRICHLET = 0, ML_ADV_EXTRAP = 1, ML_ADV_INFLOW_OUTFLOW = 2, ML_ADV_PLUME_INFLOW = 3,
+       ML_ADV_NEUMANN = 4 };
+typedef struct {
+  double advectionCoeff;   /* m_advectionCoeff (src/MushyLayerParams.cpp:714-801) */
+  int type[3][2][2];
+  double val[3][2][2];
+} ml_advect_params;
+/* computeScalDiffusion (srcSubcycle/AMRLevelMushyLayer.cpp:1280-1340): the
+ * operator applied with alpha = 0, beta = 1 and inhomogeneous BCs, i.e. the
+ * diffusion terms of (H, Theta) at the state phi (2 components; the
+ * operator's current face coefficients) */
+int ml_hc_diffusive_src(ml_hc_op op, double* phi, double* out);
+/* W: 2-component cell array (theta, Theta_l), e.g. the first two components
+ * of ml_hc_enthalpy_vars' output; u_face[d]: face velocities; S: the
+ * diffusive source (2 components) or NULL (diffusiveSrcForAdvection false);
+ * src: 2 components, updated in place on the valid cells; flux[d]: optional
+ * 2-component face outputs (edgeScalTotal), NULL to skip.  Ghosts of W and S
+ * are not read (periodic sides wrap, the slopes next to a wall are
+ * one-sided).  The operator keeps a device workspace for the stages. */
+int ml_hc_advect_src(ml_hc_op op, const ml_advect_params* ap, double dt, const double* W,
+                     const double* const u_face[], const double* S, double* src, double* const flux[]);
+/* CellToEdge of a no-slip cell-centred velocity (1-component arrays per
+ * direction): interior and periodic faces 0.5*(u_lo + u_hi), wall faces 0.
+ * The advection velocity a caller without the reference's MAC-projected
+ * predictor passes to ml_hc_advect_src. */
+int ml_vc_face_velocity(ml_vc_op op, const double* const u[], double* const u_face[]);
+
 /* ---- multi-box levels ----------------------------------------------------
  * A level of the reference is a DisjointBoxLayout (the default
  * max_grid_size 128, srcSubcycle/MushyLayerSubcycleUtils.cpp:1011-1012,

@@ -54,6 +54,37 @@ class FlowParams(C.Structure):
                 ("scalePressure", C.c_int)]
 
 
+class AdvectParams(C.Structure):
+    _fields_ = [("advectionCoeff", C.c_double), ("type", C.c_int * 2 * 2 * 3), ("val", C.c_double * 2 * 2 * 3)]
+
+
+# advection boundary types, AdvectIBC's numbering (BCutil/AdvectIBC.H:137-144)
+ADV_DIRICHLET, ADV_EXTRAP, ADV_INFLOW_OUTFLOW, ADV_PLUME_INFLOW, ADV_NEUMANN = 0, 1, 2, 3, 4
+
+
+def make_advect_params(hc_bc=None, coeff=1.0, spec=None):
+    """Advection BCs of (theta, Theta_l): from the HC operator's BC as
+    PhysBCUtil::scalarTraceTSl_IBC converts them (Dirichlet -> Dirichlet,
+    Neumann -> Neumann, same values; BCutil/PhysBCUtil.cpp:3380-3434,
+    4059-4087), else extrapolation; spec {(dir, side, comp): (type, value)}
+    overrides.  coeff: m_advectionCoeff."""
+    a = AdvectParams()
+    a.advectionCoeff = coeff
+    for d in range(3):
+        for s in range(2):
+            for c in range(2):
+                t, v = ADV_EXTRAP, 0.0
+                if hc_bc is not None:
+                    t = ADV_DIRICHLET if hc_bc.type[d][s][c] == 0 else ADV_NEUMANN
+                    v = hc_bc.val[d][s][c]
+                a.type[d][s][c] = t
+                a.val[d][s][c] = v
+    for (d, s, c), (t, v) in (spec or {}).items():
+        a.type[d][s][c] = t
+        a.val[d][s][c] = v
+    return a
+
+
 PERM_PURE_FLUID, PERM_CUBIC, PERM_KOZENY_CARMAN, PERM_LOG, PERM_POROSITY = 0, 1, 2, 3, 4
 PERM_LAWS = {"pure_fluid": PERM_PURE_FLUID, "cubic": PERM_CUBIC, "kozeny_carman": PERM_KOZENY_CARMAN,
              "log": PERM_LOG, "porosity": PERM_POROSITY}
@@ -157,6 +188,10 @@ SIGNATURES = {
     "ml_vc_pressure_faces": (C.c_int, [_vp, _vp, _vp, C.POINTER(_vp)]),
     "ml_vc_cc_correct": (C.c_int, [_vp, C.POINTER(FlowParams), C.c_double, _vp, _vp, _vp, C.POINTER(_vp)]),
     "ml_vc_darcy_ccoef_law": (C.c_int, [_vp, _vp, C.c_double, C.c_int, C.c_double, _vp]),
+    "ml_hc_diffusive_src": (C.c_int, [_vp, _vp, _vp]),
+    "ml_hc_advect_src": (C.c_int, [_vp, C.POINTER(AdvectParams), C.c_double, _vp, C.POINTER(_vp), _vp, _vp,
+                                   C.POINTER(_vp)]),
+    "ml_vc_face_velocity": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
     "ml_level_gather": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_int, C.c_int, _ip, _ip, C.POINTER(_vp), _vp]),
     "ml_level_scatter": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_int, C.c_int, _ip, _ip, _vp, C.POINTER(_vp)]),
     "ml_level_gather_faces": (C.c_int, [_vp, C.c_int, _ip, C.c_int, C.c_int, _ip, _ip, C.c_int, C.POINTER(_vp),
@@ -396,6 +431,7 @@ class HCOp:
         ctx.check(ctx.lib.ml_hc_op_create(ctx.h, self.dim, lo, hi, dx, _iarr(periodic), C.byref(params),
                                           C.byref(bc), C.byref(h)), "ml_hc_op_create")
         self.h = h
+        self.bc = bc
         self.nlev = ctx.lib.ml_hc_num_levels(h)
 
     def level_n(self, depth):
@@ -464,6 +500,27 @@ class HCOp:
 
     def porosity(self, phi, chi):
         self.ctx.check(self.ctx.lib.ml_hc_porosity(self.h, phi.ptr, chi.ptr), "ml_hc_porosity")
+
+    def diffusive_src(self, phi, out):
+        """out = the operator with alpha = 0, beta = 1 applied to phi
+        (computeScalDiffusion)."""
+        self.ctx.check(self.ctx.lib.ml_hc_diffusive_src(self.h, phi.ptr, out.ptr), "ml_hc_diffusive_src")
+
+    def advect_src(self, ap, dt, W, u_face, S, src, flux=None):
+        """src -= div(W_half u_face coeff) (computeScalarAdvectiveSrcHC).  W: a
+        DeviceArray or device address of the 2-component (theta, Theta_l)
+        field; S: the diffusive source or None; flux: optional face outputs."""
+        uf = (_vp * 3)()
+        for d, x in enumerate(u_face):
+            uf[d] = x.ptr
+        fl = None
+        if flux is not None:
+            fl = (_vp * 3)()
+            for d, x in enumerate(flux):
+                fl[d] = x.ptr
+        w = W.ptr if hasattr(W, "ptr") else W
+        self.ctx.check(self.ctx.lib.ml_hc_advect_src(self.h, C.byref(ap), dt, w, uf, S.ptr if S is not None else None,
+                                                     src.ptr, fl), "ml_hc_advect_src")
 
     def assemble_bcoef(self, chi, b):
         """b (face arrays) from the cell porosity chi (a DeviceArray or a
@@ -644,6 +701,11 @@ class VCOp:
     def pressure_faces(self, vold, vnew, b):
         self.ctx.check(self.ctx.lib.ml_vc_pressure_faces(self.h, vold.ptr, vnew.ptr, self._arr(b)),
                        "ml_vc_pressure_faces")
+
+    def face_velocity(self, u, u_face):
+        """u_face[d] = CellToEdge of the no-slip cell velocity (0 on walls)."""
+        self.ctx.check(self.ctx.lib.ml_vc_face_velocity(self.h, self._arr(u), self._arr(u_face)),
+                       "ml_vc_face_velocity")
 
     def cc_correct(self, fp, dt, vold, vnew, pi, u):
         self.ctx.check(self.ctx.lib.ml_vc_cc_correct(self.h, C.byref(fp), dt, vold.ptr, vnew.ptr, pi.ptr,

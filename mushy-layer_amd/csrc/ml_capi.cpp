@@ -332,6 +332,7 @@ int ml_hc_op_destroy(ml_hc_op op) {
     if (L.a) hipFree(L.a);
     if (L.tmp) hipFree(L.tmp);
   }
+  if (op->adv) hipFree(op->adv);
   delete op;
   return ML_OK;
 }

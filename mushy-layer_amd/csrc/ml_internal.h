@@ -104,6 +104,8 @@ struct ml_hc_op_s {
   std::unique_ptr<ml::HostDriver, void (*)(ml::HostDriver*)> host_driver{nullptr, nullptr};
   std::unique_ptr<ml::HCEngine, void (*)(ml::HCEngine*)> engine{nullptr, nullptr};
   std::unique_ptr<ml::AdjWork, void (*)(ml::AdjWork*)> adj{nullptr, nullptr};
+  double* adv = nullptr;   // workspace of the advective source (allocated on first use)
+  size_t adv_bytes = 0;
 
   ml::HCArgs<double> args(int depth) const {
     ml::HCArgs<double> A;

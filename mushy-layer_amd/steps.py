@@ -19,12 +19,18 @@ the (H, Theta) step, the enthalpy variables at t^{n+1}
     rhs div(u)/dt, faces -(porosity at the half time), solve from pi = 0
     (Projector::levelProject, srcSubcycle/Projector.cpp:1340-1486,
     solveMGlevel :3164-3173), u -= dt chi grad pi, zero where solid.
-Everything stays on the device; no advective source enters (the solve path
-has no advection).
+Before the (H, Theta) solve the stepper forms its advective source
+(computeScalarAdvectiveSrcHC, srcSubcycle/AMRLevelMushyLayer.cpp:2074-2104):
+the diffusive source of (H, Theta)^n (computeScalDiffusion), the face
+average of the velocity u^n as the advection velocity (the reference
+advects with its MAC-projected predictor, which is not built here), and
+the Godunov face values of (theta, Theta_l)^n.  Everything stays on the
+device.
 """
 import numpy as np
 
-from . import (DeviceArray, HCOp, VCOp, default_opts, face_shape, make_bc, make_flow_params, make_params, valid)
+from . import (DeviceArray, HCOp, VCOp, default_opts, face_shape, make_advect_params, make_bc, make_flow_params,
+               make_params, valid)
 
 # multigrid options of the velocity solves: U* (VelocityMultigrid: smoothing
 # 2, 10 iterations, tolerance 1e-10) and the projection
@@ -33,6 +39,9 @@ from . import (DeviceArray, HCOp, VCOp, default_opts, face_shape, make_bc, make_
 U_OPTS = dict(nu1=2, nu2=2, bottom=8, max_iter=10, eps=1e-10, hang=1e-10, norm_thresh=1e-10)
 P_OPTS = dict(nu1=2, nu2=4, bottom=4, max_iter=20, eps=1e-15, hang=1e-15, norm_thresh=1e-15)
 POROSITY_LIMIT = 0.05   # solidPorosity (srcSubcycle/MushyLayerSubcycleUtils.cpp:223-233)
+# hang rule off: a solve stops on eps, norm_thresh or max_iter only
+# (|r| > (1 - hang)*|r_prev| never holds for hang = -inf)
+NO_HANG = float("-inf")
 
 
 def flow_coeffs(Pr, Da, Ra_C, Ra_T=0.0):
@@ -70,13 +79,18 @@ class HCStepper:
     def set_flags(self, flags):
         self.opts.flags = flags
 
+    def hc_src(self):
+        """Source of the (H, Theta) step (none without flow)."""
+        return None
+
     def hc_step(self):
         op = self.op
         op.enthalpy_vars(self.phi, self.vold)
         op.assemble_bcoef(self.vold.comp_ptr(3), self.b)
         op.set_coefs(None, self.b, 1.0, -self.dt)
+        src = self.hc_src()
         self.copy(self.old, self.phi)
-        st = op.solve_be(self.phi, self.old, self.dt, opts=self.opts)
+        st = op.solve_be(self.phi, self.old, self.dt, src=src, opts=self.opts)
         self.stats.append((st.exit_status, st.iterations, st.final_residual))
 
     def step(self):
@@ -95,11 +109,12 @@ class DarcyBrinkmanStepper(HCStepper):
     """The Darcy-Brinkman step (module docstring).  `flow`: Pr, Da, Ra_C
     (Ra_T), the velocity and pressure BC specs, the permeability law."""
 
-    def __init__(self, ml, ctx, case, opts, flow):
+    def __init__(self, ml, ctx, case, opts, flow, advect=True):
         super().__init__(ml, ctx, case, opts)
         n = self.n
         dim = len(n)
         self.flow = flow
+        self.advect = advect
         self.coef = flow_coeffs(flow["Pr"], flow["Da"], flow["Ra_C"], flow.get("Ra_T", 0.0))
         self.law = flow.get("permeability", "kozeny_carman")
         self.fp = make_flow_params(self.coef["buoyT"], self.coef["buoyS"], POROSITY_LIMIT, 1)
@@ -116,6 +131,11 @@ class DarcyBrinkmanStepper(HCStepper):
         self.bp = [DeviceArray(ctx, face_shape(n, d, 1)) for d in range(dim)]
         self.prhs = DeviceArray(ctx, shp1)
         self.pi = DeviceArray(ctx, shp1)
+        shp2 = (2,) + case["phi"].shape[1:]
+        self.dsrc = DeviceArray(ctx, shp2)
+        self.hsrc = DeviceArray(ctx, shp2)
+        self.uf = [DeviceArray(ctx, face_shape(n, d, 1)) for d in range(dim)]
+        self.ap = make_advect_params(self.op.bc, 1.0)
         self.uopts = default_opts(**U_OPTS)
         self.popts = default_opts(**P_OPTS)
         self.vstats = []
@@ -133,6 +153,18 @@ class DarcyBrinkmanStepper(HCStepper):
         super().set_flags(flags)
         self.uopts.flags = flags
         self.popts.flags = flags
+
+    def hc_src(self):
+        """The advective source: diffusive source of (H, Theta)^n, face
+        velocities of u^n, src = -div(Godunov fluxes of (theta, Theta_l)^n)."""
+        if not self.advect:
+            return None
+        op = self.op
+        op.diffusive_src(self.phi, self.dsrc)
+        self.uop.face_velocity(self.u, self.uf)
+        self._zero(self.hsrc)
+        op.advect_src(self.ap, self.dt, self.vold, self.uf, self.dsrc, self.hsrc)
+        return self.hsrc
 
     def step(self):
         self.hc_step()

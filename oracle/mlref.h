@@ -208,6 +208,20 @@ void mlr_cc_correct(const mlr_grid* g, double dt, double limit, int scaleP, cons
                     const double* vnew, double* pi, double* const* u);
 void mlr_darcy_ccoef(const mlr_grid* g, const double* chi, double coeff, int law, double hele_shaw, double* c);
 
+/* ---- advective source of the (H, Theta) equations (mlref_advect.c) ------
+ * Advection boundary conditions per [dir][side][comp] with AdvectIBC's
+ * numbering (BCutil/AdvectIBC.H:137-144): 0 Dirichlet, 1 extrapolation,
+ * 2 inflow/outflow, 4 Neumann (3, the plume inflow, is not supported). */
+typedef struct {
+  int type[3][2][2];
+  double val[3][2][2];
+} mlr_advect_bc;
+void mlr_edge_to_cell(const mlr_grid* g, const double* const* uf, double* const* vc);
+void mlr_face_velocity(const mlr_grid* g, const double* const* u, double* const* uf);
+int mlr_advect_src(const mlr_grid* g, const mlr_advect_bc* bc, double dt, double advCoeff, const double* W,
+                   const double* const* uf, const double* S, double* src, double* const* flux,
+                   double* const* whalf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,10 @@ class BC(C.Structure):
     _fields_ = [("type", C.c_int * 2 * 2 * 3), ("val", C.c_double * 2 * 2 * 3), ("apply", C.c_int)]
 
 
+class AdvectBC(C.Structure):
+    _fields_ = [("type", C.c_int * 2 * 2 * 3), ("val", C.c_double * 2 * 2 * 3)]
+
+
 class Opts(C.Structure):
     _fields_ = [("nu1", C.c_int), ("nu2", C.c_int), ("bottom", C.c_int), ("max_iter", C.c_int),
                 ("eps", C.c_double), ("hang", C.c_double), ("norm_thresh", C.c_double)]
@@ -116,6 +120,9 @@ def _declare(L):
         "mlr_hc_precond": (None, [G, P, B, C.c_double, _dp, C.c_double, pp, _dp, _dp, _dp, _dp]),
         "mlr_hc_get_flux": (None, [G, P, B, C.c_double, pp, _dp, pp, _dp]),
         "mlr_darcy_ccoef": (None, [G, _dp, C.c_double, C.c_int, C.c_double, _dp]),
+        "mlr_edge_to_cell": (None, [G, pp, pp]),
+        "mlr_face_velocity": (None, [G, pp, pp]),
+        "mlr_advect_src": (C.c_int, [G, C.POINTER(AdvectBC), C.c_double, C.c_double, _dp, pp, _dp, _dp, pp, pp]),
         "mlr_lin_mg_solve": (C.c_int, [G, C.c_int, B, C.c_double, _dp, C.c_double, _dp, pp, _dp, _dp, C.c_int,
                                        C.POINTER(Opts), C.POINTER(Stats)]),
     }
@@ -466,6 +473,61 @@ def darcy_ccoef(g, chi, coeff, law="kozeny_carman", hele_shaw=0.0):
     lib().mlr_darcy_ccoef(C.byref(g), ptr(np.ascontiguousarray(chi)), coeff, PERM_LAWS.get(law, law), hele_shaw,
                           ptr(c))
     return c
+
+
+# advection BC types (AdvectIBC numbering, BCutil/AdvectIBC.H:137-144)
+ADV_DIRICHLET, ADV_EXTRAP, ADV_INFLOW_OUTFLOW, ADV_NEUMANN = 0, 1, 2, 4
+
+
+def make_advect_bc(spec=None, hc_bc=None):
+    """Advection BC of (theta, Theta_l).  hc_bc: the HC operator's BC, converted
+    as PhysBCUtil::convertBCType does (Dirichlet -> Dirichlet, Neumann ->
+    Neumann, same values; BCutil/PhysBCUtil.cpp:3380-3434, 4059-4087);
+    spec: {(dir, side, comp): (type, value)} overrides.  Default extrapolation."""
+    b = AdvectBC()
+    for d in range(3):
+        for s in range(2):
+            for c in range(2):
+                t, v = ADV_EXTRAP, 0.0
+                if hc_bc is not None:
+                    t = ADV_DIRICHLET if hc_bc.type[d][s][c] == 0 else ADV_NEUMANN
+                    v = hc_bc.val[d][s][c]
+                b.type[d][s][c] = t
+                b.val[d][s][c] = v
+    for (d, s, c), (t, v) in (spec or {}).items():
+        b.type[d][s][c] = t
+        b.val[d][s][c] = v
+    return b
+
+
+def edge_to_cell(g, uf):
+    vc = [np.zeros(cell_shape(g, 1)) for _ in range(g.dim)]
+    lib().mlr_edge_to_cell(C.byref(g), ptrs([np.ascontiguousarray(u) for u in uf]), ptrs(vc))
+    return vc
+
+
+def face_velocity(g, u):
+    """Face velocities (CellToEdge, zero on walls) of the velocity components u."""
+    uf = [np.zeros(face_shape(g, d, 1)) for d in range(g.dim)]
+    lib().mlr_face_velocity(C.byref(g), ptrs([np.ascontiguousarray(x) for x in u]), ptrs(uf))
+    return uf
+
+
+def advect_src(g, abc, dt, coeff, W, uf, S=None, src=None, whalf=False):
+    """Advective source of (H, Theta): returns (src - div(flux), flux[d],
+    whalf[d] or None).  W: 2-component (theta, Theta_l) cell array; uf: face
+    velocities; S: 2-component diffusive source or None."""
+    out = np.zeros(cell_shape(g, 2)) if src is None else np.array(src, dtype=np.float64, copy=True)
+    flux = [np.zeros(face_shape(g, d, 2)) for d in range(g.dim)]
+    wh = [np.zeros(face_shape(g, d, 2)) for d in range(g.dim)] if whalf else None
+    W = np.ascontiguousarray(W[:2])
+    rc = lib().mlr_advect_src(C.byref(g), C.byref(abc), dt, coeff, ptr(W),
+                              ptrs([np.ascontiguousarray(u) for u in uf]),
+                              ptr(np.ascontiguousarray(S)) if S is not None else None, ptr(out), ptrs(flux),
+                              ptrs(wh) if wh is not None else None)
+    if rc != 0:
+        raise ValueError("unsupported advection boundary condition")
+    return out, flux, wh
 
 
 def set_threads(n):

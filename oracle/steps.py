@@ -8,8 +8,12 @@ level, in the reference's order (AMRLevelMushyLayer::advance,
 srcSubcycle/AMRLevelMushyLayer.cpp:451-787):
   1. enthalpy variables at t^n; HC face coefficients from the porosity at
      t^n (srcSubcycle/AMRLevelMushyLayerInit.cpp:844-953);
-  2. backward-Euler FAS solve of (H, Theta) (multiCompAdvectDiffuse,
-     :1645-1831), no advective source;
+  2. the advective source of (H, Theta) (computeScalarAdvectiveSrcHC,
+     :2074-2104): (theta, Theta_l)^n traced to the faces with the diffusive
+     source of (H, Theta)^n, advected by the face average of the velocity
+     u^n (the reference's MAC-projected advection velocity is not built);
+     then the backward-Euler FAS solve of (H, Theta) with that source
+     (multiCompAdvectDiffuse, :1645-1831);
   3. enthalpy variables at t^{n+1} (updateEnthalpyVariables, :729);
   4. cell-centred velocity (computeCCvelocity,
      srcSubcycle/AMRLevelMushyLayerVel.cpp:673-917):
@@ -37,6 +41,9 @@ U_OPTS = dict(nu1=2, nu2=2, bottom=8, max_iter=10, eps=1e-10, hang=1e-10, norm_t
 P_OPTS = dict(nu1=2, nu2=4, bottom=4, max_iter=20, eps=1e-15, hang=1e-15, norm_thresh=1e-15)
 # solidPorosity = ccVelPorosityLimit (srcSubcycle/MushyLayerSubcycleUtils.cpp:223-233)
 POROSITY_LIMIT = 0.05
+# hang rule off: a solve stops on eps, norm_thresh or max_iter only
+# (|r| > (1 - hang)*|r_prev| never holds for hang = -inf)
+NO_HANG = float("-inf")
 
 
 def flow_coeffs(Pr, Da, Ra_C, Ra_T=0.0):
@@ -61,7 +68,7 @@ def darcy_ccoef(chi, coeff, law="kozeny_carman"):
 class HCStep:
     """The (H, Theta) backward-Euler step alone."""
 
-    def __init__(self, case, max_iter=100, hang=-1.0):
+    def __init__(self, case, max_iter=100, hang=NO_HANG):
         self.n = list(case["n"])
         per = list(case["periodic"]) + [0] * (3 - len(case["periodic"]))
         self.g = O.make_grid(self.n, case["dx"], per)
@@ -73,13 +80,18 @@ class HCStep:
         self.steps = 0
         self.hc_stats = []
 
+    def hc_src(self, vold, b):
+        """Source of the (H, Theta) step (none without flow)."""
+        return None
+
     def hc_step(self):
         g, p = self.g, self.p
         vold = O.enthalpy_vars(g, p, self.phi)
         chi = np.ascontiguousarray(vold[3:4])
         b = O.hc_bcoef(g, p, chi)
+        src = self.hc_src(vold, b)
         old = self.phi.copy()
-        r = O.hc_be_step(g, p, self.bc, None, b, self.dt, self.phi, old, opts=self.opts)
+        r = O.hc_be_step(g, p, self.bc, None, b, self.dt, self.phi, old, src=src, opts=self.opts)
         self.hc_stats.append((r.exit_status, r.iterations))
         return vold
 
@@ -91,9 +103,11 @@ class HCStep:
 class DarcyBrinkmanStep(HCStep):
     """The Darcy-Brinkman step (module docstring) on the oracle."""
 
-    def __init__(self, case, flow, max_iter=100, hang=-1.0):
+    def __init__(self, case, flow, max_iter=100, hang=NO_HANG, advect=True):
         super().__init__(case, max_iter, hang)
         self.flow = flow
+        self.advect = advect
+        self.abc = O.make_advect_bc(hc_bc=self.bc)
         dim = len(self.n)
         g = self.g
         self.bcu = O.make_bc(flow["bc_u"])
@@ -108,6 +122,17 @@ class DarcyBrinkmanStep(HCStep):
         self.uopts = O.default_opts(**U_OPTS)
         self.popts = O.default_opts(**P_OPTS)
         self.vstats = []
+
+    def hc_src(self, vold, b):
+        """The advective source (step 2): diffusive source of (H, Theta)^n,
+        face velocities of u^n, the Godunov fluxes' divergence."""
+        if not self.advect:
+            return None
+        g = self.g
+        S = O.hc_apply_op(g, self.p, self.bc, 0.0, None, 1.0, b, self.phi.copy())
+        uf = O.face_velocity(g, self.u)
+        src, _, _ = O.advect_src(g, self.abc, self.dt, 1.0, vold, uf, S=S)
+        return src
 
     def step(self):
         g, dt = self.g, self.dt

@@ -139,7 +139,7 @@ def test_darcy_coefficient_laws(ml, gpu_ctx, law):
 
 def _db_compare(ml, ctx, case, flow, steps, max_iter=100):
     stp = importlib.import_module("mushy-layer_amd.steps")
-    opts = ml.default_opts(max_iter=max_iter, hang=-1.0)
+    opts = ml.default_opts(max_iter=max_iter, hang=S.NO_HANG)
     gpu = stp.DarcyBrinkmanStepper(ml, ctx, case, opts, flow)
     cpu = S.DarcyBrinkmanStep(case, flow, max_iter=max_iter)
     n = case["n"]
