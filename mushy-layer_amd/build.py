@@ -1,10 +1,14 @@
-"""Build libmlhip.so (HIP kernels for gfx950 + C ABI + host C++ drivers) in-tree.
+"""Build libmlhip.so (HIP kernels for gfx950 + C ABI + host C++ drivers) and
+libmlio.so (Chombo HDF5 checkpoint/plot files, host only) in-tree.
 
     python mushy-layer_amd/build.py
 
-Every source is compiled with hipcc for gfx950 only, with floating-point
+Every HIP source is compiled with hipcc for gfx950 only, with floating-point
 contraction off (so the FP64 kernels evaluate the reference's expressions as
-written) and packed-FP32 instruction generation disabled.
+written) and packed-FP32 instruction generation disabled.  libmlio.so is
+plain C++ (g++) against the HDF5 C library of HDF5_DIR (default /opt/conda,
+HDF5 1.10); without one it is not built and the package's file functions
+raise.
 """
 import concurrent.futures as cf
 import os
@@ -55,6 +59,33 @@ def build(verbose=False):
     return LIB
 
 
+IO_LIB = os.path.join(HERE, "libmlio.so")
+HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
+
+
+def build_io(verbose=False):
+    """libmlio.so from io/ml_chombo_h5.cpp; returns its path, or None when no
+    HDF5 library is found."""
+    inc = os.path.join(HDF5_DIR, "include")
+    libdir = os.path.join(HDF5_DIR, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(os.path.join(libdir, "libhdf5.so"))):
+        if verbose:
+            print("libmlio.so not built: no HDF5 under %s" % HDF5_DIR)
+        return None
+    tmp = IO_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
+           "-I", inc, os.path.join(HERE, "io", "ml_chombo_h5.cpp"), "-o", tmp, "-L", libdir, "-lhdf5",
+           "-Wl,-rpath," + libdir]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("libmlio build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(tmp, IO_LIB)
+    if verbose:
+        print("built", IO_LIB)
+    return IO_LIB
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_io(verbose=True)
     sys.exit(0)

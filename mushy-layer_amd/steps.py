@@ -96,6 +96,42 @@ class HCStepper:
     def step(self):
         self.hc_step()
 
+    # ---- checkpoint / restart (the reference's Chombo HDF5 layout,
+    # srcSubcycle/AMRLevelMushyLayerIO.cpp:131-420)
+    def checkpoint_fields(self):
+        """Scalar and vector fields of a checkpoint, host copies in the
+        package's layout: Enthalpy and Bulk concentration."""
+        phi = self.phi.get()
+        return {"Enthalpy": phi[0:1], "Bulk concentration": phi[1:2]}, {}, {}
+
+    def save_checkpoint(self, path, iteration, time):
+        """A single-level checkpoint of the current state (writeCheckpointHeader
+        / writeCheckpointLevel's layout; needs libmlio.so)."""
+        from . import chombo_io
+        self.ctx.sync()
+        scalars, vectors, faces = self.checkpoint_fields()
+        c = self.case
+        chombo_io.write_checkpoint(path, self.n, c["dx"], self.dt, time, iteration, c["periodic"], scalars, vectors,
+                                   faces)
+
+    def load_checkpoint(self, path):
+        """Restore the state from a checkpoint of the same grid
+        (readCheckpointLevel); returns the file's (iteration, time)."""
+        from . import chombo_io
+        lev = chombo_io.read_level(path, 0)
+        h = lev["header"]
+        if [h["hi"][d] - h["lo"][d] + 1 for d in range(len(self.n))] != list(self.n):
+            raise ValueError("checkpoint grid %s differs from %s" % (h, self.n))
+        phi = np.concatenate([chombo_io.domain_array(lev, "Enthalpy"),
+                              chombo_io.domain_array(lev, "Bulk concentration")], axis=0)
+        self.phi.set(np.ascontiguousarray(phi))
+        self.restore_extra(lev)
+        self.ctx.sync()
+        return lev["root"]["iteration"], lev["root"]["time"]
+
+    def restore_extra(self, lev):
+        pass
+
     def porosity(self):
         """Host copy of the porosity of the current state (valid cells)."""
         self.op.enthalpy_vars(self.phi, self.vnew)
@@ -186,6 +222,28 @@ class DarcyBrinkmanStepper(HCStepper):
         vs.append((st.exit_status, st.iterations))
         uop.cc_correct(self.fp, dt, self.vold, self.vnew, self.pi, self.u)
         self.vstats.append(vs)
+
+    def checkpoint_fields(self):
+        """The reference's checkpoint variables of a Darcy-Brinkman state
+        (Enthalpy, Bulk concentration, Pressure, Darcy velocity, advVel), and
+        U*, the initial guess of the next U* solves (an extra variable, so a
+        restart continues bit for bit)."""
+        scalars, _, _ = super().checkpoint_fields()
+        scalars["Pressure"] = self.pi.get()
+        vectors = {"Darcy velocity": np.concatenate([u.get() for u in self.u], axis=0),
+                   "U*": np.concatenate([u.get() for u in self.ustar], axis=0)}
+        self.uop.face_velocity(self.u, self.uf)
+        faces = {"advVel": [f.get() for f in self.uf]}
+        return scalars, vectors, faces
+
+    def restore_extra(self, lev):
+        from . import chombo_io
+        self.pi.set(np.ascontiguousarray(chombo_io.domain_array(lev, "Pressure")))
+        vel = chombo_io.domain_array(lev, "Darcy velocity")
+        ust = chombo_io.domain_array(lev, "U*") if "U*" in lev["fabs"] else np.zeros_like(vel)
+        for d in range(len(self.n)):
+            self.u[d].set(np.ascontiguousarray(vel[d:d + 1]))
+            self.ustar[d].set(np.ascontiguousarray(ust[d:d + 1]))
 
     def flow_fields(self):
         """Host copies of the velocity components and the pressure (valid cells)."""

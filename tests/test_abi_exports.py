@@ -37,3 +37,18 @@ def test_layout_helpers_without_gpu(ml):
     o = ml.default_opts()
     assert (o.nu1, o.nu2, o.bottom, o.max_iter) == (1, 4, 40, 10)
     assert o.eps == o.hang == o.norm_thresh == 1e-10
+
+
+def test_io_library_exports_every_declared_symbol():
+    import importlib
+    b = importlib.import_module("mushy-layer_amd.build")
+    cio = importlib.import_module("mushy-layer_amd.chombo_io")
+    if not os.path.exists(cio.IO_LIB) and b.build_io() is None:
+        import pytest
+        pytest.skip("no HDF5 library to build libmlio.so")
+    src = open(os.path.join(ROOT, "include", "mlio.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(ml_h5_[a-z0-9_]+)\s*\(", src)))
+    lib = cio.lib()
+    assert [n for n in names if not hasattr(lib, n)] == []
+    assert sorted(cio._SIG) == names
