@@ -142,10 +142,12 @@ def smoother_bandwidth(ml, wl, ctx, n, launches=20):
     op.set_coefs(None, bu, c, 1.0, -db["dt"])
     u = ml.DeviceArray(ctx, shp1, host=db["u0"])
     r = ml.DeviceArray(ctx, shp1, host=db["src"])
-    # levels above 512^2 cells run the wide short-halo tiles (2 sweeps per
-    # launch at most), smaller ones the 16x16 deep-halo tiles
+    # levels above 512^2 cells relax by streaming along y (one red+black
+    # sweep per launch, the whole grid resident), smaller ones in 16x16
+    # deep-halo tiles
     wide = n * n > 512 * 512
-    tile, hmax = ("64,8,1,2,256", 2) if wide else ("16,16,1,9,1024", 9)
+    kname, hmax = (("vc_relax_stream_kernel<double,2,252,1,2,256>", 1) if wide else
+                   ("vc_relax_tile_kernel<double,2,16,16,1,9,1024>", 4))
     for sweeps in (1, 4):
         op.relax(0, u, r, sweeps)
         ctx.timing_start(ml.KT_VC_RELAX, launches)
@@ -154,8 +156,8 @@ def smoother_bandwidth(ml, wl, ctx, n, launches=20):
         n1, ms, nb, _ = ctx.timing_stop()
         gbs = nb / (ms * 1e-3) / 1e9
         out["vc_relax_%d_sweep" % sweeps] = {
-            "kernel": "vc_relax_tile_kernel<double,2,%s> (%d red+black sweep%s per launch)"
-                      % (tile, min(sweeps, hmax), "" if min(sweeps, hmax) == 1 else "s"),
+            "kernel": "%s (%d red+black sweep%s per launch)"
+                      % (kname, min(sweeps, hmax), "" if min(sweeps, hmax) == 1 else "s"),
             "avg_launch_us": round(ms / n1 * 1e3, 2), "bytes_per_launch": nb / n1,
             "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     op.close()
@@ -507,8 +509,7 @@ def run_adjoint(ml, wl, ctx, args):
     F_true = wl.flux_profile(nx, F0=12.0, modes=((1, 0.3, 0.0), (2, 0.1, 1.0)))
     St_true = 5.5
     tphi, tchi, cps = model.forward(F_true, St_true)
-    for x in cps.values():
-        x.free()
+    cps.close()
 
     def loss_only(F, St):
         phi, chi, c = model.forward(F, St)

@@ -15,10 +15,39 @@ sequence is porosity -> bCoef -> set_coefs(1, -dt) -> heat-flux source ->
 solve_be (multiCompAdvectDiffuse with the coefficients at t^n,
 srcSubcycle/AMRLevelMushyLayer.cpp:1645-1831).
 """
+import weakref
+
 import numpy as np
 import torch
 
 from . import (DeviceArray, HCOp, default_opts, face_shape, make_bc, make_params, valid)
+
+
+def _free_arrays(arrays):
+    for a in arrays:
+        try:
+            a.free()
+        except Exception:   # the context may already be closed at interpreter exit
+            pass
+
+
+class Checkpoints:
+    """Device checkpoints of one forward pass (states at steps 0, k, 2k, ...).
+    They live as long as this object -- the autograd graph that holds it --
+    so backward may run more than once (retain_graph=True); the memory is
+    freed when the graph is released, or by close()."""
+
+    def __init__(self, states):
+        self.states = states
+        self._fin = weakref.finalize(self, _free_arrays, list(states.values()))
+
+    def close(self):
+        self._fin()
+        self.states = {}
+
+    def check(self):
+        if not self.states:
+            raise RuntimeError("the forward checkpoints were released (Checkpoints.close)")
 
 
 class HeatFluxAdjoint:
@@ -76,26 +105,46 @@ class HeatFluxAdjoint:
         self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, y.ptr, x.ptr, x.nbytes), "copy")
         return y
 
-    def forward(self, F, St):
-        """Final state (host array) and its porosity; device checkpoints of
-        the states at steps 0, k, 2k, ..."""
+    def set_flux(self, F):
+        """F: host values, or (device address, nbytes) of a float64 device buffer."""
+        if isinstance(F, tuple):
+            self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, self.flux.ptr, F[0], F[1]), "copy")
+        else:
+            self.flux.set(np.asarray(F, dtype=np.float64))
+
+    def forward_device(self, F, St):
+        """Final state and its porosity as device arrays, and the device
+        checkpoints (a Checkpoints holder)."""
         op = self.op(St)
-        self.flux.set(np.asarray(F, dtype=np.float64))
+        self.set_flux(F)
         phi = self._copy(self.phi0)
         cps = {0: self._copy(phi)}
         for n in range(1, self.steps + 1):
             self.advance(op, phi)
             if n % self.checkpoint == 0 and n < self.steps:
                 cps[n] = self._copy(phi)
-        op.porosity(phi, self.chi)
-        out = phi.get(), self.chi.get()
+        chi = DeviceArray(self.ctx, (1,) + self.shape[1:])
+        op.porosity(phi, chi)
+        return phi, chi, Checkpoints(cps)
+
+    def forward(self, F, St):
+        """Final state (host array) and its porosity; device checkpoints of
+        the states at steps 0, k, 2k, ..."""
+        phi, chi, cps = self.forward_device(F, St)
+        out = phi.get(), chi.get()
         phi.free()
+        chi.free()
         return out[0], out[1], cps
 
-    def backward(self, F, St, cps, g_phi, g_chi):
-        """dL/dF and dL/dSt from dL/dphi_N and dL/dchi_N (host arrays)."""
+    def backward(self, F, St, cps, g_phi, g_chi, device_out=False):
+        """dL/dF and dL/dSt from dL/dphi_N and dL/dchi_N: host arrays, or
+        device addresses of FAB-shaped float64 buffers whose ghost cells are
+        zero.  dL/dF is a host array, or a DeviceArray with device_out."""
         op = self.op(St)
-        self.flux.set(np.asarray(F, dtype=np.float64))
+        if isinstance(cps, Checkpoints):
+            cps.check()
+            cps = cps.states
+        self.set_flux(F)
         gF = DeviceArray(self.ctx, (self.n[0],), host=np.zeros(self.n[0]))
         g = DeviceArray(self.ctx, self.shape, host=np.zeros(self.shape))
         gold = DeviceArray(self.ctx, self.shape, host=np.zeros(self.shape))
@@ -110,12 +159,17 @@ class HeatFluxAdjoint:
                 self.advance(op, x)
                 states.append(x)
             if s1 == self.steps:
-                gp = np.zeros(self.shape)
-                valid(self.n, gp)[:] = valid(self.n, np.asarray(g_phi, dtype=np.float64))
-                g.set(gp)
-                gc = np.zeros((1,) + self.shape[1:])
-                valid(self.n, gc)[:] = valid(self.n, np.asarray(g_chi, dtype=np.float64))
-                gcd = DeviceArray(self.ctx, gc.shape, host=gc)
+                gcd = DeviceArray(self.ctx, (1,) + self.shape[1:])
+                if isinstance(g_phi, int):
+                    self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, g.ptr, g_phi, g.nbytes), "copy")
+                    self.ctx.check(self.ctx.lib.ml_buf_copy(self.ctx.h, gcd.ptr, g_chi, gcd.nbytes), "copy")
+                else:
+                    gp = np.zeros(self.shape)
+                    valid(self.n, gp)[:] = valid(self.n, np.asarray(g_phi, dtype=np.float64))
+                    g.set(gp)
+                    gc = np.zeros((1,) + self.shape[1:])
+                    valid(self.n, gc)[:] = valid(self.n, np.asarray(g_chi, dtype=np.float64))
+                    gcd.set(gc)
                 gst += op.porosity_vjp(states[-1], gcd, g)
                 gcd.free()
             for k in range(len(states) - 1, 0, -1):
@@ -128,9 +182,12 @@ class HeatFluxAdjoint:
                 g, gold = gold, g
             for x in states:
                 x.free()
-        out = gF.get()
-        for a in (gF, g, gold):
+        for a in (g, gold):
             a.free()
+        if device_out:
+            return gF, gst
+        out = gF.get()
+        gF.free()
         return out, gst
 
     def close(self):
@@ -141,7 +198,19 @@ class HeatFluxAdjoint:
 
 class ImplicitSteps(torch.autograd.Function):
     """(flux[nx], permeability prefactor, St) -> (final state, its porosity),
-    both FAB-shaped float64 tensors on the flux's device."""
+    both FAB-shaped float64 tensors on the flux's device.
+
+    The states, checkpoints and adjoint iterates stay in the library's device
+    memory; only the flux profile, the final state, its porosity and the
+    gradients cross this boundary, as host arrays.  (torch's wheel carries
+    its own HIP/HSA runtime, distinct from the system ROCm runtime the
+    library links, and two HSA runtimes cannot both open the GPU in one
+    process, so torch device tensors are not exchanged.)  The gradient with
+    respect to the permeability prefactor is structurally zero: without flow
+    the (H, Theta) step does not depend on the permeability (the HC face
+    coefficients use the porosity only), so the returned zeros are exact,
+    not an approximation.  The forward checkpoints live with the graph
+    (Checkpoints), so backward may be repeated (retain_graph=True)."""
 
     @staticmethod
     def forward(ctx, flux, perm, stefan, model):
@@ -155,12 +224,11 @@ class ImplicitSteps(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g_phi, g_chi):
-        gp = g_phi.detach().to("cpu", torch.float64).numpy() if g_phi is not None else np.zeros(ctx.model.shape)
+        model = ctx.model
+        gp = g_phi.detach().to("cpu", torch.float64).numpy() if g_phi is not None else np.zeros(model.shape)
         gc = (g_chi.detach().to("cpu", torch.float64).numpy() if g_chi is not None
-              else np.zeros((1,) + ctx.model.shape[1:]))
-        gF, gSt = ctx.model.backward(ctx.F, ctx.St, ctx.cps, gp, gc)
-        for x in ctx.cps.values():
-            x.free()
+              else np.zeros((1,) + model.shape[1:]))
+        gF, gSt = model.backward(ctx.F, ctx.St, ctx.cps, gp, gc)
         dev = g_phi.device if g_phi is not None else g_chi.device
         return (torch.from_numpy(gF).to(dev), torch.zeros(ctx.perm_shape, dtype=torch.float64, device=dev),
                 torch.tensor(gSt, dtype=torch.float64, device=dev), None)

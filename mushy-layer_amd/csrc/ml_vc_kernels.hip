@@ -718,6 +718,407 @@ hipError_t vc_relax_cfg(const VCArgs<T>& A, T* const buf[2], int* cur, const T* 
   return hipSuccess;
 }
 
+// ---- streaming relaxation for large levels -------------------------------
+// A workgroup owns a block of columns (2D: TX cells in x; 3D: TX x TY cells
+// in (x, y)) and a segment of planes along the last direction, and marches
+// through them with a window of P+2 planes in LDS.  With plane R just
+// loaded, colour pass p runs on plane R-1-p (p = 0..P-1), so plane R-P is
+// final and is written: the passes form a pipeline along the march instead
+// of a halo in that direction.  Pass p covers the lateral positions within
+// P-1-p of the block and the planes within P-1-p of the segment, so the
+// core ends exact after P passes, as in the tiled kernel.  Every input value
+// of the extended block is read once per launch; a plane's coefficients
+// stay in registers for the P steps that use them and are loaded one plane
+// ahead of their first use, so their loads overlap the previous plane's
+// passes.  Stencil, lambda and update are the tiled kernel's, so results
+// are identical bit for bit.
+template <int N>
+struct IC {
+  static constexpr int value = N;
+};
+
+template <typename T, int DIM>
+struct SCoef {
+  Faces1<T, DIM> F;
+  T aq, cq, rr, lam;
+};
+
+// lateral position of the extended block: global coordinates (wrapped in
+// periodic directions), kind 1 cell / 2 wall ghost / 0 unused, Chebyshev
+// distance to the core, owner flag, and for a ghost its wall (direction,
+// side) and the LDS offset of its interior neighbour
+struct SPos {
+  int gx, gy;
+  int kind, dist, owner;
+  int gd, gs, eint;
+  int gtype;   // a ghost's boundary type (resolved once: a per-thread index
+               // into the kernel arguments would be a memory load)
+};
+
+template <int DIM, int TX, int TY, int P>
+__device__ inline void spos_init(const Geom& g, int x0, int y0, int e, SPos& s) {
+  constexpr int EX = TX + 2 * P;
+  const int ex = e % EX, ey = e / EX;
+  const int lc[2] = {ex - P, DIM == 3 ? ey - P : 0};
+  const int org[2] = {x0, y0};
+  const int nl = DIM == 3 ? 2 : 1;
+  s.kind = 1;
+  s.dist = 0;
+  s.owner = 1;
+  s.gd = -1;
+  s.gs = 0;
+  s.eint = e;
+  int gv[2] = {0, 0};
+  int outside = 0;
+  for (int d = 0; d < nl; ++d) {
+    const int ext = d == 0 ? TX : TY;
+    const int l = lc[d];
+    const int dd = l < 0 ? -l : (l >= ext ? l - ext + 1 : 0);
+    if (dd > s.dist) s.dist = dd;
+    int v = org[d] + l;
+    const int n = g.n[d];
+    if (v >= n || v < 0) s.owner = 0;
+    if (v >= 0 && v < n) {
+      gv[d] = v;
+    } else if (g.periodic[d]) {
+      gv[d] = v < 0 ? v + n : v - n;
+    } else if (v == -1 || v == n) {
+      ++outside;
+      s.gd = d;
+      s.gs = v == n ? 1 : 0;
+      gv[d] = v == n ? n - 1 : 0;   // the interior neighbour
+      s.eint = e + (v == n ? -1 : 1) * (d == 0 ? 1 : EX);
+    } else {
+      outside += 2;
+    }
+  }
+  if (s.dist > 0) s.owner = 0;
+  if (outside == 1) s.kind = 2;
+  else if (outside > 1) s.kind = 0;
+  // every position gets in-domain coordinates (unused ones clamped), so the
+  // prefetch can load unconditionally
+  for (int d = 0; d < nl; ++d) gv[d] = gv[d] < 0 ? 0 : (gv[d] >= g.n[d] ? g.n[d] - 1 : gv[d]);
+  s.gx = gv[0];
+  s.gy = gv[1];
+}
+
+template <typename T, int DIM, int TX, int TY, int P, int NT>
+__global__ __launch_bounds__(NT) void vc_relax_stream_kernel(VCArgs<T> A, const T* __restrict__ in,
+                                                             T* __restrict__ out, const T* __restrict__ rhs,
+                                                             int first, int zero_init, Geom gc,
+                                                             const T* __restrict__ pc, int nbx, int nby, int seg) {
+  if (A.skip && *A.skip) return;
+  constexpr int EX = TX + 2 * P;
+  constexpr int EL = EX * (DIM == 3 ? TY + 2 * P : 1);
+  constexpr int K = (EL + NT - 1) / NT;
+  static_assert(P >= 1 && P <= 2, "one or two colour passes per launch");
+  constexpr int W = 4;   // planes in the window (P+2 <= 4; a power of two)
+  constexpr int MD = DIM - 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* su = reinterpret_cast<T*>(smem_raw);
+  const Geom& g = A.g;
+  const int nz = g.n[MD];
+  const bool zper = g.periodic[MD] != 0;
+  const T dxinv = T(1) / (T(g.dx) * T(g.dx));
+  const bool hasc = A.c != nullptr;
+  int b = blockIdx.x;
+  const int bx = b % nbx;
+  b /= nbx;
+  const int by = DIM == 3 ? b % nby : 0;
+  const int bz = DIM == 3 ? b / nby : b;
+  const int x0 = bx * TX, y0 = by * TY;
+  const int z0 = bz * seg;
+  const int z1 = z0 + seg < nz ? z0 + seg : nz;
+  SPos S[K];
+#pragma unroll
+  for (int kk = 0; kk < K; ++kk) {
+    const int e = threadIdx.x + kk * NT;
+    if (e < EL) {
+      spos_init<DIM, TX, TY, P>(g, x0, y0, e, S[kk]);
+      S[kk].gtype = 0;
+      for (int d = 0; d < DIM - 1; ++d)
+        for (int sd = 0; sd < 2; ++sd)
+          if (S[kk].kind == 2 && S[kk].gd == d && S[kk].gs == sd) S[kk].gtype = A.bc.type[d][sd][0];
+    } else {
+      S[kk].kind = 0;
+      S[kk].dist = P;
+      S[kk].owner = 0;
+    }
+  }
+  auto zwrap = [&](int z) { return z < 0 ? z + nz : (z >= nz ? z - nz : z); };
+  auto real_plane = [&](int z) { return zper || (z >= 0 && z < nz); };
+  // 32-bit element offsets: a position's lateral part, computed once, plus a
+  // plane part common to the workgroup (levels below 2^29 cells per array)
+  const unsigned csz = unsigned(DIM == 3 ? g.sz : g.sy);   // cell stride between planes
+  unsigned fpl[DIM], fup[DIM];                              // face plane stride, offset to the upper face
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) fpl[d] = unsigned(g.fn[d][0] * (DIM == 3 ? g.fn[d][1] : 1));
+  fup[0] = 1;
+  if (DIM == 3) fup[1] = unsigned(g.fn[1][0]);
+  fup[MD] = fpl[MD];
+  const unsigned cgsz = unsigned(DIM == 3 ? gc.sz : gc.sy);
+  unsigned cl[K], fl[K][DIM], ccl[K];
+#pragma unroll
+  for (int kk = 0; kk < K; ++kk) {
+    const SPos& s = S[kk];
+    cl[kk] = unsigned(s.gx + 1) + (DIM == 3 ? unsigned(g.sy) * unsigned(s.gy + 1) : 0u);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      fl[kk][d] = unsigned(s.gx) + (DIM == 3 ? unsigned(g.fn[d][0]) * unsigned(s.gy) : 0u);
+    ccl[kk] = unsigned(s.gx / 2 + 1) + (DIM == 3 ? unsigned(gc.sy) * unsigned(s.gy / 2 + 1) : 0u);
+  }
+  auto ld = [](const T* p, unsigned i) -> T {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + (i * unsigned(sizeof(T))));
+  };
+  // register sets of P+3 planes (R-P .. R+2 at step R), addressed by
+  // compile-time indices: plane zl+m lives in set m % NS.  Loads go straight
+  // into their set and are used only at that plane's commit two steps later,
+  // so no instruction touches a load in flight.
+  constexpr int NS = P + 3;
+  SCoef<T, DIM> cs[NS][K];
+  T sv[NS][K], scv[NS][K];   // state and coarse correction as loaded
+  // the same loads for every position and plane (in-domain addresses; an
+  // absent array is replaced by a duplicate read of rhs), so that the waits
+  // for a set's loads can count the loads issued after it exactly
+  const T* vin = zero_init ? rhs : in;
+  const T* ain = A.a ? A.a : rhs;
+  const T* cin = hasc ? A.c : rhs;
+  const T* cvp = pc ? pc : rhs;
+  auto prefetch = [&](int R, SCoef<T, DIM>(&pf)[K], T(&v)[K], T(&cv)[K]) {
+    const int zw = zper ? zwrap(R) : (R < 0 ? 0 : (R >= nz ? nz - 1 : R));
+    const unsigned qz = unsigned(zw + 1) * csz, cz = unsigned(zw / 2 + 1) * cgsz;
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) {
+      const unsigned q = cl[kk] + qz;   // a ghost reads its interior neighbour
+      v[kk] = ld(vin, q);
+      cv[kk] = ld(cvp, pc ? ccl[kk] + cz : q);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const unsigned f = fl[kk][d] + unsigned(zw) * fpl[d];
+        pf[kk].F.v[2 * d] = ld(A.b[d], f);
+        pf[kk].F.v[2 * d + 1] = ld(A.b[d], f + fup[d]);
+      }
+      pf[kk].aq = ld(ain, q);
+      pf[kk].cq = ld(cin, q);
+      pf[kk].rr = ld(rhs, q);
+    }
+  };
+  const int zl = (!zper && z0 - P < 0) ? 0 : z0 - P;
+  const int zend = z1 - 1 + P;   // last step: plane z1 - 1 is final
+  // window slot of plane R (the bottom wall's ghost plane -1 is slot 3)
+  auto slot = [&](int R) { return su + ((R - zl) & (W - 1)) * EL; };
+  auto step = [&](auto J, int R) {
+    constexpr int j = decltype(J)::value;
+    // A. commit plane R (or the top wall's ghost plane)
+    T* pl = slot(R);
+    if (real_plane(R)) {
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) {
+        const SPos& s = S[kk];
+        const int e = threadIdx.x + kk * NT;
+        if (s.kind == 0) continue;
+        T val = zero_init ? T(0) : sv[j][kk];
+        if (pc) val = val + scv[j][kk];
+        if (s.kind == 1) {
+          pl[e] = val;
+          if (!zper && R == 0) slot(-1)[e] = bc_ghost<T>(A.bc.type[MD][0][0], T(0), val, -1, T(g.dx));
+          if (!A.a) cs[j][kk].aq = T(1);
+          if (!hasc) cs[j][kk].cq = T(0);
+          if (s.dist <= P - 1)
+            cs[j][kk].lam = vc_lambda<T, DIM>(cs[j][kk].F, cs[j][kk].aq, hasc, cs[j][kk].cq, A.alpha, A.beta, dxinv);
+        } else {
+          pl[e] = bc_ghost<T>(s.gtype, T(0), val, s.gs == 0 ? -1 : 1, T(g.dx));
+        }
+      }
+    } else if (!zper && R == nz) {
+      const T* below = slot(nz - 1);
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) {
+        const int e = threadIdx.x + kk * NT;
+        if (S[kk].kind == 1) pl[e] = bc_ghost<T>(A.bc.type[MD][1][0], T(0), below[e], 1, T(g.dx));
+      }
+    }
+    // B. loads of plane R+2 into the set of plane R+2-NS, no longer in use
+    constexpr int jn = (j + 2) % NS;
+    prefetch(R + 2, cs[jn], sv[jn], scv[jn]);   // beyond the segment: a clamped plane, unused
+    __syncthreads();
+    // C. the colour passes; pass p on plane R-1-p uses set (j-1-p) mod NS
+    auto pass = [&](auto PP) {
+      constexpr int p = decltype(PP)::value;
+      constexpr int jp = (j - 1 - p + 2 * NS) % NS;
+      const int z = R - 1 - p;
+      const int reach = P - 1 - p;
+      const bool run = z >= z0 - reach && z < z1 + reach && real_plane(z);
+      if (run) {
+        const int zw = zwrap(z);
+        const int colour = (first + p) & 1;
+        T* pz = slot(z);
+        const T* pm = slot(z - 1);
+        const T* pp = slot(z + 1);
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+          const SPos& s = S[kk];
+          if (s.kind != 1 || s.dist > reach || ((s.gx + s.gy + zw + colour) & 1)) continue;
+          const int e = threadIdx.x + kk * NT;
+          T st[7];
+          st[0] = pz[e];
+          st[1] = pz[e + 1];
+          st[2] = pz[e - 1];
+          if (DIM == 3) {
+            st[3] = pz[e + EX];
+            st[4] = pz[e - EX];
+            st[5] = pp[e];
+            st[6] = pm[e];
+          } else {
+            st[3] = pp[e];
+            st[4] = pm[e];
+          }
+          const SCoef<T, DIM>& C = cs[jp][kk];
+          const T lof = vc_lof<T, DIM>(C.F, st, C.aq, hasc, C.cq, A.alpha, A.beta, dxinv);
+          const T nv = st[0] - C.lam * (lof - C.rr);
+          pz[e] = nv;
+          // wall ghosts of this cell: lateral, then along the march
+          const int gl[2] = {s.gx, s.gy};
+#pragma unroll
+          for (int d = 0; d < (DIM == 3 ? 2 : 1); ++d) {
+            if (g.periodic[d]) continue;
+            const int st1 = d == 0 ? 1 : EX;
+            if (gl[d] == 0) pz[e - st1] = bc_ghost<T>(A.bc.type[d][0][0], T(0), nv, -1, T(g.dx));
+            if (gl[d] == g.n[d] - 1) pz[e + st1] = bc_ghost<T>(A.bc.type[d][1][0], T(0), nv, 1, T(g.dx));
+          }
+          if (!zper) {
+            if (zw == 0) slot(-1)[e] = bc_ghost<T>(A.bc.type[MD][0][0], T(0), nv, -1, T(g.dx));
+            if (zw == nz - 1) slot(nz)[e] = bc_ghost<T>(A.bc.type[MD][1][0], T(0), nv, 1, T(g.dx));
+          }
+        }
+      }
+      __syncthreads();
+    };
+    pass(IC<0>{});
+    if constexpr (P > 1) pass(IC<1>{});
+    // D. plane R-P is final
+    const int zs = R - P;
+    if (zs >= z0 && zs < z1) {
+      const T* ps = slot(zs);
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) {
+        const SPos& s = S[kk];
+        if (s.owner)
+          *reinterpret_cast<T*>(reinterpret_cast<char*>(out) + ((cl[kk] + unsigned(zs + 1) * csz) * unsigned(sizeof(T)))) =
+              ps[threadIdx.x + kk * NT];
+      }
+    }
+  };
+  prefetch(zl, cs[0], sv[0], scv[0]);
+  prefetch(zl + 1, cs[1], sv[1], scv[1]);
+  for (int R = zl; R <= zend; R += NS) {
+    step(IC<0>{}, R);
+    if (R + 1 <= zend) step(IC<1>{}, R + 1);
+    if (R + 2 <= zend) step(IC<2>{}, R + 2);
+    if constexpr (NS > 3) {
+      if (R + 3 <= zend) step(IC<3 % NS>{}, R + 3);
+    }
+    if constexpr (NS > 4) {
+      if (R + 4 <= zend) step(IC<4 % NS>{}, R + 4);
+    }
+  }
+}
+
+template <int DIM>
+struct VCStream;
+template <>
+struct VCStream<2> {
+  static constexpr int NT = 256, TY = 1, SEGMIN = 8;
+};
+template <>
+struct VCStream<3> {
+  static constexpr int NT = 256, TY = 12, SEGMIN = 8;
+};
+
+// levels on which plain relaxation launches stream (2D above 512^2, 3D
+// above 96^3); the tiled kernel keeps the fused restriction / residual
+constexpr long long kStreamCells2 = 512 * 512;
+constexpr long long kStreamCells3 = 96LL * 96 * 96;
+inline bool vc_streams(const Geom& g) {
+  long long nf = 0;
+  for (int d = 0; d < g.dim; ++d) nf = nf > g.nface[d] ? nf : g.nface[d];
+  const long long most = g.ncell > nf ? g.ncell : nf;
+  return g.nvalid() > (g.dim == 2 ? kStreamCells2 : kStreamCells3) && g.n[g.dim - 1] >= 8 &&
+         most < (1LL << 29);   // byte offsets of every array within 32 bits
+}
+
+template <typename T, int DIM, int P>
+hipError_t vc_stream_launch(const VCArgs<T>& A, const T* in, T* out, const T* rhs, int first, bool zero_init,
+                            const Geom& gc, const T* pc, hipStream_t s, const LaunchTimer* tm) {
+  using D = VCStream<DIM>;
+  constexpr int TX = DIM == 2 ? D::NT - 2 * P : D::TY;
+  constexpr int TY = DIM == 2 ? 1 : D::TY;
+  constexpr int EL = (TX + 2 * P) * (DIM == 3 ? TY + 2 * P : 1);
+  const Geom& g = A.g;
+  const int nbx = (g.n[0] + TX - 1) / TX;
+  const int nby = DIM == 3 ? (g.n[1] + TY - 1) / TY : 1;
+  const size_t lds = size_t(4) * EL * sizeof(T);
+  auto kern = vc_relax_stream_kernel<T, DIM, TX, TY, P, D::NT>;
+  // segments sized so that the whole grid is resident at once (one round of
+  // workgroups; measured best at 2048^2: a grid that fills every CU's
+  // workgroup slots beats both fewer, longer segments and a second round)
+  static int resident = 0;
+  if (!resident) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
+                                                 hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(kern), D::NT, lds) ==
+            hipSuccess &&
+        cus > 0 && per > 0)
+      resident = cus * per;
+    else
+      resident = 768;
+  }
+  const int nz = g.n[DIM - 1];
+  const int columns = nbx * nby;
+  const int per_column = resident / columns > 1 ? resident / columns : 1;
+  int seg = (nz + per_column - 1) / per_column;
+  if (seg < D::SEGMIN) seg = D::SEGMIN;
+  const int nbz = (nz + seg - 1) / seg;
+  const bool timed = tm && *tm->n < tm->cap;
+  if (tm && !timed) ++*tm->missed;
+  hipExtLaunchKernelGGL(kern, dim3(nbx * nby * nbz), dim3(D::NT), lds, s, timed ? tm->start[*tm->n] : nullptr,
+                        timed ? tm->stop[*tm->n] : nullptr, 0, A, in, out, rhs, first, zero_init ? 1 : 0, gc, pc, nbx,
+                        nby, seg);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (timed) {
+    const double cells = double(g.nvalid());
+    double faces = 0;
+    for (int d = 0; d < DIM; ++d) faces += double(g.nface[d]);
+    double by = cells * (zero_init ? 0 : 1) + (pc ? cells / double(DIM == 3 ? 8 : 4) : 0) + cells +
+                (A.a ? cells : 0) + (A.c ? cells : 0) + faces + cells;
+    *tm->bytes += by * sizeof(T);
+    ++*tm->n;
+  }
+  return hipSuccess;
+}
+
+// npass colour passes by streaming launches of up to 2 passes each
+template <typename T, int DIM>
+hipError_t vc_relax_stream(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
+                           const Geom* gc, const T* pc, hipStream_t s, const LaunchTimer* tm) {
+  const Geom gcc = gc ? *gc : A.g;
+  int done = 0;
+  while (done < npass) {
+    const int p = (npass - done) >= 2 ? 2 : (npass - done);
+    const bool zi = done == 0 && zero_init;
+    const T* pcl = done == 0 ? pc : nullptr;
+    hipError_t e = p == 2 ? vc_stream_launch<T, DIM, 2>(A, buf[*cur], buf[1 - *cur], rhs, done & 1, zi, gcc, pcl, s, tm)
+                          : vc_stream_launch<T, DIM, 1>(A, buf[*cur], buf[1 - *cur], rhs, done & 1, zi, gcc, pcl, s,
+                                                        tm);
+    if (e != hipSuccess) return e;
+    *cur = 1 - *cur;
+    done += p;
+  }
+  return hipSuccess;
+}
+
 // tiles of the 2D relaxation on large levels (short halo, wide rows;
 // measured best of 6 shapes at 2048^2)
 struct VCRelaxWide2 {
@@ -736,6 +1137,8 @@ template <typename T, int DIM>
 hipError_t vc_relax_dim(const VCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, bool zero_init,
                         const Geom* gc, const T* pc, hipStream_t s, const Geom* gr = nullptr, T* resc = nullptr,
                         const LaunchTimer* tm = nullptr, const VCFinal<T>* fin = nullptr) {
+  if (!resc && !fin && npass > 0 && vc_streams(A.g))
+    return vc_relax_stream<T, DIM>(A, buf, cur, rhs, npass, zero_init, gc, pc, s, tm);
   if constexpr (DIM == 2) {
     if (vc_wide(A.g))
       return vc_relax_cfg<T, 2, VCRelaxWide2>(A, buf, cur, rhs, npass, zero_init, gc, pc, s, gr, resc, tm, fin);

@@ -29,8 +29,7 @@ def _setup(ml, gpu_ctx, wl):
     model = adj.HeatFluxAdjoint(gpu_ctx, case, checkpoint=5)
     F_true = wl.flux_profile(32, F0=12.0, modes=((1, 0.3, 0.0), (2, 0.1, 1.0)))
     tphi, tchi, cps = model.forward(F_true, 5.5)
-    for x in cps.values():
-        x.free()
+    cps.close()
     return torch, adj, case, model, tphi, tchi
 
 
@@ -41,8 +40,7 @@ def _loss_np(n, phi, chi, tphi, tchi):
 
 def _gpu_loss(model, F, St, tphi, tchi):
     phi, chi, cps = model.forward(F, St)
-    for x in cps.values():
-        x.free()
+    cps.close()
     return _loss_np(model.n, phi, chi, tphi, tchi)
 
 
@@ -112,8 +110,7 @@ def test_adjoint_matches_oracle_finite_differences(ml, gpu_ctx, wl):
     F = wl.flux_profile(32, F0=10.0)
     St = 5.0
     phi_gpu, _, cps = model.forward(F, St)
-    for x in cps.values():
-        x.free()
+    cps.close()
     phi_ref, _ = _oracle_forward(case, F, St)
     n = case["n"]
     err = HC.rel_err(HC.valid(n, phi_gpu), HC.valid(n, phi_ref))
@@ -136,3 +133,26 @@ def test_adjoint_matches_oracle_finite_differences(ml, gpu_ctx, wl):
     print("oracle St: adjoint %.9e, central difference %.9e" % (gSt, fd))
     assert abs(gSt - fd) <= FD_TOL * abs(fd)
     model.close()
+
+
+def test_adjoint_repeated_backward(ml, gpu_ctx, wl):
+    """backward can be repeated (retain_graph): the checkpoints live with the
+    graph, not with one backward; the permeability gradient is the
+    structural zero."""
+    torch, adj, case, model, tphi, tchi = _setup(ml, gpu_ctx, wl)
+    F = wl.flux_profile(32, F0=10.0)
+    flux = torch.tensor(F, dtype=torch.float64, requires_grad=True)
+    perm = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    st = torch.tensor(5.0, dtype=torch.float64, requires_grad=True)
+    phi, chi = adj.ImplicitSteps.apply(flux, perm, st, model)
+    loss = adj.field_loss(model.n, phi, chi, tphi, tchi)
+    loss.backward(retain_graph=True)
+    g1 = flux.grad.detach().numpy().copy()
+    s1 = float(st.grad)
+    assert float(perm.grad) == 0.0
+    flux.grad = None
+    st.grad = None
+    loss.backward()
+    np.testing.assert_array_equal(flux.grad.detach().numpy(), g1)
+    assert float(st.grad) == s1
+    assert np.abs(g1).max() > 0

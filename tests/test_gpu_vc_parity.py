@@ -292,3 +292,43 @@ def test_host_driver_equals_engine(ml, gpu_ctx, name):
         res[flags] = ([x.get() for x in dn], [(s.exit_status, s.iterations) for s in sts])
     assert res[0][1] == res[ml.SOLVE_HOST_DRIVER][1]
     assert all(HC.bitwise_equal(a, b) for a, b in zip(res[0][0], res[ml.SOLVE_HOST_DRIVER][0]))
+
+
+# Large levels relax by streaming along the last direction (2D above 512^2
+# cells, 3D above 96^3): wall and periodic sides in every direction, ragged
+# block and segment counts, and several launches per call.
+STREAM = {
+    "2d_700x600_walls": ((700, 600), (0, 0), {(0, 0, 0): (0, 0.0), (0, 1, 0): (1, 0.0), (1, 0, 0): (1, 0.0),
+                                              (1, 1, 0): (0, 0.0)}),
+    "2d_530x520_periodic": ((530, 520), (1, 1), {}),
+    "2d_1030x261_mixed": ((1030, 261), (1, 0), {(1, 0, 0): (0, 0.0), (1, 1, 0): (0, 0.0)}),
+    "3d_100x98x97_walls_xz": ((100, 98, 97), (0, 1, 0), {(0, 0, 0): (0, 0.0), (0, 1, 0): (0, 0.0),
+                                                         (2, 0, 0): (1, 0.0), (2, 1, 0): (0, 0.0)}),
+    "3d_100x100x100_periodic_z": ((100, 100, 100), (1, 0, 1), {(1, 0, 0): (0, 0.0), (1, 1, 0): (1, 0.0)}),
+}
+
+
+@pytest.mark.parametrize("name", sorted(STREAM))
+def test_vc_relax_stream_parity(ml, gpu_ctx, name):
+    n, per, spec = STREAM[name]
+    dim = len(n)
+    g = O.make_grid(list(n), 1.0 / n[0], list(per) + [0] * (3 - dim))
+    rng = np.random.default_rng(sum(n))
+    b = [0.5 + rng.random(O.face_shape(g, d, 1)) for d in range(dim)]
+    c = _rand(g, 7)
+    c = np.abs(c) * 50.0
+    bc_o = O.make_bc(spec)
+    op = ml.VCOp(gpu_ctx, list(n), g.dx, list(per), ml.make_bc(spec))
+    db = [ml.DeviceArray(gpu_ctx, x.shape, host=x) for x in b]
+    dc = ml.DeviceArray(gpu_ctx, c.shape, host=c)
+    alpha, beta = 1.0, -2e-3
+    op.set_coefs(None, db, dc, alpha, beta)
+    phi, rhs = _rand(g, 11), _rand(g, 12)
+    dphi = ml.DeviceArray(gpu_ctx, phi.shape, host=phi)
+    drhs = ml.DeviceArray(gpu_ctx, rhs.shape, host=rhs)
+    ref = phi.copy()
+    for iters in (1, 2):   # 2 then 4 colour passes: one and two streaming launches
+        ref = O.lin_gsrb(g, 1, bc_o, alpha, None, beta, c, b, ref, rhs, iters=iters)
+        op.relax(0, dphi, drhs, iters)
+        _check(list(n), dphi.get(), ref)
+    op.close()
