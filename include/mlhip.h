@@ -92,13 +92,23 @@ typedef struct {
  * measurements).  ML_SOLVE_NO_GRAPH launches every kernel from the host, each
  * step of an iteration as its own kernel, instead of replaying the
  * iterations captured as a HIP graph with the residual fused into the last
- * relaxation: same results. */
+ * relaxation: same results.
+ * bottom_solver (linear operator solves only): ML_BOTTOM_RELAX relaxes the
+ * coarsest MG level `bottom` times; ML_BOTTOM_BICGSTAB follows those
+ * relaxations with BiCGStab of at most bottom_max_iter iterations, the
+ * reference's bottom solver of the projection (Projector::makeBottomSolvers,
+ * srcSubcycle/Projector.cpp:3004-3023, m_imax 20) and of the U* solves
+ * (s_botSolverUStar, srcSubcycle/AMRLevelMushyLayerInit.cpp:739, m_imax 80).
+ * BiCGStab needs a coarsest level of at most 1024 cells (it runs inside the
+ * single-workgroup V-cycle; ML_ENOTSUP otherwise).  Defaults: relax, 80. */
 typedef struct {
   int nu1, nu2, bottom, max_iter;
   double eps, hang, norm_thresh;
   int precision;
   int flags;
+  int bottom_solver, bottom_max_iter;
 } ml_solve_opts;
+enum { ML_BOTTOM_RELAX = 0, ML_BOTTOM_BICGSTAB = 1 };
 
 enum { ML_SOLVE_HOST_DRIVER = 1, ML_SOLVE_NO_LOOKAHEAD = 2, ML_SOLVE_NO_GRAPH = 4 };
 
@@ -258,6 +268,14 @@ int ml_vc_residual(ml_vc_op op, int depth, double* res, double* phi, const doubl
                    int homogeneous);
 /* applyOpI (src/DarcyBrinkmanOp.cpp:123-137): lhs = L(phi) */
 int ml_vc_apply_op(ml_vc_op op, int depth, double* lhs, double* phi, int homogeneous);
+/* preCond (src/DarcyBrinkmanOp.cpp:89-121): phi = lambda*rhs on the valid
+ * cells, then two red/black iterations (homogeneous BCs) */
+int ml_vc_precond(ml_vc_op op, int depth, double* phi, const double* rhs);
+/* dotProduct (AMRPoissonOp, inherited by DarcyBrinkmanOp, src/DarcyBrinkmanOp.H:27):
+ * *out = (x, y) over the valid cells of level `depth`, in the fixed order of
+ * the BiCGStab bottom (64 interleaved partial sums, then a pairwise tree);
+ * blocks until the value is on the host */
+int ml_vc_dot(ml_vc_op op, int depth, const double* x, const double* y, double* out);
 /* restrictResidual (src/DarcyBrinkmanOp.cpp:183-240), homogeneous BCs */
 int ml_vc_restrict_residual(ml_vc_op op, int depth, double* resCoarse, double* phiFine,
                             const double* rhsFine);

@@ -37,7 +37,11 @@ class BC(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("nu1", C.c_int), ("nu2", C.c_int), ("bottom", C.c_int), ("max_iter", C.c_int),
                 ("eps", C.c_double), ("hang", C.c_double), ("norm_thresh", C.c_double),
-                ("precision", C.c_int), ("flags", C.c_int)]
+                ("precision", C.c_int), ("flags", C.c_int),
+                ("bottom_solver", C.c_int), ("bottom_max_iter", C.c_int)]
+
+
+BOTTOM_RELAX, BOTTOM_BICGSTAB = 0, 1   # SolveOpts.bottom_solver (linear operator solves)
 
 
 SOLVE_HOST_DRIVER = 1
@@ -159,6 +163,8 @@ SIGNATURES = {
     "ml_vc_relax": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
     "ml_vc_residual": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int]),
     "ml_vc_apply_op": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int]),
+    "ml_vc_precond": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "ml_vc_dot": (C.c_int, [_vp, C.c_int, _vp, _vp, C.POINTER(C.c_double)]),
     "ml_vc_restrict_residual": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
     "ml_vc_prolong_increment": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "ml_vc_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
@@ -662,6 +668,16 @@ class VCOp:
 
     def apply_op(self, depth, lhs, phi, homogeneous=0):
         self.ctx.check(self.ctx.lib.ml_vc_apply_op(self.h, depth, lhs.ptr, phi.ptr, homogeneous), "ml_vc_apply_op")
+
+    def precond(self, depth, phi, rhs):
+        """preCond: phi = lambda*rhs, then two red/black iterations."""
+        self.ctx.check(self.ctx.lib.ml_vc_precond(self.h, depth, phi.ptr, rhs.ptr), "ml_vc_precond")
+
+    def dot(self, depth, x, y):
+        """(x, y) over the valid cells, in the BiCGStab bottom's order."""
+        out = C.c_double(0.0)
+        self.ctx.check(self.ctx.lib.ml_vc_dot(self.h, depth, x.ptr, y.ptr, C.byref(out)), "ml_vc_dot")
+        return out.value
 
     def restrict_residual(self, depth, resc, phi, rhs):
         self.ctx.check(self.ctx.lib.ml_vc_restrict_residual(self.h, depth, resc.ptr, phi.ptr, rhs.ptr),

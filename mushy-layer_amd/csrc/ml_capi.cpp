@@ -31,6 +31,22 @@ int max_lds(ml_ctx ctx, size_t* out) {
   return ML_OK;
 }
 
+// A level that does not coarsen and holds at most 1024 cells is relaxed by a
+// single-workgroup kernel that reads periodic images in place; with an odd
+// periodic length the two cells across the seam have the same colour, so
+// their red/black order is not the reference's (which reads the seam from
+// ghost cells filled before each colour).  Coarser levels are always even.
+bool odd_periodic_small_level(int dim, const int n[], const int periodic[]) {
+  if (ml_mg_depth(dim, n) != 1) return false;
+  long long cells = 1;
+  bool odd = false;
+  for (int d = 0; d < dim; ++d) {
+    cells *= n[d];
+    if (periodic && periodic[d] && (n[d] & 1)) odd = true;
+  }
+  return odd && cells <= 1024;
+}
+
 bool valid_grid(int dim, const int n[], const int periodic[], double dx) {
   if (dim != 2 && dim != 3) return false;
   if (!(dx > 0)) return false;
@@ -237,6 +253,8 @@ void ml_default_opts(ml_solve_opts* o) {
   o->norm_thresh = 1e-10;
   o->precision = ML_PRECISION_FP64;
   o->flags = 0;
+  o->bottom_solver = ML_BOTTOM_RELAX;
+  o->bottom_max_iter = 80;
 }
 
 int ml_norm_max(ml_ctx ctx, int dim, const int n[], int ncomp, const double* x, double* out) {
@@ -304,6 +322,8 @@ int ml_hc_op_create(ml_ctx ctx, int dim, const int lo[], const int hi[], double 
   op->bc.apply = bc->apply;
   const int depth = ml_mg_depth(dim, n);
   if (depth > ml::kMaxLevels) return fail(ctx, ML_ENOTSUP, "too many MG levels");
+  if (ml::odd_periodic_small_level(dim, n, periodic))
+    return fail(ctx, ML_ENOTSUP, "odd periodic length on a level of at most 1024 cells that does not coarsen");
   ml::Geom g0 = ml::make_geom(dim, n, periodic, dx);
   op->lev.resize(depth);
   for (int l = 0; l < depth; ++l) {

@@ -71,6 +71,7 @@ int scratch_init(ml_ctx ctx, int device);
 // largest dynamic LDS of one workgroup on the context's device (bytes)
 int max_lds(ml_ctx ctx, size_t* out);
 bool valid_grid(int dim, const int n[], const int periodic[], double dx);
+bool odd_periodic_small_level(int dim, const int n[], const int periodic[]);
 
 #define ML_HIP(ctx, expr)                                                                     \
   do {                                                                                        \
@@ -226,7 +227,7 @@ struct VCWork {
   hipGraphExec_t gexec = nullptr;
   const double* g_phi = nullptr;
   const double* g_rhs = nullptr;
-  int g_nu1 = -1, g_nu2 = -1, g_bottom = -1;
+  int g_nu1 = -1, g_nu2 = -1, g_bottom = -1, g_bsolver = -1, g_bimax = -1;
   double g_alpha = 0, g_beta = 0;
 };
 }  // namespace ml
@@ -242,6 +243,7 @@ struct ml_vc_op_s {
   int nslots = 1024;                // norm slots per workspace (max_iter + 1)
   std::vector<ml::VCWork> work;     // solve workspaces; [0] runs on the context stream
   size_t coarse_lds = 0;            // dynamic LDS of the single-workgroup V-cycle
+  size_t coarse_lds_bicg = 0;       // ... with the BiCGStab bottom (0: does not fit)
   // the reference-API operator objects and linear MG driver behind
   // ML_SOLVE_HOST_DRIVER (built at its first use)
   std::unique_ptr<ml::VCHostDriver, void (*)(ml::VCHostDriver*)> host_driver{nullptr, nullptr};

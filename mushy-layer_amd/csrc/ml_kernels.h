@@ -600,6 +600,7 @@ struct VCLevel {
 template <typename T>
 struct VCDesc {
   int l0, nlev, nu1, nu2, bottom;
+  int bottom_solver, bottom_imax;   // BiCGStab on the coarsest level after `bottom` relaxations (solver 1)
   BCDesc bc;
   T alpha, beta;
   const int* skip;
@@ -637,9 +638,15 @@ hipError_t vc_restrict_residual(const VCArgs<T>& A, const Geom& gc, const T* phi
                                 hipStream_t s);
 template <typename T>
 hipError_t vc_coarse_vcycle(const VCDesc<T>* dev_desc, int dim, size_t lds, hipStream_t s);
+// phi = rhs * lambda on the valid cells (preCond's first step)
+template <typename T>
+hipError_t vc_lambda_mult(const VCArgs<T>& A, const T* rhs, T* phi, hipStream_t s);
+// *out = (x, y) over the valid cells, in the BiCGStab bottom's fixed order
+template <typename T>
+hipError_t vc_dot(const Geom& g, const T* x, const T* y, T* out, hipStream_t s);
 // dynamic LDS of vc_coarse_vcycle for levels l0..nlev-1 of g
 template <typename T>
-size_t vc_coarse_lds_bytes(const Geom* g, int l0, int nlev);
+size_t vc_coarse_lds_bytes(const Geom* g, int l0, int nlev, bool bicgstab = false);
 bool vc_small_level(const Geom& g);
 // levels (the small level and all coarser) one vc_coarse_vcycle launch handles
 int vc_coarse_levels_max(int dim);

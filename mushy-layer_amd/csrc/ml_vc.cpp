@@ -69,6 +69,8 @@ int vc_solve_host(ml_vc_op op, int n, double* const phi[], const double* const r
     p.eps = o.eps;
     p.hang = o.hang;
     p.norm_thresh = o.norm_thresh;
+    p.bicgstab_bottom = o.bottom_solver == ML_BOTTOM_BICGSTAB;
+    p.bottom_imax = o.bottom_max_iter;
     const ml::Geom& g = op->lev[0].g;
     for (int i = 0; i < n; ++i) {
       mlhost::LevelData vphi, vrhs;
@@ -189,6 +191,8 @@ int vc_upload_desc(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o) {
   D.nu1 = o.nu1;
   D.nu2 = o.nu2;
   D.bottom = o.bottom;
+  D.bottom_solver = o.bottom_solver;
+  D.bottom_imax = o.bottom_max_iter;
   D.bc = op->bc;
   D.alpha = op->alpha;
   D.beta = op->beta;
@@ -243,7 +247,10 @@ int vc_vcycle(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o, int* ecur, con
                                                W.r[l + 1], s, tm));
   }
   if (lc < L) {
-    ML_HIP(ctx, ml::vc_coarse_vcycle<double>(W.desc, op->dim, op->coarse_lds, s));
+    ML_HIP(ctx, ml::vc_coarse_vcycle<double>(W.desc, op->dim,
+                                             o.bottom_solver == ML_BOTTOM_BICGSTAB ? op->coarse_lds_bicg
+                                                                                   : op->coarse_lds,
+                                             s));
   } else {
     buf(L - 1, B);
     ML_HIP(ctx, ml::vc_relax<double>(args(L - 1), B, &ecur[L - 1], R(L - 1), 2 * o.bottom, true, nullptr, nullptr,
@@ -319,7 +326,8 @@ int vc_iteration(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o, int* ecur, 
 // not usable (the V-cycle's ping-pong state does not return to its start).
 bool vc_capture(ml_vc_op op, ml::VCWork& W, VCSolve& S, const ml_solve_opts& o) {
   if (W.gexec && W.g_phi == S.phi && W.g_rhs == S.rhs && W.g_nu1 == o.nu1 && W.g_nu2 == o.nu2 &&
-      W.g_bottom == o.bottom && W.g_alpha == op->alpha && W.g_beta == op->beta)
+      W.g_bottom == o.bottom && W.g_bsolver == o.bottom_solver && W.g_bimax == o.bottom_max_iter &&
+      W.g_alpha == op->alpha && W.g_beta == op->beta)
     return true;
   if (W.gexec) hipGraphExecDestroy(W.gexec);
   W.gexec = nullptr;
@@ -343,6 +351,8 @@ bool vc_capture(ml_vc_op op, ml::VCWork& W, VCSolve& S, const ml_solve_opts& o) 
   W.g_nu1 = o.nu1;
   W.g_nu2 = o.nu2;
   W.g_bottom = o.bottom;
+  W.g_bsolver = o.bottom_solver;
+  W.g_bimax = o.bottom_max_iter;
   W.g_alpha = op->alpha;
   W.g_beta = op->beta;
   return true;
@@ -353,6 +363,15 @@ int vc_solve_check(ml_vc_op op, const ml_solve_opts& o) {
   if (o.max_iter + 1 > op->nslots) return fail(ctx, ML_EINVAL, "max_iter too large");
   if (o.nu1 < 0 || o.nu2 < 0 || o.bottom < 0) return fail(ctx, ML_EINVAL, "negative smoothing count");
   if (o.precision != ML_PRECISION_FP64) return fail(ctx, ML_ENOTSUP, "the linear operator solves in FP64 only");
+  if (o.bottom_solver != ML_BOTTOM_RELAX && o.bottom_solver != ML_BOTTOM_BICGSTAB)
+    return fail(ctx, ML_EINVAL, "unknown bottom solver");
+  if (o.bottom_solver == ML_BOTTOM_BICGSTAB) {
+    if (o.bottom_max_iter < 0) return fail(ctx, ML_EINVAL, "negative bottom_max_iter");
+    // the BiCGStab bottom runs inside the single-workgroup V-cycle
+    if (op->lc >= int(op->lev.size()))
+      return fail(ctx, ML_ENOTSUP, "BiCGStab bottom: the coarsest MG level exceeds one workgroup's cells");
+    if (op->coarse_lds_bicg == 0) return fail(ctx, ML_ENOTSUP, "BiCGStab bottom: the coarse levels exceed the LDS");
+  }
   return ML_OK;
 }
 
@@ -493,6 +512,8 @@ int ml_vc_op_create(ml_ctx ctx, int dim, const int lo[], const int hi[], double 
   op->bc.apply = 1;
   const int depth = ml_mg_depth(dim, n);
   if (depth > ml::kMaxLevels) return fail(ctx, ML_ENOTSUP, "too many MG levels");
+  if (ml::odd_periodic_small_level(dim, n, periodic))
+    return fail(ctx, ML_ENOTSUP, "odd periodic length on a level of at most 1024 cells that does not coarsen");
   const ml::Geom g0 = ml::make_geom(dim, n, periodic, dx);
   op->lev.resize(depth);
   op->lc = depth;
@@ -522,6 +543,8 @@ int ml_vc_op_create(ml_ctx ctx, int dim, const int lo[], const int hi[], double 
     raw->coarse_lds = ml::vc_coarse_lds_bytes<double>(gs.data(), raw->lc, depth);
     size_t mx = 0;
     if ((rc = ml::max_lds(ctx, &mx))) return bail(rc);
+    const size_t lb = ml::vc_coarse_lds_bytes<double>(gs.data(), raw->lc, depth, true);
+    raw->coarse_lds_bicg = lb <= mx ? lb : 0;
     if (raw->coarse_lds > mx) {
       ml::fail(ctx, ML_ENOTSUP, "the coarse MG levels exceed the workgroup's LDS");
       return bail(ML_ENOTSUP);
@@ -637,6 +660,27 @@ int ml_vc_apply_op(ml_vc_op op, int depth, double* lhs, double* phi, int homogen
   VC_DEPTH_CHECK(op, depth);
   if (!lhs || !phi) return ML_EINVAL;
   ML_HIP(op->ctx, ml::vc_apply_op<double>(op->args(depth), phi, lhs, homogeneous, op->ctx->stream));
+  return ML_OK;
+}
+
+int ml_vc_precond(ml_vc_op op, int depth, double* phi, const double* rhs) {
+  VC_DEPTH_CHECK(op, depth);
+  if (!phi || !rhs) return ML_EINVAL;
+  ML_HIP(op->ctx, ml::vc_lambda_mult<double>(op->args(depth), rhs, phi, op->ctx->stream));
+  return ml_vc_relax(op, depth, phi, rhs, 2);
+}
+
+int ml_vc_dot(ml_vc_op op, int depth, const double* x, const double* y, double* out) {
+  VC_DEPTH_CHECK(op, depth);
+  if (!x || !y || !out) return ML_EINVAL;
+  ml_ctx ctx = op->ctx;
+  ml::VCLevelStore& L = op->lev[depth];
+  int rc;
+  if (!L.tmp && (rc = vc_alloc(ctx, L.g.ncell, &L.tmp))) return rc;
+  // the level's partner array holds the result (a relaxation overwrites it)
+  ML_HIP(ctx, ml::vc_dot<double>(L.g, x, y, L.tmp, ctx->stream));
+  ML_HIP(ctx, hipMemcpyAsync(out, L.tmp, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  ML_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return ML_OK;
 }
 

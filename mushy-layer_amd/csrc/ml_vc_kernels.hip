@@ -22,6 +22,8 @@
 #include <hip/hip_ext.h>
 
 #include "ml_common.h"
+#include <algorithm>
+
 #include "ml_kernels.h"
 #include "ml_tile.h"
 
@@ -526,6 +528,171 @@ __device__ __forceinline__ void vcx_prolong(const VCCtx<T, DIM>& X, const CLev& 
   __syncthreads();
 }
 
+// BiCGStab on the coarsest level L (held in LDS) from its relaxed
+// correction: L e = r with homogeneous BCs, preconditioner preCond (e =
+// lambda*r, then two red/black iterations).  Preconditioned BiCGStab with
+// restarts; eps 1e-6, reps 1e-12, hang 1e-8, small 1e-30, 5 restarts, L2
+// norms.  Dot products run redundantly in every participating wavefront in a
+// fixed order: 64 interleaved lane sums over the compact cell index, then the
+// pairwise tree s_t + s_{t+h}, h = 32 .. 1.  Levels whose owners fit one
+// wavefront run on wave 0 alone with wavefront-scope syncs.  wk holds 8
+// vectors of L.N values.
+template <typename T, int DIM>
+__device__ __forceinline__ void vcx_bicgstab(const VCCtx<T, DIM>& X, const CLev& L, int bcm, const VCLds<T, DIM>& S, bool hasc,
+                             T alpha, T beta, int imax, T* wk) {
+  const int N = L.N;
+  const bool small = cown_active<kVCWG>(L) <= 64 && N <= 64 * 2;
+  if (small && threadIdx.x >= 64) {
+    __syncthreads();
+    return;
+  }
+  const int tid = threadIdx.x, stride = small ? 64 : kVCWG, lane = threadIdx.x & 63;
+  auto sync = [&]() __attribute__((always_inline)) {
+    if (small)
+      wave_sync();
+    else
+      __syncthreads();
+  };
+  const T dx = T(L.dx);
+  const T dxinv = T(1) / (dx * dx);
+  T* phi = S.se + L.off;
+  const T* rhs = S.sr + L.off;
+  T *r = wk, *rt = wk + N, *e = wk + 2 * N, *p = wk + 3 * N, *pt = wk + 4 * N, *st = wk + 5 * N, *t = wk + 6 * N,
+    *v = wk + 7 * N;
+  auto lof_of = [&](int m, const T* u) __attribute__((always_inline)) {
+    T stc[7];
+    vcx_stencil<T, DIM>(X, m, bcm, dx, u, stc);
+    return vc_lof<T, DIM>(X.F[m], stc, X.aq[m], hasc, X.cv[m], alpha, beta, dxinv);
+  };
+  auto apply = [&](T* out, const T* u) __attribute__((always_inline)) {
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+      if (X.pk[m] >= 0) out[X.c[m]] = lof_of(m, u);
+    sync();
+  };
+  auto residual = [&](T* out, const T* u, const T* f) __attribute__((always_inline)) {
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+      if (X.pk[m] >= 0) out[X.c[m]] = f[X.c[m]] - lof_of(m, u);
+    sync();
+  };
+  auto precond = [&](T* z, const T* f) __attribute__((always_inline)) {
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+      if (X.pk[m] >= 0) z[X.c[m]] = f[X.c[m]] * X.lam[m];
+    sync();
+    for (int pass = 0; pass < 4; ++pass) {
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        if (X.pk[m] < 0 || X.col[m] != (pass & 1)) continue;
+        T stc[7];
+        vcx_stencil<T, DIM>(X, m, bcm, dx, z, stc);
+        const T lof = vc_lof<T, DIM>(X.F[m], stc, X.aq[m], hasc, X.cv[m], alpha, beta, dxinv);
+        z[X.c[m]] = stc[0] - X.lam[m] * (lof - f[X.c[m]]);
+      }
+      sync();
+    }
+  };
+  auto dot = [&](const T* x, const T* y) __attribute__((always_inline)) {
+    T s = T(0);
+    for (int c = lane; c < N; c += 64) s = s + x[c] * y[c];
+#pragma unroll
+    for (int h = 32; h >= 1; h >>= 1) s = s + __shfl_down(s, h, 64);
+    return __shfl(s, 0, 64);
+  };
+  const T eps = T(1e-6), reps = T(1e-12), hang = T(1e-8), tiny = T(1e-30);
+  residual(r, phi, rhs);
+  for (int c = tid; c < N; c += stride) {
+    rt[c] = r[c];
+    e[c] = T(0);
+  }
+  T norm0 = sqrt(dot(r, r)), norm1 = norm0;
+  const T r0 = norm0;
+  T rho0 = 0, rho1 = 0, alpha0 = 0, alpha1 = 0, omega0 = 0, omega1 = 0;
+  bool init = true;
+  int restarts = 0, recount = 0, i = 0;
+  sync();
+  while (i < imax && norm0 > eps * norm1 && norm1 > T(0)) {
+    ++i;
+    norm1 = norm0;
+    alpha1 = alpha0;
+    omega1 = omega0;
+    rho1 = rho0;
+    rho0 = dot(rt, r);
+    if (rho0 == T(0)) break;
+    if (init) {
+      for (int c = tid; c < N; c += stride) p[c] = r[c];
+      init = false;
+    } else {
+      const T bt = (rho0 / rho1) * (alpha1 / omega1);
+      const T sv = -bt * omega1;
+      for (int c = tid; c < N; c += stride) {
+        T q = bt * p[c];
+        q = q + sv * v[c];
+        p[c] = q + T(1) * r[c];
+      }
+    }
+    sync();
+    precond(pt, p);
+    apply(v, pt);
+    const T m = dot(rt, v);
+    alpha0 = rho0 / m;
+    if (fabs(m) > tiny * fabs(rho0)) {
+      const T na = -alpha0;
+      for (int c = tid; c < N; c += stride) {
+        r[c] = r[c] + na * v[c];
+        e[c] = e[c] + alpha0 * pt[c];
+      }
+      sync();
+      norm0 = sqrt(dot(r, r));
+    } else {
+      for (int c = tid; c < N; c += stride) r[c] = T(0);
+      sync();
+      norm0 = T(0);
+    }
+    if (norm0 > eps * r0 && norm0 > reps * r0) {
+      precond(st, r);
+      apply(t, st);
+      omega0 = dot(t, r) / dot(t, t);
+      sync();   // every wavefront has read r before it changes
+      const T nw = -omega0;
+      for (int c = tid; c < N; c += stride) {
+        e[c] = e[c] + omega0 * st[c];
+        r[c] = r[c] + nw * t[c];
+      }
+      sync();
+      norm0 = sqrt(dot(r, r));
+    }
+    if (norm0 <= eps * r0 || norm0 <= reps * r0) break;
+    if (omega0 == T(0) || norm0 > (T(1) - hang) * norm1) {
+      if (recount == 0) {
+        recount = 1;
+      } else {
+        recount = 0;
+        for (int c = tid; c < N; c += stride) phi[c] = phi[c] + T(1) * e[c];
+        if (restarts == 5) {
+          for (int c = tid; c < N; c += stride) e[c] = T(0);
+          break;
+        }
+        sync();
+        residual(r, phi, rhs);
+        norm0 = sqrt(dot(r, r));
+        rho0 = alpha0 = omega0 = T(0);
+        for (int c = tid; c < N; c += stride) {
+          rt[c] = r[c];
+          e[c] = T(0);
+        }
+        sync();
+        ++restarts;
+        init = true;
+      }
+    }
+  }
+  sync();
+  for (int c = tid; c < N; c += stride) phi[c] = phi[c] + T(1) * e[c];
+  __syncthreads();
+}
+
 // one cell's operator data (and right-hand side) read from global memory
 template <typename T, int DIM>
 struct VCPre {
@@ -620,6 +787,15 @@ __global__ __launch_bounds__(kVCWG) void vc_coarse_vcycle_kernel(const VCDesc<T>
     vcx_build<T, DIM>(X, G[l], per, S);
     vcx_relax<T, DIM>(X, G[l], bcm, S, hasc, alpha, beta, 2 * (l == nl - 1 ? D.bottom : D.nu1));
     if (l + 1 < nl) vcx_restrict<T, DIM>(X, G[l], G[l + 1], bcm, S, hasc, alpha, beta);
+  }
+  // the bottom solver on the coarsest level (after its relaxations)
+  if (D.bottom_solver == 1) {
+    const Geom& gb = D.lev[D.nlev - 1].g;
+    CLev Lb;
+    clev_set(Lb, gb, tot - int(gb.nvalid()));
+    VCCtx<T, DIM> X;
+    vcx_build<T, DIM>(X, Lb, per, S);
+    vcx_bicgstab<T, DIM>(X, Lb, bcm, S, hasc, alpha, beta, D.bottom_imax, S.sres + kVCWGCells);
   }
   // up: prolongation and post-smoothing
 #pragma unroll
@@ -1225,11 +1401,70 @@ hipError_t vc_restrict_residual(const VCArgs<T>& A, const Geom& gc, const T* phi
   return A.g.dim == 2 ? vc_tile_dim<T, 2, kVCRestrict>(A, io, s) : vc_tile_dim<T, 3, kVCRestrict>(A, io, s);
 }
 
+// phi = rhs * lambda on the valid cells (the first step of preCond)
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void vc_lambda_mult_kernel(VCArgs<T> A, const T* __restrict__ rhs,
+                                                             T* __restrict__ phi) {
+  const Geom& g = A.g;
+  const long long n = g.nvalid();
+  const T dxinv = T(1) / (T(g.dx) * T(g.dx));
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    const int i = int(c % g.n[0]);
+    const long long r = c / g.n[0];
+    const int j = int(r % g.n[1]);
+    const int k = int(r / g.n[1]);
+    Faces1<T, DIM> F;
+    load_faces1<T, DIM>(A, i, j, k, F);
+    const long long q = g.ci(i, j, k);
+    const T aq = A.a ? A.a[q] : T(1);
+    const T cv = A.c ? A.c[q] : T(0);
+    phi[q] = rhs[q] * vc_lambda<T, DIM>(F, aq, A.c != nullptr, cv, A.alpha, A.beta, dxinv);
+  }
+}
+
+// (x, y) over the valid cells in the order of the coarse V-cycle's BiCGStab:
+// one wavefront, 64 interleaved lane sums over the compact cell index, then
+// the pairwise tree s_t + s_{t+h}, h = 32 .. 1
 template <typename T>
-size_t vc_coarse_lds_bytes(const Geom* g, int l0, int nlev) {
+__global__ __launch_bounds__(64) void vc_dot_kernel(Geom g, const T* __restrict__ x, const T* __restrict__ y,
+                                                    T* __restrict__ out) {
+  const long long n = g.nvalid();
+  T s = T(0);
+  for (long long c = threadIdx.x; c < n; c += 64) {
+    const int i = int(c % g.n[0]);
+    const long long r = c / g.n[0];
+    const long long q = g.ci(i, int(r % g.n[1]), int(r / g.n[1]));
+    s = s + x[q] * y[q];
+  }
+#pragma unroll
+  for (int h = 32; h >= 1; h >>= 1) s = s + __shfl_down(s, h, 64);
+  if (threadIdx.x == 0) *out = s;
+}
+
+template <typename T>
+hipError_t vc_lambda_mult(const VCArgs<T>& A, const T* rhs, T* phi, hipStream_t s) {
+  const long long n = A.g.nvalid();
+  const int blocks = int(std::min<long long>((n + 255) / 256, 4096));
+  if (A.g.dim == 2)
+    hipLaunchKernelGGL((vc_lambda_mult_kernel<T, 2>), dim3(blocks), dim3(256), 0, s, A, rhs, phi);
+  else
+    hipLaunchKernelGGL((vc_lambda_mult_kernel<T, 3>), dim3(blocks), dim3(256), 0, s, A, rhs, phi);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t vc_dot(const Geom& g, const T* x, const T* y, T* out, hipStream_t s) {
+  hipLaunchKernelGGL((vc_dot_kernel<T>), dim3(1), dim3(64), 0, s, g, x, y, out);
+  return hipGetLastError();
+}
+
+template <typename T>
+size_t vc_coarse_lds_bytes(const Geom* g, int l0, int nlev, bool bicgstab) {
   long long tot = 0;
   for (int l = l0; l < nlev; ++l) tot += g[l].nvalid();
-  return vc_coarse_lds_elems<T>(g[l0].dim, tot) * sizeof(T);
+  // BiCGStab: 8 vectors of the coarsest level after the restriction scratch
+  const size_t extra = bicgstab ? size_t(8 * g[nlev - 1].nvalid()) : 0;
+  return (vc_coarse_lds_elems<T>(g[l0].dim, tot) + extra) * sizeof(T);
 }
 
 template <typename T>
@@ -1255,9 +1490,11 @@ template hipError_t vc_residual<double>(const VCArgs<double>&, const double*, co
                                         const double*, double*, unsigned long long*, int, hipStream_t,
                                         const CtlFuse&);
 template hipError_t vc_apply_op<double>(const VCArgs<double>&, const double*, double*, int, hipStream_t);
+template hipError_t vc_lambda_mult<double>(const VCArgs<double>&, const double*, double*, hipStream_t);
+template hipError_t vc_dot<double>(const Geom&, const double*, const double*, double*, hipStream_t);
 template hipError_t vc_restrict_residual<double>(const VCArgs<double>&, const Geom&, const double*, const double*,
                                                  double*, hipStream_t);
 template hipError_t vc_coarse_vcycle<double>(const VCDesc<double>*, int, size_t, hipStream_t);
-template size_t vc_coarse_lds_bytes<double>(const Geom*, int, int);
+template size_t vc_coarse_lds_bytes<double>(const Geom*, int, int, bool);
 
 }  // namespace ml

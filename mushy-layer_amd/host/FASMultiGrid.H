@@ -24,6 +24,9 @@ namespace mlhost {
 struct FASParams {
   int nu1 = 1, nu2 = 4, bottom = 40, max_iter = 10;
   double eps = 1e-10, hang = 1e-10, norm_thresh = 1e-10;
+  // linear MG: BiCGStab on the coarsest level after the `bottom` relaxations
+  bool bicgstab_bottom = false;
+  int bottom_imax = 80;
 };
 
 struct SolveStats {

@@ -65,6 +65,16 @@ class HipVCLevelOp {
   void applyOp(LevelData& lhs, LevelData& phi, bool homogeneous = false) {
     check(ml_vc_apply_op(m_op, m_depth, lhs.data(), phi.data(), homogeneous), m_ctx, "applyOp");
   }
+  // preCond (src/DarcyBrinkmanOp.cpp:89-121): lambda*rhs, then relax(phi, rhs, 2)
+  void preCond(LevelData& phi, const LevelData& rhs) {
+    check(ml_vc_precond(m_op, m_depth, phi.data(), rhs.data()), m_ctx, "preCond");
+  }
+  // dotProduct over the valid cells (in the BiCGStab bottom's fixed order)
+  double dotProduct(const LevelData& x, const LevelData& y) const {
+    double v = 0;
+    check(ml_vc_dot(m_op, m_depth, x.data(), y.data(), &v), m_ctx, "dotProduct");
+    return v;
+  }
   // relax -> levelGSRB x iters, homogeneous BCs (src/DarcyBrinkmanOp.cpp:503-576)
   void relax(LevelData& e, const LevelData& r, int iters) {
     check(ml_vc_relax(m_op, m_depth, e.data(), r.data(), iters), m_ctx, "relax");

@@ -11,12 +11,14 @@
 //   per iteration: e = 0; V(0) on L e = res (homogeneous BCs); phi += e;
 //                  res = rhs - L(phi); stop rule
 //   V(l): relax(e, r, nu1); r_c = restrictResidual(e, r); e_c = 0; V(l+1);
-//         e += P e_c; relax(e, r, nu2).  Coarsest: relax(e, r, bottom)
+//         e += P e_c; relax(e, r, nu2).  Coarsest: relax(e, r, bottom), then
+//         (bicgstab_bottom) BiCGStabSolver on L e = r from that e
 //   stop:  |r| <= eps*|r0| (1), |r| <= norm_thresh (8), hang (4), max (2)
 #pragma once
 
 #include <vector>
 
+#include "BiCGStabSolver.H"
 #include "FASMultiGrid.H"
 #include "LevelData.H"
 
@@ -75,6 +77,10 @@ class LinearMultiGrid {
     const int L = int(m_ops.size());
     if (l == L - 1) {
       op.relax(m_e[l], m_r[l], m_params.bottom);
+      if (m_params.bicgstab_bottom) {
+        m_bottom.m_imax = m_params.bottom_imax;
+        m_bottom.solve(op, m_e[l], m_r[l]);
+      }
       return;
     }
     Op& opc = *m_ops[l + 1];
@@ -89,6 +95,7 @@ class LinearMultiGrid {
   std::vector<Op*> m_ops;
   std::vector<LevelData> m_e, m_r;
   FASParams m_params;
+  BiCGStabSolver<Op> m_bottom;
 };
 
 // LevelBackwardEuler::updateSoln with a linear operator (the U* solve,

@@ -32,12 +32,18 @@ import numpy as np
 from . import (DeviceArray, HCOp, VCOp, default_opts, face_shape, make_advect_params, make_bc, make_flow_params,
                make_params, valid)
 
-# multigrid options of the velocity solves: U* (VelocityMultigrid: smoothing
-# 2, 10 iterations, tolerance 1e-10) and the projection
-# (srcSubcycle/Projector.cpp:45-64: down 2, up 4, tolerance and hang 1e-15,
-# 20 iterations); relaxation bottoms
-U_OPTS = dict(nu1=2, nu2=2, bottom=8, max_iter=10, eps=1e-10, hang=1e-10, norm_thresh=1e-10)
-P_OPTS = dict(nu1=2, nu2=4, bottom=4, max_iter=20, eps=1e-15, hang=1e-15, norm_thresh=1e-15)
+# multigrid options of the velocity solves: U* (VelocityMultigrid,
+# srcSubcycle/MushyLayerSubcycleUtils.cpp:654-659: smoothing 2 down, up and
+# before the bottom, 10 iterations, tolerance 1e-10; BiCGStab bottom
+# s_botSolverUStar with Chombo's default 80 iterations,
+# srcSubcycle/AMRLevelMushyLayerInit.cpp:739) and the projection
+# (srcSubcycle/Projector.cpp:53-63: down 2, up 4, 4 before the bottom,
+# tolerance and hang 1e-15, 20 iterations; BiCGStab bottom of 20 iterations,
+# :3004-3023)
+U_OPTS = dict(nu1=2, nu2=2, bottom=2, max_iter=10, eps=1e-10, hang=1e-10, norm_thresh=1e-10,
+              bottom_solver=1, bottom_max_iter=80)
+P_OPTS = dict(nu1=2, nu2=4, bottom=4, max_iter=20, eps=1e-15, hang=1e-15, norm_thresh=1e-15,
+              bottom_solver=1, bottom_max_iter=20)
 POROSITY_LIMIT = 0.05   # solidPorosity (srcSubcycle/MushyLayerSubcycleUtils.cpp:223-233)
 # hang rule off: a solve stops on eps, norm_thresh or max_iter only
 # (|r| > (1 - hang)*|r_prev| never holds for hang = -inf)

@@ -67,7 +67,12 @@ typedef struct {
 typedef struct {
   int nu1, nu2, bottom, max_iter;
   double eps, hang, norm_thresh;
+  /* linear MG only: after the `bottom` relaxations the coarsest level is
+   * solved by BiCGStab (bottom_solver MLR_BOTTOM_BICGSTAB, at most
+   * bottom_max_iter iterations) or not (MLR_BOTTOM_RELAX) */
+  int bottom_solver, bottom_max_iter;
 } mlr_opts;
+enum { MLR_BOTTOM_RELAX = 0, MLR_BOTTOM_BICGSTAB = 1 };
 
 typedef struct {
   int exit_status;       /* bit set: 1 |r| <= eps*r0, 2 iteration max,
@@ -174,6 +179,20 @@ void mlr_lin_restrict_residual(const mlr_grid* gf, int ncomp,
                                double beta, const double* c,
                                const double* const* b, double* phi,
                                const double* rhs, double* resc);
+/* (x, y) over the valid cells in the fixed order of mlref_lin.c */
+double mlr_lin_dot(const mlr_grid* g, int ncomp, const double* x, const double* y);
+/* preCond: phi = lambda*rhs, then 2 GSRB iterations (homogeneous BCs) */
+void mlr_lin_precond(const mlr_grid* g, int ncomp, const mlr_bc* bc, double alpha,
+                     const double* a, double beta, const double* c,
+                     const double* const* b, const double* lambda, double* phi,
+                     const double* rhs);
+/* BiCGStab on L phi = rhs with homogeneous BCs from the given phi; returns
+ * the exit status (1 converged, 2 breakdown or iteration limit, 3 restart
+ * limit), *iters and *final_norm (L2) when non-NULL */
+int mlr_lin_bicgstab(const mlr_grid* g, int ncomp, const mlr_bc* bc, double alpha,
+                     const double* a, double beta, const double* c,
+                     const double* const* b, const double* lambda, double* phi,
+                     const double* rhs, int imax, int* iters, double* final_norm);
 int mlr_lin_mg_solve(const mlr_grid* g, int ncomp, const mlr_bc* bc,
                      double alpha, const double* a, double beta,
                      const double* c, const double* const* b, double* phi,

@@ -40,7 +40,11 @@ class AdvectBC(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("nu1", C.c_int), ("nu2", C.c_int), ("bottom", C.c_int), ("max_iter", C.c_int),
-                ("eps", C.c_double), ("hang", C.c_double), ("norm_thresh", C.c_double)]
+                ("eps", C.c_double), ("hang", C.c_double), ("norm_thresh", C.c_double),
+                ("bottom_solver", C.c_int), ("bottom_max_iter", C.c_int)]
+
+
+BOTTOM_RELAX, BOTTOM_BICGSTAB = 0, 1
 
 
 class Stats(C.Structure):
@@ -125,6 +129,10 @@ def _declare(L):
         "mlr_advect_src": (C.c_int, [G, C.POINTER(AdvectBC), C.c_double, C.c_double, _dp, pp, _dp, _dp, pp, pp]),
         "mlr_lin_mg_solve": (C.c_int, [G, C.c_int, B, C.c_double, _dp, C.c_double, _dp, pp, _dp, _dp, C.c_int,
                                        C.POINTER(Opts), C.POINTER(Stats)]),
+        "mlr_lin_dot": (C.c_double, [G, C.c_int, _dp, _dp]),
+        "mlr_lin_precond": (None, [G, C.c_int, B, C.c_double, _dp, C.c_double, _dp, pp, _dp, _dp, _dp]),
+        "mlr_lin_bicgstab": (C.c_int, [G, C.c_int, B, C.c_double, _dp, C.c_double, _dp, pp, _dp, _dp, _dp,
+                                       C.c_int, C.POINTER(C.c_int), _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -412,6 +420,30 @@ def lin_restrict_residual(g, ncomp, bc, alpha, a, beta, c, b, phi, rhs):
     lib().mlr_lin_restrict_residual(C.byref(g), ncomp, C.byref(bc), alpha, ptr(a), beta, ptr(c), ptrs(b),
                                     ptr(phi), ptr(rhs), ptr(resc))
     return resc
+
+
+def lin_dot(g, ncomp, x, y):
+    return lib().mlr_lin_dot(C.byref(g), ncomp, ptr(x), ptr(y))
+
+
+def lin_precond(g, ncomp, bc, alpha, a, beta, c, b, rhs):
+    """preCond: lambda*rhs, then two GSRB iterations (homogeneous BCs)."""
+    lm = lam(g, ncomp, alpha, a, beta, b, c)
+    phi = np.zeros(cell_shape(g, ncomp))
+    lib().mlr_lin_precond(C.byref(g), ncomp, C.byref(bc), alpha, ptr(a), beta, ptr(c), ptrs(b), ptr(lm), ptr(phi),
+                          ptr(rhs))
+    return phi
+
+
+def lin_bicgstab(g, ncomp, bc, alpha, a, beta, c, b, phi, rhs, imax=80):
+    """BiCGStab on L phi = rhs (homogeneous BCs), phi updated in place:
+    (exit status, iterations, final L2 residual norm)."""
+    lm = lam(g, ncomp, alpha, a, beta, b, c)
+    it = C.c_int(0)
+    fn = C.c_double(0)
+    status = lib().mlr_lin_bicgstab(C.byref(g), ncomp, C.byref(bc), alpha, ptr(a), beta, ptr(c), ptrs(b), ptr(lm),
+                                    ptr(phi), ptr(rhs), int(imax), C.byref(it), C.byref(fn))
+    return status, it.value, fn.value
 
 
 def lin_mg_solve(g, ncomp, bc, alpha, a, beta, c, b, phi, rhs, prolong_linear=0, opts=None):

@@ -378,6 +378,8 @@ void mlr_default_opts(mlr_opts* o) {
   o->eps = 1e-10;
   o->hang = 1e-10;
   o->norm_thresh = 1e-10;
+  o->bottom_solver = MLR_BOTTOM_RELAX;
+  o->bottom_max_iter = 80;
 }
 
 typedef struct {

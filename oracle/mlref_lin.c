@@ -159,6 +159,171 @@ void mlr_lin_restrict_residual(const mlr_grid* gf, int ncomp,
         }
 }
 
+/* ---- BiCGStab bottom solver --------------------------------------------
+ * The reference's bottom solver of the projection (Projector::
+ * makeBottomSolvers, srcSubcycle/Projector.cpp:3004-3023: BiCGStab unless
+ * projection.relax_bottom_solver, :58, with m_imax = bottomSolveMaxIter = 20,
+ * :63) and of the U* solves (s_botSolverUStar,
+ * srcSubcycle/AMRLevelMushyLayerInit.cpp:739, Chombo's default m_imax 80).
+ * Chombo's BiCGStabSolver is not in the reference tree; build spec: van der
+ * Vorst's preconditioned BiCGStab (SIAM J. Sci. Stat. Comput. 13 (1992)
+ * 631-644) with restarts,
+ *   r = rhs - L(phi) (homogeneous), r~ = r, e = 0, |.| the L2 norm, r0 = |r|
+ *   while i < imax and |r| > eps |r_prev| and |r_prev| > 0:
+ *     rho = (r~, r); rho == 0: stop
+ *     p = r (after a (re)start) or p = beta p + (-beta omega) v + r,
+ *         beta = (rho / rho_prev) (alpha / omega)
+ *     p~ = preCond(p); v = L p~; m = (r~, v); alpha = rho / m
+ *     |m| > small |rho|: r += (-alpha) v, e += alpha p~;  else r = 0
+ *     |r| > eps r0 and |r| > reps r0: s~ = preCond(r); t = L s~;
+ *         omega = (t, r) / (t, t); e += omega s~; r += (-omega) t
+ *     |r| <= eps r0 or |r| <= reps r0: converged
+ *     omega == 0 or |r| > (1 - hang) |r_prev|: every second time, phi += e
+ *         and restart from the true residual (status 3 after 5 restarts)
+ *   phi += e
+ * with eps 1e-6, reps 1e-12, hang 1e-8, small 1e-30.  Vector updates are
+ * y = y + s*x (one rounding each), scaling y = s*y. */
+enum { DOT_LANES = 64 };
+
+/* (x, y) over the valid cells (compact index c = cell + n*nvalid, x
+ * fastest): 64 interleaved partial sums s_t = sum of x_c*y_c over c = t,
+ * t+64, ... in increasing c, then the pairwise tree s_t = s_t + s_{t+h} for
+ * h = 32, 16, ..., 1 (build spec: the order of the GPU's one-wavefront
+ * reduction; Chombo sums box by box). */
+double mlr_lin_dot(const mlr_grid* g, int ncomp, const double* x, const double* y) {
+  mlr_lay L;
+  mlr_layout(g, &L);
+  double s[DOT_LANES];
+  for (int t = 0; t < DOT_LANES; ++t) s[t] = 0.0;
+  long c = 0;
+  for (int n = 0; n < ncomp; ++n)
+    for (int k = 0; k < L.nz; ++k)
+      for (int j = 0; j < g->n[1]; ++j)
+        for (int i = 0; i < g->n[0]; ++i, ++c) {
+          const long q = mlr_ci(&L, i, j, k) + n * L.ncell;
+          s[c % DOT_LANES] = s[c % DOT_LANES] + x[q] * y[q];
+        }
+  for (int h = DOT_LANES / 2; h >= 1; h /= 2)
+    for (int t = 0; t < h; ++t) s[t] = s[t] + s[t + h];
+  return s[0];
+}
+
+/* DarcyBrinkmanOp::preCond (src/DarcyBrinkmanOp.cpp:89-121; VCAMRPoissonOp2's
+ * has the same form): phi = rhs*lambda on the valid cells, then relax(phi,
+ * rhs, 2) */
+void mlr_lin_precond(const mlr_grid* g, int ncomp, const mlr_bc* bc, double alpha,
+                     const double* a, double beta, const double* c,
+                     const double* const* b, const double* lambda, double* phi,
+                     const double* rhs) {
+  mlr_lay L;
+  mlr_layout(g, &L);
+  for (int n = 0; n < ncomp; ++n)
+    for (int k = 0; k < L.nz; ++k)
+      for (int j = 0; j < g->n[1]; ++j)
+        for (int i = 0; i < g->n[0]; ++i) {
+          const long q = mlr_ci(&L, i, j, k) + n * L.ncell;
+          phi[q] = rhs[q] * lambda[q];
+        }
+  for (int it = 0; it < 2; ++it) mlr_lin_gsrb(g, ncomp, bc, alpha, a, beta, c, b, lambda, phi, rhs);
+}
+
+/* y = y + s*x and y = s*y over every stored value */
+static void lin_incr(long n, double* y, const double* x, double s) {
+  for (long q = 0; q < n; ++q) y[q] = y[q] + s * x[q];
+}
+static void lin_scale(long n, double* y, double s) {
+  for (long q = 0; q < n; ++q) y[q] = s * y[q];
+}
+
+int mlr_lin_bicgstab(const mlr_grid* g, int ncomp, const mlr_bc* bc, double alpha,
+                     const double* a, double beta, const double* c,
+                     const double* const* b, const double* lambda, double* phi,
+                     const double* rhs, int imax, int* iters, double* final_norm) {
+  const double eps = 1e-6, reps = 1e-12, hang = 1e-8, small = 1e-30;
+  const int max_restarts = 5;
+  const long n = ncomp * mlr_ncell(g);
+  double* w = (double*)calloc((size_t)(8 * n), sizeof(double));
+  double *r = w, *rt = w + n, *e = w + 2 * n, *p = w + 3 * n, *pt = w + 4 * n, *st = w + 5 * n, *t = w + 6 * n,
+         *v = w + 7 * n;
+  mlr_lin_residual(g, ncomp, bc, alpha, a, beta, c, b, phi, rhs, r, 1);
+  memcpy(rt, r, sizeof(double) * n);
+  double norm0 = sqrt(mlr_lin_dot(g, ncomp, r, r)), norm1 = norm0;
+  const double r0 = norm0;
+  double rho0 = 0, rho1 = 0, alpha0 = 0, alpha1 = 0, omega0 = 0, omega1 = 0;
+  int init = 1, restarts = 0, recount = 0, i = 0, status = 0;
+  while (i < imax && norm0 > eps * norm1 && norm1 > 0) {
+    ++i;
+    norm1 = norm0;
+    alpha1 = alpha0;
+    omega1 = omega0;
+    rho1 = rho0;
+    rho0 = mlr_lin_dot(g, ncomp, rt, r);
+    if (rho0 == 0.0) {
+      status = 2;
+      break;
+    }
+    if (init) {
+      memcpy(p, r, sizeof(double) * n);
+      init = 0;
+    } else {
+      const double bt = (rho0 / rho1) * (alpha1 / omega1);
+      lin_scale(n, p, bt);
+      lin_incr(n, p, v, -bt * omega1);
+      lin_incr(n, p, r, 1.0);
+    }
+    mlr_lin_precond(g, ncomp, bc, alpha, a, beta, c, b, lambda, pt, p);
+    mlr_lin_apply_op(g, ncomp, bc, alpha, a, beta, c, b, pt, v, 1);
+    const double m = mlr_lin_dot(g, ncomp, rt, v);
+    alpha0 = rho0 / m;
+    if (fabs(m) > small * fabs(rho0)) {
+      lin_incr(n, r, v, -alpha0);
+      norm0 = sqrt(mlr_lin_dot(g, ncomp, r, r));
+      lin_incr(n, e, pt, alpha0);
+    } else {
+      memset(r, 0, sizeof(double) * n);
+      norm0 = 0.0;
+    }
+    if (norm0 > eps * r0 && norm0 > reps * r0) {
+      mlr_lin_precond(g, ncomp, bc, alpha, a, beta, c, b, lambda, st, r);
+      mlr_lin_apply_op(g, ncomp, bc, alpha, a, beta, c, b, st, t, 1);
+      omega0 = mlr_lin_dot(g, ncomp, t, r) / mlr_lin_dot(g, ncomp, t, t);
+      lin_incr(n, e, st, omega0);
+      lin_incr(n, r, t, -omega0);
+      norm0 = sqrt(mlr_lin_dot(g, ncomp, r, r));
+    }
+    if (norm0 <= eps * r0 || norm0 <= reps * r0) {
+      status = 1;
+      break;
+    }
+    if (omega0 == 0.0 || norm0 > (1.0 - hang) * norm1) {
+      if (recount == 0) {
+        recount = 1;
+      } else {
+        recount = 0;
+        lin_incr(n, phi, e, 1.0);
+        if (restarts == max_restarts) {
+          status = 3;
+          memset(e, 0, sizeof(double) * n);
+          break;
+        }
+        mlr_lin_residual(g, ncomp, bc, alpha, a, beta, c, b, phi, rhs, r, 1);
+        norm0 = sqrt(mlr_lin_dot(g, ncomp, r, r));
+        rho0 = alpha0 = omega0 = 0.0;
+        memcpy(rt, r, sizeof(double) * n);
+        memset(e, 0, sizeof(double) * n);
+        ++restarts;
+        init = 1;
+      }
+    }
+  }
+  if (status == 0) status = 2;
+  lin_incr(n, phi, e, 1.0);
+  if (iters) *iters = i;
+  if (final_norm) *final_norm = norm0;
+  free(w);
+  return status;
+}
+
 /* ---- linear multigrid (build spec for Chombo AMRMultiGrid::solve on one
  * AMR level and MultiGrid::cycle):
  *   res = rhs - L(phi) (inhomogeneous), r0 = |res|_inf
@@ -166,7 +331,9 @@ void mlr_lin_restrict_residual(const mlr_grid* gf, int ncomp,
  *                  res = rhs - L(phi); stop as the FAS solve does
  *   V-cycle on level l: relax(e, r, nu1); if l is not the coarsest:
  *     r_c = restrictResidual(e, r); e_c = 0; V-cycle(l+1); e += P e_c;
- *     relax(e, r, nu2);  else relax(e, r, bottom)
+ *     relax(e, r, nu2);  else relax(e, r, bottom), then (bottom_solver
+ *     BiCGStab) BiCGStab on L e = r from that e (MultiGrid's m_bottom
+ *     smoothings before the bottom solve, srcSubcycle/Projector.cpp:3031-3032)
  *   Coefficients of level l are arithmetic averages of the finest level's
  *   (DarcyBrinkmanOpFactory::MGnewOp, src/DarcyBrinkmanOp.cpp:977-1008).
  */
@@ -184,6 +351,8 @@ static void lin_vcycle(lin_level* lev, int l, int nlev, int ncomp,
   if (l == nlev - 1) {
     for (int it = 0; it < o->bottom; ++it)
       mlr_lin_gsrb(&L->g, ncomp, bc, alpha, L->a, beta, L->c, bb, L->lam, L->e, L->r);
+    if (o->bottom_solver == MLR_BOTTOM_BICGSTAB)
+      mlr_lin_bicgstab(&L->g, ncomp, bc, alpha, L->a, beta, L->c, bb, L->lam, L->e, L->r, o->bottom_max_iter, NULL, NULL);
     return;
   }
   for (int it = 0; it < o->nu1; ++it)

@@ -33,12 +33,17 @@ import numpy as np
 
 from oracle import mlref as O
 
-# multigrid options of the velocity solves: U* (VelocityMultigrid: smoothing
-# 2, 10 iterations, tolerance 1e-10) and the projection (Projector.cpp:45-64:
-# down 2, up 4, tolerance and hang 1e-15, 20 iterations); relaxation bottoms
-# (build spec)
-U_OPTS = dict(nu1=2, nu2=2, bottom=8, max_iter=10, eps=1e-10, hang=1e-10, norm_thresh=1e-10)
-P_OPTS = dict(nu1=2, nu2=4, bottom=4, max_iter=20, eps=1e-15, hang=1e-15, norm_thresh=1e-15)
+# multigrid options of the velocity solves: U* (VelocityMultigrid,
+# srcSubcycle/MushyLayerSubcycleUtils.cpp:654-659: smoothing 2 down, up and
+# before the bottom, 10 iterations, tolerance 1e-10; BiCGStab bottom with
+# Chombo's default 80 iterations, srcSubcycle/AMRLevelMushyLayerInit.cpp:739)
+# and the projection (srcSubcycle/Projector.cpp:53-63: down 2, up 4, 4 before
+# the bottom, tolerance and hang 1e-15, 20 iterations; BiCGStab bottom of 20
+# iterations, :3004-3023)
+U_OPTS = dict(nu1=2, nu2=2, bottom=2, max_iter=10, eps=1e-10, hang=1e-10, norm_thresh=1e-10,
+              bottom_solver=O.BOTTOM_BICGSTAB, bottom_max_iter=80)
+P_OPTS = dict(nu1=2, nu2=4, bottom=4, max_iter=20, eps=1e-15, hang=1e-15, norm_thresh=1e-15,
+              bottom_solver=O.BOTTOM_BICGSTAB, bottom_max_iter=20)
 # solidPorosity = ccVelPorosityLimit (srcSubcycle/MushyLayerSubcycleUtils.cpp:223-233)
 POROSITY_LIMIT = 0.05
 # hang rule off: a solve stops on eps, norm_thresh or max_iter only
