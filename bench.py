@@ -68,6 +68,8 @@ def parse():
                     help="grid of the smoother bandwidth measurement (0: skip)")
     ap.add_argument("--batch", type=int, default=0,
                     help="> 0: the batched form on this many independent instances (64^2 .. 512x256, 8:4:2:1)")
+    ap.add_argument("--smoother-only", action="store_true",
+                    help="only the cache-busting smoother measurement at --large-n (for counter runs)")
     ap.add_argument("--adjoint", action="store_true",
                     help="the adjoint workload: 128^2 heat-flux solidification, 50 steps, gradients and a fit")
     ap.add_argument("--fit-iters", type=int, default=6)
@@ -203,7 +205,14 @@ def field_errors(ml, ctx, case, sa, sb):
            "porosity_abs": float(np.max(np.abs(sa.porosity() - sb.porosity())))}
     fa, fb = flow_fields(sa), flow_fields(sb)
     for k in fa:
-        out[k] = rel_max(fa[k], fb[k])
+        if k == "p":
+            # the projection (periodic sides, Neumann walls) fixes the
+            # pressure up to a constant, which the BiCGStab bottom of a
+            # singular coarsest level leaves free: compare mean-free fields
+            out[k] = rel_max(fa[k] - fa[k].mean(), fb[k] - fb[k].mean())
+            out["p_mean_offset"] = float(abs(fa[k].mean() - fb[k].mean()) / max(np.abs(fb[k]).max(), 1e-300))
+        else:
+            out[k] = rel_max(fa[k], fb[k])
     return out
 
 
@@ -606,6 +615,12 @@ def main():
     if args.adjoint:
         run_adjoint(ml, wl, ml.Context(0), args)
         return
+    if args.smoother_only:
+        ctx = ml.Context(0)
+        large = smoother_bandwidth(ml, wl, ctx, args.large_n or 2048)
+        calibration_copy(ml, ctx)
+        print(json.dumps({"smoother_hbm": large}))
+        return
     dim = args.dim
     nn = args.n or (256 if dim == 2 else 128)
     shape = (nn,) * dim
@@ -663,8 +678,10 @@ def main():
                        "pressure start at 0" % ("x".join(map(str, shape)), what, corr),
            "form": head, "tolerance": "1e-10 rel or abs, max-norm",
            "max_iter": args.max_iter, "hang": "disabled (hang = -inf: a solve stops on eps/norm_thresh or max_iter only)",
-           "velocity_solves": {"ustar": "V(2,2), bottom 8, max 10, eps = hang = 1e-10, from the previous U*",
-                               "projection": "V(2,4), bottom 4, max 20, eps = hang = 1e-15, from pi = 0 each step"},
+           "velocity_solves": {"ustar": "V(2,2), 2 relaxations + BiCGStab (max 80) at the bottom, max 10, "
+                                         "eps = hang = 1e-10, from the previous U*",
+                               "projection": "V(2,4), 4 relaxations + BiCGStab (max 20) at the bottom, max 20, "
+                                             "eps = hang = 1e-15, from pi = 0 each step"},
            "advection": "HC advective source each step (computeScalarAdvectiveSrcHC): PLM Godunov predictor with "
                         "van Leer limiting and corner-transport corrections, advection velocity = face average of "
                         "the projected cell velocity u^n (the reference's MAC-projected predictor is not built)",
