@@ -132,7 +132,11 @@ def smoother_bandwidth(ml, wl, ctx, n, launches=20):
         st.op.relax(0, st.phi, st.old, 1)
     n1, ms, nb, _ = ctx.timing_stop()
     gbs = nb / (ms * 1e-3) / 1e9
-    out["hc_relax"] = {"kernel": "hc_relax_tile_kernel<double,2,32,8,1,2,256> (1 sweep per launch)",
+    # levels above 512^2 cells relax by streaming along y (both colour passes
+    # of a sweep pipelined in one launch), smaller ones in deep-halo tiles
+    hname = ("hc_relax_stream_kernel<double,2,252,1,2,256,false>" if n * n > 512 * 512 else
+             "hc_relax_tile_kernel<double,2,32,8,1,8,512>")
+    out["hc_relax"] = {"kernel": "%s (1 sweep per launch)" % hname,
                        "avg_launch_us": round(ms / n1 * 1e3, 2), "bytes_per_launch": nb / n1,
                        "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     st.op.close()

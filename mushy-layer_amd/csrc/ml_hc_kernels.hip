@@ -694,6 +694,290 @@ __global__ void hc_diffused_var_kernel(HCArgs<T> A, const T* __restrict__ phi, T
   }
 }
 
+// ---- streaming relaxation for large levels -------------------------------
+// The (H, Theta) counterpart of the linear operator's streaming kernel: a
+// workgroup owns a block of columns and a segment of planes along the last
+// direction and marches through them with a window of 4 planes in LDS
+// holding (H, Theta) and their closure (theta, Theta_l).  With plane R just
+// loaded (and its closure taken), colour pass p runs on plane R-1-p, so
+// plane R-P is final and is written.  A cell's pass reads its own state and
+// the closure of its other-colour neighbours, all as of the start of that
+// pass, exactly as in the tiled kernel; an updated cell's closure (and its
+// wall ghosts') is refreshed at once, which its same-colour cells never read
+// (after the launch's last pass nothing reads it, so it is skipped there).
+// Per plane and position the face coefficients, rhs (and aCoef) stay in
+// registers in compile-time-indexed sets, loaded two planes ahead; lambda is
+// formed where it is used.  Every input is read once per launch.
+template <typename T, int DIM>
+struct HCoef {
+  CellFaces<T, DIM> F;
+  T rr[2], aq[2], v[2];   // rhs, aCoef, state as loaded
+};
+
+template <typename T, int DIM, int TX, int TY, int P, int NT, bool HASA>
+__global__ __launch_bounds__(NT) void hc_relax_stream_kernel(HCArgs<T> A, const T* __restrict__ in,
+                                                             T* __restrict__ out, const T* __restrict__ rhs,
+                                                             int first, int nbx, int nby, int seg) {
+  if (A.skip && *A.skip) return;
+  constexpr int EX = TX + 2 * P;
+  constexpr int EL = EX * (DIM == 3 ? TY + 2 * P : 1);
+  constexpr int K = (EL + NT - 1) / NT;
+  static_assert(P >= 1 && P <= 2, "one or two colour passes per launch");
+  constexpr int W = 4;   // planes in the window
+  constexpr int MD = DIM - 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* su = reinterpret_cast<T*>(smem_raw);
+  const Geom& g = A.g;
+  const int nz = g.n[MD];
+  const bool zper = g.periodic[MD] != 0;
+  const T dxinv = T(1) / (T(g.dx) * T(g.dx));
+  const T dxg = T(g.dx);
+  int b = blockIdx.x;
+  const int bx = b % nbx;
+  b /= nbx;
+  const int by = DIM == 3 ? b % nby : 0;
+  const int bz = DIM == 3 ? b / nby : b;
+  const int x0 = bx * TX, y0 = by * TY;
+  const int z0 = bz * seg;
+  const int z1 = z0 + seg < nz ? z0 + seg : nz;
+  SPos S[K];
+  int gt[K][2];   // a lateral ghost's BC type and value per component
+  T gval[K][2];
+#pragma unroll
+  for (int kk = 0; kk < K; ++kk) {
+    const int e = threadIdx.x + kk * NT;
+    gt[kk][0] = gt[kk][1] = 0;
+    gval[kk][0] = gval[kk][1] = T(0);
+    if (e < EL) {
+      spos_init<DIM, TX, TY, P>(g, x0, y0, e, S[kk]);
+      for (int d = 0; d < DIM - 1; ++d)
+        for (int sd = 0; sd < 2; ++sd)
+          if (S[kk].kind == 2 && S[kk].gd == d && S[kk].gs == sd)
+            for (int n = 0; n < 2; ++n) {
+              gt[kk][n] = A.bc.type[d][sd][n];
+              gval[kk][n] = T(A.bc.val[d][sd][n]);
+            }
+    } else {
+      S[kk].kind = 0;
+      S[kk].dist = P;
+      S[kk].owner = 0;
+    }
+  }
+  // derived-variable ghost beyond wall (d, sd) of component n from the
+  // interior closure value x (computeDiffusedVar's BC)
+  auto ghost = [&](int d, int sd, int n, T x) {
+    return A.bc.apply ? bc_ghost<T>(A.bc.type[d][sd][n], T(A.bc.val[d][sd][n]), x, sd ? 1 : -1, dxg) : x;
+  };
+  auto zwrap = [&](int z) { return z < 0 ? z + nz : (z >= nz ? z - nz : z); };
+  auto real_plane = [&](int z) { return zper || (z >= 0 && z < nz); };
+  // 32-bit element offsets (levels below 2^28 cells per component)
+  const unsigned csz = unsigned(DIM == 3 ? g.sz : g.sy);
+  const unsigned cn = unsigned(g.ncell);
+  unsigned fpl[DIM], fup[DIM], fn[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    fpl[d] = unsigned(g.fn[d][0] * (DIM == 3 ? g.fn[d][1] : 1));
+    fn[d] = unsigned(g.nface[d]);
+  }
+  fup[0] = 1;
+  if (DIM == 3) fup[1] = unsigned(g.fn[1][0]);
+  fup[MD] = fpl[MD];
+  unsigned cl[K], fl[K][DIM];
+#pragma unroll
+  for (int kk = 0; kk < K; ++kk) {
+    const SPos& s = S[kk];
+    cl[kk] = unsigned(s.gx + 1) + (DIM == 3 ? unsigned(g.sy) * unsigned(s.gy + 1) : 0u);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      fl[kk][d] = unsigned(s.gx) + (DIM == 3 ? unsigned(g.fn[d][0]) * unsigned(s.gy) : 0u);
+  }
+  auto ld = [](const T* p, unsigned i) -> T {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + (i * unsigned(sizeof(T))));
+  };
+  constexpr int NS = P + 3;
+  HCoef<T, DIM> cs[NS][K];
+  auto prefetch = [&](int R, HCoef<T, DIM>(&pf)[K]) {
+    const int zw = zper ? zwrap(R) : (R < 0 ? 0 : (R >= nz ? nz - 1 : R));
+    const unsigned qz = unsigned(zw + 1) * csz;
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) {
+      const unsigned q = cl[kk] + qz;   // a ghost reads its interior neighbour
+#pragma unroll
+      for (int n = 0; n < 2; ++n) {
+        pf[kk].v[n] = ld(in, q + unsigned(n) * cn);
+        pf[kk].rr[n] = ld(rhs, q + unsigned(n) * cn);
+        if (HASA) pf[kk].aq[n] = ld(A.a, q + unsigned(n) * cn);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          const unsigned f = fl[kk][d] + unsigned(zw) * fpl[d] + unsigned(n) * fn[d];
+          pf[kk].F.v[n][2 * d] = ld(A.b[d], f);
+          pf[kk].F.v[n][2 * d + 1] = ld(A.b[d], f + fup[d]);
+        }
+      }
+    }
+  };
+  const int zl = (!zper && z0 - P < 0) ? 0 : z0 - P;
+  const int zend = z1 - 1 + P;
+  // window slot of plane R: (H, Theta, theta, Theta_l) of EL positions each
+  // (the bottom wall's ghost plane -1 is slot 3)
+  auto slot = [&](int R) { return su + ((R - zl) & (W - 1)) * (4 * EL); };
+  auto step = [&](auto J, int R) {
+    constexpr int j = decltype(J)::value;
+    // A. commit plane R: state, closure, lateral ghosts (or the top wall's
+    // ghost plane)
+    T* pl = slot(R);
+    if (real_plane(R)) {
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) {
+        const SPos& s = S[kk];
+        const int e = threadIdx.x + kk * NT;
+        if (s.kind == 0) continue;
+        const T h = cs[j][kk].v[0], c = cs[j][kk].v[1];
+        T th, clv;
+        closure<T>(h, c, A.cp, th, clv);
+        if (s.kind == 1) {
+          pl[e] = h;
+          pl[EL + e] = c;
+          pl[2 * EL + e] = th;
+          pl[3 * EL + e] = clv;
+          if (!zper && R == 0) {
+            T* gp = slot(-1);
+            gp[2 * EL + e] = ghost(MD, 0, 0, th);
+            gp[3 * EL + e] = ghost(MD, 0, 1, clv);
+          }
+        } else {
+          const int isign = s.gs == 0 ? -1 : 1;
+          pl[2 * EL + e] = A.bc.apply ? bc_ghost<T>(gt[kk][0], gval[kk][0], th, isign, dxg) : th;
+          pl[3 * EL + e] = A.bc.apply ? bc_ghost<T>(gt[kk][1], gval[kk][1], clv, isign, dxg) : clv;
+        }
+      }
+    } else if (!zper && R == nz) {
+      const T* below = slot(nz - 1);
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) {
+        const int e = threadIdx.x + kk * NT;
+        if (S[kk].kind != 1) continue;
+        pl[2 * EL + e] = ghost(MD, 1, 0, below[2 * EL + e]);
+        pl[3 * EL + e] = ghost(MD, 1, 1, below[3 * EL + e]);
+      }
+    }
+    // B. loads of plane R+2 into the set of plane R+2-NS, no longer in use
+    constexpr int jn = (j + 2) % NS;
+    prefetch(R + 2, cs[jn]);
+    __syncthreads();
+    // C. the colour passes; pass p on plane R-1-p uses set (j-1-p) mod NS
+    auto pass = [&](auto PP) {
+      constexpr int p = decltype(PP)::value;
+      constexpr int jp = (j - 1 - p + 2 * NS) % NS;
+      const int z = R - 1 - p;
+      const int reach = P - 1 - p;
+      const bool run = z >= z0 - reach && z < z1 + reach && real_plane(z);
+      if (run) {
+        const int zw = zwrap(z);
+        const int colour = (first + p) & 1;
+        T* pz = slot(z);
+        const T* pm = slot(z - 1);
+        const T* pp = slot(z + 1);
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+          const SPos& s = S[kk];
+          if (s.kind != 1 || s.dist > reach || ((s.gx + s.gy + zw + colour) & 1)) continue;
+          const int e = threadIdx.x + kk * NT;
+          const HCoef<T, DIM>& C = cs[jp][kk];
+          T nv[2];
+#pragma unroll
+          for (int n = 0; n < 2; ++n) {
+            const T* lz = pz + (2 + n) * EL;
+            T st[7];
+            st[0] = lz[e];
+            st[1] = lz[e + 1];
+            st[2] = lz[e - 1];
+            if (DIM == 3) {
+              st[3] = lz[e + EX];
+              st[4] = lz[e - EX];
+              st[5] = pp[(2 + n) * EL + e];
+              st[6] = pm[(2 + n) * EL + e];
+            } else {
+              st[3] = pp[(2 + n) * EL + e];
+              st[4] = pm[(2 + n) * EL + e];
+            }
+            const T ph = pz[n * EL + e];
+            const T aq = HASA ? C.aq[n] : T(1);
+            const T lam = lambda_of<T, DIM>(C.F, n, aq, A.alpha, A.beta, dxinv);
+            const T lof = lof_of<T, DIM>(C.F, n, st, aq, ph, A.alpha, A.beta, dxinv);
+            nv[n] = ph - lam * (lof - C.rr[n]);
+          }
+          pz[e] = nv[0];
+          pz[EL + e] = nv[1];
+          // the last pass's closure is never read: its cells' neighbours
+          // (the other colour) are not updated again in this launch
+          if constexpr (p == P - 1) continue;
+          T th, clv;
+          closure<T>(nv[0], nv[1], A.cp, th, clv);
+          pz[2 * EL + e] = th;
+          pz[3 * EL + e] = clv;
+          // wall ghosts of this cell: lateral, then along the march
+          const int glc[2] = {s.gx, s.gy};
+#pragma unroll
+          for (int d = 0; d < (DIM == 3 ? 2 : 1); ++d) {
+            if (g.periodic[d]) continue;
+            const int st1 = d == 0 ? 1 : EX;
+            if (glc[d] == 0) {
+              pz[2 * EL + e - st1] = ghost(d, 0, 0, th);
+              pz[3 * EL + e - st1] = ghost(d, 0, 1, clv);
+            }
+            if (glc[d] == g.n[d] - 1) {
+              pz[2 * EL + e + st1] = ghost(d, 1, 0, th);
+              pz[3 * EL + e + st1] = ghost(d, 1, 1, clv);
+            }
+          }
+          if (!zper) {
+            if (zw == 0) {
+              T* gp = slot(-1);
+              gp[2 * EL + e] = ghost(MD, 0, 0, th);
+              gp[3 * EL + e] = ghost(MD, 0, 1, clv);
+            }
+            if (zw == nz - 1) {
+              T* gp = slot(nz);
+              gp[2 * EL + e] = ghost(MD, 1, 0, th);
+              gp[3 * EL + e] = ghost(MD, 1, 1, clv);
+            }
+          }
+        }
+      }
+      __syncthreads();
+    };
+    pass(IC<0>{});
+    if constexpr (P > 1) pass(IC<1>{});
+    // D. plane R-P is final
+    const int zs = R - P;
+    if (zs >= z0 && zs < z1) {
+      const T* ps = slot(zs);
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) {
+        if (!S[kk].owner) continue;
+        const unsigned q = cl[kk] + unsigned(zs + 1) * csz;
+        const int e = threadIdx.x + kk * NT;
+        *reinterpret_cast<T*>(reinterpret_cast<char*>(out) + q * unsigned(sizeof(T))) = ps[e];
+        *reinterpret_cast<T*>(reinterpret_cast<char*>(out) + (q + cn) * unsigned(sizeof(T))) = ps[EL + e];
+      }
+    }
+  };
+  prefetch(zl, cs[0]);
+  prefetch(zl + 1, cs[1]);
+  for (int R = zl; R <= zend; R += NS) {
+    step(IC<0>{}, R);
+    if (R + 1 <= zend) step(IC<1>{}, R + 1);
+    if (R + 2 <= zend) step(IC<2>{}, R + 2);
+    if constexpr (NS > 3) {
+      if (R + 3 <= zend) step(IC<3 % NS>{}, R + 3);
+    }
+    if constexpr (NS > 4) {
+      if (R + 4 <= zend) step(IC<4 % NS>{}, R + 4);
+    }
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
@@ -813,9 +1097,105 @@ static hipError_t launch_relax_tile(const HCArgs<T>& A, TileIO<T> io, hipStream_
   return err;
 }
 
+// streaming launch shapes: 2D columns of 252 cells (256 threads), 3D blocks
+// of 12 x 12 columns
+template <int DIM>
+struct HCStream;
+template <>
+struct HCStream<2> {
+  static constexpr int NT = 256, TY = 1, SEGMIN = 8;
+};
+template <>
+struct HCStream<3> {
+  static constexpr int NT = 256, TY = 12, SEGMIN = 8;
+};
+
+// levels whose plain relaxations stream (2D above 512^2 cells, 3D above
+// 96^3), with 32-bit offsets for both components of every array
+constexpr long long kHCStreamCells2 = 512 * 512;
+constexpr long long kHCStreamCells3 = 96LL * 96 * 96;
+inline bool hc_streams(const Geom& g) {
+  long long nf = 0;
+  for (int d = 0; d < g.dim; ++d) nf = nf > g.nface[d] ? nf : g.nface[d];
+  const long long most = 2 * (g.ncell > nf ? g.ncell : nf);
+  return g.nvalid() > (g.dim == 2 ? kHCStreamCells2 : kHCStreamCells3) && g.n[g.dim - 1] >= 8 && most < (1LL << 29);
+}
+
+template <typename T, int DIM, int P, bool HASA>
+hipError_t hc_stream_launch(const HCArgs<T>& A, const T* in, T* out, const T* rhs, int first, hipStream_t s,
+                            const LaunchTimer* tm) {
+  using D = HCStream<DIM>;
+  constexpr int TX = DIM == 2 ? D::NT - 2 * P : D::TY;
+  constexpr int TY = DIM == 2 ? 1 : D::TY;
+  constexpr int EL = (TX + 2 * P) * (DIM == 3 ? TY + 2 * P : 1);
+  const Geom& g = A.g;
+  const int nbx = (g.n[0] + TX - 1) / TX;
+  const int nby = DIM == 3 ? (g.n[1] + TY - 1) / TY : 1;
+  const size_t lds = size_t(4) * 4 * EL * sizeof(T);
+  auto kern = hc_relax_stream_kernel<T, DIM, TX, TY, P, D::NT, HASA>;
+  // segments sized so that the whole grid is resident at once
+  static int resident = 0;
+  if (!resident) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
+                                                 hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(kern), D::NT, lds) ==
+            hipSuccess &&
+        cus > 0 && per > 0)
+      resident = cus * per;
+    else
+      resident = 768;
+  }
+  const int nz = g.n[DIM - 1];
+  const int columns = nbx * nby;
+  const int per_column = resident / columns > 1 ? resident / columns : 1;
+  int seg = (nz + per_column - 1) / per_column;
+  if (seg < D::SEGMIN) seg = D::SEGMIN;
+  const int nbz = (nz + seg - 1) / seg;
+  const bool timed = tm && *tm->n < tm->cap;
+  if (tm && !timed) ++*tm->missed;
+  hipExtLaunchKernelGGL(kern, dim3(nbx * nby * nbz), dim3(D::NT), lds, s, timed ? tm->start[*tm->n] : nullptr,
+                        timed ? tm->stop[*tm->n] : nullptr, 0, A, in, out, rhs, first, nbx, nby, seg);
+  const hipError_t e = hipGetLastError();
+  if (e == hipSuccess && timed) {
+    const double cells = double(g.nvalid());
+    double faces = 0;
+    for (int d = 0; d < DIM; ++d) faces += double(g.nface[d]);
+    const double by = 2 * cells + 2 * cells + (HASA ? 2 * cells : 0) + 2 * faces + 2 * cells;
+    *tm->bytes += by * sizeof(T);
+    ++*tm->n;
+  }
+  return e;
+}
+
+template <typename T, int DIM>
+hipError_t hc_relax_stream(const HCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, int first,
+                           hipStream_t s, const LaunchTimer* tm) {
+  int done = 0;
+  while (done < npass) {
+    const int p = (npass - done) >= 2 ? 2 : 1;
+    const int f = (first + done) & 1;
+    hipError_t e;
+    if (A.a)
+      e = p == 2 ? hc_stream_launch<T, DIM, 2, true>(A, buf[*cur], buf[1 - *cur], rhs, f, s, tm)
+                 : hc_stream_launch<T, DIM, 1, true>(A, buf[*cur], buf[1 - *cur], rhs, f, s, tm);
+    else
+      e = p == 2 ? hc_stream_launch<T, DIM, 2, false>(A, buf[*cur], buf[1 - *cur], rhs, f, s, tm)
+                 : hc_stream_launch<T, DIM, 1, false>(A, buf[*cur], buf[1 - *cur], rhs, f, s, tm);
+    if (e != hipSuccess) return e;
+    *cur = 1 - *cur;
+    done += p;
+  }
+  return hipSuccess;
+}
+
 template <typename T>
 hipError_t hc_relax_tiles(const HCArgs<T>& A, T* const buf[2], int* cur, const T* rhs, int npass, int first,
                           const Geom* gc, const T* pc, const T* pc0, hipStream_t s, const LaunchTimer* tm) {
+  // plain relaxations of large levels stream along the last direction
+  if (!pc && npass > 0 && hc_streams(A.g))
+    return A.g.dim == 2 ? hc_relax_stream<T, 2>(A, buf, cur, rhs, npass, first, s, tm)
+                        : hc_relax_stream<T, 3>(A, buf, cur, rhs, npass, first, s, tm);
   const int pmax = A.g.dim == 2 ? relax_passes_per_launch<T, 2>(A.g) : relax_passes_per_launch<T, 3>(A.g);
   int done = 0;
   while (done < npass) {

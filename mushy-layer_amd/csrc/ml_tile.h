@@ -303,4 +303,72 @@ __device__ __forceinline__ void clev_nbrs(const CLev& L, const int* per, int i, 
   }
 }
 
+// ---- streaming (plane-marching) kernels ----------------------------------
+// A workgroup owns a block of columns (2D: TX cells in x; 3D: TX x TY cells
+// in (x, y)) extended by a lateral halo of P cells, and marches along the
+// last direction.  IC<N> passes a compile-time index to a generic lambda.
+template <int N>
+struct IC {
+  static constexpr int value = N;
+};
+
+// lateral position of the extended block: global coordinates (wrapped in
+// periodic directions), kind 1 cell / 2 wall ghost / 0 unused, Chebyshev
+// distance to the core, owner flag, and for a ghost its wall (direction,
+// side) and the LDS offset of its interior neighbour
+struct SPos {
+  int gx, gy;
+  int kind, dist, owner;
+  int gd, gs, eint;
+  int gtype;   // a ghost's boundary type (resolved once: a per-thread index
+               // into the kernel arguments would be a memory load)
+};
+
+template <int DIM, int TX, int TY, int P>
+__device__ inline void spos_init(const Geom& g, int x0, int y0, int e, SPos& s) {
+  constexpr int EX = TX + 2 * P;
+  const int ex = e % EX, ey = e / EX;
+  const int lc[2] = {ex - P, DIM == 3 ? ey - P : 0};
+  const int org[2] = {x0, y0};
+  const int nl = DIM == 3 ? 2 : 1;
+  s.kind = 1;
+  s.dist = 0;
+  s.owner = 1;
+  s.gd = -1;
+  s.gs = 0;
+  s.eint = e;
+  int gv[2] = {0, 0};
+  int outside = 0;
+  for (int d = 0; d < nl; ++d) {
+    const int ext = d == 0 ? TX : TY;
+    const int l = lc[d];
+    const int dd = l < 0 ? -l : (l >= ext ? l - ext + 1 : 0);
+    if (dd > s.dist) s.dist = dd;
+    int v = org[d] + l;
+    const int n = g.n[d];
+    if (v >= n || v < 0) s.owner = 0;
+    if (v >= 0 && v < n) {
+      gv[d] = v;
+    } else if (g.periodic[d]) {
+      gv[d] = v < 0 ? v + n : v - n;
+    } else if (v == -1 || v == n) {
+      ++outside;
+      s.gd = d;
+      s.gs = v == n ? 1 : 0;
+      gv[d] = v == n ? n - 1 : 0;   // the interior neighbour
+      s.eint = e + (v == n ? -1 : 1) * (d == 0 ? 1 : EX);
+    } else {
+      outside += 2;
+    }
+  }
+  if (s.dist > 0) s.owner = 0;
+  if (outside == 1) s.kind = 2;
+  else if (outside > 1) s.kind = 0;
+  // every position gets in-domain coordinates (unused ones clamped), so the
+  // prefetch can load unconditionally
+  for (int d = 0; d < nl; ++d) gv[d] = gv[d] < 0 ? 0 : (gv[d] >= g.n[d] ? g.n[d] - 1 : gv[d]);
+  s.gx = gv[0];
+  s.gy = gv[1];
+}
+
 }  // namespace ml

@@ -184,8 +184,19 @@ def test_be_step_parity(ml, gpu_ctx, wl, name, max_iter, driver):
     assert HC.bitwise_equal(HC.valid(n, out), HC.valid(n, new))
 
 
+WALLS = {(0, 0, 0): (1, 0.3), (0, 1, 0): (0, 0.5), (0, 0, 1): (1, -0.2), (0, 1, 1): (0, -1.0),
+         (1, 0, 0): (0, -1.0), (1, 1, 0): (0, 1.2), (1, 0, 1): (0, 0.0), (1, 1, 1): (1, 0.1),
+         (2, 0, 0): (0, -0.5), (2, 1, 0): (1, 0.2), (2, 0, 1): (1, 0.0), (2, 1, 1): (0, -0.8)}
 LARGE = {
     "2d_512": dict(n=(512, 512)),
+    # levels above 512^2 (2D) / 96^3 (3D) cells relax by streaming along the
+    # last direction: walls with inhomogeneous derived-variable values on
+    # every side, periodic march direction, ragged blocks and segments
+    "2d_700x600_walls": dict(n=(700, 600), periodic=[0, 0], bc=WALLS),
+    "2d_530x520_periodic": dict(n=(530, 520), periodic=[1, 1]),
+    "2d_1030x261_no_derived_bc": dict(n=(1030, 261), apply_bc=0),
+    "3d_100x98x97_walls_xz": dict(n=(100, 98, 97), periodic=[0, 1, 0], bc=WALLS),
+    "3d_100_periodic_z": dict(n=(100, 100, 100), periodic=[1, 0, 1]),
     "2d_1024x256": dict(n=(1024, 256), periodic=[0, 0]),
     # more than 1024 tiles: one sweep per launch on the wide tiles
     "2d_1024x512": dict(n=(1024, 512)),
