@@ -172,6 +172,8 @@ __global__ __launch_bounds__(NT) void hc_relax_tile_kernel(HCArgs<T> A, TileIO<T
       }
       sphi[e] = nv[0];
       sphi[E + e] = nv[1];
+      // after the last pass only the state is written out
+      if (p == io.npass - 1) continue;
       T th, cl;
       closure<T>(nv[0], nv[1], A.cp, th, cl);
       sphl[e] = th;
