@@ -298,6 +298,28 @@ int ml_vc_solve_be(ml_vc_op op, double* phi_new, const double* phi_old, const do
  * them).  Solve i gives exactly the result of ml_vc_solve_be on
  * (phi_new[i], phi_old[i], src ? src[i] : NULL); stats[i] its statistics. */
 enum { ML_VC_MAX_SOLVES = 4 };
+
+/* ---- batched linear solves (no counterpart in the reference: its runs
+ * advance one mushy layer per process) ----------------------------------
+ * nops 2D operators (distinct, one context; every level at most 512x512
+ * cells and at least one level above the single-workgroup coarse level)
+ * with per_op (<= ML_VC_MAX_SOLVES) solves each: unit u is solve
+ * u % per_op of ops[u / per_op].  One call iterates every unit together,
+ * each launch covering all units (2K + 1 launches per iteration, K the
+ * deepest tiled-level count), and gives every unit exactly the states and
+ * statistics of ml_vc_solve_many / ml_vc_solve_be_many on its operator
+ * (the U* solves of many Darcy-Brinkman instances, or their projections).
+ * Needs nu2 >= 1 and 2*nu + 1 passes within one relaxation launch. */
+typedef struct ml_vc_batch_s* ml_vc_batch;
+int ml_vc_batch_create(ml_ctx ctx, int nops, const ml_vc_op ops[], int per_op, ml_vc_batch* out);
+int ml_vc_batch_destroy(ml_vc_batch b);
+/* phi[u] updated in place from its value, rhs[u]; stats[u] may be NULL */
+int ml_vc_batch_solve(ml_vc_batch b, double* const phi[], const double* const rhs[], const ml_solve_opts* opts,
+                      ml_solve_stats stats[]);
+/* backward-Euler form: alpha = 1, beta = -dt on every operator, rhs_u =
+ * phi_old[u] + dt*src[u] (src or src[u] may be NULL), solved from phi_new[u] */
+int ml_vc_batch_solve_be(ml_vc_batch b, double* const phi_new[], const double* const phi_old[],
+                         const double* const src[], double dt, const ml_solve_opts* opts, ml_solve_stats stats[]);
 int ml_vc_solve_be_many(ml_vc_op op, int n, double* const phi_new[], const double* const phi_old[],
                         const double* const src[], double dt, const ml_solve_opts* opts,
                         ml_solve_stats* stats);

@@ -181,6 +181,11 @@ SIGNATURES = {
     "ml_hc_batch_set_tile": (C.c_int, [_vp, C.c_int]),
     "ml_hc_batch_tile": (C.c_int, [_vp, _ip, _ip, _ip, _ip, C.POINTER(C.c_float), C.c_int]),
     "ml_hc_batch_step": (C.c_int, [_vp, C.POINTER(_vp), C.c_double, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "ml_vc_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.c_int, C.POINTER(_vp)]),
+    "ml_vc_batch_destroy": (C.c_int, [_vp]),
+    "ml_vc_batch_solve": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "ml_vc_batch_solve_be": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.c_double,
+                                       C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "ml_hc_flux_source": (C.c_int, [_vp, _vp, _vp]),
     "ml_hc_adjoint_step": (C.c_int, [_vp, _vp, _vp, C.c_double, _vp, _vp, _vp, _dp, C.POINTER(SolveOpts),
                                      C.POINTER(SolveStats)]),
@@ -628,6 +633,48 @@ class HCBatch:
     def close(self):
         if self.h:
             self.ctx.lib.ml_hc_batch_destroy(self.h)
+            self.h = None
+
+
+class VCBatch:
+    """Batched linear solves: `per_op` solves on each of many VCOps (the U*
+    components or the projection of many instances) iterate together, every
+    launch covering all of them; unit u is solve u % per_op of ops[u //
+    per_op], and gets exactly the result and statistics of solve_many /
+    solve_be_many on its operator."""
+
+    def __init__(self, ctx, ops, per_op=1):
+        self.ctx = ctx
+        self.ops = list(ops)
+        self.per_op = per_op
+        n = len(self.ops)
+        h = _vp()
+        ctx.check(ctx.lib.ml_vc_batch_create(ctx.h, n, (_vp * n)(*[op.h for op in self.ops]), per_op, C.byref(h)),
+                  "ml_vc_batch_create")
+        self.h = h
+
+    def solve(self, phis, rhss, opts=None):
+        n = len(phis)
+        st = (SolveStats * n)()
+        o = opts or default_opts()
+        self.ctx.check(self.ctx.lib.ml_vc_batch_solve(self.h, (_vp * n)(*[x.ptr for x in phis]),
+                                                      (_vp * n)(*[x.ptr for x in rhss]), C.byref(o), st),
+                       "ml_vc_batch_solve")
+        return list(st)
+
+    def solve_be(self, news, olds, dt, srcs=None, opts=None):
+        n = len(news)
+        st = (SolveStats * n)()
+        o = opts or default_opts()
+        src = (_vp * n)(*[x.ptr if x is not None else None for x in srcs]) if srcs is not None else None
+        self.ctx.check(self.ctx.lib.ml_vc_batch_solve_be(self.h, (_vp * n)(*[x.ptr for x in news]),
+                                                         (_vp * n)(*[x.ptr for x in olds]), src, dt, C.byref(o), st),
+                       "ml_vc_batch_solve_be")
+        return list(st)
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.ml_vc_batch_destroy(self.h)
             self.h = None
 
 

@@ -597,6 +597,22 @@ struct VCLevel {
   T* r;   // residual right-hand side
 };
 
+// one (instance, level) relaxation of a batched launch (the arguments of
+// vc_relax_tile_kernel)
+template <typename T>
+struct VCRelaxJob {
+  VCArgs<T> A;
+  const T* in;
+  T* out;
+  const T* rhs;
+  int npass, first, zero_init, ntx, nty;
+  Geom gc;
+  const T* pc;
+  Geom gr;
+  T* resc;
+  VCFinal<T> fin;
+};
+
 template <typename T>
 struct VCDesc {
   int l0, nlev, nu1, nu2, bottom;
@@ -644,6 +660,20 @@ hipError_t vc_lambda_mult(const VCArgs<T>& A, const T* rhs, T* phi, hipStream_t 
 // *out = (x, y) over the valid cells, in the BiCGStab bottom's fixed order
 template <typename T>
 hipError_t vc_dot(const Geom& g, const T* x, const T* y, T* out, hipStream_t s);
+// batched form: relaxation launches whose workgroups are (job, tile) pairs
+// (map[w] = job << kBatchTileBits | tile), every job on the <= 512^2-cell
+// tile configuration; one coarse V-cycle workgroup per descriptor
+bool vc_batch_level_ok(const Geom& g);
+int vc_batch_hmax(int dim);
+int vc_batch_tiles(const Geom& g, int* ntx, int* nty);
+size_t vc_batch_lds(int dim, int npass, bool resc, bool fin);
+template <typename T>
+hipError_t vc_relax_batch(int dim, const VCRelaxJob<T>* jobs, const unsigned* map, int nblocks, size_t lds,
+                          hipStream_t s, const LaunchTimer* tm = nullptr);
+template <typename T>
+hipError_t vc_coarse_batch(int dim, const VCDesc<T>* const* descs, int n, size_t lds, hipStream_t s);
+// out[i] = *ctls[i] (the iteration control blocks of many solves, for one copy to the host)
+hipError_t ctl_gather(SolveCtl* const* ctls, SolveCtl* out, int n, hipStream_t s);
 // dynamic LDS of vc_coarse_vcycle for levels l0..nlev-1 of g
 template <typename T>
 size_t vc_coarse_lds_bytes(const Geom* g, int l0, int nlev, bool bicgstab = false);

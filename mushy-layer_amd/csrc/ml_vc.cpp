@@ -14,6 +14,7 @@
 // Tiled kernels serve the levels above `lc`; one single-workgroup kernel
 // runs the V-cycle from `lc` down.
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 
@@ -182,7 +183,7 @@ int vc_work_ensure(ml_vc_op op, int n) {
   return ML_OK;
 }
 
-int vc_upload_desc(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o) {
+int vc_upload_desc(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o, hipStream_t stream = nullptr) {
   ml_ctx ctx = op->ctx;
   ml::VCDesc<double> D;
   std::memset(&D, 0, sizeof(D));
@@ -207,7 +208,7 @@ int vc_upload_desc(ml_vc_op op, ml::VCWork& W, const ml_solve_opts& o) {
     L.e = W.e0[l];
     L.r = W.r[l];
   }
-  ML_HIP(ctx, W.stage.put(W.desc, W.hdesc, &D, sizeof(D), W.stream));
+  ML_HIP(ctx, W.stage.put(W.desc, W.hdesc, &D, sizeof(D), stream ? stream : W.stream));
   return ML_OK;
 }
 
@@ -758,6 +759,367 @@ int ml_vc_porosity_faces(ml_vc_op op, const double* chi, double scale, double* c
     if (!bCoef[d]) return ML_EINVAL;
   ML_HIP(op->ctx, ml::porosity_faces(op->lev[0].g, scale, chi, bCoef, op->ctx->stream));
   return ML_OK;
+}
+
+}  // extern "C"
+
+// ---- batched linear solves -----------------------------------------------
+// Many solve units (an operator and one of its workspaces each) iterate
+// together: per iteration the units' tiled levels are aligned by their
+// distance from the single-workgroup level, so down-sweep launch d relaxes
+// and restricts level d - (K - t_u) of every unit with t_u tiled levels
+// (K = max t_u); one launch runs every unit's coarse V-cycle (one workgroup
+// each); the up-sweep mirrors the down-sweep, its last launch forming every
+// unit's phi + e, residual, norm and stopping rule.  2K + 1 launches per
+// iteration for the whole batch.  A unit's launches are exactly those of its
+// single-unit solve (same kernels' bodies, same buffers, same stopping
+// rule), so results and statistics are bit-identical to ml_vc_solve_many.
+
+namespace ml {
+namespace {
+
+struct VCBatchUnit {
+  ml_vc_op op = nullptr;
+  int w = 0;          // workspace index
+  int tiled = 0;      // tiled levels (lc)
+};
+
+struct VCBatchLaunch {
+  size_t job_off = 0, map_off = 0;
+  int blocks = 0;
+  size_t lds = 0;
+};
+
+}  // namespace
+}  // namespace ml
+
+struct ml_vc_batch_s {
+  ml_ctx ctx = nullptr;
+  int dim = 2;
+  std::vector<ml::VCBatchUnit> units;
+  int K = 0;
+  size_t coarse_lds = 0;
+  // device tables: [jobs, maps of both iteration parities | coarse
+  // descriptor pointers | control pointers | gathered controls], one pinned mirror
+  unsigned char* arena = nullptr;
+  unsigned char* harena = nullptr;
+  size_t arena_cap = 0;
+  std::vector<ml::VCBatchLaunch> launch[2];   // per parity: K down, K up
+  size_t desc_off = 0, ctlp_off = 0, ctl_off = 0;
+  std::vector<const void*> key;   // what the uploaded tables were built for
+  int batch_iters = 4;
+
+  ~ml_vc_batch_s() {
+    if (arena) hipFree(arena);
+    if (harena) hipHostFree(harena);
+  }
+};
+
+namespace {
+
+using ml::fail;
+
+size_t vcb_align(size_t x) { return (x + 255) & ~size_t(255); }
+
+// the job and workgroup tables of both iteration parities, the coarse
+// descriptor and control pointers, built for phi/rhs and uploaded
+int vcb_tables(ml_vc_batch b, double* const phi[], const double* const rhs[], const ml_solve_opts& o) {
+  ml_ctx ctx = b->ctx;
+  const int n = int(b->units.size());
+  // the jobs depend on the state and rhs buffers, alpha and beta (through
+  // the operator arguments) and the smoothing counts
+  std::vector<const void*> key;
+  auto bits_of = [](double v) {
+    std::uint64_t x;
+    std::memcpy(&x, &v, sizeof(x));
+    return reinterpret_cast<const void*>(std::uintptr_t(x));
+  };
+  for (int u = 0; u < n; ++u) {
+    key.push_back(phi[u]);
+    key.push_back(rhs[u]);
+    key.push_back(bits_of(b->units[u].op->alpha));
+    key.push_back(bits_of(b->units[u].op->beta));
+  }
+  key.push_back(reinterpret_cast<const void*>(std::uintptr_t(o.nu1 * 1000 + o.nu2)));
+  if (key == b->key) return ML_OK;
+  b->key.clear();
+
+  const int K = b->K;
+  const int hmax = ml::vc_batch_hmax(b->dim);
+  std::vector<ml::VCRelaxJob<double>> jobs[2];
+  std::vector<unsigned> maps[2];
+  for (int par = 0; par < 2; ++par) {
+    b->launch[par].assign(2 * K, ml::VCBatchLaunch{});
+    for (int k = 0; k < 2 * K; ++k) {
+      const bool down = k < K;
+      const int d = down ? k : 2 * K - 1 - k;   // aligned depth of the launch
+      ml::VCBatchLaunch& Lh = b->launch[par][k];
+      Lh.job_off = jobs[par].size();
+      Lh.map_off = maps[par].size();
+      for (int u = 0; u < n; ++u) {
+        const ml::VCBatchUnit& U = b->units[u];
+        const int l = d - (K - U.tiled);
+        if (l < 0) continue;
+        ml_vc_op op = U.op;
+        ml::VCWork& W = op->work[U.w];
+        ml::VCRelaxJob<double> J{};
+        J.A = op->args(l);
+        J.A.skip = &W.ctl->done;
+        double* const r_in = par == 0 ? W.r[0] : W.r0alt;
+        double* const rnext = par == 0 ? W.r0alt : W.r[0];
+        double* const phi_in = par == 0 ? phi[u] : W.alt0;
+        double* const phi_out = par == 0 ? W.alt0 : phi[u];
+        J.rhs = l == 0 ? r_in : W.r[l];
+        J.first = 0;
+        J.gc = J.A.g;
+        J.gr = J.A.g;
+        if (down) {
+          // relax from zero, restrict r - L e into the next level's rhs
+          J.in = W.e0[l];
+          J.out = W.e1[l];
+          J.npass = 2 * o.nu1;
+          J.zero_init = 1;
+          J.gr = op->lev[l + 1].g;
+          J.resc = W.r[l + 1];
+        } else {
+          // prolong the next level's correction, relax; level 0 also forms
+          // phi + e, the residual and the stopping rule
+          J.in = W.e1[l];
+          J.out = W.e0[l];
+          J.npass = 2 * o.nu2;
+          J.zero_init = 0;
+          J.gc = op->lev[l + 1].g;
+          J.pc = W.e0[l + 1];
+        }
+        int ntx = 0, nty = 0;
+        const int nt = ml::vc_batch_tiles(J.A.g, &ntx, &nty);
+        J.ntx = ntx;
+        J.nty = nty;
+        if (!down && l == 0) {
+          J.fin.phi = phi_in;
+          J.fin.phi_out = phi_out;
+          J.fin.rhs = rhs[u];
+          J.fin.res = rnext;
+          J.fin.fuse = ml::CtlFuse{W.ctl, W.slots, W.arrive, 0, unsigned(nt)};
+        }
+        if (J.npass + 1 > hmax) return fail(ctx, ML_ENOTSUP, "batched solve: too many smoothing passes per launch");
+        const size_t lds = ml::vc_batch_lds(b->dim, J.npass, J.resc != nullptr, J.fin.phi != nullptr);
+        Lh.lds = std::max(Lh.lds, lds);
+        const unsigned jl = unsigned(jobs[par].size() - Lh.job_off);
+        for (int t = 0; t < nt; ++t) maps[par].push_back((jl << ml::kBatchTileBits) | unsigned(t));
+        jobs[par].push_back(J);
+      }
+      Lh.blocks = int(maps[par].size() - Lh.map_off);
+    }
+  }
+  // layout: jobs0 | maps0 | jobs1 | maps1 | descs | ctl pointers | gathered ctls
+  size_t off = 0;
+  size_t joff[2], moff[2];
+  for (int par = 0; par < 2; ++par) {
+    joff[par] = off;
+    off = vcb_align(off + jobs[par].size() * sizeof(ml::VCRelaxJob<double>));
+    moff[par] = off;
+    off = vcb_align(off + maps[par].size() * sizeof(unsigned));
+  }
+  b->desc_off = off;
+  off = vcb_align(off + n * sizeof(void*));
+  b->ctlp_off = off;
+  off = vcb_align(off + n * sizeof(void*));
+  b->ctl_off = off;
+  off = vcb_align(off + n * sizeof(ml::SolveCtl));
+  if (off > b->arena_cap) {
+    ML_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (b->arena) hipFree(b->arena);
+    if (b->harena) hipHostFree(b->harena);
+    b->arena = nullptr;
+    b->harena = nullptr;
+    b->arena_cap = 0;
+    void* p = nullptr;
+    ML_HIP(ctx, hipMalloc(&p, off));
+    b->arena = static_cast<unsigned char*>(p);
+    ML_HIP(ctx, hipHostMalloc(&p, off, hipHostMallocDefault));
+    b->harena = static_cast<unsigned char*>(p);
+    b->arena_cap = off;
+  } else {
+    // the previous upload may still be in flight from the pinned mirror
+    ML_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  for (int par = 0; par < 2; ++par) {
+    std::memcpy(b->harena + joff[par], jobs[par].data(), jobs[par].size() * sizeof(ml::VCRelaxJob<double>));
+    std::memcpy(b->harena + moff[par], maps[par].data(), maps[par].size() * sizeof(unsigned));
+    for (auto& Lh : b->launch[par]) {
+      Lh.job_off = joff[par] + Lh.job_off * sizeof(ml::VCRelaxJob<double>);
+      Lh.map_off = moff[par] + Lh.map_off * sizeof(unsigned);
+    }
+  }
+  void** dp = reinterpret_cast<void**>(b->harena + b->desc_off);
+  void** cp = reinterpret_cast<void**>(b->harena + b->ctlp_off);
+  for (int u = 0; u < n; ++u) {
+    ml::VCWork& W = b->units[u].op->work[b->units[u].w];
+    dp[u] = W.desc;
+    cp[u] = W.ctl;
+  }
+  ML_HIP(ctx, hipMemcpyAsync(b->arena, b->harena, b->ctl_off, hipMemcpyHostToDevice, ctx->stream));
+  b->key = key;
+  return ML_OK;
+}
+
+// one iteration of every unit (parity par = (iteration - 1) & 1)
+int vcb_iteration(ml_vc_batch b, int par) {
+  ml_ctx ctx = b->ctx;
+  hipStream_t s = ctx->stream;
+  const int K = b->K;
+  auto launch = [&](const ml::VCBatchLaunch& Lh) -> int {
+    ML_HIP(ctx, ml::vc_relax_batch<double>(b->dim, reinterpret_cast<const ml::VCRelaxJob<double>*>(b->arena + Lh.job_off),
+                                           reinterpret_cast<const unsigned*>(b->arena + Lh.map_off), Lh.blocks, Lh.lds,
+                                           s));
+    return ML_OK;
+  };
+  int rc;
+  for (int k = 0; k < K; ++k)
+    if ((rc = launch(b->launch[par][k]))) return rc;
+  ML_HIP(ctx, ml::vc_coarse_batch<double>(b->dim, reinterpret_cast<const ml::VCDesc<double>* const*>(b->arena + b->desc_off),
+                                          int(b->units.size()), b->coarse_lds, s));
+  for (int k = K; k < 2 * K; ++k)
+    if ((rc = launch(b->launch[par][k]))) return rc;
+  return ML_OK;
+}
+
+int vcb_solve(ml_vc_batch b, double* const phi[], const double* const rhs[], const ml_solve_opts& o,
+              ml_solve_stats* st) {
+  ml_ctx ctx = b->ctx;
+  hipStream_t s = ctx->stream;
+  const int n = int(b->units.size());
+  if (o.flags & ML_SOLVE_HOST_DRIVER) return fail(ctx, ML_ENOTSUP, "batched solve: no host driver");
+  if (o.nu2 < 1) return fail(ctx, ML_ENOTSUP, "batched solve: needs post-smoothing (nu2 >= 1)");
+  int rc;
+  for (int u = 0; u < n; ++u)
+    if ((rc = vc_solve_check(b->units[u].op, o))) return rc;
+  // coarse LDS of the bottom solver in use
+  b->coarse_lds = 0;
+  for (auto& U : b->units)
+    b->coarse_lds = std::max(b->coarse_lds, o.bottom_solver == ML_BOTTOM_BICGSTAB ? U.op->coarse_lds_bicg : U.op->coarse_lds);
+  // begin: descriptors, phi copies, initial residuals and control blocks
+  for (int u = 0; u < n; ++u) {
+    ml_vc_op op = b->units[u].op;
+    ml::VCWork& W = op->work[b->units[u].w];
+    if ((rc = vc_upload_desc(op, W, o, s))) return rc;
+    const ml::Geom& g0 = op->lev[0].g;
+    ML_HIP(ctx, hipMemcpyAsync(W.alt0, phi[u], sizeof(double) * g0.ncell, hipMemcpyDeviceToDevice, s));
+    ML_HIP(ctx, ml::zero_u64(W.slots, o.max_iter + 1, s));
+    ML_HIP(ctx, ml::vc_residual<double>(op->args(0), phi[u], nullptr, nullptr, rhs[u], W.r[0], W.slots, 0, s));
+    ML_HIP(ctx, ml::ctl_init(W.ctl, W.slots, o.max_iter, o.eps, o.hang, o.norm_thresh, s));
+  }
+  if ((rc = vcb_tables(b, phi, rhs, o))) return rc;
+  std::vector<ml::SolveCtl> c(n);
+  int launched = 0;
+  int nb = std::max(1, std::min(b->batch_iters, o.max_iter));
+  for (;;) {
+    for (int i = 0; i < nb && launched < o.max_iter; ++i) {
+      ++launched;
+      if ((rc = vcb_iteration(b, (launched - 1) & 1))) return rc;
+    }
+    ML_HIP(ctx, ml::ctl_gather(reinterpret_cast<ml::SolveCtl* const*>(b->arena + b->ctlp_off),
+                               reinterpret_cast<ml::SolveCtl*>(b->arena + b->ctl_off), n, s));
+    ML_HIP(ctx, hipMemcpyAsync(c.data(), b->arena + b->ctl_off, n * sizeof(ml::SolveCtl), hipMemcpyDeviceToHost, s));
+    ML_HIP(ctx, hipStreamSynchronize(s));
+    bool all = true;
+    for (int u = 0; u < n; ++u) all = all && c[u].done;
+    if (all || launched >= o.max_iter) break;
+    nb = 2;
+  }
+  int most = 1;
+  for (int u = 0; u < n; ++u) {
+    ml_vc_op op = b->units[u].op;
+    ml::VCWork& W = op->work[b->units[u].w];
+    most = std::max(most, c[u].iter);
+    if (c[u].iter & 1)
+      ML_HIP(ctx, hipMemcpyAsync(phi[u], W.alt0, sizeof(double) * op->lev[0].g.ncell, hipMemcpyDeviceToDevice, s));
+    if (st) {
+      st[u].exit_status = c[u].status != 0 ? c[u].status : 2;
+      st[u].iterations = c[u].iter;
+      st[u].initial_residual = c[u].r0;
+      st[u].final_residual = c[u].norm;
+    }
+  }
+  b->batch_iters = most;
+  return ML_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ml_vc_batch_create(ml_ctx ctx, int nops, const ml_vc_op ops[], int per_op, ml_vc_batch* out) {
+  if (!ctx || nops < 1 || !ops || per_op < 1 || per_op > ML_VC_MAX_SOLVES || !out) return ML_EINVAL;
+  *out = nullptr;
+  std::unique_ptr<ml_vc_batch_s> b(new (std::nothrow) ml_vc_batch_s());
+  if (!b) return ML_ENOMEM;
+  b->ctx = ctx;
+  b->dim = ops[0] ? ops[0]->dim : 2;
+  for (int i = 0; i < nops; ++i) {
+    ml_vc_op op = ops[i];
+    if (!op || op->ctx != ctx) return fail(ctx, ML_EINVAL, "ml_vc_batch_create: operator of another context");
+    for (int j = 0; j < i; ++j)
+      if (ops[j] == op) return fail(ctx, ML_EINVAL, "ml_vc_batch_create: an operator appears twice");
+    if (op->dim != 2) return fail(ctx, ML_ENOTSUP, "ml_vc_batch_create: 2D operators only");
+    const int L = int(op->lev.size());
+    if (op->lc >= L || op->lc < 1)
+      return fail(ctx, ML_ENOTSUP, "ml_vc_batch_create: needs tiled levels above a single-workgroup coarse level");
+    for (int l = 0; l < L; ++l)
+      if (!ml::vc_batch_level_ok(op->lev[l].g))
+        return fail(ctx, ML_ENOTSUP, "ml_vc_batch_create: levels above 512^2 cells are not batched");
+    int rc = vc_work_ensure(op, per_op);
+    if (rc) return rc;
+    for (int w = 0; w < per_op; ++w) b->units.push_back(ml::VCBatchUnit{op, w, op->lc});
+    b->K = std::max(b->K, op->lc);
+  }
+  *out = b.release();
+  return ML_OK;
+}
+
+int ml_vc_batch_destroy(ml_vc_batch b) {
+  if (!b) return ML_EINVAL;
+  hipStreamSynchronize(b->ctx->stream);
+  delete b;
+  return ML_OK;
+}
+
+int ml_vc_batch_solve(ml_vc_batch b, double* const phi[], const double* const rhs[], const ml_solve_opts* opts,
+                      ml_solve_stats stats[]) {
+  if (!b || !phi || !rhs) return ML_EINVAL;
+  const int n = int(b->units.size());
+  for (int u = 0; u < n; ++u)
+    if (!phi[u] || !rhs[u]) return ML_EINVAL;
+  ml_solve_opts o;
+  if (opts)
+    o = *opts;
+  else
+    ml_default_opts(&o);
+  return vcb_solve(b, phi, rhs, o, stats);
+}
+
+int ml_vc_batch_solve_be(ml_vc_batch b, double* const phi_new[], const double* const phi_old[],
+                         const double* const src[], double dt, const ml_solve_opts* opts, ml_solve_stats stats[]) {
+  if (!b || !phi_new || !phi_old) return ML_EINVAL;
+  const int n = int(b->units.size());
+  ml_solve_opts o;
+  if (opts)
+    o = *opts;
+  else
+    ml_default_opts(&o);
+  std::vector<const double*> rhs(n);
+  for (int u = 0; u < n; ++u) {
+    if (!phi_new[u] || !phi_old[u]) return ML_EINVAL;
+    ml_vc_op op = b->units[u].op;
+    op->alpha = 1.0;
+    op->beta = -dt;
+    ml::VCWork& W = op->work[b->units[u].w];
+    ML_HIP(b->ctx, ml::be_rhs<double>(op->lev[0].g, 1, W.rhs_be, phi_old[u], op->has_a ? op->lev[0].a : nullptr, dt,
+                                      src ? src[u] : nullptr, b->ctx->stream));
+    rhs[u] = W.rhs_be;
+  }
+  return vcb_solve(b, phi_new, rhs.data(), o, stats);
 }
 
 }  // extern "C"

@@ -81,12 +81,14 @@ __device__ inline T vc_lof(const Faces1<T, DIM>& F, const T* p, T aq, bool hasc,
 // coefficients, lambda and rhs of every position a thread may update are
 // loaded once into registers; a pass updates one colour and refreshes the
 // homogeneous wall ghosts of the cells it updated, so one barrier per pass.
+// The body works on tile `tile` of the level (a batched launch maps its
+// workgroups to (instance, tile) pairs).
 template <typename T, int DIM, int TX, int TY, int TZ, int HMAX, int NT>
-__global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T* __restrict__ in,
-                                                           T* __restrict__ out, const T* __restrict__ rhs, int npass,
-                                                           int first, int zero_init, Geom gc,
-                                                           const T* __restrict__ pc, int ntx, int nty, Geom gr,
-                                                           T* __restrict__ resc, VCFinal<T> fin) {
+__device__ __forceinline__ void vc_relax_tile_body(const VCArgs<T>& A, const T* __restrict__ in, T* __restrict__ out,
+                                                   const T* __restrict__ rhs, int npass, int first, int zero_init,
+                                                   const Geom& gc, const T* __restrict__ pc, int ntx, int nty,
+                                                   const Geom& gr, T* __restrict__ resc, const VCFinal<T>& fin,
+                                                   int tile) {
   if (A.skip && *A.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2 * HMAX) * (TY + 2 * HMAX) * (DIM == 3 ? TZ + 2 * HMAX : 1);
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T*
   // with a restriction or a residual to produce, one more halo cell keeps
   // the core's neighbours exact after the last pass
   const bool final = fin.phi != nullptr;
-  t.init(npass + ((resc || final) ? 1 : 0), ntx, nty);
+  t.init_at(npass + ((resc || final) ? 1 : 0), ntx, nty, tile);
   T* su = reinterpret_cast<T*>(smem_raw);
   const T dxinv = T(1) / (T(g.dx) * T(g.dx));
   PosRec R[KMAX];
@@ -246,6 +248,26 @@ __global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T*
           sum = sum + sres[(2 * cx + ii) + TX * ((2 * cy + jj) + TY * (DIM == 3 ? 2 * cz + kk : 0))];
     resc[gr.ci(I, J, K)] = sum;
   }
+}
+
+template <typename T, int DIM, int TX, int TY, int TZ, int HMAX, int NT>
+__global__ __launch_bounds__(NT) void vc_relax_tile_kernel(VCArgs<T> A, const T* __restrict__ in,
+                                                           T* __restrict__ out, const T* __restrict__ rhs, int npass,
+                                                           int first, int zero_init, Geom gc,
+                                                           const T* __restrict__ pc, int ntx, int nty, Geom gr,
+                                                           T* __restrict__ resc, VCFinal<T> fin) {
+  vc_relax_tile_body<T, DIM, TX, TY, TZ, HMAX, NT>(A, in, out, rhs, npass, first, zero_init, gc, pc, ntx, nty, gr,
+                                                   resc, fin, int(blockIdx.x));
+}
+
+// batched form: workgroup w relaxes tile map[w] & mask of job map[w] >> bits
+template <typename T, int DIM, int TX, int TY, int TZ, int HMAX, int NT>
+__global__ __launch_bounds__(NT) void vc_relax_batch_kernel(const VCRelaxJob<T>* __restrict__ jobs,
+                                                            const unsigned* __restrict__ map) {
+  const unsigned w = map[blockIdx.x];
+  const VCRelaxJob<T>& J = jobs[w >> kBatchTileBits];
+  vc_relax_tile_body<T, DIM, TX, TY, TZ, HMAX, NT>(J.A, J.in, J.out, J.rhs, J.npass, J.first, J.zero_init, J.gc, J.pc,
+                                                   J.ntx, J.nty, J.gr, J.resc, J.fin, int(w & kBatchTileMask));
 }
 
 // Residual / operator / restricted residual over a tile with a one-cell halo.
@@ -733,7 +755,7 @@ __device__ __forceinline__ void vcpre_store(const VCPre<T, DIM>& P, const VCLds<
 }
 
 template <typename T, int DIM>
-__global__ __launch_bounds__(kVCWG) void vc_coarse_vcycle_kernel(const VCDesc<T>* __restrict__ Dp) {
+__device__ __forceinline__ void vc_coarse_vcycle_body(const VCDesc<T>* __restrict__ Dp) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int LL = VCLevels<DIM>::n;
   const VCDesc<T>& D = *Dp;
@@ -812,6 +834,20 @@ __global__ __launch_bounds__(kVCWG) void vc_coarse_vcycle_kernel(const VCDesc<T>
     clev_coords(G[0], c, i, j, k);
     lv.e[lv.g.ci(i, j, k)] = S.se[c];
   }
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(kVCWG) void vc_coarse_vcycle_kernel(const VCDesc<T>* __restrict__ Dp) {
+  vc_coarse_vcycle_body<T, DIM>(Dp);
+}
+
+// batched form: workgroup w runs the coarse V-cycle of descriptor descs[w]
+template <typename T, int DIM>
+__global__ __launch_bounds__(kVCWG) void vc_coarse_batch_kernel(const VCDesc<T>* const* __restrict__ descs) {
+  // the descriptor address is uniform over the workgroup: keep it scalar
+  const unsigned long long a = reinterpret_cast<unsigned long long>(descs[blockIdx.x]);
+  const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(a)), hi = __builtin_amdgcn_readfirstlane(unsigned(a >> 32));
+  vc_coarse_vcycle_body<T, DIM>(reinterpret_cast<const VCDesc<T>*>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
 template <int DIM>
@@ -1394,6 +1430,71 @@ hipError_t vc_dot(const Geom& g, const T* x, const T* y, T* out, hipStream_t s) 
   return hipGetLastError();
 }
 
+// ---- batched form --------------------------------------------------------
+bool vc_batch_level_ok(const Geom& g) { return !vc_wide(g); }
+
+int vc_batch_hmax(int dim) { return dim == 2 ? VCRelaxTile<2>::HMAX : VCRelaxTile<3>::HMAX; }
+
+int vc_batch_tiles(const Geom& g, int* ntx, int* nty) {
+  return g.dim == 2 ? vc_tiles<VCRelaxTile<2>::TX, VCRelaxTile<2>::TY, VCRelaxTile<2>::TZ>(g, *ntx, *nty)
+                    : vc_tiles<VCRelaxTile<3>::TX, VCRelaxTile<3>::TY, VCRelaxTile<3>::TZ>(g, *ntx, *nty);
+}
+
+template <int DIM>
+static size_t vc_batch_lds_dim(int npass, bool resc, bool fin) {
+  using D = VCRelaxTile<DIM>;
+  const int h = npass + ((resc || fin) ? 1 : 0);
+  size_t lds = size_t(D::TX + 2 * h) * (D::TY + 2 * h) * (DIM == 3 ? D::TZ + 2 * h : 1);
+  if (resc) lds += size_t(D::TX) * D::TY * D::TZ;
+  if (fin) lds *= 2;
+  return lds;
+}
+
+size_t vc_batch_lds(int dim, int npass, bool resc, bool fin) {
+  return (dim == 2 ? vc_batch_lds_dim<2>(npass, resc, fin) : vc_batch_lds_dim<3>(npass, resc, fin)) * sizeof(double);
+}
+
+template <typename T>
+hipError_t vc_relax_batch(int dim, const VCRelaxJob<T>* jobs, const unsigned* map, int nblocks, size_t lds,
+                          hipStream_t s, const LaunchTimer* tm) {
+  if (nblocks <= 0) return hipSuccess;
+  const bool timed = tm && *tm->n < tm->cap;
+  if (tm && !timed) ++*tm->missed;
+  hipEvent_t e0 = timed ? tm->start[*tm->n] : nullptr, e1 = timed ? tm->stop[*tm->n] : nullptr;
+  if (dim == 2) {
+    using D = VCRelaxTile<2>;
+    hipExtLaunchKernelGGL((vc_relax_batch_kernel<T, 2, D::TX, D::TY, D::TZ, D::HMAX, D::NT>), dim3(nblocks),
+                          dim3(D::NT), lds, s, e0, e1, 0, jobs, map);
+  } else {
+    using D = VCRelaxTile<3>;
+    hipExtLaunchKernelGGL((vc_relax_batch_kernel<T, 3, D::TX, D::TY, D::TZ, D::HMAX, D::NT>), dim3(nblocks),
+                          dim3(D::NT), lds, s, e0, e1, 0, jobs, map);
+  }
+  if (timed) ++*tm->n;
+  return hipGetLastError();
+}
+
+__global__ void ctl_gather_kernel(SolveCtl* const* __restrict__ ctls, SolveCtl* __restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = *ctls[i];
+}
+
+hipError_t ctl_gather(SolveCtl* const* ctls, SolveCtl* out, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ctl_gather_kernel, dim3((n + 255) / 256), dim3(256), 0, s, ctls, out, n);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t vc_coarse_batch(int dim, const VCDesc<T>* const* descs, int n, size_t lds, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (dim == 2)
+    hipLaunchKernelGGL((vc_coarse_batch_kernel<T, 2>), dim3(n), dim3(kVCWG), lds, s, descs);
+  else
+    hipLaunchKernelGGL((vc_coarse_batch_kernel<T, 3>), dim3(n), dim3(kVCWG), lds, s, descs);
+  return hipGetLastError();
+}
+
 template <typename T>
 size_t vc_coarse_lds_bytes(const Geom* g, int l0, int nlev, bool bicgstab) {
   long long tot = 0;
@@ -1432,5 +1533,8 @@ template hipError_t vc_restrict_residual<double>(const VCArgs<double>&, const Ge
                                                  double*, hipStream_t);
 template hipError_t vc_coarse_vcycle<double>(const VCDesc<double>*, int, size_t, hipStream_t);
 template size_t vc_coarse_lds_bytes<double>(const Geom*, int, int, bool);
+template hipError_t vc_relax_batch<double>(int, const VCRelaxJob<double>*, const unsigned*, int, size_t, hipStream_t,
+                                           const LaunchTimer*);
+template hipError_t vc_coarse_batch<double>(int, const VCDesc<double>* const*, int, size_t, hipStream_t);
 
 }  // namespace ml

@@ -1,0 +1,119 @@
+"""Batched linear solves (ml_vc_batch, include/mlhip.h): many Darcy-Brinkman
+instances' U* solves (two components each) and projections iterate together,
+every launch covering all of them.  Each unit's result and statistics must
+equal the single-operator solve of the same unit bit for bit (the U* solves
+of ml_vc_solve_be_many, the projection of ml_vc_solve), with relaxation and
+BiCGStab bottoms, on uneven 2D grid sizes, and on a second call that
+reuses the batch's tables."""
+import importlib
+
+import numpy as np
+import pytest
+
+from tests import hc_cases as HC
+
+pytestmark = pytest.mark.gpu
+
+wlm = importlib.import_module("mushy-layer_amd.workloads")
+
+U_KW = dict(nu1=2, nu2=2, bottom=2, max_iter=10, eps=1e-10, hang=1e-10, norm_thresh=1e-10, bottom_solver=1,
+            bottom_max_iter=80)
+P_KW = dict(nu1=2, nu2=4, bottom=4, max_iter=20, eps=1e-15, hang=1e-15, norm_thresh=1e-15, bottom_solver=1,
+            bottom_max_iter=20)
+SIZES = {2: [(64, 64), (128, 128), (96, 40), (256, 256), (512, 256), (64, 64)]}
+
+
+def _instances(ml, ctx, dim):
+    out = []
+    for i, n in enumerate(SIZES[dim]):
+        w = wlm.darcy_brinkman(n=n, seed=10 + i, corr=0.15, Da=10 ** (-3 + 0.4 * i))
+        uop = ml.VCOp(ctx, n, w["dx"], w["periodic"], ml.make_bc(w["bc_u"]))
+        pop = ml.VCOp(ctx, n, w["dx"], w["periodic"], ml.make_bc(w["bc_p"]))
+        keep = [ml.DeviceArray(ctx, x.shape, host=x) for x in w["b_u"]] + [ml.DeviceArray(ctx, w["c"].shape, host=w["c"])]
+        uop.set_coefs(None, keep[:dim], keep[dim], 1.0, -w["dt"])
+        bp = [ml.DeviceArray(ctx, x.shape, host=x) for x in w["b_p"]]
+        pop.set_coefs(None, bp, None, 0.0, 1.0)
+        out.append(dict(w=w, uop=uop, pop=pop, keep=keep + bp))
+    return out
+
+
+def _rand(shape, seed, zero_mean=False, n=None):
+    rng = np.random.default_rng(seed)
+    x = np.zeros(shape)
+    v = HC.valid(n, x)
+    v[...] = rng.standard_normal(v.shape)
+    if zero_mean:
+        v[...] -= v.mean()
+    return x
+
+
+def _stats(s):
+    return (s.exit_status, s.iterations, s.initial_residual, s.final_residual)
+
+
+@pytest.mark.parametrize("dim", [2])
+@pytest.mark.parametrize("bottom", [0, 1])
+def test_batched_solves_equal_single_solves(ml, gpu_ctx, dim, bottom):
+    ctx = gpu_ctx
+    inst = _instances(ml, ctx, dim)
+    ukw = dict(U_KW, bottom_solver=bottom, bottom=2 if bottom else 8)
+    pkw = dict(P_KW, bottom_solver=bottom)
+    ub = ml.VCBatch(ctx, [I["uop"] for I in inst], per_op=dim)
+    pb = ml.VCBatch(ctx, [I["pop"] for I in inst], per_op=1)
+    for call in range(2):   # the second call reuses the tables with new values
+        olds, srcs, rhss = [], [], []
+        for i, I in enumerate(inst):
+            n, shp = I["w"]["n"], I["w"]["u0"].shape
+            olds.append([_rand(shp, 100 * call + 10 * i + c, n=n) for c in range(dim)])
+            srcs.append([_rand(shp, 1000 + 100 * call + 10 * i + c, n=n) for c in range(dim)])
+            rhss.append(_rand(shp, 5000 + 100 * call + i, zero_mean=True, n=n))
+        # single-operator solves
+        ref_u, ref_us, ref_p, ref_ps = [], [], [], []
+        for i, I in enumerate(inst):
+            dt = I["w"]["dt"]
+            new = [ml.DeviceArray(ctx, x.shape, host=x) for x in olds[i]]
+            old = [ml.DeviceArray(ctx, x.shape, host=x) for x in olds[i]]
+            src = [ml.DeviceArray(ctx, x.shape, host=x) for x in srcs[i]]
+            sts = I["uop"].solve_be_many(new, old, dt, src=src, opts=ml.default_opts(**ukw))
+            ref_u.append([x.get() for x in new])
+            ref_us.append([_stats(s) for s in sts])
+            p = ml.DeviceArray(ctx, rhss[i].shape, host=np.zeros_like(rhss[i]))
+            r = ml.DeviceArray(ctx, rhss[i].shape, host=rhss[i])
+            st = I["pop"].solve(p, r, opts=ml.default_opts(**pkw))
+            ref_p.append(p.get())
+            ref_ps.append(_stats(st))
+        # batched: U* needs one dt for the batch, the instances share it
+        dt = inst[0]["w"]["dt"]
+        assert all(I["w"]["dt"] == dt for I in inst)
+        news = [ml.DeviceArray(ctx, x.shape, host=x) for xs in olds for x in xs]
+        oldd = [ml.DeviceArray(ctx, x.shape, host=x) for xs in olds for x in xs]
+        srcd = [ml.DeviceArray(ctx, x.shape, host=x) for xs in srcs for x in xs]
+        sts = ub.solve_be(news, oldd, dt, srcs=srcd, opts=ml.default_opts(**ukw))
+        pd = [ml.DeviceArray(ctx, x.shape, host=np.zeros_like(x)) for x in rhss]
+        rd = [ml.DeviceArray(ctx, x.shape, host=x) for x in rhss]
+        psts = pb.solve(pd, rd, opts=ml.default_opts(**pkw))
+        for i in range(len(inst)):
+            for c in range(dim):
+                k = i * dim + c
+                assert _stats(sts[k]) == ref_us[i][c], (call, i, c)
+                assert HC.bitwise_equal(news[k].get(), ref_u[i][c]), (call, i, c)
+            assert _stats(psts[i]) == ref_ps[i], (call, i)
+            assert HC.bitwise_equal(pd[i].get(), ref_p[i]), (call, i)
+    ub.close()
+    pb.close()
+
+
+def test_batch_refuses_unsupported_layouts(ml, gpu_ctx):
+    ctx = gpu_ctx
+    w = wlm.darcy_brinkman(n=(32, 32), seed=1)
+    small = ml.VCOp(ctx, (32, 32), w["dx"], w["periodic"], ml.make_bc(w["bc_u"]))
+    with pytest.raises(RuntimeError, match="tiled levels"):
+        ml.VCBatch(ctx, [small], per_op=1)
+    w2 = wlm.darcy_brinkman(n=(64, 64), seed=1)
+    op = ml.VCOp(ctx, (64, 64), w2["dx"], w2["periodic"], ml.make_bc(w2["bc_u"]))
+    with pytest.raises(RuntimeError, match="appears twice"):
+        ml.VCBatch(ctx, [op, op], per_op=1)
+    w3 = wlm.darcy_brinkman(n=(16, 16, 16), seed=1)
+    op3 = ml.VCOp(ctx, (16, 16, 16), w3["dx"], w3["periodic"], ml.make_bc(w3["bc_u"]))
+    with pytest.raises(RuntimeError, match="2D"):
+        ml.VCBatch(ctx, [op3], per_op=1)
