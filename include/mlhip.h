@@ -368,6 +368,11 @@ int ml_hc_batch_tile(ml_hc_batch b, int* variant, int* tile_x, int* tile_y, int*
  * in place; stats[i] (may be NULL) its solve statistics */
 int ml_hc_batch_step(ml_hc_batch b, double* const phi[], double dt, const ml_solve_opts* opts,
                      ml_solve_stats stats[]);
+/* the same step with a source per instance (src[i] or src may be NULL): the
+ * single-instance sequence with ml_hc_solve_be(op_i, phi[i], copy, src[i],
+ * dt, opts), e.g. the advective source of a Darcy-Brinkman step */
+int ml_hc_batch_step_src(ml_hc_batch b, double* const phi[], const double* const src[], double dt,
+                         const ml_solve_opts* opts, ml_solve_stats stats[]);
 
 /* ---- adjoint of the implicit step (no counterpart in the reference) ----
  * Differentiable time stepping of the no-flow (H, Theta) problem with a

@@ -181,6 +181,8 @@ SIGNATURES = {
     "ml_hc_batch_set_tile": (C.c_int, [_vp, C.c_int]),
     "ml_hc_batch_tile": (C.c_int, [_vp, _ip, _ip, _ip, _ip, C.POINTER(C.c_float), C.c_int]),
     "ml_hc_batch_step": (C.c_int, [_vp, C.POINTER(_vp), C.c_double, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "ml_hc_batch_step_src": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.c_double, C.POINTER(SolveOpts),
+                                       C.POINTER(SolveStats)]),
     "ml_vc_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.c_int, C.POINTER(_vp)]),
     "ml_vc_batch_destroy": (C.c_int, [_vp]),
     "ml_vc_batch_solve": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
@@ -601,14 +603,21 @@ class HCBatch:
                   "ml_hc_batch_create")
         self.h = h
 
-    def step(self, phis, dt, opts=None):
+    def step(self, phis, dt, opts=None, srcs=None):
         """Advance every state (list of DeviceArray, one per instance) in
-        place; returns the per-instance SolveStats."""
+        place, with a source per instance when `srcs` is given; returns the
+        per-instance SolveStats."""
         n = len(self.ops)
         st = (SolveStats * n)()
         o = opts or default_opts(precision=PRECISION_FP32_REFINED)
-        self.ctx.check(self.ctx.lib.ml_hc_batch_step(self.h, (_vp * n)(*[x.ptr for x in phis]), dt, C.byref(o), st),
-                       "ml_hc_batch_step")
+        if srcs is None:
+            self.ctx.check(self.ctx.lib.ml_hc_batch_step(self.h, (_vp * n)(*[x.ptr for x in phis]), dt, C.byref(o),
+                                                         st), "ml_hc_batch_step")
+        else:
+            self.ctx.check(self.ctx.lib.ml_hc_batch_step_src(self.h, (_vp * n)(*[x.ptr for x in phis]),
+                                                             (_vp * n)(*[x.ptr if x is not None else None
+                                                                         for x in srcs]), dt, C.byref(o), st),
+                           "ml_hc_batch_step_src")
         return list(st)
 
     def memory(self):

@@ -82,6 +82,7 @@ struct ml_hc_batch_s {
   // what the uploaded tables were built for
   bool tables_ok = false;
   std::vector<double*> key_phi;
+  std::vector<const double*> key_src;
   double key_dt = 0;
   int key_nu1 = -1, key_nu2 = -1, key_bottom = -1;
   // workgroup maps over the instances still running (stragglers), built
@@ -368,7 +369,7 @@ void fill_lin_maps(ml_hc_batch b) {
 }
 
 // job tables for the given states, dt and cycle parameters
-int build_tables(ml_hc_batch b, double* const* phi, double dt, const ml_solve_opts& o) {
+int build_tables(ml_hc_batch b, double* const* phi, const double* const* src, double dt, const ml_solve_opts& o) {
   ml_ctx ctx = b->ctx;
   const int n = int(b->inst.size());
   // the pinned mirror may still be read by an earlier upload
@@ -394,6 +395,8 @@ int build_tables(ml_hc_batch b, double* const* phi, double dt, const ml_solve_op
     S.phi = phi[i];
     S.alt = I.alt0;
     S.rhs = I.rhs;
+    S.src = src ? src[i] : nullptr;
+    S.dt = dt;
     for (int d = 0; d < 3; ++d) {
       S.b64[d] = d < op->dim ? op->lev[0].b[d] : nullptr;
       S.b32[d] = d < op->dim ? const_cast<float*>(I.lin[0].b[d]) : nullptr;
@@ -489,6 +492,8 @@ int build_tables(ml_hc_batch b, double* const* phi, double dt, const ml_solve_op
   fill_lin_maps(b);
   ML_HIP(ctx, hipMemcpyAsync(b->arena, b->harena, b->arena_bytes, hipMemcpyHostToDevice, ctx->stream));
   b->key_phi.assign(phi, phi + n);
+  b->key_src.assign(n, nullptr);
+  if (src) b->key_src.assign(src, src + n);
   b->key_dt = dt;
   b->key_nu1 = o.nu1;
   b->key_nu2 = o.nu2;
@@ -582,7 +587,7 @@ hipError_t enqueue_iteration(ml_hc_batch b, const ml_solve_opts& o, hipStream_t 
 // scratch, so the trial runs leave no trace in the results.
 constexpr int kTuneReps = 3;
 
-int autotune(ml_hc_batch b, double* const* phi, double dt, const ml_solve_opts& o) {
+int autotune(ml_hc_batch b, double* const* phi, const double* const* src, double dt, const ml_solve_opts& o) {
   ml_ctx ctx = b->ctx;
   hipStream_t s = ctx->stream;
   const int nv = lin_relax_batch_tiles();
@@ -593,7 +598,7 @@ int autotune(ml_hc_batch b, double* const* phi, double dt, const ml_solve_opts& 
   int best = 0, rc = ML_OK;
   for (int v = 0; v < nv && rc == ML_OK; ++v) {
     b->variant = v;
-    if ((rc = build_tables(b, phi, dt, o))) break;
+    if ((rc = build_tables(b, phi, src, dt, o))) break;
     for (int rep = 0; rep <= kTuneReps && rc == ML_OK; ++rep) {
       if (rep == 1 && hipEventRecord(e0, s) != hipSuccess) rc = fail(ctx, ML_EHIP, "autotune event");
       for (int k = 0; k < b->K && rc == ML_OK; ++k)
@@ -620,7 +625,7 @@ int autotune(ml_hc_batch b, double* const* phi, double dt, const ml_solve_opts& 
   if (rc != ML_OK) return rc;
   b->variant = best;
   b->tuned = true;
-  return build_tables(b, phi, dt, o);
+  return build_tables(b, phi, src, dt, o);
 }
 
 int capture(ml_hc_batch b, const ml_solve_opts& o) {
@@ -782,6 +787,11 @@ int ml_hc_batch_tile(ml_hc_batch b, int* variant, int* tile_x, int* tile_y, int*
 }
 
 int ml_hc_batch_step(ml_hc_batch b, double* const phi[], double dt, const ml_solve_opts* opts, ml_solve_stats stats[]) {
+  return ml_hc_batch_step_src(b, phi, nullptr, dt, opts, stats);
+}
+
+int ml_hc_batch_step_src(ml_hc_batch b, double* const phi[], const double* const src[], double dt,
+                         const ml_solve_opts* opts, ml_solve_stats stats[]) {
   if (!b || !phi) return ML_EINVAL;
   ml_ctx ctx = b->ctx;
   hipStream_t s = ctx->stream;
@@ -804,10 +814,10 @@ int ml_hc_batch_step(ml_hc_batch b, double* const phi[], double dt, const ml_sol
   int rc;
   bool same = b->tables_ok && b->key_dt == dt && b->key_nu1 == o.nu1 && b->key_nu2 == o.nu2 &&
               b->key_bottom == o.bottom;
-  for (int i = 0; same && i < n; ++i) same = b->key_phi[i] == phi[i];
+  for (int i = 0; same && i < n; ++i) same = b->key_phi[i] == phi[i] && b->key_src[i] == (src ? src[i] : nullptr);
   if (!b->tuned) {
-    if ((rc = autotune(b, phi, dt, o))) return rc;
-  } else if (!same && (rc = build_tables(b, phi, dt, o))) {
+    if ((rc = autotune(b, phi, src, dt, o))) return rc;
+  } else if (!same && (rc = build_tables(b, phi, src, dt, o))) {
     return rc;
   }
   for (BatchInst& I : b->inst) {

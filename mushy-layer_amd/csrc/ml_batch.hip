@@ -11,7 +11,7 @@
 //   -> the MG hierarchy of bCoef, each coarse face the average of its fine
 //   sub-faces (MGnewOp, src/AMRNonLinearMultiCompOp.cpp:1423-1494)
 //   -> FP32 copies of every level's faces
-//   -> backward-Euler rhs a*phi_old with a == 1 and no source.
+//   -> backward-Euler rhs a*phi_old with a == 1, plus dt*src with a source.
 #include <hip/hip_runtime.h>
 
 #include "ml_kernels.h"
@@ -50,7 +50,9 @@ __global__ __launch_bounds__(kChunkThreads) void batch_setup_kernel(const BatchS
     if (t < ncopy) {
       const double v = J.phi[t];
       J.alt[t] = v;
-      J.rhs[t] = v * 1.0;
+      double r = v * 1.0;
+      if (J.src) r = r + J.dt * J.src[t];
+      J.rhs[t] = r;
       continue;
     }
     if (t >= fbeg[3]) {

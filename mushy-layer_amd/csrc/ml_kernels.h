@@ -402,7 +402,9 @@ struct BatchSetupJob {
   double kcond, DH, DC;   // bCoef parameters
   double* phi;            // state (in; the wrap-up copies the final state back)
   double* alt;            // its partner, a full copy
-  double* rhs;            // backward-Euler rhs (aCoef == 1, no source)
+  double* rhs;            // backward-Euler rhs: phi*aCoef (aCoef == 1), + dt*src with a source
+  const double* src;      // source (nullable)
+  double dt;
   double* b64[3];         // level-0 faces
   float* b32[3];
   unsigned long long* slots;   // norm slots, zeroed

@@ -261,3 +261,84 @@ class DarcyBrinkmanStepper(HCStepper):
         super().close()
         self.uop.close()
         self.pop.close()
+
+
+class BatchDarcyBrinkmanStepper:
+    """Many independent Darcy-Brinkman instances (own grid size, St, CR,
+    Ra_C, Da) stepping together: each instance's step is exactly the
+    DarcyBrinkmanStepper step with the FP32-refined (H, Theta) solve, but
+    the solves of all instances share launches -- the (H, Theta) solves in
+    one HCBatch step with every instance's advective source, the U* solves
+    (both components of every instance) in one VCBatch, the projections in
+    another.  The per-instance closure, source and correction kernels run
+    instance by instance.  `cases`: `workloads.batch_instances` cases (each
+    carrying Ra_C and Da); `flow_of(case)`: the flow dict of a case."""
+
+    def __init__(self, ml, ctx, cases, flow_of, hc_opts):
+        from . import HCBatch, VCBatch
+        self.ml, self.ctx = ml, ctx
+        self.dt = cases[0]["dt"]
+        if any(c["dt"] != self.dt for c in cases):
+            raise ValueError("the instances of a batch share dt")
+        self.st = [DarcyBrinkmanStepper(ml, ctx, c, hc_opts, flow_of(c)) for c in cases]
+        self.hc_opts = hc_opts
+        self.hcb = HCBatch(ctx, [s.op for s in self.st])
+        self.ub = VCBatch(ctx, [s.uop for s in self.st], per_op=len(self.st[0].n))
+        self.pb = VCBatch(ctx, [s.pop for s in self.st], per_op=1)
+        self.uopts = default_opts(**U_OPTS)
+        self.popts = default_opts(**P_OPTS)
+        self.stats = []
+
+    def reset(self):
+        for s in self.st:
+            s.reset()
+        self.stats = []
+
+    def step(self):
+        dt = self.dt
+        sts = self.st
+        # (H, Theta): coefficients at t^n, then every instance's advective
+        # source, then one batched solve (which assembles the same
+        # coefficients again from the state)
+        for s in sts:
+            s.op.enthalpy_vars(s.phi, s.vold)
+            s.op.assemble_bcoef(s.vold.comp_ptr(3), s.b)
+            s.op.set_coefs(None, s.b, 1.0, -dt)
+        srcs = [s.hc_src() for s in sts]
+        hst = self.hcb.step([s.phi for s in sts], dt, self.hc_opts, srcs=srcs)
+        # enthalpy variables at t^{n+1}, U* coefficients and sources
+        for s in sts:
+            s.op.enthalpy_vars(s.phi, s.vnew)
+            s.uop.darcy_ccoef_law(s.vold.comp_ptr(3), s.coef["darcy"], s.law, s.c)
+            s.uop.set_coefs(None, s.bu, s.c, 1.0, -dt)
+            s.uop.ustar_src(s.fp, s.vold, s.vnew, s.pi, s.src)
+            for u, uo in zip(s.u, s.uold):
+                s.copy(uo, u)
+        ust = self.ub.solve_be([x for s in sts for x in s.ustar], [x for s in sts for x in s.uold], dt,
+                               srcs=[x for s in sts for x in s.src], opts=self.uopts)
+        # projection
+        for s in sts:
+            s.uop.cc_project_rhs(s.fp, dt, s.vold, s.vnew, s.pi, s.ustar, s.u, s.prhs)
+            s.pop.pressure_faces(s.vold, s.vnew, s.bp)
+            s.pop.set_coefs(None, s.bp, None, 0.0, 1.0)
+            s._zero(s.pi)
+        pst = self.pb.solve([s.pi for s in sts], [s.prhs for s in sts], opts=self.popts)
+        for s in sts:
+            s.uop.cc_correct(s.fp, dt, s.vold, s.vnew, s.pi, s.u)
+        d = len(sts[0].n)
+        out = []
+        for i, s in enumerate(sts):
+            vs = [(x.exit_status, x.iterations) for x in ust[i * d:(i + 1) * d]] + \
+                 [(pst[i].exit_status, pst[i].iterations)]
+            s.stats.append((hst[i].exit_status, hst[i].iterations, hst[i].final_residual))
+            s.vstats.append(vs)
+            out.append((hst[i], vs))
+        self.stats.append(out)
+        return out
+
+    def close(self):
+        self.hcb.close()
+        self.ub.close()
+        self.pb.close()
+        for s in self.st:
+            s.close()
