@@ -68,6 +68,8 @@ def parse():
                     help="grid of the smoother bandwidth measurement (0: skip)")
     ap.add_argument("--batch", type=int, default=0,
                     help="> 0: the batched form on this many independent instances (64^2 .. 512x256, 8:4:2:1)")
+    ap.add_argument("--no-flow", action="store_true",
+                    help="--batch: skip the batched Darcy-Brinkman steps (the (H, Theta) solves alone)")
     ap.add_argument("--smoother-only", action="store_true",
                     help="only the cache-busting smoother measurement at --large-n (for counter runs)")
     ap.add_argument("--adjoint", action="store_true",
@@ -471,6 +473,7 @@ def run_batch(ml, wl, ctx, args):
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "avg_launch_us": round(ms / n1 * 1e3, 3), "bytes_per_launch": round(nb / n1, 1),
             "launches_timed": n1, "launches_untimed": missed}
+    db_step = None if args.no_flow else batch_darcy_brinkman(ml, wl, ctx, cases, args, steps)
     calibration_copy(ml, ctx)
     sizes = {}
     for c in cases:
@@ -496,6 +499,8 @@ def run_batch(ml, wl, ctx, args):
            "relax_tile_autotune": {"chosen": "%dx%d/%d" % tshape,
                                    "ms_per_sweep_set": {"shape%d" % v: round(t, 4) for v, t in enumerate(tms)}},
            "error_vs_fp64": compare and compare["error_vs_fp64"]}
+    if db_step:
+        cfg["darcy_brinkman_batched"] = db_step
     out = {
         "metric": METRIC, "value": round(cells * steps / wall, 2), "unit": "cell-updates/s", "n_gpus": 1,
         "steps": steps, "warmup": args.warmup, "ms_per_step": round(wall / steps * 1e3, 4),
@@ -503,6 +508,46 @@ def run_batch(ml, wl, ctx, args):
         "data": "synthetic", "config": cfg, "roofline": roof, "cpu_baseline": None,
     }
     print(json.dumps(out))
+
+
+def batch_darcy_brinkman(ml, wl, ctx, cases, args, steps):
+    """The same instances as Darcy-Brinkman mushy layers (Pr 10, each
+    instance's own Ra_C and Da, Kozeny-Carman): every step the advective
+    source and FP32-refined (H, Theta) solve of every instance in one batched
+    step, the U* solves of both velocity components and the projections of
+    all instances in batched linear solves (BiCGStab bottoms, the reference's
+    options), the closure, source and correction kernels instance by
+    instance.  Timed like the (H, Theta) batch: `steps` steps after the
+    warm-up steps, from the initial states."""
+    stp = importlib.import_module("mushy-layer_amd.steps")
+    hc_opts = ml.default_opts(max_iter=args.max_iter, precision=ml.PRECISION_FP32_REFINED)
+    b = stp.BatchDarcyBrinkmanStepper(ml, ctx, cases, lambda c: wl.flow(dim=2, Pr=10.0, Da=c["Da"], Ra_C=c["Ra_C"]),
+                                      hc_opts)
+    for _ in range(args.warmup):
+        b.step()
+    b.reset()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    cells = int(sum(np.prod(c["n"]) for c in cases))
+    hc_it = [o[0].iterations for st in b.stats for o in st]
+    us_it = [x[1] for st in b.stats for o in st for x in o[1][:-1]]
+    pr_it = [o[1][-1][1] for st in b.stats for o in st]
+    status = sorted(set(o[0].exit_status for st in b.stats for o in st))
+    vstatus = sorted(set(x[0] for st in b.stats for o in st for x in o[1]))
+    umax = [float(max(np.abs(ml.valid(s.n, u.get())).max() for u in s.u)) for s in b.st]
+    b.close()
+    return {"value": round(cells * steps / wall, 2), "unit": "cell-updates/s", "ms_per_step": round(wall / steps * 1e3, 4),
+            "steps": steps, "hc_iterations_per_solve": round(float(np.mean(hc_it)), 2), "hc_exit_status": status,
+            "ustar_iterations_per_solve": round(float(np.mean(us_it)), 2),
+            "projection_iterations_per_solve": round(float(np.mean(pr_it)), 2), "velocity_exit_status": vstatus,
+            "max_velocity_range": [round(min(umax), 6), round(max(umax), 6)],
+            "launches": "per step: one (H, Theta) batch step (2K+2 launches per outer iteration, graph), U* and "
+                        "projection batches (2K+1 launches per iteration each), per-instance closure, source and "
+                        "correction kernels"}
 
 
 def run_adjoint(ml, wl, ctx, args):
