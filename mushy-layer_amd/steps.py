@@ -181,6 +181,15 @@ class DarcyBrinkmanStepper(HCStepper):
         self.uopts = default_opts(**U_OPTS)
         self.popts = default_opts(**P_OPTS)
         self.vstats = []
+        # the U* components share every launch of one batched solve where the
+        # batched engine covers the grid (2D, levels up to 512^2 cells);
+        # otherwise they run concurrently on two streams (same results)
+        self.ub = None
+        try:
+            from . import VCBatch
+            self.ub = VCBatch(ctx, [self.uop], per_op=dim)
+        except RuntimeError:
+            self.ub = None
 
     def _zero(self, x):
         self.ctx.check(self.ctx.lib.ml_vec_set(self.ctx.h, x.nbytes // 8, x.ptr, 0.0), "ml_vec_set")
@@ -218,7 +227,10 @@ class DarcyBrinkmanStepper(HCStepper):
         uop.ustar_src(self.fp, self.vold, self.vnew, self.pi, self.src)
         for u, uo in zip(self.u, self.uold):
             self.copy(uo, u)
-        sts = uop.solve_be_many(self.ustar, self.uold, dt, src=self.src, opts=self.uopts)
+        if self.ub is not None and self.uopts.flags == 0:
+            sts = self.ub.solve_be(self.ustar, self.uold, dt, srcs=self.src, opts=self.uopts)
+        else:
+            sts = uop.solve_be_many(self.ustar, self.uold, dt, src=self.src, opts=self.uopts)
         vs = [(st.exit_status, st.iterations) for st in sts]
         uop.cc_project_rhs(self.fp, dt, self.vold, self.vnew, self.pi, self.ustar, self.u, self.prhs)
         pop.pressure_faces(self.vold, self.vnew, self.bp)
@@ -258,6 +270,8 @@ class DarcyBrinkmanStepper(HCStepper):
         return out
 
     def close(self):
+        if self.ub is not None:
+            self.ub.close()
         super().close()
         self.uop.close()
         self.pop.close()
