@@ -273,7 +273,8 @@ int ml_vc_apply_op(ml_vc_op op, int depth, double* lhs, double* phi, int homogen
 int ml_vc_precond(ml_vc_op op, int depth, double* phi, const double* rhs);
 /* dotProduct (AMRPoissonOp, inherited by DarcyBrinkmanOp, src/DarcyBrinkmanOp.H:27):
  * *out = (x, y) over the valid cells of level `depth`, in the fixed order of
- * the BiCGStab bottom (64 interleaved partial sums, then a pairwise tree);
+ * the BiCGStab bottom (64 interleaved partial sums, then a pairwise tree over
+ * the smallest power of two of them covering min(cells, 64));
  * blocks until the value is on the host */
 int ml_vc_dot(ml_vc_op op, int depth, const double* x, const double* y, double* out);
 /* restrictResidual (src/DarcyBrinkmanOp.cpp:183-240), homogeneous BCs */

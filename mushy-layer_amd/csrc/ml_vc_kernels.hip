@@ -556,7 +556,8 @@ __device__ __forceinline__ void vcx_prolong(const VCCtx<T, DIM>& X, const CLev& 
 // restarts; eps 1e-6, reps 1e-12, hang 1e-8, small 1e-30, 5 restarts, L2
 // norms.  Dot products run redundantly in every participating wavefront in a
 // fixed order: 64 interleaved lane sums over the compact cell index, then the
-// pairwise tree s_t + s_{t+h}, h = 32 .. 1.  Levels whose owners fit one
+// pairwise tree s_t + s_{t+h}, h = P/2 .. 1 with P the smallest power of two
+// >= min(N, 64).  Levels whose owners fit one
 // wavefront run on wave 0 alone with wavefront-scope syncs.  wk holds 8
 // vectors of L.N values.
 template <typename T, int DIM>
@@ -615,11 +616,13 @@ __device__ __forceinline__ void vcx_bicgstab(const VCCtx<T, DIM>& X, const CLev&
       sync();
     }
   };
+  // tree over the smallest power of two of lanes >= min(N, 64)
+  int P2 = 1;
+  while (P2 < 64 && P2 < N) P2 <<= 1;
   auto dot = [&](const T* x, const T* y) __attribute__((always_inline)) {
     T s = T(0);
     for (int c = lane; c < N; c += 64) s = s + x[c] * y[c];
-#pragma unroll
-    for (int h = 32; h >= 1; h >>= 1) s = s + __shfl_down(s, h, 64);
+    for (int h = P2 >> 1; h >= 1; h >>= 1) s = s + __shfl_down(s, h, 64);
     return __shfl(s, 0, 64);
   };
   const T eps = T(1e-6), reps = T(1e-12), hang = T(1e-8), tiny = T(1e-30);
@@ -1396,7 +1399,8 @@ __global__ __launch_bounds__(256) void vc_lambda_mult_kernel(VCArgs<T> A, const 
 
 // (x, y) over the valid cells in the order of the coarse V-cycle's BiCGStab:
 // one wavefront, 64 interleaved lane sums over the compact cell index, then
-// the pairwise tree s_t + s_{t+h}, h = 32 .. 1
+// the pairwise tree s_t + s_{t+h}, h = P/2 .. 1 (P the smallest power of two
+// >= min(cells, 64))
 template <typename T>
 __global__ __launch_bounds__(64) void vc_dot_kernel(Geom g, const T* __restrict__ x, const T* __restrict__ y,
                                                     T* __restrict__ out) {
@@ -1408,8 +1412,9 @@ __global__ __launch_bounds__(64) void vc_dot_kernel(Geom g, const T* __restrict_
     const long long q = g.ci(i, int(r % g.n[1]), int(r / g.n[1]));
     s = s + x[q] * y[q];
   }
-#pragma unroll
-  for (int h = 32; h >= 1; h >>= 1) s = s + __shfl_down(s, h, 64);
+  int P2 = 1;
+  while (P2 < 64 && P2 < n) P2 <<= 1;
+  for (int h = P2 >> 1; h >= 1; h >>= 1) s = s + __shfl_down(s, h, 64);
   if (threadIdx.x == 0) *out = s;
 }
 

@@ -188,8 +188,9 @@ enum { DOT_LANES = 64 };
 /* (x, y) over the valid cells (compact index c = cell + n*nvalid, x
  * fastest): 64 interleaved partial sums s_t = sum of x_c*y_c over c = t,
  * t+64, ... in increasing c, then the pairwise tree s_t = s_t + s_{t+h} for
- * h = 32, 16, ..., 1 (build spec: the order of the GPU's one-wavefront
- * reduction; Chombo sums box by box). */
+ * h = P/2, ..., 2, 1, P the smallest power of two >= min(cells, 64) (build
+ * spec: the order of the GPU's one-wavefront reduction; Chombo sums box by
+ * box). */
 double mlr_lin_dot(const mlr_grid* g, int ncomp, const double* x, const double* y) {
   mlr_lay L;
   mlr_layout(g, &L);
@@ -203,7 +204,9 @@ double mlr_lin_dot(const mlr_grid* g, int ncomp, const double* x, const double* 
           const long q = mlr_ci(&L, i, j, k) + n * L.ncell;
           s[c % DOT_LANES] = s[c % DOT_LANES] + x[q] * y[q];
         }
-  for (int h = DOT_LANES / 2; h >= 1; h /= 2)
+  int P = 1;
+  while (P < DOT_LANES && P < c) P *= 2;
+  for (int h = P / 2; h >= 1; h /= 2)
     for (int t = 0; t < h; ++t) s[t] = s[t] + s[t + h];
   return s[0];
 }
