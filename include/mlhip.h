@@ -390,9 +390,11 @@ int ml_hc_batch_step_src(ml_hc_batch b, double* const phi[], const double* const
  * porosity of phi_old, as the step assembled it), writes g_old = dL/dphi^{n-1}
  * through this step (valid cells; ghosts untouched), adds dL/dF to g_flux
  * (may be NULL) and dL/dSt to *g_stefan (may be NULL).  Solves J^T lambda =
- * g_new (J the step's Jacobian with the linearised closure) by BiCGStab with
- * the 2x2 block-diagonal preconditioner to opts->eps relative (max_iter
- * iterations); stats: status 1 converged / 2 iteration limit / 4 breakdown,
+ * g_new (J the step's Jacobian with the linearised closure) by BiCGStab to
+ * opts->eps relative (max_iter iterations), preconditioned by opts->nu1
+ * sweeps of the transposed smoother (2x2-block red/black Gauss-Seidel on J^T,
+ * black first: the transpose of the forward red-then-black smoother), or by
+ * the 2x2 block diagonal when nu1 == 0; stats: status 1 converged / 2 iteration limit / 4 breakdown,
  * iterations, |g_new|_2, final residual.  Overwrites the operator's level-0
  * bCoef.
  * ml_hc_porosity_vjp: g_phi += g_chi dchi/dphi (valid cells) and, when

@@ -1,7 +1,8 @@
 """Differentiable implicit time stepping of the (H, Theta) solidification
 problem: the hand-written adjoint of the backward-Euler step
 (ml_hc_adjoint_step, GPU kernels + BiCGStab on the transposed step
-Jacobian) behind a torch.autograd.Function, with the forward states
+Jacobian, preconditioned by nu1 sweeps of the transposed smoother -- one by
+default) behind a torch.autograd.Function, with the forward states
 recomputed from checkpoints every `checkpoint` steps.
 
 Parameters: the bottom heat-flux profile F (one value per bottom face), the

@@ -22,9 +22,11 @@
 //   mu_f          = lambda^T dA/db_f,
 //   dL/dF_x      += dt lambda_H(bottom cell x) b_H,face / dx,
 //   dL/dSt       += beta/dx^2 (S lambda)^T dphi_l/dSt + (dL/dchi)^T dchi/dSt.
-// The solve is BiCGStab with the 2x2 block-diagonal of J^T as
-// preconditioner, all vectors on the device, dot products reduced per
-// workgroup in a fixed order and summed on the host in a fixed order.
+// The solve is BiCGStab preconditioned by the transposed smoother (block
+// red/black Gauss-Seidel on J^T in the reverse colour order of the forward
+// smoother) or by the 2x2 block-diagonal of J^T, all vectors on the device,
+// dot products reduced per workgroup in a fixed order and summed on the host
+// in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -244,6 +246,48 @@ __global__ void adj_jacobi_kernel(AdjGeo G, const double* __restrict__ M, const 
     const double p10 = -G.bd * m1 * sd[0], p11 = G.alpha - G.bd * m3 * sd[1];
     const double det = p00 * p11 - p01 * p10;
     const double r0 = r[q], r1 = r[q + g.ncell];
+    z[q] = (p11 * r0 - p01 * r1) / det;
+    z[q + g.ncell] = (p00 * r1 - p10 * r0) / det;
+  }
+}
+
+// One colour of the transposed smoother: block Gauss-Seidel on J^T
+// z = P^{-1} r, updating the cells with (i + j) % 2 == colour from the
+// other colour's current z:  z_q = D_q^{-1} (r_q + bd M_q^T w_q), w_q,n the
+// face-weighted sum of the interior neighbours' z_n (the wall terms sit in
+// D_q, the 2x2 diagonal block of J^T).  The forward smoother relaxes red
+// then black; its transpose relaxes black then red.
+__global__ void adj_gs_kernel(AdjGeo G, const double* __restrict__ M, const double* __restrict__ r, double* z,
+                              int colour) {
+  const Geom& g = G.g;
+  const long long N = g.nvalid();
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N; c += (long long)gridDim.x * blockDim.x) {
+    int i, j;
+    cell_of(g, c, i, j);
+    if (((i + j) & 1) != colour) continue;
+    const long long q = g.ci(i, j, 0);
+    double sd[2] = {0.0, 0.0}, wn[2] = {0.0, 0.0};
+#pragma unroll
+    for (int n = 0; n < 2; ++n)
+#pragma unroll
+      for (int d = 0; d < 2; ++d)
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          const double bf = face_b(G, i, j, d, side, n);
+          const long long nb = nbr(g, i, j, d, side);
+          if (nb >= 0) {
+            sd[n] += -bf;
+            wn[n] += bf * z[nb + n * g.ncell];
+          } else {
+            sd[n] += bf * (wall_s(G.bc, d, side, n) - 1.0);
+          }
+        }
+    const double m0 = M[q], m1 = M[q + g.ncell], m2 = M[q + 2 * g.ncell], m3 = M[q + 3 * g.ncell];
+    const double p00 = G.alpha - G.bd * m0 * sd[0], p01 = -G.bd * m2 * sd[1];
+    const double p10 = -G.bd * m1 * sd[0], p11 = G.alpha - G.bd * m3 * sd[1];
+    const double det = p00 * p11 - p01 * p10;
+    const double r0 = r[q] + G.bd * (m0 * wn[0] + m2 * wn[1]);
+    const double r1 = r[q + g.ncell] + G.bd * (m1 * wn[0] + m3 * wn[1]);
     z[q] = (p11 * r0 - p01 * r1) / det;
     z[q + g.ncell] = (p00 * r1 - p10 * r0) / det;
   }
@@ -536,9 +580,27 @@ int ml_hc_adjoint_step(ml_hc_op op, const double* phi_new, const double* phi_old
     ML_HIP(ctx, hipGetLastError());
     return ML_OK;
   };
+  // preconditioner: o.nu1 sweeps of the transposed smoother (block
+  // Gauss-Seidel on J^T, black then red) from z = 0; nu1 == 0: the 2x2
+  // block-diagonal (Jacobi)
+  // (an odd periodic length would put two cells of one colour side by side
+  // across the seam: the block diagonal is used there)
+  bool odd_seam = false;
+  for (int d = 0; d < 2; ++d) odd_seam = odd_seam || (g.periodic[d] && (g.n[d] & 1));
+  const int sweeps = (o.nu1 > 0 && !odd_seam) ? o.nu1 : 0;
   auto prec = [&](const double* r, double* z) -> int {
-    hipLaunchKernelGGL(adj_jacobi_kernel, dim3(nb), dim3(kT), 0, s, G, w->M, r, z);
+    if (sweeps == 0) {
+      hipLaunchKernelGGL(adj_jacobi_kernel, dim3(nb), dim3(kT), 0, s, G, w->M, r, z);
+      ML_HIP(ctx, hipGetLastError());
+      return ML_OK;
+    }
+    hipLaunchKernelGGL(adj_axpby_kernel, dim3(nb), dim3(kT), 0, s, g, 0.0, r, 0.0, nullptr, z);
     ML_HIP(ctx, hipGetLastError());
+    for (int k = 0; k < sweeps; ++k)
+      for (int colour = 1; colour >= 0; --colour) {
+        hipLaunchKernelGGL(adj_gs_kernel, dim3(nb), dim3(kT), 0, s, G, w->M, r, z, colour);
+        ML_HIP(ctx, hipGetLastError());
+      }
     return ML_OK;
   };
   auto axpby = [&](double a, const double* x, double b, const double* y, double* out) -> int {
