@@ -571,8 +571,7 @@ def run_adjoint(ml, wl, ctx, args):
 
     def loss_only(F, St):
         phi, chi, c = model.forward(F, St)
-        for x in c.values():
-            x.free()
+        c.close()
         return float(np.mean((phi[1, :, 1:-1, 1:-1] - tphi[1, :, 1:-1, 1:-1]) ** 2) +
                      np.mean((chi[0, :, 1:-1, 1:-1] - tchi[0, :, 1:-1, 1:-1]) ** 2))
 

@@ -153,8 +153,7 @@ def test_adjoint_128_fifty_steps_against_oracle(ml, gpu_ctx, wl, oracle):
     model = adj.HeatFluxAdjoint(gpu_ctx, case, checkpoint=10)
     F_true = wl.flux_profile(128, F0=12.0, modes=((1, 0.3, 0.0), (2, 0.1, 1.0)))
     tphi, tchi, cps = model.forward(F_true, 5.5)
-    for x in cps.values():
-        x.free()
+    cps.close()
     F = wl.flux_profile(128, F0=10.0)
     St = 5.0
     L, gF, gperm, gSt = _adjoint_grads(torch, adj, model, F, St, tphi, tchi)
