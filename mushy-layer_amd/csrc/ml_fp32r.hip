@@ -611,19 +611,21 @@ __device__ __forceinline__ void lcx_relax(const LCCtx<DIM>& X, const CLev& L, in
       su[N + c] = X.M[m][2] * e0 + X.M[m][3] * e1;
     }
   };
-  if (cown_active<kWGT>(L) <= 64) {
-    if (threadIdx.x < 64)
-      for (int p = 0; p < npass; ++p) {
-        pass(p);
-        wave_sync();
-      }
-    __syncthreads();
-  } else {
+  // levels whose owners fit one wavefront: wave 0 alone with wavefront-scope
+  // syncs (small is uniform, so the workgroup barrier is reached by all
+  // waves or by none)
+  const bool small = cown_active<kWGT>(L) <= 64;
+  if (!small || threadIdx.x < 64) {
+#pragma unroll 1
     for (int p = 0; p < npass; ++p) {
       pass(p);
-      __syncthreads();
+      if (small)
+        wave_sync();
+      else
+        __syncthreads();
     }
   }
+  if (small) __syncthreads();
 }
 
 // r_c = R(r - J e), M_c = R(M); the coarse correction starts at zero.  The
@@ -939,10 +941,13 @@ __global__ __launch_bounds__(kWGT) void lin_coarse_vcycle_kernel(const LinDesc* 
   for (int l = 1; l < LL; ++l)
     if (l < nl) linpre_store<DIM>(P1[l - 1], S, G[l], threadIdx.x, false, direct && lb == l, bF, bA);
   __syncthreads();
+  // The level visits are runtime loops over one copy of their code (level
+  // geometry from LDS): unrolled per level, the kernel's code is several
+  // times the instruction cache and a launch spends most of its time
+  // fetching it.
   // down: pre-smoothing and restriction to the coarsest level
-#pragma unroll
-  for (int l = 0; l < LL; ++l) {
-    if (l + 1 >= nl) continue;
+#pragma unroll 1
+  for (int l = 0; l + 1 < nl; ++l) {
     LCCtx<DIM> X;
     lcx_build<DIM>(X, G[l], per, S, alpha, beta, true);
     lcx_mvec<DIM>(X, G[l], S);
@@ -989,9 +994,8 @@ __global__ __launch_bounds__(kWGT) void lin_coarse_vcycle_kernel(const LinDesc* 
     }
   }
   // up: prolongation and post-smoothing
-#pragma unroll
-  for (int l = LL - 2; l >= 0; --l) {
-    if (l + 1 >= nl) continue;
+#pragma unroll 1
+  for (int l = nl - 2; l >= 0; --l) {
     LCCtx<DIM> X;
     lcx_build<DIM>(X, G[l], per, S, alpha, beta, false);
     lcx_prolong<DIM>(X, G[l], G[l + 1], per, S);

@@ -489,19 +489,21 @@ __device__ __forceinline__ void vcx_relax(const VCCtx<T, DIM>& X, const CLev& L,
       se[X.c[m]] = st[0] - X.lam[m] * (lof - X.rr[m]);
     }
   };
-  if (cown_active<kVCWG>(L) <= 64) {
-    if (threadIdx.x < 64)
-      for (int p = 0; p < npass; ++p) {
-        pass(p);
-        wave_sync();
-      }
-    __syncthreads();
-  } else {
+  // levels whose owners fit one wavefront: wave 0 alone with wavefront-scope
+  // syncs (small is uniform, so the workgroup barrier is reached by all
+  // waves or by none)
+  const bool small = cown_active<kVCWG>(L) <= 64;
+  if (!small || threadIdx.x < 64) {
+#pragma unroll 1
     for (int p = 0; p < npass; ++p) {
       pass(p);
-      __syncthreads();
+      if (small)
+        wave_sync();
+      else
+        __syncthreads();
     }
   }
+  if (small) __syncthreads();
 }
 
 // restricted residual (homogeneous BCs, RESTRICTRESDB order) of level Lf into
@@ -604,6 +606,7 @@ __device__ __forceinline__ void vcx_bicgstab(const VCCtx<T, DIM>& X, const CLev&
     for (int m = 0; m < 2; ++m)
       if (X.pk[m] >= 0) z[X.c[m]] = f[X.c[m]] * X.lam[m];
     sync();
+#pragma unroll 1
     for (int pass = 0; pass < 4; ++pass) {
 #pragma unroll
       for (int m = 0; m < 2; ++m) {
@@ -803,15 +806,26 @@ __device__ __forceinline__ void vc_coarse_vcycle_body(const VCDesc<T>* __restric
   for (int l = 1; l < LL; ++l)
     if (l < nl) vcpre_store<T, DIM>(P1[l - 1], S, G[l], threadIdx.x, hasc, alpha, beta, false);
   __syncthreads();
+  // The level visits below are runtime loops over one copy of their code
+  // (levels rebuilt from the descriptor, offsets by prefix sum): unrolled
+  // per level, the kernel's code is several times the instruction cache and
+  // a launch spends most of its time fetching it.
+  auto level = [&](int l) __attribute__((always_inline)) {
+    CLev L;
+    int off = 0;
+    for (int q = 0; q < l; ++q) off += int(D.lev[l0 + q].g.nvalid());
+    clev_set(L, D.lev[l0 + l].g, off);
+    return L;
+  };
   // down: pre-smoothing and restriction; the coarsest level gets the bottom
   // smoothing
-#pragma unroll
-  for (int l = 0; l < LL; ++l) {
-    if (l >= nl) continue;
+#pragma unroll 1
+  for (int l = 0; l < nl; ++l) {
+    const CLev Ll = level(l);
     VCCtx<T, DIM> X;
-    vcx_build<T, DIM>(X, G[l], per, S);
-    vcx_relax<T, DIM>(X, G[l], bcm, S, hasc, alpha, beta, 2 * (l == nl - 1 ? D.bottom : D.nu1));
-    if (l + 1 < nl) vcx_restrict<T, DIM>(X, G[l], G[l + 1], bcm, S, hasc, alpha, beta);
+    vcx_build<T, DIM>(X, Ll, per, S);
+    vcx_relax<T, DIM>(X, Ll, bcm, S, hasc, alpha, beta, 2 * (l == nl - 1 ? D.bottom : D.nu1));
+    if (l + 1 < nl) vcx_restrict<T, DIM>(X, Ll, level(l + 1), bcm, S, hasc, alpha, beta);
   }
   // the bottom solver on the coarsest level (after its relaxations)
   if (D.bottom_solver == 1) {
@@ -823,13 +837,13 @@ __device__ __forceinline__ void vc_coarse_vcycle_body(const VCDesc<T>* __restric
     vcx_bicgstab<T, DIM>(X, Lb, bcm, S, hasc, alpha, beta, D.bottom_imax, S.sres + kVCWGCells);
   }
   // up: prolongation and post-smoothing
-#pragma unroll
-  for (int l = LL - 2; l >= 0; --l) {
-    if (l + 1 >= nl) continue;
+#pragma unroll 1
+  for (int l = nl - 2; l >= 0; --l) {
+    const CLev Ll = level(l);
     VCCtx<T, DIM> X;
-    vcx_build<T, DIM>(X, G[l], per, S);
-    vcx_prolong<T, DIM>(X, G[l], G[l + 1], S.se);
-    vcx_relax<T, DIM>(X, G[l], bcm, S, hasc, alpha, beta, 2 * D.nu2);
+    vcx_build<T, DIM>(X, Ll, per, S);
+    vcx_prolong<T, DIM>(X, Ll, level(l + 1), S.se);
+    vcx_relax<T, DIM>(X, Ll, bcm, S, hasc, alpha, beta, 2 * D.nu2);
   }
   const VCLevel<T>& lv = D.lev[l0];
   for (int c = threadIdx.x; c < G[0].N; c += kVCWG) {
