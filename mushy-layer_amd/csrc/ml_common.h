@@ -35,6 +35,11 @@ struct Geom {
   }
 };
 
+// the bit pattern of a value (exact comparison, NaN payloads and signed
+// zeros included)
+__device__ __forceinline__ unsigned long long bits_of(double x) { return (unsigned long long)__double_as_longlong(x); }
+__device__ __forceinline__ unsigned bits_of(float x) { return __float_as_uint(x); }
+
 __host__ __device__ inline Geom make_geom(int dim, const int n[3], const int periodic[3], double dx) {
   Geom g{};
   g.dim = dim;

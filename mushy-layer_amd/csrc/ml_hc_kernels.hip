@@ -460,16 +460,23 @@ struct WGCellOp {
 // npass colour passes (first colour red) on the level whose phi is staged in
 // sphi and whose phi_l (closure of sphi) is staged in sphl; rr = rhs of the
 // thread's cell.  Leaves sphi/sphl consistent and writes phi to global memory.
+// With fixed_stop, the passes end after the first red/black iteration that
+// leaves every cell's state unchanged to the bit: each later iteration would
+// start from the same state and repeat it exactly, so the result is that of
+// all npass passes.
 template <typename T, int DIM>
 __device__ void wg_relax(const WGConst<T>& K, const WGLevel<T, DIM>& W, const WGCellOp<T, DIM>& O,
-                         const T* rr, T* sphi, T* sphl, int npass) {
+                         const T* rr, T* sphi, T* sphl, int npass, bool fixed_stop = false) {
+  bool changed = false;
   auto pass = [&](int p) {
     if (W.own && !((W.i + W.j + W.k + p) & 1)) {
       T lof[2];
       wg_lof<T, DIM>(K, W, O.F, O.aq, sphi, sphl, lof);
       const int c = W.c;
-      const T h = sphi[c] - O.lam[0] * (lof[0] - rr[0]);
-      const T t = sphi[W.N + c] - O.lam[1] * (lof[1] - rr[1]);
+      const T h0 = sphi[c], t0 = sphi[W.N + c];
+      const T h = h0 - O.lam[0] * (lof[0] - rr[0]);
+      const T t = t0 - O.lam[1] * (lof[1] - rr[1]);
+      changed = changed || bits_of(h) != bits_of(h0) || bits_of(t) != bits_of(t0);
       sphi[c] = h;
       sphi[W.N + c] = t;
       T th, cl;
@@ -485,12 +492,21 @@ __device__ void wg_relax(const WGConst<T>& K, const WGLevel<T, DIM>& W, const WG
       for (int p = 0; p < npass; ++p) {
         pass(p);
         wave_sync();
+        if (fixed_stop && (p & 1)) {
+          if (!__any(changed)) break;
+          changed = false;
+        }
       }
     __syncthreads();
   } else {
     for (int p = 0; p < npass; ++p) {
       pass(p);
-      __syncthreads();
+      if (fixed_stop && (p & 1)) {
+        if (!__syncthreads_or(changed)) break;
+        changed = false;
+      } else {
+        __syncthreads();
+      }
     }
   }
   if (W.own) {
@@ -636,7 +652,7 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
     WGCellOp<T, DIM> O;
     O.load(K, W);
     wg_stage<T, DIM>(K, W, ph, sphi, sphl);
-    wg_relax<T, DIM>(K, W, O, rr, sphi, sphl, 2 * bottom);
+    wg_relax<T, DIM>(K, W, O, rr, sphi, sphl, 2 * bottom, true);
   }
   for (int l = L - 2; l >= l0; --l) {
     // coarse correction phi_{l+1} - phi0_{l+1} into LDS (compact layout)
