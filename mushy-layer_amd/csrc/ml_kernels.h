@@ -664,13 +664,14 @@ template <typename T>
 hipError_t vc_dot(const Geom& g, const T* x, const T* y, T* out, hipStream_t s);
 // batched form: relaxation launches whose workgroups are (job, tile) pairs
 // (map[w] = job << kBatchTileBits | tile), every job on the <= 512^2-cell
-// tile configuration; one coarse V-cycle workgroup per descriptor
+// tile configuration (2D shape 0: 16x16 cores, shape 1: 32x32 cores for
+// large batches); one coarse V-cycle workgroup per descriptor
 bool vc_batch_level_ok(const Geom& g);
 int vc_batch_hmax(int dim);
-int vc_batch_tiles(const Geom& g, int* ntx, int* nty);
-size_t vc_batch_lds(int dim, int npass, bool resc, bool fin);
+int vc_batch_tiles(const Geom& g, int shape, int* ntx, int* nty);
+size_t vc_batch_lds(int dim, int shape, int npass, bool resc, bool fin);
 template <typename T>
-hipError_t vc_relax_batch(int dim, const VCRelaxJob<T>* jobs, const unsigned* map, int nblocks, size_t lds,
+hipError_t vc_relax_batch(int dim, int shape, const VCRelaxJob<T>* jobs, const unsigned* map, int nblocks, size_t lds,
                           hipStream_t s, const LaunchTimer* tm = nullptr);
 template <typename T>
 hipError_t vc_coarse_batch(int dim, const VCDesc<T>* const* descs, int n, size_t lds, hipStream_t s);

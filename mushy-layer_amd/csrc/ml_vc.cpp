@@ -793,6 +793,9 @@ struct VCBatchLaunch {
 }  // namespace
 }  // namespace ml
 
+// level-0 cells of a batch from which its relaxation uses 32x32 tiles
+constexpr long long kVCBatchWideCells = 1LL << 19;
+
 struct ml_vc_batch_s {
   ml_ctx ctx = nullptr;
   int dim = 2;
@@ -808,6 +811,7 @@ struct ml_vc_batch_s {
   size_t desc_off = 0, ctlp_off = 0, ctl_off = 0;
   std::vector<const void*> key;   // what the uploaded tables were built for
   int batch_iters = 4;
+  int shape = 0;   // relaxation tile shape (vc_batch_tiles)
 
   ~ml_vc_batch_s() {
     if (arena) hipFree(arena);
@@ -892,7 +896,7 @@ int vcb_tables(ml_vc_batch b, double* const phi[], const double* const rhs[], co
           J.pc = W.e0[l + 1];
         }
         int ntx = 0, nty = 0;
-        const int nt = ml::vc_batch_tiles(J.A.g, &ntx, &nty);
+        const int nt = ml::vc_batch_tiles(J.A.g, b->shape, &ntx, &nty);
         J.ntx = ntx;
         J.nty = nty;
         if (!down && l == 0) {
@@ -903,7 +907,7 @@ int vcb_tables(ml_vc_batch b, double* const phi[], const double* const rhs[], co
           J.fin.fuse = ml::CtlFuse{W.ctl, W.slots, W.arrive, 0, unsigned(nt)};
         }
         if (J.npass + 1 > hmax) return fail(ctx, ML_ENOTSUP, "batched solve: too many smoothing passes per launch");
-        const size_t lds = ml::vc_batch_lds(b->dim, J.npass, J.resc != nullptr, J.fin.phi != nullptr);
+        const size_t lds = ml::vc_batch_lds(b->dim, b->shape, J.npass, J.resc != nullptr, J.fin.phi != nullptr);
         Lh.lds = std::max(Lh.lds, lds);
         const unsigned jl = unsigned(jobs[par].size() - Lh.job_off);
         for (int t = 0; t < nt; ++t) maps[par].push_back((jl << ml::kBatchTileBits) | unsigned(t));
@@ -970,7 +974,7 @@ int vcb_iteration(ml_vc_batch b, int par) {
   hipStream_t s = ctx->stream;
   const int K = b->K;
   auto launch = [&](const ml::VCBatchLaunch& Lh) -> int {
-    ML_HIP(ctx, ml::vc_relax_batch<double>(b->dim, reinterpret_cast<const ml::VCRelaxJob<double>*>(b->arena + Lh.job_off),
+    ML_HIP(ctx, ml::vc_relax_batch<double>(b->dim, b->shape, reinterpret_cast<const ml::VCRelaxJob<double>*>(b->arena + Lh.job_off),
                                            reinterpret_cast<const unsigned*>(b->arena + Lh.map_off), Lh.blocks, Lh.lds,
                                            s));
     return ML_OK;
@@ -1074,6 +1078,10 @@ int ml_vc_batch_create(ml_ctx ctx, int nops, const ml_vc_op ops[], int per_op, m
     for (int w = 0; w < per_op; ++w) b->units.push_back(ml::VCBatchUnit{op, w, op->lc});
     b->K = std::max(b->K, op->lc);
   }
+  // large batches are bandwidth-bound: wider tiles restage less halo
+  long long cells = 0;
+  for (const auto& U : b->units) cells += U.op->lev[0].g.nvalid();
+  b->shape = (b->dim == 2 && cells >= kVCBatchWideCells) ? 1 : 0;
   *out = b.release();
   return ML_OK;
 }

@@ -4,7 +4,8 @@ every launch covering all of them.  Each unit's result and statistics must
 equal the single-operator solve of the same unit bit for bit (the U* solves
 of ml_vc_solve_be_many, the projection of ml_vc_solve), with relaxation and
 BiCGStab bottoms, on uneven 2D grid sizes, and on a second call that
-reuses the batch's tables."""
+reuses the batch's tables; batches of at least 2^19 level-0 cells relax
+on 32x32 tiles instead of 16x16, with the same results."""
 import importlib
 
 import numpy as np
@@ -23,9 +24,9 @@ P_KW = dict(nu1=2, nu2=4, bottom=4, max_iter=20, eps=1e-15, hang=1e-15, norm_thr
 SIZES = {2: [(64, 64), (128, 128), (96, 40), (256, 256), (512, 256), (64, 64)]}
 
 
-def _instances(ml, ctx, dim):
+def _instances(ml, ctx, dim, sizes=None):
     out = []
-    for i, n in enumerate(SIZES[dim]):
+    for i, n in enumerate(sizes or SIZES[dim]):
         w = wlm.darcy_brinkman(n=n, seed=10 + i, corr=0.15, Da=10 ** (-3 + 0.4 * i))
         uop = ml.VCOp(ctx, n, w["dx"], w["periodic"], ml.make_bc(w["bc_u"]))
         pop = ml.VCOp(ctx, n, w["dx"], w["periodic"], ml.make_bc(w["bc_p"]))
@@ -117,3 +118,47 @@ def test_batch_refuses_unsupported_layouts(ml, gpu_ctx):
     op3 = ml.VCOp(ctx, (16, 16, 16), w3["dx"], w3["periodic"], ml.make_bc(w3["bc_u"]))
     with pytest.raises(RuntimeError, match="2D"):
         ml.VCBatch(ctx, [op3], per_op=1)
+
+
+def test_wide_tile_batches_equal_single_solves(ml, gpu_ctx):
+    """Ten 256^2 projections and five 256^2 U* pairs (655,360 level-0 cells
+    per batch: the 32x32-tile relaxation) against single-operator solves."""
+    ctx = gpu_ctx
+    inst = _instances(ml, ctx, 2, sizes=[(256, 256)] * 10)
+    pkw = dict(P_KW)
+    ukw = dict(U_KW)
+    pb = ml.VCBatch(ctx, [I["pop"] for I in inst], per_op=1)
+    ub = ml.VCBatch(ctx, [I["uop"] for I in inst[:5]], per_op=2)
+    rhss = [_rand(I["w"]["u0"].shape, 7000 + i, zero_mean=True, n=I["w"]["n"]) for i, I in enumerate(inst)]
+    ref = []
+    for i, I in enumerate(inst):
+        p = ml.DeviceArray(ctx, rhss[i].shape, host=np.zeros_like(rhss[i]))
+        r = ml.DeviceArray(ctx, rhss[i].shape, host=rhss[i])
+        st = I["pop"].solve(p, r, opts=ml.default_opts(**pkw))
+        ref.append((p.get(), _stats(st)))
+    pd = [ml.DeviceArray(ctx, x.shape, host=np.zeros_like(x)) for x in rhss]
+    rd = [ml.DeviceArray(ctx, x.shape, host=x) for x in rhss]
+    psts = pb.solve(pd, rd, opts=ml.default_opts(**pkw))
+    for i in range(len(inst)):
+        assert _stats(psts[i]) == ref[i][1], i
+        assert HC.bitwise_equal(pd[i].get(), ref[i][0]), i
+    dt = inst[0]["w"]["dt"]
+    olds = [[_rand(I["w"]["u0"].shape, 300 + 10 * i + c, n=I["w"]["n"]) for c in range(2)] for i, I in enumerate(inst[:5])]
+    srcs = [[_rand(I["w"]["u0"].shape, 900 + 10 * i + c, n=I["w"]["n"]) for c in range(2)] for i, I in enumerate(inst[:5])]
+    uref = []
+    for i, I in enumerate(inst[:5]):
+        new = [ml.DeviceArray(ctx, x.shape, host=x) for x in olds[i]]
+        old = [ml.DeviceArray(ctx, x.shape, host=x) for x in olds[i]]
+        src = [ml.DeviceArray(ctx, x.shape, host=x) for x in srcs[i]]
+        sts = I["uop"].solve_be_many(new, old, dt, src=src, opts=ml.default_opts(**ukw))
+        uref.append(([x.get() for x in new], [_stats(s) for s in sts]))
+    news = [ml.DeviceArray(ctx, x.shape, host=x) for xs in olds for x in xs]
+    oldd = [ml.DeviceArray(ctx, x.shape, host=x) for xs in olds for x in xs]
+    srcd = [ml.DeviceArray(ctx, x.shape, host=x) for xs in srcs for x in xs]
+    sts = ub.solve_be(news, oldd, dt, srcs=srcd, opts=ml.default_opts(**ukw))
+    for i in range(5):
+        for c in range(2):
+            assert _stats(sts[2 * i + c]) == uref[i][1][c], (i, c)
+            assert HC.bitwise_equal(news[2 * i + c].get(), uref[i][0][c]), (i, c)
+    pb.close()
+    ub.close()
