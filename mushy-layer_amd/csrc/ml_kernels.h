@@ -615,6 +615,21 @@ struct VCRelaxJob {
   VCFinal<T> fin;
 };
 
+// one batched residual (vc_residual's kVCResid tile pass) of one operator level
+template <typename T>
+struct VCResidJob {
+  VCArgs<T> A;
+  VCTileIO<T> io;
+};
+
+// dst[0..n) = src[0..n) as 8-byte words, or zeros when src is null
+struct WordsJob {
+  const unsigned long long* src;
+  unsigned long long* dst;
+  long long n;
+};
+constexpr int kWordsChunk = 4096;   // words per workgroup of words_batch
+
 template <typename T>
 struct VCDesc {
   int l0, nlev, nu1, nu2, bottom;
@@ -677,6 +692,15 @@ template <typename T>
 hipError_t vc_coarse_batch(int dim, const VCDesc<T>* const* descs, int n, size_t lds, hipStream_t s);
 // out[i] = *ctls[i] (the iteration control blocks of many solves, for one copy to the host)
 hipError_t ctl_gather(SolveCtl* const* ctls, SolveCtl* out, int n, hipStream_t s);
+// batched residuals over (job, tile) maps of the residual tile (vc_resid_tiles)
+int vc_resid_tiles(const Geom& g, int* ntx, int* nty);
+template <typename T>
+hipError_t vc_resid_batch(int dim, const VCResidJob<T>* jobs, const unsigned* map, int nblocks, hipStream_t s);
+// batched copies / zero fills over (job, chunk of kWordsChunk words) maps
+hipError_t words_batch(const WordsJob* jobs, const unsigned* map, int nblocks, hipStream_t s);
+// ctl_init of n control blocks (ctls[i] from slots[i][0]), one thread each
+hipError_t ctl_init_many(SolveCtl* const* ctls, const unsigned long long* const* slots, int n, int max_iter, double eps,
+                         double hang, double thresh, hipStream_t s);
 // dynamic LDS of vc_coarse_vcycle for levels l0..nlev-1 of g
 template <typename T>
 size_t vc_coarse_lds_bytes(const Geom* g, int l0, int nlev, bool bicgstab = false);

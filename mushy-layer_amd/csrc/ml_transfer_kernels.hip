@@ -106,9 +106,8 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
 // stopping rule of the multigrid solves (status bits: 1 |r| <= eps*r0,
 // 2 iteration maximum, 4 hang, 8 |r| <= thresh), evaluated by one thread
 
-__global__ void ctl_init_kernel(SolveCtl* c, const unsigned long long* slots, int max_iter, double eps, double hang,
-                                double thresh) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void ctl_init_one(SolveCtl* c, const unsigned long long* slots, int max_iter, double eps, double hang,
+                             double thresh) {
   const double r0 = __longlong_as_double((long long)slots[0]);
   c->iter = 0;
   c->need_host = 0;
@@ -128,6 +127,18 @@ __global__ void ctl_init_kernel(SolveCtl* c, const unsigned long long* slots, in
     c->status = 2;
     c->done = 1;
   }
+}
+
+__global__ void ctl_init_kernel(SolveCtl* c, const unsigned long long* slots, int max_iter, double eps, double hang,
+                                double thresh) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  ctl_init_one(c, slots, max_iter, eps, hang, thresh);
+}
+
+__global__ void ctl_init_many_kernel(SolveCtl* const* ctls, const unsigned long long* const* slots, int n,
+                                     int max_iter, double eps, double hang, double thresh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ctl_init_one(ctls[i], slots[i], max_iter, eps, hang, thresh);
 }
 
 __global__ void ctl_check_kernel(SolveCtl* c, const unsigned long long* slots, int refined) {
@@ -409,6 +420,14 @@ hipError_t zero_u64(unsigned long long* p, int n, hipStream_t s) {
 hipError_t ctl_init(SolveCtl* ctl, const unsigned long long* slots, int max_iter, double eps, double hang,
                     double thresh, hipStream_t s) {
   hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(64), 0, s, ctl, slots, max_iter, eps, hang, thresh);
+  return hipGetLastError();
+}
+
+hipError_t ctl_init_many(SolveCtl* const* ctls, const unsigned long long* const* slots, int n, int max_iter, double eps,
+                         double hang, double thresh, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ctl_init_many_kernel, dim3((n + 63) / 64), dim3(64), 0, s, ctls, slots, n, max_iter, eps, hang,
+                     thresh);
   return hipGetLastError();
 }
 

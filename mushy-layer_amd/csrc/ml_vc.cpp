@@ -809,6 +809,11 @@ struct ml_vc_batch_s {
   size_t arena_cap = 0;
   std::vector<ml::VCBatchLaunch> launch[2];   // per parity: K down, K up
   size_t desc_off = 0, ctlp_off = 0, ctl_off = 0;
+  // set-up tables of a solve: phi copies and slot zero fills, residuals,
+  // slot pointers; the copy-back table written after the iterations
+  size_t bw_job_off = 0, bw_map_off = 0, br_job_off = 0, br_map_off = 0, slotp_off = 0;
+  size_t ew_job_off = 0, ew_map_off = 0;
+  int bw_blocks = 0, br_blocks = 0;
   std::vector<const void*> key;   // what the uploaded tables were built for
   int batch_iters = 4;
   int shape = 0;   // relaxation tile shape (vc_batch_tiles)
@@ -845,6 +850,7 @@ int vcb_tables(ml_vc_batch b, double* const phi[], const double* const rhs[], co
     key.push_back(bits_of(b->units[u].op->beta));
   }
   key.push_back(reinterpret_cast<const void*>(std::uintptr_t(o.nu1 * 1000 + o.nu2)));
+  key.push_back(reinterpret_cast<const void*>(std::uintptr_t(o.max_iter)));
   if (key == b->key) return ML_OK;
   b->key.clear();
 
@@ -916,7 +922,44 @@ int vcb_tables(ml_vc_batch b, double* const phi[], const double* const rhs[], co
       Lh.blocks = int(maps[par].size() - Lh.map_off);
     }
   }
-  // layout: jobs0 | maps0 | jobs1 | maps1 | descs | ctl pointers | gathered ctls
+  // set-up of a solve: every unit's phi copied to its ping-pong partner and
+  // its norm slots zeroed, then its initial residual, then its control block
+  std::vector<ml::WordsJob> bw, ew;
+  std::vector<unsigned> bwm, ewm;
+  std::vector<ml::VCResidJob<double>> br;
+  std::vector<unsigned> brm;
+  auto add_words = [](std::vector<ml::WordsJob>& jv, std::vector<unsigned>& mv, const void* src, void* dst,
+                      long long nw) {
+    const unsigned j = unsigned(jv.size());
+    jv.push_back(ml::WordsJob{static_cast<const unsigned long long*>(src), static_cast<unsigned long long*>(dst), nw});
+    for (long long c = 0; c * ml::kWordsChunk < nw; ++c) mv.push_back((j << ml::kBatchTileBits) | unsigned(c));
+  };
+  for (int u = 0; u < n; ++u) {
+    ml_vc_op op = b->units[u].op;
+    ml::VCWork& W = op->work[b->units[u].w];
+    const ml::Geom& g0 = op->lev[0].g;
+    add_words(bw, bwm, phi[u], W.alt0, g0.ncell);
+    add_words(bw, bwm, nullptr, W.slots, o.max_iter + 1);
+    add_words(ew, ewm, W.alt0, phi[u], g0.ncell);   // capacity of the copy-back table
+    ml::VCResidJob<double> R{};
+    R.A = op->args(0);
+    R.io.phi = phi[u];
+    R.io.rhs = rhs[u];
+    R.io.out = W.r[0];
+    R.io.norm = W.slots;
+    R.io.homog = 0;
+    int ntx = 0, nty = 0;
+    const int nt = ml::vc_resid_tiles(g0, &ntx, &nty);
+    R.io.ntx = ntx;
+    R.io.nty = nty;
+    const unsigned j = unsigned(br.size());
+    for (int t = 0; t < nt; ++t) brm.push_back((j << ml::kBatchTileBits) | unsigned(t));
+    br.push_back(R);
+  }
+  if (bw.size() > (size_t(1) << (32 - ml::kBatchTileBits)))
+    return fail(ctx, ML_ENOTSUP, "batched solve: more than 2048 units");
+  // layout: jobs0 | maps0 | jobs1 | maps1 | descs | ctl pointers | set-up
+  // tables | slot pointers | copy-back tables | gathered ctls
   size_t off = 0;
   size_t joff[2], moff[2];
   for (int par = 0; par < 2; ++par) {
@@ -929,6 +972,22 @@ int vcb_tables(ml_vc_batch b, double* const phi[], const double* const rhs[], co
   off = vcb_align(off + n * sizeof(void*));
   b->ctlp_off = off;
   off = vcb_align(off + n * sizeof(void*));
+  b->bw_job_off = off;
+  off = vcb_align(off + bw.size() * sizeof(ml::WordsJob));
+  b->bw_map_off = off;
+  off = vcb_align(off + bwm.size() * sizeof(unsigned));
+  b->br_job_off = off;
+  off = vcb_align(off + br.size() * sizeof(ml::VCResidJob<double>));
+  b->br_map_off = off;
+  off = vcb_align(off + brm.size() * sizeof(unsigned));
+  b->slotp_off = off;
+  off = vcb_align(off + n * sizeof(void*));
+  b->ew_job_off = off;
+  off = vcb_align(off + ew.size() * sizeof(ml::WordsJob));
+  b->ew_map_off = off;
+  off = vcb_align(off + ewm.size() * sizeof(unsigned));
+  b->bw_blocks = int(bwm.size());
+  b->br_blocks = int(brm.size());
   b->ctl_off = off;
   off = vcb_align(off + n * sizeof(ml::SolveCtl));
   if (off > b->arena_cap) {
@@ -958,11 +1017,17 @@ int vcb_tables(ml_vc_batch b, double* const phi[], const double* const rhs[], co
   }
   void** dp = reinterpret_cast<void**>(b->harena + b->desc_off);
   void** cp = reinterpret_cast<void**>(b->harena + b->ctlp_off);
+  void** sp = reinterpret_cast<void**>(b->harena + b->slotp_off);
   for (int u = 0; u < n; ++u) {
     ml::VCWork& W = b->units[u].op->work[b->units[u].w];
     dp[u] = W.desc;
     cp[u] = W.ctl;
+    sp[u] = W.slots;
   }
+  std::memcpy(b->harena + b->bw_job_off, bw.data(), bw.size() * sizeof(ml::WordsJob));
+  std::memcpy(b->harena + b->bw_map_off, bwm.data(), bwm.size() * sizeof(unsigned));
+  std::memcpy(b->harena + b->br_job_off, br.data(), br.size() * sizeof(ml::VCResidJob<double>));
+  std::memcpy(b->harena + b->br_map_off, brm.data(), brm.size() * sizeof(unsigned));
   ML_HIP(ctx, hipMemcpyAsync(b->arena, b->harena, b->ctl_off, hipMemcpyHostToDevice, ctx->stream));
   b->key = key;
   return ML_OK;
@@ -1003,18 +1068,20 @@ int vcb_solve(ml_vc_batch b, double* const phi[], const double* const rhs[], con
   b->coarse_lds = 0;
   for (auto& U : b->units)
     b->coarse_lds = std::max(b->coarse_lds, o.bottom_solver == ML_BOTTOM_BICGSTAB ? U.op->coarse_lds_bicg : U.op->coarse_lds);
-  // begin: descriptors, phi copies, initial residuals and control blocks
+  // begin: descriptors, then for every unit in three launches its phi copy
+  // and zeroed norm slots, its initial residual, its control block
   for (int u = 0; u < n; ++u) {
     ml_vc_op op = b->units[u].op;
-    ml::VCWork& W = op->work[b->units[u].w];
-    if ((rc = vc_upload_desc(op, W, o, s))) return rc;
-    const ml::Geom& g0 = op->lev[0].g;
-    ML_HIP(ctx, hipMemcpyAsync(W.alt0, phi[u], sizeof(double) * g0.ncell, hipMemcpyDeviceToDevice, s));
-    ML_HIP(ctx, ml::zero_u64(W.slots, o.max_iter + 1, s));
-    ML_HIP(ctx, ml::vc_residual<double>(op->args(0), phi[u], nullptr, nullptr, rhs[u], W.r[0], W.slots, 0, s));
-    ML_HIP(ctx, ml::ctl_init(W.ctl, W.slots, o.max_iter, o.eps, o.hang, o.norm_thresh, s));
+    if ((rc = vc_upload_desc(op, op->work[b->units[u].w], o, s))) return rc;
   }
   if ((rc = vcb_tables(b, phi, rhs, o))) return rc;
+  ML_HIP(ctx, ml::words_batch(reinterpret_cast<const ml::WordsJob*>(b->arena + b->bw_job_off),
+                              reinterpret_cast<const unsigned*>(b->arena + b->bw_map_off), b->bw_blocks, s));
+  ML_HIP(ctx, ml::vc_resid_batch<double>(b->dim, reinterpret_cast<const ml::VCResidJob<double>*>(b->arena + b->br_job_off),
+                                         reinterpret_cast<const unsigned*>(b->arena + b->br_map_off), b->br_blocks, s));
+  ML_HIP(ctx, ml::ctl_init_many(reinterpret_cast<ml::SolveCtl* const*>(b->arena + b->ctlp_off),
+                                reinterpret_cast<const unsigned long long* const*>(b->arena + b->slotp_off), n,
+                                o.max_iter, o.eps, o.hang, o.norm_thresh, s));
   std::vector<ml::SolveCtl> c(n);
   int launched = 0;
   int nb = std::max(1, std::min(b->batch_iters, o.max_iter));
@@ -1032,13 +1099,33 @@ int vcb_solve(ml_vc_batch b, double* const phi[], const double* const rhs[], con
     if (all || launched >= o.max_iter) break;
     nb = 2;
   }
+  // units that stopped after an odd number of iterations hold their state
+  // in the partner buffer: copy it back, all in one launch
   int most = 1;
+  std::vector<ml::WordsJob> ew;
+  std::vector<unsigned> ewm;
   for (int u = 0; u < n; ++u) {
     ml_vc_op op = b->units[u].op;
     ml::VCWork& W = op->work[b->units[u].w];
     most = std::max(most, c[u].iter);
-    if (c[u].iter & 1)
-      ML_HIP(ctx, hipMemcpyAsync(phi[u], W.alt0, sizeof(double) * op->lev[0].g.ncell, hipMemcpyDeviceToDevice, s));
+    if (c[u].iter & 1) {
+      const long long nw = op->lev[0].g.ncell;
+      const unsigned j = unsigned(ew.size());
+      ew.push_back(ml::WordsJob{reinterpret_cast<const unsigned long long*>(W.alt0),
+                                reinterpret_cast<unsigned long long*>(phi[u]), nw});
+      for (long long ch = 0; ch * ml::kWordsChunk < nw; ++ch) ewm.push_back((j << ml::kBatchTileBits) | unsigned(ch));
+    }
+  }
+  if (!ew.empty()) {
+    // the host mirror is idle here (the stream was synchronised above)
+    std::memcpy(b->harena + b->ew_job_off, ew.data(), ew.size() * sizeof(ml::WordsJob));
+    std::memcpy(b->harena + b->ew_map_off, ewm.data(), ewm.size() * sizeof(unsigned));
+    ML_HIP(ctx, hipMemcpyAsync(b->arena + b->ew_job_off, b->harena + b->ew_job_off, b->ctl_off - b->ew_job_off,
+                               hipMemcpyHostToDevice, s));
+    ML_HIP(ctx, ml::words_batch(reinterpret_cast<const ml::WordsJob*>(b->arena + b->ew_job_off),
+                                reinterpret_cast<const unsigned*>(b->arena + b->ew_map_off), int(ewm.size()), s));
+  }
+  for (int u = 0; u < n; ++u) {
     if (st) {
       st[u].exit_status = c[u].status != 0 ? c[u].status : 2;
       st[u].iterations = c[u].iter;

@@ -275,14 +275,14 @@ __global__ __launch_bounds__(NT) void vc_relax_batch_kernel(const VCRelaxJob<T>*
 //           (res optional), max-norm into *norm (optional);
 // kVCApply: out = L u;  kVCRestrict: out (coarse) = child sums of (rhs - L u)/2^d.
 template <typename T, int DIM, int TX, int TY, int TZ, int MODE>
-__global__ __launch_bounds__(kVCThreads) void vc_tile_kernel(VCArgs<T> A, VCTileIO<T> io) {
+__device__ __forceinline__ void vc_tile_body(const VCArgs<T>& A, const VCTileIO<T>& io, int tile) {
   if (A.skip && *A.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int EMAX = (TX + 2) * (TY + 2) * (DIM == 3 ? TZ + 2 : 1);
   constexpr int KMAX = (EMAX + kVCThreads - 1) / kVCThreads;
   const Geom& g = A.g;
   Tile<DIM, TX, TY, TZ> t;
-  t.init(1, io.ntx, io.nty);
+  t.init_at(1, io.ntx, io.nty, tile);
   T* su = reinterpret_cast<T*>(smem_raw);
   const T dxinv = T(1) / (T(g.dx) * T(g.dx));
   PosRec R[KMAX];
@@ -387,6 +387,31 @@ __global__ __launch_bounds__(kVCThreads) void vc_tile_kernel(VCArgs<T> A, VCTile
           s = s + sres[(2 * cx + ii) + TX * ((2 * cy + jj) + TY * (DIM == 3 ? 2 * cz + kk : 0))];
     io.out[gcc.ci(I, J, K)] = s;
   }
+}
+
+template <typename T, int DIM, int TX, int TY, int TZ, int MODE>
+__global__ __launch_bounds__(kVCThreads) void vc_tile_kernel(VCArgs<T> A, VCTileIO<T> io) {
+  vc_tile_body<T, DIM, TX, TY, TZ, MODE>(A, io, int(blockIdx.x));
+}
+
+// batched residuals: workgroup w computes tile map[w] & mask of job map[w] >> bits
+template <typename T, int DIM, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kVCThreads) void vc_resid_batch_kernel(const VCResidJob<T>* __restrict__ jobs,
+                                                                    const unsigned* __restrict__ map) {
+  const unsigned w = map[blockIdx.x];
+  const VCResidJob<T>& J = jobs[w >> kBatchTileBits];
+  vc_tile_body<T, DIM, TX, TY, TZ, kVCResid>(J.A, J.io, int(w & kBatchTileMask));
+}
+
+// batched 8-byte copies and zero fills: workgroup w handles chunk map[w] &
+// mask of job map[w] >> bits
+__global__ __launch_bounds__(256) void words_batch_kernel(const WordsJob* __restrict__ jobs,
+                                                          const unsigned* __restrict__ map) {
+  const unsigned w = map[blockIdx.x];
+  const WordsJob& J = jobs[w >> kBatchTileBits];
+  const long long c0 = (long long)(w & kBatchTileMask) * kWordsChunk;
+  const long long c1 = c0 + kWordsChunk < J.n ? c0 + kWordsChunk : J.n;
+  for (long long i = c0 + threadIdx.x; i < c1; i += 256) J.dst[i] = J.src ? J.src[i] : 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -1374,6 +1399,34 @@ hipError_t vc_residual(const VCArgs<T>& A, const T* phi, const T* e, T* phi_out,
   return A.g.dim == 2 ? vc_tile_dim<T, 2, kVCResid>(A, io, s) : vc_tile_dim<T, 3, kVCResid>(A, io, s);
 }
 
+int vc_resid_tiles(const Geom& g, int* ntx, int* nty) {
+  return g.dim == 2 ? vc_tiles<VCTile<2>::TX, VCTile<2>::TY, VCTile<2>::TZ>(g, *ntx, *nty)
+                    : vc_tiles<VCTile<3>::TX, VCTile<3>::TY, VCTile<3>::TZ>(g, *ntx, *nty);
+}
+
+template <typename T>
+hipError_t vc_resid_batch(int dim, const VCResidJob<T>* jobs, const unsigned* map, int nblocks, hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  if (dim == 2) {
+    using D = VCTile<2>;
+    const size_t lds = size_t(D::TX + 2) * (D::TY + 2) * sizeof(T);
+    hipLaunchKernelGGL((vc_resid_batch_kernel<T, 2, D::TX, D::TY, D::TZ>), dim3(nblocks), dim3(kVCThreads), lds, s,
+                       jobs, map);
+  } else {
+    using D = VCTile<3>;
+    const size_t lds = size_t(D::TX + 2) * (D::TY + 2) * (D::TZ + 2) * sizeof(T);
+    hipLaunchKernelGGL((vc_resid_batch_kernel<T, 3, D::TX, D::TY, D::TZ>), dim3(nblocks), dim3(kVCThreads), lds, s,
+                       jobs, map);
+  }
+  return hipGetLastError();
+}
+
+hipError_t words_batch(const WordsJob* jobs, const unsigned* map, int nblocks, hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(words_batch_kernel, dim3(nblocks), dim3(256), 0, s, jobs, map);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t vc_apply_op(const VCArgs<T>& A, const T* phi, T* lhs, int homog, hipStream_t s) {
   VCTileIO<T> io{};
@@ -1567,5 +1620,6 @@ template size_t vc_coarse_lds_bytes<double>(const Geom*, int, int, bool);
 template hipError_t vc_relax_batch<double>(int, int, const VCRelaxJob<double>*, const unsigned*, int, size_t,
                                            hipStream_t, const LaunchTimer*);
 template hipError_t vc_coarse_batch<double>(int, const VCDesc<double>* const*, int, size_t, hipStream_t);
+template hipError_t vc_resid_batch<double>(int, const VCResidJob<double>*, const unsigned*, int, hipStream_t);
 
 }  // namespace ml
