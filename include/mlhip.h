@@ -324,6 +324,22 @@ int ml_vc_batch_solve_be(ml_vc_batch b, double* const phi_new[], const double* c
 int ml_vc_solve_be_many(ml_vc_op op, int n, double* const phi_new[], const double* const phi_old[],
                         const double* const src[], double dt, const ml_solve_opts* opts,
                         ml_solve_stats* stats);
+/* Coefficient set-up of n independent operators of one context and
+ * dimension, equal to n calls of ml_vc_set_coefs / ml_hc_set_coefs
+ * (src/DarcyBrinkmanOp.cpp:250-263,977-1008; src/AMRNonLinearMultiCompOp.H
+ * :75-93 with the hierarchy of src/AMRNonLinearMultiCompOp.cpp:1423-1494)
+ * but in three launches for all of them: the level-0 copies, then the
+ * hierarchy averages (one thread, or one workgroup, per coarse item).
+ * bCoef holds n*dim pointers (operator i's direction d at i*dim + d);
+ * aCoef and cCoef are n pointers (entries may be NULL: the operator then
+ * has no a, or no c, term) or NULL for all.  No counterpart in the
+ * reference, which sets its one operator up at a time; the batched
+ * Darcy-Brinkman steps use it. */
+int ml_vc_set_coefs_many(ml_ctx ctx, int n, const ml_vc_op ops[], const double* const aCoef[],
+                         const double* const bCoef[], const double* const cCoef[], const double alpha[],
+                         const double beta[]);
+int ml_hc_set_coefs_many(ml_ctx ctx, int n, const ml_hc_op ops[], const double* const aCoef[],
+                         const double* const bCoef[], const double alpha[], const double beta[]);
 /* U* Darcy coefficient on valid cells: cCoef = (chi / Pi(chi)) * darcy_coeff,
  * Kozeny-Carman Pi = chi^3/(1-chi)^2 (srcSubcycle/AMRLevelMushyLayerInit.cpp
  * :665-689, src/MushyLayerParams.cpp:1046-1049); darcy_coeff = Pr/Da */

@@ -184,6 +184,10 @@ SIGNATURES = {
     "ml_hc_batch_step_src": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.c_double, C.POINTER(SolveOpts),
                                        C.POINTER(SolveStats)]),
     "ml_vc_batch_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.c_int, C.POINTER(_vp)]),
+    "ml_vc_set_coefs_many": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "ml_hc_set_coefs_many": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "ml_vc_batch_destroy": (C.c_int, [_vp]),
     "ml_vc_batch_solve": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "ml_vc_batch_solve_be": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.c_double,
@@ -376,6 +380,42 @@ class DeviceArray:
         if self.ptr:
             self.ctx.free(self.ptr)
             self.ptr = None
+
+
+def _ptrs(xs):
+    arr = (_vp * max(1, len(xs)))()
+    for i, x in enumerate(xs):
+        arr[i] = x.ptr if x is not None else None
+    return arr
+
+
+def _doubles(v):
+    return (C.c_double * max(1, len(v)))(*v)
+
+
+def vc_set_coefs_many(ctx, ops, a, b, c, alpha, beta):
+    """VCOp.set_coefs of many operators in a few launches (ml_vc_set_coefs_many):
+    a, c: lists of DeviceArray or None (or None for all); b: list of lists of
+    face arrays; alpha, beta: scalars or lists."""
+    n = len(ops)
+    alpha = alpha if isinstance(alpha, (list, tuple)) else [alpha] * n
+    beta = beta if isinstance(beta, (list, tuple)) else [beta] * n
+    hs = (_vp * max(1, n))(*[op.h for op in ops])
+    ctx.check(ctx.lib.ml_vc_set_coefs_many(ctx.h, n, hs, _ptrs(a) if a is not None else None,
+                                           _ptrs([x for bs in b for x in bs]),
+                                           _ptrs(c) if c is not None else None, _doubles(alpha), _doubles(beta)),
+              "ml_vc_set_coefs_many")
+
+
+def hc_set_coefs_many(ctx, ops, a, b, alpha, beta):
+    """HCOp.set_coefs of many operators in a few launches (ml_hc_set_coefs_many)."""
+    n = len(ops)
+    alpha = alpha if isinstance(alpha, (list, tuple)) else [alpha] * n
+    beta = beta if isinstance(beta, (list, tuple)) else [beta] * n
+    hs = (_vp * max(1, n))(*[op.h for op in ops])
+    ctx.check(ctx.lib.ml_hc_set_coefs_many(ctx.h, n, hs, _ptrs(a) if a is not None else None,
+                                           _ptrs([x for bs in b for x in bs]), _doubles(alpha), _doubles(beta)),
+              "ml_hc_set_coefs_many")
 
 
 def cell_shape(n, ncomp):

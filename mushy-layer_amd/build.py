@@ -24,7 +24,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["ml_hc_kernels.hip", "ml_transfer_kernels.hip", "ml_fp32r.hip", "ml_vc_kernels.hip", "ml_capi.cpp",
            "ml_hc_engine.cpp", "ml_vc.cpp", "ml_batch.hip", "ml_batch.cpp",
            "ml_adjoint.hip", "ml_noalloc.cpp", "ml_flow_kernels.hip", "ml_flow.cpp", "ml_boxes.hip", "ml_hc_methods.hip",
-           "ml_advect.hip"]
+           "ml_advect.hip", "ml_many.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

@@ -110,6 +110,7 @@ int ml_ctx_destroy(ml_ctx ctx) {
     if (e) hipEventDestroy(e);
   for (auto& e : ctx->timing_start) hipEventDestroy(e);
   for (auto& e : ctx->timing_stop) hipEventDestroy(e);
+  ctx->tables.release();
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return ML_OK;

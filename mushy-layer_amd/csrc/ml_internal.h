@@ -15,6 +15,25 @@
 #include "ml_kernels.h"
 #include "mlhip.h"
 
+namespace ml {
+
+// Job tables of the batched calls, sent through pinned staging memory: two
+// pinned buffers used in turn (the host waits only for the copy out of the
+// buffer it refills) and one device buffer, reused in stream order (a copy
+// into it runs after every kernel enqueued before it).
+struct TableStage {
+  void* dev = nullptr;
+  size_t dev_cap = 0;
+  void* host[2] = {nullptr, nullptr};
+  size_t host_cap[2] = {0, 0};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int next = 0;
+  hipError_t put(const void* src, size_t bytes, hipStream_t s, void** out);
+  void release();
+};
+
+}  // namespace ml
+
 struct ml_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -22,6 +41,7 @@ struct ml_ctx_s {
   unsigned long long* d_norm = nullptr;   // device scratch for max-norms
   unsigned long long* h_norm = nullptr;   // pinned host mirror
   hipEvent_t events[16] = {};
+  ml::TableStage tables;                    // job tables of the batched calls
   // launch timing of one kernel family (ml_ctx_timing_start/stop)
   int timing_kernel = 0;
   int timing_n = 0;

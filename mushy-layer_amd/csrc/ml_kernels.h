@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <vector>
 
 #include "ml_common.h"
 
@@ -524,6 +525,26 @@ struct CoarsenJobs {
 template <typename T>
 hipError_t coarsen_hierarchy(const Geom& gf, int ncomp, int nlev, const T* const* bf, T* const* const* bc, const T* af,
                              T* const* ac, hipStream_t s, bool faces = true);
+// One average of a hierarchy (level, kind as in CoarsenJobs; count = coarse
+// items times components) for the batched form, which averages the
+// hierarchies of many operators in two launches (light items one thread
+// each, heavy items one workgroup each; the same arithmetic as
+// coarsen_hierarchy)
+template <typename T>
+struct CoarsenRec {
+  Geom gf;
+  int level, kind;
+  const T* src;
+  T* dst;
+  long long count;
+};
+template <typename T>
+void coarsen_records(const Geom& gf, int ncomp, int nlev, const T* const* bf, T* const* const* bc, const T* af,
+                     T* const* ac, bool faces, std::vector<CoarsenRec<T>>& light, std::vector<CoarsenRec<T>>& heavy);
+// lmap[w] = light record << 32 | chunk of 256 items, hmap[w] = heavy record << 32 | item
+template <typename T>
+hipError_t coarsen_recs(int dim, const CoarsenRec<T>* light, const unsigned long long* lmap, int lblocks,
+                        const CoarsenRec<T>* heavy, const unsigned long long* hmap, int hblocks, hipStream_t s);
 hipError_t porosity_field(const Geom& g, const ClosureP<double>& cp, const double* phi, double* chi,
                           hipStream_t s);
 hipError_t hc_bcoef(const Geom& g, double k, double DH, double DC, const double* chi, double* const* b,

@@ -152,20 +152,17 @@ __global__ void ctl_check_kernel(SolveCtl* c, const unsigned long long* slots, i
 // summed lowest transverse index fastest, times 1/ratio^(d-1).  Cells: each
 // fine row (fixed j, k) summed i fastest, the row sums added in (j, k)
 // order, times 1/ratio^d.
+// One coarse item (face or cell, all components flattened) of the average
+// of level `level` from the finest level gf: src the fine faces of
+// direction kind (kind < 3) or the fine cells (kind 3).
 template <typename T, int DIM>
-__global__ __launch_bounds__(256) void coarsen_hierarchy_kernel(Geom gf, int ncomp, CoarsenJobs<T> J, const T* bf0,
-                                                                const T* bf1, const T* bf2, const T* af) {
-  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t >= J.begin[J.njobs]) return;
-  int jb = 0;
-  while (t >= J.begin[jb + 1]) ++jb;
-  const long long item = t - J.begin[jb];
-  const int ratio = 1 << J.level[jb];
+__device__ __forceinline__ void coarsen_one(const Geom& gf, int level, int kind, const T* __restrict__ src,
+                                            T* __restrict__ dst, long long item) {
+  const int ratio = 1 << level;
   const Geom gc = coarsen_geom(gf, ratio);
-  const int kind = J.kind[jb];
   if (kind < 3) {
     const int d = kind;
-    const T* bf = d == 0 ? bf0 : (d == 1 ? bf1 : bf2);
+    const T* bf = src;
     int r[3] = {ratio, ratio, DIM == 3 ? ratio : 1};
     r[d] = 1;
     const T scale = T(1) / T(r[0] * r[1] * r[2]);
@@ -180,8 +177,9 @@ __global__ __launch_bounds__(256) void coarsen_hierarchy_kernel(Geom gf, int nco
 #pragma unroll 8
         for (int ii = 0; ii < r[0]; ++ii)
           sum = sum + bf[gf.fi(d, base[0] + ii, base[1] + jj, base[2] + kk) + n * gf.nface[d]];
-    J.dst[jb][gc.fi(d, I, Jj, K) + n * gc.nface[d]] = sum * scale;
+    dst[gc.fi(d, I, Jj, K) + n * gc.nface[d]] = sum * scale;
   } else {
+    const T* af = src;
     const int rz = DIM == 3 ? ratio : 1;
     const T scale = T(1) / T(ratio * ratio * rz);
     const long long N = gc.nvalid();
@@ -197,31 +195,48 @@ __global__ __launch_bounds__(256) void coarsen_hierarchy_kernel(Geom gf, int nco
         for (int ii = 0; ii < ratio; ++ii) rs = rs + row[ii];
         sum = sum + rs;
       }
-    J.dst[jb][gc.ci(I, Jj, K) + n * gc.ncell] = sum * scale;
+    dst[gc.ci(I, Jj, K) + n * gc.ncell] = sum * scale;
   }
 }
 
-// Heavy cell jobs of the hierarchy (large ratios): one workgroup per coarse
-// cell; each thread sums fine rows (i fastest) into LDS, then one thread
-// adds the row sums in (j, k) order.
 template <typename T, int DIM>
-__global__ __launch_bounds__(256) void coarsen_cells_block_kernel(Geom gf, int ncomp, CoarsenJobs<T> J,
-                                                                  const T* __restrict__ af, const T* bf0,
-                                                                  const T* bf1, const T* bf2) {
-  constexpr int kRows = 2048;
-  __shared__ T rows[kRows];
-  const long long t = blockIdx.x;
+__global__ __launch_bounds__(256) void coarsen_hierarchy_kernel(Geom gf, int ncomp, CoarsenJobs<T> J, const T* bf0,
+                                                                const T* bf1, const T* bf2, const T* af) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t >= J.begin[J.njobs]) return;
   int jb = 0;
   while (t >= J.begin[jb + 1]) ++jb;
-  const long long item = t - J.begin[jb];
-  const int ratio = 1 << J.level[jb];
+  const int kind = J.kind[jb];
+  const T* src = kind == 0 ? bf0 : (kind == 1 ? bf1 : (kind == 2 ? bf2 : af));
+  coarsen_one<T, DIM>(gf, J.level[jb], kind, src, J.dst[jb], t - J.begin[jb]);
+}
+
+// batched hierarchies: workgroup w averages 256 items of record map[w] >> 32
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void coarsen_rec_kernel(const CoarsenRec<T>* __restrict__ recs,
+                                                          const unsigned long long* __restrict__ map) {
+  const unsigned long long w = map[blockIdx.x];
+  const CoarsenRec<T>& R = recs[w >> 32];
+  const long long item = (long long)(w & 0xffffffffull) * 256 + threadIdx.x;
+  if (item >= R.count) return;
+  coarsen_one<T, DIM>(R.gf, R.level, R.kind, R.src, R.dst, item);
+}
+
+// Heavy items of the hierarchy (large ratios): one workgroup per coarse
+// item.  Faces: the fine sub-faces loaded in parallel, summed in the light
+// kernel's order (lowest transverse index fastest) by one thread.  Cells:
+// each thread sums fine rows (i fastest) into LDS, then one thread adds the
+// row sums in (j, k) order.
+template <typename T, int DIM>
+__device__ __forceinline__ void coarsen_block_one(const Geom& gf, int level, int kind, const T* __restrict__ src,
+                                                  T* __restrict__ dst, long long item) {
+  constexpr int kRows = 2048;
+  __shared__ T rows[kRows];
+  const int ratio = 1 << level;
   const Geom gc = coarsen_geom(gf, ratio);
-  if (J.kind[jb] < 3) {
-    // one coarse face: its fine sub-faces loaded in parallel, summed in the
-    // light kernel's order (lowest transverse index fastest) by one thread
-    const int d = J.kind[jb];
-    const T* bf = d == 0 ? bf0 : (d == 1 ? bf1 : bf2);
+  if (kind < 3) {
+    const int d = kind;
+    const T* bf = src;
     int r[3] = {ratio, ratio, DIM == 3 ? ratio : 1};
     r[d] = 1;
     const T scale = T(1) / T(r[0] * r[1] * r[2]);
@@ -243,9 +258,10 @@ __global__ __launch_bounds__(256) void coarsen_cells_block_kernel(Geom gf, int n
         for (int u = 0; u < nu; ++u) sum = sum + rows[u];
       __syncthreads();
     }
-    if (threadIdx.x == 0) J.dst[jb][gc.fi(d, I, Jj, K) + n * gc.nface[d]] = sum * scale;
+    if (threadIdx.x == 0) dst[gc.fi(d, I, Jj, K) + n * gc.nface[d]] = sum * scale;
     return;
   }
+  const T* af = src;
   const int rz = DIM == 3 ? ratio : 1;
   const T scale = T(1) / T(ratio * ratio * rz);
   const long long N = gc.nvalid();
@@ -258,9 +274,9 @@ __global__ __launch_bounds__(256) void coarsen_cells_block_kernel(Geom gf, int n
     const int nr = (nrows - r0) < kRows ? (nrows - r0) : kRows;
     for (int u = threadIdx.x; u < nr; u += blockDim.x) {
       const int row = r0 + u, jj = row % ratio, kk = row / ratio;
-      const T* src = af + gf.ci(I * ratio, Jj * ratio + jj, K * rz + kk) + n * gf.ncell;
+      const T* srow = af + gf.ci(I * ratio, Jj * ratio + jj, K * rz + kk) + n * gf.ncell;
       T rs = T(0);
-      for (int ii = 0; ii < ratio; ++ii) rs = rs + src[ii];
+      for (int ii = 0; ii < ratio; ++ii) rs = rs + srow[ii];
       rows[u] = rs;
     }
     __syncthreads();
@@ -268,10 +284,31 @@ __global__ __launch_bounds__(256) void coarsen_cells_block_kernel(Geom gf, int n
       for (int u = 0; u < nr; ++u) sum = sum + rows[u];
     __syncthreads();
   }
-  if (threadIdx.x == 0) J.dst[jb][gc.ci(I, Jj, K) + n * gc.ncell] = sum * scale;
+  if (threadIdx.x == 0) dst[gc.ci(I, Jj, K) + n * gc.ncell] = sum * scale;
 }
 
-// CALCULATEPOROSITY_MULTICOMP over valid cells (src/EnthalpyVariablesF.ChF:56-94).
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void coarsen_cells_block_kernel(Geom gf, int ncomp, CoarsenJobs<T> J,
+                                                                  const T* __restrict__ af, const T* bf0,
+                                                                  const T* bf1, const T* bf2) {
+  const long long t = blockIdx.x;
+  if (t >= J.begin[J.njobs]) return;
+  int jb = 0;
+  while (t >= J.begin[jb + 1]) ++jb;
+  const int kind = J.kind[jb];
+  const T* src = kind == 0 ? bf0 : (kind == 1 ? bf1 : (kind == 2 ? bf2 : af));
+  coarsen_block_one<T, DIM>(gf, J.level[jb], kind, src, J.dst[jb], t - J.begin[jb]);
+}
+
+// batched heavy items: workgroup w averages item map[w] & 0xffffffff of record map[w] >> 32
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void coarsen_rec_block_kernel(const CoarsenRec<T>* __restrict__ recs,
+                                                                const unsigned long long* __restrict__ map) {
+  const unsigned long long w = map[blockIdx.x];
+  const CoarsenRec<T>& R = recs[w >> 32];
+  coarsen_block_one<T, DIM>(R.gf, R.level, R.kind, R.src, R.dst, (long long)(w & 0xffffffffull));
+}
+
 __global__ void porosity_kernel(Geom g, ClosureP<double> cp, const double* __restrict__ phi,
                                 double* __restrict__ chi) {
   const long long N = g.nvalid();
@@ -501,6 +538,49 @@ hipError_t coarsen_hierarchy(const Geom& gf, int ncomp, int nlev, const T* const
                        bf[2], af);
   return hipGetLastError();
 }
+
+template <typename T>
+void coarsen_records(const Geom& gf, int ncomp, int nlev, const T* const* bf, T* const* const* bc, const T* af,
+                     T* const* ac, bool faces, std::vector<CoarsenRec<T>>& light, std::vector<CoarsenRec<T>>& heavy) {
+  // the split of coarsen_hierarchy: averages over at least 64 fine cells or
+  // faces are heavy
+  constexpr long long kHeavy = 64;
+  for (int l = nlev - 1; l >= 1; --l) {
+    const Geom gc = coarsen_geom(gf, 1 << l);
+    const long long block = (1LL << l) * (1LL << l) * (gf.dim == 3 ? (1LL << l) : 1);
+    const long long fblock = block >> l;
+    if (faces)
+      for (int d = 0; d < gf.dim; ++d)
+        (fblock >= kHeavy ? heavy : light).push_back(CoarsenRec<T>{gf, l, d, bf[d], bc[l][d], gc.nface[d] * ncomp});
+    if (af) (block >= kHeavy ? heavy : light).push_back(CoarsenRec<T>{gf, l, 3, af, ac[l], gc.nvalid() * ncomp});
+  }
+}
+
+template <typename T>
+hipError_t coarsen_recs(int dim, const CoarsenRec<T>* light, const unsigned long long* lmap, int lblocks,
+                        const CoarsenRec<T>* heavy, const unsigned long long* hmap, int hblocks, hipStream_t s) {
+  if (hblocks > 0) {
+    if (dim == 2)
+      hipLaunchKernelGGL((coarsen_rec_block_kernel<T, 2>), dim3(hblocks), dim3(256), 0, s, heavy, hmap);
+    else
+      hipLaunchKernelGGL((coarsen_rec_block_kernel<T, 3>), dim3(hblocks), dim3(256), 0, s, heavy, hmap);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (lblocks > 0) {
+    if (dim == 2)
+      hipLaunchKernelGGL((coarsen_rec_kernel<T, 2>), dim3(lblocks), dim3(256), 0, s, light, lmap);
+    else
+      hipLaunchKernelGGL((coarsen_rec_kernel<T, 3>), dim3(lblocks), dim3(256), 0, s, light, lmap);
+  }
+  return hipGetLastError();
+}
+
+template void coarsen_records<double>(const Geom&, int, int, const double* const*, double* const* const*,
+                                      const double*, double* const*, bool, std::vector<CoarsenRec<double>>&,
+                                      std::vector<CoarsenRec<double>>&);
+template hipError_t coarsen_recs<double>(int, const CoarsenRec<double>*, const unsigned long long*, int,
+                                         const CoarsenRec<double>*, const unsigned long long*, int, hipStream_t);
 
 hipError_t porosity_field(const Geom& g, const ClosureP<double>& cp, const double* phi, double* chi,
                           hipStream_t s) {

@@ -30,7 +30,7 @@ device.
 import numpy as np
 
 from . import (DeviceArray, HCOp, VCOp, default_opts, face_shape, make_advect_params, make_bc, make_flow_params,
-               make_params, valid)
+               make_params, valid, hc_set_coefs_many, vc_set_coefs_many)
 
 # multigrid options of the velocity solves: U* (VelocityMultigrid,
 # srcSubcycle/MushyLayerSubcycleUtils.cpp:654-659: smoothing 2 down, up and
@@ -317,14 +317,15 @@ class BatchDarcyBrinkmanStepper:
         for s in sts:
             s.op.enthalpy_vars(s.phi, s.vold)
             s.op.assemble_bcoef(s.vold.comp_ptr(3), s.b)
-            s.op.set_coefs(None, s.b, 1.0, -dt)
+        hc_set_coefs_many(self.ctx, [s.op for s in sts], None, [s.b for s in sts], 1.0, -dt)
         srcs = [s.hc_src() for s in sts]
         hst = self.hcb.step([s.phi for s in sts], dt, self.hc_opts, srcs=srcs)
         # enthalpy variables at t^{n+1}, U* coefficients and sources
         for s in sts:
             s.op.enthalpy_vars(s.phi, s.vnew)
             s.uop.darcy_ccoef_law(s.vold.comp_ptr(3), s.coef["darcy"], s.law, s.c)
-            s.uop.set_coefs(None, s.bu, s.c, 1.0, -dt)
+        vc_set_coefs_many(self.ctx, [s.uop for s in sts], None, [s.bu for s in sts], [s.c for s in sts], 1.0, -dt)
+        for s in sts:
             s.uop.ustar_src(s.fp, s.vold, s.vnew, s.pi, s.src)
             for u, uo in zip(s.u, s.uold):
                 s.copy(uo, u)
@@ -334,8 +335,8 @@ class BatchDarcyBrinkmanStepper:
         for s in sts:
             s.uop.cc_project_rhs(s.fp, dt, s.vold, s.vnew, s.pi, s.ustar, s.u, s.prhs)
             s.pop.pressure_faces(s.vold, s.vnew, s.bp)
-            s.pop.set_coefs(None, s.bp, None, 0.0, 1.0)
             s._zero(s.pi)
+        vc_set_coefs_many(self.ctx, [s.pop for s in sts], None, [s.bp for s in sts], None, 0.0, 1.0)
         pst = self.pb.solve([s.pi for s in sts], [s.prhs for s in sts], opts=self.popts)
         for s in sts:
             s.uop.cc_correct(s.fp, dt, s.vold, s.vnew, s.pi, s.u)

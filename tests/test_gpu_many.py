@@ -1,0 +1,113 @@
+"""Batched coefficient set-up (ml_vc_set_coefs_many, ml_hc_set_coefs_many):
+the level-0 copies and coarsened hierarchies of many operators in a few
+launches must equal one ml_vc_set_coefs / ml_hc_set_coefs per operator.
+Checked through what the hierarchies feed: solves on operators set up both
+ways, on uneven 2D and 3D sizes, with and without a and c terms, must agree
+bit for bit in state and statistics."""
+import importlib
+
+import numpy as np
+import pytest
+
+from tests import hc_cases as HC
+
+pytestmark = pytest.mark.gpu
+
+wlm = importlib.import_module("mushy-layer_amd.workloads")
+
+
+def _stats(s):
+    return (s.exit_status, s.iterations, s.initial_residual, s.final_residual)
+
+
+def _rand(shape, seed, n, zero_mean=False):
+    rng = np.random.default_rng(seed)
+    x = np.zeros(shape)
+    v = HC.valid(n, x)
+    v[...] = rng.standard_normal(v.shape)
+    if zero_mean:
+        v[...] -= v.mean()
+    return x
+
+
+@pytest.mark.parametrize("sizes", [[(64, 64), (96, 40), (256, 256), (40, 33)], [(16, 16, 16), (24, 16, 8)]],
+                         ids=["2d", "3d"])
+def test_vc_set_coefs_many_equals_per_operator(ml, gpu_ctx, sizes):
+    ctx = gpu_ctx
+    opts = ml.default_opts(nu1=2, nu2=2, bottom=4, max_iter=6, eps=1e-12, hang=1e-12, norm_thresh=1e-14)
+    ops1, ops2, a_l, b_l, c_l, keep, rhs = [], [], [], [], [], [], []
+    for i, n in enumerate(sizes):
+        w = wlm.darcy_brinkman(n=n, seed=20 + i, corr=0.2)
+        mk = lambda: ml.VCOp(ctx, n, w["dx"], w["periodic"], ml.make_bc(w["bc_u"]))
+        ops1.append(mk())
+        ops2.append(mk())
+        b = [ml.DeviceArray(ctx, x.shape, host=x) for x in w["b_u"]]
+        # operator i: a term on odd i, c term on even i
+        a = ml.DeviceArray(ctx, w["c"].shape, host=1.0 + 0.1 * np.abs(w["c"])) if i % 2 else None
+        c = ml.DeviceArray(ctx, w["c"].shape, host=w["c"]) if i % 2 == 0 else None
+        a_l.append(a)
+        b_l.append(b)
+        c_l.append(c)
+        keep += b + [x for x in (a, c) if x is not None]
+        rhs.append(_rand(w["u0"].shape, 50 + i, n))
+    for i, op in enumerate(ops1):
+        op.set_coefs(a_l[i], b_l[i], c_l[i], 1.0, -1e-3 * (i + 1))
+    ml.vc_set_coefs_many(ctx, ops2, a_l, b_l, c_l, [1.0] * len(ops2), [-1e-3 * (i + 1) for i in range(len(ops2))])
+    for i in range(len(sizes)):
+        out = []
+        for op in (ops1[i], ops2[i]):
+            p = ml.DeviceArray(ctx, rhs[i].shape, host=np.zeros_like(rhs[i]))
+            r = ml.DeviceArray(ctx, rhs[i].shape, host=rhs[i])
+            st = op.solve(p, r, opts=opts)
+            out.append((p.get(), _stats(st)))
+        assert out[0][1] == out[1][1], i
+        assert HC.bitwise_equal(out[0][0], out[1][0]), i
+    for op in ops1 + ops2:
+        op.close()
+
+
+@pytest.mark.parametrize("sizes", [[(64, 64), (128, 32), (48, 40)], [(16, 16, 16)]], ids=["2d", "3d"])
+def test_hc_set_coefs_many_equals_per_operator(ml, gpu_ctx, wl, sizes):
+    ctx = gpu_ctx
+    opts = ml.default_opts(max_iter=30, hang=-1.0)
+    cases = [wl.mushy(n=n, seed=30 + i, corr=0.2) for i, n in enumerate(sizes)]
+    ops1 = [HC.gpu_setup(ml, ctx, c) for c in cases]
+    ops2 = [HC.gpu_setup(ml, ctx, c) for c in cases]
+    bs = []
+    for c, op in zip(cases, ops1):
+        chi = ml.DeviceArray(ctx, (1,) + c["phi"].shape[1:])
+        phi = ml.DeviceArray(ctx, c["phi"].shape, host=c["phi"])
+        op.porosity(phi, chi)
+        b = [ml.DeviceArray(ctx, ml.face_shape(c["n"], d, 2)) for d in range(len(c["n"]))]
+        op.assemble_bcoef(chi, b)
+        bs.append(b)
+    for i, op in enumerate(ops1):
+        op.set_coefs(None, bs[i], 1.0, -cases[i]["dt"])
+    ml.hc_set_coefs_many(ctx, ops2, None, bs, 1.0, [-c["dt"] for c in cases])
+    for i, c in enumerate(cases):
+        out = []
+        for op in (ops1[i], ops2[i]):
+            phi = ml.DeviceArray(ctx, c["phi"].shape, host=c["phi"])
+            old = ml.DeviceArray(ctx, c["phi"].shape, host=c["phi"])
+            st = op.solve_be(phi, old, c["dt"], opts=opts)
+            out.append((phi.get(), _stats(st)))
+        assert out[0][1] == out[1][1], i
+        assert HC.bitwise_equal(out[0][0], out[1][0]), i
+    for op in ops1 + ops2:
+        op.close()
+
+
+def test_set_coefs_many_refuses_bad_input(ml, gpu_ctx):
+    ctx = gpu_ctx
+    w = wlm.darcy_brinkman(n=(32, 32), seed=1)
+    op = ml.VCOp(ctx, (32, 32), w["dx"], w["periodic"], ml.make_bc(w["bc_u"]))
+    b = [ml.DeviceArray(ctx, x.shape, host=x) for x in w["b_u"]]
+    with pytest.raises(RuntimeError, match="appears twice"):
+        ml.vc_set_coefs_many(ctx, [op, op], None, [b, b], None, 1.0, -1e-3)
+    w3 = wlm.darcy_brinkman(n=(8, 8, 8), seed=1)
+    op3 = ml.VCOp(ctx, (8, 8, 8), w3["dx"], w3["periodic"], ml.make_bc(w3["bc_u"]))
+    b3 = [ml.DeviceArray(ctx, x.shape, host=x) for x in w3["b_u"]]
+    with pytest.raises(RuntimeError, match="mixed dimension"):
+        ml.vc_set_coefs_many(ctx, [op, op3], None, [b, b3], None, 1.0, -1e-3)
+    op.close()
+    op3.close()
