@@ -284,8 +284,9 @@ class BatchDarcyBrinkmanStepper:
     the solves of all instances share launches -- the (H, Theta) solves in
     one HCBatch step with every instance's advective source, the U* solves
     (both components of every instance) in one VCBatch, the projections in
-    another.  The per-instance closure, source and correction kernels run
-    instance by instance.  `cases`: `workloads.batch_instances` cases (each
+    another -- and so do the coefficient hierarchies of each operator family
+    (hc_set_coefs_many, vc_set_coefs_many).  The per-instance closure, source
+    and correction kernels run instance by instance.  `cases`: `workloads.batch_instances` cases (each
     carrying Ra_C and Da); `flow_of(case)`: the flow dict of a case."""
 
     def __init__(self, ml, ctx, cases, flow_of, hc_opts):
