@@ -521,6 +521,20 @@ int ml_hc_advect_src(ml_hc_op op, const ml_advect_params* ap, double dt, const d
  * The advection velocity a caller without the reference's MAC-projected
  * predictor passes to ml_hc_advect_src. */
 int ml_vc_face_velocity(ml_vc_op op, const double* const u[], double* const u_face[]);
+/* The advective source of a Darcy-Brinkman step for n independent
+ * operators of one context and dimension, equal for every i to
+ *   ml_hc_diffusive_src(ops[i], phi[i], dsrc[i]);
+ *   ml_vc_face_velocity(<ops[i]'s grid>, u_i, u_face_i);
+ *   src[i] = 0;
+ *   ml_hc_advect_src(ops[i], ap[i], dt, W[i], u_face_i, dsrc[i], src[i], NULL);
+ * (srcSubcycle/AMRLevelMushyLayer.cpp:1645-1672,2074-2104) but in one launch
+ * per stage for all of them; u and u_face hold n*dim arrays (operator i's
+ * direction d at i*dim + d).  No counterpart in the reference, which
+ * advances one mushy layer per process; the batched Darcy-Brinkman steps
+ * use it. */
+int ml_hc_db_advect_src_many(ml_ctx ctx, int n, const ml_hc_op ops[], const ml_advect_params* const ap[], double dt,
+                             double* const phi[], const double* const W[], const double* const u[],
+                             double* const dsrc[], double* const u_face[], double* const src[]);
 
 /* ---- multi-box levels ----------------------------------------------------
  * A level of the reference is a DisjointBoxLayout (the default

@@ -188,6 +188,9 @@ SIGNATURES = {
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "ml_hc_set_coefs_many": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "ml_hc_db_advect_src_many": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.c_double,
+                                           C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
+                                           C.POINTER(_vp), C.POINTER(_vp)]),
     "ml_vc_batch_destroy": (C.c_int, [_vp]),
     "ml_vc_batch_solve": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "ml_vc_batch_solve_be": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.c_double,
@@ -416,6 +419,21 @@ def hc_set_coefs_many(ctx, ops, a, b, alpha, beta):
     ctx.check(ctx.lib.ml_hc_set_coefs_many(ctx.h, n, hs, _ptrs(a) if a is not None else None,
                                            _ptrs([x for bs in b for x in bs]), _doubles(alpha), _doubles(beta)),
               "ml_hc_set_coefs_many")
+
+
+def hc_db_advect_src_many(ctx, ops, aps, dt, phi, W, u, dsrc, u_face, src):
+    """The advective source of a Darcy-Brinkman step for many operators
+    (ml_hc_db_advect_src_many): per operator i the diffusive source of phi[i]
+    into dsrc[i], face velocities of u[i] (list of per-direction arrays) into
+    u_face[i], src[i] = 0 and the advective source of W[i] (DeviceArray or
+    device address).  aps: AdvectParams per operator."""
+    n = len(ops)
+    hs = (_vp * max(1, n))(*[op.h for op in ops])
+    ap = (_vp * max(1, n))(*[C.cast(C.pointer(a), _vp) for a in aps])
+    w = (_vp * max(1, n))(*[x.ptr if hasattr(x, "ptr") else x for x in W])
+    ctx.check(ctx.lib.ml_hc_db_advect_src_many(ctx.h, n, hs, ap, dt, _ptrs(phi), w, _ptrs([x for us in u for x in us]),
+                                               _ptrs(dsrc), _ptrs([x for fs in u_face for x in fs]), _ptrs(src)),
+              "ml_hc_db_advect_src_many")
 
 
 def cell_shape(n, ncomp):

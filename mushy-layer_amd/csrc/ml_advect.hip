@@ -62,6 +62,18 @@ struct AdvArgs {
   double* flux[3];     // 2 comps, face layout
 };
 
+// grid-stride indices: of a plain launch, or of workgroup b of the nb that
+// serve one job of a batched launch
+struct GridIx {
+  __device__ long long start() const { return blockIdx.x * (long long)blockDim.x + threadIdx.x; }
+  __device__ long long stride() const { return (long long)gridDim.x * blockDim.x; }
+};
+struct JobIx {
+  long long b, nb;
+  __device__ long long start() const { return b * blockDim.x + threadIdx.x; }
+  __device__ long long stride() const { return nb * blockDim.x; }
+};
+
 __device__ inline int wrapc(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 __device__ inline long long cidx(const Geom& g, int i, int j, int k) {
@@ -114,11 +126,10 @@ __device__ inline double slope(const AdvArgs& A, int d, int c, int i, int j, int
   return copysign(lim, dWc);
 }
 
-template <int DIM>
-__global__ void adv_predict_kernel(AdvArgs A) {
+template <int DIM, class IX>
+__device__ __forceinline__ void adv_predict_body(const AdvArgs& A, IX ix) {
   const Geom& g = A.g;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < A.nv;
-       t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = ix.start(); t < A.nv; t += ix.stride()) {
     const int i = int(t % g.n[0]);
     const long long r = t / g.n[0];
     const int j = int(r % g.n[1]);
@@ -155,6 +166,11 @@ __global__ void adv_predict_kernel(AdvArgs A) {
       }
     }
   }
+}
+
+template <int DIM>
+__global__ void adv_predict_kernel(AdvArgs A) {
+  adv_predict_body<DIM>(A, GridIx{});
 }
 
 // quasilinear update (scale*u_cell[e])*(WHalf[e][hi] - WHalf[e][lo]) at a cell
@@ -232,12 +248,11 @@ struct FaceGroups {
   long long first[7];
 };
 
-template <int DIM, int STAGE>
-__global__ void adv_face_kernel(AdvArgs A, FaceGroups G) {
+template <int DIM, int STAGE, class IX>
+__device__ __forceinline__ void adv_face_body(const AdvArgs& A, const FaceGroups& G, IX ix) {
   const Geom& g = A.g;
   const long long total = G.first[G.n];
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
-       t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = ix.start(); t < total; t += ix.stride()) {
     int q = 0;
     while (q + 1 < G.n && t >= G.first[q + 1]) ++q;
     const int d = G.dir[q], d2 = G.dir2[q];
@@ -265,11 +280,15 @@ __global__ void adv_face_kernel(AdvArgs A, FaceGroups G) {
   }
 }
 
-template <int DIM>
-__global__ void adv_div_kernel(AdvArgs A, double* __restrict__ src) {
+template <int DIM, int STAGE>
+__global__ void adv_face_kernel(AdvArgs A, FaceGroups G) {
+  adv_face_body<DIM, STAGE>(A, G, GridIx{});
+}
+
+template <int DIM, class IX>
+__device__ __forceinline__ void adv_div_body(const AdvArgs& A, double* __restrict__ src, IX ix) {
   const Geom& g = A.g;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < A.nv;
-       t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = ix.start(); t < A.nv; t += ix.stride()) {
     const int i = int(t % g.n[0]);
     const long long r = t / g.n[0];
     const int j = int(r % g.n[1]);
@@ -288,12 +307,17 @@ __global__ void adv_div_kernel(AdvArgs A, double* __restrict__ src) {
   }
 }
 
+template <int DIM>
+__global__ void adv_div_kernel(AdvArgs A, double* __restrict__ src) {
+  adv_div_body<DIM>(A, src, GridIx{});
+}
+
 // CellToEdge of a no-slip cell-centred velocity: interior (and periodic)
 // faces 0.5*(u_lo + u_hi), wall faces 0 (the ghost 2*0 - u1 averages to 0)
-template <int DIM>
-__global__ void face_velocity_kernel(Geom g, int d, const double* __restrict__ u, double* __restrict__ uf) {
-  for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < g.nface[d];
-       f += (long long)gridDim.x * blockDim.x) {
+template <int DIM, class IX>
+__device__ __forceinline__ void face_velocity_body(const Geom& g, int d, const double* __restrict__ u,
+                                                   double* __restrict__ uf, IX ix) {
+  for (long long f = ix.start(); f < g.nface[d]; f += ix.stride()) {
     int c[3];
     c[0] = int(f % g.fn[d][0]);
     const long long r = f / g.fn[d][0];
@@ -312,6 +336,41 @@ __global__ void face_velocity_kernel(Geom g, int d, const double* __restrict__ u
       uf[f] = 0.5 * (u[g.ci(lo[0], lo[1], lo[2])] + u[g.ci(c[0], c[1], c[2])]);
     }
   }
+}
+
+template <int DIM>
+__global__ void face_velocity_kernel(Geom g, int d, const double* __restrict__ u, double* __restrict__ uf) {
+  face_velocity_body<DIM>(g, d, u, uf, GridIx{});
+}
+
+// ---- batched form: one launch per stage for many operators --------------
+// job of one operator: its arguments, face groups, the cells' and faces'
+// workgroup counts (nb[0] cells, nb[1] faces of every direction, nb[2] the
+// 3D corrected faces) and its face velocities
+struct AdvJob {
+  AdvArgs A;
+  FaceGroups G1, G2;
+  double* src;
+  long long nb[3];
+  const double* u[3];
+  long long nbf[3];   // face-velocity workgroups per direction
+};
+
+// STAGE 0 cells (predictor), 1 faces (WHalf1), 2 faces (3D WHalf2), 3 faces
+// (flux), 4 cells (divergence), 5 + d face velocities of direction d
+template <int DIM, int STAGE>
+__global__ __launch_bounds__(256) void adv_batch_kernel(const AdvJob* __restrict__ jobs,
+                                                        const unsigned* __restrict__ map) {
+  const unsigned w = map[blockIdx.x];
+  const AdvJob& J = jobs[w >> kBatchTileBits];
+  const long long b = w & kBatchTileMask;
+  if (STAGE == 0) adv_predict_body<DIM>(J.A, JobIx{b, J.nb[0]});
+  if (STAGE == 1) adv_face_body<DIM, 1>(J.A, J.G1, JobIx{b, J.nb[1]});
+  if (STAGE == 2) adv_face_body<DIM, 2>(J.A, J.G2, JobIx{b, J.nb[2]});
+  if (STAGE == 3) adv_face_body<DIM, 3>(J.A, J.G1, JobIx{b, J.nb[1]});
+  if (STAGE == 4) adv_div_body<DIM>(J.A, J.src, JobIx{b, J.nb[0]});
+  if (STAGE >= 5) face_velocity_body<DIM>(J.A.g, STAGE - 5, J.u[STAGE - 5], const_cast<double*>(J.A.uf[STAGE - 5]),
+                                          JobIx{b, J.nbf[STAGE - 5]});
 }
 
 inline int grid_for(long long n) {
@@ -374,47 +433,35 @@ bool fields_ok(int dim, const void* const* f) {
   return true;
 }
 
-}  // namespace
-}  // namespace ml
-
-extern "C" {
-
-int ml_hc_diffusive_src(ml_hc_op op, double* phi, double* out) {
-  if (!op || !phi || !out) return ML_EINVAL;
-  ml::HCArgs<double> A = op->args(0);
-  A.alpha = 0.0;
-  A.beta = 1.0;
-  ML_HIP(op->ctx, ml::hc_apply_op<double>(A, phi, out, nullptr, 0, op->ctx->stream));
-  return ML_OK;
-}
-
-int ml_hc_advect_src(ml_hc_op op, const ml_advect_params* ap, double dt, const double* W,
-                     const double* const u_face[], const double* S, double* src, double* const flux[]) {
-  if (!op || !ap || !W || !src || !ml::fields_ok(op->dim, reinterpret_cast<const void* const*>(u_face)) ||
+// checks and arguments of one operator's advective source (the stages'
+// workspace kept by the operator)
+int adv_prepare(ml_hc_op op, const ml_advect_params* ap, double dt, const double* W, const double* const u_face[],
+                const double* S, double* src, double* const flux[], AdvArgs* out) {
+  if (!op || !ap || !W || !src || !fields_ok(op->dim, reinterpret_cast<const void* const*>(u_face)) ||
       !(dt > 0))
     return ML_EINVAL;
   const ml::Geom& g = op->lev[0].g;
   for (int d = 0; d < g.dim; ++d) {
     if (g.periodic[d]) continue;
-    if (g.n[d] < 3) return ml::fail(op->ctx, ML_ENOTSUP, "ml_hc_advect_src: a walled direction needs at least 3 cells");
+    if (g.n[d] < 3) return fail(op->ctx, ML_ENOTSUP, "ml_hc_advect_src: a walled direction needs at least 3 cells");
     for (int s = 0; s < 2; ++s)
       for (int c = 0; c < 2; ++c) {
         const int t = ap->type[d][s][c];
         if (t != ML_ADV_DIRICHLET && t != ML_ADV_EXTRAP && t != ML_ADV_INFLOW_OUTFLOW && t != ML_ADV_NEUMANN)
-          return ml::fail(op->ctx, ML_ENOTSUP, "ml_hc_advect_src: unsupported advection boundary condition");
+          return fail(op->ctx, ML_ENOTSUP, "ml_hc_advect_src: unsupported advection boundary condition");
       }
   }
   const long long nv = g.nvalid();
   long long nface = 0;
   for (int d = 0; d < g.dim; ++d) nface += g.nface[d];
-  const bool own_flux = !ml::fields_ok(g.dim, reinterpret_cast<const void* const*>(flux));
+  const bool own_flux = !fields_ok(g.dim, reinterpret_cast<const void* const*>(flux));
   // vc (dim*nv) + pred (dim*2*2*nv) + h1 (2*sum nface) + h2 (3D: 2*2*sum
   // nface) + flux (2*sum nface, when the caller passes none)
   const long long ndbl = g.dim * nv + g.dim * 4 * nv + 2 * nface + (g.dim == 3 ? 4 * nface : 0) +
                          (own_flux ? 2 * nface : 0);
   double* ws = nullptr;
-  if (int rc = ml::adv_workspace(op, sizeof(double) * size_t(ndbl), &ws)) return rc;
-  ml::AdvArgs A{};
+  if (int rc = adv_workspace(op, sizeof(double) * size_t(ndbl), &ws)) return rc;
+  AdvArgs A{};
   A.g = g;
   A.nv = nv;
   A.dt = dt;
@@ -462,9 +509,153 @@ int ml_hc_advect_src(ml_hc_op op, const ml_advect_params* ap, double dt, const d
       A.flux[d] = flux[d];
     }
   }
-  const hipError_t e = g.dim == 2 ? ml::launch_advect<2>(A, src, op->ctx->stream)
-                                  : ml::launch_advect<3>(A, src, op->ctx->stream);
+  *out = A;
+  return ML_OK;
+}
+
+
+}  // namespace
+}  // namespace ml
+
+extern "C" {
+
+int ml_hc_diffusive_src(ml_hc_op op, double* phi, double* out) {
+  if (!op || !phi || !out) return ML_EINVAL;
+  ml::HCArgs<double> A = op->args(0);
+  A.alpha = 0.0;
+  A.beta = 1.0;
+  ML_HIP(op->ctx, ml::hc_apply_op<double>(A, phi, out, nullptr, 0, op->ctx->stream));
+  return ML_OK;
+}
+
+int ml_hc_advect_src(ml_hc_op op, const ml_advect_params* ap, double dt, const double* W,
+                     const double* const u_face[], const double* S, double* src, double* const flux[]) {
+  ml::AdvArgs A{};
+  if (int rc = ml::adv_prepare(op, ap, dt, W, u_face, S, src, flux, &A)) return rc;
+  const hipError_t e = A.g.dim == 2 ? ml::launch_advect<2>(A, src, op->ctx->stream)
+                                    : ml::launch_advect<3>(A, src, op->ctx->stream);
   if (e != hipSuccess) return ml::fail(op->ctx, ML_EHIP, std::string("ml_hc_advect_src: ") + hipGetErrorString(e));
+  return ML_OK;
+}
+
+int ml_hc_db_advect_src_many(ml_ctx ctx, int n, const ml_hc_op ops[], const ml_advect_params* const ap[], double dt,
+                             double* const phi[], const double* const W[], const double* const u[],
+                             double* const dsrc[], double* const u_face[], double* const src[]) {
+  if (!ctx || n < 0 || (n > 0 && (!ops || !ap || !phi || !W || !u || !dsrc || !u_face || !src))) return ML_EINVAL;
+  if (n == 0) return ML_OK;
+  if (n > (1 << (32 - ml::kBatchTileBits)))
+    return ml::fail(ctx, ML_ENOTSUP, "ml_hc_db_advect_src_many: more than 4096 operators");
+  const int dim = ops[0] ? ops[0]->dim : 0;
+  std::vector<ml::AdvJob> jobs(n);
+  std::vector<ml::HCApplyJob<double>> aj(n);
+  std::vector<ml::WordsJob> zj(n);
+  std::vector<unsigned> amap, zmap, mcell, mface1, mface2, mfv[3];
+  auto blocks = [](long long items) { return (items + 255) / 256; };
+  for (int i = 0; i < n; ++i) {
+    ml_hc_op op = ops[i];
+    if (!op || op->ctx != ctx || op->dim != dim)
+      return ml::fail(ctx, ML_EINVAL, "ml_hc_db_advect_src_many: operators of one context and dimension");
+    for (int j = 0; j < i; ++j)
+      if (ops[j] == op) return ml::fail(ctx, ML_EINVAL, "ml_hc_db_advect_src_many: an operator appears twice");
+    const double* ufi[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < dim; ++d) {
+      if (!u[i * dim + d] || !u_face[i * dim + d]) return ML_EINVAL;
+      ufi[d] = u_face[i * dim + d];
+    }
+    if (!phi[i] || !dsrc[i]) return ML_EINVAL;
+    ml::AdvJob& J = jobs[i];
+    if (int rc = ml::adv_prepare(op, ap[i], dt, W[i], ufi, dsrc[i], src[i], nullptr, &J.A)) return rc;
+    const ml::Geom& g = J.A.g;
+    J.src = src[i];
+    J.G1 = ml::FaceGroups{};
+    J.G1.n = dim;
+    for (int d = 0; d < dim; ++d) {
+      J.G1.dir[d] = d;
+      J.G1.dir2[d] = -1;
+      J.G1.first[d + 1] = J.G1.first[d] + g.nface[d];
+    }
+    J.G2 = ml::FaceGroups{};
+    if (dim == 3)
+      for (int d = 0; d < 3; ++d)
+        for (int e = 0; e < 3; ++e) {
+          if (e == d) continue;
+          J.G2.dir[J.G2.n] = d;
+          J.G2.dir2[J.G2.n] = e;
+          J.G2.first[J.G2.n + 1] = J.G2.first[J.G2.n] + g.nface[d];
+          ++J.G2.n;
+        }
+    J.nb[0] = blocks(J.A.nv);
+    J.nb[1] = blocks(J.G1.first[dim]);
+    J.nb[2] = dim == 3 ? blocks(J.G2.first[J.G2.n]) : 0;
+    for (int d = 0; d < 3; ++d) {
+      J.u[d] = d < dim ? u[i * dim + d] : nullptr;
+      J.nbf[d] = d < dim ? blocks(g.nface[d]) : 0;
+    }
+    const unsigned hi = unsigned(i) << ml::kBatchTileBits;
+    for (long long b = 0; b < J.nb[0]; ++b) mcell.push_back(hi | unsigned(b));
+    for (long long b = 0; b < J.nb[1]; ++b) mface1.push_back(hi | unsigned(b));
+    for (long long b = 0; b < J.nb[2]; ++b) mface2.push_back(hi | unsigned(b));
+    for (int d = 0; d < dim; ++d)
+      for (long long b = 0; b < J.nbf[d]; ++b) mfv[d].push_back(hi | unsigned(b));
+    // the diffusive source (ml_hc_diffusive_src): the operator with alpha 0, beta 1
+    ml::HCApplyJob<double>& H = aj[i];
+    H.A = op->args(0);
+    H.A.alpha = 0.0;
+    H.A.beta = 1.0;
+    H.io = ml::TileIO<double>{};
+    H.io.phi = phi[i];
+    H.io.out = dsrc[i];
+    H.io.homog = 0;
+    int ntx = 0, nty = 0;
+    const int nt = ml::hc_apply_tiles(g, &ntx, &nty);
+    H.io.ntx = ntx;
+    H.io.nty = nty;
+    for (int t = 0; t < nt; ++t) amap.push_back(hi | unsigned(t));
+    zj[i] = ml::WordsJob{nullptr, reinterpret_cast<unsigned long long*>(src[i]), 2 * g.ncell};
+    for (long long c = 0; c * ml::kWordsChunk < 2 * g.ncell; ++c) zmap.push_back(hi | unsigned(c));
+  }
+  // one staged table: jobs, then the maps of every launch
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  std::vector<std::pair<const void*, size_t>> parts = {
+      {jobs.data(), jobs.size() * sizeof(ml::AdvJob)}, {aj.data(), aj.size() * sizeof(ml::HCApplyJob<double>)},
+      {zj.data(), zj.size() * sizeof(ml::WordsJob)},   {amap.data(), amap.size() * sizeof(unsigned)},
+      {zmap.data(), zmap.size() * sizeof(unsigned)},   {mcell.data(), mcell.size() * sizeof(unsigned)},
+      {mface1.data(), mface1.size() * sizeof(unsigned)}, {mface2.data(), mface2.size() * sizeof(unsigned)},
+      {mfv[0].data(), mfv[0].size() * sizeof(unsigned)}, {mfv[1].data(), mfv[1].size() * sizeof(unsigned)},
+      {mfv[2].data(), mfv[2].size() * sizeof(unsigned)}};
+  std::vector<size_t> off(parts.size() + 1, 0);
+  for (size_t k = 0; k < parts.size(); ++k) off[k + 1] = al(off[k] + parts[k].second);
+  std::vector<unsigned char> blob(off.back());
+  for (size_t k = 0; k < parts.size(); ++k)
+    if (parts[k].second) std::memcpy(blob.data() + off[k], parts[k].first, parts[k].second);
+  void* dv = nullptr;
+  ML_HIP(ctx, ctx->tables.put(blob.data(), blob.size(), ctx->stream, &dv));
+  unsigned char* d = static_cast<unsigned char*>(dv);
+  const ml::AdvJob* dj = reinterpret_cast<const ml::AdvJob*>(d + off[0]);
+  auto map_at = [&](int k) { return reinterpret_cast<const unsigned*>(d + off[k]); };
+  hipStream_t s = ctx->stream;
+  ML_HIP(ctx, ml::hc_apply_batch<double>(dim, reinterpret_cast<const ml::HCApplyJob<double>*>(d + off[1]), map_at(3),
+                                         int(amap.size()), s));
+  ML_HIP(ctx, ml::words_batch(reinterpret_cast<const ml::WordsJob*>(d + off[2]), map_at(4), int(zmap.size()), s));
+  auto stage = [&](auto kern, int k, size_t nblk) -> int {
+    if (nblk == 0) return ML_OK;
+    hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(256), 0, s, dj, map_at(k));
+    ML_HIP(ctx, hipGetLastError());
+    return ML_OK;
+  };
+  int rc;
+  if (dim == 2) {
+    if ((rc = stage(ml::adv_batch_kernel<2, 5>, 8, mfv[0].size())) || (rc = stage(ml::adv_batch_kernel<2, 6>, 9, mfv[1].size())) ||
+        (rc = stage(ml::adv_batch_kernel<2, 0>, 5, mcell.size())) || (rc = stage(ml::adv_batch_kernel<2, 1>, 6, mface1.size())) ||
+        (rc = stage(ml::adv_batch_kernel<2, 3>, 6, mface1.size())) || (rc = stage(ml::adv_batch_kernel<2, 4>, 5, mcell.size())))
+      return rc;
+  } else {
+    if ((rc = stage(ml::adv_batch_kernel<3, 5>, 8, mfv[0].size())) || (rc = stage(ml::adv_batch_kernel<3, 6>, 9, mfv[1].size())) ||
+        (rc = stage(ml::adv_batch_kernel<3, 7>, 10, mfv[2].size())) || (rc = stage(ml::adv_batch_kernel<3, 0>, 5, mcell.size())) ||
+        (rc = stage(ml::adv_batch_kernel<3, 1>, 6, mface1.size())) || (rc = stage(ml::adv_batch_kernel<3, 2>, 7, mface2.size())) ||
+        (rc = stage(ml::adv_batch_kernel<3, 3>, 6, mface1.size())) || (rc = stage(ml::adv_batch_kernel<3, 4>, 5, mcell.size())))
+      return rc;
+  }
   return ML_OK;
 }
 

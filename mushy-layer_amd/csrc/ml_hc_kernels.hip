@@ -213,12 +213,12 @@ __global__ __launch_bounds__(NT) void hc_relax_tile_kernel(HCArgs<T> A, TileIO<T
 // Residual (with max-norm), operator application, and FAS restriction of
 // one tile with a one-cell halo.
 template <typename T, int DIM, int TX, int TY, int TZ, int MODE>
-__global__ __launch_bounds__(kThreads) void hc_tile_kernel(HCArgs<T> A, TileIO<T> io) {
+__device__ __forceinline__ void hc_tile_body(const HCArgs<T>& A, const TileIO<T>& io, int tile) {
   if (A.skip && *A.skip) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const Geom& g = A.g;
   Tile<DIM, TX, TY, TZ> t;
-  t.init(1, io.ntx, io.nty);
+  t.init_at(1, io.ntx, io.nty, tile);
   const int E = t.E;
   T* sphi = reinterpret_cast<T*>(smem_raw);
   T* sphl = sphi + 2 * E;
@@ -325,6 +325,20 @@ __global__ __launch_bounds__(kThreads) void hc_tile_kernel(HCArgs<T> A, TileIO<T
       }
     }
   }
+}
+
+template <typename T, int DIM, int TX, int TY, int TZ, int MODE>
+__global__ __launch_bounds__(kThreads) void hc_tile_kernel(HCArgs<T> A, TileIO<T> io) {
+  hc_tile_body<T, DIM, TX, TY, TZ, MODE>(A, io, int(blockIdx.x));
+}
+
+// batched operator applications: workgroup w applies tile map[w] & mask of job map[w] >> bits
+template <typename T, int DIM, int TX, int TY, int TZ>
+__global__ __launch_bounds__(kThreads) void hc_apply_batch_kernel(const HCApplyJob<T>* __restrict__ jobs,
+                                                                  const unsigned* __restrict__ map) {
+  const unsigned w = map[blockIdx.x];
+  const HCApplyJob<T>& J = jobs[w >> kBatchTileBits];
+  hc_tile_body<T, DIM, TX, TY, TZ, kApply>(J.A, J.io, int(w & kBatchTileMask));
 }
 
 // ---------------------------------------------------------------------------
@@ -1281,6 +1295,27 @@ hipError_t hc_apply_op(const HCArgs<T>& A, T* phi, T* lhs, const T* add, int hom
   return A.g.dim == 2 ? launch_tile<T, 2, kApply>(A, io, s) : launch_tile<T, 3, kApply>(A, io, s);
 }
 
+int hc_apply_tiles(const Geom& g, int* ntx, int* nty) {
+  return g.dim == 2 ? tiles_of<double, 2>(g, *ntx, *nty) : tiles_of<double, 3>(g, *ntx, *nty);
+}
+
+template <typename T>
+hipError_t hc_apply_batch(int dim, const HCApplyJob<T>* jobs, const unsigned* map, int nblocks, hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  if (dim == 2) {
+    using D = TileDims<T, 2>;
+    const size_t lds = tile_lds<T, 2>(1, false);
+    hipLaunchKernelGGL((hc_apply_batch_kernel<T, 2, D::TX, D::TY, D::TZ>), dim3(nblocks), dim3(kThreads), lds, s,
+                       jobs, map);
+  } else {
+    using D = TileDims<T, 3>;
+    const size_t lds = tile_lds<T, 3>(1, false);
+    hipLaunchKernelGGL((hc_apply_batch_kernel<T, 3, D::TX, D::TY, D::TZ>), dim3(nblocks), dim3(kThreads), lds, s,
+                       jobs, map);
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t hc_restrict_residual(const HCArgs<T>& A, const Geom& gc, T* phi, const T* rhs, T* resc,
                                 int homog, hipStream_t s) {
@@ -1321,6 +1356,7 @@ template hipError_t hc_coarse_vcycle<double>(const CoarseDesc<double>*, int, int
 template hipError_t hc_residual<double>(const HCArgs<double>&, double*, const double*, double*,
                                         unsigned long long*, int, hipStream_t, const CtlFuse&);
 template hipError_t hc_apply_op<double>(const HCArgs<double>&, double*, double*, const double*, int, hipStream_t);
+template hipError_t hc_apply_batch<double>(int, const HCApplyJob<double>*, const unsigned*, int, hipStream_t);
 template hipError_t hc_restrict_residual<double>(const HCArgs<double>&, const Geom&, double*, const double*,
                                                  double*, int, hipStream_t);
 template hipError_t hc_restrict_all<double>(const HCArgs<double>&, const Geom&, double*, const double*, double*,

@@ -164,6 +164,13 @@ struct TileIO {
   int npass, first, homog, ntx, nty;
 };
 
+// one batched operator application (hc_apply_op) of one operator
+template <typename T>
+struct HCApplyJob {
+  HCArgs<T> A;
+  TileIO<T> io;
+};
+
 // One MG level as seen by the single-workgroup kernels (device memory).
 template <typename T>
 struct LevelDesc {
@@ -522,6 +529,10 @@ struct CoarsenJobs {
   short kind[kMaxCoarsenJobs];     // 0..2 face direction, 3 cells
   T* dst[kMaxCoarsenJobs];
 };
+// batched hc_apply_op over (job, tile) maps of the residual tile (hc_apply_tiles)
+int hc_apply_tiles(const Geom& g, int* ntx, int* nty);
+template <typename T>
+hipError_t hc_apply_batch(int dim, const HCApplyJob<T>* jobs, const unsigned* map, int nblocks, hipStream_t s);
 template <typename T>
 hipError_t coarsen_hierarchy(const Geom& gf, int ncomp, int nlev, const T* const* bf, T* const* const* bc, const T* af,
                              T* const* ac, hipStream_t s, bool faces = true);

@@ -30,7 +30,7 @@ device.
 import numpy as np
 
 from . import (DeviceArray, HCOp, VCOp, default_opts, face_shape, make_advect_params, make_bc, make_flow_params,
-               make_params, valid, hc_set_coefs_many, vc_set_coefs_many)
+               make_params, valid, hc_db_advect_src_many, hc_set_coefs_many, vc_set_coefs_many)
 
 # multigrid options of the velocity solves: U* (VelocityMultigrid,
 # srcSubcycle/MushyLayerSubcycleUtils.cpp:654-659: smoothing 2 down, up and
@@ -319,7 +319,13 @@ class BatchDarcyBrinkmanStepper:
             s.op.enthalpy_vars(s.phi, s.vold)
             s.op.assemble_bcoef(s.vold.comp_ptr(3), s.b)
         hc_set_coefs_many(self.ctx, [s.op for s in sts], None, [s.b for s in sts], 1.0, -dt)
-        srcs = [s.hc_src() for s in sts]
+        if all(s.advect for s in sts):
+            hc_db_advect_src_many(self.ctx, [s.op for s in sts], [s.ap for s in sts], dt, [s.phi for s in sts],
+                                  [s.vold for s in sts], [s.u for s in sts], [s.dsrc for s in sts],
+                                  [s.uf for s in sts], [s.hsrc for s in sts])
+            srcs = [s.hsrc for s in sts]
+        else:
+            srcs = [s.hc_src() for s in sts]
         hst = self.hcb.step([s.phi for s in sts], dt, self.hc_opts, srcs=srcs)
         # enthalpy variables at t^{n+1}, U* coefficients and sources
         for s in sts:

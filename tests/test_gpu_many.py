@@ -1,9 +1,11 @@
-"""Batched coefficient set-up (ml_vc_set_coefs_many, ml_hc_set_coefs_many):
-the level-0 copies and coarsened hierarchies of many operators in a few
-launches must equal one ml_vc_set_coefs / ml_hc_set_coefs per operator.
-Checked through what the hierarchies feed: solves on operators set up both
-ways, on uneven 2D and 3D sizes, with and without a and c terms, must agree
-bit for bit in state and statistics."""
+"""Batched calls for many operators.  Coefficient set-up
+(ml_vc_set_coefs_many, ml_hc_set_coefs_many): the level-0 copies and
+coarsened hierarchies of many operators in a few launches must equal one
+ml_vc_set_coefs / ml_hc_set_coefs per operator, checked through what the
+hierarchies feed: solves on operators set up both ways, on uneven 2D and 3D
+sizes, with and without a and c terms, must agree bit for bit in state and
+statistics.  The advective source of a Darcy-Brinkman step
+(ml_hc_db_advect_src_many) must equal each stepper's own, bit for bit."""
 import importlib
 
 import numpy as np
@@ -111,3 +113,38 @@ def test_set_coefs_many_refuses_bad_input(ml, gpu_ctx):
         ml.vc_set_coefs_many(ctx, [op, op3], None, [b, b3], None, 1.0, -1e-3)
     op.close()
     op3.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_db_advect_src_many_equals_per_operator(ml, gpu_ctx, wl, dim):
+    """ml_hc_db_advect_src_many against each Darcy-Brinkman stepper's own
+    advective source (diffusive source, face velocities, Godunov fluxes),
+    after one step so that the velocities are not zero."""
+    stp = importlib.import_module("mushy-layer_amd.steps")
+    ctx = gpu_ctx
+    sizes = [(64, 64), (96, 48), (32, 32)] if dim == 2 else [(16, 16, 16), (12, 16, 8)]
+    sts = []
+    for i, n in enumerate(sizes):
+        case = wl.mushy(n=n, seed=40 + i, corr=0.25)
+        flow = wl.flow(dim=dim, Pr=10.0, Da=1e-3 * (i + 1), Ra_C=200.0 + 50 * i)
+        s = stp.DarcyBrinkmanStepper(ml, ctx, case, ml.default_opts(max_iter=60, hang=stp.NO_HANG), flow)
+        s.step()
+        s.op.enthalpy_vars(s.phi, s.vold)
+        sts.append(s)
+    ref = []
+    for s in sts:
+        s.hc_src()
+        ref.append((s.dsrc.get(), [f.get() for f in s.uf], s.hsrc.get()))
+        for x in [s.dsrc, s.hsrc] + list(s.uf):
+            x.set(np.full(x.shape, 7.0))   # stale values the batched call must overwrite
+    ml.hc_db_advect_src_many(ctx, [s.op for s in sts], [s.ap for s in sts], sts[0].dt, [s.phi for s in sts],
+                             [s.vold for s in sts], [s.u for s in sts], [s.dsrc for s in sts], [s.uf for s in sts],
+                             [s.hsrc for s in sts])
+    for i, s in enumerate(sts):
+        assert np.array_equal(HC.valid(s.n, s.dsrc.get()), HC.valid(s.n, ref[i][0])), i
+        for d in range(dim):
+            assert HC.bitwise_equal(s.uf[d].get(), ref[i][1][d]), (i, d)
+        assert HC.bitwise_equal(HC.valid(s.n, s.hsrc.get()), HC.valid(s.n, ref[i][2])), i
+        assert np.abs(HC.valid(s.n, ref[i][2])).max() > 0
+    for s in sts:
+        s.close()
