@@ -535,6 +535,37 @@ int ml_vc_face_velocity(ml_vc_op op, const double* const u[], double* const u_fa
 int ml_hc_db_advect_src_many(ml_ctx ctx, int n, const ml_hc_op ops[], const ml_advect_params* const ap[], double dt,
                              double* const phi[], const double* const W[], const double* const u[],
                              double* const dsrc[], double* const u_face[], double* const src[]);
+/* The other per-instance stages of a batched Darcy-Brinkman step, each
+ * equal for every i to the single-operator calls it names, in one launch
+ * per kernel for all operators (u, u_old, src, ustar and bCoef hold n*dim
+ * arrays, operator i's direction d at i*dim + d; no counterpart in the
+ * reference, which advances one mushy layer per process):
+ *   ml_hc_db_coefs_many:   ml_hc_enthalpy_vars(ops[i], phi[i], vars[i]);
+ *                          ml_hc_assemble_bcoef(ops[i], vars[i] + 3*ncell, bCoef_i)
+ *   ml_vc_db_ustar_many:   ml_hc_enthalpy_vars(hops[i], phi[i], vars_new[i]);
+ *                          ml_vc_darcy_ccoef_law(uops[i], vars_old[i] + 3*ncell,
+ *                              darcy_coeff[i], law[i], hele_shaw[i], cCoef[i]);
+ *                          ml_vc_ustar_src(uops[i], fp[i], vars_old[i], vars_new[i], pi[i], src_i);
+ *                          u_old_i = u_i
+ *   ml_vc_db_project_many: ml_vc_cc_project_rhs(uops[i], fp[i], dt, vars_old[i], vars_new[i],
+ *                              pi[i], ustar_i, u_i, rhs[i]);
+ *                          ml_vc_pressure_faces(uops[i], vars_old[i], vars_new[i], bCoef_i);
+ *                          pi[i] = 0
+ *   ml_vc_db_correct_many: ml_vc_cc_correct(uops[i], fp[i], dt, vars_old[i], vars_new[i], pi[i], u_i) */
+int ml_hc_db_coefs_many(ml_ctx ctx, int n, const ml_hc_op ops[], const double* const phi[], double* const vars[],
+                        double* const bCoef[]);
+int ml_vc_db_ustar_many(ml_ctx ctx, int n, const ml_hc_op hops[], const ml_vc_op uops[],
+                        const ml_flow_params* const fp[], const double darcy_coeff[], const int law[],
+                        const double hele_shaw[], const double* const phi[], const double* const vars_old[],
+                        double* const vars_new[], const double* const pi[], double* const cCoef[],
+                        double* const src[], const double* const u[], double* const u_old[]);
+int ml_vc_db_project_many(ml_ctx ctx, int n, const ml_vc_op uops[], const ml_flow_params* const fp[], double dt,
+                          const double* const vars_old[], const double* const vars_new[], double* const pi[],
+                          const double* const ustar[], double* const u[], double* const rhs[],
+                          double* const bCoef[]);
+int ml_vc_db_correct_many(ml_ctx ctx, int n, const ml_vc_op uops[], const ml_flow_params* const fp[], double dt,
+                          const double* const vars_old[], const double* const vars_new[], const double* const pi[],
+                          double* const u[]);
 
 /* ---- multi-box levels ----------------------------------------------------
  * A level of the reference is a DisjointBoxLayout (the default

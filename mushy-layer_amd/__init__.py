@@ -191,6 +191,11 @@ SIGNATURES = {
     "ml_hc_db_advect_src_many": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.c_double,
                                            C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),
                                            C.POINTER(_vp), C.POINTER(_vp)]),
+    "ml_hc_db_coefs_many": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "ml_vc_db_ustar_many": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_double), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "ml_vc_db_project_many": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.c_double, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "ml_vc_db_correct_many": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.c_double, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
     "ml_vc_batch_destroy": (C.c_int, [_vp]),
     "ml_vc_batch_solve": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "ml_vc_batch_solve_be": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp), C.c_double,
@@ -434,6 +439,49 @@ def hc_db_advect_src_many(ctx, ops, aps, dt, phi, W, u, dsrc, u_face, src):
     ctx.check(ctx.lib.ml_hc_db_advect_src_many(ctx.h, n, hs, ap, dt, _ptrs(phi), w, _ptrs([x for us in u for x in us]),
                                                _ptrs(dsrc), _ptrs([x for fs in u_face for x in fs]), _ptrs(src)),
               "ml_hc_db_advect_src_many")
+
+
+def _hs(ops):
+    return (_vp * max(1, len(ops)))(*[op.h for op in ops])
+
+
+def _structs(xs):
+    return (_vp * max(1, len(xs)))(*[C.cast(C.pointer(x), _vp) for x in xs])
+
+
+def _flat(xss):
+    return _ptrs([x for xs in xss for x in xs])
+
+
+def hc_db_coefs_many(ctx, ops, phi, vars_, b):
+    """Enthalpy variables of phi[i] into vars_[i] and the HC face coefficients
+    b[i] of their porosity, for many operators (ml_hc_db_coefs_many)."""
+    ctx.check(ctx.lib.ml_hc_db_coefs_many(ctx.h, len(ops), _hs(ops), _ptrs(phi), _ptrs(vars_), _flat(b)),
+              "ml_hc_db_coefs_many")
+
+
+def vc_db_ustar_many(ctx, hops, uops, fps, darcy, laws, hele_shaw, phi, vold, vnew, pi, c, src, u, uold):
+    """Enthalpy variables at t^{n+1}, Darcy coefficients, U* sources and the
+    saved velocities of many Darcy-Brinkman instances (ml_vc_db_ustar_many)."""
+    n = len(uops)
+    law = (C.c_int * max(1, n))(*[PERM_LAWS.get(x, x) for x in laws])
+    ctx.check(ctx.lib.ml_vc_db_ustar_many(ctx.h, n, _hs(hops), _hs(uops), _structs(fps), _doubles(darcy), law,
+                                          _doubles(hele_shaw), _ptrs(phi), _ptrs(vold), _ptrs(vnew), _ptrs(pi),
+                                          _ptrs(c), _flat(src), _flat(u), _flat(uold)), "ml_vc_db_ustar_many")
+
+
+def vc_db_project_many(ctx, uops, fps, dt, vold, vnew, pi, ustar, u, rhs, bp):
+    """Projection right-hand sides, pressure faces and pi = 0 for many
+    instances (ml_vc_db_project_many)."""
+    ctx.check(ctx.lib.ml_vc_db_project_many(ctx.h, len(uops), _hs(uops), _structs(fps), dt, _ptrs(vold), _ptrs(vnew),
+                                            _ptrs(pi), _flat(ustar), _flat(u), _ptrs(rhs), _flat(bp)),
+              "ml_vc_db_project_many")
+
+
+def vc_db_correct_many(ctx, uops, fps, dt, vold, vnew, pi, u):
+    """Projection corrections of many instances (ml_vc_db_correct_many)."""
+    ctx.check(ctx.lib.ml_vc_db_correct_many(ctx.h, len(uops), _hs(uops), _structs(fps), dt, _ptrs(vold), _ptrs(vnew),
+                                            _ptrs(pi), _flat(u)), "ml_vc_db_correct_many")
 
 
 def cell_shape(n, ncomp):

@@ -62,18 +62,6 @@ struct AdvArgs {
   double* flux[3];     // 2 comps, face layout
 };
 
-// grid-stride indices: of a plain launch, or of workgroup b of the nb that
-// serve one job of a batched launch
-struct GridIx {
-  __device__ long long start() const { return blockIdx.x * (long long)blockDim.x + threadIdx.x; }
-  __device__ long long stride() const { return (long long)gridDim.x * blockDim.x; }
-};
-struct JobIx {
-  long long b, nb;
-  __device__ long long start() const { return b * blockDim.x + threadIdx.x; }
-  __device__ long long stride() const { return nb * blockDim.x; }
-};
-
 __device__ inline int wrapc(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 __device__ inline long long cidx(const Geom& g, int i, int j, int k) {

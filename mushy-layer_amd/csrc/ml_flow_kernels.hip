@@ -153,10 +153,11 @@ __device__ inline void enthalpy_vars_point(double H, double C, const ClosureP<do
   out[3] = chi;
 }
 
-__global__ void enthalpy_vars_kernel(Geom g, ClosureP<double> cp, const double* __restrict__ phi,
-                                     double* __restrict__ vars) {
+template <class IX>
+__device__ __forceinline__ void enthalpy_vars_body(Geom g, ClosureP<double> cp, const double* __restrict__ phi,
+                                     double* __restrict__ vars, IX ix) {
   const long long N = g.nvalid();
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = ix.start(); t < N; t += ix.stride()) {
     int i, j, k;
     cell_of(g, t, i, j, k);
     const long long q = g.ci(i, j, k);
@@ -164,6 +165,11 @@ __global__ void enthalpy_vars_kernel(Geom g, ClosureP<double> cp, const double* 
     enthalpy_vars_point(phi[q], phi[q + g.ncell], cp, o);
     for (int c = 0; c < 4; ++c) vars[q + c * g.ncell] = o[c];
   }
+}
+
+__global__ void enthalpy_vars_kernel(Geom g, ClosureP<double> cp, const double* __restrict__ phi,
+                                     double* __restrict__ vars) {
+  enthalpy_vars_body(g, cp, phi, vars, GridIx{});
 }
 
 struct Vec3 {
@@ -175,12 +181,13 @@ struct CVec3 {
 
 // src_d = (0 + buoyancy_d) - P_src_d (fillBuoyancy
 // srcSubcycle/AMRLevelMushyLayerFill.cpp:870-885; fillPressureSrcTerm :787-807)
-__global__ void ustar_src_kernel(Geom g, double buoyT, double buoyS, int scaleP, const double* __restrict__ vold,
-                                 const double* __restrict__ vnew, const double* __restrict__ pi, Vec3 src) {
+template <class IX>
+__device__ __forceinline__ void ustar_src_body(Geom g, double buoyT, double buoyS, int scaleP, const double* __restrict__ vold,
+                                 const double* __restrict__ vnew, const double* __restrict__ pi, Vec3 src, IX ix) {
   const long long N = g.nvalid();
   const double factor = 0.5 / g.dx;
   const int gd = g.dim - 1;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = ix.start(); t < N; t += ix.stride()) {
     int i, j, k;
     cell_of(g, t, i, j, k);
     const long long q = g.ci(i, j, k);
@@ -205,15 +212,21 @@ __global__ void ustar_src_kernel(Geom g, double buoyT, double buoyS, int scaleP,
   }
 }
 
+__global__ void ustar_src_kernel(Geom g, double buoyT, double buoyS, int scaleP, const double* __restrict__ vold,
+                                 const double* __restrict__ vnew, const double* __restrict__ pi, Vec3 src) {
+  ustar_src_body(g, buoyT, buoyS, scaleP, vold, vnew, pi, src, GridIx{});
+}
+
 // u_d = ustar_d + (P_src_d * dt) (removeGradPressure,
 // srcSubcycle/AMRLevelMushyLayerVel.cpp:654-671); u = 0 where the new
 // porosity <= limit (SETZEROVELOCITY, src/utils.ChF:73-96)
-__global__ void cc_velocity_kernel(Geom g, double dt, double limit, int scaleP, const double* __restrict__ vold,
+template <class IX>
+__device__ __forceinline__ void cc_velocity_body(Geom g, double dt, double limit, int scaleP, const double* __restrict__ vold,
                                    const double* __restrict__ vnew, const double* __restrict__ pi, CVec3 ustar,
-                                   Vec3 u) {
+                                   Vec3 u, IX ix) {
   const long long N = g.nvalid();
   const double factor = 0.5 / g.dx;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = ix.start(); t < N; t += ix.stride()) {
     int i, j, k;
     cell_of(g, t, i, j, k);
     const long long q = g.ci(i, j, k);
@@ -228,14 +241,21 @@ __global__ void cc_velocity_kernel(Geom g, double dt, double limit, int scaleP, 
   }
 }
 
+__global__ void cc_velocity_kernel(Geom g, double dt, double limit, int scaleP, const double* __restrict__ vold,
+                                   const double* __restrict__ vnew, const double* __restrict__ pi, CVec3 ustar,
+                                   Vec3 u) {
+  cc_velocity_body(g, dt, limit, scaleP, vold, vnew, pi, ustar, u, GridIx{});
+}
+
 // rhs = (sum_d (one/dx)*(u_face(i+1/2) - u_face(i-1/2))) * (1/dt), face
 // values the arithmetic mean of the two cells (DIVERGENCE,
 // util/DivergenceF.ChF:22-49)
-__global__ void cc_divergence_kernel(Geom g, double dt, CVec3 u, double* __restrict__ rhs) {
+template <class IX>
+__device__ __forceinline__ void cc_divergence_body(Geom g, double dt, CVec3 u, double* __restrict__ rhs, IX ix) {
   const long long N = g.nvalid();
   const double one_on_dx = 1.0 / g.dx;
   const double dtScale = 1.0 / dt;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = ix.start(); t < N; t += ix.stride()) {
     int i, j, k;
     cell_of(g, t, i, j, k);
     const long long q = g.ci(i, j, k);
@@ -253,14 +273,18 @@ __global__ void cc_divergence_kernel(Geom g, double dt, CVec3 u, double* __restr
   }
 }
 
+__global__ void cc_divergence_kernel(Geom g, double dt, CVec3 u, double* __restrict__ rhs) {
+  cc_divergence_body(g, dt, u, rhs, GridIx{});
+}
+
 // b = -(face porosity at the half time): arithmetic mean inside and across
 // periodic sides, the adjacent cell on walls
-template <int DIM>
-__global__ void pressure_faces_kernel(Geom g, int d, const double* __restrict__ vold, const double* __restrict__ vnew,
-                                      double* __restrict__ b) {
+template <int DIM, class IX>
+__device__ __forceinline__ void pressure_faces_body(Geom g, int d, const double* __restrict__ vold, const double* __restrict__ vnew,
+                                      double* __restrict__ b, IX ix) {
   const long long N = g.nface[d];
   const long long c3 = 3 * g.ncell;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = ix.start(); t < N; t += ix.stride()) {
     int i, j, k;
     unflatten(t, int(g.fn[d][0]), int(g.fn[d][1]), i, j, k);
     int lo[3] = {i, j, DIM == 3 ? k : 0}, hi[3] = {i, j, DIM == 3 ? k : 0};
@@ -285,15 +309,22 @@ __global__ void pressure_faces_kernel(Geom g, int d, const double* __restrict__ 
   }
 }
 
+template <int DIM>
+__global__ void pressure_faces_kernel(Geom g, int d, const double* __restrict__ vold, const double* __restrict__ vnew,
+                                      double* __restrict__ b) {
+  pressure_faces_body<DIM>(g, d, vold, vnew, b, GridIx{});
+}
+
 // u_d = u_d - (GRADCC(pi)_d * chi) * (1/(1/dt)); u = 0 where the new
 // porosity <= limit
-__global__ void cc_correct_kernel(Geom g, double dt, double limit, int scaleP, const double* __restrict__ vold,
-                                  const double* __restrict__ vnew, const double* __restrict__ pi, Vec3 u) {
+template <class IX>
+__device__ __forceinline__ void cc_correct_body(Geom g, double dt, double limit, int scaleP, const double* __restrict__ vold,
+                                  const double* __restrict__ vnew, const double* __restrict__ pi, Vec3 u, IX ix) {
   const long long N = g.nvalid();
   const double factor = 0.5 / g.dx;
   const double dtScale = 1.0 / dt;
   const double scale = 1 / dtScale;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = ix.start(); t < N; t += ix.stride()) {
     int i, j, k;
     cell_of(g, t, i, j, k);
     const long long q = g.ci(i, j, k);
@@ -309,15 +340,21 @@ __global__ void cc_correct_kernel(Geom g, double dt, double limit, int scaleP, c
   }
 }
 
+__global__ void cc_correct_kernel(Geom g, double dt, double limit, int scaleP, const double* __restrict__ vold,
+                                  const double* __restrict__ vnew, const double* __restrict__ pi, Vec3 u) {
+  cc_correct_body(g, dt, limit, scaleP, vold, vnew, pi, u, GridIx{});
+}
+
 // c = (chi / Pi(chi)) * coeff (srcSubcycle/AMRLevelMushyLayerInit.cpp:665-689)
 // with MushyLayerParams::calculatePermeability (src/MushyLayerParams.cpp
 // :1031-1090; pow(x, 3.0) and pow(x, 2.0) as the once-rounded cube and
 // square); hele_shaw > 0: harmonic combination with the Hele-Shaw cell
 // permeability 1/(1/hele_shaw + 1/Pi)
-__global__ void darcy_ccoef_law_kernel(Geom g, double coeff, int law, double hele_shaw,
-                                       const double* __restrict__ chi, double* __restrict__ c) {
+template <class IX>
+__device__ __forceinline__ void darcy_ccoef_law_body(Geom g, double coeff, int law, double hele_shaw,
+                                       const double* __restrict__ chi, double* __restrict__ c, IX ix) {
   const long long N = g.nvalid();
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = ix.start(); t < N; t += ix.stride()) {
     int i, j, k;
     cell_of(g, t, i, j, k);
     const long long q = g.ci(i, j, k);
@@ -338,7 +375,61 @@ __global__ void darcy_ccoef_law_kernel(Geom g, double coeff, int law, double hel
   }
 }
 
+__global__ void darcy_ccoef_law_kernel(Geom g, double coeff, int law, double hele_shaw,
+                                       const double* __restrict__ chi, double* __restrict__ c) {
+  darcy_ccoef_law_body(g, coeff, law, hele_shaw, chi, c, GridIx{});
+}
+
+template <int DIM, int STAGE>
+__global__ __launch_bounds__(256) void flow_batch_kernel(const FlowJob* __restrict__ jobs,
+                                                         const unsigned* __restrict__ map) {
+  const unsigned w = map[blockIdx.x];
+  const FlowJob& J = jobs[w >> kBatchTileBits];
+  const long long b = w & kBatchTileMask;
+  const JobIx cells{b, J.nbc};
+  if (STAGE == kFlowEnthalpy) enthalpy_vars_body(J.g, J.cp, J.phi, J.vars, cells);
+  if (STAGE == kFlowDarcy) darcy_ccoef_law_body(J.g, J.coeff, J.law, J.hele_shaw, J.chi, J.c, cells);
+  if (STAGE == kFlowUstarSrc)
+    ustar_src_body(J.g, J.buoyT, J.buoyS, J.scaleP, J.vold, J.vnew, J.pi, Vec3{{J.src[0], J.src[1], J.src[2]}}, cells);
+  if (STAGE == kFlowVelocity)
+    cc_velocity_body(J.g, J.dt, J.limit, J.scaleP, J.vold, J.vnew, J.pi, CVec3{{J.ustar[0], J.ustar[1], J.ustar[2]}},
+                     Vec3{{J.u[0], J.u[1], J.u[2]}}, cells);
+  if (STAGE == kFlowDivergence) cc_divergence_body(J.g, J.dt, CVec3{{J.u[0], J.u[1], J.u[2]}}, J.rhs, cells);
+  if (STAGE >= kFlowPressFaces && STAGE < kFlowPressFaces + 3)
+    pressure_faces_body<DIM>(J.g, STAGE - kFlowPressFaces, J.vold, J.vnew, J.b[STAGE - kFlowPressFaces],
+                             JobIx{b, J.nbf[STAGE - kFlowPressFaces]});
+  if (STAGE == kFlowCorrect)
+    cc_correct_body(J.g, J.dt, J.limit, J.scaleP, J.vold, J.vnew, J.pi, Vec3{{J.u[0], J.u[1], J.u[2]}}, cells);
+}
+
+template <int DIM>
+hipError_t flow_batch_dim(int stage, const FlowJob* jobs, const unsigned* map, int nblocks, hipStream_t s) {
+  const dim3 gr(nblocks), bl(256);
+  switch (stage) {
+    case kFlowEnthalpy: hipLaunchKernelGGL((flow_batch_kernel<DIM, kFlowEnthalpy>), gr, bl, 0, s, jobs, map); break;
+    case kFlowDarcy: hipLaunchKernelGGL((flow_batch_kernel<DIM, kFlowDarcy>), gr, bl, 0, s, jobs, map); break;
+    case kFlowUstarSrc: hipLaunchKernelGGL((flow_batch_kernel<DIM, kFlowUstarSrc>), gr, bl, 0, s, jobs, map); break;
+    case kFlowVelocity: hipLaunchKernelGGL((flow_batch_kernel<DIM, kFlowVelocity>), gr, bl, 0, s, jobs, map); break;
+    case kFlowDivergence: hipLaunchKernelGGL((flow_batch_kernel<DIM, kFlowDivergence>), gr, bl, 0, s, jobs, map); break;
+    case kFlowPressFaces: hipLaunchKernelGGL((flow_batch_kernel<DIM, kFlowPressFaces>), gr, bl, 0, s, jobs, map); break;
+    case kFlowPressFaces + 1:
+      hipLaunchKernelGGL((flow_batch_kernel<DIM, kFlowPressFaces + 1>), gr, bl, 0, s, jobs, map);
+      break;
+    case kFlowPressFaces + 2:
+      hipLaunchKernelGGL((flow_batch_kernel<DIM, kFlowPressFaces + 2>), gr, bl, 0, s, jobs, map);
+      break;
+    case kFlowCorrect: hipLaunchKernelGGL((flow_batch_kernel<DIM, kFlowCorrect>), gr, bl, 0, s, jobs, map); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t flow_batch(int dim, int stage, const FlowJob* jobs, const unsigned* map, int nblocks, hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  return dim == 2 ? flow_batch_dim<2>(stage, jobs, map, nblocks, s) : flow_batch_dim<3>(stage, jobs, map, nblocks, s);
+}
 
 hipError_t enthalpy_vars(const Geom& g, const ClosureP<double>& cp, const double* phi, double* vars, hipStream_t s) {
   hipLaunchKernelGGL(enthalpy_vars_kernel, dim3(grid_for(g.nvalid())), dim3(256), 0, s, g, cp, phi, vars);

@@ -164,6 +164,54 @@ struct TileIO {
   int npass, first, homog, ntx, nty;
 };
 
+// grid-stride indices: of a plain launch, or of workgroup b of the nb that
+// serve one job of a batched launch
+struct GridIx {
+  __device__ long long start() const { return blockIdx.x * (long long)blockDim.x + threadIdx.x; }
+  __device__ long long stride() const { return (long long)gridDim.x * blockDim.x; }
+};
+struct JobIx {
+  long long b, nb;
+  __device__ long long start() const { return b * blockDim.x + threadIdx.x; }
+  __device__ long long stride() const { return nb * blockDim.x; }
+};
+
+// Per-operator arguments of the batched Darcy-Brinkman step kernels
+// (flow_batch, hc_bcoef_batch): one job per operator, nbc workgroups over
+// its cells and nbf[d] over its faces of direction d.
+struct FlowJob {
+  Geom g;
+  ClosureP<double> cp;
+  const double* phi;
+  double* vars;
+  const double* vold;
+  const double* vnew;
+  const double* pi;
+  const double* chi;
+  double* c;
+  double coeff, hele_shaw, buoyT, buoyS, limit, dt, kcond, DH, DC;
+  int law, scaleP;
+  double* src[3];
+  const double* ustar[3];
+  double* u[3];
+  double* rhs;
+  double* b[3];
+  long long nbc, nbf[3];
+};
+// stages of flow_batch (ml_flow_kernels.hip), each over every job:
+enum {
+  kFlowEnthalpy = 0,   // vars = enthalpy variables of phi
+  kFlowDarcy = 1,      // c = Darcy coefficient of chi
+  kFlowUstarSrc = 2,   // src = U* source from vold, vnew, pi
+  kFlowVelocity = 3,   // u = ustar + grad pi (zero where solid)
+  kFlowDivergence = 4, // rhs = div(u)/dt
+  kFlowPressFaces = 5, // b[d] = pressure faces, stage 5 + d
+  kFlowCorrect = 8     // u -= grad pi (zero where solid)
+};
+hipError_t flow_batch(int dim, int stage, const FlowJob* jobs, const unsigned* map, int nblocks, hipStream_t s);
+// b[d] = HC face coefficients of chi for every job (ml_transfer_kernels.hip)
+hipError_t hc_bcoef_batch(int dim, int d, const FlowJob* jobs, const unsigned* map, int nblocks, hipStream_t s);
+
 // one batched operator application (hc_apply_op) of one operator
 template <typename T>
 struct HCApplyJob {

@@ -323,12 +323,12 @@ __global__ void porosity_kernel(Geom g, ClosureP<double> cp, const double* __res
 
 // HC bCoef assembly (srcSubcycle/AMRLevelMushyLayerInit.cpp:914-953) with
 // geometric face porosity (src/CellToEdge2F.ChF:20-46).
-template <int DIM>
-__global__ void hc_bcoef_kernel(Geom g, int d, double kcond, double DH, double DC,
-                                const double* __restrict__ chi, double* __restrict__ b) {
+template <int DIM, class IX>
+__device__ __forceinline__ void hc_bcoef_body(Geom g, int d, double kcond, double DH, double DC,
+                                const double* __restrict__ chi, double* __restrict__ b, IX ix) {
   const long long N = g.nface[d];
-  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < N;
-       c += (long long)gridDim.x * blockDim.x) {
+  for (long long c = ix.start(); c < N;
+       c += ix.stride()) {
     int i, j, k;
     unflatten(c, int(g.fn[d][0]), int(g.fn[d][1]), i, j, k);
     const int li = wrap_clamp(i - (d == 0), g.n[0], g.periodic[0]);
@@ -348,6 +348,20 @@ __global__ void hc_bcoef_kernel(Geom g, int d, double kcond, double DH, double D
     b[c] = bH;
     b[c + N] = bC;
   }
+}
+
+template <int DIM>
+__global__ void hc_bcoef_kernel(Geom g, int d, double kcond, double DH, double DC,
+                                const double* __restrict__ chi, double* __restrict__ b) {
+  hc_bcoef_body<DIM>(g, d, kcond, DH, DC, chi, b, GridIx{});
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void hc_bcoef_batch_kernel(int d, const FlowJob* __restrict__ jobs,
+                                                             const unsigned* __restrict__ map) {
+  const unsigned w = map[blockIdx.x];
+  const FlowJob& J = jobs[w >> kBatchTileBits];
+  hc_bcoef_body<DIM>(J.g, d, J.kcond, J.DH, J.DC, J.chi, J.b[d], JobIx{w & kBatchTileMask, J.nbf[d]});
 }
 
 // Darcy coefficient of the U* solve: c = (chi / Pi(chi)) * coeff with the
@@ -597,6 +611,15 @@ hipError_t hc_bcoef(const Geom& g, double k, double DH, double DC, const double*
     else
       hipLaunchKernelGGL((hc_bcoef_kernel<3>), dim3(blocks), dim3(256), 0, s, g, d, k, DH, DC, chi, b[d]);
   }
+  return hipGetLastError();
+}
+
+hipError_t hc_bcoef_batch(int dim, int d, const FlowJob* jobs, const unsigned* map, int nblocks, hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  if (dim == 2)
+    hipLaunchKernelGGL((hc_bcoef_batch_kernel<2>), dim3(nblocks), dim3(256), 0, s, d, jobs, map);
+  else
+    hipLaunchKernelGGL((hc_bcoef_batch_kernel<3>), dim3(nblocks), dim3(256), 0, s, d, jobs, map);
   return hipGetLastError();
 }
 

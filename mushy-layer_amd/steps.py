@@ -30,7 +30,8 @@ device.
 import numpy as np
 
 from . import (DeviceArray, HCOp, VCOp, default_opts, face_shape, make_advect_params, make_bc, make_flow_params,
-               make_params, valid, hc_db_advect_src_many, hc_set_coefs_many, vc_set_coefs_many)
+               make_params, valid, hc_db_advect_src_many, hc_db_coefs_many, hc_set_coefs_many, vc_db_correct_many,
+               vc_db_project_many, vc_db_ustar_many, vc_set_coefs_many)
 
 # multigrid options of the velocity solves: U* (VelocityMultigrid,
 # srcSubcycle/MushyLayerSubcycleUtils.cpp:654-659: smoothing 2 down, up and
@@ -285,8 +286,11 @@ class BatchDarcyBrinkmanStepper:
     one HCBatch step with every instance's advective source, the U* solves
     (both components of every instance) in one VCBatch, the projections in
     another -- and so do the coefficient hierarchies of each operator family
-    (hc_set_coefs_many, vc_set_coefs_many).  The per-instance closure, source
-    and correction kernels run instance by instance.  `cases`: `workloads.batch_instances` cases (each
+    (hc_set_coefs_many, vc_set_coefs_many), the advective sources
+    (hc_db_advect_src_many) and the closure, source, projection-rhs and
+    correction kernels (hc_db_coefs_many, vc_db_ustar_many,
+    vc_db_project_many, vc_db_correct_many): a step is a fixed number of
+    launches whatever the number of instances.  `cases`: `workloads.batch_instances` cases (each
     carrying Ra_C and Da); `flow_of(case)`: the flow dict of a case."""
 
     def __init__(self, ml, ctx, cases, flow_of, hc_opts):
@@ -312,15 +316,14 @@ class BatchDarcyBrinkmanStepper:
     def step(self):
         dt = self.dt
         sts = self.st
-        # (H, Theta): coefficients at t^n, then every instance's advective
-        # source, then one batched solve (which assembles the same
-        # coefficients again from the state)
-        for s in sts:
-            s.op.enthalpy_vars(s.phi, s.vold)
-            s.op.assemble_bcoef(s.vold.comp_ptr(3), s.b)
-        hc_set_coefs_many(self.ctx, [s.op for s in sts], None, [s.b for s in sts], 1.0, -dt)
+        ctx = self.ctx
+        # (H, Theta): enthalpy variables and coefficients at t^n, then every
+        # instance's advective source, then one batched solve (which
+        # assembles the same coefficients again from the state)
+        hc_db_coefs_many(ctx, [s.op for s in sts], [s.phi for s in sts], [s.vold for s in sts], [s.b for s in sts])
+        hc_set_coefs_many(ctx, [s.op for s in sts], None, [s.b for s in sts], 1.0, -dt)
         if all(s.advect for s in sts):
-            hc_db_advect_src_many(self.ctx, [s.op for s in sts], [s.ap for s in sts], dt, [s.phi for s in sts],
+            hc_db_advect_src_many(ctx, [s.op for s in sts], [s.ap for s in sts], dt, [s.phi for s in sts],
                                   [s.vold for s in sts], [s.u for s in sts], [s.dsrc for s in sts],
                                   [s.uf for s in sts], [s.hsrc for s in sts])
             srcs = [s.hsrc for s in sts]
@@ -328,25 +331,21 @@ class BatchDarcyBrinkmanStepper:
             srcs = [s.hc_src() for s in sts]
         hst = self.hcb.step([s.phi for s in sts], dt, self.hc_opts, srcs=srcs)
         # enthalpy variables at t^{n+1}, U* coefficients and sources
-        for s in sts:
-            s.op.enthalpy_vars(s.phi, s.vnew)
-            s.uop.darcy_ccoef_law(s.vold.comp_ptr(3), s.coef["darcy"], s.law, s.c)
-        vc_set_coefs_many(self.ctx, [s.uop for s in sts], None, [s.bu for s in sts], [s.c for s in sts], 1.0, -dt)
-        for s in sts:
-            s.uop.ustar_src(s.fp, s.vold, s.vnew, s.pi, s.src)
-            for u, uo in zip(s.u, s.uold):
-                s.copy(uo, u)
+        vc_db_ustar_many(ctx, [s.op for s in sts], [s.uop for s in sts], [s.fp for s in sts],
+                         [s.coef["darcy"] for s in sts], [s.law for s in sts], [0.0] * len(sts),
+                         [s.phi for s in sts], [s.vold for s in sts], [s.vnew for s in sts], [s.pi for s in sts],
+                         [s.c for s in sts], [s.src for s in sts], [s.u for s in sts], [s.uold for s in sts])
+        vc_set_coefs_many(ctx, [s.uop for s in sts], None, [s.bu for s in sts], [s.c for s in sts], 1.0, -dt)
         ust = self.ub.solve_be([x for s in sts for x in s.ustar], [x for s in sts for x in s.uold], dt,
                                srcs=[x for s in sts for x in s.src], opts=self.uopts)
         # projection
-        for s in sts:
-            s.uop.cc_project_rhs(s.fp, dt, s.vold, s.vnew, s.pi, s.ustar, s.u, s.prhs)
-            s.pop.pressure_faces(s.vold, s.vnew, s.bp)
-            s._zero(s.pi)
-        vc_set_coefs_many(self.ctx, [s.pop for s in sts], None, [s.bp for s in sts], None, 0.0, 1.0)
+        vc_db_project_many(ctx, [s.uop for s in sts], [s.fp for s in sts], dt, [s.vold for s in sts],
+                           [s.vnew for s in sts], [s.pi for s in sts], [s.ustar for s in sts], [s.u for s in sts],
+                           [s.prhs for s in sts], [s.bp for s in sts])
+        vc_set_coefs_many(ctx, [s.pop for s in sts], None, [s.bp for s in sts], None, 0.0, 1.0)
         pst = self.pb.solve([s.pi for s in sts], [s.prhs for s in sts], opts=self.popts)
-        for s in sts:
-            s.uop.cc_correct(s.fp, dt, s.vold, s.vnew, s.pi, s.u)
+        vc_db_correct_many(ctx, [s.uop for s in sts], [s.fp for s in sts], dt, [s.vold for s in sts],
+                           [s.vnew for s in sts], [s.pi for s in sts], [s.u for s in sts])
         d = len(sts[0].n)
         out = []
         for i, s in enumerate(sts):

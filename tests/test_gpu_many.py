@@ -148,3 +148,74 @@ def test_db_advect_src_many_equals_per_operator(ml, gpu_ctx, wl, dim):
         assert np.abs(HC.valid(s.n, ref[i][2])).max() > 0
     for s in sts:
         s.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_db_stage_calls_many_equal_per_operator(ml, gpu_ctx, wl, dim):
+    """ml_hc_db_coefs_many, ml_vc_db_ustar_many, ml_vc_db_project_many and
+    ml_vc_db_correct_many against the single-operator calls they batch, on
+    steppers after one step (nonzero velocity and pressure), from the same
+    state: cell fields compared on valid cells, face fields whole."""
+    stp = importlib.import_module("mushy-layer_amd.steps")
+    ctx = gpu_ctx
+    sizes = [(64, 64), (48, 40)] if dim == 2 else [(16, 16, 16), (12, 16, 8)]
+    sts = []
+    for i, n in enumerate(sizes):
+        case = wl.mushy(n=n, seed=60 + i, corr=0.25)
+        flow = wl.flow(dim=dim, Pr=10.0, Da=2e-3 * (i + 1), Ra_C=250.0 + 50 * i,
+                       permeability=["kozeny_carman", "cubic"][i % 2])
+        s = stp.DarcyBrinkmanStepper(ml, ctx, case, ml.default_opts(max_iter=60, hang=stp.NO_HANG), flow)
+        s.step()
+        sts.append(s)
+    dt = sts[0].dt
+    start = [([u.get() for u in s.u], s.pi.get()) for s in sts]
+
+    def restore(s, i):
+        for u, x in zip(s.u, start[i][0]):
+            u.set(x)
+        s.pi.set(start[i][1])
+
+    def snap(s):
+        cells = [HC.valid(s.n, x.get()).copy() for x in [s.vold, s.vnew, s.c, s.prhs] + s.src + s.uold + s.u]
+        faces = [x.get() for x in s.b + s.bp]
+        return cells, faces
+
+    ref, pis = [], []
+    for i, s in enumerate(sts):
+        restore(s, i)
+        s.op.enthalpy_vars(s.phi, s.vold)
+        s.op.assemble_bcoef(s.vold.comp_ptr(3), s.b)
+        s.op.enthalpy_vars(s.phi, s.vnew)
+        s.uop.darcy_ccoef_law(s.vold.comp_ptr(3), s.coef["darcy"], s.law, s.c)
+        s.uop.ustar_src(s.fp, s.vold, s.vnew, s.pi, s.src)
+        for u, uo in zip(s.u, s.uold):
+            s.copy(uo, u)
+        s.uop.cc_project_rhs(s.fp, dt, s.vold, s.vnew, s.pi, s.ustar, s.u, s.prhs)
+        s.pop.pressure_faces(s.vold, s.vnew, s.bp)
+        s.uop.cc_correct(s.fp, dt, s.vold, s.vnew, s.pi, s.u)
+        ref.append(snap(s))
+    for i, s in enumerate(sts):
+        restore(s, i)
+        for x in [s.vold, s.vnew, s.c, s.prhs] + s.b + s.src + s.uold + s.bp:
+            x.set(np.full(x.shape, 7.0))
+    hops, uops, fps = [s.op for s in sts], [s.uop for s in sts], [s.fp for s in sts]
+    ml.hc_db_coefs_many(ctx, hops, [s.phi for s in sts], [s.vold for s in sts], [s.b for s in sts])
+    ml.vc_db_ustar_many(ctx, hops, uops, fps, [s.coef["darcy"] for s in sts], [s.law for s in sts],
+                        [0.0] * len(sts), [s.phi for s in sts], [s.vold for s in sts], [s.vnew for s in sts],
+                        [s.pi for s in sts], [s.c for s in sts], [s.src for s in sts], [s.u for s in sts],
+                        [s.uold for s in sts])
+    ml.vc_db_project_many(ctx, uops, fps, dt, [s.vold for s in sts], [s.vnew for s in sts], [s.pi for s in sts],
+                          [s.ustar for s in sts], [s.u for s in sts], [s.prhs for s in sts], [s.bp for s in sts])
+    for i, s in enumerate(sts):
+        assert not np.any(HC.valid(s.n, s.pi.get()))   # the stage leaves pi = 0
+        s.pi.set(start[i][1])                         # the correction below uses the pressure
+    ml.vc_db_correct_many(ctx, uops, fps, dt, [s.vold for s in sts], [s.vnew for s in sts], [s.pi for s in sts],
+                          [s.u for s in sts])
+    for i, s in enumerate(sts):
+        cells, faces = snap(s)
+        for k, (a, b) in enumerate(zip(ref[i][0], cells)):
+            assert HC.bitwise_equal(a, b), (i, "cells", k)
+        for k, (a, b) in enumerate(zip(ref[i][1], faces)):
+            assert HC.bitwise_equal(a, b), (i, "faces", k)
+    for s in sts:
+        s.close()
