@@ -547,8 +547,8 @@ def batch_darcy_brinkman(ml, wl, ctx, cases, args, steps):
             "max_velocity_range": [round(min(umax), 6), round(max(umax), 6)],
             "launches": "per step: one (H, Theta) batch step (2K+2 launches per outer iteration, graph), U* and "
                         "projection batches (2K+1 launches per iteration each, set-up in three), coefficient "
-                        "hierarchies of all instances in three launches per operator family, per-instance closure, "
-                        "source and correction kernels"}
+                        "hierarchies, advective sources, closure, source, projection-rhs and correction kernels of "
+                        "all instances in one launch per kernel"}
 
 
 def run_adjoint(ml, wl, ctx, args):
