@@ -566,6 +566,11 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
 
   // state of the current level's cell in registers
   T ph[2] = {T(0), T(0)}, rr[2] = {T(0), T(0)};
+  // `staged`: sphi/sphl already hold the current level's state and closures
+  // and `Oc` its operator data (loaded with its FAS right-hand side), so the
+  // level visit that follows neither reloads nor restages them
+  WGCellOp<T, DIM> Oc;
+  bool staged = false;
   {
     WGLevel<T, DIM> W;
     W.init(&D.lev[l0]);
@@ -576,27 +581,29 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
       rr[1] = W.L->rhs[W.q + W.L->g.ncell];
     }
     if (apply_first) {
-      WGCellOp<T, DIM> O;
-      O.load(K, W);
+      Oc.load(K, W);
       wg_stage<T, DIM>(K, W, ph, sphi, sphl);
       if (W.own) {
         T lof[2];
-        wg_lof<T, DIM>(K, W, O.F, O.aq, sphi, sphl, lof);
+        wg_lof<T, DIM>(K, W, Oc.F, Oc.aq, sphi, sphl, lof);
         rr[0] = rr[0] + lof[0];
         rr[1] = rr[1] + lof[1];
         W.L->rhs[W.q] = rr[0];
         W.L->rhs[W.q + W.L->g.ncell] = rr[1];
       }
       __syncthreads();
+      staged = true;
     }
   }
 
   for (int l = l0; l < L - 1; ++l) {
     WGLevel<T, DIM> W;
     W.init(&D.lev[l]);
-    WGCellOp<T, DIM> O;
-    O.load(K, W);
-    wg_stage<T, DIM>(K, W, ph, sphi, sphl);
+    if (!staged) {
+      Oc.load(K, W);
+      wg_stage<T, DIM>(K, W, ph, sphi, sphl);
+    }
+    const WGCellOp<T, DIM>& O = Oc;   // reloaded below for the coarser level after its last use
     wg_relax<T, DIM>(K, W, O, rr, sphi, sphl, 2 * nu1);
     // restriction (RESTRICTRESNLVC / RESTRICT): each fine cell's
     // (rhs - L phi)/2^d and phi/2^d, then child sums in Fortran order
@@ -646,7 +653,6 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
     __syncthreads();
     // FAS coarse right-hand side rhs_{l+1} += L(phi_{l+1})
     {
-      WGCellOp<T, DIM> Oc;
       Oc.load(K, Wc);
       wg_stage<T, DIM>(K, Wc, ph, sphi, sphl);
       if (Wc.own) {
@@ -658,15 +664,17 @@ __global__ __launch_bounds__(kWGThreads) void hc_coarse_vcycle_kernel(const Coar
         Wc.L->rhs[Wc.q + Wc.L->g.ncell] = rr[1];
       }
       __syncthreads();
+      staged = true;
     }
   }
   {
     WGLevel<T, DIM> W;
     W.init(&D.lev[L - 1]);
-    WGCellOp<T, DIM> O;
-    O.load(K, W);
-    wg_stage<T, DIM>(K, W, ph, sphi, sphl);
-    wg_relax<T, DIM>(K, W, O, rr, sphi, sphl, 2 * bottom, true);
+    if (!staged) {
+      Oc.load(K, W);
+      wg_stage<T, DIM>(K, W, ph, sphi, sphl);
+    }
+    wg_relax<T, DIM>(K, W, Oc, rr, sphi, sphl, 2 * bottom, true);
   }
   for (int l = L - 2; l >= l0; --l) {
     // coarse correction phi_{l+1} - phi0_{l+1} into LDS (compact layout)
