@@ -178,3 +178,23 @@ def test_transposed_smoother_preconditioner(ml, gpu_ctx, wl):
         assert abs(res[sweeps][1] - res[0][1]) <= 1e-10 * abs(res[0][1])
         assert res[sweeps][2] < res[0][2]
     model.close()
+
+
+def test_fp32_refined_forward_gives_the_fp64_gradients(ml, gpu_ctx, wl):
+    """The forward solves in the FP32-refined form (then polished in FP64 to
+    the rounding floor, as the adjoint requires R(phi^n) = 0): loss and
+    gradients equal those of the FP64 forward to 1e-8 relative."""
+    torch, adj, case, model, tphi, tchi = _setup(ml, gpu_ctx, wl)
+    F = wl.flux_profile(32, F0=10.0)
+    L64, g64, _, s64 = _adjoint_grads(torch, adj, model, F, 5.0, tphi, tchi)
+    m32 = adj.HeatFluxAdjoint(gpu_ctx, case, checkpoint=5,
+                              opts=ml.default_opts(max_iter=400, hang=-1.0, precision=ml.PRECISION_FP32_REFINED))
+    L32, g32, gperm, s32 = _adjoint_grads(torch, adj, m32, F, 5.0, tphi, tchi)
+    print("FP64 forward: loss %.12e dL/dSt %.12e | FP32-refined forward: loss %.12e dL/dSt %.12e"
+          % (L64, s64, L32, s32))
+    assert gperm == 0.0
+    assert abs(L32 - L64) <= 1e-10 * abs(L64)
+    assert np.max(np.abs(g32 - g64)) <= 1e-8 * np.max(np.abs(g64))
+    assert abs(s32 - s64) <= 1e-8 * abs(s64)
+    m32.close()
+    model.close()
