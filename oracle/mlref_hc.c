@@ -457,7 +457,7 @@ static void relax(hc_hier* h, int l, int iters) {
                 L->lambda, L->phi, L->rhs, L->scratch);
 }
 
-/* FAS V-cycle (build spec, survey Appendix A.8 of the project notes):
+/* FAS V-cycle (build spec of Chombo's AMRFASMultiGrid V-cycle on one level):
  * relax nu1; phi_c = R phi (kept as phi_c0); rhs_c = R(rhs - L phi) + L_c(phi_c);
  * recurse; phi += P(phi_c - phi_c0); relax nu2.  Coarsest: `bottom` GSRB
  * iterations (RelaxSolver with m_imax = 40,
