@@ -556,9 +556,11 @@ hipError_t coarsen_hierarchy(const Geom& gf, int ncomp, int nlev, const T* const
 template <typename T>
 void coarsen_records(const Geom& gf, int ncomp, int nlev, const T* const* bf, T* const* const* bc, const T* af,
                      T* const* ac, bool faces, std::vector<CoarsenRec<T>>& light, std::vector<CoarsenRec<T>>& heavy) {
-  // the split of coarsen_hierarchy: averages over at least 64 fine cells or
-  // faces are heavy
-  constexpr long long kHeavy = 64;
+  // averages over at least 1024 fine cells or faces are heavy (one
+  // workgroup each); below that one thread sums an item, which in a batch of
+  // many operators keeps every lane busy.  Light and heavy items add in the
+  // same order, so the split does not change a bit of the result.
+  constexpr long long kHeavy = 1024;
   for (int l = nlev - 1; l >= 1; --l) {
     const Geom gc = coarsen_geom(gf, 1 << l);
     const long long block = (1LL << l) * (1LL << l) * (gf.dim == 3 ? (1LL << l) : 1);
